@@ -1,0 +1,201 @@
+/*
+ * scihpc_hip.h -- C ABI of the MI355X (gfx950) projection-step hot path.
+ *
+ * Plain C: opaque handles, host or device pointers plus sizes, int status
+ * codes.  No torch, HIP or C++ types cross this boundary.  Every entry point
+ * names the SciHPC interface it replaces (paths relative to the SciHPC
+ * source tree).  Handles are not thread safe; all work of one handle is
+ * ordered on that handle's HIP stream, and calls that return host data or
+ * host scalars synchronise that stream.
+ *
+ * Array layouts
+ *   MG layout     : unpadded, x fastest, idx = i + nx*(j + ny*k), exactly
+ *                   multigrid_base::of() (multigrid_base.cpp:8-14).
+ *   padded layout : scalar_data's data[i][j][k] made contiguous, k fastest,
+ *                   ghost width 3 on every axis < ndim
+ *                   (scalar_data.cpp:73-158, global.cpp:7-19).
+ */
+#ifndef SCIHPC_HIP_H
+#define SCIHPC_HIP_H
+
+#if defined(__GNUC__) || defined(__clang__)
+#define SCIHPC_API __attribute__((visibility("default")))
+#else
+#define SCIHPC_API
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes (the reference has none: assert only, bc_factory.cpp:8-21) */
+enum {
+  SCIHPC_OK = 0,
+  SCIHPC_NOT_CONVERGED = 1,
+  SCIHPC_BAD_ARG = 2,
+  SCIHPC_HIP_ERROR = 3,
+  SCIHPC_FELL_BACK = 4
+};
+
+/* multigrid boundary operator: init_NeumannBC / init_DirichletBC
+ * (multigrid_base.cpp:236-345) */
+enum { SCIHPC_MG_NEUMANN = 0, SCIHPC_MG_DIRICHLET = 1 };
+
+/* precision of the inner V-cycle */
+enum { SCIHPC_PREC_FP64 = 0, SCIHPC_PREC_MIXED = 1 };
+
+/* boundary condition types, same meaning as bc_factory.h:18-22 */
+enum {
+  SCIHPC_BC_NEUMANN = 0,
+  SCIHPC_BC_DIRICHLET = 1,
+  SCIHPC_BC_PERIODIC = 2,
+  SCIHPC_BC_NO_SLIP = 3,
+  SCIHPC_BC_SLIP = 4
+};
+
+/* last error message of the calling thread ("" when none) */
+SCIHPC_API const char *scihpc_last_error(void);
+
+/* ------------------------------------------------------------------ */
+/* Pressure-Poisson multigrid                                          */
+typedef struct scihpc_mg scihpc_mg;
+
+/* multigrid::multigrid(scalar_data*, structured_grid*) (multigrid.cpp:81-122)
+ * plus shared_solvers_mg_init_Neumann/_Dirichlet (shared_solvers.cpp:17-28):
+ * builds the level list with the reference's coarsening rule (ratio 2, else
+ * 3, while every coarse size stays > 3, multigrid.cpp:8-41) and the
+ * constant-coefficient operator of every level.  ndim 2 ignores nz/dz. */
+SCIHPC_API int scihpc_mg_create(int ndim, int nx, int ny, int nz, double dx, double dy,
+                     double dz, int bc, scihpc_mg **out);
+SCIHPC_API int scihpc_mg_destroy(scihpc_mg *h);
+/* multigrid::level_num (multigrid.h:21) */
+SCIHPC_API int scihpc_mg_num_levels(const scihpc_mg *h, int *levels);
+/* multigrid_base::{nx,ny,nz,degree} of at[level] (multigrid_base.h:31) */
+SCIHPC_API int scihpc_mg_level_shape(const scihpc_mg *h, int level, int *nx, int *ny,
+                          int *nz, int *degree);
+
+/* Variable-coefficient operator A(rho): init_ppe_proj_coefficients
+ * (projection_method.cpp:404-465), face coefficient 1/(0.5*(rho_i+rho_j))/h^2,
+ * zero at walls, diagonal -sum.  rho: n cells, MG layout, host pointer.
+ * Coarse levels average the fine face coefficients on each coarse face.
+ * Only valid with SCIHPC_MG_NEUMANN walls (as the reference's PPE). */
+SCIHPC_API int scihpc_mg_set_density(scihpc_mg *h, const double *rho);
+/* same, rho given as the padded level-set-grid density array
+ * (pressure->dummy->density, projection_method.cpp:416-419) */
+SCIHPC_API int scihpc_mg_set_density_padded(scihpc_mg *h, const double *rho_padded);
+
+/* multigrid_base::rhs / sol / res of level `level` (multigrid_base.h:28):
+ * host copies in MG layout, n = nx*ny*nz of that level. */
+SCIHPC_API int scihpc_mg_set_rhs(scihpc_mg *h, int level, const double *src);
+SCIHPC_API int scihpc_mg_get_rhs(scihpc_mg *h, int level, double *dst);
+SCIHPC_API int scihpc_mg_set_sol(scihpc_mg *h, int level, const double *src);
+SCIHPC_API int scihpc_mg_get_sol(scihpc_mg *h, int level, double *dst);
+SCIHPC_API int scihpc_mg_get_res(scihpc_mg *h, int level, double *dst);
+/* device-pointer variants for level 0 (d2d copies on the handle stream) */
+SCIHPC_API int scihpc_mg_set_rhs_device(scihpc_mg *h, const double *dsrc);
+SCIHPC_API int scihpc_mg_get_sol_device(scihpc_mg *h, double *ddst);
+/* multigrid_base::no_compatibility (multigrid_base.h:33) */
+SCIHPC_API int scihpc_mg_set_nocompat(scihpc_mg *h, int level, int flag);
+
+/* multigrid_base::residual() (multigrid_base.cpp:69-72): res = rhs - A sol,
+ * returns ||res||_2 (absolute) */
+SCIHPC_API int scihpc_mg_residual(scihpc_mg *h, int level, double *norm);
+/* multigrid_base::Ax (multigrid_base.cpp:388-390) on a host vector */
+SCIHPC_API int scihpc_mg_apply(scihpc_mg *h, int level, const double *x, double *y);
+/* multigrid_base::compatibility_condition (multigrid_base.cpp:47-67) on
+ * which = 0 rhs, 1 sol */
+SCIHPC_API int scihpc_mg_compat(scihpc_mg *h, int level, int which);
+
+/* Reference-faithful pieces on the GPU (same algorithm as the reference):
+ * relax = steepest descent (multigrid_base.cpp:16-45); restriction =
+ * relax(10) + residual + child average (:74-113); prolongation = slope
+ * reconstruction + relax(10) (:115-208); coarse_solve = direct solve with
+ * refinement (:210-234); vcycle_ref / fullcycle_ref = multigrid.cpp:43-79. */
+SCIHPC_API int scihpc_mg_relax(scihpc_mg *h, int level, int iters);
+SCIHPC_API int scihpc_mg_restriction(scihpc_mg *h, int level);
+SCIHPC_API int scihpc_mg_prolongation(scihpc_mg *h, int level);
+SCIHPC_API int scihpc_mg_coarse_solve(scihpc_mg *h);
+SCIHPC_API int scihpc_mg_vcycle_ref(scihpc_mg *h);
+SCIHPC_API int scihpc_mg_fullcycle_ref(scihpc_mg *h);
+
+/* The MI355X V-cycle (replaces multigrid::v_cycle, multigrid.cpp:43-55, as
+ * the solver behind the same interface): FP64 red-black Gauss-Seidel
+ * V(nu1,nu2), fused residual + child-average restriction, (bi/tri)linear
+ * prolongation, dense coarsest solve; in place on level 0 (rhs, sol). */
+SCIHPC_API int scihpc_mg_vcycle(scihpc_mg *h, int nu1, int nu2);
+
+typedef struct {
+  double rel_tol;    /* stop when ||b - A p||_2 / ||b||_2 <= rel_tol */
+  int max_cycles;    /* cap on V-cycles (outer corrections in mixed mode) */
+  int nu1, nu2;      /* smoothing sweeps down / up */
+  int precision;     /* SCIHPC_PREC_FP64 or SCIHPC_PREC_MIXED */
+  int use_learned;   /* 1: learned initial guess/correction (if attached) */
+  int zero_guess;    /* 1: start from p = 0, else from the current level-0 sol */
+} scihpc_solve_opts;
+
+typedef struct {
+  int cycles;            /* V-cycles (= outer corrections in mixed mode) */
+  int learned_accepted;  /* learned steps accepted by the FP64 residual test */
+  int learned_rejected;  /* learned steps discarded (plain MG used instead) */
+  double rel_residual;   /* final ||b - A p|| / ||b||, FP64 */
+  double ms;             /* device time of the solve (HIP events) */
+} scihpc_solve_stats;
+
+/* Solve A p = b on level 0 to opts->rel_tol (the call behind
+ * projection_method::fast_pressure_correction's loop,
+ * projection_method.cpp:694-706).  For Neumann operators b is used as given
+ * (callers remove its mean, as the reference's drivers do) and p is returned
+ * mean-free (compatibility_condition).  Returns SCIHPC_NOT_CONVERGED when
+ * max_cycles is reached first. */
+SCIHPC_API int scihpc_mg_solve(scihpc_mg *h, const scihpc_solve_opts *opts,
+                    scihpc_solve_stats *stats);
+
+/* Timing hook: runs `iters` launches of one level-0 kernel between HIP
+ * events on the handle stream; which = 0 FP64 smoother sweep, 1 FP32
+ * smoother sweep, 2 FP64 outer defect-correction kernel, 3 FP64 residual +
+ * restriction.  Returns the mean milliseconds per launch and the algorithmic
+ * HBM bytes of one launch. */
+SCIHPC_API int scihpc_mg_time_kernel(scihpc_mg *h, int which, int iters, double *ms,
+                          double *bytes);
+
+/* ------------------------------------------------------------------ */
+/* Level-set advection: WENO5-Z derivatives + TVD-RK3 (+ BCs)          */
+typedef struct scihpc_ls scihpc_ls;
+
+/* weno_solver(scalar_data*, structured_grid*) (weno_solver.cpp:7-14) and
+ * runge_kutta(Nx,Ny,Nz) (runge_kutta.cpp:7-11) for one padded grid.
+ * bc_types/bc_values: 6 entries xl,xr,yl,yr,zl,zr (bc_factory.h:42-48). */
+SCIHPC_API int scihpc_ls_create(int ndim, int nx, int ny, int nz, double dx, double dy,
+                     double dz, const int *bc_types, const double *bc_values,
+                     scihpc_ls **out);
+SCIHPC_API int scihpc_ls_destroy(scihpc_ls *h);
+/* padded host arrays: phi (scalar->data), u/v/w node velocities (nvel) */
+SCIHPC_API int scihpc_ls_set_phi(scihpc_ls *h, const double *phi);
+SCIHPC_API int scihpc_ls_get_phi(scihpc_ls *h, double *phi);
+SCIHPC_API int scihpc_ls_set_velocity(scihpc_ls *h, const double *u, const double *v,
+                           const double *w);
+/* weno_solver::weno5_find_derivatives (weno_solver.cpp:270-278) of `flux`
+ * (padded, host); fx/fy/fz padded outputs, interior cells written. */
+SCIHPC_API int scihpc_ls_weno5_derivatives(scihpc_ls *h, const double *flux, double *fx,
+                                double *fy, double *fz);
+/* runge_kutta::tvd_rk3(phi, nvel, identity_flux, convection) with WENO
+ * derivatives (runge_kutta.cpp:13-49, source.cpp:8-40, flux.cpp:7-17),
+ * bc_factory::apply_node_centered after each stage; nsteps steps of dt. */
+SCIHPC_API int scihpc_ls_tvd_rk3(scihpc_ls *h, double dt, int nsteps);
+/* bc_factory::apply_node_centered / apply_node_staggered_{x,y,z}
+ * (bc_factory.cpp:61-375) on a padded host array with this handle's BCs;
+ * stag = -1 node centred, 0/1/2 staggered axis. */
+SCIHPC_API int scihpc_ls_apply_bc(scihpc_ls *h, double *f, int stag);
+/* find_density (wrapper_func.cpp:387-402, lsm.cpp:8-18) of the handle's
+ * phi into a padded host array. */
+SCIHPC_API int scihpc_ls_density(scihpc_ls *h, double ls_width, double ratio,
+                      int positive_ref, double *rho);
+/* Timing hook: `iters` TVD-RK3 steps between HIP events; mean ms per step
+ * and algorithmic HBM bytes per step. */
+SCIHPC_API int scihpc_ls_time_step(scihpc_ls *h, double dt, int iters, double *ms,
+                        double *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

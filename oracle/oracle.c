@@ -1,0 +1,1399 @@
+/*
+ * oracle.c -- CPU restatement of SciHPC's projection-step hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker, never the product): loaded by
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg.  The HIP
+ * library in scihpc_amd/ never links or calls it.
+ *
+ * Written from the behaviour of the reference sources cited per function;
+ * arithmetic expressions keep the reference's evaluation order so that the
+ * compiled reference (oracle/_ref) and this file agree to the last bit where
+ * the reference itself is deterministic.  Build: plain C11, no FMA
+ * contraction (-ffp-contract=off), like the reference's default x86-64
+ * build.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define ORC_MAXL 32
+#define ORC_GHC 3
+
+static const double orc_eps = 1e-8; /* global.h:14 */
+
+/* ================================================================== */
+/* hierarchy: multigrid.cpp:8-41 (min_grid = 3), :81-122               */
+static const int orc_min_grid = 3;
+
+int orc_mg_levels(int ndim, int nx, int ny, int nz, int maxl, int *lnx,
+                  int *lny, int *lnz, int *ldeg) {
+  int L = 0;
+  lnx[0] = nx;
+  lny[0] = ny;
+  lnz[0] = (ndim == 3) ? nz : 1;
+  ldeg[0] = 1;
+  L = 1;
+  while (L < maxl) {
+    int cx = lnx[L - 1], cy = lny[L - 1], cz = lnz[L - 1];
+    int r = 1;
+    if (ndim == 2) {
+      if (cx % 2 == 0 && cy % 2 == 0 && cx / 2 > orc_min_grid &&
+          cy / 2 > orc_min_grid)
+        r = 2;
+      else if (cx % 3 == 0 && cy % 3 == 0 && cx / 3 > orc_min_grid &&
+               cy / 3 > orc_min_grid)
+        r = 3;
+    } else {
+      if (cx % 2 == 0 && cy % 2 == 0 && cz % 2 == 0 && cx / 2 > orc_min_grid &&
+          cy / 2 > orc_min_grid && cz / 2 > orc_min_grid)
+        r = 2;
+      else if (cx % 3 == 0 && cy % 3 == 0 && cz % 3 == 0 &&
+               cx / 3 > orc_min_grid && cy / 3 > orc_min_grid &&
+               cz / 3 > orc_min_grid)
+        r = 3;
+    }
+    if (r == 1) break;
+    lnx[L] = cx / r;
+    lny[L] = cy / r;
+    lnz[L] = (ndim == 3) ? cz / r : 1;
+    ldeg[L] = ldeg[L - 1] * r;
+    L++;
+  }
+  return L;
+}
+
+/* ================================================================== */
+/* multigrid object                                                    */
+typedef struct {
+  int nx, ny, nz, n, deg;
+  double dx, dy, dz;
+  int nocompat;
+  double *rhs, *sol, *res, *buf;
+  /* operator: diag + coefficient to the +x/+y/+z neighbour (0 when absent);
+   * the coefficient to the -x neighbour of cell c is cx[c-1]           */
+  double *diag, *cx, *cy, *cz;
+  /* coarsest-level dense factorisation */
+  double *lu;
+  int *piv;
+  int lu_ready;
+} orc_lvl;
+
+typedef struct {
+  int ndim, bc, L, variable;
+  int rkind; /* restriction: 0 child average, 1 full weighting (P^T/r^d) */
+  int pkind; /* prolongation: 0 (bi/tri)linear, 1 piecewise constant     */
+  orc_lvl lv[ORC_MAXL];
+} orc_mg;
+
+void orc_mg_set_transfer(void *h, int rkind, int pkind) {
+  ((orc_mg *)h)->rkind = rkind;
+  ((orc_mg *)h)->pkind = pkind;
+}
+
+static void lvl_const_operator(orc_mg *m, orc_lvl *l) {
+  /* init_NeumannBC (multigrid_base.cpp:236-291) and init_DirichletBC
+   * (:293-345): off-diagonals 1/dx/dx; Neumann diag = -(sum of present
+   * neighbours) in the order i-, i+, j-, j+, k-, k+; Dirichlet diag =
+   * -2/dx/dx-2/dy/dy(-2/dz/dz).                                        */
+  const int nx = l->nx, ny = l->ny, nz = l->nz;
+  const double ax = 1.0 / l->dx / l->dx, ay = 1.0 / l->dy / l->dy;
+  const double az = (m->ndim == 3) ? 1.0 / l->dz / l->dz : 0.0;
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++) {
+        long c = i + (long)nx * (j + (long)ny * k);
+        l->cx[c] = (i < nx - 1) ? ax : 0.0;
+        l->cy[c] = (j < ny - 1) ? ay : 0.0;
+        l->cz[c] = (m->ndim == 3 && k < nz - 1) ? az : 0.0;
+        if (m->bc == 0) {
+          double cc = 0.0;
+          if (i > 0) cc += 1.0 / l->dx / l->dx;
+          if (i < nx - 1) cc += 1.0 / l->dx / l->dx;
+          if (j > 0) cc += 1.0 / l->dy / l->dy;
+          if (j < ny - 1) cc += 1.0 / l->dy / l->dy;
+          if (m->ndim == 3) {
+            if (k > 0) cc += 1.0 / l->dz / l->dz;
+            if (k < nz - 1) cc += 1.0 / l->dz / l->dz;
+          }
+          l->diag[c] = -cc;
+        } else {
+          if (m->ndim == 2)
+            l->diag[c] = -2.0 / l->dx / l->dx - 2.0 / l->dy / l->dy;
+          else
+            l->diag[c] = -2.0 / l->dx / l->dx - 2.0 / l->dy / l->dy -
+                         2.0 / l->dz / l->dz;
+        }
+      }
+}
+
+void *orc_mg_create(int ndim, int nx, int ny, int nz, double dx, double dy,
+                    double dz, int bc) {
+  orc_mg *m = (orc_mg *)calloc(1, sizeof(orc_mg));
+  int lnx[ORC_MAXL], lny[ORC_MAXL], lnz[ORC_MAXL], ldeg[ORC_MAXL];
+  m->ndim = ndim;
+  m->bc = bc;
+  m->L = orc_mg_levels(ndim, nx, ny, nz, ORC_MAXL, lnx, lny, lnz, ldeg);
+  for (int l = 0; l < m->L; l++) {
+    orc_lvl *v = &m->lv[l];
+    v->nx = lnx[l];
+    v->ny = lny[l];
+    v->nz = lnz[l];
+    v->deg = ldeg[l];
+    v->n = v->nx * v->ny * v->nz;
+    v->dx = dx * ldeg[l];
+    v->dy = dy * ldeg[l];
+    v->dz = (ndim == 3) ? dz * ldeg[l] : 1.0; /* multigrid_base.cpp:425-434 */
+    v->nocompat = (bc == 1);
+    v->rhs = (double *)calloc(v->n, sizeof(double));
+    v->sol = (double *)calloc(v->n, sizeof(double));
+    v->res = (double *)calloc(v->n, sizeof(double));
+    v->buf = (double *)calloc(v->n, sizeof(double));
+    v->diag = (double *)calloc(v->n, sizeof(double));
+    v->cx = (double *)calloc(v->n, sizeof(double));
+    v->cy = (double *)calloc(v->n, sizeof(double));
+    v->cz = (double *)calloc(v->n, sizeof(double));
+    lvl_const_operator(m, v);
+  }
+  return m;
+}
+
+void orc_mg_destroy(void *h) {
+  orc_mg *m = (orc_mg *)h;
+  if (!m) return;
+  for (int l = 0; l < m->L; l++) {
+    orc_lvl *v = &m->lv[l];
+    free(v->rhs);
+    free(v->sol);
+    free(v->res);
+    free(v->buf);
+    free(v->diag);
+    free(v->cx);
+    free(v->cy);
+    free(v->cz);
+    free(v->lu);
+    free(v->piv);
+  }
+  free(m);
+}
+
+int orc_mg_nlevels(void *h) { return ((orc_mg *)h)->L; }
+
+void orc_mg_level_shape(void *h, int level, int *shape) {
+  orc_lvl *v = &((orc_mg *)h)->lv[level];
+  shape[0] = v->nx;
+  shape[1] = v->ny;
+  shape[2] = v->nz;
+  shape[3] = v->deg;
+  shape[4] = v->n;
+}
+
+double *orc_mg_vec(void *h, int level, int which) {
+  orc_lvl *v = &((orc_mg *)h)->lv[level];
+  return which == 0 ? v->rhs : which == 1 ? v->sol : v->res;
+}
+
+void orc_mg_set_nocompat(void *h, int level, int flag) {
+  ((orc_mg *)h)->lv[level].nocompat = flag;
+}
+
+/* diag from face arrays in the order of init_ppe_proj_coefficients:
+ * CC = -(CR + CL + CU + CD); CC -= (CF + CB)   (projection_method.cpp:456-460) */
+static void lvl_diag_from_faces(orc_mg *m, orc_lvl *l) {
+  const int nx = l->nx, ny = l->ny, nz = l->nz;
+  const long sxy = (long)nx * ny;
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++) {
+        long c = i + (long)nx * (j + (long)ny * k);
+        double CR = l->cx[c], CL = (i > 0) ? l->cx[c - 1] : 0.0;
+        double CU = l->cy[c], CD = (j > 0) ? l->cy[c - nx] : 0.0;
+        double cc = -(CR + CL + CU + CD);
+        if (m->ndim == 3) {
+          double CF = l->cz[c], CB = (k > 0) ? l->cz[c - sxy] : 0.0;
+          cc -= (CF + CB);
+        }
+        l->diag[c] = cc;
+      }
+}
+
+void orc_mg_set_rho(void *h, const double *rho) {
+  orc_mg *m = (orc_mg *)h;
+  m->variable = 1;
+  /* level 0: projection_method.cpp:411-460 */
+  orc_lvl *l = &m->lv[0];
+  const int nx = l->nx, ny = l->ny, nz = l->nz;
+  const long sxy = (long)nx * ny;
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++) {
+        long c = i + (long)nx * (j + (long)ny * k);
+        if (i < nx - 1) {
+          double R = 0.5 * (rho[c] + rho[c + 1]);
+          l->cx[c] = 1.0 / R / l->dx / l->dx;
+        } else
+          l->cx[c] = 0.0;
+        if (j < ny - 1) {
+          double U = 0.5 * (rho[c] + rho[c + nx]);
+          l->cy[c] = 1.0 / U / l->dy / l->dy;
+        } else
+          l->cy[c] = 0.0;
+        if (m->ndim == 3 && k < nz - 1) {
+          double F = 0.5 * (rho[c] + rho[c + sxy]);
+          l->cz[c] = 1.0 / F / l->dz / l->dz;
+        } else
+          l->cz[c] = 0.0;
+      }
+  lvl_diag_from_faces(m, l);
+  /* coarse levels: mean of the fine faces on the coarse face / r^2      */
+  for (int lv = 1; lv < m->L; lv++) {
+    orc_lvl *f = &m->lv[lv - 1], *g = &m->lv[lv];
+    const int r = g->deg / f->deg;
+    const int rz = (m->ndim == 3) ? r : 1;
+    const double scale = 1.0 / (double)(r * r);
+    const double cnt = (m->ndim == 3) ? (double)(r * r) : (double)r;
+    for (int K = 0; K < g->nz; K++)
+      for (int J = 0; J < g->ny; J++)
+        for (int I = 0; I < g->nx; I++) {
+          long C = I + (long)g->nx * (J + (long)g->ny * K);
+          double sx = 0.0, sy = 0.0, sz = 0.0;
+          for (int b = 0; b < rz; b++)
+            for (int a = 0; a < r; a++) {
+              /* x face: fine i = (I+1)r-1, j = Jr+a, k = Kr+b */
+              long fxc = ((I + 1) * r - 1) +
+                         (long)f->nx * ((J * r + a) + (long)f->ny * (K * rz + b));
+              sx += f->cx[fxc];
+              long fyc = (I * r + a) +
+                         (long)f->nx * (((J + 1) * r - 1) + (long)f->ny * (K * rz + b));
+              sy += f->cy[fyc];
+              if (m->ndim == 3) {
+                long fzc = (I * r + a) +
+                           (long)f->nx * ((J * r + b) + (long)f->ny * ((K + 1) * r - 1));
+                sz += f->cz[fzc];
+              }
+            }
+          g->cx[C] = (I < g->nx - 1) ? sx / cnt * scale : 0.0;
+          g->cy[C] = (J < g->ny - 1) ? sy / cnt * scale : 0.0;
+          g->cz[C] = (m->ndim == 3 && K < g->nz - 1) ? sz / cnt * scale : 0.0;
+        }
+    lvl_diag_from_faces(m, g);
+    g->lu_ready = 0;
+  }
+  m->lv[m->L - 1].lu_ready = 0;
+}
+
+void orc_mg_get_operator(void *h, int level, double *diag, double *cx,
+                         double *cy, double *cz) {
+  orc_lvl *l = &((orc_mg *)h)->lv[level];
+  memcpy(diag, l->diag, sizeof(double) * l->n);
+  memcpy(cx, l->cx, sizeof(double) * l->n);
+  memcpy(cy, l->cy, sizeof(double) * l->n);
+  memcpy(cz, l->cz, sizeof(double) * l->n);
+}
+
+/* y = A x with the accumulation order of Eigen's column-major sparse
+ * product (columns ascending): k-, j-, i-, diag, i+, j+, k+.            */
+static void lvl_apply(const orc_lvl *l, const double *x, double *y) {
+  const int nx = l->nx, ny = l->ny, nz = l->nz;
+  const long sxy = (long)nx * ny;
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++) {
+        long c = i + (long)nx * (j + (long)ny * k);
+        double t = 0.0;
+        if (k > 0 && l->cz[c - sxy] != 0.0) t += l->cz[c - sxy] * x[c - sxy];
+        if (j > 0 && l->cy[c - nx] != 0.0) t += l->cy[c - nx] * x[c - nx];
+        if (i > 0 && l->cx[c - 1] != 0.0) t += l->cx[c - 1] * x[c - 1];
+        t += l->diag[c] * x[c];
+        if (i < nx - 1 && l->cx[c] != 0.0) t += l->cx[c] * x[c + 1];
+        if (j < ny - 1 && l->cy[c] != 0.0) t += l->cy[c] * x[c + nx];
+        if (k < nz - 1 && l->cz[c] != 0.0) t += l->cz[c] * x[c + sxy];
+        y[c] = t;
+      }
+}
+
+void orc_mg_apply(void *h, int level, const double *x, double *y) {
+  lvl_apply(&((orc_mg *)h)->lv[level], x, y);
+}
+
+static double lvl_residual(orc_lvl *l) {
+  /* multigrid_base.cpp:69-72: res = rhs - A*sol; return res.norm().  Eigen
+   * evaluates this as res = rhs; res -= A*sol, accumulating the column-major
+   * product column by column (ascending) into res.                      */
+  const int nx = l->nx, ny = l->ny, nz = l->nz;
+  const long sxy = (long)nx * ny;
+  const double *x = l->sol;
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++) {
+        long c = i + (long)nx * (j + (long)ny * k);
+        double t = l->rhs[c];
+        if (k > 0 && l->cz[c - sxy] != 0.0) t += l->cz[c - sxy] * -x[c - sxy];
+        if (j > 0 && l->cy[c - nx] != 0.0) t += l->cy[c - nx] * -x[c - nx];
+        if (i > 0 && l->cx[c - 1] != 0.0) t += l->cx[c - 1] * -x[c - 1];
+        t += l->diag[c] * -x[c];
+        if (i < nx - 1 && l->cx[c] != 0.0) t += l->cx[c] * -x[c + 1];
+        if (j < ny - 1 && l->cy[c] != 0.0) t += l->cy[c] * -x[c + nx];
+        if (k < nz - 1 && l->cz[c] != 0.0) t += l->cz[c] * -x[c + sxy];
+        l->res[c] = t;
+      }
+  double s = 0.0;
+  for (int c = 0; c < l->n; c++) s += l->res[c] * l->res[c];
+  return sqrt(s);
+}
+
+double orc_mg_residual(void *h, int level) {
+  return lvl_residual(&((orc_mg *)h)->lv[level]);
+}
+
+static void lvl_compat(orc_lvl *l, double *f) {
+  /* multigrid_base.cpp:47-67 */
+  if (l->nocompat) return;
+  double sum = 0.0;
+  for (int c = 0; c < l->n; c++) sum += f[c];
+  sum = sum / l->n;
+  for (int c = 0; c < l->n; c++) f[c] -= sum;
+}
+
+void orc_mg_compat(void *h, int level, int which) {
+  orc_lvl *l = &((orc_mg *)h)->lv[level];
+  lvl_compat(l, which == 0 ? l->rhs : which == 1 ? l->sol : l->res);
+}
+
+static void lvl_relax(orc_lvl *l, int iter) {
+  /* multigrid_base.cpp:16-45: steepest descent on the negative definite A */
+  for (int cnt = 0; cnt < iter; ++cnt) {
+    if (lvl_residual(l) < 1e-16) break;
+    double rdot = 0.0;
+    for (int c = 0; c < l->n; c++) rdot += l->res[c] * l->res[c];
+    lvl_apply(l, l->res, l->buf);
+    double alpha = 0.0;
+    for (int c = 0; c < l->n; c++) alpha += l->res[c] * l->buf[c];
+    alpha = rdot / alpha;
+    for (int c = 0; c < l->n; c++) l->sol[c] += alpha * l->res[c];
+    lvl_compat(l, l->sol);
+  }
+}
+
+void orc_mg_relax(void *h, int level, int iter) {
+  lvl_relax(&((orc_mg *)h)->lv[level], iter);
+}
+
+static const int orc_base_step = 10; /* multigrid_base.h:35 */
+
+void orc_mg_restriction(void *h, int level) {
+  /* multigrid_base.cpp:74-113 */
+  orc_mg *m = (orc_mg *)h;
+  orc_lvl *f = &m->lv[level], *g = &m->lv[level + 1];
+  lvl_relax(f, orc_base_step);
+  lvl_residual(f);
+  const int r = g->deg / f->deg;
+  if (m->ndim == 2) {
+    for (int i = 0; i < g->nx; ++i)
+      for (int j = 0; j < g->ny; ++j) {
+        long C = i + (long)g->nx * j;
+        g->rhs[C] = 0.0;
+        for (int ii = 0; ii < r; ++ii)
+          for (int jj = 0; jj < r; ++jj)
+            g->rhs[C] += f->res[(i * r + ii) + (long)f->nx * (j * r + jj)];
+        g->rhs[C] /= r * r;
+        g->sol[C] = 0.0;
+      }
+  } else {
+    for (int i = 0; i < g->nx; ++i)
+      for (int j = 0; j < g->ny; ++j)
+        for (int k = 0; k < g->nz; ++k) {
+          long C = i + (long)g->nx * (j + (long)g->ny * k);
+          g->rhs[C] = 0.0;
+          for (int ii = 0; ii < r; ++ii)
+            for (int jj = 0; jj < r; ++jj)
+              for (int kk = 0; kk < r; ++kk)
+                g->rhs[C] += f->res[(i * r + ii) +
+                                    (long)f->nx * ((j * r + jj) +
+                                                   (long)f->ny * (k * r + kk))];
+          g->rhs[C] /= r * r * r;
+          g->sol[C] = 0.0;
+        }
+  }
+  lvl_compat(g, g->rhs);
+}
+
+void orc_mg_prolongation(void *h, int level) {
+  /* multigrid_base.cpp:115-208: slope reconstruction added to the fine sol */
+  orc_mg *m = (orc_mg *)h;
+  orc_lvl *c = &m->lv[level], *d = &m->lv[level - 1];
+  const int r = c->deg / d->deg;
+  const int nx = c->nx, ny = c->ny, nz = c->nz;
+  const long sxy = (long)nx * ny;
+  const double *s = c->sol;
+  if (m->ndim == 2) {
+    for (int i = 0; i < nx; ++i)
+      for (int j = 0; j < ny; ++j) {
+        long o = i + (long)nx * j;
+        double fx, fy;
+        if (i > 0 && i < nx - 1)
+          fx = 0.5 * (s[o + 1] - s[o - 1]) / c->dx;
+        else if (i == 0)
+          fx = (s[o + 1] - s[o]) / c->dx;
+        else
+          fx = (s[o] - s[o - 1]) / c->dx;
+        if (j > 0 && j < ny - 1)
+          fy = 0.5 * (s[o + nx] - s[o - nx]) / c->dy;
+        else if (j == 0)
+          fy = (s[o + nx] - s[o]) / c->dy;
+        else
+          fy = (s[o] - s[o - nx]) / c->dy;
+        for (int ii = 0; ii < r; ++ii)
+          for (int jj = 0; jj < r; ++jj) {
+            double x = (0.5 + ii) * d->dx;
+            double y = (0.5 + jj) * d->dy;
+            d->sol[(i * r + ii) + (long)d->nx * (j * r + jj)] +=
+                s[o] + fx * (x - 0.5 * c->dx) + fy * (y - 0.5 * c->dy);
+          }
+      }
+  } else {
+    for (int i = 0; i < nx; ++i)
+      for (int j = 0; j < ny; ++j)
+        for (int k = 0; k < nz; ++k) {
+          long o = i + (long)nx * (j + (long)ny * k);
+          double fx, fy, fz;
+          if (i > 0 && i < nx - 1)
+            fx = 0.5 * (s[o + 1] - s[o - 1]) / c->dx;
+          else if (i == 0)
+            fx = (s[o + 1] - s[o]) / c->dx;
+          else
+            fx = (s[o] - s[o - 1]) / c->dx;
+          if (j > 0 && j < ny - 1)
+            fy = 0.5 * (s[o + nx] - s[o - nx]) / c->dy;
+          else if (j == 0)
+            fy = (s[o + nx] - s[o]) / c->dy;
+          else
+            fy = (s[o] - s[o - nx]) / c->dy;
+          if (k > 0 && k < nz - 1)
+            fz = 0.5 * (s[o + sxy] - s[o - sxy]) / c->dz;
+          else if (k == 0)
+            fz = (s[o + sxy] - s[o]) / c->dz;
+          else
+            fz = (s[o] - s[o - sxy]) / c->dz;
+          for (int ii = 0; ii < r; ++ii)
+            for (int jj = 0; jj < r; ++jj)
+              for (int kk = 0; kk < r; ++kk) {
+                double x = (0.5 + ii) * d->dx;
+                double y = (0.5 + jj) * d->dy;
+                double z = (0.5 + kk) * d->dz;
+                d->sol[(i * r + ii) +
+                       (long)d->nx * ((j * r + jj) + (long)d->ny * (k * r + kk))] +=
+                    s[o] + fx * (x - 0.5 * c->dx) + fy * (y - 0.5 * c->dy) +
+                    fz * (z - 0.5 * c->dz);
+              }
+        }
+  }
+  lvl_relax(d, orc_base_step);
+}
+
+/* ---------------- dense LU with partial pivoting ------------------ */
+static int dense_lu(double *a, int *piv, int n) {
+  for (int k = 0; k < n; k++) {
+    int p = k;
+    double best = fabs(a[(long)k * n + k]);
+    for (int i = k + 1; i < n; i++) {
+      double v = fabs(a[(long)i * n + k]);
+      if (v > best) {
+        best = v;
+        p = i;
+      }
+    }
+    piv[k] = p;
+    if (best == 0.0) return -1;
+    if (p != k)
+      for (int j = 0; j < n; j++) {
+        double t = a[(long)k * n + j];
+        a[(long)k * n + j] = a[(long)p * n + j];
+        a[(long)p * n + j] = t;
+      }
+    double inv = 1.0 / a[(long)k * n + k];
+    for (int i = k + 1; i < n; i++) {
+      double f = a[(long)i * n + k] * inv;
+      a[(long)i * n + k] = f;
+      if (f != 0.0)
+        for (int j = k + 1; j < n; j++) a[(long)i * n + j] -= f * a[(long)k * n + j];
+    }
+  }
+  return 0;
+}
+
+static void dense_lu_solve(const double *a, const int *piv, int n,
+                           const double *b, double *x) {
+  for (int i = 0; i < n; i++) x[i] = b[i];
+  for (int k = 0; k < n; k++)
+    if (piv[k] != k) {
+      double t = x[k];
+      x[k] = x[piv[k]];
+      x[piv[k]] = t;
+    }
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < i; j++) x[i] -= a[(long)i * n + j] * x[j];
+  for (int i = n - 1; i >= 0; i--) {
+    for (int j = i + 1; j < n; j++) x[i] -= a[(long)i * n + j] * x[j];
+    x[i] /= a[(long)i * n + i];
+  }
+}
+
+/* Dense matrix of the level operator; for the all-Neumann (singular)
+ * operator the constant null space is pinned: M = A - (s/n) 1 1^T with
+ * s = max|diag|, which equals A on mean-free vectors and is nonsingular. */
+static void lvl_dense_matrix(const orc_mg *m, const orc_lvl *l, double *a) {
+  const int n = l->n;
+  double *e = (double *)calloc(n, sizeof(double));
+  double *col = (double *)calloc(n, sizeof(double));
+  for (int j = 0; j < n; j++) {
+    e[j] = 1.0;
+    lvl_apply(l, e, col);
+    e[j] = 0.0;
+    for (int i = 0; i < n; i++) a[(long)i * n + j] = col[i];
+  }
+  if (m->bc == 0) {
+    double s = 0.0;
+    for (int i = 0; i < n; i++) s = fmax(s, fabs(l->diag[i]));
+    for (long i = 0; i < (long)n * n; i++) a[i] -= s / n;
+  }
+  free(e);
+  free(col);
+}
+
+static void lvl_factor(const orc_mg *m, orc_lvl *l) {
+  if (l->lu_ready) return;
+  free(l->lu);
+  free(l->piv);
+  l->lu = (double *)malloc(sizeof(double) * (long)l->n * l->n);
+  l->piv = (int *)malloc(sizeof(int) * l->n);
+  lvl_dense_matrix(m, l, l->lu);
+  dense_lu(l->lu, l->piv, l->n);
+  l->lu_ready = 1;
+}
+
+void orc_mg_coarse_solve(void *h) {
+  /* multigrid_base.cpp:210-234: factorise once, solve, compatibility,
+   * iterative refinement while the residual drops by more than 10 %.    */
+  orc_mg *m = (orc_mg *)h;
+  orc_lvl *l = &m->lv[m->L - 1];
+  lvl_factor(m, l);
+  dense_lu_solve(l->lu, l->piv, l->n, l->rhs, l->sol);
+  lvl_compat(l, l->sol);
+  double old_res = -1.0, new_res = 0.0;
+  double *dx = (double *)malloc(sizeof(double) * l->n);
+  do {
+    if (old_res < 0.0)
+      old_res = lvl_residual(l);
+    else
+      old_res = new_res;
+    dense_lu_solve(l->lu, l->piv, l->n, l->res, dx);
+    for (int c = 0; c < l->n; c++) l->sol[c] += dx[c];
+    lvl_compat(l, l->sol);
+    new_res = lvl_residual(l);
+  } while (1.0 - new_res / old_res > 0.1);
+  free(dx);
+}
+
+void orc_mg_vcycle_ref(void *h) {
+  /* multigrid.cpp:43-55 */
+  orc_mg *m = (orc_mg *)h;
+  for (int i = 0; i < m->L - 1; ++i) orc_mg_restriction(h, i);
+  orc_mg_coarse_solve(h);
+  for (int i = m->L - 1; i > 0; --i) orc_mg_prolongation(h, i);
+}
+
+void orc_mg_fullcycle_ref(void *h) {
+  /* multigrid.cpp:57-79; the inner prolongation loop at :66 never runs
+   * (i starts at L-1 and requires i < L-1-iter), so it is omitted.     */
+  orc_mg *m = (orc_mg *)h;
+  const int L = m->L;
+  for (int i = 0; i < L - 1; ++i) orc_mg_restriction(h, i);
+  orc_mg_coarse_solve(h);
+  for (int iter = 1; iter < L; ++iter) {
+    for (int i = L - 1 - iter; i < L - 1; ++i) orc_mg_restriction(h, i);
+    orc_mg_coarse_solve(h);
+  }
+  for (int i = L - 1; i > 0; --i) orc_mg_prolongation(h, i);
+}
+
+/* ================================================================== */
+/* build's algorithm: red-black GS V-cycle                             */
+static void lvl_gs_color(const orc_lvl *l, int color) {
+  const int nx = l->nx, ny = l->ny, nz = l->nz;
+  const long sxy = (long)nx * ny;
+  double *x = l->sol;
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++) {
+        if (((i + j + k) & 1) != color) continue;
+        long c = i + (long)nx * (j + (long)ny * k);
+        double t = 0.0;
+        if (i > 0) t += l->cx[c - 1] * x[c - 1];
+        if (i < nx - 1) t += l->cx[c] * x[c + 1];
+        if (j > 0) t += l->cy[c - nx] * x[c - nx];
+        if (j < ny - 1) t += l->cy[c] * x[c + nx];
+        if (k > 0) t += l->cz[c - sxy] * x[c - sxy];
+        if (k < nz - 1) t += l->cz[c] * x[c + sxy];
+        x[c] = (l->rhs[c] - t) / l->diag[c];
+      }
+}
+
+void orc_mg_smooth_rb(void *h, int level, int sweeps, int black_first) {
+  orc_lvl *l = &((orc_mg *)h)->lv[level];
+  for (int s = 0; s < sweeps; s++) {
+    lvl_gs_color(l, black_first ? 1 : 0);
+    lvl_gs_color(l, black_first ? 0 : 1);
+  }
+}
+
+/* weight of coarse cell C in the linear interpolation to fine cell i
+ * (transpose of interp_axis below, same ghost rule)                    */
+static double interp_weight(int i, int r, int C, int nc, int dirichlet);
+
+/* full weighting: coarse rhs = (1/r^d) P^T res, P the linear prolongation
+ * of orc_mg_prolong_rb (so the V-cycle is symmetric).                   */
+void orc_mg_restrict_rb(void *h, int level) {
+  orc_mg *m = (orc_mg *)h;
+  orc_lvl *f = &m->lv[level], *g = &m->lv[level + 1];
+  lvl_apply(f, f->sol, f->buf);
+  for (int c = 0; c < f->n; c++) f->res[c] = f->rhs[c] - f->buf[c];
+  const int r = g->deg / f->deg;
+  const int dir = m->bc == 1;
+  const double inv = 1.0 / (double)(m->ndim == 3 ? r * r * r : r * r);
+  if (m->rkind == 0) {
+    const int rz = (m->ndim == 3) ? r : 1;
+    for (int K = 0; K < g->nz; K++)
+      for (int J = 0; J < g->ny; J++)
+        for (int I = 0; I < g->nx; I++) {
+          long C = I + (long)g->nx * (J + (long)g->ny * K);
+          double s = 0.0;
+          for (int kk = 0; kk < rz; kk++)
+            for (int jj = 0; jj < r; jj++)
+              for (int ii = 0; ii < r; ii++)
+                s += f->res[(I * r + ii) + (long)f->nx * ((J * r + jj) + (long)f->ny * (K * rz + kk))];
+          g->rhs[C] = s * inv;
+          g->sol[C] = 0.0;
+        }
+    return;
+  }
+  for (int K = 0; K < g->nz; K++)
+    for (int J = 0; J < g->ny; J++)
+      for (int I = 0; I < g->nx; I++) {
+        long C = I + (long)g->nx * (J + (long)g->ny * K);
+        int k0 = 0, k1 = 0;
+        if (m->ndim == 3) {
+          k0 = K * r - r;
+          k1 = K * r + 2 * r - 1;
+          if (k0 < 0) k0 = 0;
+          if (k1 > f->nz - 1) k1 = f->nz - 1;
+        }
+        int j0 = J * r - r, j1 = J * r + 2 * r - 1;
+        int i0 = I * r - r, i1 = I * r + 2 * r - 1;
+        if (j0 < 0) j0 = 0;
+        if (j1 > f->ny - 1) j1 = f->ny - 1;
+        if (i0 < 0) i0 = 0;
+        if (i1 > f->nx - 1) i1 = f->nx - 1;
+        double s = 0.0;
+        for (int k = k0; k <= k1; k++) {
+          double wz = (m->ndim == 3) ? interp_weight(k, r, K, g->nz, dir) : 1.0;
+          if (wz == 0.0) continue;
+          for (int j = j0; j <= j1; j++) {
+            double wy = interp_weight(j, r, J, g->ny, dir);
+            if (wy == 0.0) continue;
+            for (int i = i0; i <= i1; i++) {
+              double wx = interp_weight(i, r, I, g->nx, dir);
+              if (wx == 0.0) continue;
+              s += (wx * wy * wz) * f->res[i + (long)f->nx * (j + (long)f->ny * k)];
+            }
+          }
+        }
+        g->rhs[C] = s * inv;
+        g->sol[C] = 0.0;
+      }
+}
+
+/* Cell-centred linear interpolation weight along one axis for fine index
+ * i with ratio r: coarse position X = (2i+1-r)/(2r); I0 = floor(X),
+ * t = X - I0 in exact integer arithmetic.                               */
+static void interp_axis(int i, int r, int *I0, double *t) {
+  int num = 2 * i + 1 - r, den = 2 * r;
+  int q = (num >= 0) ? num / den : -((-num + den - 1) / den);
+  *I0 = q;
+  *t = (double)(num - q * den) / (double)den;
+}
+
+static double interp_weight(int i, int r, int C, int nc, int dirichlet) {
+  int I0;
+  double t, w = 0.0;
+  interp_axis(i, r, &I0, &t);
+  int a = I0, b = I0 + 1;
+  if (!dirichlet) {
+    if (a < 0) a = 0;
+    if (a > nc - 1) a = nc - 1;
+    if (b < 0) b = 0;
+    if (b > nc - 1) b = nc - 1;
+  }
+  if (a == C) w += 1.0 - t;
+  if (b == C) w += t;
+  return w;
+}
+
+static double coarse_at(const orc_mg *m, const orc_lvl *g, int I, int J, int K) {
+  /* ghost rule: Neumann -> nearest interior value; Dirichlet -> 0        */
+  int out = (I < 0 || I >= g->nx || J < 0 || J >= g->ny || K < 0 || K >= g->nz);
+  if (out && m->bc == 1) return 0.0;
+  if (I < 0) I = 0;
+  if (I >= g->nx) I = g->nx - 1;
+  if (J < 0) J = 0;
+  if (J >= g->ny) J = g->ny - 1;
+  if (K < 0) K = 0;
+  if (K >= g->nz) K = g->nz - 1;
+  return g->sol[I + (long)g->nx * (J + (long)g->ny * K)];
+}
+
+void orc_mg_prolong_rb(void *h, int level) {
+  orc_mg *m = (orc_mg *)h;
+  orc_lvl *g = &m->lv[level], *f = &m->lv[level - 1];
+  const int r = g->deg / f->deg;
+  if (m->pkind == 1) {
+    for (int k = 0; k < f->nz; k++)
+      for (int j = 0; j < f->ny; j++)
+        for (int i = 0; i < f->nx; i++)
+          f->sol[i + (long)f->nx * (j + (long)f->ny * k)] +=
+              g->sol[i / r + (long)g->nx * (j / r + (long)g->ny * (m->ndim == 3 ? k / r : 0))];
+    return;
+  }
+  for (int k = 0; k < f->nz; k++)
+    for (int j = 0; j < f->ny; j++)
+      for (int i = 0; i < f->nx; i++) {
+        int I0, J0, K0 = 0;
+        double tx, ty, tz = 0.0;
+        interp_axis(i, r, &I0, &tx);
+        interp_axis(j, r, &J0, &ty);
+        if (m->ndim == 3) interp_axis(k, r, &K0, &tz);
+        double v;
+        double v00 = (1.0 - tx) * coarse_at(m, g, I0, J0, K0) + tx * coarse_at(m, g, I0 + 1, J0, K0);
+        double v10 = (1.0 - tx) * coarse_at(m, g, I0, J0 + 1, K0) + tx * coarse_at(m, g, I0 + 1, J0 + 1, K0);
+        double v0 = (1.0 - ty) * v00 + ty * v10;
+        if (m->ndim == 3) {
+          double v01 = (1.0 - tx) * coarse_at(m, g, I0, J0, K0 + 1) + tx * coarse_at(m, g, I0 + 1, J0, K0 + 1);
+          double v11 = (1.0 - tx) * coarse_at(m, g, I0, J0 + 1, K0 + 1) + tx * coarse_at(m, g, I0 + 1, J0 + 1, K0 + 1);
+          double v1 = (1.0 - ty) * v01 + ty * v11;
+          v = (1.0 - tz) * v0 + tz * v1;
+        } else
+          v = v0;
+        f->sol[i + (long)f->nx * (j + (long)f->ny * k)] += v;
+      }
+}
+
+void orc_mg_coarse_direct(void *h) {
+  orc_mg *m = (orc_mg *)h;
+  orc_lvl *l = &m->lv[m->L - 1];
+  lvl_factor(m, l);
+  dense_lu_solve(l->lu, l->piv, l->n, l->rhs, l->sol);
+  if (m->bc == 0) {
+    double s = 0.0;
+    for (int c = 0; c < l->n; c++) s += l->sol[c];
+    s /= l->n;
+    for (int c = 0; c < l->n; c++) l->sol[c] -= s;
+  }
+}
+
+void orc_mg_vcycle_rb(void *h, int nu1, int nu2) {
+  orc_mg *m = (orc_mg *)h;
+  for (int l = 0; l < m->L - 1; l++) {
+    orc_mg_smooth_rb(h, l, nu1, 0);
+    orc_mg_restrict_rb(h, l);
+  }
+  orc_mg_coarse_direct(h);
+  for (int l = m->L - 1; l > 0; l--) {
+    orc_mg_prolong_rb(h, l);
+    orc_mg_smooth_rb(h, l - 1, nu2, 1);
+  }
+}
+
+static double vec_norm(const double *v, long n) {
+  double s = 0.0;
+  for (long i = 0; i < n; i++) s += v[i] * v[i];
+  return sqrt(s);
+}
+
+int orc_mg_solve_rb(void *h, double rel_tol, int max_cycles, int nu1, int nu2,
+                    double *hist) {
+  orc_mg *m = (orc_mg *)h;
+  orc_lvl *l = &m->lv[0];
+  double bn = vec_norm(l->rhs, l->n);
+  if (bn == 0.0) bn = 1.0;
+  int c = 0;
+  double rr = lvl_residual(l) / bn;
+  while (rr > rel_tol && c < max_cycles) {
+    orc_mg_vcycle_rb(h, nu1, nu2);
+    rr = lvl_residual(l) / bn;
+    if (hist) hist[c] = rr;
+    c++;
+  }
+  if (m->bc == 0) {
+    double s = 0.0;
+    for (int i = 0; i < l->n; i++) s += l->sol[i];
+    s /= l->n;
+    for (int i = 0; i < l->n; i++) l->sol[i] -= s;
+  }
+  return c;
+}
+
+int orc_mg_dense_solve_level0(void *h, const double *b, double *x) {
+  orc_mg *m = (orc_mg *)h;
+  orc_lvl *l = &m->lv[0];
+  if (l->n > 4096) return -1;
+  double *a = (double *)malloc(sizeof(double) * (long)l->n * l->n);
+  int *piv = (int *)malloc(sizeof(int) * l->n);
+  lvl_dense_matrix(m, l, a);
+  int rc = dense_lu(a, piv, l->n);
+  if (rc == 0) {
+    dense_lu_solve(a, piv, l->n, b, x);
+    if (m->bc == 0) {
+      double s = 0.0;
+      for (int i = 0; i < l->n; i++) s += x[i];
+      s /= l->n;
+      for (int i = 0; i < l->n; i++) x[i] -= s;
+    }
+  }
+  free(a);
+  free(piv);
+  return rc;
+}
+
+/* ================================================================== */
+/* padded fields                                                       */
+typedef struct {
+  int ndim, n[3], N[3], off[3];
+  long s[3];
+} orc_pf;
+
+static orc_pf pf_make(int ndim, int nx, int ny, int nz) {
+  orc_pf p;
+  int n[3] = {nx, ny, nz};
+  p.ndim = ndim;
+  for (int a = 0; a < 3; a++) {
+    if (a < ndim) {
+      p.n[a] = n[a];
+      p.N[a] = n[a] + 2 * ORC_GHC;
+      p.off[a] = ORC_GHC;
+    } else {
+      p.n[a] = 1;
+      p.N[a] = 1;
+      p.off[a] = 0;
+    }
+  }
+  p.s[2] = 1;
+  p.s[1] = p.N[2];
+  p.s[0] = (long)p.N[1] * p.N[2];
+  return p;
+}
+
+int orc_padded_size(int ndim, int nx, int ny, int nz, int *N) {
+  orc_pf p = pf_make(ndim, nx, ny, nz);
+  N[0] = p.N[0];
+  N[1] = p.N[1];
+  N[2] = p.N[2];
+  return p.N[0] * p.N[1] * p.N[2];
+}
+
+/* index of padded position (I,J,K) */
+#define PIDX(p, I, J, K) ((I) * (p).s[0] + (J) * (p).s[1] + (K) * (p).s[2])
+
+/* one boundary line along axis a; `b` points to padded position 0 of the
+ * axis in this line; st = stride along the axis.  Formulas follow
+ * neumann.cpp:7-164, dirichlet.cpp:7-118, periodic.cpp:7-113,
+ * no_slip.cpp:7-103, slip.cpp:7-103 (same pattern on every axis).      */
+static void bc_line(double *b, long st, int n, int N, int type, int right,
+                    int face, double U, double h) {
+  const int g = ORC_GHC;
+#define F(q) b[(long)(q) * st]
+  if (right) {
+    const int I = n - 1 + g; /* index_mapping(n, ...) */
+    switch (type) {
+      case 0: /* neumann */
+        if (face) F(I) = F(I - 1) + U * h;
+        for (int i = 1; i <= g; ++i) F(I + i) = F(I + i - 1) + U * h;
+        break;
+      case 1: /* dirichlet */
+        if (face)
+          F(I) = U;
+        else
+          F(I) = F(I - 1) / 3.0 + 2.0 * U / 3.0;
+        for (int i = 1; i <= g; ++i) F(I + i) = 2.0 * F(I + i - 1) - F(I + i - 2);
+        break;
+      case 2: /* periodic (node and face identical) */
+        for (int i = 1; i <= g; ++i) F(I + i) = F(g + (i - 1));
+        break;
+      case 3: /* no slip */
+        if (face) {
+          for (int i = 1; i <= g; ++i) F(I + i) = -F(I - i);
+          F(I) = 0.0;
+        } else
+          for (int i = 1; i <= g; ++i) F(I + i) = -F(I + 1 - i);
+        break;
+      case 4: /* slip */
+        if (face) {
+          for (int i = 1; i <= g; ++i) F(I + i) = F(I - i);
+          F(I) = 0.0;
+        } else
+          for (int i = 1; i <= g; ++i) F(I + i) = F(I + 1 - i);
+        break;
+    }
+  } else {
+    switch (type) {
+      case 0: { /* neumann */
+        if (face) {
+          const int I = g - 1; /* index_mapping(0, ...) */
+          F(I) = F(I + 1) - U * h;
+          for (int i = 1; i <= g - 1; ++i) F(I - i) = F(I - i + 1) - U * h;
+        } else {
+          const int I = g;
+          for (int i = 1; i <= g; ++i) F(I - i) = F(I - i + 1) - U * h;
+        }
+        break;
+      }
+      case 1: { /* dirichlet */
+        const int I = g;
+        if (face) {
+          F(I - 1) = U;
+          for (int i = 2; i <= g; ++i) F(I - i) = 2.0 * F(I - i + 1) - F(I - i + 2);
+        } else {
+          F(I) = F(I + 1) / 3.0 + 2.0 * U / 3.0;
+          for (int i = 1; i <= g; ++i) F(I - i) = 2.0 * F(I - i + 1) - F(I - i + 2);
+        }
+        break;
+      }
+      case 2: {
+        const int I = g;
+        for (int i = 1; i <= g; ++i) F(I - i) = F(N - g - 1 - (i - 1));
+        break;
+      }
+      case 3: {
+        const int I = g;
+        if (face) {
+          for (int i = 1; i <= g; ++i) F(I - i) = -F(I - 2 + i);
+          F(I - 1) = 0.0;
+        } else
+          for (int i = 1; i <= g; ++i) F(I - i) = -F(I - 1 + i);
+        break;
+      }
+      case 4: {
+        const int I = g;
+        if (face) {
+          for (int i = 1; i <= g; ++i) F(I - i) = F(I - 2 + i);
+          F(I - 1) = 0.0;
+        } else
+          for (int i = 1; i <= g; ++i) F(I - i) = F(I - 1 + i);
+        break;
+      }
+    }
+  }
+#undef F
+}
+
+static void bc_side(const orc_pf *p, double *f, int a, int right, int type,
+                    int face, double U, double h) {
+  int t1 = (a + 1) % 3, t2 = (a + 2) % 3;
+  for (int u = 0; u < p->n[t1]; u++)
+    for (int v = 0; v < p->n[t2]; v++) {
+      long base = (long)(u + p->off[t1]) * p->s[t1] + (long)(v + p->off[t2]) * p->s[t2];
+      bc_line(f + base, p->s[a], p->n[a], p->N[a], type, right, face, U, h);
+    }
+}
+
+void orc_bc_apply(int ndim, int nx, int ny, int nz, double dx, double dy,
+                  double dz, const int *types, const double *values, int stag,
+                  double *f) {
+  orc_pf p = pf_make(ndim, nx, ny, nz);
+  double hh[3] = {dx, dy, dz};
+  /* order: x-right, x-left, y-right, y-left, z-right, z-left
+   * (bc_factory.cpp:61-138)                                         */
+  for (int a = 0; a < ndim; a++) {
+    bc_side(&p, f, a, 1, types[2 * a + 1], stag == a, values[2 * a + 1], hh[a]);
+    bc_side(&p, f, a, 0, types[2 * a], stag == a, values[2 * a], hh[a]);
+  }
+}
+
+/* ---------------- WENO-Z (weno_solver.cpp:16-66) ------------------ */
+static void weno_ind_p(double a, double b, double c, double d, double e,
+                       double *o) {
+  o[0] = 13.0 * (c - 2.0 * d + e) * (c - 2.0 * d + e) +
+         3.0 * (3.0 * c - 4.0 * d + e) * (3.0 * c - 4.0 * d + e);
+  o[1] = 13.0 * (b - 2.0 * c + d) * (b - 2.0 * c + d) + 3.0 * (b - d) * (b - d);
+  o[2] = 13.0 * (a - 2.0 * b + c) * (a - 2.0 * b + c) +
+         3.0 * (a - 4.0 * b + 3.0 * c) * (a - 4.0 * b + 3.0 * c);
+}
+static void weno_ind_m(double a, double b, double c, double d, double e,
+                       double *o) {
+  o[0] = 13.0 * (a - 2.0 * b + c) * (a - 2.0 * b + c) +
+         3.0 * (a - 4.0 * b + 3.0 * c) * (a - 4.0 * b + 3.0 * c);
+  o[1] = 13.0 * (b - 2.0 * c + d) * (b - 2.0 * c + d) + 3.0 * (b - d) * (b - d);
+  o[2] = 13.0 * (c - 2.0 * d + e) * (c - 2.0 * d + e) +
+         3.0 * (3.0 * c - 4.0 * d + e) * (3.0 * c - 4.0 * d + e);
+}
+static void weno_int_p(double a, double b, double c, double d, double e,
+                       double *o) {
+  o[2] = (-a + 5.0 * b + 2.0 * c) / 6.0;
+  o[1] = (2.0 * b + 5.0 * c - d) / 6.0;
+  o[0] = (11.0 * c - 7.0 * d + 2.0 * e) / 6.0;
+}
+static void weno_int_m(double a, double b, double c, double d, double e,
+                       double *o) {
+  o[2] = (-e + 5.0 * d + 2.0 * c) / 6.0;
+  o[1] = (2.0 * d + 5.0 * c - b) / 6.0;
+  o[0] = (11.0 * c - 7.0 * b + 2.0 * a) / 6.0;
+}
+static void weno_z_weights(double b0, double b1, double b2, double *w) {
+  double a0 = 1.0 * (1.0 + fabs(b0 - b2) / (orc_eps + b0));
+  double a1 = 6.0 * (1.0 + fabs(b0 - b2) / (orc_eps + b1));
+  double a2 = 3.0 * (1.0 + fabs(b0 - b2) / (orc_eps + b2));
+  w[0] = a0 / (a0 + a1 + a2);
+  w[1] = a1 / (a0 + a1 + a2);
+  w[2] = a2 / (a0 + a1 + a2);
+}
+
+/* derivative along axis a (weno5_flux_x + weno5_find_fx, :68-105, :185-210) */
+static void weno_axis(const orc_pf *p, int a, double h, const double *flux,
+                      const double *vel, double *out) {
+  const int t1 = (a + 1) % 3, t2 = (a + 2) % 3;
+  const long st = p->s[a];
+  const int g = ORC_GHC;
+  double *fh = (double *)malloc(sizeof(double) * (p->n[a] + 2 * g + 2));
+  for (int u = 0; u < p->n[t1]; u++)
+    for (int v = 0; v < p->n[t2]; v++) {
+      long base = (long)(u + p->off[t1]) * p->s[t1] + (long)(v + p->off[t2]) * p->s[t2];
+      const double *F = flux + base;
+      const double *V = vel + base;
+      /* faces P = g-1 .. g+n-1 */
+      for (int P = g - 1; P <= g + p->n[a] - 1; P++) {
+        double in[3], ind[3], w[3], fm, fp;
+        double A = F[(P - 2) * st], B = F[(P - 1) * st], C = F[P * st],
+               D = F[(P + 1) * st], E = F[(P + 2) * st];
+        weno_int_m(A, B, C, D, E, in);
+        weno_ind_m(A, B, C, D, E, ind);
+        weno_z_weights(ind[0], ind[1], ind[2], w);
+        fm = w[0] * in[0] + w[1] * in[1] + w[2] * in[2];
+        A = F[(P - 1) * st];
+        B = F[P * st];
+        C = F[(P + 1) * st];
+        D = F[(P + 2) * st];
+        E = F[(P + 3) * st];
+        weno_int_p(A, B, C, D, E, in);
+        weno_ind_p(A, B, C, D, E, ind);
+        weno_z_weights(ind[0], ind[1], ind[2], w);
+        fp = w[0] * in[0] + w[1] * in[1] + w[2] * in[2];
+        fh[P] = (V[P * st] + V[(P + 1) * st] > 0.0) ? fm : fp;
+      }
+      for (int P = g; P <= g + p->n[a] - 1; P++)
+        out[base + P * st] = (fh[P] - fh[P - 1]) / h;
+    }
+  free(fh);
+}
+
+void orc_weno5_derivatives(int ndim, int nx, int ny, int nz, double dx,
+                           double dy, double dz, const double *flux,
+                           const double *u, const double *v, const double *w,
+                           double *fx, double *fy, double *fz) {
+  orc_pf p = pf_make(ndim, nx, ny, nz);
+  weno_axis(&p, 0, dx, flux, u, fx);
+  if (ndim > 1) weno_axis(&p, 1, dy, flux, v, fy);
+  if (ndim > 2) weno_axis(&p, 2, dz, flux, w, fz);
+}
+
+void orc_convection_weno(int ndim, int nx, int ny, int nz, double dx,
+                         double dy, double dz, const double *phi,
+                         const double *u, const double *v, const double *w,
+                         double *fx, double *fy, double *fz, double *s) {
+  orc_pf p = pf_make(ndim, nx, ny, nz);
+  long tot = (long)p.N[0] * p.N[1] * p.N[2];
+  /* identity_flux: flux <- data (flux.cpp:7-17); phi is used as flux */
+  orc_weno5_derivatives(ndim, nx, ny, nz, dx, dy, dz, phi, u, v, w, fx, fy, fz);
+  for (long i = 0; i < tot; i++) s[i] = -u[i] * fx[i];
+  if (ndim > 1)
+    for (long i = 0; i < tot; i++) s[i] -= v[i] * fy[i];
+  if (ndim > 2)
+    for (long i = 0; i < tot; i++) s[i] -= w[i] * fz[i];
+}
+
+void orc_tvd_rk3_weno(int ndim, int nx, int ny, int nz, double dx, double dy,
+                      double dz, double dt, const int *types,
+                      const double *values, double *phi, const double *u,
+                      const double *v, const double *w, double *fx,
+                      double *fy, double *fz, int nsteps) {
+  orc_pf p = pf_make(ndim, nx, ny, nz);
+  long tot = (long)p.N[0] * p.N[1] * p.N[2];
+  double *s1 = (double *)calloc(tot, sizeof(double));
+  double *s2 = (double *)calloc(tot, sizeof(double));
+  double *s3 = (double *)calloc(tot, sizeof(double));
+  for (int st = 0; st < nsteps; st++) {
+    /* runge_kutta.cpp:13-49 */
+    orc_convection_weno(ndim, nx, ny, nz, dx, dy, dz, phi, u, v, w, fx, fy, fz, s1);
+    for (long i = 0; i < tot; i++) phi[i] += dt * s1[i];
+    orc_bc_apply(ndim, nx, ny, nz, dx, dy, dz, types, values, -1, phi);
+    orc_convection_weno(ndim, nx, ny, nz, dx, dy, dz, phi, u, v, w, fx, fy, fz, s2);
+    for (long i = 0; i < tot; i++) phi[i] += dt * (s2[i] - 3.0 * s1[i]) / 4.0;
+    orc_bc_apply(ndim, nx, ny, nz, dx, dy, dz, types, values, -1, phi);
+    orc_convection_weno(ndim, nx, ny, nz, dx, dy, dz, phi, u, v, w, fx, fy, fz, s3);
+    for (long i = 0; i < tot; i++)
+      phi[i] += dt * (-s1[i] - s2[i] + 8.0 * s3[i]) / 12.0;
+    orc_bc_apply(ndim, nx, ny, nz, dx, dy, dz, types, values, -1, phi);
+  }
+  free(s1);
+  free(s2);
+  free(s3);
+}
+
+/* ---------------- density (lsm.cpp:8-18, wrapper_func.cpp:387-402) -- */
+static double orc_heaviside(double x, double w) {
+  const double pi = acos(-1.0); /* global.h:13 */
+  if (x < -w) return 0;
+  if (x > w) return 1;
+  return 0.5 * (1 + x / w + sin(pi * x / w) / pi);
+}
+
+void orc_density(long n, const double *phi, double ls_width, double ratio,
+                 int positive_ref, double *rho) {
+  for (long i = 0; i < n; i++) {
+    double h = orc_heaviside(phi[i], ls_width);
+    rho[i] = positive_ref ? h + (1.0 - h) * ratio : (1.0 - h) + h * ratio;
+  }
+}
+
+/* ---------------- projection pieces -------------------------------- */
+void orc_ppe_coefficients(int ndim, int nx, int ny, int nz, double dx,
+                          double dy, double dz, const double *rho, double *CC,
+                          double *CR, double *CL, double *CU, double *CD,
+                          double *CF, double *CB) {
+  /* projection_method.cpp:404-465 */
+  orc_pf p = pf_make(ndim, nx, ny, nz);
+  for (int I = 0; I < nx; ++I)
+    for (int J = 0; J < ny; ++J)
+      for (int K = 0; K < (ndim > 2 ? nz : 1); ++K) {
+        long c = PIDX(p, I + p.off[0], J + p.off[1], K + p.off[2]);
+        double R = 0.5 * (rho[c] + rho[c + p.s[0]]);
+        double L = 0.5 * (rho[c] + rho[c - p.s[0]]);
+        double Uu = 0.5 * (rho[c] + rho[c + p.s[1]]);
+        double D = 0.5 * (rho[c] + rho[c - p.s[1]]);
+        CR[c] = 1.0 / R / dx / dx;
+        CL[c] = 1.0 / L / dx / dx;
+        CU[c] = 1.0 / Uu / dy / dy;
+        CD[c] = 1.0 / D / dy / dy;
+        if (I == 0) CL[c] = 0.0;
+        if (I == nx - 1) CR[c] = 0.0;
+        if (J == 0) CD[c] = 0.0;
+        if (J == ny - 1) CU[c] = 0.0;
+        if (ndim > 2) {
+          double Fz = 0.5 * (rho[c] + rho[c + p.s[2]]);
+          double B = 0.5 * (rho[c] + rho[c - p.s[2]]);
+          CF[c] = 1.0 / Fz / dz / dz;
+          CB[c] = 1.0 / B / dz / dz;
+          if (K == 0) CB[c] = 0.0;
+          if (K == nz - 1) CF[c] = 0.0;
+        }
+        CC[c] = -(CR[c] + CL[c] + CU[c] + CD[c]);
+        if (ndim > 2) CC[c] -= (CF[c] + CB[c]);
+      }
+}
+
+void orc_projection_rhs(int ndim, int nx, int ny, int nz, double dx,
+                        double dy, double dz, double dt, const double *u,
+                        const double *v, const double *w, double *rhs) {
+  /* projection_method.cpp:546-568 */
+  orc_pf p = pf_make(ndim, nx, ny, nz);
+  for (int I = 0; I < nx; ++I)
+    for (int J = 0; J < ny; ++J)
+      for (int K = 0; K < (ndim > 2 ? nz : 1); ++K) {
+        long c = PIDX(p, I + p.off[0], J + p.off[1], K + p.off[2]);
+        rhs[c] = (u[c] - u[c - p.s[0]]) / dx + (v[c] - v[c - p.s[1]]) / dy;
+        if (ndim > 2) rhs[c] += (w[c] - w[c - p.s[2]]) / dz;
+        rhs[c] /= dt;
+      }
+}
+
+void orc_projection_velocity(int ndim, int nx, int ny, int nz, double dx,
+                             double dy, double dz, double dt,
+                             const double *rho, const double *pr, double *u,
+                             double *v, double *w) {
+  /* projection_method.cpp:630-654 (loops over the padded range) */
+  orc_pf p = pf_make(ndim, nx, ny, nz);
+  for (int i = 0; i < p.N[0] - 1; ++i)
+    for (int j = 0; j < p.N[1] - 1; ++j)
+      for (int k = 0; k < p.N[2]; ++k) {
+        long c = PIDX(p, i, j, k);
+        double density = 0.5 * (rho[c] + rho[c + p.s[0]]);
+        u[c] -= dt / dx / density * (pr[c + p.s[0]] - pr[c]);
+        density = 0.5 * (rho[c] + rho[c + p.s[1]]);
+        v[c] -= dt / dy / density * (pr[c + p.s[1]] - pr[c]);
+        if (ndim > 2 && k < p.N[2] - 1) {
+          density = 0.5 * (rho[c] + rho[c + p.s[2]]);
+          w[c] -= dt / dz / density * (pr[c + p.s[2]] - pr[c]);
+        }
+      }
+}
+
+int orc_projection_sor(int ndim, int nx, int ny, int nz, double omega,
+                       int max_iter, double tol, double tol2,
+                       const double *CC, const double *CR, const double *CL,
+                       const double *CU, const double *CD, const double *CF,
+                       const double *CB, const double *rhs, double *pr,
+                       double *tmp) {
+  /* projection_method.cpp:572-626, one thread */
+  orc_pf p = pf_make(ndim, nx, ny, nz);
+  long tot = (long)p.N[0] * p.N[1] * p.N[2];
+  int types[6] = {0, 0, 0, 0, 0, 0};
+  double vals[6] = {0, 0, 0, 0, 0, 0};
+  int iter = 0;
+  double error, sump, linf;
+  do {
+    memcpy(tmp, pr, sizeof(double) * tot);
+    sump = 0.0;
+    error = 0.0;
+    for (int i = 0; i < nx; ++i)
+      for (int j = 0; j < ny; ++j)
+        for (int k = 0; k < (ndim > 2 ? nz : 1); ++k) {
+          long c = PIDX(p, i + p.off[0], j + p.off[1], k + p.off[2]);
+          double pnew = (rhs[c] - (CL[c] * pr[c - p.s[0]] + CR[c] * pr[c + p.s[0]] +
+                                   CD[c] * pr[c - p.s[1]] + CU[c] * pr[c + p.s[1]]));
+          if (ndim > 2)
+            pnew -= (CB[c] * pr[c - p.s[2]] + CF[c] * pr[c + p.s[2]]);
+          pnew /= CC[c];
+          pr[c] = omega * pnew + (1.0 - omega) * pr[c];
+          sump += pr[c];
+          error += fabs(pnew * CC[c] - CC[c] * pr[c]);
+        }
+    sump = sump / (nx * ny);
+    if (ndim > 2) sump = sump / nz;
+    for (int i = 0; i < nx; ++i)
+      for (int j = 0; j < ny; ++j)
+        for (int k = 0; k < (ndim > 2 ? nz : 1); ++k)
+          pr[PIDX(p, i + p.off[0], j + p.off[1], k + p.off[2])] -= sump;
+    orc_bc_apply(ndim, nx, ny, nz, 1.0, 1.0, 1.0, types, vals, -1, pr);
+    ++iter;
+    /* linfity (wrapper_func.cpp:420-432) */
+    linf = 0.0;
+    for (long i = 0; i < tot; i++) linf = fmax(linf, fabs(tmp[i] - pr[i]));
+    linf = sqrt(linf / (double)tot);
+  } while (iter < max_iter && error > tol && linf > tol2);
+  return iter;
+}
+
+void orc_fpc_coefficients(int ndim, int nx, int ny, int nz, double dx,
+                          double dy, double dz, double rho12,
+                          const double *rho, const double *f, double *CC,
+                          double *CR, double *CL, double *CU, double *CD,
+                          double *CF, double *CB, double *RHS) {
+  /* projection_method.cpp:467-541 */
+  orc_pf p = pf_make(ndim, nx, ny, nz);
+  for (int I = 0; I < nx; ++I)
+    for (int J = 0; J < ny; ++J)
+      for (int K = 0; K < (ndim > 2 ? nz : 1); ++K) {
+        long c = PIDX(p, I + p.off[0], J + p.off[1], K + p.off[2]);
+        double R = 0.5 * (rho[c] + rho[c + p.s[0]]);
+        double L = 0.5 * (rho[c] + rho[c - p.s[0]]);
+        double Uu = 0.5 * (rho[c] + rho[c + p.s[1]]);
+        double D = 0.5 * (rho[c] + rho[c - p.s[1]]);
+        CR[c] = (1.0 - rho12 / R) / dx / dx;
+        CL[c] = (1.0 - rho12 / L) / dx / dx;
+        CU[c] = (1.0 - rho12 / Uu) / dy / dy;
+        CD[c] = (1.0 - rho12 / D) / dy / dy;
+        if (I == 0) CL[c] = 0.0;
+        if (I == nx - 1) CR[c] = 0.0;
+        if (J == 0) CD[c] = 0.0;
+        if (J == ny - 1) CU[c] = 0.0;
+        if (ndim > 2) {
+          double Fz = 0.5 * (rho[c] + rho[c + p.s[2]]);
+          double B = 0.5 * (rho[c] + rho[c - p.s[2]]);
+          CF[c] = (1.0 - rho12 / Fz) / dz / dz;
+          CB[c] = (1.0 - rho12 / B) / dz / dz;
+          if (K == 0) CB[c] = 0.0;
+          if (K == nz - 1) CF[c] = 0.0;
+        }
+        CC[c] = -(CR[c] + CL[c] + CU[c] + CD[c]);
+        if (ndim > 2) CC[c] -= (CF[c] + CB[c]);
+        RHS[c] = CC[c] * f[c] + CR[c] * f[c + p.s[0]] + CL[c] * f[c - p.s[0]] +
+                 CU[c] * f[c + p.s[1]] + CD[c] * f[c - p.s[1]];
+        if (ndim > 2) RHS[c] += CF[c] * f[c + p.s[2]] + CB[c] * f[c - p.s[2]];
+      }
+}
+
+void orc_fpc_rhs(int ndim, int nx, int ny, int nz, double dx, double dy,
+                 double dz, double dt, double rho12, const double *u,
+                 const double *v, const double *w, const double *RHS,
+                 double *mg_rhs) {
+  /* projection_method.cpp:667-692 */
+  orc_pf p = pf_make(ndim, nx, ny, nz);
+  const int nzz = (ndim > 2 ? nz : 1);
+  for (int I = 0; I < nx; ++I)
+    for (int J = 0; J < ny; ++J)
+      for (int K = 0; K < nzz; ++K) {
+        long c = PIDX(p, I + p.off[0], J + p.off[1], K + p.off[2]);
+        long pos = I + (long)nx * (J + (long)ny * K);
+        mg_rhs[pos] = (u[c] - u[c - p.s[0]]) / dx + (v[c] - v[c - p.s[1]]) / dy;
+        if (ndim > 2) mg_rhs[pos] += (w[c] - w[c - p.s[2]]) / dz;
+        mg_rhs[pos] *= rho12 / dt;
+        mg_rhs[pos] += RHS[c];
+      }
+}
+
+void orc_mg_to_padded(int ndim, int nx, int ny, int nz, const double *sol,
+                      double *pr) {
+  orc_pf p = pf_make(ndim, nx, ny, nz);
+  const int nzz = (ndim > 2 ? nz : 1);
+  for (int I = 0; I < nx; ++I)
+    for (int J = 0; J < ny; ++J)
+      for (int K = 0; K < nzz; ++K)
+        pr[PIDX(p, I + p.off[0], J + p.off[1], K + p.off[2])] =
+            sol[I + (long)nx * (J + (long)ny * K)];
+}
+
+void orc_padded_to_mg(int ndim, int nx, int ny, int nz, const double *pr,
+                      double *sol) {
+  orc_pf p = pf_make(ndim, nx, ny, nz);
+  const int nzz = (ndim > 2 ? nz : 1);
+  for (int I = 0; I < nx; ++I)
+    for (int J = 0; J < ny; ++J)
+      for (int K = 0; K < nzz; ++K)
+        sol[I + (long)nx * (J + (long)ny * K)] =
+            pr[PIDX(p, I + p.off[0], J + p.off[1], K + p.off[2])];
+}
+
+void orc_fpc_velocity(int ndim, int nx, int ny, int nz, double dx, double dy,
+                      double dz, double dt, double rho12, const double *rho,
+                      const double *pr, const double *ph, double *u,
+                      double *v, double *w) {
+  /* projection_method.cpp:723-750 */
+  orc_pf p = pf_make(ndim, nx, ny, nz);
+  for (int i = 0; i < p.N[0] - 1; ++i)
+    for (int j = 0; j < p.N[1] - 1; ++j)
+      for (int k = 0; k < p.N[2]; ++k) {
+        long c = PIDX(p, i, j, k);
+        double px = (pr[c + p.s[0]] - pr[c]) / dx;
+        double px2 = (ph[c + p.s[0]] - ph[c]) / dx;
+        double density = 0.5 * (rho[c] + rho[c + p.s[0]]);
+        u[c] -= dt * ((px - px2) / rho12 + px2 / density);
+        double py = (pr[c + p.s[1]] - pr[c]) / dy;
+        double py2 = (ph[c + p.s[1]] - ph[c]) / dy;
+        density = 0.5 * (rho[c] + rho[c + p.s[1]]);
+        v[c] -= dt * ((py - py2) / rho12 + py2 / density);
+        if (ndim > 2 && k < p.N[2] - 1) {
+          double pz = (pr[c + p.s[2]] - pr[c]) / dz;
+          double pz2 = (ph[c + p.s[2]] - ph[c]) / dz;
+          density = 0.5 * (rho[c] + rho[c + p.s[2]]);
+          w[c] -= dt * ((pz - pz2) / rho12 + pz2 / density);
+        }
+      }
+}
+
+void orc_pressure_guess(long n, double dt_old, double dt_old2,
+                        const double *old, const double *old2, double *phat) {
+  /* projection_method.cpp:803-815 */
+  for (long i = 0; i < n; i++) {
+    double dpdt = (old[i] - old2[i]) / dt_old2;
+    phat[i] = old[i] + dt_old * dpdt;
+  }
+}

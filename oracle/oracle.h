@@ -1,0 +1,187 @@
+/*
+ * oracle.h -- CPU restatement of SciHPC's projection-step hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path:
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load it.  Nothing in scihpc_amd/ links, loads or falls back to it.
+ *
+ * Every function cites the reference file:line it restates (paths relative to
+ * the SciHPC source tree).  Pinning status of each function (what checks it
+ * besides retyping its formula) is listed next to its declaration:
+ *   [golden]  compared with vectors written by the compiled reference
+ *   [math]    closed form / invariant / brute force checks in tests/
+ *   [unpinned] nothing independent checks it yet
+ *
+ * Layouts
+ *   MG vector : unpadded, x fastest, idx = i + nx*(j + ny*k)
+ *               (multigrid_base.cpp:8-14)
+ *   padded    : reference scalar_data layout data[i][j][k] made contiguous,
+ *               k fastest, idx = (I*Ny + J)*Nz + K with ghost width ghc=3 on
+ *               every axis whose interior size is > 1 (scalar_data.cpp:73-158)
+ *
+ * BC type codes: 0 neumann, 1 dirichlet, 2 periodic, 3 no_slip, 4 slip
+ * (bc_factory.h:18-22).  Face order in 6-arrays: xl, xr, yl, yr, zl, zr.
+ */
+#ifndef SCIHPC_ORACLE_H
+#define SCIHPC_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ */
+/* multigrid hierarchy (multigrid.cpp:8-41, 81-122)          [golden]   */
+int orc_mg_levels(int ndim, int nx, int ny, int nz, int maxl,
+                  int *lnx, int *lny, int *lnz, int *ldeg);
+
+/* ------------------------------------------------------------------ */
+/* multigrid object.  bc: 0 = init_NeumannBC (multigrid_base.cpp:236-291),
+ * 1 = init_DirichletBC (multigrid_base.cpp:293-345).  Level spacing is
+ * dx*degree (multigrid.cpp:116-119).                                  */
+void *orc_mg_create(int ndim, int nx, int ny, int nz,
+                    double dx, double dy, double dz, int bc);
+void orc_mg_destroy(void *h);
+int orc_mg_nlevels(void *h);
+void orc_mg_level_shape(void *h, int level, int *shape /* nx,ny,nz,deg,n */);
+/* which: 0 rhs, 1 sol, 2 res */
+double *orc_mg_vec(void *h, int level, int which);
+void orc_mg_set_nocompat(void *h, int level, int flag);
+
+/* Variable coefficient operator A(rho) on level 0, face coefficient
+ * 1/(0.5*(rho_i+rho_nb))/dx/dx, zero on wall faces, diagonal
+ * -(CR+CL+CU+CD)-(CF+CB) (projection_method.cpp:404-465).  Coarse levels
+ * get face coefficients averaged over the fine faces lying on each coarse
+ * face, scaled by (h_fine/h_coarse)^2 (build's choice, no reference).
+ * rho: MG layout, level-0 cells.                           [golden lvl0] */
+void orc_mg_set_rho(void *h, const double *rho);
+/* read back the level's operator: diag and face arrays (cx = face i+1/2) */
+void orc_mg_get_operator(void *h, int level, double *diag, double *cx,
+                         double *cy, double *cz);
+
+/* reference-faithful pieces (multigrid_base.cpp)                      */
+void orc_mg_apply(void *h, int level, const double *x, double *y); /* :388-390 [golden] */
+double orc_mg_residual(void *h, int level);        /* :69-72   [golden] */
+void orc_mg_compat(void *h, int level, int which); /* :47-67   [math]   */
+void orc_mg_relax(void *h, int level, int iter);   /* :16-45   [golden] */
+void orc_mg_restriction(void *h, int level);       /* :74-113  [golden] */
+void orc_mg_prolongation(void *h, int level);      /* :115-208 [golden] */
+void orc_mg_coarse_solve(void *h);                 /* :210-234 [golden] */
+void orc_mg_vcycle_ref(void *h);                   /* multigrid.cpp:43-55 [golden] */
+void orc_mg_fullcycle_ref(void *h);                /* multigrid.cpp:57-79 [golden] */
+
+/* build's algorithm (no reference counterpart; pinned by converging to the
+ * same discrete solution as the reference cycles and a dense solve)  [math]
+ * V(nu1,nu2): red-black Gauss-Seidel (red first going down, black first
+ * going up), residual, child-average restriction, cell-centred
+ * (bi/tri)linear prolongation, dense coarsest solve (pinned for Neumann). */
+/* transfer kinds: rkind 0 child average (default), 1 full weighting
+ * (P^T/r^d); pkind 0 linear (default), 1 piecewise constant           */
+void orc_mg_set_transfer(void *h, int rkind, int pkind);
+void orc_mg_smooth_rb(void *h, int level, int sweeps, int black_first);
+void orc_mg_restrict_rb(void *h, int level);
+void orc_mg_prolong_rb(void *h, int level);
+void orc_mg_coarse_direct(void *h);
+void orc_mg_vcycle_rb(void *h, int nu1, int nu2);
+/* V-cycles on level 0 until ||rhs-A sol||/||rhs|| <= rel_tol; returns cycles
+ * used, writes the relative residual after each cycle into hist[0..]    */
+int orc_mg_solve_rb(void *h, double rel_tol, int max_cycles, int nu1, int nu2,
+                    double *hist);
+/* dense direct solve of level 0 (n <= 4096): Dirichlet -> plain LU;
+ * Neumann -> pinned system, mean-free result.                   [math]   */
+int orc_mg_dense_solve_level0(void *h, const double *b, double *x);
+
+/* ------------------------------------------------------------------ */
+/* padded-field kernels                                                */
+int orc_padded_size(int ndim, int nx, int ny, int nz, int *N /*Nx,Ny,Nz*/);
+
+/* bc_factory::apply_node_centered (bc_factory.cpp:61-138) when stag < 0,
+ * apply_node_staggered_{x,y,z} (bc_factory.cpp:140-375) when stag = 0/1/2.
+ * Per-type formulas: neumann.cpp, dirichlet.cpp, periodic.cpp, no_slip.cpp,
+ * slip.cpp.                                                   [golden]  */
+void orc_bc_apply(int ndim, int nx, int ny, int nz, double dx, double dy,
+                  double dz, const int *types, const double *values,
+                  int stag, double *f);
+
+/* weno_solver::weno5_find_derivatives (weno_solver.cpp:16-278): writes
+ * fx/fy/fz on interior cells only; fm/fp/fh are padded scratch. [golden] */
+void orc_weno5_derivatives(int ndim, int nx, int ny, int nz, double dx,
+                           double dy, double dz, const double *flux,
+                           const double *u, const double *v, const double *w,
+                           double *fx, double *fy, double *fz);
+
+/* convection with WENO derivatives (source.cpp:8-40 with the weno call of
+ * :11): identity_flux (flux.cpp:7-17), derivatives, s over the padded
+ * array.                                                      [golden]  */
+void orc_convection_weno(int ndim, int nx, int ny, int nz, double dx,
+                         double dy, double dz, const double *phi,
+                         const double *u, const double *v, const double *w,
+                         double *fx, double *fy, double *fz, double *s);
+
+/* runge_kutta::tvd_rk3 (runge_kutta.cpp:13-49) with rhs = convection_weno
+ * and bc_factory::apply_node_centered after each stage.  fx/fy/fz persist
+ * between stages exactly like scalar_data's arrays.            [golden]  */
+void orc_tvd_rk3_weno(int ndim, int nx, int ny, int nz, double dx, double dy,
+                      double dz, double dt, const int *types,
+                      const double *values, double *phi, const double *u,
+                      const double *v, const double *w, double *fx,
+                      double *fy, double *fz, int nsteps);
+
+/* lsm.cpp:8-18 Heaviside; wrapper_func.cpp:387-402 find_density [golden] */
+void orc_density(long n, const double *phi, double ls_width, double ratio,
+                 int positive_ref, double *rho);
+
+/* projection_method.cpp:404-465 init_ppe_proj_coefficients on padded
+ * arrays (interior cells written).                             [golden] */
+void orc_ppe_coefficients(int ndim, int nx, int ny, int nz, double dx,
+                          double dy, double dz, const double *rho, double *CC,
+                          double *CR, double *CL, double *CU, double *CD,
+                          double *CF, double *CB);
+/* projection_method.cpp:546-568 RHS = div(u*)/dt on padded arrays [golden] */
+void orc_projection_rhs(int ndim, int nx, int ny, int nz, double dx,
+                        double dy, double dz, double dt, const double *u,
+                        const double *v, const double *w, double *rhs);
+/* projection_method.cpp:630-654 velocity update u -= dt/dx/rho_face*dp [golden] */
+void orc_projection_velocity(int ndim, int nx, int ny, int nz, double dx,
+                             double dy, double dz, double dt,
+                             const double *rho, const double *p, double *u,
+                             double *v, double *w);
+/* projection_method.cpp:572-626 SOR sweeps on the padded pressure, run
+ * serially (the reference loop is a data race under OpenMP).  Neumann BC
+ * after each sweep as the pressure wrapper does.  Returns sweeps. [golden] */
+int orc_projection_sor(int ndim, int nx, int ny, int nz, double omega,
+                       int max_iter, double tol, double tol2,
+                       const double *CC, const double *CR, const double *CL,
+                       const double *CU, const double *CD, const double *CF,
+                       const double *CB, const double *rhs, double *p,
+                       double *tmp);
+
+/* projection_method.cpp:467-541 FPC coefficients and RHS_fpc = sum C*p_hat
+ * (p_hat = pressure->scalar->flux).                            [unpinned] */
+void orc_fpc_coefficients(int ndim, int nx, int ny, int nz, double dx,
+                          double dy, double dz, double rho12,
+                          const double *rho, const double *phat, double *CC,
+                          double *CR, double *CL, double *CU, double *CD,
+                          double *CF, double *CB, double *RHS);
+/* projection_method.cpp:667-692 MG rhs (x-fastest) = rho12/dt*div u + RHS */
+void orc_fpc_rhs(int ndim, int nx, int ny, int nz, double dx, double dy,
+                 double dz, double dt, double rho12, const double *u,
+                 const double *v, const double *w, const double *RHS,
+                 double *mg_rhs);
+/* projection_method.cpp:708-720 p <- MG sol (padded <- x-fastest)        */
+void orc_mg_to_padded(int ndim, int nx, int ny, int nz, const double *sol,
+                      double *p);
+void orc_padded_to_mg(int ndim, int nx, int ny, int nz, const double *p,
+                      double *sol);
+/* projection_method.cpp:723-750 FPC velocity correction          [unpinned] */
+void orc_fpc_velocity(int ndim, int nx, int ny, int nz, double dx, double dy,
+                      double dz, double dt, double rho12, const double *rho,
+                      const double *p, const double *phat, double *u,
+                      double *v, double *w);
+/* projection_method.cpp:803-815 pressure guess p_hat           [unpinned] */
+void orc_pressure_guess(long n, double dt_old, double dt_old2,
+                        const double *old, const double *old2, double *phat);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

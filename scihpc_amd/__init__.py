@@ -1,0 +1,209 @@
+"""MI355X (gfx950) projection-step hot path of SciHPC.
+
+Host-side mirror of the reference's solver classes over the C ABI
+(include/scihpc_hip.h): ``Multigrid`` stands for multigrid/multigrid_base
+(multigrid.h:13-25, multigrid_base.h:17-55), ``LevelSet`` for weno_solver +
+runge_kutta + bc_factory on one padded grid (weno_solver.h:17-48,
+runge_kutta.h:12-23, bc_factory.h:37-59).  Every call runs HIP kernels;
+nothing here computes on the CPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._capi import (BC_CODES, MG_DIRICHLET, MG_NEUMANN, NOT_CONVERGED, OK, PREC_FP64,
+                    PREC_MIXED, ScihpcError, SolveOpts, SolveStats, check, lib)
+
+__all__ = ["Multigrid", "LevelSet", "ScihpcError", "PREC_FP64", "PREC_MIXED", "padded_shape"]
+
+GHC = 3
+
+
+def padded_shape(ndim, nx, ny=1, nz=1):
+    n = (nx, ny, nz)
+    return tuple(n[a] + 2 * GHC if a < ndim else 1 for a in range(3))
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64).ravel()
+
+
+class Multigrid:
+    """multigrid(scalar_data*, structured_grid*) on the GPU.
+
+    ``bc`` is "neumann" (init_NeumannBC) or "dirichlet" (init_DirichletBC).
+    Vectors are numpy arrays in the reference's MG layout (x fastest).
+    """
+
+    def __init__(self, ndim, nx, ny, nz=1, dx=1.0, dy=1.0, dz=1.0, bc="neumann"):
+        self._lib = lib()
+        h = C.c_void_p()
+        code = MG_NEUMANN if bc == "neumann" else MG_DIRICHLET
+        check(self._lib.scihpc_mg_create(ndim, nx, ny, nz, dx, dy, dz, code, C.byref(h)))
+        self.h = h
+        self.ndim = ndim
+        L = C.c_int()
+        check(self._lib.scihpc_mg_num_levels(h, C.byref(L)))
+        self.level_num = L.value
+        self.shapes = []
+        for l in range(self.level_num):
+            a, b, c, d = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+            check(self._lib.scihpc_mg_level_shape(h, l, C.byref(a), C.byref(b), C.byref(c), C.byref(d)))
+            self.shapes.append((a.value, b.value, c.value, d.value))
+        self.n = [s[0] * s[1] * s[2] for s in self.shapes]
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._lib.scihpc_mg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------ data
+    def set_density(self, rho):
+        check(self._lib.scihpc_mg_set_density(self.h, _f64(rho)))
+
+    def set_density_padded(self, rho_padded):
+        check(self._lib.scihpc_mg_set_density_padded(self.h, _f64(rho_padded)))
+
+    def set_rhs(self, x, level=0):
+        check(self._lib.scihpc_mg_set_rhs(self.h, level, _f64(x)))
+
+    def set_sol(self, x, level=0):
+        check(self._lib.scihpc_mg_set_sol(self.h, level, _f64(x)))
+
+    def _get(self, fn, level):
+        out = np.empty(self.n[level])
+        check(fn(self.h, level, out))
+        return out
+
+    def rhs(self, level=0):
+        return self._get(self._lib.scihpc_mg_get_rhs, level)
+
+    def sol(self, level=0):
+        return self._get(self._lib.scihpc_mg_get_sol, level)
+
+    def res(self, level=0):
+        return self._get(self._lib.scihpc_mg_get_res, level)
+
+    def set_nocompat(self, level, flag):
+        check(self._lib.scihpc_mg_set_nocompat(self.h, level, int(flag)))
+
+    # ------------------------------------------------------- operators
+    def residual(self, level=0):
+        r = C.c_double()
+        check(self._lib.scihpc_mg_residual(self.h, level, C.byref(r)))
+        return r.value
+
+    def apply(self, x, level=0):
+        y = np.empty(self.n[level])
+        check(self._lib.scihpc_mg_apply(self.h, level, _f64(x), y))
+        return y
+
+    def compatibility_condition(self, level, which="sol"):
+        check(self._lib.scihpc_mg_compat(self.h, level, 0 if which == "rhs" else 1))
+
+    # ------------------------------------ reference-faithful algorithm
+    def relax(self, level, iters):
+        check(self._lib.scihpc_mg_relax(self.h, level, iters))
+
+    def restriction(self, level):
+        check(self._lib.scihpc_mg_restriction(self.h, level))
+
+    def prolongation(self, level):
+        check(self._lib.scihpc_mg_prolongation(self.h, level))
+
+    def coarse_solve(self):
+        check(self._lib.scihpc_mg_coarse_solve(self.h))
+
+    def v_cycle_ref(self):
+        check(self._lib.scihpc_mg_vcycle_ref(self.h))
+
+    def full_cycle_ref(self):
+        check(self._lib.scihpc_mg_fullcycle_ref(self.h))
+
+    # ------------------------------------------------ MI355X algorithm
+    def v_cycle(self, nu1=2, nu2=2):
+        check(self._lib.scihpc_mg_vcycle(self.h, nu1, nu2))
+
+    def solve(self, rel_tol=1e-8, max_cycles=200, nu1=2, nu2=2, precision=PREC_FP64,
+              use_learned=False, zero_guess=True):
+        o = SolveOpts(rel_tol, max_cycles, nu1, nu2, precision, int(use_learned), int(zero_guess))
+        s = SolveStats()
+        rc = check(self._lib.scihpc_mg_solve(self.h, C.byref(o), C.byref(s)), allow=(OK, NOT_CONVERGED))
+        return {"converged": rc == OK, "cycles": s.cycles, "rel_residual": s.rel_residual,
+                "ms": s.ms, "learned_accepted": s.learned_accepted,
+                "learned_rejected": s.learned_rejected}
+
+    def time_kernel(self, which, iters=20):
+        ms, b = C.c_double(), C.c_double()
+        check(self._lib.scihpc_mg_time_kernel(self.h, which, iters, C.byref(ms), C.byref(b)))
+        return ms.value, b.value
+
+
+class LevelSet:
+    """weno_solver + runge_kutta + bc_factory for one padded grid, on the GPU."""
+
+    def __init__(self, ndim, nx, ny, nz=1, dx=1.0, dy=1.0, dz=1.0, bc_types=None, bc_values=None):
+        self._lib = lib()
+        types = [BC_CODES[t] if isinstance(t, str) else int(t) for t in (bc_types or ["neumann"] * 6)]
+        types = np.ascontiguousarray((types + [0] * 6)[:6], np.int32)
+        vals = np.ascontiguousarray((list(bc_values or []) + [0.0] * 6)[:6], np.float64)
+        h = C.c_void_p()
+        check(self._lib.scihpc_ls_create(ndim, nx, ny, nz, dx, dy, dz, types, vals, C.byref(h)))
+        self.h = h
+        self.ndim = ndim
+        self.shape = padded_shape(ndim, nx, ny, nz)
+        self.size = int(np.prod(self.shape))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._lib.scihpc_ls_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_phi(self, phi):
+        check(self._lib.scihpc_ls_set_phi(self.h, _f64(phi)))
+
+    def phi(self):
+        out = np.empty(self.size)
+        check(self._lib.scihpc_ls_get_phi(self.h, out))
+        return out.reshape(self.shape)
+
+    def set_velocity(self, u, v, w=None):
+        w = np.zeros(self.size) if w is None else w
+        check(self._lib.scihpc_ls_set_velocity(self.h, _f64(u), _f64(v), _f64(w)))
+
+    def weno5_derivatives(self, flux):
+        fx, fy, fz = (np.zeros(self.size) for _ in range(3))
+        check(self._lib.scihpc_ls_weno5_derivatives(self.h, _f64(flux), fx, fy, fz))
+        return fx.reshape(self.shape), fy.reshape(self.shape), fz.reshape(self.shape)
+
+    def tvd_rk3(self, dt, nsteps=1):
+        check(self._lib.scihpc_ls_tvd_rk3(self.h, dt, nsteps))
+
+    def apply_bc(self, f, stag=-1):
+        a = _f64(f).copy()
+        check(self._lib.scihpc_ls_apply_bc(self.h, a, stag))
+        return a.reshape(self.shape)
+
+    def density(self, ls_width, ratio, positive_ref=True):
+        out = np.empty(self.size)
+        check(self._lib.scihpc_ls_density(self.h, ls_width, ratio, int(positive_ref), out))
+        return out.reshape(self.shape)
+
+    def time_step(self, dt, iters=10):
+        ms, b = C.c_double(), C.c_double()
+        check(self._lib.scihpc_ls_time_step(self.h, dt, iters, C.byref(ms), C.byref(b)))
+        return ms.value, b.value

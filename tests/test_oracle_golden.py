@@ -1,0 +1,130 @@
+"""The oracle (oracle/oracle.c) against vectors written by the compiled
+SciHPC reference (tests/golden/*.npz, made by tests/golden/make_golden.py).
+
+Bars: BCs, WENO, density and coefficient arrays bit-exact (same expression
+order, no FMA on either side); steepest-descent and cycle iterates within
+1e-9..1e-12 (the only differences are the order of Eigen's vectorised norm
+and the rounding of the coarsest direct solve, which the all-Neumann
+operator's singularity amplifies)."""
+import numpy as np
+import pytest
+
+from golden_util import load, spacing
+
+MG = load("mg")
+BCS = load("bc")
+WENO = load("weno")
+RK3 = load("rk3")
+FIELDS = load("fields")
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def mk(oracle, c):
+    ndim, n = int(c["ndim"]), c["n"]
+    bc = "neumann" if int(c["bc"]) == 0 else "dirichlet"
+    return oracle.mg(ndim, *map(int, n), *spacing(c), bc)
+
+
+@pytest.mark.parametrize("name", sorted(MG))
+def test_mg_levels_residual_and_pieces(oracle, name):
+    c = MG[name]
+    m = mk(oracle, c)
+    assert [s[:5] for s in m.shapes] == [tuple(int(v) for v in s) for s in c["shapes"]]
+    m.vec(0, "rhs")[:] = c["rhs"]
+    m.vec(0, "sol")[:] = c["x0"]
+    assert abs(m.residual(0) - float(c["r0"])) <= 1e-14 * float(c["r0"])
+    assert np.array_equal(m.vec(0, "res"), c["res0"])
+    oracle.lib.orc_mg_relax(m.h, 0, 3)
+    assert rel(m.vec(0, "sol"), c["relax3"]) < 1e-12
+    oracle.lib.orc_mg_restriction(m.h, 0)
+    assert rel(m.vec(1, "rhs"), c["coarse_rhs"]) < 1e-11
+    m.vec(1, "sol")[:] = c["xc"]
+    m.vec(0, "sol")[:] = c["relax3"]
+    oracle.lib.orc_mg_prolongation(m.h, 1)
+    assert rel(m.vec(0, "sol"), c["prolonged"]) < 1e-11
+
+
+@pytest.mark.parametrize("name", sorted(MG))
+def test_mg_cycles_follow_reference(oracle, name):
+    c = MG[name]
+    m = mk(oracle, c)
+    m.vec(0, "rhs")[:] = c["rhs"]
+    cyc = oracle.lib.orc_mg_vcycle_ref if str(c["kind"]) == "v" else oracle.lib.orc_mg_fullcycle_ref
+    # The all-Neumann coarsest operator is singular: the reference factorises
+    # it as is (multigrid_base.cpp:212-219), the oracle pins its null space,
+    # so the two coarse solutions differ by rounding amplified by the tiny
+    # pivot.  Iterates still agree to 1e-8; the (small) residual norms to 1e-3.
+    dirichlet = int(c["bc"]) == 1
+    tol, rtol = (1e-11, 1e-9) if dirichlet else (1e-8, 1e-3)
+    for k in range(len(c["res"])):
+        cyc(m.h)
+        assert rel(m.vec(0, "sol"), c["sols"][k]) < tol, k
+        assert abs(m.residual(0) - c["res"][k]) <= rtol * c["res"][k]
+
+
+@pytest.mark.parametrize("name", sorted(BCS))
+def test_bc_bit_exact(oracle, name):
+    c = BCS[name]
+    f = c["f"].copy()
+    oracle.bc_apply(f, int(c["ndim"]), tuple(int(v) for v in c["n"]), spacing(c), c["types"].tolist(),
+                    c["vals"], int(c["stag"]))
+    assert np.array_equal(f, c["g"])
+
+
+@pytest.mark.parametrize("name", sorted(WENO))
+def test_weno_bit_exact(oracle, name):
+    c = WENO[name]
+    fx, fy, fz = oracle.weno5(int(c["ndim"]), tuple(int(v) for v in c["n"]), spacing(c), c["f"], c["u"],
+                              c["v"], c["w"])
+    assert np.array_equal(fx, c["fx"])
+    assert np.array_equal(fy, c["fy"])
+    assert np.array_equal(fz, c["fz"])
+
+
+@pytest.mark.parametrize("name", sorted(RK3))
+def test_tvd_rk3_bit_exact(oracle, name):
+    c = RK3[name]
+    phi = c["phi0"].copy()
+    oracle.tvd_rk3(int(c["ndim"]), tuple(int(v) for v in c["n"]), spacing(c), float(c["dt"]),
+                   c["types"].tolist(), c["vals"], phi, c["u"], c["v"], c["w"], int(c["nsteps"]))
+    assert np.array_equal(phi, c["phi3"])
+
+
+@pytest.mark.parametrize("name", sorted(FIELDS))
+def test_density_coefficients_projection(oracle, name):
+    c = FIELDS[name]
+    ndim, n = int(c["ndim"]), tuple(int(v) for v in c["n"])
+    h = spacing(c)
+    assert np.array_equal(oracle.density(c["phi"], 0.05, 1000.0, True), c["rho"])
+    assert np.array_equal(oracle.density(c["phi"], 0.05, 0.2, False), c["rho2"])
+    tot = c["phi"].size
+    C = [np.zeros(tot) for _ in range(8)]
+    oracle.lib.orc_ppe_coefficients(ndim, *n, *h, c["rho"], *C[:7])
+    oracle.lib.orc_projection_rhs(ndim, *n, *h, float(c["dt"]), c["u"], c["v"], c["w"], C[7])
+    ref = c["coef"].reshape(8, tot)
+    for a in range(8):
+        if ndim == 2 and a in (5, 6):
+            continue  # CF/CB are never written in 2-D
+        assert np.array_equal(C[a], ref[a]), a
+    p, tmp = c["p"].copy(), np.zeros(tot)
+    sweeps = oracle.lib.orc_projection_sor(ndim, *n, 1.5, int(c["sweeps"]), 1e-30, 1e-30, *C[:7], C[7], p, tmp)
+    assert sweeps == int(c["sweeps"])
+    assert np.array_equal(p, c["p_out"])
+    u, v, w = c["u"].copy(), c["v"].copy(), c["w"].copy()
+    oracle.lib.orc_projection_velocity(ndim, *n, *h, float(c["dt"]), c["rho"], p, u, v, w)
+    for arr, stag in ((u, 0), (v, 1), (w, 2)):
+        oracle.bc_apply(arr, ndim, n, h, ["no_slip"] * 6, None, stag)
+    assert np.array_equal(u, c["u_out"])
+    assert np.array_equal(v, c["v_out"])
+    if ndim == 3:
+        assert np.array_equal(w, c["w_out"])
+    F = [np.zeros(tot) for _ in range(8)]
+    oracle.lib.orc_fpc_coefficients(ndim, *n, *h, float(c["rho12"]), c["rho"], c["phat"], *F)
+    fref = c["fcoef"].reshape(8, tot)
+    for a in range(8):
+        if ndim == 2 and a in (5, 6):
+            continue
+        assert np.array_equal(F[a], fref[a]), a

@@ -1,0 +1,233 @@
+"""Pins of the oracle that do not rest on the reference's outputs: closed
+forms, invariants, brute force on tiny inputs.  Each check is chosen so that
+a dropped term, a wrong sign/index or a transposed operand fails it."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle.pyoracle import padded_shape
+
+
+def test_level_lists(oracle):
+    # multigrid.cpp:8-41 coarsening rule on the configuration shapes
+    assert [l[:2] for l in oracle.levels(2, 512, 1024)][-1] == (4, 8)
+    assert len(oracle.levels(2, 512, 1024)) == 8
+    assert len(oracle.levels(2, 1024, 512)) == 8
+    assert len(oracle.levels(3, 256, 256, 256)) == 7
+    assert len(oracle.levels(3, 128, 128, 128)) == 6
+    assert [l[0] for l in oracle.levels(2, 144, 144)] == [144, 72, 36, 18, 9]
+    assert len(oracle.levels(2, 257, 257)) == 1          # odd size never coarsens
+    assert [l[3] for l in oracle.levels(2, 54, 54)] == [1, 2, 6]  # 54 -> 27 -> 9 (ratio 3)
+
+
+def dense(m, level=0):
+    n = m.shapes[level][4]
+    return np.array([m.apply(level, e) for e in np.eye(n)]).T
+
+
+@pytest.mark.parametrize("bc", ["neumann", "dirichlet"])
+def test_operator_closed_form(oracle, bc):
+    nx, ny, dx, dy = 6, 5, 0.2, 0.3
+    m = oracle.mg(2, nx, ny, 1, dx, dy, 1.0, bc)
+    A = dense(m)
+    assert np.allclose(A, A.T, rtol=0, atol=1e-12)
+    ax, ay = 1 / dx / dx, 1 / dy / dy
+    # interior row: 5-point Laplacian
+    c = 2 + nx * 2
+    assert A[c, c] == pytest.approx(-2 * ax - 2 * ay)
+    assert A[c, c - 1] == A[c, c + 1] == pytest.approx(ax)
+    assert A[c, c - nx] == A[c, c + nx] == pytest.approx(ay)
+    # corner row: Neumann drops the wall terms, Dirichlet keeps them in the diagonal
+    corner = -ax - ay if bc == "neumann" else -2 * ax - 2 * ay
+    assert A[0, 0] == pytest.approx(corner)
+    if bc == "neumann":
+        assert np.allclose(A @ np.ones(nx * ny), 0, atol=1e-10)
+    else:
+        assert np.all(np.linalg.eigvalsh(A) < 0)
+
+
+def test_variable_operator_closed_form(oracle):
+    nx, ny, dx, dy = 8, 8, 0.25, 0.5
+    rng = np.random.default_rng(0)
+    rho = rng.uniform(1, 1000, nx * ny)
+    m = oracle.mg(2, nx, ny, 1, dx, dy, 1.0, "neumann")
+    m.set_rho(rho)
+    A = dense(m)
+    R = rho.reshape(ny, nx)
+    for j in range(ny):
+        for i in range(nx):
+            c = i + nx * j
+            if i + 1 < nx:
+                assert A[c, c + 1] == pytest.approx(2.0 / (R[j, i] + R[j, i + 1]) / dx ** 2)
+            if j + 1 < ny:
+                assert A[c, c + nx] == pytest.approx(2.0 / (R[j, i] + R[j + 1, i]) / dy ** 2)
+    assert np.allclose(A, A.T)
+    assert np.allclose(A @ np.ones(nx * ny), 0, atol=1e-9)
+    # coarse level: face-averaged coefficients keep symmetry and the null space
+    Ac = dense(m, 1)
+    assert np.allclose(Ac, Ac.T) and np.allclose(Ac @ np.ones(Ac.shape[0]), 0, atol=1e-9)
+
+
+def test_dense_solve_against_numpy(oracle):
+    m = oracle.mg(2, 8, 6, 1, 0.1, 0.2, 1.0, "dirichlet")
+    A = dense(m)
+    b = np.random.default_rng(1).standard_normal(48)
+    assert np.allclose(m.dense_solve(b), np.linalg.solve(A, b), rtol=1e-12, atol=1e-14)
+    m2 = oracle.mg(2, 8, 6, 1, 0.1, 0.2, 1.0, "neumann")
+    A2 = dense(m2)
+    b2 = b - b.mean()
+    x = m2.dense_solve(b2)
+    assert abs(x.mean()) < 1e-12 and np.allclose(A2 @ x, b2, atol=1e-9)
+
+
+@pytest.mark.parametrize("bc", ["neumann", "dirichlet"])
+def test_rb_multigrid_converges_to_dense(oracle, bc):
+    nx, ny = 32, 48
+    m = oracle.mg(2, nx, ny, 1, 1 / nx, 1.5 / ny, 1.0, bc)
+    rng = np.random.default_rng(2)
+    rho = None
+    if bc == "neumann":  # 1000:1 bubble with a smoothed interface
+        i, j = np.meshgrid(np.arange(nx), np.arange(ny), indexing="xy")
+        phi = 0.3 - np.hypot((i + 0.5) / nx - 0.45, (j + 0.5) * 1.5 / ny - 0.8)
+        from scihpc_amd.workloads import density
+        rho = density(phi.ravel(), 1.5 / nx, 1000.0)
+    if rho is not None:
+        m.set_rho(rho)
+    b = rng.standard_normal(nx * ny)
+    if bc == "neumann":
+        b -= b.mean()
+    x = m.dense_solve(b)
+    m.vec(0, "rhs")[:] = b
+    c, hist = m.solve_rb(1e-13, 300)
+    assert hist[-1] <= 1e-13
+    assert np.linalg.norm(m.vec(0, "sol") - x) / np.linalg.norm(x) < 1e-9
+
+
+def test_rb_transfers(oracle):
+    nx, ny = 16, 8
+    m = oracle.mg(2, nx, ny, 1, 1.0, 1.0, 1.0, "neumann")
+    # restriction of a constant residual is that constant (child average)
+    m.vec(0, "rhs")[:] = 3.0
+    m.vec(0, "sol")[:] = 0.0
+    oracle.lib.orc_mg_restrict_rb(m.h, 0)
+    assert np.allclose(m.vec(1, "rhs"), 3.0) and np.all(m.vec(1, "sol") == 0)
+    # prolongation is exact for a linear coarse field on interior fine cells
+    cx, cy = m.shapes[1][:2]
+    I, J = np.meshgrid(np.arange(cx), np.arange(cy), indexing="xy")
+    m.vec(1, "sol")[:] = (2.0 * (I + 0.5) * 2 + 5.0 * (J + 0.5) * 2).ravel()
+    m.vec(0, "sol")[:] = 0.0
+    oracle.lib.orc_mg_prolong_rb(m.h, 1)
+    i, j = np.meshgrid(np.arange(nx), np.arange(ny), indexing="xy")
+    exact = 2.0 * (i + 0.5) + 5.0 * (j + 0.5)
+    f = m.vec(0, "sol").reshape(ny, nx)
+    assert np.allclose(f[1:-1, 1:-1], exact[1:-1, 1:-1])
+
+
+def test_gs_smoother_is_gauss_seidel(oracle):
+    """One red-black sweep equals GS on the red cells then the black cells,
+    brute force on the dense matrix."""
+    nx, ny = 6, 4
+    m = oracle.mg(2, nx, ny, 1, 1.0, 1.0, 1.0, "dirichlet")
+    A = dense(m)
+    rng = np.random.default_rng(3)
+    b, x = rng.standard_normal(nx * ny), rng.standard_normal(nx * ny)
+    m.vec(0, "rhs")[:] = b
+    m.vec(0, "sol")[:] = x
+    oracle.lib.orc_mg_smooth_rb(m.h, 0, 1, 0)
+    y = x.copy()
+    for color in (0, 1):
+        for c in range(nx * ny):
+            if ((c % nx) + (c // nx)) % 2 == color:
+                y[c] = (b[c] - (A[c] @ y - A[c, c] * y[c])) / A[c, c]
+    assert np.allclose(m.vec(0, "sol"), y, rtol=1e-13)
+
+
+def test_weno_exact_on_linear_and_constant(oracle):
+    n = (20, 16, 1)
+    shp = padded_shape(2, *n)
+    I, J = np.meshgrid(np.arange(shp[0]), np.arange(shp[1]), indexing="ij")
+    dx, dy = 0.1, 0.2
+    f = (1.5 * I * dx - 0.7 * J * dy + 3.0).ravel()
+    for sgn in (1.0, -1.0):
+        u = np.full(f.size, sgn)
+        fx, fy, _ = oracle.weno5(2, n, (dx, dy, 1.0), f, u, -u, u)
+        interior = np.zeros(shp, bool)
+        interior[3:-3, 3:-3] = True
+        assert np.allclose(fx.reshape(shp)[interior], 1.5, rtol=1e-12)
+        assert np.allclose(fy.reshape(shp)[interior], -0.7, rtol=1e-12)
+    fx, _, _ = oracle.weno5(2, n, (dx, dy, 1.0), np.full(f.size, 2.0), u, u, u)
+    assert np.all(fx == 0.0)
+
+
+def test_weno_fifth_order(oracle):
+    errs = []
+    for nx in (32, 64, 128):
+        shp = padded_shape(1, nx)
+        dx = 1.0 / nx
+        x = (np.arange(shp[0]) - 3 + 0.5) * dx
+        f = np.sin(2 * math.pi * x)
+        fx, _, _ = oracle.weno5(1, (nx, 1, 1), (dx, 1, 1), f, np.ones_like(f), f * 0, f * 0)
+        errs.append(np.abs(fx[3:-3] - 2 * math.pi * np.cos(2 * math.pi * x[3:-3])).max())
+    orders = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
+    assert orders[-1] > 4.5, orders
+
+
+def test_rk3_closed_forms(oracle):
+    n = (24, 20, 1)
+    h = (0.1, 0.1, 1.0)
+    shp = padded_shape(2, *n)
+    # constant field is invariant
+    phi = np.full(int(np.prod(shp)), 0.7)
+    u = np.full_like(phi, 0.3)
+    oracle.tvd_rk3(2, n, h, 0.01, ["neumann"] * 4, [0.0] * 4, phi, u, -u, u, 4)
+    assert np.all(phi == 0.7)
+    # linear field, constant velocity, periodic-free interior: exact shift
+    I, J = np.meshgrid(np.arange(shp[0]), np.arange(shp[1]), indexing="ij")
+    phi = (0.5 * I * 0.1 + 0.25 * J * 0.1).ravel()
+    phi0 = phi.copy()
+    dt = 0.01
+    oracle.tvd_rk3(2, n, h, dt, ["dirichlet"] * 4, [0.0] * 4, phi, u, -u, u, 1)
+    # away from the walls (BCs enter after the first stage) s = -(0.3*0.5 - 0.3*0.25)
+    core = np.zeros(shp, bool)
+    core[9:-9, 9:-9] = True
+    expect = phi0 + dt * -(0.3 * 0.5 - 0.3 * 0.25)
+    assert np.allclose(phi.reshape(shp)[core], expect.reshape(shp)[core], rtol=1e-12)
+
+
+def test_density_closed_form(oracle):
+    phi = np.array([-1.0, -0.05, 0.0, 0.05, 1.0, 0.025])
+    rho = oracle.density(phi, 0.05, 1000.0, True)
+    assert rho[0] == 1000.0 and rho[4] == 1.0 and rho[1] == 1000.0 and rho[3] == 1.0
+    assert rho[2] == pytest.approx(500.5)
+    H = 0.5 * (1 + 0.5 + math.sin(math.pi * 0.5) / math.pi)
+    assert rho[5] == pytest.approx(H + (1 - H) * 1000.0)
+    assert np.allclose(oracle.density(phi, 0.05, 0.2, False)[[0, 4]], [1.0, 0.2])
+
+
+def test_bc_closed_forms(oracle):
+    n = (5, 4, 1)
+    shp = padded_shape(2, *n)
+    f = np.random.default_rng(4).standard_normal(int(np.prod(shp)))
+    A = f.reshape(shp)
+    g = f.copy()
+    oracle.bc_apply(g, 2, n, (0.5, 0.5, 1.0), ["neumann", "periodic", "no_slip", "slip"][0:1] * 4, [0.2] * 4)
+    G = g.reshape(shp)
+    # Neumann df=0.2, h=0.5: each ghost adds df*h outward
+    assert np.allclose(G[8, 3:7], A[7, 3:7] + 0.1) and np.allclose(G[9, 3:7], A[7, 3:7] + 0.2)
+    assert np.allclose(G[2, 3:7], A[3, 3:7] - 0.1)
+    g = f.copy()
+    oracle.bc_apply(g, 2, n, (0.5, 0.5, 1.0), ["periodic"] * 4, [0.0] * 4)
+    G = g.reshape(shp)
+    assert np.array_equal(G[8:11, 3:7], A[3:6, 3:7]) and np.array_equal(G[0:3, 3:7], A[5:8, 3:7])
+    g = f.copy()
+    oracle.bc_apply(g, 2, n, (0.5, 0.5, 1.0), ["dirichlet"] * 4, [1.0] * 4)
+    G = g.reshape(shp)
+    # last interior cell overwritten (rows j=4,5 are untouched by the y pass)
+    assert np.allclose(G[7, 4:6], A[6, 4:6] / 3 + 2 / 3)
+    assert np.allclose(G[8, 4:6], 2 * G[7, 4:6] - A[6, 4:6])
+    g = f.copy()
+    oracle.bc_apply(g, 2, n, (0.5, 0.5, 1.0), ["no_slip"] * 4, [0.0] * 4, stag=0)
+    G = g.reshape(shp)
+    assert np.all(G[7, 3:7] == 0.0) and np.all(G[2, 3:7] == 0.0)  # wall faces
+    assert np.array_equal(G[8, 3:7], -A[6, 3:7])
