@@ -131,6 +131,9 @@ typedef struct {
   int precision;     /* SCIHPC_PREC_FP64 or SCIHPC_PREC_MIXED */
   int use_learned;   /* 1: learned initial guess/correction (if attached) */
   int zero_guess;    /* 1: start from p = 0, else from the current level-0 sol */
+  int accel;         /* 0: stationary defect correction (one V-cycle per
+                        correction); 1: flexible CG (FCG(1)) with the V-cycle
+                        as preconditioner, FP64 vectors and inner products */
 } scihpc_solve_opts;
 
 typedef struct {

@@ -74,10 +74,14 @@ typedef struct {
   /* operator: diag + coefficient to the +x/+y/+z neighbour (0 when absent);
    * the coefficient to the -x neighbour of cell c is cx[c-1]           */
   double *diag, *cx, *cy, *cz;
-  /* coarsest-level dense factorisation */
-  double *lu;
-  int *piv;
-  int lu_ready;
+  /* diagonal used by the red-black algorithm: equal to diag except on
+   * Dirichlet coarse levels, whose wall term keeps the level-0 boundary
+   * position (see lvl_rb_diag) */
+  double *diag_rb;
+  /* coarsest-level dense factorisations (reference / red-black operator) */
+  double *lu, *lu_rb;
+  int *piv, *piv_rb;
+  int lu_ready, lu_rb_ready;
 } orc_lvl;
 
 typedef struct {
@@ -128,6 +132,40 @@ static void lvl_const_operator(orc_mg *m, orc_lvl *l) {
       }
 }
 
+/* Red-black algorithm's diagonal.  The reference's Dirichlet stencil sets
+ * the value at the ghost-cell centre to zero, i.e. the boundary sits at
+ * distance h0 from the first level-0 cell centre.  A coarse level of
+ * spacing H keeps that boundary when its wall term is 1/(d H) with
+ * d = (H + h0)/2 (linear extrapolation to zero at the boundary), which is
+ * 2/(H^2 (1 + h0/H)); on level 0 this is the reference's 1/h0^2.        */
+static double rb_wall(double H, double h0) { return 2.0 / (H * H * (1.0 + h0 / H)); }
+
+static void lvl_rb_diag(orc_mg *m, orc_lvl *l, double dx0, double dy0, double dz0) {
+  const int nx = l->nx, ny = l->ny, nz = l->nz;
+  if (m->bc != 1 || l->deg == 1) {
+    memcpy(l->diag_rb, l->diag, sizeof(double) * l->n);
+    return;
+  }
+  const double ax = 1.0 / l->dx / l->dx, ay = 1.0 / l->dy / l->dy;
+  const double az = (m->ndim == 3) ? 1.0 / l->dz / l->dz : 0.0;
+  const double wx = rb_wall(l->dx, dx0), wy = rb_wall(l->dy, dy0);
+  const double wz = (m->ndim == 3) ? rb_wall(l->dz, dz0) : 0.0;
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++) {
+        double cc = 0.0;
+        cc += (i > 0) ? ax : wx;
+        cc += (i < nx - 1) ? ax : wx;
+        cc += (j > 0) ? ay : wy;
+        cc += (j < ny - 1) ? ay : wy;
+        if (m->ndim == 3) {
+          cc += (k > 0) ? az : wz;
+          cc += (k < nz - 1) ? az : wz;
+        }
+        l->diag_rb[i + (long)nx * (j + (long)ny * k)] = -cc;
+      }
+}
+
 void *orc_mg_create(int ndim, int nx, int ny, int nz, double dx, double dy,
                     double dz, int bc) {
   orc_mg *m = (orc_mg *)calloc(1, sizeof(orc_mg));
@@ -154,7 +192,9 @@ void *orc_mg_create(int ndim, int nx, int ny, int nz, double dx, double dy,
     v->cx = (double *)calloc(v->n, sizeof(double));
     v->cy = (double *)calloc(v->n, sizeof(double));
     v->cz = (double *)calloc(v->n, sizeof(double));
+    v->diag_rb = (double *)calloc(v->n, sizeof(double));
     lvl_const_operator(m, v);
+    lvl_rb_diag(m, v, dx, dy, dz);
   }
   return m;
 }
@@ -174,6 +214,9 @@ void orc_mg_destroy(void *h) {
     free(v->cz);
     free(v->lu);
     free(v->piv);
+    free(v->diag_rb);
+    free(v->lu_rb);
+    free(v->piv_rb);
   }
   free(m);
 }
@@ -246,6 +289,7 @@ void orc_mg_set_rho(void *h, const double *rho) {
           l->cz[c] = 0.0;
       }
   lvl_diag_from_faces(m, l);
+  memcpy(l->diag_rb, l->diag, sizeof(double) * l->n);
   /* coarse levels: mean of the fine faces on the coarse face / r^2      */
   for (int lv = 1; lv < m->L; lv++) {
     orc_lvl *f = &m->lv[lv - 1], *g = &m->lv[lv];
@@ -280,7 +324,11 @@ void orc_mg_set_rho(void *h, const double *rho) {
     lvl_diag_from_faces(m, g);
     g->lu_ready = 0;
   }
-  m->lv[m->L - 1].lu_ready = 0;
+  for (int lv = 0; lv < m->L; lv++) {
+    memcpy(m->lv[lv].diag_rb, m->lv[lv].diag, sizeof(double) * m->lv[lv].n);
+    m->lv[lv].lu_ready = 0;
+    m->lv[lv].lu_rb_ready = 0;
+  }
 }
 
 void orc_mg_get_operator(void *h, int level, double *diag, double *cx,
@@ -294,7 +342,7 @@ void orc_mg_get_operator(void *h, int level, double *diag, double *cx,
 
 /* y = A x with the accumulation order of Eigen's column-major sparse
  * product (columns ascending): k-, j-, i-, diag, i+, j+, k+.            */
-static void lvl_apply(const orc_lvl *l, const double *x, double *y) {
+static void lvl_apply_d(const orc_lvl *l, const double *diag, const double *x, double *y) {
   const int nx = l->nx, ny = l->ny, nz = l->nz;
   const long sxy = (long)nx * ny;
   for (int k = 0; k < nz; k++)
@@ -305,13 +353,15 @@ static void lvl_apply(const orc_lvl *l, const double *x, double *y) {
         if (k > 0 && l->cz[c - sxy] != 0.0) t += l->cz[c - sxy] * x[c - sxy];
         if (j > 0 && l->cy[c - nx] != 0.0) t += l->cy[c - nx] * x[c - nx];
         if (i > 0 && l->cx[c - 1] != 0.0) t += l->cx[c - 1] * x[c - 1];
-        t += l->diag[c] * x[c];
+        t += diag[c] * x[c];
         if (i < nx - 1 && l->cx[c] != 0.0) t += l->cx[c] * x[c + 1];
         if (j < ny - 1 && l->cy[c] != 0.0) t += l->cy[c] * x[c + nx];
         if (k < nz - 1 && l->cz[c] != 0.0) t += l->cz[c] * x[c + sxy];
         y[c] = t;
       }
 }
+
+static void lvl_apply(const orc_lvl *l, const double *x, double *y) { lvl_apply_d(l, l->diag, x, y); }
 
 void orc_mg_apply(void *h, int level, const double *x, double *y) {
   lvl_apply(&((orc_mg *)h)->lv[level], x, y);
@@ -543,13 +593,13 @@ static void dense_lu_solve(const double *a, const int *piv, int n,
 /* Dense matrix of the level operator; for the all-Neumann (singular)
  * operator the constant null space is pinned: M = A - (s/n) 1 1^T with
  * s = max|diag|, which equals A on mean-free vectors and is nonsingular. */
-static void lvl_dense_matrix(const orc_mg *m, const orc_lvl *l, double *a) {
+static void lvl_dense_matrix(const orc_mg *m, const orc_lvl *l, const double *diag, double *a) {
   const int n = l->n;
   double *e = (double *)calloc(n, sizeof(double));
   double *col = (double *)calloc(n, sizeof(double));
   for (int j = 0; j < n; j++) {
     e[j] = 1.0;
-    lvl_apply(l, e, col);
+    lvl_apply_d(l, diag, e, col);
     e[j] = 0.0;
     for (int i = 0; i < n; i++) a[(long)i * n + j] = col[i];
   }
@@ -568,9 +618,20 @@ static void lvl_factor(const orc_mg *m, orc_lvl *l) {
   free(l->piv);
   l->lu = (double *)malloc(sizeof(double) * (long)l->n * l->n);
   l->piv = (int *)malloc(sizeof(int) * l->n);
-  lvl_dense_matrix(m, l, l->lu);
+  lvl_dense_matrix(m, l, l->diag, l->lu);
   dense_lu(l->lu, l->piv, l->n);
   l->lu_ready = 1;
+}
+
+static void lvl_factor_rb(const orc_mg *m, orc_lvl *l) {
+  if (l->lu_rb_ready) return;
+  free(l->lu_rb);
+  free(l->piv_rb);
+  l->lu_rb = (double *)malloc(sizeof(double) * (long)l->n * l->n);
+  l->piv_rb = (int *)malloc(sizeof(int) * l->n);
+  lvl_dense_matrix(m, l, l->diag_rb, l->lu_rb);
+  dense_lu(l->lu_rb, l->piv_rb, l->n);
+  l->lu_rb_ready = 1;
 }
 
 void orc_mg_coarse_solve(void *h) {
@@ -636,7 +697,7 @@ static void lvl_gs_color(const orc_lvl *l, int color) {
         if (j < ny - 1) t += l->cy[c] * x[c + nx];
         if (k > 0) t += l->cz[c - sxy] * x[c - sxy];
         if (k < nz - 1) t += l->cz[c] * x[c + sxy];
-        x[c] = (l->rhs[c] - t) / l->diag[c];
+        x[c] = (l->rhs[c] - t) / l->diag_rb[c];
       }
 }
 
@@ -657,7 +718,7 @@ static double interp_weight(int i, int r, int C, int nc, int dirichlet);
 void orc_mg_restrict_rb(void *h, int level) {
   orc_mg *m = (orc_mg *)h;
   orc_lvl *f = &m->lv[level], *g = &m->lv[level + 1];
-  lvl_apply(f, f->sol, f->buf);
+  lvl_apply_d(f, f->diag_rb, f->sol, f->buf);
   for (int c = 0; c < f->n; c++) f->res[c] = f->rhs[c] - f->buf[c];
   const int r = g->deg / f->deg;
   const int dir = m->bc == 1;
@@ -791,8 +852,8 @@ void orc_mg_prolong_rb(void *h, int level) {
 void orc_mg_coarse_direct(void *h) {
   orc_mg *m = (orc_mg *)h;
   orc_lvl *l = &m->lv[m->L - 1];
-  lvl_factor(m, l);
-  dense_lu_solve(l->lu, l->piv, l->n, l->rhs, l->sol);
+  lvl_factor_rb(m, l);
+  dense_lu_solve(l->lu_rb, l->piv_rb, l->n, l->rhs, l->sol);
   if (m->bc == 0) {
     double s = 0.0;
     for (int c = 0; c < l->n; c++) s += l->sol[c];
@@ -849,7 +910,7 @@ int orc_mg_dense_solve_level0(void *h, const double *b, double *x) {
   if (l->n > 4096) return -1;
   double *a = (double *)malloc(sizeof(double) * (long)l->n * l->n);
   int *piv = (int *)malloc(sizeof(int) * l->n);
-  lvl_dense_matrix(m, l, a);
+  lvl_dense_matrix(m, l, l->diag, a);
   int rc = dense_lu(a, piv, l->n);
   if (rc == 0) {
     dense_lu_solve(a, piv, l->n, b, x);

@@ -133,8 +133,10 @@ class Multigrid:
         check(self._lib.scihpc_mg_vcycle(self.h, nu1, nu2))
 
     def solve(self, rel_tol=1e-8, max_cycles=200, nu1=2, nu2=2, precision=PREC_FP64,
-              use_learned=False, zero_guess=True):
-        o = SolveOpts(rel_tol, max_cycles, nu1, nu2, precision, int(use_learned), int(zero_guess))
+              use_learned=False, zero_guess=True, accel=0):
+        """accel 0: stationary defect correction; 1: flexible CG around the V-cycle."""
+        o = SolveOpts(rel_tol, max_cycles, nu1, nu2, precision, int(use_learned), int(zero_guess),
+                      int(accel))
         s = SolveStats()
         rc = check(self._lib.scihpc_mg_solve(self.h, C.byref(o), C.byref(s)), allow=(OK, NOT_CONVERGED))
         return {"converged": rc == OK, "cycles": s.cycles, "rel_residual": s.rel_residual,

@@ -27,7 +27,7 @@ class ScihpcError(RuntimeError):
 class SolveOpts(C.Structure):
     _fields_ = [("rel_tol", C.c_double), ("max_cycles", C.c_int), ("nu1", C.c_int),
                 ("nu2", C.c_int), ("precision", C.c_int), ("use_learned", C.c_int),
-                ("zero_guess", C.c_int)]
+                ("zero_guess", C.c_int), ("accel", C.c_int)]
 
 
 class SolveStats(C.Structure):

@@ -20,7 +20,9 @@
 //    prolongation, refinement coarse solve) for drivers that call them.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "common.hpp"
@@ -42,10 +44,15 @@ struct DLevel {
   T ax, ay, az;           // constant-mode coefficients 1/dx^2 ...
   T dconst;               // constant Dirichlet diagonal
   int dirichlet;          // constant mode: 1 = init_DirichletBC operator
+  // Dirichlet coarse levels of the red-black solver: the reference's stencil
+  // zeroes the value at the level-0 ghost centre, so a level of spacing H
+  // keeps that boundary with the wall term w = 2/(H^2 (1 + h0/H))
+  int rbwall;
+  T wx, wy, wz;
 };
 
 // scalars kept on the device by the reference-faithful path
-enum { S_R2 = 0, S_RAR = 1, S_SUM = 2, S_FLAG = 3, S_NORM = 4, S_B2 = 5, S_COUNT = 8 };
+enum { S_R2 = 0, S_RAR = 1, S_SUM = 2, S_FLAG = 3, S_NORM = 4, S_B2 = 5, S_COUNT = 16 };
 
 // -------------------------------------------------------- stencil helpers
 // off = sum over present neighbours of coef*x, diag = A(c,c).  Loads of
@@ -74,41 +81,59 @@ __device__ __forceinline__ void stencil(const DLevel<T> &L, const T *__restrict_
     diag = d;
   } else {
     T o = T(0), cc = T(0);
-    if (i > 0) { o += L.ax * x[c - 1]; cc += L.ax; }
-    if (i < nx - 1) { o += L.ax * x[c + 1]; cc += L.ax; }
-    if (j > 0) { o += L.ay * x[c - sy]; cc += L.ay; }
-    if (j < ny - 1) { o += L.ay * x[c + sy]; cc += L.ay; }
+    if (i > 0) { o += L.ax * x[c - 1]; cc += L.ax; } else if (L.rbwall) cc += L.wx;
+    if (i < nx - 1) { o += L.ax * x[c + 1]; cc += L.ax; } else if (L.rbwall) cc += L.wx;
+    if (j > 0) { o += L.ay * x[c - sy]; cc += L.ay; } else if (L.rbwall) cc += L.wy;
+    if (j < ny - 1) { o += L.ay * x[c + sy]; cc += L.ay; } else if (L.rbwall) cc += L.wy;
     if constexpr (DIM == 3) {
-      if (k > 0) { o += L.az * x[c - sz]; cc += L.az; }
-      if (k < nz - 1) { o += L.az * x[c + sz]; cc += L.az; }
+      if (k > 0) { o += L.az * x[c - sz]; cc += L.az; } else if (L.rbwall) cc += L.wz;
+      if (k < nz - 1) { o += L.az * x[c + sz]; cc += L.az; } else if (L.rbwall) cc += L.wz;
     }
     off = o;
-    diag = L.dirichlet ? L.dconst : -cc;
+    diag = (L.dirichlet && !L.rbwall) ? L.dconst : -cc;
   }
 }
 
+// 2-D five-point stencil from explicit neighbour values and coefficients
+// (used by the LDS-tiled kernels): off-diagonal sum and diagonal.
+template <typename T, bool VAR>
+__device__ __forceinline__ void stencil2(const DLevel<T> &L, int gx, int gy, T xl, T xr, T xd, T xu, T cL, T cR,
+                                         T cD, T cU, T &off, T &diag) {
+  const int nx = L.nx, ny = L.ny;
+  T o = T(0);
+  if constexpr (VAR) {
+    if (gx > 0) o += cL * xl;
+    if (gx < nx - 1) o += cR * xr;
+    if (gy > 0) o += cD * xd;
+    if (gy < ny - 1) o += cU * xu;
+    diag = -(cR + cL + cU + cD);
+  } else {
+    T cc = T(0);
+    if (gx > 0) { o += L.ax * xl; cc += L.ax; } else if (L.rbwall) cc += L.wx;
+    if (gx < nx - 1) { o += L.ax * xr; cc += L.ax; } else if (L.rbwall) cc += L.wx;
+    if (gy > 0) { o += L.ay * xd; cc += L.ay; } else if (L.rbwall) cc += L.wy;
+    if (gy < ny - 1) { o += L.ay * xu; cc += L.ay; } else if (L.rbwall) cc += L.wy;
+    diag = (L.dirichlet && !L.rbwall) ? L.dconst : -cc;
+  }
+  off = o;
+}
+
+// 32-bit index arithmetic: every level has < 2^31 cells, and 64-bit
+// division has no hardware path on CDNA
 __device__ __forceinline__ void cell_ijk(long c, int nx, int ny, int &i, int &j, int &k) {
-  i = (int)(c % nx);
-  long r = c / nx;
-  j = (int)(r % ny);
-  k = (int)(r / ny);
+  const unsigned u = (unsigned)c, ux = (unsigned)nx, uy = (unsigned)ny;
+  const unsigned r = u / ux;
+  i = (int)(u - r * ux);
+  j = (int)(r % uy);
+  k = (int)(r / uy);
 }
 
 // ---------------------------------------------------- red-black smoother
-// One colour of Gauss-Seidel: x_c = (b_c - off) / diag for cells with
-// (i+j+k)%2 == color.  zero_guess: neighbours read as 0 (first half-sweep
-// from a zero initial guess, saves clearing x).
+// Gauss-Seidel update of cell (i,j,k): x_c = (b_c - off) / diag.
+// zero_guess: neighbours read as 0 (first half-sweep from a zero initial
+// guess, which saves clearing x).
 template <typename T, int DIM, bool VAR>
-__global__ void __launch_bounds__(kBlock) k_gs_color(DLevel<T> L, int color, int zero_guess) {
-  const int hx = (L.nx + 1) >> 1;
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long total = (long)hx * L.ny * L.nz;
-  if (t >= total) return;
-  const int i2 = (int)(t % hx);
-  const long r = t / hx;
-  const int j = (int)(r % L.ny), k = (int)(r / L.ny);
-  const int i = 2 * i2 + ((j + k + color) & 1);
-  if (i >= L.nx) return;
+__device__ __forceinline__ void gs_cell(const DLevel<T> &L, int i, int j, int k, int zero_guess) {
   const long c = i + (long)L.nx * (j + (long)L.ny * k);
   T off, diag;
   stencil<T, DIM, VAR>(L, L.sol, i, j, k, c, off, diag);
@@ -116,12 +141,30 @@ __global__ void __launch_bounds__(kBlock) k_gs_color(DLevel<T> L, int color, int
   L.sol[c] = (L.rhs[c] - off) / diag;
 }
 
+// t-th cell of colour `color` ((i+j+k)%2 == color); false when out of range
+__device__ __forceinline__ bool color_cell(long t, int nx, int ny, int nz, int color, int &i, int &j, int &k) {
+  const unsigned hx = (unsigned)((nx + 1) >> 1);
+  const unsigned u = (unsigned)t;
+  if (t >= (long)hx * ny * nz) return false;
+  const unsigned r = u / hx;
+  const int i2 = (int)(u - r * hx);
+  j = (int)(r % (unsigned)ny);
+  k = (int)(r / (unsigned)ny);
+  i = 2 * i2 + ((j + k + color) & 1);
+  return i < nx;
+}
+
+template <typename T, int DIM, bool VAR>
+__global__ void __launch_bounds__(kBlock) k_gs_color(DLevel<T> L, int color, int zero_guess) {
+  int i, j, k;
+  if (color_cell((long)blockIdx.x * blockDim.x + threadIdx.x, L.nx, L.ny, L.nz, color, i, j, k))
+    gs_cell<T, DIM, VAR>(L, i, j, k, zero_guess);
+}
+
 // ------------------------------------- fused residual + child average
 // coarse.rhs = mean over the r^d children of (b - A x), coarse.sol = 0.
 template <typename T, int DIM, bool VAR>
-__global__ void __launch_bounds__(kBlock) k_residual_restrict(DLevel<T> F, DLevel<T> G, int r) {
-  const long C = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (C >= G.n) return;
+__device__ __forceinline__ void rr_cell(const DLevel<T> &F, const DLevel<T> &G, int r, long C) {
   int I, J, K;
   cell_ijk(C, G.nx, G.ny, I, J, K);
   const int rz = DIM == 3 ? r : 1;
@@ -137,6 +180,12 @@ __global__ void __launch_bounds__(kBlock) k_residual_restrict(DLevel<T> F, DLeve
       }
   G.rhs[C] = s / T(r * r * rz);
   G.sol[C] = T(0);
+}
+
+template <typename T, int DIM, bool VAR>
+__global__ void __launch_bounds__(kBlock) k_residual_restrict(DLevel<T> F, DLevel<T> G, int r) {
+  const long C = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (C < G.n) rr_cell<T, DIM, VAR>(F, G, r, C);
 }
 
 // ------------------------------------------------ linear prolongation
@@ -161,9 +210,7 @@ __device__ __forceinline__ T coarse_at(const DLevel<T> &G, int dir, int I, int J
 }
 
 template <typename T, int DIM>
-__global__ void __launch_bounds__(kBlock) k_prolong_add(DLevel<T> G, DLevel<T> F, int r, int dir) {
-  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= F.n) return;
+__device__ __forceinline__ void prolong_cell(const DLevel<T> &G, const DLevel<T> &F, int r, int dir, long c) {
   int i, j, k;
   cell_ijk(c, F.nx, F.ny, i, j, k);
   int I0, J0, K0 = 0;
@@ -190,14 +237,20 @@ __global__ void __launch_bounds__(kBlock) k_prolong_add(DLevel<T> G, DLevel<T> F
   F.sol[c] += v;
 }
 
+template <typename T, int DIM>
+__global__ void __launch_bounds__(kBlock) k_prolong_add(DLevel<T> G, DLevel<T> F, int r, int dir) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < F.n) prolong_cell<T, DIM>(G, F, r, dir, c);
+}
+
 // --------------------------------------------- dense coarsest solve
 // x = Minv b in FP64 on one workgroup; Neumann: subtract the mean (the
 // pinned inverse maps a nonzero-mean b to x0 - mean(b)/s, see set-up).
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_coarse_solve(DLevel<T> G, const double *__restrict__ Minv,
-                                                         int neumann) {
+__device__ void coarse_solve_block(const DLevel<T> &G, const double *__restrict__ Minv, int neumann) {
   __shared__ double xs[kMaxCoarse];
   __shared__ double bs[kMaxCoarse];
+  __shared__ double mean;
   const int n = (int)G.n;
   for (int c = threadIdx.x; c < n; c += blockDim.x) bs[c] = double(G.rhs[c]);
   __syncthreads();
@@ -209,11 +262,372 @@ __global__ void __launch_bounds__(kBlock) k_coarse_solve(DLevel<T> G, const doub
     xs[row] = acc;
     local += acc;
   }
-  double tot = block_sum(local);
-  __shared__ double mean;
+  const double tot = block_sum(local);
   if (threadIdx.x == 0) mean = neumann ? tot / n : 0.0;
   __syncthreads();
   for (int c = threadIdx.x; c < n; c += blockDim.x) G.sol[c] = T(xs[c] - mean);
+  __syncthreads();
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_coarse_solve(DLevel<T> G, const double *__restrict__ Minv,
+                                                         int neumann) {
+  coarse_solve_block(G, Minv, neumann);
+}
+
+template <typename T, int DIM, bool VAR>
+__device__ void block_sweep(const DLevel<T> &L, int first, int zero_guess) {
+  const int hx = (L.nx + 1) >> 1;
+  const long half = (long)hx * L.ny * L.nz;
+  for (int pass = 0; pass < 2; ++pass) {
+    const int color = pass == 0 ? first : 1 - first;
+    for (long t = threadIdx.x; t < half; t += blockDim.x) {
+      int i, j, k;
+      if (color_cell(t, L.nx, L.ny, L.nz, color, i, j, k))
+        gs_cell<T, DIM, VAR>(L, i, j, k, pass == 0 ? zero_guess : 0);
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------- one-workgroup coarse V-cycle
+// The latency-bound coarse levels (from the first level whose whole
+// remaining hierarchy fits in the workgroup's LDS) are cycled by a single
+// 1024-thread workgroup.  Every level's iterate, right-hand side and face
+// coefficients are staged into LDS once; each phase (one colour of a sweep,
+// restriction, dense coarsest solve, prolongation) is a strided loop over the
+// level followed by a workgroup barrier, so phases cost LDS latency rather
+// than global-memory round trips.  Only the top level's iterate is written
+// back.
+constexpr int kMaxBlockLevels = 12;
+constexpr int kBlockThreads = 1024;
+
+template <typename T>
+struct BlockLevels {
+  DLevel<T> lv[kMaxBlockLevels];  // global views (read/write-back)
+  int ratio[kMaxBlockLevels];     // deg(l+1)/deg(l)
+  int off[kMaxBlockLevels];       // element offset of level l in LDS
+  int nlev;
+  int arrays;                     // arrays per level in LDS (2 or 2 + ndim)
+};
+
+template <typename T, int DIM, bool VAR>
+__global__ void __launch_bounds__(kBlockThreads)
+    k_block_vcycle(BlockLevels<T> B, int nu1, int nu2, const double *__restrict__ Minv, int neumann, int dir,
+                   int zero_guess0, int load_top) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T *smem = reinterpret_cast<T *>(smem_raw);
+  const int L = B.nlev;
+  // LDS views of the same levels (kept in LDS: a per-thread array indexed by
+  // the level would live in scratch memory)
+  __shared__ DLevel<T> S[kMaxBlockLevels];
+  if (threadIdx.x == 0) {
+    for (int l = 0; l < L; ++l) {
+      S[l] = B.lv[l];
+      T *base = smem + B.off[l];
+      const long n = B.lv[l].n;
+      S[l].sol = base;
+      S[l].rhs = base + n;
+      if constexpr (VAR) {
+        S[l].cx = base + 2 * n;
+        S[l].cy = base + 3 * n;
+        if constexpr (DIM == 3) S[l].cz = base + 4 * n;
+      }
+    }
+  }
+  __syncthreads();
+  for (int l = 0; l < L; ++l) {
+    const long n = B.lv[l].n;
+    for (long c = threadIdx.x; c < n; c += blockDim.x) {
+      S[l].sol[c] = (l == 0 && load_top && !zero_guess0) ? B.lv[0].sol[c] : T(0);
+      if (l == 0) S[l].rhs[c] = B.lv[0].rhs[c];
+      if constexpr (VAR) {
+        const_cast<T *>(S[l].cx)[c] = B.lv[l].cx[c];
+        const_cast<T *>(S[l].cy)[c] = B.lv[l].cy[c];
+        if constexpr (DIM == 3) const_cast<T *>(S[l].cz)[c] = B.lv[l].cz[c];
+      }
+    }
+  }
+  __syncthreads();
+  for (int l = 0; l < L - 1; ++l) {
+    for (int s = 0; s < nu1; ++s) block_sweep<T, DIM, VAR>(S[l], 0, (l == 0 && s == 0) ? zero_guess0 : 0);
+    for (long C = threadIdx.x; C < S[l + 1].n; C += blockDim.x) rr_cell<T, DIM, VAR>(S[l], S[l + 1], B.ratio[l], C);
+    __syncthreads();
+  }
+  {  // dense coarsest solve, FP64, rows in registers
+    const DLevel<T> &G = S[L - 1];
+    const int n = (int)G.n;
+    double xr[2] = {0.0, 0.0};
+    double local = 0.0;
+    for (int q = 0; q < 2; ++q) {
+      const int row = threadIdx.x + q * blockDim.x;
+      if (row < n) {
+        const double *m = Minv + (long)row * n;
+        double acc = 0.0;
+        for (int c = 0; c < n; ++c) acc += m[c] * double(G.rhs[c]);
+        xr[q] = acc;
+        local += acc;
+      }
+    }
+    const double tot = block_sum(local);
+    __shared__ double mean;
+    if (threadIdx.x == 0) mean = neumann ? tot / n : 0.0;
+    __syncthreads();
+    for (int q = 0; q < 2; ++q) {
+      const int row = threadIdx.x + q * blockDim.x;
+      if (row < n) G.sol[row] = T(xr[q] - mean);
+    }
+    __syncthreads();
+  }
+  for (int l = L - 1; l > 0; --l) {
+    for (long c = threadIdx.x; c < S[l - 1].n; c += blockDim.x)
+      prolong_cell<T, DIM>(S[l], S[l - 1], B.ratio[l - 1], dir, c);
+    __syncthreads();
+    for (int s = 0; s < nu2; ++s) block_sweep<T, DIM, VAR>(S[l - 1], 1, 0);
+  }
+  for (long c = threadIdx.x; c < B.lv[0].n; c += blockDim.x) B.lv[0].sol[c] = S[0].sol[c];
+}
+
+// ---------------------------------------- fused 2-D fine-level kernels
+// A workgroup owns a TX x TY tile of a fine level and stages the iterate,
+// right-hand side and face coefficients of the tile plus a kFH-cell halo
+// into LDS with one batch of independent loads.  Red-black sweeps then run
+// on shrinking rings around the tile (a sweep consumes two rings: the first
+// colour is updated one ring further out than the second), recomputing the
+// ring cells that neighbouring tiles own from the same inputs, so tiles need
+// no synchronisation with each other.  Iterates are written to a second
+// buffer (neighbours always read the pre-kernel state).
+//   k_pre2d : nu1 sweeps (red first), residual, child-average restriction to
+//             the coarse level (coarse sol = 0), fine tile written.
+//   k_post2d: coarse correction prolonged onto the whole staged region, nu2
+//             sweeps (black first), fine tile written; MODE 1 also reduces the
+//             squared residual of the tile (FP64 convergence test); MODE 2 is
+//             the FP32 level 0 of the mixed solve: instead of writing the
+//             correction e it forms p_new = p + e and the FP64 residual
+//             b - A p_new of its tile, writing p_new, float(r) and the tile's
+//             sum of r^2 (the outer defect-correction step, fused).
+// Tiles are ordered so the workgroups that land on one XCD (round robin
+// over 8) own a contiguous band of rows and share halos in that XCD's L2.
+constexpr int kFH = 5;  // staged halo: 2*nu + 1 with nu <= 2
+constexpr int kFusedThreads = 256;
+
+__device__ __forceinline__ int xcd_tile(int ntiles) {
+  const int b = blockIdx.x;
+  if (ntiles % 8 != 0) return b;
+  return (b % 8) * (ntiles / 8) + b / 8;
+}
+
+template <typename T, bool VAR, int TX, int TY>
+struct Fused {
+  static constexpr int W = TX + 2 * kFH, H = TY + 2 * kFH;
+};
+
+// stage x (unless zero), b, faces of the region into LDS
+template <typename T, bool VAR, int TX, int TY>
+__device__ __forceinline__ void fused_stage(const DLevel<T> &F, int x0, int y0, bool zero_x,
+                                            T (*xs)[TX + 2 * kFH], T (*bs)[TX + 2 * kFH],
+                                            T (*cxs)[TX + 2 * kFH], T (*cys)[TX + 2 * kFH]) {
+  constexpr int W = TX + 2 * kFH, H = TY + 2 * kFH;
+  for (int q = threadIdx.x; q < W * H; q += blockDim.x) {
+    const int ly = q / W, lx = q % W;
+    const int gx = x0 - kFH + lx, gy = y0 - kFH + ly;
+    const bool in = gx >= 0 && gx < F.nx && gy >= 0 && gy < F.ny;
+    const long g = gx + (long)F.nx * gy;
+    xs[ly][lx] = (in && !zero_x) ? F.sol[g] : T(0);
+    bs[ly][lx] = in ? F.rhs[g] : T(0);
+    if constexpr (VAR) {
+      cxs[ly][lx] = in ? F.cx[g] : T(0);
+      cys[ly][lx] = in ? F.cy[g] : T(0);
+    }
+  }
+}
+
+// one colour on the tile grown by `ring`
+template <typename T, bool VAR, int TX, int TY>
+__device__ __forceinline__ void fused_pass(const DLevel<T> &F, int x0, int y0, int color, int ring,
+                                           T (*xs)[TX + 2 * kFH], T (*bs)[TX + 2 * kFH],
+                                           T (*cxs)[TX + 2 * kFH], T (*cys)[TX + 2 * kFH]) {
+  const int w = TX + 2 * ring, h = TY + 2 * ring;
+  for (int q = threadIdx.x; q < w * h; q += blockDim.x) {
+    const int ly = q / w + kFH - ring, lx = q % w + kFH - ring;
+    const int gx = x0 - kFH + lx, gy = y0 - kFH + ly;
+    if (gx < 0 || gx >= F.nx || gy < 0 || gy >= F.ny || ((gx + gy) & 1) != color) continue;
+    T off, diag;
+    T cL = T(0), cR = T(0), cD = T(0), cU = T(0);
+    if constexpr (VAR) {
+      cL = cxs[ly][lx - 1];
+      cR = cxs[ly][lx];
+      cD = cys[ly - 1][lx];
+      cU = cys[ly][lx];
+    }
+    stencil2<T, VAR>(F, gx, gy, xs[ly][lx - 1], xs[ly][lx + 1], xs[ly - 1][lx], xs[ly + 1][lx], cL, cR, cD, cU, off,
+                     diag);
+    xs[ly][lx] = (bs[ly][lx] - off) / diag;
+  }
+}
+
+// residual of tile cell (lx, ly) from LDS
+template <typename T, bool VAR, int TX>
+__device__ __forceinline__ T fused_res(const DLevel<T> &F, int gx, int gy, int lx, int ly,
+                                       T (*xs)[TX + 2 * kFH], T (*bs)[TX + 2 * kFH], T (*cxs)[TX + 2 * kFH],
+                                       T (*cys)[TX + 2 * kFH]) {
+  T off, diag;
+  T cL = T(0), cR = T(0), cD = T(0), cU = T(0);
+  if constexpr (VAR) {
+    cL = cxs[ly][lx - 1];
+    cR = cxs[ly][lx];
+    cD = cys[ly - 1][lx];
+    cU = cys[ly][lx];
+  }
+  stencil2<T, VAR>(F, gx, gy, xs[ly][lx - 1], xs[ly][lx + 1], xs[ly - 1][lx], xs[ly + 1][lx], cL, cR, cD, cU, off,
+                   diag);
+  return bs[ly][lx] - (off + diag * xs[ly][lx]);
+}
+
+template <typename T, bool VAR, int TX, int TY>
+__global__ void __launch_bounds__(kFusedThreads)
+    k_pre2d(DLevel<T> F, DLevel<T> G, T *__restrict__ out, int nu1, int zero_guess) {
+  constexpr int W = TX + 2 * kFH, H = TY + 2 * kFH;
+  __shared__ T xs[H][W];
+  __shared__ T bs[H][W];
+  __shared__ T cxs[VAR ? H : 1][W];
+  __shared__ T cys[VAR ? H : 1][W];
+  const int tiles_x = (F.nx + TX - 1) / TX, ntiles = tiles_x * ((F.ny + TY - 1) / TY);
+  const int tile = xcd_tile(ntiles);
+  if (tile >= ntiles) return;
+  const int x0 = (tile % tiles_x) * TX, y0 = (tile / tiles_x) * TY;
+  fused_stage<T, VAR, TX, TY>(F, x0, y0, zero_guess != 0, xs, bs, cxs, cys);
+  __syncthreads();
+  for (int s = 0; s < nu1; ++s) {
+    fused_pass<T, VAR, TX, TY>(F, x0, y0, 0, 2 * nu1 - 2 * s, xs, bs, cxs, cys);
+    __syncthreads();
+    fused_pass<T, VAR, TX, TY>(F, x0, y0, 1, 2 * nu1 - 1 - 2 * s, xs, bs, cxs, cys);
+    __syncthreads();
+  }
+  // residual + child average (ratio 2, tiles are even-aligned)
+  for (int q = threadIdx.x; q < (TX / 2) * (TY / 2); q += blockDim.x) {
+    const int I = x0 / 2 + q % (TX / 2), J = y0 / 2 + q / (TX / 2);
+    if (I >= G.nx || J >= G.ny) continue;
+    T sum = T(0);
+    for (int jj = 0; jj < 2; ++jj)
+      for (int ii = 0; ii < 2; ++ii) {
+        const int gx = 2 * I + ii, gy = 2 * J + jj;
+        sum += fused_res<T, VAR, TX>(F, gx, gy, gx - x0 + kFH, gy - y0 + kFH, xs, bs, cxs, cys);
+      }
+    const long C = I + (long)G.nx * J;
+    G.rhs[C] = sum / T(4);
+    G.sol[C] = T(0);
+  }
+  for (int q = threadIdx.x; q < TX * TY; q += blockDim.x) {
+    const int gx = x0 + q % TX, gy = y0 + q / TX;
+    if (gx < F.nx && gy < F.ny) out[gx + (long)F.nx * gy] = xs[gy - y0 + kFH][gx - x0 + kFH];
+  }
+}
+
+struct OuterArgs {  // FP64 level 0 of the mixed solve (MODE 2)
+  DLevel<double> P;         // rhs = b, faces / constants of the FP64 operator
+  const double *p_old;      // current FP64 iterate
+  double *p_new;            // updated FP64 iterate
+  float *r32;               // float(b - A p_new) -> next FP32 right-hand side
+};
+
+template <typename T, bool VAR, int TX, int TY, int MODE>
+__global__ void __launch_bounds__(kFusedThreads)
+    k_post2d(DLevel<T> G, DLevel<T> F, T *__restrict__ out, int nu2, int dir, double *__restrict__ partial,
+             OuterArgs oa) {
+  constexpr int W = TX + 2 * kFH, H = TY + 2 * kFH;
+  constexpr int CW = TX / 2 + 8, CH = TY / 2 + 8;
+  __shared__ T xs[H][W];
+  __shared__ T bs[H][W];
+  __shared__ T cxs[VAR ? H : 1][W];
+  __shared__ T cys[VAR ? H : 1][W];
+  __shared__ T cs[CH][CW];
+  const int tiles_x = (F.nx + TX - 1) / TX, ntiles = tiles_x * ((F.ny + TY - 1) / TY);
+  const int tile = xcd_tile(ntiles);
+  if (tile >= ntiles) {
+    if (MODE != 0 && threadIdx.x == 0) partial[blockIdx.x] = 0.0;
+    return;
+  }
+  const int x0 = (tile % tiles_x) * TX, y0 = (tile / tiles_x) * TY;
+  const int cx0 = x0 / 2 - 4, cy0 = y0 / 2 - 4;
+  fused_stage<T, VAR, TX, TY>(F, x0, y0, false, xs, bs, cxs, cys);
+  for (int q = threadIdx.x; q < CW * CH; q += blockDim.x) {
+    const int ly = q / CW, lx = q % CW;
+    int I = cx0 + lx, J = cy0 + ly;
+    const bool out_c = I < 0 || I >= G.nx || J < 0 || J >= G.ny;
+    T v = T(0);
+    if (!(out_c && dir)) {
+      I = min(max(I, 0), G.nx - 1);
+      J = min(max(J, 0), G.ny - 1);
+      v = G.sol[I + (long)G.nx * J];
+    }
+    cs[ly][lx] = v;
+  }
+  __syncthreads();
+  // prolongation onto the whole staged region (cell-centred linear, ratio 2)
+  for (int q = threadIdx.x; q < W * H; q += blockDim.x) {
+    const int ly = q / W, lx = q % W;
+    const int gx = x0 - kFH + lx, gy = y0 - kFH + ly;
+    if (gx < 0 || gx >= F.nx || gy < 0 || gy >= F.ny) continue;
+    int I0, J0;
+    float tfx, tfy;
+    double tdx, tdy;
+    interp_axis(gx, 2, I0, tfx, tdx);
+    interp_axis(gy, 2, J0, tfy, tdy);
+    const T tx = sizeof(T) == 8 ? T(tdx) : T(tfx), ty = sizeof(T) == 8 ? T(tdy) : T(tfy);
+    const int a = I0 - cx0, b = J0 - cy0;
+    const T v00 = (T(1) - tx) * cs[b][a] + tx * cs[b][a + 1];
+    const T v10 = (T(1) - tx) * cs[b + 1][a] + tx * cs[b + 1][a + 1];
+    xs[ly][lx] += (T(1) - ty) * v00 + ty * v10;
+  }
+  __syncthreads();
+  for (int s = 0; s < nu2; ++s) {
+    fused_pass<T, VAR, TX, TY>(F, x0, y0, 1, 2 * nu2 - 2 * s, xs, bs, cxs, cys);
+    __syncthreads();
+    fused_pass<T, VAR, TX, TY>(F, x0, y0, 0, 2 * nu2 - 1 - 2 * s, xs, bs, cxs, cys);
+    __syncthreads();
+  }
+  double local = 0.0;
+  for (int q = threadIdx.x; q < TX * TY; q += blockDim.x) {
+    const int gx = x0 + q % TX, gy = y0 + q / TX;
+    if (gx >= F.nx || gy >= F.ny) continue;
+    const int lx = gx - x0 + kFH, ly = gy - y0 + kFH;
+    const long g = gx + (long)F.nx * gy;
+    if constexpr (MODE == 0) {
+      out[g] = xs[ly][lx];
+    } else if constexpr (MODE == 1) {
+      out[g] = xs[ly][lx];
+      const double r = double(fused_res<T, VAR, TX>(F, gx, gy, lx, ly, xs, bs, cxs, cys));
+      local += r * r;
+    } else {
+      // p_new = p_old + e on the cell and its neighbours, FP64 residual
+      const DLevel<double> &P = oa.P;
+      const long nx = F.nx;
+      auto pv = [&](long gg, int llx, int lly) { return oa.p_old[gg] + double(xs[lly][llx]); };
+      const double pc = pv(g, lx, ly);
+      const double pl = gx > 0 ? pv(g - 1, lx - 1, ly) : 0.0;
+      const double pr = gx < F.nx - 1 ? pv(g + 1, lx + 1, ly) : 0.0;
+      const double pd = gy > 0 ? pv(g - nx, lx, ly - 1) : 0.0;
+      const double pu = gy < F.ny - 1 ? pv(g + nx, lx, ly + 1) : 0.0;
+      double cL = 0.0, cR = 0.0, cD = 0.0, cU = 0.0;
+      if constexpr (VAR) {
+        cL = gx > 0 ? P.cx[g - 1] : 0.0;
+        cR = P.cx[g];
+        cD = gy > 0 ? P.cy[g - nx] : 0.0;
+        cU = P.cy[g];
+      }
+      double off, diag;
+      stencil2<double, VAR>(P, gx, gy, pl, pr, pd, pu, cL, cR, cD, cU, off, diag);
+      const double r = P.rhs[g] - (off + diag * pc);
+      oa.p_new[g] = pc;
+      oa.r32[g] = float(r);
+      local += r * r;
+    }
+  }
+  if constexpr (MODE != 0) {
+    const double tot = block_sum(local);
+    if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+  }
 }
 
 // --------------------------------------------------- residual + norm
@@ -447,6 +861,110 @@ __global__ void __launch_bounds__(kBlock) k_outer_update(DLevel<double> L, const
   if (threadIdx.x == 0) partial[blockIdx.x] = v;
 }
 
+// ------------------------------------------- flexible CG around the V-cycle
+// FCG(1) (Notay): the V-cycle (FP64, or FP32 in the mixed solve) is the
+// preconditioner z = B r of a conjugate-gradient iteration whose vectors
+// x, r, p, q = A p and inner products stay FP64.  The preconditioner may be
+// non-symmetric and inexact, hence the flexible direction update
+// beta = -(z . q_old) / (p_old . q_old).  All scalars stay on the device.
+enum { S_ZQ = 8, S_PQ = 9, S_PR = 10, S_ALPHA_OK = 11 };
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_dot_zq(const T *__restrict__ z, const double *__restrict__ q, long n,
+                                                   double *__restrict__ partial) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double v = c < n ? double(z[c]) * q[c] : 0.0;
+  v = block_sum(v);
+  if (threadIdx.x == 0) partial[blockIdx.x] = v;
+}
+
+// p = z + beta p_old (beta = 0 on the first iteration), q = A p, partial sums
+// of p.q and p.r
+template <typename T, int DIM, bool VAR>
+__global__ void __launch_bounds__(kBlock)
+    k_fcg_direction(DLevel<double> L, const T *__restrict__ z, const double *__restrict__ p_old,
+                    double *__restrict__ p, double *__restrict__ q, const double *__restrict__ r,
+                    const double *__restrict__ scal, int first, double *__restrict__ part_pq,
+                    double *__restrict__ part_pr) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const double beta = first ? 0.0 : -scal[S_ZQ] / scal[S_PQ];
+  double vpq = 0.0, vpr = 0.0;
+  if (c < L.n) {
+    int i, j, k;
+    cell_ijk(c, L.nx, L.ny, i, j, k);
+    const long sy = L.nx, sz = (long)L.nx * L.ny;
+    auto P = [&](long g) { return double(z[g]) + (first ? 0.0 : beta * p_old[g]); };
+    double off = 0.0, diag;
+    if constexpr (VAR) {
+      const double cR = L.cx[c], cL = i > 0 ? L.cx[c - 1] : 0.0;
+      const double cU = L.cy[c], cD = j > 0 ? L.cy[c - sy] : 0.0;
+      if (i > 0) off += cL * P(c - 1);
+      if (i < L.nx - 1) off += cR * P(c + 1);
+      if (j > 0) off += cD * P(c - sy);
+      if (j < L.ny - 1) off += cU * P(c + sy);
+      diag = -(cR + cL + cU + cD);
+      if constexpr (DIM == 3) {
+        const double cF = L.cz[c], cB = k > 0 ? L.cz[c - sz] : 0.0;
+        if (k > 0) off += cB * P(c - sz);
+        if (k < L.nz - 1) off += cF * P(c + sz);
+        diag -= (cF + cB);
+      }
+    } else {
+      double cc = 0.0;
+      if (i > 0) { off += L.ax * P(c - 1); cc += L.ax; }
+      if (i < L.nx - 1) { off += L.ax * P(c + 1); cc += L.ax; }
+      if (j > 0) { off += L.ay * P(c - sy); cc += L.ay; }
+      if (j < L.ny - 1) { off += L.ay * P(c + sy); cc += L.ay; }
+      if constexpr (DIM == 3) {
+        if (k > 0) { off += L.az * P(c - sz); cc += L.az; }
+        if (k < L.nz - 1) { off += L.az * P(c + sz); cc += L.az; }
+      }
+      diag = L.dirichlet ? L.dconst : -cc;
+    }
+    const double pc = P(c);
+    const double qc = off + diag * pc;
+    p[c] = pc;
+    q[c] = qc;
+    vpq = pc * qc;
+    vpr = pc * r[c];
+  }
+  vpq = block_sum(vpq);
+  vpr = block_sum(vpr);
+  if (threadIdx.x == 0) {
+    part_pq[blockIdx.x] = vpq;
+    part_pr[blockIdx.x] = vpr;
+  }
+}
+
+// x += alpha p, r -= alpha q (alpha = p.r / p.q); r32 = float(r) when given;
+// partial sums of r.r
+__global__ void __launch_bounds__(kBlock)
+    k_fcg_update(long n, double *__restrict__ x, double *__restrict__ r, const double *__restrict__ p,
+                 const double *__restrict__ q, const double *__restrict__ scal, float *__restrict__ r32,
+                 double *__restrict__ partial) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double v = 0.0;
+  if (c < n) {
+    const double alpha = scal[S_PR] / scal[S_PQ];
+    x[c] += alpha * p[c];
+    const double rc = r[c] - alpha * q[c];
+    r[c] = rc;
+    if (r32) r32[c] = float(rc);
+    v = rc * rc;
+  }
+  v = block_sum(v);
+  if (threadIdx.x == 0) partial[blockIdx.x] = v;
+}
+
+__global__ void __launch_bounds__(kBlock) k_copy_cast(const double *__restrict__ a, double *__restrict__ b,
+                                                      float *__restrict__ b32, long n) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < n) {
+    if (b) b[c] = a[c];
+    if (b32) b32[c] = float(a[c]);
+  }
+}
+
 // ------------------------------------------------- operator set-up
 template <int DIM>
 __global__ void __launch_bounds__(kBlock) k_faces_from_rho(DLevel<double> L, const double *__restrict__ rho,
@@ -511,7 +1029,10 @@ struct Level {
   int nx = 0, ny = 0, nz = 0, deg = 1;
   long n = 0;
   double dx = 1, dy = 1, dz = 1;
-  T *sol = nullptr, *rhs = nullptr, *res = nullptr, *buf = nullptr;
+  // `sol` is the current iterate; the fused sweep writes `sol_alt` and the
+  // two pointers are swapped afterwards
+  T *sol = nullptr, *sol_alt = nullptr, *rhs = nullptr, *res = nullptr, *buf = nullptr;
+  T *rhs_alt = nullptr;  // FP32 level 0: next right-hand side written by the fused outer step
   T *cx = nullptr, *cy = nullptr, *cz = nullptr;
 };
 
@@ -525,6 +1046,20 @@ static T *dalloc(long n, hipStream_t stream) {
   return static_cast<T *>(p);
 }
 
+// Tile shapes of the LDS-tiled 2-D kernels, largest first.  A level uses the
+// largest shape that still gives every CU two workgroups (small levels are
+// latency bound: parallelism beats the halo overhead of small tiles).
+template <typename T>
+struct TileCfg;
+template <>
+struct TileCfg<double> {  // largest: 4 x 74 x 22 x 8 B + coarse tile = 57 KiB of LDS
+  static constexpr int TX[3] = {64, 32, 16}, TY[3] = {12, 16, 16};
+};
+template <>
+struct TileCfg<float> {  // largest: 4 x 74 x 42 x 4 B + coarse tile = 54 KiB of LDS
+  static constexpr int TX[3] = {64, 32, 16}, TY[3] = {32, 16, 16};
+};
+
 }  // namespace scihpc
 
 using namespace scihpc;
@@ -535,7 +1070,8 @@ struct scihpc_mg {
   std::vector<Level<double>> d;
   std::vector<Level<float>> f;  // FP32 hierarchy for the mixed path (lazy)
   std::vector<int> nocompat;
-  double *Minv = nullptr;       // dense inverse of the (pinned) coarsest operator
+  double *Minv = nullptr;       // dense inverse of the (pinned) coarsest red-black operator
+  double *Minv_ref = nullptr;   // same for the reference operator (differs for Dirichlet)
   double *partial = nullptr;    // per-block partial sums
   double *scal = nullptr;       // device scalars (S_*)
   double *host_scal = nullptr;  // pinned mirror
@@ -543,11 +1079,23 @@ struct scihpc_mg {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   scihpc::LearnedHook learned;
+  long block_max = 1L << 30;  // optional cap on the one-workgroup cycle's top level
+  long lds_max = 65536;       // LDS bytes a workgroup may use (queried at create)
+  int num_cu = 256;           // compute units (queried at create)
+  bool tiled = true;          // LDS-tiled 2-D fine-level kernels (SCIHPC_MG_TILED=0 disables)
 
   // ------------------------------------------------------------ views
+  // rb: view of the red-black solver's operator (Dirichlet coarse levels
+  // get the boundary-consistent wall term); false: the reference operator
   template <typename T>
-  DLevel<T> view(const Level<T> &l) const {
+  DLevel<T> view(const Level<T> &l, bool rb = true) const {
     DLevel<T> v;
+    const auto &l0 = d[0];
+    v.rbwall = rb && bc == SCIHPC_MG_DIRICHLET && !variable && l.deg > 1;
+    auto wall = [](double H, double h0) { return 2.0 / (H * H * (1.0 + h0 / H)); };
+    v.wx = T(wall(l.dx, l0.dx));
+    v.wy = T(wall(l.dy, l0.dy));
+    v.wz = T(ndim == 3 ? wall(l.dz, l0.dz) : 0.0);
     v.nx = l.nx;
     v.ny = l.ny;
     v.nz = l.nz;
@@ -604,29 +1152,202 @@ struct scihpc_mg {
     SCIHPC_CHECK_LAUNCH();
   }
 
-  // V(nu1,nu2) on hierarchy `lv`, in place on lv[0].sol
+  // ------------------------------------------------ tiled 2-D kernels
   template <typename T>
-  void vcycle(std::vector<Level<T>> &lv, int nu1, int nu2, bool zero_guess0) {
-    if (L == 1) {
-      coarse_solve(lv[0]);
-      return;
+  bool fused_level(const std::vector<Level<T>> &lv, int l) const {
+    return tiled && ndim == 2 && l + 1 < L && lv[l + 1].deg / lv[l].deg == 2;
+  }
+  // tile shape index for a level (see TileCfg) and its tile count
+  template <typename T>
+  int tile_kind(const Level<T> &l) const {
+    for (int t = 0; t < 2; ++t)
+      if ((long)ceil_div(l.nx, TileCfg<T>::TX[t]) * ceil_div(l.ny, TileCfg<T>::TY[t]) >= 2L * num_cu) return t;
+    return 2;
+  }
+  template <typename T>
+  unsigned tiles2d(const Level<T> &l) const {
+    const int t = tile_kind(l);
+    return ceil_div(l.nx, TileCfg<T>::TX[t]) * ceil_div(l.ny, TileCfg<T>::TY[t]);
+  }
+
+  template <typename T, bool VAR, int K>
+  void pre2d_k(std::vector<Level<T>> &lv, int l, int nu1, int zero_guess) {
+    constexpr int TX = TileCfg<T>::TX[K], TY = TileCfg<T>::TY[K];
+    auto &a = lv[l];
+    k_pre2d<T, VAR, TX, TY><<<tiles2d(a), kFusedThreads, 0, stream>>>(view(a), view(lv[l + 1]), a.sol_alt, nu1,
+                                                                        zero_guess);
+    SCIHPC_CHECK_LAUNCH();
+    std::swap(a.sol, a.sol_alt);
+  }
+  template <typename T>
+  void pre2d(std::vector<Level<T>> &lv, int l, int nu1, int zero_guess) {
+    const int k = tile_kind(lv[l]);
+    if (variable) {
+      if (k == 0) pre2d_k<T, true, 0>(lv, l, nu1, zero_guess);
+      else if (k == 1) pre2d_k<T, true, 1>(lv, l, nu1, zero_guess);
+      else pre2d_k<T, true, 2>(lv, l, nu1, zero_guess);
+    } else {
+      if (k == 0) pre2d_k<T, false, 0>(lv, l, nu1, zero_guess);
+      else if (k == 1) pre2d_k<T, false, 1>(lv, l, nu1, zero_guess);
+      else pre2d_k<T, false, 2>(lv, l, nu1, zero_guess);
     }
-    for (int l = 0; l < L - 1; ++l) {
-      for (int s = 0; s < nu1; ++s) {
-        gs_color(lv[l], 0, (l == 0 && zero_guess0 && s == 0) ? 1 : 0);
-        gs_color(lv[l], 1, 0);
-      }
-      if (nu1 == 0 && l == 0 && zero_guess0)
-        SCIHPC_HIP(hipMemsetAsync(lv[0].sol, 0, sizeof(T) * lv[0].n, stream));
-      residual_restrict(lv[l], lv[l + 1]);
+  }
+
+  template <typename T, bool VAR, int K, int MODE>
+  void post2d_k(std::vector<Level<T>> &lv, int l, int nu2, const OuterArgs &oa) {
+    constexpr int TX = TileCfg<T>::TX[K], TY = TileCfg<T>::TY[K];
+    auto &a = lv[l];
+    const int dir = bc == SCIHPC_MG_DIRICHLET;
+    k_post2d<T, VAR, TX, TY, MODE><<<tiles2d(a), kFusedThreads, 0, stream>>>(view(lv[l + 1]), view(a), a.sol_alt,
+                                                                               nu2, dir, partial, oa);
+    SCIHPC_CHECK_LAUNCH();
+    if (MODE != 2) std::swap(a.sol, a.sol_alt);
+  }
+  template <typename T, int MODE>
+  void post2d_mode(std::vector<Level<T>> &lv, int l, int nu2, const OuterArgs &oa) {
+    const int k = tile_kind(lv[l]);
+    if (variable) {
+      if (k == 0) post2d_k<T, true, 0, MODE>(lv, l, nu2, oa);
+      else if (k == 1) post2d_k<T, true, 1, MODE>(lv, l, nu2, oa);
+      else post2d_k<T, true, 2, MODE>(lv, l, nu2, oa);
+    } else {
+      if (k == 0) post2d_k<T, false, 0, MODE>(lv, l, nu2, oa);
+      else if (k == 1) post2d_k<T, false, 1, MODE>(lv, l, nu2, oa);
+      else post2d_k<T, false, 2, MODE>(lv, l, nu2, oa);
     }
-    coarse_solve(lv[L - 1]);
-    for (int l = L - 1; l > 0; --l) {
-      prolong_add(lv[l], lv[l - 1]);
-      for (int s = 0; s < nu2; ++s) {
-        gs_color(lv[l - 1], 1, 0);
-        gs_color(lv[l - 1], 0, 0);
+  }
+
+  // generic full sweep (3-D levels, ratio-3 levels): two colour kernels
+  template <typename T>
+  void sweep(Level<T> &l, int first, int zero_guess) {
+    gs_color(l, first, zero_guess);
+    gs_color(l, 1 - first, 0);
+  }
+
+  // first level cycled by the one-workgroup LDS kernel (L when even the
+  // coarsest level does not fit the LDS budget)
+  template <typename T>
+  int block_level(const std::vector<Level<T>> &lv) const {
+    const long bpc = (long)sizeof(T) * (2 + (variable ? ndim : 0));
+    const long budget = lds_max - 2048;
+    long used = 0;
+    int lb = L;
+    for (int l = L - 1; l >= 0 && l >= L - kMaxBlockLevels; --l) {
+      used += lv[l].n * bpc;
+      if (used > budget || lv[l].n > block_max) break;
+      lb = l;
+    }
+    return lb;
+  }
+
+  template <typename T>
+  void block_cycle(std::vector<Level<T>> &lv, int lb, int nu1, int nu2, int zero_guess0, int load_top) {
+    BlockLevels<T> B{};
+    B.nlev = L - lb;
+    B.arrays = 2 + (variable ? ndim : 0);
+    long off = 0;
+    for (int i = 0; i < B.nlev; ++i) {
+      B.lv[i] = view(lv[lb + i]);
+      B.ratio[i] = i + 1 < B.nlev ? lv[lb + i + 1].deg / lv[lb + i].deg : 1;
+      B.off[i] = (int)off;
+      off += lv[lb + i].n * B.arrays;
+    }
+    const size_t bytes = sizeof(T) * off;
+    const int neu = bc == SCIHPC_MG_NEUMANN, dir = bc == SCIHPC_MG_DIRICHLET;
+#define SCIHPC_BLOCK(D, V)                                                                                  \
+  do {                                                                                                      \
+    if (bytes > 65536)                                                                                      \
+      SCIHPC_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_block_vcycle<T, D, V>),             \
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));              \
+    k_block_vcycle<T, D, V><<<1, kBlockThreads, bytes, stream>>>(B, nu1, nu2, Minv, neu, dir, zero_guess0, \
+                                                                  load_top);                                \
+  } while (0)
+    if (ndim == 2) {
+      if (variable) SCIHPC_BLOCK(2, true);
+      else SCIHPC_BLOCK(2, false);
+    } else {
+      if (variable) SCIHPC_BLOCK(3, true);
+      else SCIHPC_BLOCK(3, false);
+    }
+#undef SCIHPC_BLOCK
+    SCIHPC_CHECK_LAUNCH();
+  }
+
+  // V(nu1,nu2) on hierarchy `lv`, in place on lv[0].sol.  Fine 2-D levels
+  // use the tiled pre/post kernels, the coarse levels one LDS workgroup.
+  // post0 selects what the level-0 post-smoothing kernel also does:
+  // 0 nothing, 1 FP64 residual norm (partials), 2 mixed outer step (oa).
+  // Returns the number of residual partials written (0 when post0 == 0).
+  template <typename T>
+  int vcycle(std::vector<Level<T>> &lv, int nu1, int nu2, bool zero_guess0, int post0 = 0,
+             const OuterArgs *oa = nullptr) {
+    const int lb = block_level(lv);
+    if (nu1 == 0 && zero_guess0) SCIHPC_HIP(hipMemsetAsync(lv[0].sol, 0, sizeof(T) * lv[0].n, stream));
+    const bool zg = zero_guess0 && nu1 > 0;
+    const int top = std::min(lb, L - 1);  // deepest level reached by the fine loop
+    for (int l = 0; l < top; ++l) {
+      const int z = (l == 0 && zg) ? 1 : 0;
+      if (fused_level(lv, l)) {
+        pre2d(lv, l, nu1, z);
+      } else {
+        for (int s = 0; s < nu1; ++s) sweep(lv[l], 0, s == 0 ? z : 0);
+        residual_restrict(lv[l], lv[l + 1]);
       }
+    }
+    if (lb < L) {
+      block_cycle(lv, lb, nu1, nu2, (lb == 0 && zg) ? 1 : 0, lb == 0 ? 1 : 0);
+    } else {
+      coarse_solve(lv[L - 1]);
+    }
+    int partials = 0;
+    bool post0_done = false;
+    for (int l = top; l > 0; --l) {
+      const int f = l - 1;
+      if (fused_level(lv, f)) {
+        const int mode = f == 0 ? post0 : 0;
+        OuterArgs none{};
+        if (mode == 0) post2d_mode<T, 0>(lv, f, nu2, none);
+        else if (mode == 1) post2d_mode<T, 1>(lv, f, nu2, none);
+        else post2d_mode<T, 2>(lv, f, nu2, *oa);
+        if (f == 0 && mode != 0) {
+          partials = (int)tiles2d(lv[0]);
+          post0_done = true;
+        }
+      } else {
+        prolong_add(lv[l], lv[f]);
+        for (int s = 0; s < nu2; ++s) sweep(lv[f], 1, 0);
+      }
+    }
+    if (post0 != 0 && !post0_done) partials = post0_generic(lv, post0, oa);
+    return partials;
+  }
+
+  // level-0 post work without the tiled kernel (3-D, or one-block cycles)
+  template <typename T>
+  int post0_generic(std::vector<Level<T>> &lv, int post0, const OuterArgs *oa) {
+    const unsigned nb = ceil_div(lv[0].n, kBlock);
+    if constexpr (std::is_same<T, double>::value) {
+      if (post0 == 1) SCIHPC_DISPATCH(k_residual, double, nb, view(lv[0]), 0, partial, (const double *)nullptr);
+      return (int)nb;
+    } else {
+      // e = lv[0].sol; write p_new and float(r) into oa->r32
+      if (ndim == 2) {
+        if (variable)
+          k_outer_update<2, true><<<nb, kBlock, 0, stream>>>(oa->P, oa->p_old, lv[0].sol, oa->p_new, oa->r32,
+                                                              partial, 0);
+        else
+          k_outer_update<2, false><<<nb, kBlock, 0, stream>>>(oa->P, oa->p_old, lv[0].sol, oa->p_new, oa->r32,
+                                                               partial, 0);
+      } else {
+        if (variable)
+          k_outer_update<3, true><<<nb, kBlock, 0, stream>>>(oa->P, oa->p_old, lv[0].sol, oa->p_new, oa->r32,
+                                                              partial, 0);
+        else
+          k_outer_update<3, false><<<nb, kBlock, 0, stream>>>(oa->P, oa->p_old, lv[0].sol, oa->p_new, oa->r32,
+                                                               partial, 0);
+      }
+      SCIHPC_CHECK_LAUNCH();
+      return (int)nb;
     }
   }
 
@@ -652,7 +1373,7 @@ struct scihpc_mg {
   }
   void reduce_residual(const Level<double> &l, bool store, const double *flag = nullptr) {
     const unsigned nb = ceil_div(l.n, kBlock);
-    SCIHPC_DISPATCH(k_residual, double, nb, view(l), store ? 1 : 0, partial, flag);
+    SCIHPC_DISPATCH(k_residual, double, nb, view(l, false), store ? 1 : 0, partial, flag);
     finalize(nb, S_NORM);
   }
 
@@ -686,11 +1407,11 @@ struct scihpc_mg {
     SCIHPC_HIP(hipMemsetAsync(scal + S_FLAG, 0, sizeof(double), stream));
     const double *flag = scal + S_FLAG;
     for (int it = 0; it < iters; ++it) {
-      SCIHPC_DISPATCH(k_residual, double, nb, view(l), 1, partial, flag);
+      SCIHPC_DISPATCH(k_residual, double, nb, view(l, false), 1, partial, flag);
       finalize(nb, S_R2, 1);
-      SCIHPC_DISPATCH(k_apply_dot, double, nb, view(l), l.res, l.buf, partial, flag);
+      SCIHPC_DISPATCH(k_apply_dot, double, nb, view(l, false), l.res, l.buf, partial, flag);
       finalize(nb, S_RAR);
-      k_sd_update<double><<<nb, kBlock, 0, stream>>>(view(l), scal, partial);
+      k_sd_update<double><<<nb, kBlock, 0, stream>>>(view(l, false), scal, partial);
       SCIHPC_CHECK_LAUNCH();
       if (!nocompat[level]) {
         finalize(nb, S_SUM);
@@ -704,14 +1425,14 @@ struct scihpc_mg {
     relax(level, 10);
     auto &a = d[level], &b = d[level + 1];
     reduce_residual(a, true);
-    k_child_avg<<<ceil_div(b.n, kBlock), kBlock, 0, stream>>>(view(a), view(b), b.deg / a.deg, ndim);
+    k_child_avg<<<ceil_div(b.n, kBlock), kBlock, 0, stream>>>(view(a, false), view(b, false), b.deg / a.deg, ndim);
     SCIHPC_CHECK_LAUNCH();
     compat(level + 1, b.rhs);
   }
 
   void prolongation_ref(int level) {
     auto &c = d[level], &f = d[level - 1];
-    k_slope_prolong<<<ceil_div(f.n, kBlock), kBlock, 0, stream>>>(view(c), view(f), c.deg / f.deg, ndim, c.dx,
+    k_slope_prolong<<<ceil_div(f.n, kBlock), kBlock, 0, stream>>>(view(c, false), view(f, false), c.deg / f.deg, ndim, c.dx,
                                                                   c.dy, c.dz, f.dx, f.dy, f.dz);
     SCIHPC_CHECK_LAUNCH();
     relax(level - 1, 10);
@@ -719,13 +1440,13 @@ struct scihpc_mg {
 
   void coarse_solve_ref() {
     auto &c = d[L - 1];
-    k_coarse_refine<<<1, kBlock, 0, stream>>>(view(c), Minv, 0);
+    k_coarse_refine<<<1, kBlock, 0, stream>>>(view(c, false), Minv_ref, 0);
     SCIHPC_CHECK_LAUNCH();
     compat(L - 1, c.sol);
     double old_res = -1.0, new_res = 0.0;
     do {
       old_res = old_res < 0.0 ? residual_norm(L - 1, true) : new_res;
-      k_coarse_refine<<<1, kBlock, 0, stream>>>(view(c), Minv, 1);
+      k_coarse_refine<<<1, kBlock, 0, stream>>>(view(c, false), Minv_ref, 1);
       SCIHPC_CHECK_LAUNCH();
       compat(L - 1, c.sol);
       new_res = residual_norm(L - 1, true);
@@ -733,7 +1454,10 @@ struct scihpc_mg {
   }
 
   // ------------------------------------------------ coarse inverse
-  void build_coarse_inverse() {
+  // dense inverse of the coarsest operator into `dst`; rb selects the
+  // red-black solver's operator (Dirichlet coarse wall term) over the
+  // reference's
+  void build_coarse_inverse(bool rb, double *dst) {
     auto &c = d[L - 1];
     const int n = (int)c.n;
     SCIHPC_REQUIRE(n <= kMaxCoarse, "coarsest level has " + std::to_string(n) +
@@ -779,8 +1503,16 @@ struct scihpc_mg {
               if (kk > 0) { put(q - sz, az); cc += az; }
               if (kk < c.nz - 1) { put(q + sz, az); cc += az; }
             }
-            diag = bc == SCIHPC_MG_DIRICHLET ? (ndim == 2 ? -2.0 * ax - 2.0 * ay : -2.0 * ax - 2.0 * ay - 2.0 * az)
-                                             : -cc;
+            if (bc == SCIHPC_MG_DIRICHLET && rb && c.deg > 1) {
+              const DLevel<double> v = view(c, true);
+              const int walls_x = (i == 0) + (i == c.nx - 1), walls_y = (j == 0) + (j == c.ny - 1);
+              const int walls_z = ndim == 3 ? (kk == 0) + (kk == c.nz - 1) : 0;
+              diag = -(cc + walls_x * v.wx + walls_y * v.wy + walls_z * v.wz);
+            } else {
+              diag = bc == SCIHPC_MG_DIRICHLET
+                         ? (ndim == 2 ? -2.0 * ax - 2.0 * ay : -2.0 * ax - 2.0 * ay - 2.0 * az)
+                         : -cc;
+            }
           }
           put(q, diag);
         }
@@ -819,7 +1551,7 @@ struct scihpc_mg {
         }
       }
     }
-    SCIHPC_HIP(hipMemcpyAsync(Minv, Inv.data(), sizeof(double) * n * n, hipMemcpyHostToDevice, stream));
+    SCIHPC_HIP(hipMemcpyAsync(dst, Inv.data(), sizeof(double) * n * n, hipMemcpyHostToDevice, stream));
     SCIHPC_HIP(hipStreamSynchronize(stream));
   }
 
@@ -839,7 +1571,9 @@ struct scihpc_mg {
       b.dy = a.dy;
       b.dz = a.dz;
       b.sol = dalloc<float>(b.n, stream);
+      b.sol_alt = dalloc<float>(b.n, stream);
       b.rhs = dalloc<float>(b.n, stream);
+      b.rhs_alt = dalloc<float>(b.n, stream);
       b.res = dalloc<float>(b.n, stream);
       if (variable) refresh_fp32_faces(l);
     }
@@ -888,12 +1622,98 @@ struct scihpc_mg {
     SCIHPC_HIP(hipStreamSynchronize(stream));
     if (!f.empty())
       for (int l = 0; l < L; ++l) refresh_fp32_faces(l);
-    build_coarse_inverse();
+    build_coarse_inverse(true, Minv);  // Neumann: Minv_ref aliases Minv
   }
 
   // ------------------------------------------------------- solve
+  // FP64 work vectors of the flexible-CG solve
+  double *fx_ = nullptr, *fr_ = nullptr, *fp_[2] = {nullptr, nullptr}, *fq_ = nullptr, *partial2 = nullptr;
+  void ensure_fcg() {
+    if (fx_) return;
+    const long n = d[0].n;
+    fx_ = dalloc<double>(n, stream);
+    fr_ = dalloc<double>(n, stream);
+    fp_[0] = dalloc<double>(n, stream);
+    fp_[1] = dalloc<double>(n, stream);
+    fq_ = dalloc<double>(n, stream);
+    partial2 = dalloc<double>(ceil_div(n, kBlock) + 64, stream);
+  }
+
+  // Krylov-accelerated defect correction: FCG(1) with the V-cycle (FP64 or
+  // FP32) as preconditioner; x, r, p, q and every inner product in FP64.
+  int solve_fcg(const scihpc_solve_opts &o, scihpc_solve_stats &st, bool zero, double bnorm, double scale) {
+    ensure_fcg();
+    auto &l0 = d[0];
+    const long n = l0.n;
+    const unsigned nb = ceil_div(n, kBlock);
+    const bool mixed = o.precision == SCIHPC_PREC_MIXED;
+    if (mixed) ensure_fp32();
+    double *x = fx_;
+    // r = b - A x
+    if (zero) {
+      SCIHPC_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, stream));
+      k_copy_cast<<<nb, kBlock, 0, stream>>>(l0.rhs, fr_, mixed ? f[0].rhs : nullptr, n);
+      SCIHPC_CHECK_LAUNCH();
+    } else {
+      SCIHPC_HIP(hipMemcpyAsync(x, l0.sol, sizeof(double) * n, hipMemcpyDeviceToDevice, stream));
+      reduce_residual(l0, true);
+      k_copy_cast<<<nb, kBlock, 0, stream>>>(l0.res, fr_, mixed ? f[0].rhs : nullptr, n);
+      SCIHPC_CHECK_LAUNCH();
+    }
+    // FP64 preconditioner: the level-0 right-hand side *is* r during the solve
+    if (!mixed) std::swap(l0.rhs, fr_);
+    double *r = mixed ? fr_ : l0.rhs;
+    double rr = zero ? (bnorm > 0.0 ? 1.0 : 0.0) : std::sqrt(fetch(S_NORM)) / scale;
+    int it = 0, cur = 0;
+    while (rr > o.rel_tol && it < o.max_cycles) {
+      const void *z;
+      if (mixed) {
+        vcycle(f, o.nu1, o.nu2, true);
+        z = f[0].sol;
+      } else {
+        vcycle(d, o.nu1, o.nu2, true);
+        z = l0.sol;
+      }
+      if (it > 0) {
+        if (mixed) k_dot_zq<float><<<nb, kBlock, 0, stream>>>((const float *)z, fq_, n, partial);
+        else k_dot_zq<double><<<nb, kBlock, 0, stream>>>((const double *)z, fq_, n, partial);
+        SCIHPC_CHECK_LAUNCH();
+        finalize(nb, S_ZQ);
+      }
+      double *pold = fp_[cur], *pnew = fp_[cur ^ 1];
+#define SCIHPC_FCG_DIR(T, D, V)                                                                                  \
+  k_fcg_direction<T, D, V><<<nb, kBlock, 0, stream>>>(view(l0), (const T *)z, pold, pnew, fq_, r, scal, it == 0, \
+                                                       partial, partial2)
+      if (mixed) {
+        if (ndim == 2) { if (variable) SCIHPC_FCG_DIR(float, 2, true); else SCIHPC_FCG_DIR(float, 2, false); }
+        else { if (variable) SCIHPC_FCG_DIR(float, 3, true); else SCIHPC_FCG_DIR(float, 3, false); }
+      } else {
+        if (ndim == 2) { if (variable) SCIHPC_FCG_DIR(double, 2, true); else SCIHPC_FCG_DIR(double, 2, false); }
+        else { if (variable) SCIHPC_FCG_DIR(double, 3, true); else SCIHPC_FCG_DIR(double, 3, false); }
+      }
+#undef SCIHPC_FCG_DIR
+      SCIHPC_CHECK_LAUNCH();
+      finalize(nb, S_PQ);
+      k_finalize<<<1, kBlock, 0, stream>>>(partial2, nb, scal, S_PR, 0);
+      SCIHPC_CHECK_LAUNCH();
+      cur ^= 1;
+      k_fcg_update<<<nb, kBlock, 0, stream>>>(n, x, r, fp_[cur], fq_, scal, mixed ? f[0].rhs : nullptr, partial);
+      SCIHPC_CHECK_LAUNCH();
+      finalize(nb, S_NORM);
+      rr = std::sqrt(fetch(S_NORM)) / scale;
+      ++it;
+    }
+    if (!mixed) std::swap(l0.rhs, fr_);  // restore b
+    SCIHPC_HIP(hipMemcpyAsync(l0.sol, x, sizeof(double) * n, hipMemcpyDeviceToDevice, stream));
+    if (bc == SCIHPC_MG_NEUMANN) sub_mean(l0.sol, n);
+    st.cycles = it;
+    return it;
+  }
+
   int solve(const scihpc_solve_opts &o, scihpc_solve_stats &st) {
     SCIHPC_REQUIRE(o.rel_tol > 0 && o.max_cycles >= 0 && o.nu1 >= 0 && o.nu2 >= 0, "bad solve options");
+    SCIHPC_REQUIRE(o.nu1 <= 2 && o.nu2 <= 2, "nu1, nu2 <= 2 (tiled kernels stage a 5-cell halo)");
+    SCIHPC_REQUIRE(o.accel == 0 || o.accel == 1, "accel must be 0 (stationary) or 1 (flexible CG)");
     auto &l0 = d[0];
     const unsigned nb = ceil_div(l0.n, kBlock);
     SCIHPC_HIP(hipEventRecord(ev0, stream));
@@ -902,30 +1722,33 @@ struct scihpc_mg {
     st = scihpc_solve_stats{};
     int cycles = 0;
     double rr;
-    if (o.precision == SCIHPC_PREC_FP64) {
-      if (o.zero_guess) SCIHPC_HIP(hipMemsetAsync(l0.sol, 0, sizeof(double) * l0.n, stream));
-      if (o.use_learned && learned.active()) learned_initial_guess(st);
-      rr = residual_norm(0, false) / scale;
+    bool zero = o.zero_guess != 0;
+    if (o.use_learned && learned.active()) {
+      if (zero) SCIHPC_HIP(hipMemsetAsync(l0.sol, 0, sizeof(double) * l0.n, stream));
+      learned_initial_guess(st);
+      zero = false;
+    }
+    if (o.accel == 1) {
+      solve_fcg(o, st, zero, bnorm, scale);
+      cycles = st.cycles;
+      rr = std::sqrt(host_scal[S_NORM]) / scale;
+      if (cycles == 0) rr = zero ? (bnorm > 0.0 ? 1.0 : 0.0) : rr;
+    } else if (o.precision == SCIHPC_PREC_FP64) {
+      rr = zero ? (bnorm > 0.0 ? 1.0 : 0.0) : residual_norm(0, false) / scale;
       while (rr > o.rel_tol && cycles < o.max_cycles) {
-        vcycle(d, o.nu1, o.nu2, false);
-        rr = residual_norm(0, false) / scale;
+        const int np = vcycle(d, o.nu1, o.nu2, zero && cycles == 0, 1);
+        finalize(np, S_NORM);
+        rr = std::sqrt(fetch(S_NORM)) / scale;
         ++cycles;
       }
+      if (zero && cycles == 0) SCIHPC_HIP(hipMemsetAsync(l0.sol, 0, sizeof(double) * l0.n, stream));
       if (bc == SCIHPC_MG_NEUMANN) sub_mean(l0.sol, l0.n);
     } else {
       ensure_fp32();
       auto &f0 = f[0];
-      // p lives in l0.sol / p_alt (ping-pong); e in f0.sol; r32 in f0.rhs
+      // p: l0.sol / p_alt (ping-pong); FP32 rhs0 = float(b - A p)
       double *p = l0.sol, *q = p_alt;
-      int first = o.zero_guess ? 1 : 0;
-      if (!first) SCIHPC_HIP(hipMemsetAsync(f0.sol, 0, sizeof(float) * f0.n, stream));
-      if (o.use_learned && learned.active()) {
-        if (first) SCIHPC_HIP(hipMemsetAsync(l0.sol, 0, sizeof(double) * l0.n, stream));
-        learned_initial_guess(st);
-        SCIHPC_HIP(hipMemsetAsync(f0.sol, 0, sizeof(float) * f0.n, stream));
-        first = 0;
-      }
-      auto outer = [&](int is_first) {
+      auto first = [&](int is_first) {
         if (ndim == 2) {
           if (variable)
             k_outer_update<2, true><<<nb, kBlock, 0, stream>>>(view(l0), p, f0.sol, q, f0.rhs, partial, is_first);
@@ -942,13 +1765,22 @@ struct scihpc_mg {
         std::swap(p, q);
         return std::sqrt(fetch(S_NORM)) / scale;
       };
-      rr = outer(first);
+      // initial residual into the FP32 right-hand side (e = 0)
+      if (!zero) SCIHPC_HIP(hipMemsetAsync(f0.sol, 0, sizeof(float) * f0.n, stream));
+      rr = first(zero ? 1 : 0);
       while (rr > o.rel_tol && cycles < o.max_cycles) {
-        vcycle(f, o.nu1, o.nu2, true);
-        rr = outer(0);
+        OuterArgs oa{view(l0), p, q, f0.rhs_alt};
+        const int np = vcycle(f, o.nu1, o.nu2, true, 2, &oa);
+        std::swap(f0.rhs, f0.rhs_alt);
+        std::swap(p, q);
+        finalize(np, S_NORM);
+        rr = std::sqrt(fetch(S_NORM)) / scale;
         ++cycles;
       }
-      if (p != l0.sol) SCIHPC_HIP(hipMemcpyAsync(l0.sol, p, sizeof(double) * l0.n, hipMemcpyDeviceToDevice, stream));
+      if (p != l0.sol) {
+        // keep the result in l0.sol; p_alt becomes the spare buffer
+        std::swap(l0.sol, p_alt);
+      }
       if (bc == SCIHPC_MG_NEUMANN) sub_mean(l0.sol, l0.n);
     }
     SCIHPC_HIP(hipEventRecord(ev1, stream));
@@ -1026,6 +1858,8 @@ extern "C" int scihpc_mg_create(int ndim, int nx, int ny, int nz, double dx, dou
     try {
       h->ndim = ndim;
       h->bc = bc;
+      if (const char *e = std::getenv("SCIHPC_MG_BLOCK_MAX")) h->block_max = std::atol(e);
+      if (const char *e = std::getenv("SCIHPC_MG_TILED")) h->tiled = std::atoi(e) != 0;
       std::vector<int> lx, ly, lz, ld;
       h->L = g_levels(ndim, nx, ny, nz, lx, ly, lz, ld);
       SCIHPC_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
@@ -1044,6 +1878,7 @@ extern "C" int scihpc_mg_create(int ndim, int nx, int ny, int nz, double dx, dou
         v.dy = dy * ld[l];
         v.dz = ndim == 3 ? dz * ld[l] : 1.0;
         v.sol = dalloc<double>(v.n, h->stream);
+        v.sol_alt = dalloc<double>(v.n, h->stream);
         v.rhs = dalloc<double>(v.n, h->stream);
         v.res = dalloc<double>(v.n, h->stream);
         v.buf = dalloc<double>(v.n, h->stream);
@@ -1054,7 +1889,23 @@ extern "C" int scihpc_mg_create(int ndim, int nx, int ny, int nz, double dx, dou
       h->scal = dalloc<double>(S_COUNT, h->stream);
       h->p_alt = dalloc<double>(h->d[0].n, h->stream);
       SCIHPC_HIP(hipHostMalloc((void **)&h->host_scal, sizeof(double) * S_COUNT, hipHostMallocDefault));
-      h->build_coarse_inverse();
+      {
+        int dev = 0, lds = 0;
+        SCIHPC_HIP(hipGetDevice(&dev));
+        if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && lds > 0)
+          h->lds_max = lds;
+        int cu = 0;
+        if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cu > 0)
+          h->num_cu = cu;
+      }
+      h->build_coarse_inverse(true, h->Minv);
+      if (bc == SCIHPC_MG_DIRICHLET) {
+        const long nc = std::min<long>(h->d.back().n, kMaxCoarse);
+        h->Minv_ref = dalloc<double>(nc * nc, h->stream);
+        h->build_coarse_inverse(false, h->Minv_ref);
+      } else {
+        h->Minv_ref = h->Minv;
+      }
     } catch (...) {
       scihpc_mg_destroy(h);
       throw;
@@ -1071,6 +1922,7 @@ extern "C" int scihpc_mg_destroy(scihpc_mg *h) {
   };
   for (auto &v : h->d) {
     fr(v.sol);
+    fr(v.sol_alt);
     fr(v.rhs);
     fr(v.res);
     fr(v.buf);
@@ -1080,14 +1932,23 @@ extern "C" int scihpc_mg_destroy(scihpc_mg *h) {
   }
   for (auto &v : h->f) {
     fr(v.sol);
+    fr(v.sol_alt);
+    fr(v.rhs_alt);
     fr(v.rhs);
     fr(v.res);
     fr(v.cx);
     fr(v.cy);
     fr(v.cz);
   }
+  if (h->Minv_ref != h->Minv) fr(h->Minv_ref);
   fr(h->Minv);
   fr(h->partial);
+  fr(h->fx_);
+  fr(h->fr_);
+  fr(h->fp_[0]);
+  fr(h->fp_[1]);
+  fr(h->fq_);
+  fr(h->partial2);
   fr(h->scal);
   fr(h->p_alt);
   if (h->host_scal) (void)hipHostFree(h->host_scal);
@@ -1236,11 +2097,11 @@ extern "C" int scihpc_mg_apply(scihpc_mg *h, int level, const double *x, double 
     const bool variable = hh.variable;
     hipStream_t stream = hh.stream;
     if (ndim == 2) {
-      if (variable) k_apply_dot<double, 2, true><<<nb, kBlock, 0, stream>>>(hh.view(l), l.res, l.buf, nullptr, nullptr);
-      else k_apply_dot<double, 2, false><<<nb, kBlock, 0, stream>>>(hh.view(l), l.res, l.buf, nullptr, nullptr);
+      if (variable) k_apply_dot<double, 2, true><<<nb, kBlock, 0, stream>>>(hh.view(l, false), l.res, l.buf, nullptr, nullptr);
+      else k_apply_dot<double, 2, false><<<nb, kBlock, 0, stream>>>(hh.view(l, false), l.res, l.buf, nullptr, nullptr);
     } else {
-      if (variable) k_apply_dot<double, 3, true><<<nb, kBlock, 0, stream>>>(hh.view(l), l.res, l.buf, nullptr, nullptr);
-      else k_apply_dot<double, 3, false><<<nb, kBlock, 0, stream>>>(hh.view(l), l.res, l.buf, nullptr, nullptr);
+      if (variable) k_apply_dot<double, 3, true><<<nb, kBlock, 0, stream>>>(hh.view(l, false), l.res, l.buf, nullptr, nullptr);
+      else k_apply_dot<double, 3, false><<<nb, kBlock, 0, stream>>>(hh.view(l, false), l.res, l.buf, nullptr, nullptr);
     }
     SCIHPC_CHECK_LAUNCH();
     SCIHPC_HIP(hipMemcpyAsync(y, l.buf, sizeof(double) * l.n, hipMemcpyDeviceToHost, stream));
@@ -1344,57 +2205,51 @@ extern "C" int scihpc_mg_time_kernel(scihpc_mg *h, int which, int iters, double 
   return guarded([&] {
     check_level(h, 0);
     SCIHPC_REQUIRE(iters > 0 && which >= 0 && which <= 3, "bad timing request");
+    SCIHPC_REQUIRE(h->L > 1, "timing needs at least two levels");
+    h->ensure_fp32();
     auto &l0 = h->d[0];
-    const double n = (double)l0.n;
-    const int c = h->variable ? 1 : 0;
-    const int cf = h->variable ? h->ndim : 0;  // face arrays read per launch
-    if (which == 1) h->ensure_fp32();
+    auto &f0 = h->f[0];
+    const double n = (double)l0.n, nc = (double)h->d[1].n;
+    const double nf = h->variable ? h->ndim : 0;  // face arrays per level
+    const bool tiled = h->fused_level(h->d, 0);
+    OuterArgs oa{h->view(l0), l0.sol, h->p_alt, f0.rhs_alt};
     auto launch = [&] {
       switch (which) {
-        case 0:
-          h->gs_color(l0, 0, 0);
-          h->gs_color(l0, 1, 0);
-          break;
-        case 1:
-          h->gs_color(h->f[0], 0, 0);
-          h->gs_color(h->f[0], 1, 0);
-          break;
-        case 2: {
-          h->ensure_fp32();
-          const unsigned nb = ceil_div(l0.n, kBlock);
-          auto &f0 = h->f[0];
-          if (h->ndim == 2) {
-            if (h->variable)
-              k_outer_update<2, true><<<nb, kBlock, 0, h->stream>>>(h->view(l0), l0.sol, f0.sol, h->p_alt, f0.rhs,
-                                                                   h->partial, 0);
-            else
-              k_outer_update<2, false><<<nb, kBlock, 0, h->stream>>>(h->view(l0), l0.sol, f0.sol, h->p_alt, f0.rhs,
-                                                                    h->partial, 0);
-          } else {
-            if (h->variable)
-              k_outer_update<3, true><<<nb, kBlock, 0, h->stream>>>(h->view(l0), l0.sol, f0.sol, h->p_alt, f0.rhs,
-                                                                   h->partial, 0);
-            else
-              k_outer_update<3, false><<<nb, kBlock, 0, h->stream>>>(h->view(l0), l0.sol, f0.sol, h->p_alt, f0.rhs,
-                                                                    h->partial, 0);
+        case 0:  // FP64 level-0 pre-smoothing kernel
+          if (tiled) h->pre2d(h->d, 0, 2, 0);
+          else {
+            h->sweep(l0, 0, 0);
+            h->sweep(l0, 0, 0);
+            h->residual_restrict(l0, h->d[1]);
           }
-          SCIHPC_CHECK_LAUNCH();
           break;
-        }
-        case 3:
-          if (h->L > 1) h->residual_restrict(l0, h->d[1]);
+        case 1:  // FP32 level-0 pre-smoothing kernel
+          if (tiled) h->pre2d(h->f, 0, 2, 0);
+          else {
+            h->sweep(f0, 0, 0);
+            h->sweep(f0, 0, 0);
+            h->residual_restrict(f0, h->f[1]);
+          }
+          break;
+        case 2:  // FP32 level-0 post-smoothing kernel fused with the FP64 outer step
+          if (tiled) h->post2d_mode<float, 2>(h->f, 0, 2, oa);
+          else h->post0_generic(h->f, 2, &oa);
+          break;
+        case 3:  // FP64 level-0 post-smoothing kernel with the residual norm
+          if (tiled) h->post2d_mode<double, 1>(h->d, 0, 2, OuterArgs{});
+          else h->post0_generic(h->d, 1, nullptr);
           break;
       }
     };
-    // algorithmic bytes per launch (each array read once, written once)
+    // algorithmic bytes: every array the launch touches, once per read and
+    // once per write, at its stored precision
     double b = 0;
     switch (which) {
-      case 0: b = (3.0 + cf) * 8.0 * n; break;                 // sol r+w, rhs, faces
-      case 1: b = (3.0 + cf) * 4.0 * n; break;
-      case 2: b = (8.0 + 4.0 + 8.0 + 8.0 * cf + 8.0 + 4.0) * n; break;  // p,e,b,faces -> p',r32
-      case 3: b = (2.0 + cf) * 8.0 * n + 8.0 * n / (h->ndim == 3 ? 8.0 : 4.0) * 2.0; break;
+      case 0: b = (3.0 + nf) * 8.0 * n + 2.0 * 8.0 * nc; break;
+      case 1: b = (3.0 + nf) * 4.0 * n + 2.0 * 4.0 * nc; break;
+      case 2: b = (2.0 + nf) * 4.0 * n + 4.0 * nc + (3.0 + nf) * 8.0 * n + 8.0 * n + 4.0 * n; break;
+      case 3: b = (3.0 + nf) * 8.0 * n + 8.0 * nc; break;
     }
-    (void)c;
     launch();  // warm-up
     SCIHPC_HIP(hipEventRecord(h->ev0, h->stream));
     for (int it = 0; it < iters; ++it) launch();
