@@ -52,7 +52,7 @@ struct DLevel {
 };
 
 // scalars kept on the device by the reference-faithful path
-enum { S_R2 = 0, S_RAR = 1, S_SUM = 2, S_FLAG = 3, S_NORM = 4, S_B2 = 5, S_COUNT = 16 };
+enum { S_R2 = 0, S_RAR = 1, S_SUM = 2, S_FLAG = 3, S_NORM = 4, S_B2 = 5, S_ZQ = 8, S_PQ = 9, S_PR = 10, S_COUNT = 16 };
 
 // -------------------------------------------------------- stencil helpers
 // off = sum over present neighbours of coef*x, diag = A(c,c).  Loads of
@@ -128,6 +128,15 @@ __device__ __forceinline__ void cell_ijk(long c, int nx, int ny, int &i, int &j,
   k = (int)(r / uy);
 }
 
+// x / d: IEEE division in FP64 (parity with the oracle); in the FP32
+// V-cycle, which only preconditions the FP64 iteration, the reciprocal
+// instruction and one multiply
+template <typename T>
+__device__ __forceinline__ T gs_div(T a, T d) {
+  if constexpr (sizeof(T) == 4) return __fdividef(a, d);
+  else return a / d;
+}
+
 // ---------------------------------------------------- red-black smoother
 // Gauss-Seidel update of cell (i,j,k): x_c = (b_c - off) / diag.
 // zero_guess: neighbours read as 0 (first half-sweep from a zero initial
@@ -138,7 +147,7 @@ __device__ __forceinline__ void gs_cell(const DLevel<T> &L, int i, int j, int k,
   T off, diag;
   stencil<T, DIM, VAR>(L, L.sol, i, j, k, c, off, diag);
   if (zero_guess) off = T(0);
-  L.sol[c] = (L.rhs[c] - off) / diag;
+  L.sol[c] = gs_div<T>(L.rhs[c] - off, diag);
 }
 
 // t-th cell of colour `color` ((i+j+k)%2 == color); false when out of range
@@ -311,51 +320,55 @@ struct BlockLevels {
   int arrays;                     // arrays per level in LDS (2 or 2 + ndim)
 };
 
+// LDS view of block level l: global metadata, LDS arrays.  Built from the
+// extern __shared__ base on every use so the compiler keeps LDS (not flat)
+// addressing.
+template <typename T, bool VAR, int DIM>
+__device__ __forceinline__ DLevel<T> lds_view(const BlockLevels<T> *__restrict__ B, int l, T *smem) {
+  DLevel<T> v = B->lv[l];
+  T *base = smem + B->off[l];
+  const long n = v.n;
+  v.sol = base;
+  v.rhs = base + n;
+  if constexpr (VAR) {
+    v.cx = base + 2 * n;
+    v.cy = base + 3 * n;
+    if constexpr (DIM == 3) v.cz = base + 4 * n;
+  }
+  return v;
+}
+
 template <typename T, int DIM, bool VAR>
 __global__ void __launch_bounds__(kBlockThreads)
-    k_block_vcycle(BlockLevels<T> B, int nu1, int nu2, const double *__restrict__ Minv, int neumann, int dir,
-                   int zero_guess0, int load_top) {
+    k_block_vcycle(const BlockLevels<T> *__restrict__ B, int nu1, int nu2, const double *__restrict__ Minv,
+                   int neumann, int dir, int zero_guess0, int load_top) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T *smem = reinterpret_cast<T *>(smem_raw);
-  const int L = B.nlev;
-  // LDS views of the same levels (kept in LDS: a per-thread array indexed by
-  // the level would live in scratch memory)
-  __shared__ DLevel<T> S[kMaxBlockLevels];
-  if (threadIdx.x == 0) {
-    for (int l = 0; l < L; ++l) {
-      S[l] = B.lv[l];
-      T *base = smem + B.off[l];
-      const long n = B.lv[l].n;
-      S[l].sol = base;
-      S[l].rhs = base + n;
-      if constexpr (VAR) {
-        S[l].cx = base + 2 * n;
-        S[l].cy = base + 3 * n;
-        if constexpr (DIM == 3) S[l].cz = base + 4 * n;
-      }
-    }
-  }
-  __syncthreads();
+  const int L = B->nlev;
   for (int l = 0; l < L; ++l) {
-    const long n = B.lv[l].n;
-    for (long c = threadIdx.x; c < n; c += blockDim.x) {
-      S[l].sol[c] = (l == 0 && load_top && !zero_guess0) ? B.lv[0].sol[c] : T(0);
-      if (l == 0) S[l].rhs[c] = B.lv[0].rhs[c];
+    const DLevel<T> g = B->lv[l];
+    const DLevel<T> S = lds_view<T, VAR, DIM>(B, l, smem);
+    for (long c = threadIdx.x; c < g.n; c += blockDim.x) {
+      S.sol[c] = (l == 0 && load_top && !zero_guess0) ? g.sol[c] : T(0);
+      if (l == 0) S.rhs[c] = g.rhs[c];
       if constexpr (VAR) {
-        const_cast<T *>(S[l].cx)[c] = B.lv[l].cx[c];
-        const_cast<T *>(S[l].cy)[c] = B.lv[l].cy[c];
-        if constexpr (DIM == 3) const_cast<T *>(S[l].cz)[c] = B.lv[l].cz[c];
+        const_cast<T *>(S.cx)[c] = g.cx[c];
+        const_cast<T *>(S.cy)[c] = g.cy[c];
+        if constexpr (DIM == 3) const_cast<T *>(S.cz)[c] = g.cz[c];
       }
     }
   }
   __syncthreads();
   for (int l = 0; l < L - 1; ++l) {
-    for (int s = 0; s < nu1; ++s) block_sweep<T, DIM, VAR>(S[l], 0, (l == 0 && s == 0) ? zero_guess0 : 0);
-    for (long C = threadIdx.x; C < S[l + 1].n; C += blockDim.x) rr_cell<T, DIM, VAR>(S[l], S[l + 1], B.ratio[l], C);
+    const DLevel<T> F = lds_view<T, VAR, DIM>(B, l, smem);
+    const DLevel<T> G = lds_view<T, VAR, DIM>(B, l + 1, smem);
+    for (int s = 0; s < nu1; ++s) block_sweep<T, DIM, VAR>(F, 0, (l == 0 && s == 0) ? zero_guess0 : 0);
+    const int r = B->ratio[l];
+    for (long C = threadIdx.x; C < G.n; C += blockDim.x) rr_cell<T, DIM, VAR>(F, G, r, C);
     __syncthreads();
   }
   {  // dense coarsest solve, FP64, rows in registers
-    const DLevel<T> &G = S[L - 1];
+    const DLevel<T> G = lds_view<T, VAR, DIM>(B, L - 1, smem);
     const int n = (int)G.n;
     double xr[2] = {0.0, 0.0};
     double local = 0.0;
@@ -380,12 +393,16 @@ __global__ void __launch_bounds__(kBlockThreads)
     __syncthreads();
   }
   for (int l = L - 1; l > 0; --l) {
-    for (long c = threadIdx.x; c < S[l - 1].n; c += blockDim.x)
-      prolong_cell<T, DIM>(S[l], S[l - 1], B.ratio[l - 1], dir, c);
+    const DLevel<T> G = lds_view<T, VAR, DIM>(B, l, smem);
+    const DLevel<T> F = lds_view<T, VAR, DIM>(B, l - 1, smem);
+    const int r = B->ratio[l - 1];
+    for (long c = threadIdx.x; c < F.n; c += blockDim.x) prolong_cell<T, DIM>(G, F, r, dir, c);
     __syncthreads();
-    for (int s = 0; s < nu2; ++s) block_sweep<T, DIM, VAR>(S[l - 1], 1, 0);
+    for (int s = 0; s < nu2; ++s) block_sweep<T, DIM, VAR>(F, 1, 0);
   }
-  for (long c = threadIdx.x; c < B.lv[0].n; c += blockDim.x) B.lv[0].sol[c] = S[0].sol[c];
+  const DLevel<T> top = B->lv[0];
+  const DLevel<T> S0 = lds_view<T, VAR, DIM>(B, 0, smem);
+  for (long c = threadIdx.x; c < top.n; c += blockDim.x) top.sol[c] = S0.sol[c];
 }
 
 // ---------------------------------------- fused 2-D fine-level kernels
@@ -442,16 +459,22 @@ __device__ __forceinline__ void fused_stage(const DLevel<T> &F, int x0, int y0, 
   }
 }
 
-// one colour on the tile grown by `ring`
-template <typename T, bool VAR, int TX, int TY>
-__device__ __forceinline__ void fused_pass(const DLevel<T> &F, int x0, int y0, int color, int ring,
+// one colour on the tile grown by RING: only cells of that colour are
+// enumerated (no idle lanes)
+template <typename T, bool VAR, int TX, int TY, int RING>
+__device__ __forceinline__ void fused_pass(const DLevel<T> &F, int x0, int y0, int color,
                                            T (*xs)[TX + 2 * kFH], T (*bs)[TX + 2 * kFH],
                                            T (*cxs)[TX + 2 * kFH], T (*cys)[TX + 2 * kFH]) {
-  const int w = TX + 2 * ring, h = TY + 2 * ring;
-  for (int q = threadIdx.x; q < w * h; q += blockDim.x) {
-    const int ly = q / w + kFH - ring, lx = q % w + kFH - ring;
-    const int gx = x0 - kFH + lx, gy = y0 - kFH + ly;
-    if (gx < 0 || gx >= F.nx || gy < 0 || gy >= F.ny || ((gx + gy) & 1) != color) continue;
+  constexpr int w = TX + 2 * RING, h = TY + 2 * RING, hw = (w + 1) / 2;
+  for (int q = threadIdx.x; q < h * hw; q += blockDim.x) {
+    const int ry = q / hw;
+    const int gy = y0 - RING + ry;
+    // first column of this row with (gx + gy) % 2 == color
+    const int rx = 2 * (q - ry * hw) + ((color - (x0 - RING) - gy) & 1);
+    if (rx >= w) continue;
+    const int gx = x0 - RING + rx;
+    if (gx < 0 || gx >= F.nx || gy < 0 || gy >= F.ny) continue;
+    const int lx = rx + kFH - RING, ly = ry + kFH - RING;
     T off, diag;
     T cL = T(0), cR = T(0), cD = T(0), cU = T(0);
     if constexpr (VAR) {
@@ -462,7 +485,27 @@ __device__ __forceinline__ void fused_pass(const DLevel<T> &F, int x0, int y0, i
     }
     stencil2<T, VAR>(F, gx, gy, xs[ly][lx - 1], xs[ly][lx + 1], xs[ly - 1][lx], xs[ly + 1][lx], cL, cR, cD, cU, off,
                      diag);
-    xs[ly][lx] = (bs[ly][lx] - off) / diag;
+    xs[ly][lx] = gs_div<T>(bs[ly][lx] - off, diag);
+  }
+}
+
+// nu red-black sweeps (colour `first` first) on shrinking rings: the
+// first colour of sweep s on ring 2 nu - 2 s, the second on 2 nu - 1 - 2 s
+template <typename T, bool VAR, int TX, int TY>
+__device__ __forceinline__ void fused_sweeps(const DLevel<T> &F, int x0, int y0, int nu, int first,
+                                             T (*xs)[TX + 2 * kFH], T (*bs)[TX + 2 * kFH],
+                                             T (*cxs)[TX + 2 * kFH], T (*cys)[TX + 2 * kFH]) {
+  if (nu == 2) {
+    fused_pass<T, VAR, TX, TY, 4>(F, x0, y0, first, xs, bs, cxs, cys);
+    __syncthreads();
+    fused_pass<T, VAR, TX, TY, 3>(F, x0, y0, 1 - first, xs, bs, cxs, cys);
+    __syncthreads();
+  }
+  if (nu >= 1) {
+    fused_pass<T, VAR, TX, TY, 2>(F, x0, y0, first, xs, bs, cxs, cys);
+    __syncthreads();
+    fused_pass<T, VAR, TX, TY, 1>(F, x0, y0, 1 - first, xs, bs, cxs, cys);
+    __syncthreads();
   }
 }
 
@@ -498,12 +541,7 @@ __global__ void __launch_bounds__(kFusedThreads)
   const int x0 = (tile % tiles_x) * TX, y0 = (tile / tiles_x) * TY;
   fused_stage<T, VAR, TX, TY>(F, x0, y0, zero_guess != 0, xs, bs, cxs, cys);
   __syncthreads();
-  for (int s = 0; s < nu1; ++s) {
-    fused_pass<T, VAR, TX, TY>(F, x0, y0, 0, 2 * nu1 - 2 * s, xs, bs, cxs, cys);
-    __syncthreads();
-    fused_pass<T, VAR, TX, TY>(F, x0, y0, 1, 2 * nu1 - 1 - 2 * s, xs, bs, cxs, cys);
-    __syncthreads();
-  }
+  fused_sweeps<T, VAR, TX, TY>(F, x0, y0, nu1, 0, xs, bs, cxs, cys);
   // residual + child average (ratio 2, tiles are even-aligned)
   for (int q = threadIdx.x; q < (TX / 2) * (TY / 2); q += blockDim.x) {
     const int I = x0 / 2 + q % (TX / 2), J = y0 / 2 + q / (TX / 2);
@@ -524,11 +562,12 @@ __global__ void __launch_bounds__(kFusedThreads)
   }
 }
 
-struct OuterArgs {  // FP64 level 0 of the mixed solve (MODE 2)
-  DLevel<double> P;         // rhs = b, faces / constants of the FP64 operator
-  const double *p_old;      // current FP64 iterate
-  double *p_new;            // updated FP64 iterate
-  float *r32;               // float(b - A p_new) -> next FP32 right-hand side
+struct OuterArgs {  // level-0 extras of the post-smoothing kernel
+  DLevel<double> P;         // MODE 2: rhs = b, faces / constants of the FP64 operator
+  const double *p_old;      // MODE 2: current FP64 iterate
+  double *p_new;            // MODE 2: updated FP64 iterate
+  float *r32;               // MODE 2: float(b - A p_new) -> next FP32 right-hand side
+  const double *q;          // MODE 3: previous search direction image A p
 };
 
 template <typename T, bool VAR, int TX, int TY, int MODE>
@@ -544,10 +583,7 @@ __global__ void __launch_bounds__(kFusedThreads)
   __shared__ T cs[CH][CW];
   const int tiles_x = (F.nx + TX - 1) / TX, ntiles = tiles_x * ((F.ny + TY - 1) / TY);
   const int tile = xcd_tile(ntiles);
-  if (tile >= ntiles) {
-    if (MODE != 0 && threadIdx.x == 0) partial[blockIdx.x] = 0.0;
-    return;
-  }
+  if (tile >= ntiles) return;  // never taken: the grid has exactly ntiles blocks
   const int x0 = (tile % tiles_x) * TX, y0 = (tile / tiles_x) * TY;
   const int cx0 = x0 / 2 - 4, cy0 = y0 / 2 - 4;
   fused_stage<T, VAR, TX, TY>(F, x0, y0, false, xs, bs, cxs, cys);
@@ -581,12 +617,7 @@ __global__ void __launch_bounds__(kFusedThreads)
     xs[ly][lx] += (T(1) - ty) * v00 + ty * v10;
   }
   __syncthreads();
-  for (int s = 0; s < nu2; ++s) {
-    fused_pass<T, VAR, TX, TY>(F, x0, y0, 1, 2 * nu2 - 2 * s, xs, bs, cxs, cys);
-    __syncthreads();
-    fused_pass<T, VAR, TX, TY>(F, x0, y0, 0, 2 * nu2 - 1 - 2 * s, xs, bs, cxs, cys);
-    __syncthreads();
-  }
+  fused_sweeps<T, VAR, TX, TY>(F, x0, y0, nu2, 1, xs, bs, cxs, cys);
   double local = 0.0;
   for (int q = threadIdx.x; q < TX * TY; q += blockDim.x) {
     const int gx = x0 + q % TX, gy = y0 + q / TX;
@@ -595,6 +626,9 @@ __global__ void __launch_bounds__(kFusedThreads)
     const long g = gx + (long)F.nx * gy;
     if constexpr (MODE == 0) {
       out[g] = xs[ly][lx];
+    } else if constexpr (MODE == 3) {
+      out[g] = xs[ly][lx];
+      local += double(xs[ly][lx]) * oa.q[g];
     } else if constexpr (MODE == 1) {
       out[g] = xs[ly][lx];
       const double r = double(fused_res<T, VAR, TX>(F, gx, gy, lx, ly, xs, bs, cxs, cys));
@@ -686,6 +720,21 @@ __global__ void k_finalize(const double *__restrict__ partial, int nb, double *_
       if (sqrt(s) < 1e-16) scal[S_FLAG] = 1.0;
     }
     scal[slot] = s;
+  }
+}
+
+// scal[s0] = sum partial[0..nb), scal[s1] = sum partial[nb..2nb)
+__global__ void k_finalize2(const double *__restrict__ partial, int nb, double *__restrict__ scal, int s0, int s1) {
+  double a = 0.0, b = 0.0;
+  for (int q = threadIdx.x; q < nb; q += blockDim.x) {
+    a += partial[q];
+    b += partial[nb + q];
+  }
+  a = block_sum(a);
+  b = block_sum(b);
+  if (threadIdx.x == 0) {
+    scal[s0] = a;
+    scal[s1] = b;
   }
 }
 
@@ -867,7 +916,6 @@ __global__ void __launch_bounds__(kBlock) k_outer_update(DLevel<double> L, const
 // x, r, p, q = A p and inner products stay FP64.  The preconditioner may be
 // non-symmetric and inexact, hence the flexible direction update
 // beta = -(z . q_old) / (p_old . q_old).  All scalars stay on the device.
-enum { S_ZQ = 8, S_PQ = 9, S_PR = 10, S_ALPHA_OK = 11 };
 
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_dot_zq(const T *__restrict__ z, const double *__restrict__ q, long n,
@@ -884,8 +932,7 @@ template <typename T, int DIM, bool VAR>
 __global__ void __launch_bounds__(kBlock)
     k_fcg_direction(DLevel<double> L, const T *__restrict__ z, const double *__restrict__ p_old,
                     double *__restrict__ p, double *__restrict__ q, const double *__restrict__ r,
-                    const double *__restrict__ scal, int first, double *__restrict__ part_pq,
-                    double *__restrict__ part_pr) {
+                    const double *__restrict__ scal, int first, double *__restrict__ partial) {
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const double beta = first ? 0.0 : -scal[S_ZQ] / scal[S_PQ];
   double vpq = 0.0, vpr = 0.0;
@@ -931,8 +978,8 @@ __global__ void __launch_bounds__(kBlock)
   vpq = block_sum(vpq);
   vpr = block_sum(vpr);
   if (threadIdx.x == 0) {
-    part_pq[blockIdx.x] = vpq;
-    part_pr[blockIdx.x] = vpr;
+    partial[blockIdx.x] = vpq;
+    partial[gridDim.x + blockIdx.x] = vpr;
   }
 }
 
@@ -1083,6 +1130,9 @@ struct scihpc_mg {
   long lds_max = 65536;       // LDS bytes a workgroup may use (queried at create)
   int num_cu = 256;           // compute units (queried at create)
   bool tiled = true;          // LDS-tiled 2-D fine-level kernels (SCIHPC_MG_TILED=0 disables)
+  void *blk_dev[2] = {nullptr, nullptr};   // device copies of BlockLevels<double>/<float>
+  std::vector<char> blk_host[2];           // last uploaded contents
+  double *host_part = nullptr;             // pinned copy of the partial sums
 
   // ------------------------------------------------------------ views
   // rb: view of the red-black solver's operator (Dirichlet coarse levels
@@ -1242,7 +1292,8 @@ struct scihpc_mg {
 
   template <typename T>
   void block_cycle(std::vector<Level<T>> &lv, int lb, int nu1, int nu2, int zero_guess0, int load_top) {
-    BlockLevels<T> B{};
+    BlockLevels<T> B;
+    std::memset(&B, 0, sizeof(B));
     B.nlev = L - lb;
     B.arrays = 2 + (variable ? ndim : 0);
     long off = 0;
@@ -1254,12 +1305,22 @@ struct scihpc_mg {
     }
     const size_t bytes = sizeof(T) * off;
     const int neu = bc == SCIHPC_MG_NEUMANN, dir = bc == SCIHPC_MG_DIRICHLET;
+    // the metadata lives in device memory (a by-value kernel argument indexed
+    // by level would be copied to scratch); re-uploaded only when it changes
+    const int slot = std::is_same<T, double>::value ? 0 : 1;
+    if (!blk_dev[slot]) SCIHPC_HIP(hipMalloc(&blk_dev[slot], sizeof(BlockLevels<double>)));
+    if (blk_host[slot].size() != sizeof(B) || std::memcmp(blk_host[slot].data(), &B, sizeof(B)) != 0) {
+      blk_host[slot].assign(reinterpret_cast<const char *>(&B), reinterpret_cast<const char *>(&B) + sizeof(B));
+      SCIHPC_HIP(hipMemcpyAsync(blk_dev[slot], blk_host[slot].data(), sizeof(B), hipMemcpyHostToDevice, stream));
+      SCIHPC_HIP(hipStreamSynchronize(stream));
+    }
+    const BlockLevels<T> *Bd = static_cast<const BlockLevels<T> *>(blk_dev[slot]);
 #define SCIHPC_BLOCK(D, V)                                                                                  \
   do {                                                                                                      \
     if (bytes > 65536)                                                                                      \
       SCIHPC_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_block_vcycle<T, D, V>),             \
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));              \
-    k_block_vcycle<T, D, V><<<1, kBlockThreads, bytes, stream>>>(B, nu1, nu2, Minv, neu, dir, zero_guess0, \
+    k_block_vcycle<T, D, V><<<1, kBlockThreads, bytes, stream>>>(Bd, nu1, nu2, Minv, neu, dir, zero_guess0, \
                                                                   load_top);                                \
   } while (0)
     if (ndim == 2) {
@@ -1280,14 +1341,15 @@ struct scihpc_mg {
   // Returns the number of residual partials written (0 when post0 == 0).
   template <typename T>
   int vcycle(std::vector<Level<T>> &lv, int nu1, int nu2, bool zero_guess0, int post0 = 0,
-             const OuterArgs *oa = nullptr) {
+             const OuterArgs &oa = OuterArgs{}) {
     const int lb = block_level(lv);
     if (nu1 == 0 && zero_guess0) SCIHPC_HIP(hipMemsetAsync(lv[0].sol, 0, sizeof(T) * lv[0].n, stream));
     const bool zg = zero_guess0 && nu1 > 0;
+    const bool can_tile = nu1 <= 2 && nu2 <= 2;  // the tiles stage a 5-cell halo
     const int top = std::min(lb, L - 1);  // deepest level reached by the fine loop
     for (int l = 0; l < top; ++l) {
       const int z = (l == 0 && zg) ? 1 : 0;
-      if (fused_level(lv, l)) {
+      if (can_tile && fused_level(lv, l)) {
         pre2d(lv, l, nu1, z);
       } else {
         for (int s = 0; s < nu1; ++s) sweep(lv[l], 0, s == 0 ? z : 0);
@@ -1303,15 +1365,15 @@ struct scihpc_mg {
     bool post0_done = false;
     for (int l = top; l > 0; --l) {
       const int f = l - 1;
-      if (fused_level(lv, f)) {
+      if (can_tile && fused_level(lv, f)) {
         const int mode = f == 0 ? post0 : 0;
-        OuterArgs none{};
-        if (mode == 0) post2d_mode<T, 0>(lv, f, nu2, none);
-        else if (mode == 1) post2d_mode<T, 1>(lv, f, nu2, none);
-        else post2d_mode<T, 2>(lv, f, nu2, *oa);
-        if (f == 0 && mode != 0) {
-          partials = (int)tiles2d(lv[0]);
+        if (mode == 0) post2d_mode<T, 0>(lv, f, nu2, oa);
+        else if (mode == 1) post2d_mode<T, 1>(lv, f, nu2, oa);
+        else if (mode == 2) post2d_mode<T, 2>(lv, f, nu2, oa);
+        else post2d_mode<T, 3>(lv, f, nu2, oa);
+        if (f == 0) {
           post0_done = true;
+          partials = mode != 0 ? (int)tiles2d(lv[0]) : 0;
         }
       } else {
         prolong_add(lv[l], lv[f]);
@@ -1322,33 +1384,39 @@ struct scihpc_mg {
     return partials;
   }
 
-  // level-0 post work without the tiled kernel (3-D, or one-block cycles)
+  // level-0 post work without the tiled kernel (3-D, or one-block cycles):
+  // 1 FP64 residual norm, 2 mixed outer step, 3 z.q for the flexible CG;
+  // returns the number of per-block partial sums written
   template <typename T>
-  int post0_generic(std::vector<Level<T>> &lv, int post0, const OuterArgs *oa) {
+  int post0_generic(std::vector<Level<T>> &lv, int post0, const OuterArgs &oa) {
     const unsigned nb = ceil_div(lv[0].n, kBlock);
-    if constexpr (std::is_same<T, double>::value) {
-      if (post0 == 1) SCIHPC_DISPATCH(k_residual, double, nb, view(lv[0]), 0, partial, (const double *)nullptr);
-      return (int)nb;
-    } else {
-      // e = lv[0].sol; write p_new and float(r) into oa->r32
-      if (ndim == 2) {
-        if (variable)
-          k_outer_update<2, true><<<nb, kBlock, 0, stream>>>(oa->P, oa->p_old, lv[0].sol, oa->p_new, oa->r32,
-                                                              partial, 0);
-        else
-          k_outer_update<2, false><<<nb, kBlock, 0, stream>>>(oa->P, oa->p_old, lv[0].sol, oa->p_new, oa->r32,
-                                                               partial, 0);
-      } else {
-        if (variable)
-          k_outer_update<3, true><<<nb, kBlock, 0, stream>>>(oa->P, oa->p_old, lv[0].sol, oa->p_new, oa->r32,
-                                                              partial, 0);
-        else
-          k_outer_update<3, false><<<nb, kBlock, 0, stream>>>(oa->P, oa->p_old, lv[0].sol, oa->p_new, oa->r32,
-                                                               partial, 0);
-      }
+    if (post0 == 3) {
+      k_dot_zq<T><<<nb, kBlock, 0, stream>>>(lv[0].sol, oa.q, lv[0].n, partial);
       SCIHPC_CHECK_LAUNCH();
       return (int)nb;
     }
+    if constexpr (std::is_same<T, double>::value) {
+      SCIHPC_DISPATCH(k_residual, double, nb, view(lv[0]), 0, partial, (const double *)nullptr);
+    } else {
+      // e = lv[0].sol; write p_new and float(r) into oa.r32
+#define SCIHPC_OUTER(D, V)                                                                                     \
+  k_outer_update<D, V><<<nb, kBlock, 0, stream>>>(oa.P, oa.p_old, lv[0].sol, oa.p_new, oa.r32, partial, 0)
+      if (ndim == 2) { if (variable) SCIHPC_OUTER(2, true); else SCIHPC_OUTER(2, false); }
+      else { if (variable) SCIHPC_OUTER(3, true); else SCIHPC_OUTER(3, false); }
+#undef SCIHPC_OUTER
+      SCIHPC_CHECK_LAUNCH();
+    }
+    return (int)nb;
+  }
+
+  // sum of partial[0..np) on the host (the norm the host tests anyway):
+  // one D2H copy, fixed summation order
+  double host_sum(int np) {
+    SCIHPC_HIP(hipMemcpyAsync(host_part, partial, sizeof(double) * np, hipMemcpyDeviceToHost, stream));
+    SCIHPC_HIP(hipStreamSynchronize(stream));
+    double s = 0.0;
+    for (int q = 0; q < np; ++q) s += host_part[q];
+    return s;
   }
 
   // sum of partial[0..nb) -> scal[slot]; returns nothing (device only)
@@ -1627,7 +1695,8 @@ struct scihpc_mg {
 
   // ------------------------------------------------------- solve
   // FP64 work vectors of the flexible-CG solve
-  double *fx_ = nullptr, *fr_ = nullptr, *fp_[2] = {nullptr, nullptr}, *fq_ = nullptr, *partial2 = nullptr;
+  double *fx_ = nullptr, *fr_ = nullptr, *fp_[2] = {nullptr, nullptr}, *fq_ = nullptr;
+  double last_rr_ = 0.0;  // final relative residual of the last flexible-CG solve
   void ensure_fcg() {
     if (fx_) return;
     const long n = d[0].n;
@@ -1636,7 +1705,6 @@ struct scihpc_mg {
     fp_[0] = dalloc<double>(n, stream);
     fp_[1] = dalloc<double>(n, stream);
     fq_ = dalloc<double>(n, stream);
-    partial2 = dalloc<double>(ceil_div(n, kBlock) + 64, stream);
   }
 
   // Krylov-accelerated defect correction: FCG(1) with the V-cycle (FP64 or
@@ -1666,24 +1734,24 @@ struct scihpc_mg {
     double rr = zero ? (bnorm > 0.0 ? 1.0 : 0.0) : std::sqrt(fetch(S_NORM)) / scale;
     int it = 0, cur = 0;
     while (rr > o.rel_tol && it < o.max_cycles) {
+      // z = B r; from the second iteration on the level-0 post kernel also
+      // forms z . q_old (S_ZQ) for the flexible direction update
+      OuterArgs oa{};
+      oa.q = fq_;
       const void *z;
+      int np;
       if (mixed) {
-        vcycle(f, o.nu1, o.nu2, true);
+        np = vcycle(f, o.nu1, o.nu2, true, it > 0 ? 3 : 0, oa);
         z = f[0].sol;
       } else {
-        vcycle(d, o.nu1, o.nu2, true);
+        np = vcycle(d, o.nu1, o.nu2, true, it > 0 ? 3 : 0, oa);
         z = l0.sol;
       }
-      if (it > 0) {
-        if (mixed) k_dot_zq<float><<<nb, kBlock, 0, stream>>>((const float *)z, fq_, n, partial);
-        else k_dot_zq<double><<<nb, kBlock, 0, stream>>>((const double *)z, fq_, n, partial);
-        SCIHPC_CHECK_LAUNCH();
-        finalize(nb, S_ZQ);
-      }
+      if (it > 0) finalize(np, S_ZQ);
       double *pold = fp_[cur], *pnew = fp_[cur ^ 1];
 #define SCIHPC_FCG_DIR(T, D, V)                                                                                  \
   k_fcg_direction<T, D, V><<<nb, kBlock, 0, stream>>>(view(l0), (const T *)z, pold, pnew, fq_, r, scal, it == 0, \
-                                                       partial, partial2)
+                                                       partial)
       if (mixed) {
         if (ndim == 2) { if (variable) SCIHPC_FCG_DIR(float, 2, true); else SCIHPC_FCG_DIR(float, 2, false); }
         else { if (variable) SCIHPC_FCG_DIR(float, 3, true); else SCIHPC_FCG_DIR(float, 3, false); }
@@ -1693,16 +1761,15 @@ struct scihpc_mg {
       }
 #undef SCIHPC_FCG_DIR
       SCIHPC_CHECK_LAUNCH();
-      finalize(nb, S_PQ);
-      k_finalize<<<1, kBlock, 0, stream>>>(partial2, nb, scal, S_PR, 0);
+      k_finalize2<<<1, kBlock, 0, stream>>>(partial, nb, scal, S_PQ, S_PR);
       SCIHPC_CHECK_LAUNCH();
       cur ^= 1;
       k_fcg_update<<<nb, kBlock, 0, stream>>>(n, x, r, fp_[cur], fq_, scal, mixed ? f[0].rhs : nullptr, partial);
       SCIHPC_CHECK_LAUNCH();
-      finalize(nb, S_NORM);
-      rr = std::sqrt(fetch(S_NORM)) / scale;
+      rr = std::sqrt(host_sum(nb)) / scale;
       ++it;
     }
+    last_rr_ = rr;
     if (!mixed) std::swap(l0.rhs, fr_);  // restore b
     SCIHPC_HIP(hipMemcpyAsync(l0.sol, x, sizeof(double) * n, hipMemcpyDeviceToDevice, stream));
     if (bc == SCIHPC_MG_NEUMANN) sub_mean(l0.sol, n);
@@ -1731,14 +1798,12 @@ struct scihpc_mg {
     if (o.accel == 1) {
       solve_fcg(o, st, zero, bnorm, scale);
       cycles = st.cycles;
-      rr = std::sqrt(host_scal[S_NORM]) / scale;
-      if (cycles == 0) rr = zero ? (bnorm > 0.0 ? 1.0 : 0.0) : rr;
+      rr = last_rr_;
     } else if (o.precision == SCIHPC_PREC_FP64) {
       rr = zero ? (bnorm > 0.0 ? 1.0 : 0.0) : residual_norm(0, false) / scale;
       while (rr > o.rel_tol && cycles < o.max_cycles) {
         const int np = vcycle(d, o.nu1, o.nu2, zero && cycles == 0, 1);
-        finalize(np, S_NORM);
-        rr = std::sqrt(fetch(S_NORM)) / scale;
+        rr = std::sqrt(host_sum(np)) / scale;
         ++cycles;
       }
       if (zero && cycles == 0) SCIHPC_HIP(hipMemsetAsync(l0.sol, 0, sizeof(double) * l0.n, stream));
@@ -1761,20 +1826,22 @@ struct scihpc_mg {
             k_outer_update<3, false><<<nb, kBlock, 0, stream>>>(view(l0), p, f0.sol, q, f0.rhs, partial, is_first);
         }
         SCIHPC_CHECK_LAUNCH();
-        finalize(nb, S_NORM);
         std::swap(p, q);
-        return std::sqrt(fetch(S_NORM)) / scale;
+        return std::sqrt(host_sum(nb)) / scale;
       };
       // initial residual into the FP32 right-hand side (e = 0)
       if (!zero) SCIHPC_HIP(hipMemsetAsync(f0.sol, 0, sizeof(float) * f0.n, stream));
       rr = first(zero ? 1 : 0);
       while (rr > o.rel_tol && cycles < o.max_cycles) {
-        OuterArgs oa{view(l0), p, q, f0.rhs_alt};
-        const int np = vcycle(f, o.nu1, o.nu2, true, 2, &oa);
+        OuterArgs oa{};
+        oa.P = view(l0);
+        oa.p_old = p;
+        oa.p_new = q;
+        oa.r32 = f0.rhs_alt;
+        const int np = vcycle(f, o.nu1, o.nu2, true, 2, oa);
         std::swap(f0.rhs, f0.rhs_alt);
         std::swap(p, q);
-        finalize(np, S_NORM);
-        rr = std::sqrt(fetch(S_NORM)) / scale;
+        rr = std::sqrt(host_sum(np)) / scale;
         ++cycles;
       }
       if (p != l0.sol) {
@@ -1885,7 +1952,10 @@ extern "C" int scihpc_mg_create(int ndim, int nx, int ny, int nz, double dx, dou
       }
       h->Minv = dalloc<double>((long)std::min<long>(h->d.back().n, kMaxCoarse) *
                                std::min<long>(h->d.back().n, kMaxCoarse), h->stream);
-      h->partial = dalloc<double>(ceil_div(h->d[0].n, kBlock) + 64, h->stream);
+      h->partial = dalloc<double>(4 * ((long)ceil_div(h->d[0].n, kBlock) + 64), h->stream);
+      SCIHPC_HIP(hipHostMalloc((void **)&h->host_part,
+                               sizeof(double) * 4 * ((long)ceil_div(h->d[0].n, kBlock) + 64),
+                               hipHostMallocDefault));
       h->scal = dalloc<double>(S_COUNT, h->stream);
       h->p_alt = dalloc<double>(h->d[0].n, h->stream);
       SCIHPC_HIP(hipHostMalloc((void **)&h->host_scal, sizeof(double) * S_COUNT, hipHostMallocDefault));
@@ -1943,12 +2013,14 @@ extern "C" int scihpc_mg_destroy(scihpc_mg *h) {
   if (h->Minv_ref != h->Minv) fr(h->Minv_ref);
   fr(h->Minv);
   fr(h->partial);
+  fr(h->blk_dev[0]);
+  fr(h->blk_dev[1]);
+  if (h->host_part) (void)hipHostFree(h->host_part);
   fr(h->fx_);
   fr(h->fr_);
   fr(h->fp_[0]);
   fr(h->fp_[1]);
   fr(h->fq_);
-  fr(h->partial2);
   fr(h->scal);
   fr(h->p_alt);
   if (h->host_scal) (void)hipHostFree(h->host_scal);
@@ -2212,7 +2284,11 @@ extern "C" int scihpc_mg_time_kernel(scihpc_mg *h, int which, int iters, double 
     const double n = (double)l0.n, nc = (double)h->d[1].n;
     const double nf = h->variable ? h->ndim : 0;  // face arrays per level
     const bool tiled = h->fused_level(h->d, 0);
-    OuterArgs oa{h->view(l0), l0.sol, h->p_alt, f0.rhs_alt};
+    OuterArgs oa{};
+    oa.P = h->view(l0);
+    oa.p_old = l0.sol;
+    oa.p_new = h->p_alt;
+    oa.r32 = f0.rhs_alt;
     auto launch = [&] {
       switch (which) {
         case 0:  // FP64 level-0 pre-smoothing kernel
@@ -2233,11 +2309,11 @@ extern "C" int scihpc_mg_time_kernel(scihpc_mg *h, int which, int iters, double 
           break;
         case 2:  // FP32 level-0 post-smoothing kernel fused with the FP64 outer step
           if (tiled) h->post2d_mode<float, 2>(h->f, 0, 2, oa);
-          else h->post0_generic(h->f, 2, &oa);
+          else h->post0_generic(h->f, 2, oa);
           break;
         case 3:  // FP64 level-0 post-smoothing kernel with the residual norm
-          if (tiled) h->post2d_mode<double, 1>(h->d, 0, 2, OuterArgs{});
-          else h->post0_generic(h->d, 1, nullptr);
+          if (tiled) h->post2d_mode<double, 1>(h->d, 0, 2, oa);
+          else h->post0_generic(h->d, 1, oa);
           break;
       }
     };
