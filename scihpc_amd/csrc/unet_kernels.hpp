@@ -1,0 +1,193 @@
+// unet_kernels.hpp -- BF16 convolution kernels of the learned pressure
+// preconditioner for gfx950.
+//
+// Activations are NHWC BF16 tensors addressed through channel views (base
+// pointer, channel stride of the underlying tensor, channel offset), so a
+// skip connection is a slice of the decoder's concatenation buffer and no
+// concatenation copy exists.  The 3x3 convolution is an implicit GEMM on
+// the 16x16x16 BF16 matrix cores with FP32 accumulation:
+//   M = output pixels (a workgroup owns 64 pixels of one image row, each
+//       wavefront 16 of them), N = output channels (16-wide blocks, all held
+//       by the wavefront), K = 9 taps x input channels (16-channel chunks).
+// Operand layout of v_mfma_f32_16x16x16bf16_1k (lane l of 64):
+//   A (16 x 16, M x K): row l % 16, k = 4 (l / 16) .. +3
+//   B (16 x 16, K x N): k = 4 (l / 16) .. +3, column l % 16
+//   C (16 x 16, float4): rows 4 (l / 16) + j, column l % 16, j = 0..3
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace scihpc {
+
+typedef uint16_t bf16_t;
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+__host__ __device__ __forceinline__ float bf16_to_f(bf16_t b) {
+  const unsigned u = ((unsigned)b) << 16;
+  float f;
+  __builtin_memcpy(&f, &u, 4);
+  return f;
+}
+// round to nearest even (finite inputs)
+__host__ __device__ __forceinline__ bf16_t f_to_bf16(float f) {
+  unsigned u;
+  __builtin_memcpy(&u, &f, 4);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (bf16_t)(u >> 16);
+}
+
+struct View {  // NHWC channel slice
+  bf16_t *p;
+  int stride;  // channels of the underlying tensor
+  int off;     // first channel of the slice
+};
+
+struct ConvShape {
+  int n, h, w;     // images, rows, columns
+  int cin, cout;   // channels (cin multiple of 16, cout multiple of 16)
+};
+
+constexpr int kConvThreads = 256;  // 4 wavefronts x 16 pixels = 64 pixels per workgroup
+
+// y = act(conv3x3(x, w) + b); w layout [cout][9][cin] (BF16), b FP32 (may be
+// null).  mask: when given, x is read as x * (mask > 0) (ReLU derivative of
+// the layer whose output gradient x is, for the backward-data use).
+template <int NB>
+__global__ void __launch_bounds__(kConvThreads)
+    k_conv3x3(View x, View mask, const bf16_t *__restrict__ w, const float *__restrict__ b, View y, ConvShape s,
+              int relu) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int segs = (s.w + 63) / 64;
+  int blk = blockIdx.x;
+  const int seg = blk % segs;
+  blk /= segs;
+  const int row = blk % s.h, img = blk / s.h;
+  const int px = seg * 64 + wave * 16 + (lane & 15);  // pixel of this lane's A row
+  const int kq = 4 * (lane >> 4);                    // k offset of this lane inside a chunk
+  floatx4 acc[NB];
+#pragma unroll
+  for (int nb = 0; nb < NB; ++nb) acc[nb] = floatx4{0.f, 0.f, 0.f, 0.f};
+  const int chunks = s.cin / 16;
+  for (int t = 0; t < 9; ++t) {
+    const int yy = row + t / 3 - 1, xx = px + t % 3 - 1;
+    const bool in = yy >= 0 && yy < s.h && xx >= 0 && xx < s.w;
+    const long pix = ((long)img * s.h + (in ? yy : 0)) * s.w + (in ? xx : 0);
+    for (int cc = 0; cc < chunks; ++cc) {
+      bf16x4 a = {0, 0, 0, 0};
+      if (in) {
+        const bf16_t *src = x.p + pix * x.stride + x.off + cc * 16 + kq;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) a[q] = (short)src[q];
+        if (mask.p) {
+          const bf16_t *m = mask.p + pix * mask.stride + mask.off + cc * 16 + kq;
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            if (!(bf16_to_f(m[q]) > 0.f)) a[q] = 0;
+        }
+      }
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb) {
+        const bf16_t *ws = w + ((long)(nb * 16 + (lane & 15)) * 9 + t) * s.cin + cc * 16 + kq;
+        bf16x4 bb;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) bb[q] = (short)ws[q];
+        acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bb, acc[nb], 0, 0, 0);
+      }
+    }
+  }
+  // epilogue: rows 4 (lane/16) + j of the wave's 16 pixels, column lane % 16
+#pragma unroll
+  for (int nb = 0; nb < NB; ++nb) {
+    const int co = nb * 16 + (lane & 15);
+    const float bias = b ? b[co] : 0.f;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int p = seg * 64 + wave * 16 + kq + j;
+      if (p >= s.w) continue;
+      float v = acc[nb][j] + bias;
+      if (relu) v = v > 0.f ? v : 0.f;
+      const long pix = ((long)img * s.h + row) * s.w + p;
+      y.p[pix * y.stride + y.off + co] = f_to_bf16(v);
+    }
+  }
+}
+
+// dW[co][t][ci] += sum_p g[p][co] x[p + off(t)][ci] and db[co] += sum_p g[p][co]
+// (g read through the ReLU mask of the layer output).  A workgroup owns one
+// (16 output x 16 input channel) block and a run of pixels; each wavefront
+// accumulates the 9 tap tiles over its pixels and adds them to dW (FP32
+// atomics).  A = g^T (co x pixels), B = x (pixels x ci).
+constexpr int kWgradPixels = 1024;  // pixels per workgroup
+
+__global__ void __launch_bounds__(kConvThreads)
+    k_conv3x3_wgrad(View x, View g, View mask, float *__restrict__ dw, float *__restrict__ db, ConvShape s) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int cbs = s.cout / 16, ccs = s.cin / 16;
+  int blk = blockIdx.x;
+  const int cb = blk % cbs;
+  blk /= cbs;
+  const int cc = blk % ccs;
+  const long p_begin = (long)(blk / ccs) * kWgradPixels;
+  const long npix = (long)s.n * s.h * s.w;
+  floatx4 acc[9];
+#pragma unroll
+  for (int t = 0; t < 9; ++t) acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+  float bsum = 0.f;
+  const int co = cb * 16 + (lane & 15), ci = cc * 16 + (lane & 15);
+  const int kq = 4 * (lane >> 4);
+  // each wavefront takes every 4th group of 16 pixels
+  for (long p0 = p_begin + wave * 16; p0 < p_begin + kWgradPixels && p0 < npix; p0 += 64) {
+    bf16x4 a;
+    int pr[4], pc[4], pi[4];
+    bool pv[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const long p = p0 + kq + q;
+      pv[q] = p < npix;
+      const long pp = pv[q] ? p : 0;
+      pc[q] = (int)(pp % s.w);
+      pr[q] = (int)((pp / s.w) % s.h);
+      pi[q] = (int)(pp / ((long)s.w * s.h));
+      float gv = 0.f;
+      if (pv[q]) {
+        gv = bf16_to_f(g.p[pp * g.stride + g.off + co]);
+        if (mask.p && !(bf16_to_f(mask.p[pp * mask.stride + mask.off + co]) > 0.f)) gv = 0.f;
+      }
+      a[q] = (short)f_to_bf16(gv);
+      if (cc == 0 && lane < 16) bsum += gv;  // rows of A owned by lanes 0..15 once
+    }
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+      bf16x4 bb;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int yy = pr[q] + t / 3 - 1, xx = pc[q] + t % 3 - 1;
+        bf16_t v = 0;
+        if (pv[q] && yy >= 0 && yy < s.h && xx >= 0 && xx < s.w)
+          v = x.p[(((long)pi[q] * s.h + yy) * s.w + xx) * x.stride + x.off + ci];
+        bb[q] = (short)v;
+      }
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bb, acc[t], 0, 0, 0);
+    }
+  }
+  // C rows = output channels cb*16 + kq + j, column = input channel ci
+#pragma unroll
+  for (int t = 0; t < 9; ++t)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int o = cb * 16 + kq + j;
+      atomicAdd(&dw[((long)o * 9 + t) * s.cin + ci], acc[t][j]);
+    }
+  // bias: lanes 0..15 hold column sums of their A row (= output channel co)
+  // only in the cc == 0 blocks; the other lanes added zeros
+  if (cc == 0) {
+    // lanes l and l+16, l+32, l+48 share co; lanes >= 16 never added, so a
+    // single atomic per lane < 16 suffices
+    if (lane < 16) atomicAdd(&db[co], bsum);
+  }
+}
+
+}  // namespace scihpc
