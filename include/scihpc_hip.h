@@ -24,6 +24,8 @@
 #define SCIHPC_API
 #endif
 
+#include <stdint.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -129,7 +131,8 @@ typedef struct {
   int max_cycles;    /* cap on V-cycles (outer corrections in mixed mode) */
   int nu1, nu2;      /* smoothing sweeps down / up */
   int precision;     /* SCIHPC_PREC_FP64 or SCIHPC_PREC_MIXED */
-  int use_learned;   /* 1: learned initial guess/correction (if attached) */
+  int use_learned;   /* 1: learned initial guess (if attached), judged by the
+                        FP64 residual; 2: the guess kept unjudged (ablation) */
   int zero_guess;    /* 1: start from p = 0, else from the current level-0 sol */
   int accel;         /* 0: stationary defect correction (one V-cycle per
                         correction); 1: flexible CG (FCG(1)) with the V-cycle
@@ -142,6 +145,7 @@ typedef struct {
   int learned_rejected;  /* learned steps discarded (plain MG used instead) */
   double rel_residual;   /* final ||b - A p|| / ||b||, FP64 */
   double ms;             /* device time of the solve (HIP events) */
+  double learned_residual; /* ||b - A p0|| / ||b|| of the learned guess (0 if none) */
 } scihpc_solve_stats;
 
 /* Solve A p = b on level 0 to opts->rel_tol (the call behind
@@ -153,13 +157,76 @@ typedef struct {
 SCIHPC_API int scihpc_mg_solve(scihpc_mg *h, const scihpc_solve_opts *opts,
                     scihpc_solve_stats *stats);
 
-/* Timing hook: runs `iters` launches of one level-0 kernel between HIP
- * events on the handle stream; which = 0 FP64 smoother sweep, 1 FP32
- * smoother sweep, 2 FP64 outer defect-correction kernel, 3 FP64 residual +
- * restriction.  Returns the mean milliseconds per launch and the algorithmic
- * HBM bytes of one launch. */
+/* Timing hook (no reference counterpart): runs `iters` launches of one
+ * level-0 kernel between HIP events on the handle stream; which = 0 FP64
+ * pre-smoothing (nu1 = 2 sweeps + residual + restriction), 1 the same in
+ * FP32, 2 FP32 post-smoothing fused with the FP64 outer update, 3 FP64
+ * post-smoothing with the residual norm.  Returns the mean milliseconds per
+ * launch and the algorithmic HBM bytes of one launch. */
 SCIHPC_API int scihpc_mg_time_kernel(scihpc_mg *h, int which, int iters, double *ms,
                           double *bytes);
+
+/* ------------------------------------------------------------------ */
+/* Learned pressure guess (U-Net).  No reference counterpart: SciHPC has no
+ * learned component.  A 4-level U-Net (16/32/64/128 channels, 3x3
+ * convolutions, BF16 weights and activations, FP32 accumulation on the
+ * matrix cores) maps (b / sigma_b, ln rho / ln 1000) on one multigrid level
+ * to p / (sigma_b * out_scale) there; the solve prolongs, smooths and scales
+ * it and keeps it only if it lowers the FP64 residual.                    */
+typedef struct scihpc_unet scihpc_unet;
+
+/* height x width must be multiples of 8; max_batch images per pass */
+SCIHPC_API int scihpc_unet_create(int height, int width, int max_batch,
+                                  unsigned long long seed, scihpc_unet **out);
+SCIHPC_API int scihpc_unet_destroy(scihpc_unet *h);
+SCIHPC_API int scihpc_unet_num_params(scihpc_unet *h, long long *n);
+/* layer 0..13: 3x3 convolutions (w [cout][3][3][cin] then b [cout] at
+ * offset); layer 14: the 1x1 head (16 weights then 1 bias) */
+SCIHPC_API int scihpc_unet_layer(scihpc_unet *h, int layer, int *cin, int *cout,
+                                 int *level, long long *offset);
+SCIHPC_API int scihpc_unet_get_params(scihpc_unet *h, float *host);
+SCIHPC_API int scihpc_unet_set_params(scihpc_unet *h, const float *host);
+SCIHPC_API int scihpc_unet_get_grads(scihpc_unet *h, float *host);
+SCIHPC_API int scihpc_unet_set_out_scale(scihpc_unet *h, float scale);
+SCIHPC_API int scihpc_unet_get_out_scale(scihpc_unet *h, float *scale);
+/* training samples kept on the device: feat [n][H*W][2], target [n][H*W],
+ * faces [n][H*W][2] (host FP32, as scihpc_mg_learned_features writes them).
+ * With faces the loss is the A(rho) energy norm of the error relative to the
+ * target's, sum_f c_f (e_i - e_j)^2 / sum_f c_f (t_i - t_j)^2 (what flexible
+ * CG reduces); with faces = NULL it is the mean squared error.  Set
+ * out_scale first: targets are divided by it. */
+SCIHPC_API int scihpc_unet_set_samples(scihpc_unet *h, int n, const float *feat,
+                                       const float *target, const float *faces);
+/* forward pass on stored samples; idx [n][2] = (sample, flips: bit 0 mirrors
+ * x, bit 1 mirrors y); out (host [n][H*W]) may be NULL */
+SCIHPC_API int scihpc_unet_forward(scihpc_unet *h, int n, const int *idx, float *out);
+/* loss of the last forward pass against target / out_scale, and all
+ * parameter gradients (read with scihpc_unet_get_grads) */
+SCIHPC_API int scihpc_unet_backward(scihpc_unet *h, double *loss);
+/* forward + backward + one Adam step (lr); loss may be NULL (no sync) */
+SCIHPC_API int scihpc_unet_train_step(scihpc_unet *h, int n, const int *idx, float lr,
+                                      double *loss);
+SCIHPC_API int scihpc_unet_sync(scihpc_unet *h);
+/* single 3x3 convolution on device pointers (NHWC BF16 x [n][h][w][cin],
+ * w [cout][3][3][cin] BF16, b [cout] FP32 or NULL, y [n][h][w][cout]);
+ * cin a multiple of 16, cout = 16 x {1,2,3,4,6,8,12}; synchronous */
+SCIHPC_API int scihpc_conv3x3_forward(const uint16_t *x, int n, int height, int width,
+                                      int cin, const uint16_t *w, const float *b,
+                                      int cout, int relu, uint16_t *y);
+
+/* Use `net` (NULL detaches) for the learned initial guess of
+ * scihpc_mg_solve(use_learned = 1); its size must equal multigrid level
+ * `level`'s grid.  2-D only.  The guess is scaled by the energy-optimal
+ * factor; with flexible CG it is kept only if the FP64 residual after its
+ * first preconditioned step is below ||b|| (else the solve restarts from
+ * p = 0), with stationary cycles only if its own residual is below ||b||. */
+SCIHPC_API int scihpc_mg_attach_learned(scihpc_mg *h, scihpc_unet *net, int level);
+/* The network's input features of the current level-0 b (and density) on
+ * `level`: feat [H*W][2]; target [H*W] (may be NULL) = block mean of the
+ * current level-0 solution / sigma_b, the training target; faces [H*W][2]
+ * (may be NULL) = the level's face coefficients x h^2 toward x+1 and y+1. */
+SCIHPC_API int scihpc_mg_learned_features(scihpc_mg *h, int level, float *feat,
+                                          float *target, float *faces);
 
 /* ------------------------------------------------------------------ */
 /* Level-set advection: WENO5-Z derivatives + TVD-RK3 (+ BCs)          */

@@ -16,7 +16,7 @@ import numpy as np
 from ._capi import (BC_CODES, MG_DIRICHLET, MG_NEUMANN, NOT_CONVERGED, OK, PREC_FP64,
                     PREC_MIXED, ScihpcError, SolveOpts, SolveStats, check, lib)
 
-__all__ = ["Multigrid", "LevelSet", "ScihpcError", "PREC_FP64", "PREC_MIXED", "padded_shape"]
+__all__ = ["Multigrid", "LevelSet", "UNet", "ScihpcError", "PREC_FP64", "PREC_MIXED", "padded_shape"]
 
 GHC = 3
 
@@ -141,12 +141,126 @@ class Multigrid:
         rc = check(self._lib.scihpc_mg_solve(self.h, C.byref(o), C.byref(s)), allow=(OK, NOT_CONVERGED))
         return {"converged": rc == OK, "cycles": s.cycles, "rel_residual": s.rel_residual,
                 "ms": s.ms, "learned_accepted": s.learned_accepted,
-                "learned_rejected": s.learned_rejected}
+                "learned_rejected": s.learned_rejected, "learned_residual": s.learned_residual}
+
+    # ---------------------------------------------- learned guess
+    def attach_learned(self, net, level=1):
+        """Use UNet `net` (None detaches) for solve(use_learned=True)."""
+        self._net = net
+        check(self._lib.scihpc_mg_attach_learned(self.h, net.h if net is not None else None, level))
+
+    def learned_features(self, level=1, with_target=False, with_faces=False):
+        """(features [H*W, 2], target [H*W] or None, faces [H*W, 2] or None)
+        of the current b, rho and sol on `level`."""
+        nx, ny = self.shapes[level][0], self.shapes[level][1]
+        feat = np.empty(2 * nx * ny, np.float32)
+        tgt = np.empty(nx * ny, np.float32) if with_target else None
+        faces = np.empty(2 * nx * ny, np.float32) if with_faces else None
+        check(self._lib.scihpc_mg_learned_features(self.h, level, feat, tgt, faces))
+        return feat.reshape(ny * nx, 2), tgt, None if faces is None else faces.reshape(ny * nx, 2)
 
     def time_kernel(self, which, iters=20):
         ms, b = C.c_double(), C.c_double()
         check(self._lib.scihpc_mg_time_kernel(self.h, which, iters, C.byref(ms), C.byref(b)))
         return ms.value, b.value
+
+
+class UNet:
+    """The learned pressure guess: 4-level U-Net (16/32/64/128 channels) with
+    BF16 MFMA convolutions; no SciHPC counterpart.  height x width is the
+    grid of the multigrid level it serves (multiples of 8)."""
+
+    N_LAYERS = 15  # 14 3x3 convolutions + the 1x1 head
+
+    def __init__(self, height, width, max_batch=1, seed=0):
+        self._lib = lib()
+        h = C.c_void_p()
+        check(self._lib.scihpc_unet_create(height, width, max_batch, seed, C.byref(h)))
+        self.h = h
+        self.H, self.W, self.max_batch = height, width, max_batch
+        n = C.c_longlong()
+        check(self._lib.scihpc_unet_num_params(h, C.byref(n)))
+        self.num_params = n.value
+        self.layers = []
+        for i in range(self.N_LAYERS):
+            a, b, c, o = C.c_int(), C.c_int(), C.c_int(), C.c_longlong()
+            check(self._lib.scihpc_unet_layer(h, i, C.byref(a), C.byref(b), C.byref(c), C.byref(o)))
+            self.layers.append(dict(cin=a.value, cout=b.value, level=c.value, offset=o.value))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._lib.scihpc_unet_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self):
+        out = np.empty(self.num_params, np.float32)
+        check(self._lib.scihpc_unet_get_params(self.h, out))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.num_params
+        check(self._lib.scihpc_unet_set_params(self.h, p))
+
+    def grads(self):
+        out = np.empty(self.num_params, np.float32)
+        check(self._lib.scihpc_unet_get_grads(self.h, out))
+        return out
+
+    @property
+    def out_scale(self):
+        s = C.c_float()
+        check(self._lib.scihpc_unet_get_out_scale(self.h, C.byref(s)))
+        return s.value
+
+    @out_scale.setter
+    def out_scale(self, v):
+        check(self._lib.scihpc_unet_set_out_scale(self.h, float(v)))
+
+    def set_samples(self, feat, target, faces=None):
+        """feat [n, H*W, 2], target [n, H*W], faces [n, H*W, 2] or None (FP32,
+        host); faces select the energy-norm loss.  Set out_scale first."""
+        feat = np.ascontiguousarray(feat, np.float32)
+        target = np.ascontiguousarray(target, np.float32)
+        n = feat.shape[0]
+        assert feat.size == n * self.H * self.W * 2 and target.size == n * self.H * self.W
+        if faces is not None:
+            faces = np.ascontiguousarray(faces, np.float32).ravel()
+            assert faces.size == feat.size
+        check(self._lib.scihpc_unet_set_samples(self.h, n, feat.ravel(), target.ravel(), faces))
+
+    @staticmethod
+    def _idx(samples, flips=None):
+        samples = np.asarray(samples, np.int32)
+        flips = np.zeros_like(samples) if flips is None else np.asarray(flips, np.int32)
+        return np.ascontiguousarray(np.stack([samples, flips], 1).ravel(), np.int32)
+
+    def forward(self, samples, flips=None):
+        idx = self._idx(samples, flips)
+        out = np.empty(len(samples) * self.H * self.W, np.float32)
+        check(self._lib.scihpc_unet_forward(self.h, len(samples), idx, out))
+        return out.reshape(len(samples), self.H * self.W)
+
+    def backward(self):
+        loss = C.c_double()
+        check(self._lib.scihpc_unet_backward(self.h, C.byref(loss)))
+        return loss.value
+
+    def train_step(self, samples, lr, flips=None, want_loss=True):
+        idx = self._idx(samples, flips)
+        loss = C.c_double()
+        check(self._lib.scihpc_unet_train_step(self.h, len(samples), idx, float(lr),
+                                               C.byref(loss) if want_loss else None))
+        return loss.value if want_loss else None
+
+    def sync(self):
+        check(self._lib.scihpc_unet_sync(self.h))
 
 
 class LevelSet:

@@ -32,12 +32,27 @@ class SolveOpts(C.Structure):
 
 class SolveStats(C.Structure):
     _fields_ = [("cycles", C.c_int), ("learned_accepted", C.c_int),
-                ("learned_rejected", C.c_int), ("rel_residual", C.c_double), ("ms", C.c_double)]
+                ("learned_rejected", C.c_int), ("rel_residual", C.c_double), ("ms", C.c_double),
+                ("learned_residual", C.c_double)]
 
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+def _nullable(base):
+    """ndpointer that also accepts None (passed as NULL)."""
+    class _P(base):
+        @classmethod
+        def from_param(cls, obj):
+            if obj is None:
+                return None
+            return base.from_param(obj)
+    return _P
+
+
+_fpn = _nullable(_fp)
 _vp = C.c_void_p
 _i, _d = C.c_int, C.c_double
 _pi = C.POINTER(C.c_int)
@@ -71,6 +86,23 @@ SIGNATURES = {
     "scihpc_mg_vcycle": [_vp, _i, _i],
     "scihpc_mg_solve": [_vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)],
     "scihpc_mg_time_kernel": [_vp, _i, _i, _pd, _pd],
+    "scihpc_mg_attach_learned": [_vp, _vp, _i],
+    "scihpc_mg_learned_features": [_vp, _i, _fp, _fpn, _fpn],
+    "scihpc_unet_create": [_i, _i, _i, C.c_ulonglong, C.POINTER(_vp)],
+    "scihpc_unet_destroy": [_vp],
+    "scihpc_unet_num_params": [_vp, C.POINTER(C.c_longlong)],
+    "scihpc_unet_layer": [_vp, _i, _pi, _pi, _pi, C.POINTER(C.c_longlong)],
+    "scihpc_unet_get_params": [_vp, _fp],
+    "scihpc_unet_set_params": [_vp, _fp],
+    "scihpc_unet_get_grads": [_vp, _fp],
+    "scihpc_unet_set_out_scale": [_vp, C.c_float],
+    "scihpc_unet_get_out_scale": [_vp, C.POINTER(C.c_float)],
+    "scihpc_unet_set_samples": [_vp, _i, _fp, _fp, _fpn],
+    "scihpc_unet_forward": [_vp, _i, _ip, _fpn],
+    "scihpc_unet_backward": [_vp, _pd],
+    "scihpc_unet_train_step": [_vp, _i, _ip, C.c_float, _pd],
+    "scihpc_unet_sync": [_vp],
+    "scihpc_conv3x3_forward": [_vp, _i, _i, _i, _i, _vp, _vp, _i, _i, _vp],
     "scihpc_ls_create": [_i, _i, _i, _i, _d, _d, _d, _ip, _dp, C.POINTER(_vp)],
     "scihpc_ls_destroy": [_vp],
     "scihpc_ls_set_phi": [_vp, _dp],

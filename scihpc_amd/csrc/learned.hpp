@@ -1,20 +1,25 @@
-// learned.hpp -- hook through which the multigrid solve asks the learned
-// preconditioner (U-Net, unet.hip) for a pressure guess.  The solver owns
-// the acceptance test (FP64 residual) and the fallback to plain multigrid.
+// learned.hpp -- how the multigrid solve reaches the learned pressure guess
+// (U-Net, unet.hip).  The solver owns the features, the prolongation of the
+// guess, the acceptance test on the FP64 residual and the fallback to plain
+// multigrid; the network only maps features to a coarse-level guess.
 #pragma once
 
 #include <hip/hip_runtime.h>
 
+struct scihpc_unet;
+
 namespace scihpc {
 
-struct LearnedModel;  // defined in unet.hip
+// out[H*W] = network(feat[H*W][2]) in normalised units, stream ordered
+void unet_infer(scihpc_unet *net, const float *feat, float *out, hipStream_t stream);
+int unet_height(const scihpc_unet *net);
+int unet_width(const scihpc_unet *net);
+float unet_out_scale(const scihpc_unet *net);
 
 struct LearnedHook {
-  LearnedModel *model = nullptr;
-  const double *rho = nullptr;  // level-0 density (MG layout, device)
-  bool active() const { return model != nullptr; }
-  // x = network(b, log rho), FP64 in/out on the device, stream ordered
-  void apply(const double *b, double *x, hipStream_t stream);
+  scihpc_unet *net = nullptr;
+  int level = 1;  // multigrid level whose grid the network works on
+  bool active() const { return net != nullptr; }
 };
 
 }  // namespace scihpc

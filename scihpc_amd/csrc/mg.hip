@@ -1070,6 +1070,86 @@ __global__ void __launch_bounds__(kBlock) k_padded_to_mg(const double *__restric
   m[c] = p[((long)(i + g) * Ny + (j + g)) * Nz + (k + ko)];
 }
 
+// ------------------------------------------------------ learned guess
+// network features on a level of block size bs (2-D): the block mean of b
+// scaled by 1/sigma_b and the block mean of ln(rho) / ln(1000); optionally
+// the block mean of the level-0 solution scaled by 1/sigma_b (training target)
+__global__ void __launch_bounds__(kBlock) k_learned_features(const double *__restrict__ rhs0,
+                                                             const double *__restrict__ rho0,
+                                                             const double *__restrict__ sol0, int nx0, int bs, int W,
+                                                             int H, double inv_sigma, float *__restrict__ feat,
+                                                             float *__restrict__ tgt) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= (long)W * H) return;
+  const int x = (int)(p % W), y = (int)(p / W);
+  double sb = 0.0, sr = 0.0, sp = 0.0;
+  for (int b = 0; b < bs; ++b)
+    for (int a = 0; a < bs; ++a) {
+      const long c = (long)(y * bs + b) * nx0 + (x * bs + a);
+      sb += rhs0[c];
+      if (rho0) sr += log(rho0[c]);
+      if (sol0) sp += sol0[c];
+    }
+  const double inv = 1.0 / ((double)bs * bs);
+  feat[2 * p] = (float)(sb * inv * inv_sigma);
+  feat[2 * p + 1] = (float)(sr * inv / 6.907755278982137);  // ln(1000)
+  if (tgt) tgt[p] = (float)(sp * inv * inv_sigma);
+}
+
+// faces of a level scaled by its h^2 (face coefficient x h^2 = 1 / mean rho
+// of the face), network layout face[p] = (x+1 face, y+1 face), zero on walls
+__global__ void __launch_bounds__(kBlock) k_learned_faces(const double *__restrict__ cx, const double *__restrict__ cy,
+                                                          int W, int H, double hx2, double hy2,
+                                                          float *__restrict__ face) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= (long)W * H) return;
+  const int x = (int)(p % W), y = (int)(p / W);
+  face[2 * p] = x + 1 < W ? (float)(cx[p] * hx2) : 0.f;
+  face[2 * p + 1] = y + 1 < H ? (float)(cy[p] * hy2) : 0.f;
+}
+
+__global__ void __launch_bounds__(kBlock) k_net_to_level(const float *__restrict__ out, double *__restrict__ sol,
+                                                         long n, double scale) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < n) sol[c] = (double)out[c] * scale;
+}
+
+// four inner products of a guess p with q = A p: p.b, p.q, b.q, q.q
+// (partial[k*nb + block]); they give the energy-optimal scale of p and the
+// residual norm of the scaled guess without another pass
+template <int DIM, bool VAR>
+__global__ void __launch_bounds__(kBlock) k_guess_dots(DLevel<double> L, const double *__restrict__ p,
+                                                       double *__restrict__ partial, int nb) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
+  if (c < L.n) {
+    int i, j, k;
+    cell_ijk(c, L.nx, L.ny, i, j, k);
+    double off, diag;
+    stencil<double, DIM, VAR>(L, p, i, j, k, c, off, diag);
+    const double q = off + diag * p[c], b = L.rhs[c];
+    v0 = p[c] * b;
+    v1 = p[c] * q;
+    v2 = b * q;
+    v3 = q * q;
+  }
+  v0 = block_sum(v0);
+  v1 = block_sum(v1);
+  v2 = block_sum(v2);
+  v3 = block_sum(v3);
+  if (threadIdx.x == 0) {
+    partial[blockIdx.x] = v0;
+    partial[nb + blockIdx.x] = v1;
+    partial[2 * nb + blockIdx.x] = v2;
+    partial[3 * nb + blockIdx.x] = v3;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_scale(double *__restrict__ x, long n, double a) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < n) x[c] *= a;
+}
+
 // ============================================================ host side
 template <typename T>
 struct Level {
@@ -1133,6 +1213,8 @@ struct scihpc_mg {
   void *blk_dev[2] = {nullptr, nullptr};   // device copies of BlockLevels<double>/<float>
   std::vector<char> blk_host[2];           // last uploaded contents
   double *host_part = nullptr;             // pinned copy of the partial sums
+  double *rho0 = nullptr;                  // level-0 density (MG layout) of the variable operator
+  float *lfeat = nullptr, *lout = nullptr; // learned-guess features and network output
 
   // ------------------------------------------------------------ views
   // rb: view of the red-black solver's operator (Dirichlet coarse levels
@@ -1709,7 +1791,11 @@ struct scihpc_mg {
 
   // Krylov-accelerated defect correction: FCG(1) with the V-cycle (FP64 or
   // FP32) as preconditioner; x, r, p, q and every inner product in FP64.
-  int solve_fcg(const scihpc_solve_opts &o, scihpc_solve_stats &st, bool zero, double bnorm, double scale) {
+  // judge: x0 is a learned guess, kept only if the FP64 residual after the
+  // first preconditioned step is below ||b||; otherwise the iteration
+  // restarts from x = 0 (the rejected step counts in st.cycles)
+  int solve_fcg(const scihpc_solve_opts &o, scihpc_solve_stats &st, bool zero, double bnorm, double scale,
+                bool judge = false) {
     ensure_fcg();
     auto &l0 = d[0];
     const long n = l0.n;
@@ -1732,8 +1818,8 @@ struct scihpc_mg {
     if (!mixed) std::swap(l0.rhs, fr_);
     double *r = mixed ? fr_ : l0.rhs;
     double rr = zero ? (bnorm > 0.0 ? 1.0 : 0.0) : std::sqrt(fetch(S_NORM)) / scale;
-    int it = 0, cur = 0;
-    while (rr > o.rel_tol && it < o.max_cycles) {
+    int it = 0, cur = 0, spent = 0;
+    while (rr > o.rel_tol && it + spent < o.max_cycles) {
       // z = B r; from the second iteration on the level-0 post kernel also
       // forms z . q_old (S_ZQ) for the flexible direction update
       OuterArgs oa{};
@@ -1768,7 +1854,25 @@ struct scihpc_mg {
       SCIHPC_CHECK_LAUNCH();
       rr = std::sqrt(host_sum(nb)) / scale;
       ++it;
+      if (judge && it == 1) {
+        judge = false;
+        st.learned_residual = rr;
+        if (std::isfinite(rr) && rr < 1.0) {
+          st.learned_accepted++;
+        } else {  // fall back to plain multigrid from x = 0
+          st.learned_rejected++;
+          spent = it;
+          it = 0;
+          cur = 0;
+          SCIHPC_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, stream));
+          if (mixed) k_copy_cast<<<nb, kBlock, 0, stream>>>(l0.rhs, fr_, f[0].rhs, n);
+          else SCIHPC_HIP(hipMemcpyAsync(r, fr_, sizeof(double) * n, hipMemcpyDeviceToDevice, stream));
+          SCIHPC_CHECK_LAUNCH();
+          rr = bnorm > 0.0 ? 1.0 : 0.0;
+        }
+      }
     }
+    it += spent;
     last_rr_ = rr;
     if (!mixed) std::swap(l0.rhs, fr_);  // restore b
     SCIHPC_HIP(hipMemcpyAsync(l0.sol, x, sizeof(double) * n, hipMemcpyDeviceToDevice, stream));
@@ -1790,13 +1894,21 @@ struct scihpc_mg {
     int cycles = 0;
     double rr;
     bool zero = o.zero_guess != 0;
-    if (o.use_learned && learned.active()) {
-      if (zero) SCIHPC_HIP(hipMemsetAsync(l0.sol, 0, sizeof(double) * l0.n, stream));
-      learned_initial_guess(st);
-      zero = false;
+    bool judge = false;
+    if (o.use_learned && learned.active() && bnorm > 0.0) {
+      // with flexible CG the guess is judged after its first preconditioned
+      // step (solve_fcg); the stationary loops judge the guess itself
+      // use_learned == 2 keeps the guess unjudged (ablation measurements)
+      if (learned_initial_guess(st, bnorm, o.accel == 1 || o.use_learned == 2)) {
+        zero = false;
+        judge = o.accel == 1 && o.use_learned != 2;
+      } else {  // fall back to plain multigrid from p = 0
+        SCIHPC_HIP(hipMemsetAsync(l0.sol, 0, sizeof(double) * l0.n, stream));
+        zero = true;
+      }
     }
     if (o.accel == 1) {
-      solve_fcg(o, st, zero, bnorm, scale);
+      solve_fcg(o, st, zero, bnorm, scale, judge);
       cycles = st.cycles;
       rr = last_rr_;
     } else if (o.precision == SCIHPC_PREC_FP64) {
@@ -1860,19 +1972,78 @@ struct scihpc_mg {
     return rr <= o.rel_tol ? SCIHPC_OK : SCIHPC_NOT_CONVERGED;
   }
 
-  // learned initial guess: p = N(b, rho), kept only if it lowers the FP64
-  // residual below ||b|| (the zero guess); otherwise p = 0.
-  void learned_initial_guess(scihpc_solve_stats &st) {
+  // learned features of the current b (and sol when tgt) on the hook's level
+  void learned_features(int lvl, double sigma, float *feat, float *tgt) {
+    auto &a = d[lvl];
+    const int bs = a.deg;
+    k_learned_features<<<ceil_div((long)a.nx * a.ny, kBlock), kBlock, 0, stream>>>(
+        d[0].rhs, variable ? rho0 : nullptr, tgt ? d[0].sol : nullptr, d[0].nx, bs, a.nx, a.ny,
+        sigma > 0.0 ? 1.0 / sigma : 0.0, feat, tgt);
+    SCIHPC_CHECK_LAUNCH();
+  }
+
+  // learned initial guess: p0 = alpha * S P S N(b, ln rho) where N runs on
+  // the hook's level, S is two red-black sweeps on a level (against the
+  // child-averaged b), P prolongs one level up, and alpha = (p.b)/(p.A p) is
+  // the energy-optimal scale.  Stationary solves keep it only if
+  // ||b - A p0|| < ||b|| in FP64; flexible CG (provisional) keeps it if alpha
+  // is finite and judges it after its first step.  Otherwise the solve
+  // starts from p = 0 (plain multigrid).
+  bool learned_initial_guess(scihpc_solve_stats &st, double bnorm, bool provisional) {
     auto &l0 = d[0];
-    const double b2 = vec_norm2(l0.rhs, l0.n);
-    learned.apply(l0.rhs, l0.sol, stream);
-    const double r = residual_norm(0, false);
-    if (std::isfinite(r) && r * r < b2) {
-      st.learned_accepted++;
-    } else {
-      SCIHPC_HIP(hipMemsetAsync(l0.sol, 0, sizeof(double) * l0.n, stream));
-      st.learned_rejected++;
+    const int lvl = learned.level;
+    auto &a = d[lvl];
+    const long hw = (long)a.nx * a.ny;
+    if (!lfeat) {
+      lfeat = dalloc<float>(2 * hw, stream);
+      lout = dalloc<float>(hw, stream);
     }
+    const double sigma = bnorm / std::sqrt((double)l0.n);
+    learned_features(lvl, sigma, lfeat, nullptr);
+    unet_infer(learned.net, lfeat, lout, stream);
+    // b on the levels down to the network's (child averages of b, sol = 0)
+    SCIHPC_HIP(hipMemsetAsync(l0.sol, 0, sizeof(double) * l0.n, stream));
+    for (int l = 0; l < lvl; ++l) residual_restrict(d[l], d[l + 1]);
+    k_net_to_level<<<ceil_div(hw, kBlock), kBlock, 0, stream>>>(lout, a.sol, hw,
+                                                                 sigma * (double)unet_out_scale(learned.net));
+    SCIHPC_CHECK_LAUNCH();
+    // two red-black sweeps on every level from the network's up to level 0
+    // remove the network's pixel-scale noise and the interpolation's kinks
+    sweep(a, 0, 0);
+    sweep(a, 0, 0);
+    for (int l = lvl - 1; l >= 0; --l) {
+      SCIHPC_HIP(hipMemsetAsync(d[l].sol, 0, sizeof(double) * d[l].n, stream));
+      prolong_add(d[l + 1], d[l]);
+      sweep(d[l], 0, 0);
+      sweep(d[l], 0, 0);
+    }
+    if (bc == SCIHPC_MG_NEUMANN) sub_mean(l0.sol, l0.n);
+    const unsigned nb = ceil_div(l0.n, kBlock);
+    if (ndim == 2) {
+      if (variable) k_guess_dots<2, true><<<nb, kBlock, 0, stream>>>(view(l0), l0.sol, partial, (int)nb);
+      else k_guess_dots<2, false><<<nb, kBlock, 0, stream>>>(view(l0), l0.sol, partial, (int)nb);
+    } else {
+      if (variable) k_guess_dots<3, true><<<nb, kBlock, 0, stream>>>(view(l0), l0.sol, partial, (int)nb);
+      else k_guess_dots<3, false><<<nb, kBlock, 0, stream>>>(view(l0), l0.sol, partial, (int)nb);
+    }
+    SCIHPC_CHECK_LAUNCH();
+    SCIHPC_HIP(hipMemcpyAsync(host_part, partial, sizeof(double) * 4 * nb, hipMemcpyDeviceToHost, stream));
+    SCIHPC_HIP(hipStreamSynchronize(stream));
+    double s[4] = {0, 0, 0, 0};
+    for (int k = 0; k < 4; ++k)
+      for (unsigned q = 0; q < nb; ++q) s[k] += host_part[k * nb + q];
+    const double b2 = bnorm * bnorm;
+    const double alpha = s[1] != 0.0 ? s[0] / s[1] : 0.0;
+    const double r2 = b2 - 2.0 * alpha * s[2] + alpha * alpha * s[3];
+    st.learned_residual = std::isfinite(r2) && b2 > 0.0 ? std::sqrt(std::fabs(r2) / b2) : -1.0;
+    if (std::isfinite(alpha) && std::isfinite(r2) && alpha != 0.0 && (provisional || r2 < b2)) {
+      k_scale<<<nb, kBlock, 0, stream>>>(l0.sol, l0.n, alpha);
+      SCIHPC_CHECK_LAUNCH();
+      if (!provisional) st.learned_accepted++;
+      return true;
+    }
+    st.learned_rejected++;
+    return false;
   }
 };
 
@@ -2023,6 +2194,9 @@ extern "C" int scihpc_mg_destroy(scihpc_mg *h) {
   fr(h->fq_);
   fr(h->scal);
   fr(h->p_alt);
+  fr(h->rho0);
+  fr(h->lfeat);
+  fr(h->lout);
   if (h->host_scal) (void)hipHostFree(h->host_scal);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
@@ -2060,15 +2234,9 @@ extern "C" int scihpc_mg_set_density(scihpc_mg *h, const double *rho) {
     SCIHPC_REQUIRE(rho, "rho is null");
     SCIHPC_REQUIRE(h->bc == SCIHPC_MG_NEUMANN, "A(rho) is defined with wall (Neumann) faces only");
     const long n = h->d[0].n;
-    double *drho = dalloc<double>(n, h->stream);
-    SCIHPC_HIP(hipMemcpyAsync(drho, rho, sizeof(double) * n, hipMemcpyHostToDevice, h->stream));
-    try {
-      h->set_faces_from_rho(drho);
-    } catch (...) {
-      (void)hipFree(drho);
-      throw;
-    }
-    SCIHPC_HIP(hipFree(drho));
+    if (!h->rho0) h->rho0 = dalloc<double>(n, h->stream);
+    SCIHPC_HIP(hipMemcpyAsync(h->rho0, rho, sizeof(double) * n, hipMemcpyHostToDevice, h->stream));
+    h->set_faces_from_rho(h->rho0);
   });
 }
 
@@ -2081,19 +2249,17 @@ extern "C" int scihpc_mg_set_density_padded(scihpc_mg *h, const double *rho_padd
     const int g = 3;
     const long np = (long)(l0.nx + 2 * g) * (l0.ny + 2 * g) * (h->ndim == 3 ? l0.nz + 2 * g : 1);
     double *dp = dalloc<double>(np, h->stream);
-    double *drho = dalloc<double>(l0.n, h->stream);
+    if (!h->rho0) h->rho0 = dalloc<double>(l0.n, h->stream);
     try {
       SCIHPC_HIP(hipMemcpyAsync(dp, rho_padded, sizeof(double) * np, hipMemcpyHostToDevice, h->stream));
-      k_padded_to_mg<<<ceil_div(l0.n, kBlock), kBlock, 0, h->stream>>>(dp, drho, l0.nx, l0.ny, l0.nz, h->ndim);
+      k_padded_to_mg<<<ceil_div(l0.n, kBlock), kBlock, 0, h->stream>>>(dp, h->rho0, l0.nx, l0.ny, l0.nz, h->ndim);
       SCIHPC_CHECK_LAUNCH();
-      h->set_faces_from_rho(drho);
+      h->set_faces_from_rho(h->rho0);
     } catch (...) {
       (void)hipFree(dp);
-      (void)hipFree(drho);
       throw;
     }
     SCIHPC_HIP(hipFree(dp));
-    SCIHPC_HIP(hipFree(drho));
   });
 }
 
@@ -2271,6 +2437,67 @@ extern "C" int scihpc_mg_solve(scihpc_mg *h, const scihpc_solve_opts *opts, scih
     if (stats) *stats = st;
   });
   return g != SCIHPC_OK ? g : rc;
+}
+
+extern "C" int scihpc_mg_attach_learned(scihpc_mg *h, scihpc_unet *net, int level) {
+  return guarded([&] {
+    check_level(h, 0);
+    if (!net) {
+      h->learned = scihpc::LearnedHook{};
+      return;
+    }
+    SCIHPC_REQUIRE(h->ndim == 2, "the learned guess is 2-D");
+    SCIHPC_REQUIRE(level >= 0 && level < h->L, "level out of range");
+    const auto &a = h->d[level];
+    SCIHPC_REQUIRE(scihpc::unet_width(net) == a.nx && scihpc::unet_height(net) == a.ny,
+                   "network size differs from the level's grid");
+    SCIHPC_REQUIRE(a.nx * a.deg == h->d[0].nx && a.ny * a.deg == h->d[0].ny, "level is not a block coarsening");
+    if (h->lfeat && (h->learned.level != level)) {
+      SCIHPC_HIP(hipStreamSynchronize(h->stream));
+      SCIHPC_HIP(hipFree(h->lfeat));
+      SCIHPC_HIP(hipFree(h->lout));
+      h->lfeat = h->lout = nullptr;
+    }
+    h->learned.net = net;
+    h->learned.level = level;
+  });
+}
+
+extern "C" int scihpc_mg_learned_features(scihpc_mg *h, int level, float *feat, float *target, float *faces) {
+  return guarded([&] {
+    check_level(h, level);
+    SCIHPC_REQUIRE(feat, "feat is null");
+    SCIHPC_REQUIRE(h->ndim == 2, "the learned guess is 2-D");
+    SCIHPC_REQUIRE(!faces || h->variable, "faces need the variable operator (set_density)");
+    const auto &a = h->d[level];
+    SCIHPC_REQUIRE(a.nx * a.deg == h->d[0].nx && a.ny * a.deg == h->d[0].ny, "level is not a block coarsening");
+    const long hw = (long)a.nx * a.ny;
+    const double bnorm = std::sqrt(h->vec_norm2(h->d[0].rhs, h->d[0].n));
+    const double sigma = bnorm / std::sqrt((double)h->d[0].n);
+    float *df = dalloc<float>(2 * hw, h->stream);
+    float *dt = target ? dalloc<float>(hw, h->stream) : nullptr;
+    float *dc = faces ? dalloc<float>(2 * hw, h->stream) : nullptr;
+    try {
+      h->learned_features(level, sigma, df, dt);
+      SCIHPC_HIP(hipMemcpyAsync(feat, df, sizeof(float) * 2 * hw, hipMemcpyDeviceToHost, h->stream));
+      if (target) SCIHPC_HIP(hipMemcpyAsync(target, dt, sizeof(float) * hw, hipMemcpyDeviceToHost, h->stream));
+      if (faces) {
+        k_learned_faces<<<ceil_div(hw, kBlock), kBlock, 0, h->stream>>>(a.cx, a.cy, a.nx, a.ny, a.dx * a.dx,
+                                                                          a.dy * a.dy, dc);
+        SCIHPC_CHECK_LAUNCH();
+        SCIHPC_HIP(hipMemcpyAsync(faces, dc, sizeof(float) * 2 * hw, hipMemcpyDeviceToHost, h->stream));
+      }
+      SCIHPC_HIP(hipStreamSynchronize(h->stream));
+    } catch (...) {
+      (void)hipFree(df);
+      if (dt) (void)hipFree(dt);
+      if (dc) (void)hipFree(dc);
+      throw;
+    }
+    SCIHPC_HIP(hipFree(df));
+    if (dt) SCIHPC_HIP(hipFree(dt));
+    if (dc) SCIHPC_HIP(hipFree(dc));
+  });
 }
 
 extern "C" int scihpc_mg_time_kernel(scihpc_mg *h, int which, int iters, double *ms, double *bytes) {

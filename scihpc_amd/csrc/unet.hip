@@ -1,5 +1,801 @@
-// placeholder definition until unet.hip lands
+// unet.hip -- the learned pressure guess: a 4-level U-Net (16/32/64/128
+// channels, two 3x3 convolutions + ReLU per level, 2x2 average pooling,
+// nearest upsampling with skip concatenation, 1x1 output head), its
+// forward and training kernels (backward data, backward weight, loss, Adam),
+// and the C ABI.  No SciHPC counterpart: the reference has no learned part.
+//
+// Storage: NHWC BF16 activations (pixel index = y*W + x, identical to the
+// multigrid's x-fastest level layout), FP32 master parameters with BF16
+// copies for the matrix cores, FP32 Adam moments.  Every layer of one
+// parameter set is [w: cout x 9 x cin][b: cout] at its offset.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <random>
+#include <vector>
+
+#include "common.hpp"
 #include "learned.hpp"
+#include "scihpc_hip.h"
+#include "unet_kernels.hpp"
+
 namespace scihpc {
-void LearnedHook::apply(const double *, double *, hipStream_t) {}
+namespace {
+
+constexpr int kEB = 256;
+constexpr int kLayers = 14;
+constexpr int kHeadC = 16;  // channels entering the 1x1 head
+
+// ------------------------------------------------------- elementwise
+// dst (coarse h x w) = 2x2 average of src (2h x 2w); thread per (pixel, channel)
+__global__ void __launch_bounds__(kEB) k_pool2(View src, View dst, int n, int h, int w, int c) {
+  const long e = (long)blockIdx.x * kEB + threadIdx.x;
+  if (e >= (long)n * h * w * c) return;
+  const int ch = (int)(e % c);
+  const long p = e / c;
+  const int x = (int)(p % w), y = (int)((p / w) % h), img = (int)(p / ((long)w * h));
+  const long f = ((long)img * 2 * h + 2 * y) * (2 * w) + 2 * x;
+  const long W2 = 2 * w;
+  const float s = bf16_to_f(src.p[f * src.stride + src.off + ch]) +
+                  bf16_to_f(src.p[(f + 1) * src.stride + src.off + ch]) +
+                  bf16_to_f(src.p[(f + W2) * src.stride + src.off + ch]) +
+                  bf16_to_f(src.p[(f + W2 + 1) * src.stride + src.off + ch]);
+  dst.p[p * dst.stride + dst.off + ch] = f_to_bf16(0.25f * s);
+}
+
+// gf (fine, 2h x 2w) += gc[parent] / 4: gradient of k_pool2 added to the
+// gradient that the same tensor already received through its skip
+__global__ void __launch_bounds__(kEB) k_pool2_bwd_add(View gf, View gc, int n, int h, int w, int c) {
+  const long e = (long)blockIdx.x * kEB + threadIdx.x;
+  const long W2 = 2 * w, H2 = 2 * h;
+  if (e >= (long)n * H2 * W2 * c) return;
+  const int ch = (int)(e % c);
+  const long p = e / c;
+  const int x = (int)(p % W2), y = (int)((p / W2) % H2), img = (int)(p / (W2 * H2));
+  const long pc = ((long)img * h + y / 2) * w + x / 2;
+  const float v = bf16_to_f(gf.p[p * gf.stride + gf.off + ch]) + 0.25f * bf16_to_f(gc.p[pc * gc.stride + gc.off + ch]);
+  gf.p[p * gf.stride + gf.off + ch] = f_to_bf16(v);
+}
+
+// dst (fine 2h x 2w) = src[parent] (nearest upsampling)
+__global__ void __launch_bounds__(kEB) k_up2(View src, View dst, int n, int h, int w, int c) {
+  const long e = (long)blockIdx.x * kEB + threadIdx.x;
+  const long W2 = 2 * w, H2 = 2 * h;
+  if (e >= (long)n * H2 * W2 * c) return;
+  const int ch = (int)(e % c);
+  const long p = e / c;
+  const int x = (int)(p % W2), y = (int)((p / W2) % H2), img = (int)(p / (W2 * H2));
+  const long pc = ((long)img * h + y / 2) * w + x / 2;
+  dst.p[p * dst.stride + dst.off + ch] = src.p[pc * src.stride + src.off + ch];
+}
+
+// gc (coarse) = sum of the 4 children of gf: gradient of k_up2
+__global__ void __launch_bounds__(kEB) k_up2_bwd(View gf, View gc, int n, int h, int w, int c) {
+  const long e = (long)blockIdx.x * kEB + threadIdx.x;
+  if (e >= (long)n * h * w * c) return;
+  const int ch = (int)(e % c);
+  const long p = e / c;
+  const int x = (int)(p % w), y = (int)((p / w) % h), img = (int)(p / ((long)w * h));
+  const long f = ((long)img * 2 * h + 2 * y) * (2 * w) + 2 * x;
+  const long W2 = 2 * w;
+  const float s = bf16_to_f(gf.p[f * gf.stride + gf.off + ch]) + bf16_to_f(gf.p[(f + 1) * gf.stride + gf.off + ch]) +
+                  bf16_to_f(gf.p[(f + W2) * gf.stride + gf.off + ch]) +
+                  bf16_to_f(gf.p[(f + W2 + 1) * gf.stride + gf.off + ch]);
+  gc.p[p * gc.stride + gc.off + ch] = f_to_bf16(s);
+}
+
+// out[p] = b + sum_c x[p][c] w[c] (FP32 head weights)
+__global__ void __launch_bounds__(kEB) k_head_fwd(View x, const float *__restrict__ wb, float *__restrict__ out,
+                                                  long npix) {
+  const long p = (long)blockIdx.x * kEB + threadIdx.x;
+  if (p >= npix) return;
+  float s = wb[kHeadC];
+#pragma unroll
+  for (int c = 0; c < kHeadC; ++c) s += bf16_to_f(x.p[p * x.stride + x.off + c]) * wb[c];
+  out[p] = s;
+}
+
+// mean-squared error: dout = 2 (out - tgt) / npix, part[block] = sum (out - tgt)^2 / npix
+__global__ void __launch_bounds__(kEB) k_mse(const float *__restrict__ out, const float *__restrict__ tgt,
+                                             float *__restrict__ dout, long npix, double *__restrict__ part) {
+  const long p = (long)blockIdx.x * kEB + threadIdx.x;
+  double v = 0.0;
+  if (p < npix) {
+    const float d = out[p] - tgt[p];
+    dout[p] = 2.0f * d / (float)npix;
+    v = (double)d * (double)d / (double)npix;
+  }
+  v = block_sum(v);
+  if (threadIdx.x == 0) part[blockIdx.x] = v;
+}
+
+// energy loss of one image: E(e) / E(t) with e = out - tgt and
+// E(v) = sum over faces c_f (v_i - v_j)^2 (the discrete A(rho) energy on the
+// network's grid; face[p] = (c to x+1, c to y+1), zero on walls);
+// dout = 2 sum_f c_f (e_p - e_nb) / E(t) / n
+__global__ void __launch_bounds__(kEB) k_energy_loss(const float *__restrict__ out, const float *__restrict__ tgt,
+                                                     const float *__restrict__ face, const float *__restrict__ et,
+                                                     float *__restrict__ dout, int n, int h, int w,
+                                                     double *__restrict__ part) {
+  const long p = (long)blockIdx.x * kEB + threadIdx.x;
+  double v = 0.0;
+  if (p < (long)n * h * w) {
+    const int x = (int)(p % w), y = (int)((p / w) % h), img = (int)(p / ((long)w * h));
+    const float e = out[p] - tgt[p];
+    const float inv = 1.f / (et[img] * (float)n);
+    float g = 0.f;
+    const float cR = face[2 * p], cU = face[2 * p + 1];
+    if (x + 1 < w) {
+      const float d = e - (out[p + 1] - tgt[p + 1]);
+      g += cR * d;
+      v += (double)cR * d * d * inv;
+    }
+    if (y + 1 < h) {
+      const float d = e - (out[p + w] - tgt[p + w]);
+      g += cU * d;
+      v += (double)cU * d * d * inv;
+    }
+    if (x > 0) g += face[2 * (p - 1)] * (e - (out[p - 1] - tgt[p - 1]));
+    if (y > 0) g += face[2 * (p - w) + 1] * (e - (out[p - w] - tgt[p - w]));
+    dout[p] = 2.f * g * inv;
+  }
+  v = block_sum(v);
+  if (threadIdx.x == 0) part[blockIdx.x] = v;
+}
+
+// head backward: gx[p][c] = dout[p] w[c] (BF16); part[block][c] = sum_p dout x[p][c], part[block][16] = sum dout
+__global__ void __launch_bounds__(kEB) k_head_bwd(const float *__restrict__ dout, View x, const float *__restrict__ wb,
+                                                  View gx, float *__restrict__ part, long npix) {
+  __shared__ float red[kEB / 64][kHeadC + 1];
+  const long p = (long)blockIdx.x * kEB + threadIdx.x;
+  const float g = p < npix ? dout[p] : 0.f;
+  float v[kHeadC + 1];
+#pragma unroll
+  for (int c = 0; c < kHeadC; ++c) {
+    float xv = 0.f;
+    if (p < npix) {
+      xv = bf16_to_f(x.p[p * x.stride + x.off + c]);
+      gx.p[p * gx.stride + gx.off + c] = f_to_bf16(g * wb[c]);
+    }
+    v[c] = g * xv;
+  }
+  v[kHeadC] = g;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int c = 0; c <= kHeadC; ++c) {
+    const float s = wave_sum(v[c]);
+    if (lane == 0) red[wave][c] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x <= kHeadC) {
+    float s = 0.f;
+    for (int wv = 0; wv < kEB / 64; ++wv) s += red[wv][threadIdx.x];
+    part[(long)blockIdx.x * (kHeadC + 1) + threadIdx.x] = s;
+  }
+}
+
+// dst[c] = sum_r part[r][c] in row order (deterministic)
+__global__ void __launch_bounds__(kEB) k_sum_rows(const float *__restrict__ part, int rows, long cols,
+                                                  float *__restrict__ dst) {
+  const long c = (long)blockIdx.x * kEB + threadIdx.x;
+  if (c >= cols) return;
+  float s = 0.f;
+  for (int r = 0; r < rows; ++r) s += part[(long)r * cols + c];
+  dst[c] = s;
+}
+
+__global__ void __launch_bounds__(kEB) k_adam(float *__restrict__ P, const float *__restrict__ G, float *__restrict__ M,
+                                              float *__restrict__ V, bf16_t *__restrict__ Pb, long n, float lr,
+                                              float b1, float b2, float eps, float c1, float c2) {
+  const long i = (long)blockIdx.x * kEB + threadIdx.x;
+  if (i >= n) return;
+  const float g = G[i];
+  const float m = b1 * M[i] + (1.f - b1) * g;
+  const float v = b2 * V[i] + (1.f - b2) * g * g;
+  M[i] = m;
+  V[i] = v;
+  const float p = P[i] - lr * (m / c1) / (sqrtf(v / c2) + eps);
+  P[i] = p;
+  Pb[i] = f_to_bf16(p);
+}
+
+__global__ void __launch_bounds__(kEB) k_to_bf16(const float *__restrict__ a, bf16_t *__restrict__ b, long n) {
+  const long i = (long)blockIdx.x * kEB + threadIdx.x;
+  if (i < n) b[i] = f_to_bf16(a[i]);
+}
+
+// Wt[ci][8 - t][co] = W[co][t][ci]: the backward-data convolution is the
+// forward kernel on the gradient with these weights
+__global__ void __launch_bounds__(kEB) k_flip(const bf16_t *__restrict__ W, bf16_t *__restrict__ Wt, int cin,
+                                              int cout) {
+  const long e = (long)blockIdx.x * kEB + threadIdx.x;
+  if (e >= (long)cout * 9 * cin) return;
+  const int ci = (int)(e % cin);
+  const int t = (int)((e / cin) % 9);
+  const int co = (int)(e / ((long)cin * 9));
+  Wt[((long)ci * 9 + (8 - t)) * cout + co] = W[e];
+}
+
+// network input: channels (feat0, feat1, 0 ... 0) from FP32 features
+// [sample][pixel][2]; optional sample indices and x/y flips per batch entry;
+// optional targets tgt = store_tgt[sample][pixel] * inv_scale and faces of the
+// mirrored grid (a face between cells x and x+1 maps to W-2-x under an x
+// mirror; the wall face stays zero) with the per-image energy norm
+__global__ void __launch_bounds__(kEB) k_make_input(const float *__restrict__ feat, const float *__restrict__ tgt_store,
+                                                    const float *__restrict__ face_store,
+                                                    const float *__restrict__ e_store, const int *__restrict__ idx,
+                                                    bf16_t *__restrict__ in, float *__restrict__ tgt,
+                                                    float *__restrict__ face, float *__restrict__ et, int n, int h,
+                                                    int w, float inv_scale) {
+  const long p = (long)blockIdx.x * kEB + threadIdx.x;
+  if (p >= (long)n * h * w) return;
+  const int x = (int)(p % w), y = (int)((p / w) % h), img = (int)(p / ((long)w * h));
+  int s = img, fl = 0;
+  if (idx) {
+    s = idx[2 * img];
+    fl = idx[2 * img + 1];
+  }
+  const bool fx = fl & 1, fy = fl & 2;
+  const int sx = fx ? w - 1 - x : x, sy = fy ? h - 1 - y : y;
+  const long base = (long)s * h * w;
+  const long q = base + (long)sy * w + sx;
+  bf16_t *o = in + p * 16;
+  o[0] = f_to_bf16(feat[2 * q]);
+  o[1] = f_to_bf16(feat[2 * q + 1]);
+#pragma unroll
+  for (int c = 2; c < 16; ++c) o[c] = 0;
+  if (tgt) tgt[p] = tgt_store[q] * inv_scale;
+  if (face) {
+    float cx = 0.f, cy = 0.f;
+    // x face to the right of (x, y) in the mirrored image
+    if (x + 1 < w) {
+      const int fxs = fx ? w - 2 - x : x;
+      cx = face_store[2 * (base + (long)sy * w + fxs)];
+    }
+    if (y + 1 < h) {
+      const int fys = fy ? h - 2 - y : y;
+      cy = face_store[2 * (base + (long)fys * w + sx) + 1];
+    }
+    face[2 * p] = cx;
+    face[2 * p + 1] = cy;
+    if (x == 0 && y == 0) et[img] = e_store[s];
+  }
+}
+
+inline unsigned nblk(long n) { return ceil_div(n, kEB); }
+
+struct Layer {
+  int cin, cout, lev;
+  long off;  // w at off, b at off + cout*9*cin
+  long nw() const { return (long)cout * 9 * cin; }
+  long size() const { return nw() + cout; }
+};
+
+}  // namespace
+}  // namespace scihpc
+
+using namespace scihpc;
+
+struct scihpc_unet {
+  int H = 0, W = 0, nmax = 0;
+  hipStream_t stream = nullptr;
+  Layer L[kLayers];
+  long head_off = 0, np = 0;
+  float *P = nullptr, *G = nullptr, *M = nullptr, *V = nullptr;
+  bf16_t *Pb = nullptr, *Pt = nullptr;
+  long step = 0;
+  float out_scale = 1.f;
+  // activations (BF16, NHWC) and their gradients
+  bf16_t *in0, *t0a, *cat0, *p1, *t1a, *cat1, *p2, *t2a, *cat2, *p3, *t3a, *t3b, *t2c, *t2d, *t1c, *t1d, *t0c, *t0d;
+  bf16_t *g_t0a, *g_cat0, *g_p1, *g_t1a, *g_cat1, *g_p2, *g_t2a, *g_cat2, *g_p3, *g_t3a, *g_t3b, *g_t2c, *g_t2d,
+      *g_t1c, *g_t1d, *g_t0c, *g_t0d;
+  float *out = nullptr, *tgt = nullptr, *dout = nullptr, *part = nullptr;
+  long part_n = 0;
+  double *lpart = nullptr, *host_l = nullptr;
+  int *d_idx = nullptr;
+  float *store_feat = nullptr, *store_tgt = nullptr, *store_face = nullptr, *store_e = nullptr;
+  int store_n = 0;
+  bool energy = false;  // loss: energy norm of the error (faces given) or mean squared error
+  float *bface = nullptr, *be = nullptr;  // faces and target energy of the current batch
+  int n_fwd = 0;  // batch of the last forward pass
+  std::vector<void *> allocs;
+
+  long pix(int lev, int n) const { return (long)n * (H >> lev) * (W >> lev); }
+
+  template <typename T>
+  T *alloc(long n) {
+    void *p = nullptr;
+    SCIHPC_HIP(hipMalloc(&p, sizeof(T) * (size_t)(n > 0 ? n : 1)));
+    allocs.push_back(p);
+    SCIHPC_HIP(hipMemsetAsync(p, 0, sizeof(T) * (size_t)(n > 0 ? n : 1), stream));
+    return static_cast<T *>(p);
+  }
+  bf16_t *act(int lev, int c) { return alloc<bf16_t>(pix(lev, nmax) * c); }
+
+  void create(int h, int w, int n, unsigned long long seed) {
+    SCIHPC_REQUIRE(h >= 8 && w >= 8 && h % 8 == 0 && w % 8 == 0, "U-Net size must be a multiple of 8, >= 8");
+    SCIHPC_REQUIRE(n >= 1 && (long)n * h * w <= (1L << 26), "batch x pixels must be in [1, 2^26]");
+    H = h;
+    W = w;
+    nmax = n;
+    SCIHPC_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    const int spec[kLayers][3] = {{16, 16, 0},  {16, 16, 0}, {16, 32, 1}, {32, 32, 1}, {32, 64, 2},
+                                  {64, 64, 2},  {64, 128, 3}, {128, 128, 3}, {192, 64, 2}, {64, 64, 2},
+                                  {96, 32, 1},  {32, 32, 1}, {48, 16, 0}, {16, 16, 0}};
+    long off = 0;
+    for (int i = 0; i < kLayers; ++i) {
+      L[i] = Layer{spec[i][0], spec[i][1], spec[i][2], off};
+      off += L[i].size();
+    }
+    head_off = off;
+    np = off + kHeadC + 1;
+    P = alloc<float>(np);
+    G = alloc<float>(np);
+    M = alloc<float>(np);
+    V = alloc<float>(np);
+    Pb = alloc<bf16_t>(np);
+    Pt = alloc<bf16_t>(np);
+    // activations
+    in0 = act(0, 16); t0a = act(0, 16); cat0 = act(0, 48); p1 = act(1, 16); t1a = act(1, 32); cat1 = act(1, 96);
+    p2 = act(2, 32); t2a = act(2, 64); cat2 = act(2, 192); p3 = act(3, 64); t3a = act(3, 128); t3b = act(3, 128);
+    t2c = act(2, 64); t2d = act(2, 64); t1c = act(1, 32); t1d = act(1, 32); t0c = act(0, 16); t0d = act(0, 16);
+    g_t0a = act(0, 16); g_cat0 = act(0, 48); g_p1 = act(1, 16); g_t1a = act(1, 32); g_cat1 = act(1, 96);
+    g_p2 = act(2, 32); g_t2a = act(2, 64); g_cat2 = act(2, 192); g_p3 = act(3, 64); g_t3a = act(3, 128);
+    g_t3b = act(3, 128); g_t2c = act(2, 64); g_t2d = act(2, 64); g_t1c = act(1, 32); g_t1d = act(1, 32);
+    g_t0c = act(0, 16); g_t0d = act(0, 16);
+    out = alloc<float>(pix(0, nmax));
+    tgt = alloc<float>(pix(0, nmax));
+    dout = alloc<float>(pix(0, nmax));
+    lpart = alloc<double>(nblk(pix(0, nmax)));
+    d_idx = alloc<int>(2 * nmax);
+    bface = alloc<float>(2 * pix(0, nmax));
+    be = alloc<float>(nmax);
+    part_n = (long)nblk(pix(0, nmax)) * (kHeadC + 1);
+    for (int i = 0; i < kLayers; ++i) {
+      int chunks, cpix;
+      wgrad_chunks(i, nmax, chunks, cpix);
+      part_n = std::max(part_n, (long)chunks * L[i].size());
+    }
+    part = alloc<float>(part_n);
+    SCIHPC_HIP(hipHostMalloc(reinterpret_cast<void **>(&host_l), sizeof(double) * nblk(pix(0, nmax)),
+                             hipHostMallocDefault));
+    init_params(seed);
+  }
+
+  void destroy() {
+    if (stream) hipStreamSynchronize(stream);
+    for (float *b : {store_feat, store_tgt, store_face, store_e})
+      if (b) hipFree(b);
+    store_feat = store_tgt = store_face = store_e = nullptr;
+    for (void *p : allocs) hipFree(p);
+    allocs.clear();
+    if (host_l) hipHostFree(host_l);
+    host_l = nullptr;
+    if (stream) hipStreamDestroy(stream);
+    stream = nullptr;
+  }
+
+  // He-normal 3x3 weights (fan-in 9 cin; the first layer's fan-in counts its
+  // two real input channels), zero biases, a small head
+  void init_params(unsigned long long seed) {
+    std::vector<float> h(np, 0.f);
+    std::mt19937_64 rng(seed);
+    std::normal_distribution<double> nd(0.0, 1.0);
+    for (int i = 0; i < kLayers; ++i) {
+      const int fan = 9 * (i == 0 ? 2 : L[i].cin);
+      const double sd = std::sqrt(2.0 / fan);
+      for (long e = 0; e < L[i].nw(); ++e) {
+        const int ci = (int)(e % L[i].cin);
+        h[L[i].off + e] = (i == 0 && ci >= 2) ? 0.f : (float)(sd * nd(rng));
+      }
+    }
+    for (int c = 0; c < kHeadC; ++c) h[head_off + c] = (float)(0.25 * nd(rng));
+    set_params(h.data());
+  }
+
+  void refresh_derived() {
+    k_to_bf16<<<nblk(np), kEB, 0, stream>>>(P, Pb, np);
+    for (int i = 0; i < kLayers; ++i)
+      k_flip<<<nblk(L[i].nw()), kEB, 0, stream>>>(Pb + L[i].off, Pt + L[i].off, L[i].cin, L[i].cout);
+    SCIHPC_CHECK_LAUNCH();
+  }
+  void set_params(const float *host) {
+    SCIHPC_HIP(hipMemcpyAsync(P, host, sizeof(float) * np, hipMemcpyHostToDevice, stream));
+    SCIHPC_HIP(hipMemsetAsync(M, 0, sizeof(float) * np, stream));
+    SCIHPC_HIP(hipMemsetAsync(V, 0, sizeof(float) * np, stream));
+    step = 0;
+    refresh_derived();
+    SCIHPC_HIP(hipStreamSynchronize(stream));
+  }
+
+  // ------------------------------------------------------- launches
+  ConvShape shape(int i, int n) const { return ConvShape{n, H >> L[i].lev, W >> L[i].lev, L[i].cin, L[i].cout}; }
+
+  void conv_fwd(int i, int n, View x, View y, hipStream_t st) {
+    const ConvShape s = shape(i, n);
+    const unsigned grid = (unsigned)((long)n * s.h * ((s.w + 63) / 64));
+    const View none{nullptr, 0, 0};
+    const bf16_t *w = Pb + L[i].off;
+    const float *b = P + L[i].off + L[i].nw();
+    switch (s.cout / 16) {
+      case 1: k_conv3x3<1><<<grid, kConvThreads, 0, st>>>(x, none, w, b, y, s, 1); break;
+      case 2: k_conv3x3<2><<<grid, kConvThreads, 0, st>>>(x, none, w, b, y, s, 1); break;
+      case 4: k_conv3x3<4><<<grid, kConvThreads, 0, st>>>(x, none, w, b, y, s, 1); break;
+      case 8: k_conv3x3<8><<<grid, kConvThreads, 0, st>>>(x, none, w, b, y, s, 1); break;
+      default: throw Error(SCIHPC_BAD_ARG, "unsupported output width");
+    }
+    SCIHPC_CHECK_LAUNCH();
+  }
+  // gx = conv^T(gy * (y > 0))
+  void conv_bwd_data(int i, int n, View gy, View y, View gx) {
+    ConvShape s = shape(i, n);
+    std::swap(s.cin, s.cout);
+    const unsigned grid = (unsigned)((long)n * s.h * ((s.w + 63) / 64));
+    const bf16_t *w = Pt + L[i].off;
+    switch (s.cout / 16) {
+      case 1: k_conv3x3<1><<<grid, kConvThreads, 0, stream>>>(gy, y, w, nullptr, gx, s, 0); break;
+      case 2: k_conv3x3<2><<<grid, kConvThreads, 0, stream>>>(gy, y, w, nullptr, gx, s, 0); break;
+      case 3: k_conv3x3<3><<<grid, kConvThreads, 0, stream>>>(gy, y, w, nullptr, gx, s, 0); break;
+      case 4: k_conv3x3<4><<<grid, kConvThreads, 0, stream>>>(gy, y, w, nullptr, gx, s, 0); break;
+      case 6: k_conv3x3<6><<<grid, kConvThreads, 0, stream>>>(gy, y, w, nullptr, gx, s, 0); break;
+      case 8: k_conv3x3<8><<<grid, kConvThreads, 0, stream>>>(gy, y, w, nullptr, gx, s, 0); break;
+      case 12: k_conv3x3<12><<<grid, kConvThreads, 0, stream>>>(gy, y, w, nullptr, gx, s, 0); break;
+      default: throw Error(SCIHPC_BAD_ARG, "unsupported input width");
+    }
+    SCIHPC_CHECK_LAUNCH();
+  }
+  void wgrad_chunks(int i, int n, int &chunks, int &cpix) const {
+    const long npix = pix(L[i].lev, n);
+    const int pairs = (L[i].cout / 16) * (L[i].cin / 16);
+    long c = std::max(1, std::min(64, (1024 + pairs - 1) / pairs));
+    c = std::min(c, std::max(1L, npix / 64));
+    long cp = (npix + c - 1) / c;
+    cp = (cp + 63) / 64 * 64;
+    chunks = (int)((npix + cp - 1) / cp);
+    cpix = (int)cp;
+  }
+  void conv_wgrad(int i, int n, View x, View gy, View y) {
+    const ConvShape s = shape(i, n);
+    int chunks, cpix;
+    wgrad_chunks(i, n, chunks, cpix);
+    const int pairs = (L[i].cout / 16) * (L[i].cin / 16);
+    k_conv3x3_wgrad<<<(unsigned)(pairs * chunks), kConvThreads, 0, stream>>>(x, gy, y, part, s, cpix);
+    k_sum_rows<<<nblk(L[i].size()), kEB, 0, stream>>>(part, chunks, L[i].size(), G + L[i].off);
+    SCIHPC_CHECK_LAUNCH();
+  }
+  void conv_back(int i, int n, View x, View y, View gy, View gx) {
+    conv_wgrad(i, n, x, gy, y);
+    if (gx.p) conv_bwd_data(i, n, gy, y, gx);
+  }
+  void pool(View src, View dst, int lev, int n, int c, hipStream_t st) {  // lev = coarse level
+    const int h = H >> lev, w = W >> lev;
+    k_pool2<<<nblk((long)n * h * w * c), kEB, 0, st>>>(src, dst, n, h, w, c);
+    SCIHPC_CHECK_LAUNCH();
+  }
+  void up(View src, View dst, int lev, int n, int c, hipStream_t st) {  // lev = coarse level
+    const int h = H >> lev, w = W >> lev;
+    k_up2<<<nblk((long)n * 4 * h * w * c), kEB, 0, st>>>(src, dst, n, h, w, c);
+    SCIHPC_CHECK_LAUNCH();
+  }
+
+  static View v(bf16_t *p, int stride, int off = 0) { return View{p, stride, off}; }
+
+  void forward(int n, hipStream_t st) {
+    SCIHPC_REQUIRE(n >= 1 && n <= nmax, "batch exceeds the U-Net's capacity");
+    conv_fwd(0, n, v(in0, 16), v(t0a, 16), st);
+    conv_fwd(1, n, v(t0a, 16), v(cat0, 48, 32), st);
+    pool(v(cat0, 48, 32), v(p1, 16), 1, n, 16, st);
+    conv_fwd(2, n, v(p1, 16), v(t1a, 32), st);
+    conv_fwd(3, n, v(t1a, 32), v(cat1, 96, 64), st);
+    pool(v(cat1, 96, 64), v(p2, 32), 2, n, 32, st);
+    conv_fwd(4, n, v(p2, 32), v(t2a, 64), st);
+    conv_fwd(5, n, v(t2a, 64), v(cat2, 192, 128), st);
+    pool(v(cat2, 192, 128), v(p3, 64), 3, n, 64, st);
+    conv_fwd(6, n, v(p3, 64), v(t3a, 128), st);
+    conv_fwd(7, n, v(t3a, 128), v(t3b, 128), st);
+    up(v(t3b, 128), v(cat2, 192, 0), 3, n, 128, st);
+    conv_fwd(8, n, v(cat2, 192), v(t2c, 64), st);
+    conv_fwd(9, n, v(t2c, 64), v(t2d, 64), st);
+    up(v(t2d, 64), v(cat1, 96, 0), 2, n, 64, st);
+    conv_fwd(10, n, v(cat1, 96), v(t1c, 32), st);
+    conv_fwd(11, n, v(t1c, 32), v(t1d, 32), st);
+    up(v(t1d, 32), v(cat0, 48, 0), 1, n, 32, st);
+    conv_fwd(12, n, v(cat0, 48), v(t0c, 16), st);
+    conv_fwd(13, n, v(t0c, 16), v(t0d, 16), st);
+    k_head_fwd<<<nblk(pix(0, n)), kEB, 0, st>>>(v(t0d, 16), P + head_off, out, pix(0, n));
+    SCIHPC_CHECK_LAUNCH();
+    n_fwd = n;
+  }
+
+  // loss and all parameter gradients of the last forward pass (into G)
+  double backward(bool want_loss) {
+    const int n = n_fwd;
+    SCIHPC_REQUIRE(n >= 1, "backward needs a forward pass with targets");
+    const long np0 = pix(0, n);
+    const unsigned nb0 = nblk(np0);
+    if (energy)
+      k_energy_loss<<<nb0, kEB, 0, stream>>>(out, tgt, bface, be, dout, n, H, W, lpart);
+    else
+      k_mse<<<nb0, kEB, 0, stream>>>(out, tgt, dout, np0, lpart);
+    k_head_bwd<<<nb0, kEB, 0, stream>>>(dout, v(t0d, 16), P + head_off, v(g_t0d, 16), part, np0);
+    k_sum_rows<<<1, kEB, 0, stream>>>(part, (int)nb0, kHeadC + 1, G + head_off);
+    SCIHPC_CHECK_LAUNCH();
+    const View none{nullptr, 0, 0};
+    conv_back(13, n, v(t0c, 16), v(t0d, 16), v(g_t0d, 16), v(g_t0c, 16));
+    conv_back(12, n, v(cat0, 48), v(t0c, 16), v(g_t0c, 16), v(g_cat0, 48));
+    k_up2_bwd<<<nblk(pix(1, n) * 32), kEB, 0, stream>>>(v(g_cat0, 48, 0), v(g_t1d, 32), n, H >> 1, W >> 1, 32);
+    conv_back(11, n, v(t1c, 32), v(t1d, 32), v(g_t1d, 32), v(g_t1c, 32));
+    conv_back(10, n, v(cat1, 96), v(t1c, 32), v(g_t1c, 32), v(g_cat1, 96));
+    k_up2_bwd<<<nblk(pix(2, n) * 64), kEB, 0, stream>>>(v(g_cat1, 96, 0), v(g_t2d, 64), n, H >> 2, W >> 2, 64);
+    conv_back(9, n, v(t2c, 64), v(t2d, 64), v(g_t2d, 64), v(g_t2c, 64));
+    conv_back(8, n, v(cat2, 192), v(t2c, 64), v(g_t2c, 64), v(g_cat2, 192));
+    k_up2_bwd<<<nblk(pix(3, n) * 128), kEB, 0, stream>>>(v(g_cat2, 192, 0), v(g_t3b, 128), n, H >> 3, W >> 3, 128);
+    conv_back(7, n, v(t3a, 128), v(t3b, 128), v(g_t3b, 128), v(g_t3a, 128));
+    conv_back(6, n, v(p3, 64), v(t3a, 128), v(g_t3a, 128), v(g_p3, 64));
+    k_pool2_bwd_add<<<nblk(pix(2, n) * 64), kEB, 0, stream>>>(v(g_cat2, 192, 128), v(g_p3, 64), n, H >> 3, W >> 3, 64);
+    conv_back(5, n, v(t2a, 64), v(cat2, 192, 128), v(g_cat2, 192, 128), v(g_t2a, 64));
+    conv_back(4, n, v(p2, 32), v(t2a, 64), v(g_t2a, 64), v(g_p2, 32));
+    k_pool2_bwd_add<<<nblk(pix(1, n) * 32), kEB, 0, stream>>>(v(g_cat1, 96, 64), v(g_p2, 32), n, H >> 2, W >> 2, 32);
+    conv_back(3, n, v(t1a, 32), v(cat1, 96, 64), v(g_cat1, 96, 64), v(g_t1a, 32));
+    conv_back(2, n, v(p1, 16), v(t1a, 32), v(g_t1a, 32), v(g_p1, 16));
+    k_pool2_bwd_add<<<nblk(pix(0, n) * 16), kEB, 0, stream>>>(v(g_cat0, 48, 32), v(g_p1, 16), n, H >> 1, W >> 1, 16);
+    conv_back(1, n, v(t0a, 16), v(cat0, 48, 32), v(g_cat0, 48, 32), v(g_t0a, 16));
+    conv_back(0, n, v(in0, 16), v(t0a, 16), v(g_t0a, 16), none);
+    SCIHPC_CHECK_LAUNCH();
+    if (!want_loss) return 0.0;
+    SCIHPC_HIP(hipMemcpyAsync(host_l, lpart, sizeof(double) * nb0, hipMemcpyDeviceToHost, stream));
+    SCIHPC_HIP(hipStreamSynchronize(stream));
+    double s = 0.0;
+    for (unsigned b = 0; b < nb0; ++b) s += host_l[b];
+    return s;
+  }
+
+  void adam(float lr) {
+    ++step;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.f - (float)std::pow((double)b1, (double)step);
+    const float c2 = 1.f - (float)std::pow((double)b2, (double)step);
+    k_adam<<<nblk(np), kEB, 0, stream>>>(P, G, M, V, Pb, np, lr, b1, b2, 1e-8f, c1, c2);
+    SCIHPC_CHECK_LAUNCH();
+    for (int i = 0; i < kLayers; ++i)
+      k_flip<<<nblk(L[i].nw()), kEB, 0, stream>>>(Pb + L[i].off, Pt + L[i].off, L[i].cin, L[i].cout);
+    SCIHPC_CHECK_LAUNCH();
+  }
+
+  // batch from the device sample store; idx = host [n][2] (sample, flip bits)
+  void load_batch(int n, const int *idx) {
+    SCIHPC_REQUIRE(store_feat && store_n > 0, "no training samples stored");
+    SCIHPC_REQUIRE(n >= 1 && n <= nmax, "batch exceeds the U-Net's capacity");
+    for (int b = 0; b < n; ++b)
+      SCIHPC_REQUIRE(idx[2 * b] >= 0 && idx[2 * b] < store_n && idx[2 * b + 1] >= 0 && idx[2 * b + 1] <= 3,
+                     "bad sample index or flip code");
+    SCIHPC_HIP(hipMemcpyAsync(d_idx, idx, sizeof(int) * 2 * n, hipMemcpyHostToDevice, stream));
+    k_make_input<<<nblk(pix(0, n)), kEB, 0, stream>>>(store_feat, store_tgt, energy ? store_face : nullptr,
+                                                       store_e, d_idx, in0, tgt, energy ? bface : nullptr, be, n,
+                                                       H, W, 1.f / out_scale);
+    SCIHPC_CHECK_LAUNCH();
+  }
+};
+
+// ------------------------------------------------ multigrid hook
+namespace scihpc {
+void unet_infer(scihpc_unet *net, const float *feat, float *out, hipStream_t stream) {
+  k_make_input<<<nblk(net->pix(0, 1)), kEB, 0, stream>>>(feat, nullptr, nullptr, nullptr, nullptr, net->in0,
+                                                          nullptr, nullptr, nullptr, 1, net->H, net->W, 1.f);
+  SCIHPC_CHECK_LAUNCH();
+  net->forward(1, stream);
+  SCIHPC_HIP(hipMemcpyAsync(out, net->out, sizeof(float) * net->pix(0, 1), hipMemcpyDeviceToDevice, stream));
+}
+int unet_height(const scihpc_unet *net) { return net->H; }
+int unet_width(const scihpc_unet *net) { return net->W; }
+float unet_out_scale(const scihpc_unet *net) { return net->out_scale; }
+}  // namespace scihpc
+
+// ================================================================ C ABI
+extern "C" int scihpc_unet_create(int height, int width, int max_batch, unsigned long long seed, scihpc_unet **out) {
+  return guarded([&] {
+    SCIHPC_REQUIRE(out, "null output handle");
+    *out = nullptr;
+    auto *h = new scihpc_unet();
+    try {
+      h->create(height, width, max_batch, seed);
+    } catch (...) {
+      h->destroy();
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+extern "C" int scihpc_unet_destroy(scihpc_unet *h) {
+  return guarded([&] {
+    if (!h) return;
+    h->destroy();
+    delete h;
+  });
+}
+
+extern "C" int scihpc_unet_num_params(scihpc_unet *h, long long *n) {
+  return guarded([&] {
+    SCIHPC_REQUIRE(h && n, "null argument");
+    *n = h->np;
+  });
+}
+
+extern "C" int scihpc_unet_layer(scihpc_unet *h, int layer, int *cin, int *cout, int *level, long long *offset) {
+  return guarded([&] {
+    SCIHPC_REQUIRE(h, "null handle");
+    SCIHPC_REQUIRE(layer >= 0 && layer <= kLayers, "layer out of range");
+    if (layer == kLayers) {  // the 1x1 head: 16 weights then 1 bias
+      if (cin) *cin = kHeadC;
+      if (cout) *cout = 1;
+      if (level) *level = 0;
+      if (offset) *offset = h->head_off;
+      return;
+    }
+    if (cin) *cin = h->L[layer].cin;
+    if (cout) *cout = h->L[layer].cout;
+    if (level) *level = h->L[layer].lev;
+    if (offset) *offset = h->L[layer].off;
+  });
+}
+
+extern "C" int scihpc_unet_get_params(scihpc_unet *h, float *host) {
+  return guarded([&] {
+    SCIHPC_REQUIRE(h && host, "null argument");
+    SCIHPC_HIP(hipMemcpyAsync(host, h->P, sizeof(float) * h->np, hipMemcpyDeviceToHost, h->stream));
+    SCIHPC_HIP(hipStreamSynchronize(h->stream));
+  });
+}
+
+extern "C" int scihpc_unet_set_params(scihpc_unet *h, const float *host) {
+  return guarded([&] {
+    SCIHPC_REQUIRE(h && host, "null argument");
+    h->set_params(host);
+  });
+}
+
+extern "C" int scihpc_unet_get_grads(scihpc_unet *h, float *host) {
+  return guarded([&] {
+    SCIHPC_REQUIRE(h && host, "null argument");
+    SCIHPC_HIP(hipMemcpyAsync(host, h->G, sizeof(float) * h->np, hipMemcpyDeviceToHost, h->stream));
+    SCIHPC_HIP(hipStreamSynchronize(h->stream));
+  });
+}
+
+extern "C" int scihpc_unet_set_out_scale(scihpc_unet *h, float s) {
+  return guarded([&] {
+    SCIHPC_REQUIRE(h && std::isfinite(s) && s > 0.f, "output scale must be finite and > 0");
+    h->out_scale = s;
+  });
+}
+
+extern "C" int scihpc_unet_get_out_scale(scihpc_unet *h, float *s) {
+  return guarded([&] {
+    SCIHPC_REQUIRE(h && s, "null argument");
+    *s = h->out_scale;
+  });
+}
+
+// features: host [n][H*W][2] FP32; targets: host [n][H*W]; faces: host
+// [n][H*W][2] (NULL: mean-squared-error loss, else energy-norm loss)
+extern "C" int scihpc_unet_set_samples(scihpc_unet *h, int n, const float *feat, const float *target,
+                                       const float *faces) {
+  return guarded([&] {
+    SCIHPC_REQUIRE(h && feat && target && n >= 1, "bad sample arguments");
+    const long hw = (long)h->H * h->W;
+    std::vector<float> e;
+    if (faces) {
+      // energy norm of each target (flip invariant), FP64 on the host
+      e.resize(n);
+      for (int s = 0; s < n; ++s) {
+        double acc = 0.0;
+        const float *t = target + (long)s * hw;
+        const float *f = faces + (long)s * hw * 2;
+        for (int y = 0; y < h->H; ++y)
+          for (int x = 0; x < h->W; ++x) {
+            const long p = (long)y * h->W + x;
+            if (x + 1 < h->W) acc += (double)f[2 * p] * ((double)t[p] - t[p + 1]) * ((double)t[p] - t[p + 1]);
+            if (y + 1 < h->H)
+              acc += (double)f[2 * p + 1] * ((double)t[p] - t[p + h->W]) * ((double)t[p] - t[p + h->W]);
+          }
+        SCIHPC_REQUIRE(acc > 0.0 && std::isfinite(acc), "a target has zero or non-finite energy");
+        e[s] = (float)(acc / ((double)h->out_scale * h->out_scale));
+      }
+    }
+    SCIHPC_HIP(hipStreamSynchronize(h->stream));
+    for (float **b : {&h->store_feat, &h->store_tgt, &h->store_face, &h->store_e})
+      if (*b) {
+        SCIHPC_HIP(hipFree(*b));
+        *b = nullptr;
+      }
+    h->store_n = 0;
+    SCIHPC_HIP(hipMalloc(reinterpret_cast<void **>(&h->store_feat), sizeof(float) * 2 * hw * n));
+    SCIHPC_HIP(hipMalloc(reinterpret_cast<void **>(&h->store_tgt), sizeof(float) * hw * n));
+    SCIHPC_HIP(hipMemcpyAsync(h->store_feat, feat, sizeof(float) * 2 * hw * n, hipMemcpyHostToDevice, h->stream));
+    SCIHPC_HIP(hipMemcpyAsync(h->store_tgt, target, sizeof(float) * hw * n, hipMemcpyHostToDevice, h->stream));
+    if (faces) {
+      SCIHPC_HIP(hipMalloc(reinterpret_cast<void **>(&h->store_face), sizeof(float) * 2 * hw * n));
+      SCIHPC_HIP(hipMalloc(reinterpret_cast<void **>(&h->store_e), sizeof(float) * n));
+      SCIHPC_HIP(hipMemcpyAsync(h->store_face, faces, sizeof(float) * 2 * hw * n, hipMemcpyHostToDevice, h->stream));
+      SCIHPC_HIP(hipMemcpyAsync(h->store_e, e.data(), sizeof(float) * n, hipMemcpyHostToDevice, h->stream));
+    }
+    h->energy = faces != nullptr;
+    h->store_n = n;
+    SCIHPC_HIP(hipStreamSynchronize(h->stream));
+  });
+}
+
+// forward on stored samples idx[n][2] = (sample, flip bits: 1 = x, 2 = y);
+// out (host [n][H*W], NULL to skip) receives the network output
+extern "C" int scihpc_unet_forward(scihpc_unet *h, int n, const int *idx, float *out) {
+  return guarded([&] {
+    SCIHPC_REQUIRE(h && idx, "null argument");
+    h->load_batch(n, idx);
+    h->forward(n, h->stream);
+    if (out) {
+      SCIHPC_HIP(hipMemcpyAsync(out, h->out, sizeof(float) * h->pix(0, n), hipMemcpyDeviceToHost, h->stream));
+    }
+    SCIHPC_HIP(hipStreamSynchronize(h->stream));
+  });
+}
+
+// loss (energy norm or mean squared error of the error against the stored
+// targets / out_scale) and gradients of the last forward pass
+extern "C" int scihpc_unet_backward(scihpc_unet *h, double *loss) {
+  return guarded([&] {
+    SCIHPC_REQUIRE(h, "null handle");
+    const double l = h->backward(true);
+    if (loss) *loss = l;
+  });
+}
+
+// one Adam step: forward + backward + update; loss (optional) is the batch
+// loss before the update
+extern "C" int scihpc_unet_train_step(scihpc_unet *h, int n, const int *idx, float lr, double *loss) {
+  return guarded([&] {
+    SCIHPC_REQUIRE(h && idx && std::isfinite(lr) && lr > 0.f, "bad training arguments");
+    h->load_batch(n, idx);
+    h->forward(n, h->stream);
+    const double l = h->backward(loss != nullptr);
+    h->adam(lr);
+    if (loss) *loss = l;
+  });
+}
+
+extern "C" int scihpc_unet_sync(scihpc_unet *h) {
+  return guarded([&] {
+    SCIHPC_REQUIRE(h, "null handle");
+    SCIHPC_HIP(hipStreamSynchronize(h->stream));
+  });
+}
+
+// ------------------------------------------------ single-kernel entry points
+// (device pointers; used by the kernel tests)
+extern "C" int scihpc_conv3x3_forward(const uint16_t *x, int n, int height, int width, int cin, const uint16_t *w,
+                                      const float *b, int cout, int relu, uint16_t *y) {
+  return guarded([&] {
+    SCIHPC_REQUIRE(x && w && y && n > 0 && height > 0 && width > 0, "bad convolution arguments");
+    SCIHPC_REQUIRE(cin % 16 == 0 && cin >= 16 && cin <= 256, "cin must be a multiple of 16 in [16, 256]");
+    const int nb = cout / 16;
+    SCIHPC_REQUIRE(cout % 16 == 0 && (nb == 1 || nb == 2 || nb == 3 || nb == 4 || nb == 6 || nb == 8 || nb == 12),
+                   "cout must be 16 x {1,2,3,4,6,8,12}");
+    const ConvShape s{n, height, width, cin, cout};
+    const View xv{const_cast<bf16_t *>(x), cin, 0}, yv{y, cout, 0}, none{nullptr, 0, 0};
+    const unsigned grid = (unsigned)((long)n * height * ((width + 63) / 64));
+    switch (nb) {
+      case 1: k_conv3x3<1><<<grid, kConvThreads>>>(xv, none, w, b, yv, s, relu); break;
+      case 2: k_conv3x3<2><<<grid, kConvThreads>>>(xv, none, w, b, yv, s, relu); break;
+      case 3: k_conv3x3<3><<<grid, kConvThreads>>>(xv, none, w, b, yv, s, relu); break;
+      case 4: k_conv3x3<4><<<grid, kConvThreads>>>(xv, none, w, b, yv, s, relu); break;
+      case 6: k_conv3x3<6><<<grid, kConvThreads>>>(xv, none, w, b, yv, s, relu); break;
+      case 8: k_conv3x3<8><<<grid, kConvThreads>>>(xv, none, w, b, yv, s, relu); break;
+      case 12: k_conv3x3<12><<<grid, kConvThreads>>>(xv, none, w, b, yv, s, relu); break;
+    }
+    SCIHPC_CHECK_LAUNCH();
+    SCIHPC_HIP(hipDeviceSynchronize());
+  });
 }

@@ -78,9 +78,7 @@ __global__ void __launch_bounds__(kConvThreads)
     for (int cc = 0; cc < chunks; ++cc) {
       bf16x4 a = {0, 0, 0, 0};
       if (in) {
-        const bf16_t *src = x.p + pix * x.stride + x.off + cc * 16 + kq;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) a[q] = (short)src[q];
+        a = *reinterpret_cast<const bf16x4 *>(x.p + pix * x.stride + x.off + cc * 16 + kq);
         if (mask.p) {
           const bf16_t *m = mask.p + pix * mask.stride + mask.off + cc * 16 + kq;
 #pragma unroll
@@ -90,10 +88,8 @@ __global__ void __launch_bounds__(kConvThreads)
       }
 #pragma unroll
       for (int nb = 0; nb < NB; ++nb) {
-        const bf16_t *ws = w + ((long)(nb * 16 + (lane & 15)) * 9 + t) * s.cin + cc * 16 + kq;
-        bf16x4 bb;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) bb[q] = (short)ws[q];
+        const bf16x4 bb =
+            *reinterpret_cast<const bf16x4 *>(w + ((long)(nb * 16 + (lane & 15)) * 9 + t) * s.cin + cc * 16 + kq);
         acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bb, acc[nb], 0, 0, 0);
       }
     }
@@ -115,49 +111,55 @@ __global__ void __launch_bounds__(kConvThreads)
   }
 }
 
-// dW[co][t][ci] += sum_p g[p][co] x[p + off(t)][ci] and db[co] += sum_p g[p][co]
-// (g read through the ReLU mask of the layer output).  A workgroup owns one
-// (16 output x 16 input channel) block and a run of pixels; each wavefront
-// accumulates the 9 tap tiles over its pixels and adds them to dW (FP32
-// atomics).  A = g^T (co x pixels), B = x (pixels x ci).
-constexpr int kWgradPixels = 1024;  // pixels per workgroup
-
+// Weight and bias gradient of the 3x3 convolution y = relu(conv(x, w) + b):
+//   dW[co][t][ci] = sum_p gz[p][co] x[p + off(t)][ci],  db[co] = sum_p gz[p][co]
+// with gz = g * (mask > 0) (mask = the layer's stored output).  A = gz^T
+// (output channels x pixels), B = x shifted by the tap (pixels x input
+// channels).  A workgroup owns one 16 x 16 channel block and one pixel chunk;
+// its four wavefronts take interleaved groups of 16 pixels and are summed
+// through LDS, and the workgroup writes its partial (no atomics) to
+//   part[chunk][co][t][ci], part[chunk][cout*9*cin + co]   (bias, cc == 0 only)
+// which k_sum_rows reduces over chunks in a fixed order.
 __global__ void __launch_bounds__(kConvThreads)
-    k_conv3x3_wgrad(View x, View g, View mask, float *__restrict__ dw, float *__restrict__ db, ConvShape s) {
+    k_conv3x3_wgrad(View x, View g, View mask, float *__restrict__ part, ConvShape s, int chunk_pix) {
+  __shared__ float red[4][36][64];
+  __shared__ float bred[4][16];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int cbs = s.cout / 16, ccs = s.cin / 16;
   int blk = blockIdx.x;
   const int cb = blk % cbs;
   blk /= cbs;
   const int cc = blk % ccs;
-  const long p_begin = (long)(blk / ccs) * kWgradPixels;
+  const int chunk = blk / ccs;
   const long npix = (long)s.n * s.h * s.w;
+  const long p_begin = (long)chunk * chunk_pix;
+  const long p_end = p_begin + chunk_pix < npix ? p_begin + chunk_pix : npix;
   floatx4 acc[9];
 #pragma unroll
   for (int t = 0; t < 9; ++t) acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
   float bsum = 0.f;
   const int co = cb * 16 + (lane & 15), ci = cc * 16 + (lane & 15);
   const int kq = 4 * (lane >> 4);
-  // each wavefront takes every 4th group of 16 pixels
-  for (long p0 = p_begin + wave * 16; p0 < p_begin + kWgradPixels && p0 < npix; p0 += 64) {
+  for (long p0 = p_begin + wave * 16; p0 < p_end; p0 += 64) {
     bf16x4 a;
     int pr[4], pc[4], pi[4];
     bool pv[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const long p = p0 + kq + q;
-      pv[q] = p < npix;
-      const long pp = pv[q] ? p : 0;
+      pv[q] = p < p_end;
+      const long pp = pv[q] ? p : p_begin;
       pc[q] = (int)(pp % s.w);
-      pr[q] = (int)((pp / s.w) % s.h);
-      pi[q] = (int)(pp / ((long)s.w * s.h));
+      const long rest = pp / s.w;
+      pr[q] = (int)(rest % s.h);
+      pi[q] = (int)(rest / s.h);
       float gv = 0.f;
       if (pv[q]) {
         gv = bf16_to_f(g.p[pp * g.stride + g.off + co]);
         if (mask.p && !(bf16_to_f(mask.p[pp * mask.stride + mask.off + co]) > 0.f)) gv = 0.f;
       }
-      a[q] = (short)f_to_bf16(gv);
-      if (cc == 0 && lane < 16) bsum += gv;  // rows of A owned by lanes 0..15 once
+      a[q] = (short)f_to_bf16(gv);  // exact: gv is a BF16 value or zero
+      bsum += gv;
     }
 #pragma unroll
     for (int t = 0; t < 9; ++t) {
@@ -173,21 +175,27 @@ __global__ void __launch_bounds__(kConvThreads)
       acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bb, acc[t], 0, 0, 0);
     }
   }
-  // C rows = output channels cb*16 + kq + j, column = input channel ci
 #pragma unroll
   for (int t = 0; t < 9; ++t)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int o = cb * 16 + kq + j;
-      atomicAdd(&dw[((long)o * 9 + t) * s.cin + ci], acc[t][j]);
-    }
-  // bias: lanes 0..15 hold column sums of their A row (= output channel co)
-  // only in the cc == 0 blocks; the other lanes added zeros
-  if (cc == 0) {
-    // lanes l and l+16, l+32, l+48 share co; lanes >= 16 never added, so a
-    // single atomic per lane < 16 suffices
-    if (lane < 16) atomicAdd(&db[co], bsum);
+    for (int j = 0; j < 4; ++j) red[wave][t * 4 + j][lane] = acc[t][j];
+  // lanes l, l^16, l^32, l^48 share an output channel
+  bsum += __shfl_xor(bsum, 16, 64);
+  bsum += __shfl_xor(bsum, 32, 64);
+  if (lane < 16) bred[wave][lane] = bsum;
+  __syncthreads();
+  const long row = (long)s.cout * 9 * s.cin + s.cout;
+  float *dst = part + (long)chunk * row;
+  for (int e = threadIdx.x; e < 36 * 64; e += kConvThreads) {
+    const int tj = e >> 6, l = e & 63;
+    const int t = tj >> 2, j = tj & 3;
+    const float v = red[0][tj][l] + red[1][tj][l] + red[2][tj][l] + red[3][tj][l];
+    const int o = cb * 16 + 4 * (l >> 4) + j, c = cc * 16 + (l & 15);
+    dst[((long)o * 9 + t) * s.cin + c] = v;
   }
+  if (cc == 0 && threadIdx.x < 16)
+    dst[(long)s.cout * 9 * s.cin + cb * 16 + threadIdx.x] =
+        bred[0][threadIdx.x] + bred[1][threadIdx.x] + bred[2][threadIdx.x] + bred[3][threadIdx.x];
 }
 
 }  // namespace scihpc

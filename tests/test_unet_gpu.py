@@ -1,0 +1,267 @@
+"""GPU parity of the learned pressure guess (U-Net kernels through the C ABI)
+against the CPU oracle (oracle/unet_oracle.py), and its use in the solve.
+
+Bars: the oracle rounds every BF16-stored tensor where the GPU does, so the
+two differ only by FP32 accumulation order, which flips an occasional BF16
+rounding: single convolution within 1 BF16 ulp (2^-7 relative) of the
+rounded oracle value; whole-network output within 1e-2 and every layer's
+gradient within 3e-2 relative L2.  The solve with a learned guess must
+reach the same FP64 relative residual as without it, and a guess that does
+not lower the FP64 residual must be rejected.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from oracle import unet_oracle as U
+from scihpc_amd import PREC_FP64, PREC_MIXED, Multigrid, UNet
+from scihpc_amd import _capi
+from scihpc_amd import learned as LRN
+from scihpc_amd import workloads as W
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / max(np.linalg.norm(np.ravel(b)), 1e-300))
+
+
+def _bf16_bits(a):
+    f = np.ascontiguousarray(a, np.float32)
+    u = f.view(np.uint32).astype(np.uint64)
+    u = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+    return u
+
+
+def _bits_to_f(u):
+    return (u.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+
+
+CONV_CASES = [(1, 8, 64, 16, 16), (2, 5, 40, 48, 16), (1, 7, 100, 32, 64), (1, 4, 16, 192, 64),
+              (1, 3, 130, 128, 128), (2, 6, 24, 16, 48), (1, 1, 9, 64, 96), (1, 4, 12, 16, 192)]
+
+
+@pytest.mark.parametrize("case", CONV_CASES, ids=lambda c: "x".join(map(str, c)))
+def test_conv3x3_forward(case):
+    torch = pytest.importorskip("torch")
+    n, h, w, cin, cout = case
+    rng = np.random.default_rng(sum(case))
+    xb = _bf16_bits(rng.standard_normal((n, h, w, cin)))
+    wb = _bf16_bits(rng.standard_normal((cout, 3, 3, cin)) / np.sqrt(9 * cin))
+    b = rng.standard_normal(cout).astype(np.float32)
+    dev = torch.device("cuda:0")
+    tx = torch.from_numpy(xb.view(np.int16)).to(dev)
+    tw = torch.from_numpy(wb.view(np.int16)).to(dev)
+    tb = torch.from_numpy(b).to(dev)
+    ty = torch.zeros((n, h, w, cout), dtype=torch.int16, device=dev)
+    L = _capi.lib()
+    for relu in (0, 1):
+        print(f"conv3x3 {case} relu={relu}", flush=True)
+        rc = L.scihpc_conv3x3_forward(tx.data_ptr(), n, h, w, cin, tw.data_ptr(), tb.data_ptr(), cout, relu,
+                                      ty.data_ptr())
+        _capi.check(rc)
+        got = _bits_to_f(ty.cpu().numpy().view(np.uint16))
+        ref = U.conv3x3(_bits_to_f(xb), _bits_to_f(wb), b.astype(np.float64))
+        if relu:
+            ref = np.maximum(ref, 0.0)
+        ref_b = U.bf16(ref)
+        tol = 2.0 ** -7 * np.abs(ref) + 1e-6 * np.abs(ref).max()
+        assert np.all(np.abs(got - ref_b) <= tol), np.abs(got - ref_b).max()
+
+
+def _net_problem(n=2, H=16, W=32, seed=0):
+    rng = np.random.default_rng(seed)
+    p = U.init_params(seed).astype(np.float32)
+    feat = rng.standard_normal((n, H * W, 2)).astype(np.float32)
+    tgt = rng.standard_normal((n, H * W)).astype(np.float32)
+    return p, feat, tgt
+
+
+@pytest.mark.parametrize("shape", [(2, 16, 32), (1, 24, 16), (3, 8, 72)], ids=lambda s: "x".join(map(str, s)))
+def test_unet_forward_backward(shape):
+    n, H, Wd = shape
+    p, feat, tgt = _net_problem(n, H, Wd, seed=H + Wd)
+    print(f"unet fwd/bwd {shape}", flush=True)
+    net = UNet(H, Wd, n)
+    net.set_params(p)
+    net.out_scale = 1.0
+    net.set_samples(feat, tgt)
+    out = net.forward(np.arange(n))
+    loss = net.backward()
+    g = net.grads()
+    inp = feat.reshape(n, H, Wd, 2).astype(np.float64)
+    l_ref, g_ref, out_ref = U.loss_and_grads(p.astype(np.float64), inp, tgt.reshape(n, H, Wd).astype(np.float64))
+    assert rel(out, out_ref) < 1e-2
+    assert abs(loss - l_ref) < 1e-2 * l_ref
+    offs, head, total = U.layout()
+    assert g.size == total
+    for i, ((cin, cout, _), o) in enumerate(zip(U.SPEC, offs)):
+        nw = cout * 9 * cin
+        sl = slice(o, o + nw + cout)
+        if i == 0:  # only the two real input channels carry gradient
+            wg = g[o:o + nw].reshape(cout, 9, cin)
+            assert np.all(wg[:, :, 2:] == 0.0)
+        assert rel(g[sl], g_ref[sl]) < 3e-2, (i, rel(g[sl], g_ref[sl]))
+    assert rel(g[head:], g_ref[head:]) < 3e-2
+    # BF16 storage changes the exact-arithmetic gradient only a little
+    _, g_exact, _ = U.loss_and_grads(p.astype(np.float64), inp, tgt.reshape(n, H, Wd).astype(np.float64),
+                                     emulate=False)
+    assert rel(g, g_exact) < 1e-1
+
+
+def test_unet_energy_loss_gradients():
+    n, H, Wd = 2, 16, 24
+    p, feat, tgt = _net_problem(n, H, Wd, seed=9)
+    rng = np.random.default_rng(10)
+    faces = rng.uniform(0.001, 1.0, (n, H, Wd, 2)).astype(np.float32)
+    faces[:, :, -1, 0] = 0.0
+    faces[:, -1, :, 1] = 0.0
+    net = UNet(H, Wd, n)
+    net.set_params(p)
+    net.out_scale = 1.0
+    net.set_samples(feat, tgt, faces.reshape(n, H * Wd, 2))
+    net.forward(np.arange(n))
+    loss = net.backward()
+    g = net.grads()
+    inp = feat.reshape(n, H, Wd, 2).astype(np.float64)
+    l_ref, g_ref, _ = U.loss_and_grads(p.astype(np.float64), inp, tgt.reshape(n, H, Wd).astype(np.float64),
+                                       faces=faces.astype(np.float64))
+    assert abs(loss - l_ref) < 1e-2 * l_ref
+    offs, head, _ = U.layout()
+    for i, ((cin, cout, _), o) in enumerate(zip(U.SPEC, offs)):
+        sl = slice(o, o + cout * 9 * cin + cout)
+        assert rel(g[sl], g_ref[sl]) < 3e-2, (i, rel(g[sl], g_ref[sl]))
+    assert rel(g[head:], g_ref[head:]) < 3e-2
+    # mirrored faces: an x (y) mirror maps the face between x and x+1 to W-2-x
+    # (H-2-y); the loss of a mirrored batch equals the oracle's on mirrored data
+    for fl, mirror, fmir in [
+        (1, lambda a: a[:, :, ::-1], lambda f: np.stack([np.concatenate([f[:, :, -2::-1, 0], f[:, :, -1:, 0]], 2),
+                                                         f[:, :, ::-1, 1]], -1)),
+        (2, lambda a: a[:, ::-1, :], lambda f: np.stack([f[:, ::-1, :, 0],
+                                                         np.concatenate([f[:, -2::-1, :, 1], f[:, -1:, :, 1]], 1)],
+                                                        -1))]:
+        net.forward([0], [fl])
+        lf = net.backward()
+        mi = np.ascontiguousarray(mirror(inp[:1]))
+        mt = np.ascontiguousarray(mirror(tgt.reshape(n, H, Wd)[:1].astype(np.float64)))
+        mf = np.ascontiguousarray(fmir(faces[:1].astype(np.float64)))
+        l_m, _, _ = U.loss_and_grads(p.astype(np.float64), mi, mt, faces=mf)
+        assert abs(lf - l_m) < 1e-2 * l_m, (fl, lf, l_m)
+
+
+def test_unet_input_flips():
+    n, H, Wd = 1, 16, 24
+    p, feat, tgt = _net_problem(n, H, Wd, seed=5)
+    net = UNet(H, Wd, 1)
+    net.set_params(p)
+    net.set_samples(feat, tgt)
+    inp = feat.reshape(1, H, Wd, 2).astype(np.float64)
+    for fl, mirror in [(1, lambda a: a[:, :, ::-1]), (2, lambda a: a[:, ::-1, :]), (3, lambda a: a[:, ::-1, ::-1])]:
+        out = net.forward([0], [fl]).reshape(1, H, Wd)
+        ref, _ = U.forward(p.astype(np.float64), np.ascontiguousarray(mirror(inp)))
+        assert rel(out, ref) < 1e-2, fl
+
+
+def test_unet_training_reduces_loss():
+    """Adam on the GPU fits a smooth target field: loss falls by >= 10x."""
+    H, Wd, n = 32, 32, 4
+    rng = np.random.default_rng(3)
+    y, x = np.meshgrid(np.linspace(0, 1, H), np.linspace(0, 1, Wd), indexing="ij")
+    feats, tgts = [], []
+    for k in range(n):
+        a, b = rng.uniform(0.5, 2.0, 2)
+        f0 = np.sin(np.pi * a * x) * np.cos(np.pi * b * y)
+        f1 = np.cos(np.pi * b * x)
+        feats.append(np.stack([f0, f1], -1).reshape(H * Wd, 2))
+        tgts.append((0.5 * f0 + 0.3 * f1 * f0).reshape(H * Wd))
+    feat = np.array(feats, np.float32)
+    tgt = np.array(tgts, np.float32)
+    net = UNet(H, Wd, 4, seed=2)
+    hist = LRN.train(net, feat, tgt, steps=300, batch=4, lr=2e-3, log_every=50)
+    assert hist[-1][1] < 0.1 * hist[0][1], hist
+
+
+def test_learned_guess_fallback_on_bad_network():
+    """A network whose guess does not lower the FP64 residual (NaN weights)
+    is rejected and the solve equals the plain solve."""
+    w = W.bubble_2d(64, 128, seed=11)
+    mg = Multigrid(2, 64, 128, 1, 1 / 64, 2 / 128, 1.0, "neumann")
+    mg.set_density(w["rho"])
+    mg.set_rhs(w["rhs"])
+    plain = mg.solve(rel_tol=1e-8, precision=PREC_MIXED, accel=1)
+    x_plain = mg.sol(0)
+    H, Wd = mg.shapes[1][1], mg.shapes[1][0]
+    net = UNet(H, Wd, 1)
+    p = net.params()
+    p[:] = np.nan
+    net.set_params(p)
+    mg.attach_learned(net, 1)
+    st = mg.solve(rel_tol=1e-8, precision=PREC_MIXED, accel=1, use_learned=True)
+    assert st["converged"] and st["learned_rejected"] == 1 and st["learned_accepted"] == 0
+    assert st["cycles"] == plain["cycles"]
+    assert rel(mg.sol(0), x_plain) < 1e-12
+    # a zero network (guess 0) is rejected as well: it cannot lower ||b||
+    net.set_params(np.zeros_like(p))
+    st = mg.solve(rel_tol=1e-8, precision=PREC_FP64, accel=0, use_learned=True)
+    assert st["converged"] and st["learned_rejected"] == 1
+    mg.attach_learned(None)
+    st = mg.solve(rel_tol=1e-8, precision=PREC_FP64, accel=0, use_learned=True)
+    assert st["learned_rejected"] == 0 and st["learned_accepted"] == 0
+
+
+def test_learned_features_match_block_means():
+    w = W.bubble_2d(64, 128, seed=12)
+    mg = Multigrid(2, 64, 128, 1, 1 / 64, 2 / 128, 1.0, "neumann")
+    mg.set_density(w["rho"])
+    mg.set_rhs(w["rhs"])
+    mg.solve(rel_tol=1e-10, precision=PREC_FP64, accel=1)
+    p = mg.sol(0)
+    for level in (1, 2):
+        f, t, c = mg.learned_features(level, with_target=True, with_faces=True)
+        bs = 2 ** level
+        b = w["rhs"].reshape(128, 64)
+        sigma = np.sqrt(np.mean(b ** 2))
+
+        def bm(a):
+            return a.reshape(128 // bs, bs, 64 // bs, bs).mean(axis=(1, 3)).ravel()
+
+        np.testing.assert_allclose(f[:, 0], bm(b) / sigma, rtol=1e-5, atol=1e-6)
+        np.testing.assert_allclose(f[:, 1], bm(np.log(w["rho"].reshape(128, 64))) / np.log(1000.0), rtol=1e-5,
+                                   atol=1e-6)
+        np.testing.assert_allclose(t, bm(p.reshape(128, 64)) / sigma, rtol=1e-5,
+                                   atol=1e-6 * np.abs(t).max())
+        # faces x h^2: level 0 is 1 / (arithmetic mean rho) (init_ppe_proj_coefficients,
+        # projection_method.cpp:404-465); coarse faces average the fine faces they cover
+        rho = w["rho"].reshape(128, 64)
+        fx0 = np.zeros((128, 64))
+        fy0 = np.zeros((128, 64))
+        fx0[:, :-1] = 1.0 / (0.5 * (rho[:, :-1] + rho[:, 1:]))
+        fy0[:-1, :] = 1.0 / (0.5 * (rho[:-1, :] + rho[1:, :]))
+        ex = fx0[:, bs - 1::bs].reshape(128 // bs, bs, 64 // bs).mean(axis=1)
+        ey = fy0[bs - 1::bs, :].reshape(128 // bs, 64 // bs, bs).mean(axis=2)
+        ex[:, -1] = 0.0
+        ey[-1, :] = 0.0
+        np.testing.assert_allclose(c[:, 0], ex.ravel(), rtol=1e-6, atol=1e-9)
+        np.testing.assert_allclose(c[:, 1], ey.ravel(), rtol=1e-6, atol=1e-9)
+
+
+def test_learned_guess_trained_gains():
+    """Ablation on held-out bubbles: the trained network's guess is accepted,
+    reaches the same 1e-8 FP64 residual, and needs fewer FCG iterations than
+    the same solver started from p = 0."""
+    import os
+    if not os.path.exists(LRN.DEFAULT_WEIGHTS):
+        pytest.skip("trained weights are produced by python -m scihpc_amd.learned")
+    net, meta = LRN.load_net()
+    mg = Multigrid(2, 512, 1024, 1, 1 / 512, 2 / 1024, 1.0, "neumann")
+    mg.attach_learned(net, meta["level"])
+    rows = LRN.held_out_gain(mg, LRN.bubble_configs(4, LRN.EVAL_SEED))
+    saved = 0
+    for r in rows:
+        print(r["plain"]["cycles"], r["learned"]["cycles"], r["learned"]["learned_residual"], flush=True)
+        assert r["plain"]["converged"] and r["learned"]["converged"]
+        assert r["learned"]["rel_residual"] <= 1e-8
+        assert r["learned"]["learned_accepted"] == 1
+        saved += r["plain"]["cycles"] - r["learned"]["cycles"]
+    assert saved >= len(rows)
