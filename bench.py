@@ -6,10 +6,14 @@ box, rho 1000:1 across a circle of radius 0.25 at (0.5, 0.5) (smoothed
 Heaviside, width 1.5h), b = divergence of a seeded smooth velocity field,
 all-Neumann walls.  Inputs are resident in HBM before the timed region.
 
-`value` is the headline path (FP32 V-cycle inside the FP64 defect-correction
-loop, learned initial guess when a trained model is attached); the plain FP64
-multigrid solve of the same problem is reported beside it.  The roofline
-object is the dominant kernel timed with HIP events on the solver stream;
+`value` is the path the metric names: the FP32 V-cycle inside the FP64
+flexible-CG defect correction, with the trained U-Net guess evaluated and
+judged by the FP64 residual on every solve (kept only if it lowers it).
+Beside it: plain FP64 multigrid (stationary V-cycles), FP64 + flexible CG,
+and the mixed path without the network.  The roofline object is the
+dominant kernel timed with HIP events on the solver stream; the U-Net's
+MFMA rate is reported from its own timing hook; the 256^3 variable-density
+3-D solve (1000:1 jump across a wavy plane) is reported as well; and
 cpu_baseline is the oracle's C port of the same FP64 multigrid on one host
 core, on a bounded number of solves.
 """
@@ -26,6 +30,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "ms/pressure solve to 1e-8 residual: FP64 MG vs mixed+learned; HBM GB/s"
 HBM_PEAK_GBS = 8000.0
+MFMA_BF16_PEAK = 2500.0  # TFLOP/s, dense
 
 
 def log(msg):
@@ -58,6 +63,77 @@ def cpu_baseline(w, seconds=12.0):
                       f"({cycles} V(2,2) cycles each) by oracle/oracle.c, 1 thread"}
 
 
+def kernel_only(args):
+    """20 launches of the FP32 level-0 pre-smoothing kernel of the bubble
+    solve (after one solve), then 5 copies of 256 MiB with torch: reading
+    and writing a known byte count calibrates the FETCH_SIZE / WRITE_SIZE
+    counters."""
+    import torch
+
+    from scihpc_amd import PREC_MIXED, Multigrid
+    from scihpc_amd import workloads as W
+
+    w = W.bubble_2d()
+    nx, ny, _ = w["shape"]
+    dx, dy, _ = w["spacing"]
+    g = Multigrid(2, nx, ny, 1, dx, dy, 1.0, "neumann")
+    g.set_density(w["rho"])
+    g.set_rhs(w["rhs"])
+    g.solve(rel_tol=1e-8, precision=PREC_MIXED, accel=1)
+    ms, b = g.time_kernel(1, iters=20)
+    a = torch.empty(64 * 2 ** 20, dtype=torch.float32, device="cuda")
+    c = torch.ones_like(a)
+    for _ in range(5):
+        a.copy_(c)
+    torch.cuda.synchronize()
+    print(json.dumps({"kernel": "k_pre2d<float,true,32,16>", "us": ms * 1e3, "bytes": b,
+                      "calibration_copy_bytes_each_way": a.numel() * 4}), flush=True)
+
+
+def bench_3d(args):
+    """256^3 variable-density solve (1000:1 across z = 0.5 + 0.05 sin sin),
+    mixed vs FP64, and the level-0 kernels with HIP events."""
+    import numpy as np
+    import torch
+
+    from scihpc_amd import PREC_FP64, PREC_MIXED, Multigrid
+    from scihpc_amd import workloads as W
+
+    n = 256
+    log("bench: building the 256^3 plane workload")
+    w = W.poisson_3d(n, variable=True)
+    g = Multigrid(3, n, n, n, 1 / n, 1 / n, 1 / n, "neumann")
+    g.set_density(w["rho"])
+    g.set_rhs(w["rhs"])
+    res = {"workload": "plane_3d 256^3 variable-density pressure Poisson, all-Neumann", "rel_tol": 1e-8}
+    xs = {}
+    for name, prec, acc in [("fp64_mg", PREC_FP64, 0), ("fp64_mg_fcg", PREC_FP64, 1), ("mixed", PREC_MIXED, 1)]:
+        g.solve(rel_tol=1e-8, max_cycles=100, precision=prec, accel=acc)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        reps = 3
+        for _ in range(reps):
+            st = g.solve(rel_tol=1e-8, max_cycles=100, precision=prec, accel=acc)
+        torch.cuda.synchronize()
+        ms = 1e3 * (time.perf_counter() - t0) / reps
+        assert st["converged"], st
+        xs[name] = g.sol(0)
+        res[name] = {"ms_per_solve": round(ms, 3), "cycles": st["cycles"], "rel_residual": st["rel_residual"]}
+        log(f"bench: 3-D {name} {ms:.2f} ms/solve, {st['cycles']} cycles")
+    res["mixed"]["rel_l2_vs_fp64"] = float(np.linalg.norm(xs["mixed"] - xs["fp64_mg"]) /
+                                           np.linalg.norm(xs["fp64_mg"]))
+    res["speedup_mixed_vs_fp64"] = round(res["fp64_mg"]["ms_per_solve"] / res["mixed"]["ms_per_solve"], 3)
+    kern = {}
+    for which, name in [(0, "fp64_pre_smooth_restrict"), (1, "fp32_pre_smooth_restrict"),
+                        (2, "fp32_post_smooth_fp64_outer"), (3, "fp64_post_smooth_norm")]:
+        ms_k, b_k = g.time_kernel(which, iters=5)
+        kern[name] = {"us": round(ms_k * 1e3, 1), "bytes": b_k, "GBps": round(b_k / (ms_k * 1e-3) / 1e9, 1),
+                      "frac": round(b_k / (ms_k * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    res["kernels"] = kern
+    g.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -65,6 +141,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-3d", action="store_true", help="skip the 256^3 3-D solves")
+    ap.add_argument("--no-unet", action="store_true", help="skip the U-Net training-step timing")
+    ap.add_argument("--kernel-only", action="store_true",
+                    help="only launch the dominant kernel (and a 256 MiB copy for calibration): "
+                         "the command profiled with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the multigrid path "
@@ -76,9 +157,13 @@ def main():
 
     assert torch.cuda.is_available(), "bench needs a GPU"
     torch.cuda.set_device(0)
-    from scihpc_amd import PREC_FP64, PREC_MIXED, Multigrid
+    from scihpc_amd import PREC_FP64, PREC_MIXED, Multigrid, UNet
+    from scihpc_amd import learned as LRN
     from scihpc_amd import workloads as W
 
+    if args.kernel_only:
+        kernel_only(args)
+        return
     log("bench: building the 512x1024 bubble workload")
     w = W.bubble_2d()
     nx, ny, _ = w["shape"]
@@ -88,14 +173,14 @@ def main():
     g.set_rhs(w["rhs"])
     tol = 1e-8
 
-    def run(precision, steps, warmup, accel=0):
+    def run(precision, steps, warmup, accel=0, learned=False):
         stats = None
         for _ in range(warmup):
-            stats = g.solve(rel_tol=tol, max_cycles=200, precision=precision, accel=accel)
+            stats = g.solve(rel_tol=tol, max_cycles=200, precision=precision, accel=accel, use_learned=learned)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(steps):
-            stats = g.solve(rel_tol=tol, max_cycles=200, precision=precision, accel=accel)
+            stats = g.solve(rel_tol=tol, max_cycles=200, precision=precision, accel=accel, use_learned=learned)
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t0) / steps
         assert stats["converged"], stats
@@ -114,6 +199,31 @@ def main():
     log(f"bench: mixed {ms_mx:.3f} ms/solve, {st_mx['cycles']} cycles")
     err = float(np.linalg.norm(xmx - x64) / np.linalg.norm(x64))
 
+    learned = {"attached": False}
+    ms_ml, st_ml = ms_mx, st_mx
+    if os.path.exists(LRN.DEFAULT_WEIGHTS):
+        net, meta = LRN.load_net()
+        g.attach_learned(net, meta["level"])
+        log("bench: mixed + learned guess (judged by the FP64 residual every solve)")
+        ms_ml, st_ml = run(PREC_MIXED, args.steps, args.warmup, accel=1, learned=True)
+        xml = g.sol(0)
+        _, st_forced = run(PREC_MIXED, 1, 0, accel=1, learned=2)
+        f, t, _ = g.learned_features(meta["level"], with_target=True)
+        net.set_samples(f[None], t[None])
+        ms_inf, fl_inf = net.time(1, train=False, iters=20)
+        learned = {"attached": True, "weights": os.path.relpath(LRN.DEFAULT_WEIGHTS, ROOT),
+                   "net_level": meta["level"], "net_grid": [meta["width"], meta["height"]],
+                   "training": meta["meta"], "ms_per_solve": round(ms_ml, 4), "cycles": st_ml["cycles"],
+                   "accepted": st_ml["learned_accepted"], "rejected": st_ml["learned_rejected"],
+                   "guess_rel_residual": st_ml["learned_residual"],
+                   "cycles_if_guess_forced": st_forced["cycles"],
+                   "rel_l2_vs_fp64": float(np.linalg.norm(xml - x64) / np.linalg.norm(x64)),
+                   "inference_us": round(ms_inf * 1e3, 2),
+                   "inference_mfma_tflops": round(fl_inf / (ms_inf * 1e-3) / 1e12, 3)}
+        log(f"bench: mixed+learned {ms_ml:.3f} ms/solve, {st_ml['cycles']} cycles, "
+            f"accepted {st_ml['learned_accepted']}, guess residual {st_ml['learned_residual']:.3e}")
+        g.attach_learned(None)
+
     # level-0 kernels, HIP events on the solver's stream
     kernels = {}
     for which, name in [(0, "fp64_pre_smooth_restrict"), (1, "fp32_pre_smooth_restrict"),
@@ -125,22 +235,40 @@ def main():
     kms, kbytes = kernels["fp32_pre_smooth_restrict"]["us"] * 1e-3, kernels["fp32_pre_smooth_restrict"]["bytes"]
     achieved = kbytes / (kms * 1e-3) / 1e9
 
+    # U-Net training throughput on the matrix cores (level-1 grid, batch 4)
+    unet = None
+    if not args.no_unet:
+        log("bench: U-Net training step (256x512, batch 4)")
+        nt = UNet(512, 256, 4, seed=3)
+        rng = np.random.default_rng(5)
+        nt.out_scale = 1.0
+        nt.set_samples(rng.standard_normal((4, 512 * 256, 2)).astype(np.float32),
+                       rng.standard_normal((4, 512 * 256)).astype(np.float32))
+        ms_tr, fl_tr = nt.time(4, train=True, iters=10)
+        ms_fw, fl_fw = nt.time(4, train=False, iters=10)
+        unet = {"grid": [256, 512], "batch": 4,
+                "train_step_ms": round(ms_tr, 3), "train_tflops": round(fl_tr / (ms_tr * 1e-3) / 1e12, 2),
+                "forward_ms": round(ms_fw, 3), "forward_tflops": round(fl_fw / (ms_fw * 1e-3) / 1e12, 2),
+                "mfma_peak_tflops": MFMA_BF16_PEAK,
+                "mfma_frac_train": round(fl_tr / (ms_tr * 1e-3) / 1e12 / MFMA_BF16_PEAK, 4)}
+        nt.close()
+
     out = {
         "metric": METRIC,
-        "value": round(ms_mx, 4),
+        "value": round(ms_ml, 4),
         "unit": "ms/solve",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_mx, 4),
+        "ms_per_step": round(ms_ml, 4),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 V-cycle + fp64 defect correction",
+        "dtype": "fp32 V-cycle + fp64 defect correction; bf16 U-Net",
         "data": "synthetic (seeded bubble density 1000:1, random smooth divergence RHS)",
         "config": {"workload": "bubble_2d 512x1024 variable-density pressure Poisson, all-Neumann",
                    "rel_tol": tol, "cycle": "V(2,2) red-black GS", "levels": g.level_num,
-                   "outer": "FCG(1) defect correction"},
+                   "outer": "FCG(1) defect correction", "learned": "U-Net guess, FP64-judged"},
         "fp64_mg": {"ms_per_solve": round(ms64, 4), "cycles": st64["cycles"],
                     "rel_residual": st64["rel_residual"], "accel": "stationary V-cycles"},
         "fp64_mg_fcg": {"ms_per_solve": round(ms64k, 4), "cycles": st64k["cycles"],
@@ -148,6 +276,8 @@ def main():
         "mixed": {"ms_per_solve": round(ms_mx, 4), "cycles": st_mx["cycles"],
                   "rel_residual": st_mx["rel_residual"], "rel_l2_vs_fp64": err,
                   "accel": "flexible CG (FCG(1)), FP64 vectors", "learned": False},
+        "mixed_learned": learned,
+        "speedup_vs_fp64": round(ms64 / ms_ml, 3),
         "speedup_mixed_vs_fp64": round(ms64 / ms_mx, 3),
         "speedup_mixed_vs_fp64_fcg": round(ms64k / ms_mx, 3),
         "kernels": kernels,
@@ -155,7 +285,10 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "k_pre2d<float,true> level 0 (2 RB-GS sweeps + residual + restriction)",
                      "bytes_per_launch": kbytes, "us_per_launch": round(kms * 1e3, 3)},
+        "unet": unet,
     }
+    if not args.no_3d:
+        out["plane_3d_256"] = bench_3d(args)
     if not args.no_cpu_baseline:
         log("bench: cpu baseline (oracle port, 1 core)")
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_seconds)

@@ -207,19 +207,23 @@ SCIHPC_API int scihpc_unet_backward(scihpc_unet *h, double *loss);
 SCIHPC_API int scihpc_unet_train_step(scihpc_unet *h, int n, const int *idx, float lr,
                                       double *loss);
 SCIHPC_API int scihpc_unet_sync(scihpc_unet *h);
-/* single 3x3 convolution on device pointers (NHWC BF16 x [n][h][w][cin],
- * w [cout][3][3][cin] BF16, b [cout] FP32 or NULL, y [n][h][w][cout]);
- * cin a multiple of 16, cout = 16 x {1,2,3,4,6,8,12}; synchronous */
+/* device ms of one forward pass (which 0) or training step (which 1, with
+ * a negligible learning rate) on stored samples 0..n-1, and its MFMA FLOPs */
+SCIHPC_API int scihpc_unet_time(scihpc_unet *h, int n, int which, int iters, double *ms,
+                                double *flops);
+/* single 3x3 convolution (NHWC BF16 x [n][h][w][cin], w [cout][3][3][cin]
+ * BF16, b [cout] FP32 or NULL, y [n][h][w][cout]; host arrays, copied to
+ * and from the device); cin a multiple of 16, cout = 16 x {1,2,3,4,6,8,12} */
 SCIHPC_API int scihpc_conv3x3_forward(const uint16_t *x, int n, int height, int width,
                                       int cin, const uint16_t *w, const float *b,
                                       int cout, int relu, uint16_t *y);
 
 /* Use `net` (NULL detaches) for the learned initial guess of
  * scihpc_mg_solve(use_learned = 1); its size must equal multigrid level
- * `level`'s grid.  2-D only.  The guess is scaled by the energy-optimal
- * factor; with flexible CG it is kept only if the FP64 residual after its
- * first preconditioned step is below ||b|| (else the solve restarts from
- * p = 0), with stationary cycles only if its own residual is below ||b||. */
+ * `level`'s grid.  2-D only.  The guess (network output, smoothed and
+ * prolonged level by level, scaled by the energy-optimal factor) is kept
+ * only if its FP64 residual is below ||b||; otherwise the solve starts from
+ * p = 0 (plain multigrid) and reports learned_rejected. */
 SCIHPC_API int scihpc_mg_attach_learned(scihpc_mg *h, scihpc_unet *net, int level);
 /* The network's input features of the current level-0 b (and density) on
  * `level`: feat [H*W][2]; target [H*W] (may be NULL) = block mean of the

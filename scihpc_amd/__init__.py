@@ -262,6 +262,12 @@ class UNet:
     def sync(self):
         check(self._lib.scihpc_unet_sync(self.h))
 
+    def time(self, n=1, train=False, iters=10):
+        """(device ms, MFMA FLOPs) of one forward pass or training step."""
+        ms, fl = C.c_double(), C.c_double()
+        check(self._lib.scihpc_unet_time(self.h, n, 1 if train else 0, iters, C.byref(ms), C.byref(fl)))
+        return ms.value, fl.value
+
 
 class LevelSet:
     """weno_solver + runge_kutta + bc_factory for one padded grid, on the GPU."""

@@ -53,6 +53,7 @@ def _nullable(base):
 
 
 _fpn = _nullable(_fp)
+_u16p = np.ctypeslib.ndpointer(dtype=np.uint16, flags="C_CONTIGUOUS")
 _vp = C.c_void_p
 _i, _d = C.c_int, C.c_double
 _pi = C.POINTER(C.c_int)
@@ -102,7 +103,8 @@ SIGNATURES = {
     "scihpc_unet_backward": [_vp, _pd],
     "scihpc_unet_train_step": [_vp, _i, _ip, C.c_float, _pd],
     "scihpc_unet_sync": [_vp],
-    "scihpc_conv3x3_forward": [_vp, _i, _i, _i, _i, _vp, _vp, _i, _i, _vp],
+    "scihpc_unet_time": [_vp, _i, _i, _i, _pd, _pd],
+    "scihpc_conv3x3_forward": [_u16p, _i, _i, _i, _i, _u16p, _fpn, _i, _i, _u16p],
     "scihpc_ls_create": [_i, _i, _i, _i, _d, _d, _d, _ip, _dp, C.POINTER(_vp)],
     "scihpc_ls_destroy": [_vp],
     "scihpc_ls_set_phi": [_vp, _dp],

@@ -664,6 +664,233 @@ __global__ void __launch_bounds__(kFusedThreads)
   }
 }
 
+// ======================================================= 3-D z-streaming
+// One red-black Gauss-Seidel sweep of a 3-D level per launch.  A workgroup
+// owns a TX x TY column tile and a chunk of z planes and marches up the
+// chunk with a ring of four planes in LDS (the tile plus a 2-cell ring).
+// Step `pl` stages plane pl, updates the first colour on plane pl-1 over the
+// tile and a 1-cell ring (ring cells are computed redundantly by the
+// neighbouring tiles, from the same inputs), then the second colour on plane
+// pl-2 over the tile; plane pl-2 is then final and goes to `out`.  `in` is
+// never written, so tiles and chunks never read each other's new values.
+// The next plane is fetched into registers while the current step runs.
+// A chunk also updates the first colour on the plane below and above it
+// (redundantly), which its second colour needs.
+// COEF 0: constant operator (ax, ay, az; Dirichlet wall terms as stencil());
+// COEF 1: face arrays cx, cy, cz; COEF 2: density rho with the face
+// 1 / (0.5 (rho_i + rho_j)) / h / h formed on the fly, the level-0 operator
+// of init_ppe_proj_coefficients (projection_method.cpp:404-465); cells
+// outside the domain hold rho = +inf, so wall faces vanish.
+constexpr int kZ3Threads = 256;
+
+template <int COEF>
+struct Z3 {
+  static constexpr int NA = COEF == 1 ? 5 : (COEF == 2 ? 3 : 2);  // sol, rhs, coefficient arrays
+};
+
+// face between cells of density a and b, spacing h: 2 / (a + b) / h^2
+// (FP64: one IEEE division; equal to init_ppe_proj_coefficients'
+// 1 / (0.5 (a + b)) / h / h up to rounding)
+template <typename T>
+__device__ __forceinline__ T rho_face(T a, T b, T h, T inv_h2) {
+  if constexpr (sizeof(T) == 4) return __fdividef(2.0f * inv_h2, a + b);
+  else return inv_h2 * (2.0 / (a + b));
+}
+
+// off-diagonal sum and diagonal of cell (gx, gy, gz) from LDS neighbours;
+// same accumulation order as stencil()
+template <typename T, int COEF>
+__device__ __forceinline__ void z3_stencil(const DLevel<T> &F, int gx, int gy, int gz, T xl, T xr, T xd, T xu,
+                                           T xb, T xf, T cL, T cR, T cD, T cU, T cB, T cF, T &off, T &diag) {
+  T o = T(0);
+  if constexpr (COEF != 0) {
+    if (gx > 0) o += cL * xl;
+    if (gx < F.nx - 1) o += cR * xr;
+    if (gy > 0) o += cD * xd;
+    if (gy < F.ny - 1) o += cU * xu;
+    T d = -(cR + cL + cU + cD);
+    if (gz > 0) o += cB * xb;
+    if (gz < F.nz - 1) o += cF * xf;
+    d -= (cF + cB);
+    diag = d;
+  } else {
+    T cc = T(0);
+    if (gx > 0) { o += F.ax * xl; cc += F.ax; } else if (F.rbwall) cc += F.wx;
+    if (gx < F.nx - 1) { o += F.ax * xr; cc += F.ax; } else if (F.rbwall) cc += F.wx;
+    if (gy > 0) { o += F.ay * xd; cc += F.ay; } else if (F.rbwall) cc += F.wy;
+    if (gy < F.ny - 1) { o += F.ay * xu; cc += F.ay; } else if (F.rbwall) cc += F.wy;
+    if (gz > 0) { o += F.az * xb; cc += F.az; } else if (F.rbwall) cc += F.wz;
+    if (gz < F.nz - 1) { o += F.az * xf; cc += F.az; } else if (F.rbwall) cc += F.wz;
+    diag = (F.dirichlet && !F.rbwall) ? F.dconst : -cc;
+  }
+  off = o;
+}
+
+template <typename T, int COEF, int TX, int TY>
+__global__ void __launch_bounds__(kZ3Threads)
+    k_sweep3d(DLevel<T> F, const T *__restrict__ in, T *__restrict__ out, const T *__restrict__ rho, T hx, T hy,
+              T hz, int first, int zero_guess, int zchunk) {
+  constexpr int W = TX + 4, H = TY + 4, P = W * H;
+  constexpr int NA = Z3<COEF>::NA;
+  constexpr int PER = (P + kZ3Threads - 1) / kZ3Threads;
+  __shared__ T sm[NA][4][P];
+  const int tiles_x = (F.nx + TX - 1) / TX;
+  const int ntiles = tiles_x * ((F.ny + TY - 1) / TY);
+  const int nblocks = ntiles * ((F.nz + zchunk - 1) / zchunk);
+  const int b = xcd_tile(nblocks);
+  const int tile = b % ntiles, chunk = b / ntiles;
+  const int x0 = (tile % tiles_x) * TX, y0 = (tile / tiles_x) * TY;
+  const int z0 = chunk * zchunk, z1 = min(z0 + zchunk, F.nz);
+  const int c0 = first, c1 = 1 - first;
+  const long sxy = (long)F.nx * F.ny;
+
+  // two register buffers: plane pl+2 is in flight while plane pl+1 waits
+  // and plane pl is being computed
+  T reg0[NA][PER], reg1[NA][PER];
+  auto fetch = [&](T (&reg)[NA][PER], int pl) {
+#pragma unroll
+    for (int r = 0; r < PER; ++r) {
+      const int q = threadIdx.x + r * kZ3Threads;
+      const int ly = q / W, lx = q - ly * W;
+      const int gx = x0 - 2 + lx, gy = y0 - 2 + ly;
+      const bool inside = q < P && pl >= 0 && pl < F.nz && gx >= 0 && gx < F.nx && gy >= 0 && gy < F.ny;
+      const long g = inside ? gx + (long)F.nx * gy + sxy * pl : 0;
+      reg[0][r] = (inside && !zero_guess) ? in[g] : T(0);
+      reg[1][r] = inside ? F.rhs[g] : T(0);
+      if constexpr (COEF == 1) {
+        reg[2][r] = inside ? F.cx[g] : T(0);
+        reg[3][r] = inside ? F.cy[g] : T(0);
+        reg[4][r] = inside ? F.cz[g] : T(0);
+      }
+      if constexpr (COEF == 2) reg[2][r] = inside ? rho[g] : T(INFINITY);
+    }
+  };
+  auto commit = [&](const T (&reg)[NA][PER], int slot) {
+#pragma unroll
+    for (int r = 0; r < PER; ++r) {
+      const int q = threadIdx.x + r * kZ3Threads;
+      if (q < P) {
+#pragma unroll
+        for (int a = 0; a < NA; ++a) sm[a][slot][q] = reg[a][r];
+      }
+    }
+  };
+  auto update = [&](int q, int sc, int sb, int sf, int gx, int gy, int gz) {
+    T cL = T(0), cR = T(0), cD = T(0), cU = T(0), cB = T(0), cF = T(0);
+    if constexpr (COEF == 1) {
+      cL = sm[2][sc][q - 1];
+      cR = sm[2][sc][q];
+      cD = sm[3][sc][q - W];
+      cU = sm[3][sc][q];
+      cB = sm[4][sb][q];
+      cF = sm[4][sc][q];
+    }
+    if constexpr (COEF == 2) {
+      const T rc = sm[2][sc][q];
+      cL = rho_face<T>(sm[2][sc][q - 1], rc, hx, F.ax);
+      cR = rho_face<T>(rc, sm[2][sc][q + 1], hx, F.ax);
+      cD = rho_face<T>(sm[2][sc][q - W], rc, hy, F.ay);
+      cU = rho_face<T>(rc, sm[2][sc][q + W], hy, F.ay);
+      cB = rho_face<T>(sm[2][sb][q], rc, hz, F.az);
+      cF = rho_face<T>(rc, sm[2][sf][q], hz, F.az);
+    }
+    T off, diag;
+    z3_stencil<T, COEF>(F, gx, gy, gz, sm[0][sc][q - 1], sm[0][sc][q + 1], sm[0][sc][q - W], sm[0][sc][q + W],
+                        sm[0][sb][q], sm[0][sf][q], cL, cR, cD, cU, cB, cF, off, diag);
+    sm[0][sc][q] = gs_div<T>(sm[1][sc][q] - off, diag);
+  };
+
+  const int zs = z0 - 2, ze = z1 + 1;  // planes staged
+  auto step = [&](int pl) {
+      // first colour on plane k = pl - 1, tile + 1-cell ring
+      const int k = pl - 1;
+      if (k >= z0 - 1 && k >= 0 && k < F.nz) {
+        const int sc = (k + 4) & 3, sb = (k + 3) & 3, sf = (k + 5) & 3;
+        constexpr int RW = TX + 2, RH = TY + 2, HW = (RW + 1) / 2;
+        for (int t = threadIdx.x; t < RH * HW; t += kZ3Threads) {
+          const int ry = t / HW;
+          const int gy = y0 - 1 + ry;
+          const int rx = 2 * (t - ry * HW) + ((c0 - (x0 - 1) - gy - k) & 1);
+          const int gx = x0 - 1 + rx;
+          if (rx >= RW || gx < 0 || gx >= F.nx || gy < 0 || gy >= F.ny) continue;
+          update((ry + 1) * W + rx + 1, sc, sb, sf, gx, gy, k);
+        }
+      }
+      __syncthreads();
+      // second colour on plane k1 = pl - 2, tile only; then plane k1 is final
+      const int k1 = pl - 2;
+      if (k1 >= z0 && k1 < z1) {
+        const int sc = (k1 + 4) & 3, sb = (k1 + 3) & 3, sf = (k1 + 5) & 3;
+        constexpr int HW = (TX + 1) / 2;
+        for (int t = threadIdx.x; t < TY * HW; t += kZ3Threads) {
+          const int ry = t / HW;
+          const int gy = y0 + ry;
+          const int rx = 2 * (t - ry * HW) + ((c1 - x0 - gy - k1) & 1);
+          const int gx = x0 + rx;
+          if (rx >= TX || gx >= F.nx || gy >= F.ny) continue;
+          update((ry + 2) * W + rx + 2, sc, sb, sf, gx, gy, k1);
+        }
+        __syncthreads();
+        for (int t = threadIdx.x; t < TX * TY; t += kZ3Threads) {
+          const int ry = t / TX, rx = t - ry * TX;
+          const int gx = x0 + rx, gy = y0 + ry;
+          if (gx < F.nx && gy < F.ny) out[gx + (long)F.nx * gy + sxy * k1] = sm[0][sc][(ry + 2) * W + rx + 2];
+        }
+      }
+  };
+  fetch(reg0, zs);
+  fetch(reg1, zs + 1);
+  for (int pl = zs; pl <= ze; pl += 2) {
+    commit(reg0, (pl + 4) & 3);
+    __syncthreads();
+    if (pl + 2 <= ze) fetch(reg0, pl + 2);
+    step(pl);
+    if (pl + 1 > ze) break;
+    commit(reg1, (pl + 5) & 3);
+    __syncthreads();
+    if (pl + 3 <= ze) fetch(reg1, pl + 3);
+    step(pl + 1);
+  }
+}
+
+// coarse.rhs = child average of (b - A x), coarse.sol = 0 on a 3-D level
+// whose operator is given by rho (COEF 2) or as stencil() (COEF 0/1)
+template <typename T, int COEF>
+__global__ void __launch_bounds__(kBlock) k_rr3d(DLevel<T> F, DLevel<T> G, const T *__restrict__ rho, T hx, T hy,
+                                                 T hz) {
+  const long C = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (C >= G.n) return;
+  int I, J, K;
+  cell_ijk(C, G.nx, G.ny, I, J, K);
+  const long sy = F.nx, sz = (long)F.nx * F.ny;
+  T s = T(0);
+  for (int kk = 0; kk < 2; ++kk)
+    for (int jj = 0; jj < 2; ++jj)
+      for (int ii = 0; ii < 2; ++ii) {
+        const int i = 2 * I + ii, j = 2 * J + jj, k = 2 * K + kk;
+        const long c = i + sy * j + sz * k;
+        T off, diag;
+        if constexpr (COEF == 2) {
+          const T inf = T(INFINITY), rc = rho[c];
+          const T cL = rho_face<T>(i > 0 ? rho[c - 1] : inf, rc, hx, F.ax);
+          const T cR = rho_face<T>(rc, i < F.nx - 1 ? rho[c + 1] : inf, hx, F.ax);
+          const T cD = rho_face<T>(j > 0 ? rho[c - sy] : inf, rc, hy, F.ay);
+          const T cU = rho_face<T>(rc, j < F.ny - 1 ? rho[c + sy] : inf, hy, F.ay);
+          const T cB = rho_face<T>(k > 0 ? rho[c - sz] : inf, rc, hz, F.az);
+          const T cF = rho_face<T>(rc, k < F.nz - 1 ? rho[c + sz] : inf, hz, F.az);
+          const T *x = F.sol;
+          z3_stencil<T, 2>(F, i, j, k, i > 0 ? x[c - 1] : T(0), i < F.nx - 1 ? x[c + 1] : T(0),
+                           j > 0 ? x[c - sy] : T(0), j < F.ny - 1 ? x[c + sy] : T(0), k > 0 ? x[c - sz] : T(0),
+                           k < F.nz - 1 ? x[c + sz] : T(0), cL, cR, cD, cU, cB, cF, off, diag);
+        } else {
+          stencil<T, 3, COEF == 1>(F, F.sol, i, j, k, c, off, diag);
+        }
+        s += F.rhs[c] - (off + diag * F.sol[c]);
+      }
+  G.rhs[C] = s / T(8);
+  G.sol[C] = T(0);
+}
+
 // --------------------------------------------------- residual + norm
 // res = b - A x (stored when store != 0); partial[block] = sum res^2.
 template <typename T, int DIM, bool VAR>
@@ -721,6 +948,17 @@ __global__ void k_finalize(const double *__restrict__ partial, int nb, double *_
     }
     scal[slot] = s;
   }
+}
+
+// out[b] = sum of in[b*group .. (b+1)*group) (first stage of a large
+// reduction; fixed order, so results are deterministic)
+__global__ void __launch_bounds__(kBlock) k_compress(const double *__restrict__ in, int n, int group,
+                                                     double *__restrict__ out) {
+  const int b0 = blockIdx.x * group, b1 = min(n, b0 + group);
+  double v = 0.0;
+  for (int i = b0 + threadIdx.x; i < b1; i += blockDim.x) v += in[i];
+  v = block_sum(v);
+  if (threadIdx.x == 0) out[blockIdx.x] = v;
 }
 
 // scal[s0] = sum partial[0..nb), scal[s1] = sum partial[nb..2nb)
@@ -1210,6 +1448,9 @@ struct scihpc_mg {
   long lds_max = 65536;       // LDS bytes a workgroup may use (queried at create)
   int num_cu = 256;           // compute units (queried at create)
   bool tiled = true;          // LDS-tiled 2-D fine-level kernels (SCIHPC_MG_TILED=0 disables)
+  bool tiled3d = true;        // z-streaming 3-D sweeps (SCIHPC_MG_TILED3D=0 disables)
+  int z3_chunk = 0;           // z planes per 3-D workgroup (0: chosen per level)
+  float *rho0f = nullptr;     // FP32 copy of rho0 for the mixed V-cycle
   void *blk_dev[2] = {nullptr, nullptr};   // device copies of BlockLevels<double>/<float>
   std::vector<char> blk_host[2];           // last uploaded contents
   double *host_part = nullptr;             // pinned copy of the partial sums
@@ -1349,6 +1590,62 @@ struct scihpc_mg {
     }
   }
 
+  // ------------------------------------------- z-streaming 3-D kernels
+  template <typename T>
+  bool z3_level(const std::vector<Level<T>> &lv, int l) const {
+    const auto &a = lv[l];
+    return tiled3d && ndim == 3 && a.nx >= 16 && a.ny >= 8 && a.nz >= 8;
+  }
+  // coefficient source of a level: 2 density (level 0 of A(rho)), 1 faces, 0 constant
+  template <typename T>
+  int z3_coef(int l) const {
+    if (!variable) return 0;
+    return (l == 0 && rho0 && (std::is_same<T, double>::value || rho0f)) ? 2 : 1;
+  }
+  template <typename T>
+  const T *z3_rho() const {
+    if constexpr (std::is_same<T, double>::value) return rho0;
+    else return rho0f;
+  }
+  int z3_chunk_for(long ntiles, int nz) const {
+    if (z3_chunk > 0) return std::min(z3_chunk, nz);
+    long chunks = std::max(1L, (4L * num_cu + ntiles - 1) / ntiles);
+    chunks = std::min<long>(chunks, std::max(1, nz / 8));
+    return (int)((nz + chunks - 1) / chunks);
+  }
+  template <typename T, int COEF, int TX, int TY>
+  void sweep3_k(Level<T> &a, const T *rho, int first, int zero_guess) {
+    const long ntiles = (long)ceil_div(a.nx, TX) * ceil_div(a.ny, TY);
+    const int zc = z3_chunk_for(ntiles, a.nz);
+    const unsigned grid = (unsigned)(ntiles * ceil_div(a.nz, zc));
+    k_sweep3d<T, COEF, TX, TY><<<grid, kZ3Threads, 0, stream>>>(view(a), a.sol, a.sol_alt, rho, T(a.dx), T(a.dy),
+                                                                T(a.dz), first, zero_guess, zc);
+    SCIHPC_CHECK_LAUNCH();
+    std::swap(a.sol, a.sol_alt);
+  }
+  template <typename T>
+  void sweep3(std::vector<Level<T>> &lv, int l, int first, int zero_guess) {
+    auto &a = lv[l];
+    const int coef = z3_coef<T>(l);
+    constexpr bool dbl = std::is_same<T, double>::value;
+    if (coef == 2) sweep3_k<T, 2, 32, dbl ? 8 : 16>(a, z3_rho<T>(), first, zero_guess);
+    else if (coef == 1) sweep3_k<T, 1, dbl ? 16 : 32, dbl ? 8 : 16>(a, nullptr, first, zero_guess);
+    else sweep3_k<T, 0, 32, dbl ? 8 : 16>(a, nullptr, first, zero_guess);
+  }
+  template <typename T>
+  void rr3(std::vector<Level<T>> &lv, int l) {
+    auto &a = lv[l], &b = lv[l + 1];
+    const unsigned g = ceil_div(b.n, kBlock);
+    const int coef = z3_coef<T>(l);
+    if (coef == 2)
+      k_rr3d<T, 2><<<g, kBlock, 0, stream>>>(view(a), view(b), z3_rho<T>(), T(a.dx), T(a.dy), T(a.dz));
+    else if (coef == 1)
+      k_rr3d<T, 1><<<g, kBlock, 0, stream>>>(view(a), view(b), nullptr, T(a.dx), T(a.dy), T(a.dz));
+    else
+      k_rr3d<T, 0><<<g, kBlock, 0, stream>>>(view(a), view(b), nullptr, T(a.dx), T(a.dy), T(a.dz));
+    SCIHPC_CHECK_LAUNCH();
+  }
+
   // generic full sweep (3-D levels, ratio-3 levels): two colour kernels
   template <typename T>
   void sweep(Level<T> &l, int first, int zero_guess) {
@@ -1433,6 +1730,9 @@ struct scihpc_mg {
       const int z = (l == 0 && zg) ? 1 : 0;
       if (can_tile && fused_level(lv, l)) {
         pre2d(lv, l, nu1, z);
+      } else if (z3_level(lv, l) && lv[l + 1].deg / lv[l].deg == 2) {
+        for (int s = 0; s < nu1; ++s) sweep3(lv, l, 0, s == 0 ? z : 0);
+        rr3(lv, l);
       } else {
         for (int s = 0; s < nu1; ++s) sweep(lv[l], 0, s == 0 ? z : 0);
         residual_restrict(lv[l], lv[l + 1]);
@@ -1457,6 +1757,9 @@ struct scihpc_mg {
           post0_done = true;
           partials = mode != 0 ? (int)tiles2d(lv[0]) : 0;
         }
+      } else if (z3_level(lv, f)) {
+        prolong_add(lv[l], lv[f]);
+        for (int s = 0; s < nu2; ++s) sweep3(lv, f, 1, 0);
       } else {
         prolong_add(lv[l], lv[f]);
         for (int s = 0; s < nu2; ++s) sweep(lv[f], 1, 0);
@@ -1493,8 +1796,23 @@ struct scihpc_mg {
 
   // sum of partial[0..np) on the host (the norm the host tests anyway):
   // one D2H copy, fixed summation order
+  // partial sums beyond kCompressAbove are first summed in groups of 256 on
+  // the device (one wide launch instead of one long single-block loop)
+  static constexpr int kCompressAbove = 2048;
+  double *partial2 = nullptr;
+  int compress(const double *src, int n, double *dst) {
+    const int m = (n + kBlock - 1) / kBlock;
+    k_compress<<<m, kBlock, 0, stream>>>(src, n, kBlock, dst);
+    SCIHPC_CHECK_LAUNCH();
+    return m;
+  }
   double host_sum(int np) {
-    SCIHPC_HIP(hipMemcpyAsync(host_part, partial, sizeof(double) * np, hipMemcpyDeviceToHost, stream));
+    const double *src = partial;
+    if (np > kCompressAbove) {
+      np = compress(partial, np, partial2);
+      src = partial2;
+    }
+    SCIHPC_HIP(hipMemcpyAsync(host_part, src, sizeof(double) * np, hipMemcpyDeviceToHost, stream));
     SCIHPC_HIP(hipStreamSynchronize(stream));
     double s = 0.0;
     for (int q = 0; q < np; ++q) s += host_part[q];
@@ -1503,7 +1821,23 @@ struct scihpc_mg {
 
   // sum of partial[0..nb) -> scal[slot]; returns nothing (device only)
   void finalize(int nb, int slot, int op = 0) {
-    k_finalize<<<1, kBlock, 0, stream>>>(partial, nb, scal, slot, op);
+    if (nb > kCompressAbove) {
+      const int m = compress(partial, nb, partial2);
+      k_finalize<<<1, kBlock, 0, stream>>>(partial2, m, scal, slot, op);
+    } else {
+      k_finalize<<<1, kBlock, 0, stream>>>(partial, nb, scal, slot, op);
+    }
+    SCIHPC_CHECK_LAUNCH();
+  }
+  // scal[s0] = sum partial[0..nb), scal[s1] = sum partial[nb..2nb)
+  void finalize2(int nb, int s0, int s1) {
+    if (nb > kCompressAbove) {
+      const int m = compress(partial, nb, partial2);
+      compress(partial + nb, nb, partial2 + m);
+      k_finalize2<<<1, kBlock, 0, stream>>>(partial2, m, scal, s0, s1);
+    } else {
+      k_finalize2<<<1, kBlock, 0, stream>>>(partial, nb, scal, s0, s1);
+    }
     SCIHPC_CHECK_LAUNCH();
   }
 
@@ -1737,6 +2071,10 @@ struct scihpc_mg {
       b.cy = dalloc<float>(b.n, stream);
       if (ndim == 3) b.cz = dalloc<float>(b.n, stream);
     }
+    if (l == 0 && rho0) {
+      if (!rho0f) rho0f = dalloc<float>(a.n, stream);
+      k_to_float<<<nb, kBlock, 0, stream>>>(rho0, rho0f, a.n);
+    }
     k_to_float<<<nb, kBlock, 0, stream>>>(a.cx, b.cx, a.n);
     k_to_float<<<nb, kBlock, 0, stream>>>(a.cy, b.cy, a.n);
     if (ndim == 3) k_to_float<<<nb, kBlock, 0, stream>>>(a.cz, b.cz, a.n);
@@ -1791,11 +2129,7 @@ struct scihpc_mg {
 
   // Krylov-accelerated defect correction: FCG(1) with the V-cycle (FP64 or
   // FP32) as preconditioner; x, r, p, q and every inner product in FP64.
-  // judge: x0 is a learned guess, kept only if the FP64 residual after the
-  // first preconditioned step is below ||b||; otherwise the iteration
-  // restarts from x = 0 (the rejected step counts in st.cycles)
-  int solve_fcg(const scihpc_solve_opts &o, scihpc_solve_stats &st, bool zero, double bnorm, double scale,
-                bool judge = false) {
+  int solve_fcg(const scihpc_solve_opts &o, scihpc_solve_stats &st, bool zero, double bnorm, double scale) {
     ensure_fcg();
     auto &l0 = d[0];
     const long n = l0.n;
@@ -1818,8 +2152,8 @@ struct scihpc_mg {
     if (!mixed) std::swap(l0.rhs, fr_);
     double *r = mixed ? fr_ : l0.rhs;
     double rr = zero ? (bnorm > 0.0 ? 1.0 : 0.0) : std::sqrt(fetch(S_NORM)) / scale;
-    int it = 0, cur = 0, spent = 0;
-    while (rr > o.rel_tol && it + spent < o.max_cycles) {
+    int it = 0, cur = 0;
+    while (rr > o.rel_tol && it < o.max_cycles) {
       // z = B r; from the second iteration on the level-0 post kernel also
       // forms z . q_old (S_ZQ) for the flexible direction update
       OuterArgs oa{};
@@ -1847,32 +2181,13 @@ struct scihpc_mg {
       }
 #undef SCIHPC_FCG_DIR
       SCIHPC_CHECK_LAUNCH();
-      k_finalize2<<<1, kBlock, 0, stream>>>(partial, nb, scal, S_PQ, S_PR);
-      SCIHPC_CHECK_LAUNCH();
+      finalize2(nb, S_PQ, S_PR);
       cur ^= 1;
       k_fcg_update<<<nb, kBlock, 0, stream>>>(n, x, r, fp_[cur], fq_, scal, mixed ? f[0].rhs : nullptr, partial);
       SCIHPC_CHECK_LAUNCH();
       rr = std::sqrt(host_sum(nb)) / scale;
       ++it;
-      if (judge && it == 1) {
-        judge = false;
-        st.learned_residual = rr;
-        if (std::isfinite(rr) && rr < 1.0) {
-          st.learned_accepted++;
-        } else {  // fall back to plain multigrid from x = 0
-          st.learned_rejected++;
-          spent = it;
-          it = 0;
-          cur = 0;
-          SCIHPC_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, stream));
-          if (mixed) k_copy_cast<<<nb, kBlock, 0, stream>>>(l0.rhs, fr_, f[0].rhs, n);
-          else SCIHPC_HIP(hipMemcpyAsync(r, fr_, sizeof(double) * n, hipMemcpyDeviceToDevice, stream));
-          SCIHPC_CHECK_LAUNCH();
-          rr = bnorm > 0.0 ? 1.0 : 0.0;
-        }
-      }
     }
-    it += spent;
     last_rr_ = rr;
     if (!mixed) std::swap(l0.rhs, fr_);  // restore b
     SCIHPC_HIP(hipMemcpyAsync(l0.sol, x, sizeof(double) * n, hipMemcpyDeviceToDevice, stream));
@@ -1894,21 +2209,17 @@ struct scihpc_mg {
     int cycles = 0;
     double rr;
     bool zero = o.zero_guess != 0;
-    bool judge = false;
     if (o.use_learned && learned.active() && bnorm > 0.0) {
-      // with flexible CG the guess is judged after its first preconditioned
-      // step (solve_fcg); the stationary loops judge the guess itself
       // use_learned == 2 keeps the guess unjudged (ablation measurements)
-      if (learned_initial_guess(st, bnorm, o.accel == 1 || o.use_learned == 2)) {
+      if (learned_initial_guess(st, bnorm, o.use_learned == 2)) {
         zero = false;
-        judge = o.accel == 1 && o.use_learned != 2;
       } else {  // fall back to plain multigrid from p = 0
         SCIHPC_HIP(hipMemsetAsync(l0.sol, 0, sizeof(double) * l0.n, stream));
         zero = true;
       }
     }
     if (o.accel == 1) {
-      solve_fcg(o, st, zero, bnorm, scale, judge);
+      solve_fcg(o, st, zero, bnorm, scale);
       cycles = st.cycles;
       rr = last_rr_;
     } else if (o.precision == SCIHPC_PREC_FP64) {
@@ -1985,10 +2296,9 @@ struct scihpc_mg {
   // learned initial guess: p0 = alpha * S P S N(b, ln rho) where N runs on
   // the hook's level, S is two red-black sweeps on a level (against the
   // child-averaged b), P prolongs one level up, and alpha = (p.b)/(p.A p) is
-  // the energy-optimal scale.  Stationary solves keep it only if
-  // ||b - A p0|| < ||b|| in FP64; flexible CG (provisional) keeps it if alpha
-  // is finite and judges it after its first step.  Otherwise the solve
-  // starts from p = 0 (plain multigrid).
+  // the energy-optimal scale.  It is kept only if ||b - A p0|| < ||b|| in
+  // FP64 (provisional: kept unjudged whenever alpha is finite); otherwise the
+  // solve starts from p = 0 (plain multigrid).
   bool learned_initial_guess(scihpc_solve_stats &st, double bnorm, bool provisional) {
     auto &l0 = d[0];
     const int lvl = learned.level;
@@ -2098,6 +2408,8 @@ extern "C" int scihpc_mg_create(int ndim, int nx, int ny, int nz, double dx, dou
       h->bc = bc;
       if (const char *e = std::getenv("SCIHPC_MG_BLOCK_MAX")) h->block_max = std::atol(e);
       if (const char *e = std::getenv("SCIHPC_MG_TILED")) h->tiled = std::atoi(e) != 0;
+      if (const char *e = std::getenv("SCIHPC_MG_TILED3D")) h->tiled3d = std::atoi(e) != 0;
+      if (const char *e = std::getenv("SCIHPC_MG_Z3_CHUNK")) h->z3_chunk = std::atoi(e);
       std::vector<int> lx, ly, lz, ld;
       h->L = g_levels(ndim, nx, ny, nz, lx, ly, lz, ld);
       SCIHPC_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
@@ -2124,6 +2436,7 @@ extern "C" int scihpc_mg_create(int ndim, int nx, int ny, int nz, double dx, dou
       h->Minv = dalloc<double>((long)std::min<long>(h->d.back().n, kMaxCoarse) *
                                std::min<long>(h->d.back().n, kMaxCoarse), h->stream);
       h->partial = dalloc<double>(4 * ((long)ceil_div(h->d[0].n, kBlock) + 64), h->stream);
+      h->partial2 = dalloc<double>(4 * ((long)ceil_div(h->d[0].n, (long)kBlock * kBlock) + 64), h->stream);
       SCIHPC_HIP(hipHostMalloc((void **)&h->host_part,
                                sizeof(double) * 4 * ((long)ceil_div(h->d[0].n, kBlock) + 64),
                                hipHostMallocDefault));
@@ -2184,6 +2497,7 @@ extern "C" int scihpc_mg_destroy(scihpc_mg *h) {
   if (h->Minv_ref != h->Minv) fr(h->Minv_ref);
   fr(h->Minv);
   fr(h->partial);
+  fr(h->partial2);
   fr(h->blk_dev[0]);
   fr(h->blk_dev[1]);
   if (h->host_part) (void)hipHostFree(h->host_part);
@@ -2195,6 +2509,7 @@ extern "C" int scihpc_mg_destroy(scihpc_mg *h) {
   fr(h->scal);
   fr(h->p_alt);
   fr(h->rho0);
+  fr(h->rho0f);
   fr(h->lfeat);
   fr(h->lout);
   if (h->host_scal) (void)hipHostFree(h->host_scal);
@@ -2511,47 +2826,69 @@ extern "C" int scihpc_mg_time_kernel(scihpc_mg *h, int which, int iters, double 
     const double n = (double)l0.n, nc = (double)h->d[1].n;
     const double nf = h->variable ? h->ndim : 0;  // face arrays per level
     const bool tiled = h->fused_level(h->d, 0);
+    const bool z3 = !tiled && h->z3_level(h->d, 0) && h->d[1].deg == 2;
+    // coefficient arrays a level-0 sweep reads: rho (z-streaming, A(rho)), faces, none
+    const double na = z3 ? (h->variable ? 1.0 : 0.0) : nf;
     OuterArgs oa{};
     oa.P = h->view(l0);
     oa.p_old = l0.sol;
     oa.p_new = h->p_alt;
     oa.r32 = f0.rhs_alt;
+    auto pre = [&](auto &lv) {
+      for (int s = 0; s < 2; ++s) {
+        if (z3) h->sweep3(lv, 0, 0, 0);
+        else h->sweep(lv[0], 0, 0);
+      }
+      if (z3) h->rr3(lv, 0);
+      else h->residual_restrict(lv[0], lv[1]);
+    };
+    auto post = [&](auto &lv, int mode) {
+      h->prolong_add(lv[1], lv[0]);
+      for (int s = 0; s < 2; ++s) {
+        if (z3) h->sweep3(lv, 0, 1, 0);
+        else h->sweep(lv[0], 1, 0);
+      }
+      h->post0_generic(lv, mode, oa);
+    };
     auto launch = [&] {
       switch (which) {
-        case 0:  // FP64 level-0 pre-smoothing kernel
+        case 0:  // FP64 level-0 pre-smoothing (2 sweeps + residual + restriction)
           if (tiled) h->pre2d(h->d, 0, 2, 0);
-          else {
-            h->sweep(l0, 0, 0);
-            h->sweep(l0, 0, 0);
-            h->residual_restrict(l0, h->d[1]);
-          }
+          else pre(h->d);
           break;
-        case 1:  // FP32 level-0 pre-smoothing kernel
+        case 1:  // the same in FP32
           if (tiled) h->pre2d(h->f, 0, 2, 0);
-          else {
-            h->sweep(f0, 0, 0);
-            h->sweep(f0, 0, 0);
-            h->residual_restrict(f0, h->f[1]);
-          }
+          else pre(h->f);
           break;
-        case 2:  // FP32 level-0 post-smoothing kernel fused with the FP64 outer step
+        case 2:  // FP32 level-0 post-smoothing with the FP64 outer update
           if (tiled) h->post2d_mode<float, 2>(h->f, 0, 2, oa);
-          else h->post0_generic(h->f, 2, oa);
+          else post(h->f, 2);
           break;
-        case 3:  // FP64 level-0 post-smoothing kernel with the residual norm
+        case 3:  // FP64 level-0 post-smoothing with the residual norm
           if (tiled) h->post2d_mode<double, 1>(h->d, 0, 2, oa);
-          else h->post0_generic(h->d, 1, oa);
+          else post(h->d, 1);
           break;
       }
     };
     // algorithmic bytes: every array the launch touches, once per read and
     // once per write, at its stored precision
     double b = 0;
-    switch (which) {
-      case 0: b = (3.0 + nf) * 8.0 * n + 2.0 * 8.0 * nc; break;
-      case 1: b = (3.0 + nf) * 4.0 * n + 2.0 * 4.0 * nc; break;
-      case 2: b = (2.0 + nf) * 4.0 * n + 4.0 * nc + (3.0 + nf) * 8.0 * n + 8.0 * n + 4.0 * n; break;
-      case 3: b = (3.0 + nf) * 8.0 * n + 8.0 * nc; break;
+    if (tiled) {  // one fused kernel per launch
+      switch (which) {
+        case 0: b = (3.0 + nf) * 8.0 * n + 2.0 * 8.0 * nc; break;
+        case 1: b = (3.0 + nf) * 4.0 * n + 2.0 * 4.0 * nc; break;
+        case 2: b = (2.0 + nf) * 4.0 * n + 4.0 * nc + (3.0 + nf) * 8.0 * n + 8.0 * n + 4.0 * n; break;
+        case 3: b = (3.0 + nf) * 8.0 * n + 8.0 * nc; break;
+      }
+    } else {
+      // sweep (3 + na) s N; residual + restriction (2 + na) s N + 2 s Nc;
+      // prolong-add 2 s N + s Nc; FP64 outer update (32 + 8 nf) N;
+      // FP64 residual norm (16 + 8 nf) N
+      const double sd = which == 1 || which == 2 ? 4.0 : 8.0;
+      if (which <= 1) b = 2.0 * (3.0 + na) * sd * n + (2.0 + na) * sd * n + 2.0 * sd * nc;
+      else
+        b = 2.0 * sd * n + sd * nc + 2.0 * (3.0 + na) * sd * n +
+            (which == 2 ? (32.0 + 8.0 * nf) * n : (16.0 + 8.0 * nf) * n);
     }
     launch();  // warm-up
     SCIHPC_HIP(hipEventRecord(h->ev0, h->stream));

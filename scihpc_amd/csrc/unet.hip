@@ -773,8 +773,52 @@ extern "C" int scihpc_unet_sync(scihpc_unet *h) {
   });
 }
 
-// ------------------------------------------------ single-kernel entry points
-// (device pointers; used by the kernel tests)
+// device time of `iters` forward passes (which = 0) or training steps
+// (which = 1: forward, backward, Adam) on stored samples 0..n-1 (HIP events
+// on the network's stream), and the matrix-core FLOPs of one: 2 x 9 x cin x
+// cout per pixel for every 3x3 convolution, doubled in training for the
+// data and weight gradients (the first layer has no data gradient)
+extern "C" int scihpc_unet_time(scihpc_unet *h, int n, int which, int iters, double *ms, double *flops) {
+  return guarded([&] {
+    SCIHPC_REQUIRE(h && iters > 0 && (which == 0 || which == 1), "bad timing request");
+    SCIHPC_REQUIRE(h->store_n >= 1, "store samples first");
+    std::vector<int> idx(2 * n);
+    for (int b = 0; b < n; ++b) {
+      idx[2 * b] = b % h->store_n;
+      idx[2 * b + 1] = 0;
+    }
+    double f = 0.0;
+    for (int i = 0; i < kLayers; ++i) {
+      const double c = 2.0 * 9.0 * h->L[i].cin * h->L[i].cout * (double)h->pix(h->L[i].lev, n);
+      f += which == 0 ? c : (i == 0 ? 2.0 * c : 3.0 * c);
+    }
+    h->load_batch(n, idx.data());
+    auto once = [&] {
+      h->forward(n, h->stream);
+      if (which == 1) {
+        h->backward(false);
+        h->adam(1e-12f);
+      }
+    };
+    once();
+    hipEvent_t e0, e1;
+    SCIHPC_HIP(hipEventCreate(&e0));
+    SCIHPC_HIP(hipEventCreate(&e1));
+    SCIHPC_HIP(hipEventRecord(e0, h->stream));
+    for (int it = 0; it < iters; ++it) once();
+    SCIHPC_HIP(hipEventRecord(e1, h->stream));
+    SCIHPC_HIP(hipEventSynchronize(e1));
+    float t = 0.f;
+    SCIHPC_HIP(hipEventElapsedTime(&t, e0, e1));
+    SCIHPC_HIP(hipEventDestroy(e0));
+    SCIHPC_HIP(hipEventDestroy(e1));
+    if (ms) *ms = t / iters;
+    if (flops) *flops = f;
+  });
+}
+
+// ------------------------------------------------ single-kernel entry point
+// (host arrays, copied to and from the device; used by the kernel tests)
 extern "C" int scihpc_conv3x3_forward(const uint16_t *x, int n, int height, int width, int cin, const uint16_t *w,
                                       const float *b, int cout, int relu, uint16_t *y) {
   return guarded([&] {
@@ -783,19 +827,48 @@ extern "C" int scihpc_conv3x3_forward(const uint16_t *x, int n, int height, int 
     const int nb = cout / 16;
     SCIHPC_REQUIRE(cout % 16 == 0 && (nb == 1 || nb == 2 || nb == 3 || nb == 4 || nb == 6 || nb == 8 || nb == 12),
                    "cout must be 16 x {1,2,3,4,6,8,12}");
-    const ConvShape s{n, height, width, cin, cout};
-    const View xv{const_cast<bf16_t *>(x), cin, 0}, yv{y, cout, 0}, none{nullptr, 0, 0};
-    const unsigned grid = (unsigned)((long)n * height * ((width + 63) / 64));
-    switch (nb) {
-      case 1: k_conv3x3<1><<<grid, kConvThreads>>>(xv, none, w, b, yv, s, relu); break;
-      case 2: k_conv3x3<2><<<grid, kConvThreads>>>(xv, none, w, b, yv, s, relu); break;
-      case 3: k_conv3x3<3><<<grid, kConvThreads>>>(xv, none, w, b, yv, s, relu); break;
-      case 4: k_conv3x3<4><<<grid, kConvThreads>>>(xv, none, w, b, yv, s, relu); break;
-      case 6: k_conv3x3<6><<<grid, kConvThreads>>>(xv, none, w, b, yv, s, relu); break;
-      case 8: k_conv3x3<8><<<grid, kConvThreads>>>(xv, none, w, b, yv, s, relu); break;
-      case 12: k_conv3x3<12><<<grid, kConvThreads>>>(xv, none, w, b, yv, s, relu); break;
+    const long npix = (long)n * height * width;
+    SCIHPC_REQUIRE(npix <= (1L << 26), "too many pixels");
+    const size_t sx = sizeof(bf16_t) * npix * cin, sy = sizeof(bf16_t) * npix * cout;
+    const size_t sw = sizeof(bf16_t) * (size_t)cout * 9 * cin, sb = sizeof(float) * cout;
+    hipStream_t st = nullptr;
+    void *dx = nullptr, *dy = nullptr, *dw = nullptr, *db = nullptr;
+    auto cleanup = [&] {
+      if (st) (void)hipStreamSynchronize(st);
+      for (void *p : {dx, dy, dw, db})
+        if (p) (void)hipFree(p);
+      if (st) (void)hipStreamDestroy(st);
+    };
+    try {
+      SCIHPC_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+      SCIHPC_HIP(hipMalloc(&dx, sx));
+      SCIHPC_HIP(hipMalloc(&dy, sy));
+      SCIHPC_HIP(hipMalloc(&dw, sw));
+      SCIHPC_HIP(hipMalloc(&db, sb));
+      SCIHPC_HIP(hipMemcpyAsync(dx, x, sx, hipMemcpyHostToDevice, st));
+      SCIHPC_HIP(hipMemcpyAsync(dw, w, sw, hipMemcpyHostToDevice, st));
+      if (b) SCIHPC_HIP(hipMemcpyAsync(db, b, sb, hipMemcpyHostToDevice, st));
+      const ConvShape s{n, height, width, cin, cout};
+      const View xv{static_cast<bf16_t *>(dx), cin, 0}, yv{static_cast<bf16_t *>(dy), cout, 0}, none{nullptr, 0, 0};
+      const bf16_t *wp = static_cast<const bf16_t *>(dw);
+      const float *bp = b ? static_cast<const float *>(db) : nullptr;
+      const unsigned grid = (unsigned)((long)n * height * ((width + 63) / 64));
+      switch (nb) {
+        case 1: k_conv3x3<1><<<grid, kConvThreads, 0, st>>>(xv, none, wp, bp, yv, s, relu); break;
+        case 2: k_conv3x3<2><<<grid, kConvThreads, 0, st>>>(xv, none, wp, bp, yv, s, relu); break;
+        case 3: k_conv3x3<3><<<grid, kConvThreads, 0, st>>>(xv, none, wp, bp, yv, s, relu); break;
+        case 4: k_conv3x3<4><<<grid, kConvThreads, 0, st>>>(xv, none, wp, bp, yv, s, relu); break;
+        case 6: k_conv3x3<6><<<grid, kConvThreads, 0, st>>>(xv, none, wp, bp, yv, s, relu); break;
+        case 8: k_conv3x3<8><<<grid, kConvThreads, 0, st>>>(xv, none, wp, bp, yv, s, relu); break;
+        case 12: k_conv3x3<12><<<grid, kConvThreads, 0, st>>>(xv, none, wp, bp, yv, s, relu); break;
+      }
+      SCIHPC_CHECK_LAUNCH();
+      SCIHPC_HIP(hipMemcpyAsync(y, dy, sy, hipMemcpyDeviceToHost, st));
+      SCIHPC_HIP(hipStreamSynchronize(st));
+    } catch (...) {
+      cleanup();
+      throw;
     }
-    SCIHPC_CHECK_LAUNCH();
-    SCIHPC_HIP(hipDeviceSynchronize());
+    cleanup();
   });
 }

@@ -30,6 +30,9 @@ CASES = [
     (3, (24, 24, 24), (1 / 24, 1 / 24, 1 / 24), "dirichlet", False),
     (3, (16, 16, 32), (1 / 16, 1 / 16, 1 / 16), "neumann", True),
     (2, (54, 54, 1), (1 / 54, 1 / 54, 1.0), "neumann", False),  # ratio-3 coarsening
+    # z-streaming 3-D sweeps on levels 0 and 1 (density and face coefficients)
+    (3, (32, 32, 32), (1 / 32, 1 / 32, 1 / 32), "neumann", True),
+    (3, (32, 48, 32), (1 / 32, 1 / 32, 1 / 32), "dirichlet", False),
 ]
 
 
@@ -244,3 +247,32 @@ def test_errors_are_status_codes():
     g.set_rhs(np.random.default_rng(0).standard_normal(64 * 64))
     st = g.solve(rel_tol=1e-12, max_cycles=1)
     assert not st["converged"] and st["cycles"] == 1
+
+
+@pytest.mark.parametrize("case", [c for c in CASES if c[0] == 3],
+                         ids=lambda c: f"{c[1]}-{c[3]}-{'var' if c[4] else 'const'}")
+@pytest.mark.parametrize("precision", [PREC_FP64, PREC_MIXED], ids=["fp64", "mixed"])
+def test_z3_sweeps_match_colour_kernels(oracle, case, precision, monkeypatch):
+    """The z-streaming 3-D sweep is the same red-black Gauss-Seidel as the
+    two colour kernels: equal V-cycle iterates (FP64 to rounding) and equal
+    solve results."""
+    rng = np.random.default_rng(7)
+    sols, stats = [], []
+    for flag in ("1", "0"):
+        monkeypatch.setenv("SCIHPC_MG_TILED3D", flag)
+        g, _, _ = make_pair(oracle, case)
+        b = rng.standard_normal(g.n[0]) if not sols else b
+        if case[3] == "neumann":
+            b = b - b.mean()
+        g.set_rhs(b)
+        g.set_sol(np.zeros_like(b))
+        for _ in range(2):
+            g.v_cycle(2, 2)
+        it = g.sol(0)
+        st = g.solve(rel_tol=1e-8, max_cycles=100, precision=precision, accel=1)
+        sols.append((it, g.sol(0)))
+        stats.append(st)
+    assert rel(sols[0][0], sols[1][0]) < 1e-12
+    assert stats[0]["converged"] and stats[1]["converged"]
+    assert abs(stats[0]["cycles"] - stats[1]["cycles"]) <= (0 if precision == PREC_FP64 else 1)
+    assert rel(sols[0][1], sols[1][1]) < (1e-10 if precision == PREC_FP64 else 1e-6)
