@@ -31,6 +31,11 @@ sys.path.insert(0, ROOT)
 METRIC = "ms/pressure solve to 1e-8 residual: FP64 MG vs mixed+learned; HBM GB/s"
 HBM_PEAK_GBS = 8000.0
 MFMA_BF16_PEAK = 2500.0  # TFLOP/s, dense
+# HBM bytes per launch of the dominant kernel from two rocprofv3 --pmc passes
+# (FETCH_SIZE, WRITE_SIZE) of `bench.py --kernel-only`, each scaled by what the
+# counter reads for a 256 MiB device copy (FETCH x 2.0, WRITE x 1.0):
+# profiles/pmc_traffic_k_pre2d_fp32_bubble_512x1024.md
+PRE2D_FP32_TRAFFIC_BYTES = (3977.4 * 2.0 + 3072.0 * 1.0) * 1024
 
 
 def log(msg):
@@ -282,7 +287,8 @@ def main():
         "speedup_mixed_vs_fp64_fcg": round(ms64k / ms_mx, 3),
         "kernels": kernels,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": round(PRE2D_FP32_TRAFFIC_BYTES),
                      "kernel": "k_pre2d<float,true> level 0 (2 RB-GS sweeps + residual + restriction)",
                      "bytes_per_launch": kbytes, "us_per_launch": round(kms * 1e3, 3)},
         "unet": unet,

@@ -221,9 +221,13 @@ SCIHPC_API int scihpc_conv3x3_forward(const uint16_t *x, int n, int height, int 
 /* Use `net` (NULL detaches) for the learned initial guess of
  * scihpc_mg_solve(use_learned = 1); its size must equal multigrid level
  * `level`'s grid.  2-D only.  The guess (network output, smoothed and
- * prolonged level by level, scaled by the energy-optimal factor) is kept
- * only if its FP64 residual is below ||b||; otherwise the solve starts from
- * p = 0 (plain multigrid) and reports learned_rejected. */
+ * prolonged level by level, scaled by the energy-optimal factor) is judged
+ * by its FP64 residual on every solve.  In the mixed flexible-CG solve it is
+ * formed on a second stream while the iterations from p = 0 run, and when it
+ * is ready it replaces the iterate only if its residual is below both ||b||
+ * and the iterate's; otherwise it is formed first and kept only if its
+ * residual is below ||b||.  A rejected guess leaves plain multigrid from
+ * p = 0 (learned_rejected). */
 SCIHPC_API int scihpc_mg_attach_learned(scihpc_mg *h, scihpc_unet *net, int level);
 /* The network's input features of the current level-0 b (and density) on
  * `level`: feat [H*W][2]; target [H*W] (may be NULL) = block mean of the

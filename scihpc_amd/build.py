@@ -2,7 +2,9 @@
 
 Every HIP source is compiled for gfx950 only, with packed-FP32 instruction
 selection disabled, into one shared library exporting the C ABI of
-include/scihpc_hip.h.  ``build()`` always recompiles.
+include/scihpc_hip.h.  The C++ drivers under drivers/ (written against
+include/scihpc_facade.hpp) are then compiled with g++ and linked to it.
+``build()`` always recompiles.
 """
 from __future__ import annotations
 
@@ -16,6 +18,7 @@ CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libscihpc_hip.so")
 
 SOURCES = ["mg.hip", "advect.hip", "unet.hip"]
+DRIVERS = ["facade_demo"]
 
 HIPCC_FLAGS = [
     "-O3",
@@ -46,7 +49,24 @@ def build(verbose: bool = False) -> str:
         sys.stderr.write(res.stdout + res.stderr)
         raise RuntimeError("hipcc failed building libscihpc_hip.so")
     os.replace(tmp, LIB)
+    for d in DRIVERS:
+        build_driver(d, verbose)
     return LIB
+
+
+def build_driver(name: str, verbose: bool = False) -> str:
+    src = os.path.join(PKG, "drivers", name + ".cpp")
+    out = os.path.join(PKG, "drivers", name)
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"), src, "-L", PKG,
+           "-lscihpc_hip", "-Wl,-rpath,$ORIGIN/..", "-o", out + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError(f"g++ failed building {name}")
+    os.replace(out + ".tmp", out)
+    return out
 
 
 if __name__ == "__main__":

@@ -2129,7 +2129,10 @@ struct scihpc_mg {
 
   // Krylov-accelerated defect correction: FCG(1) with the V-cycle (FP64 or
   // FP32) as preconditioner; x, r, p, q and every inner product in FP64.
-  int solve_fcg(const scihpc_solve_opts &o, scihpc_solve_stats &st, bool zero, double bnorm, double scale) {
+  // learned: a guess is being formed on lstream; after the first iteration
+  // it replaces that iteration if its FP64 residual is lower
+  int solve_fcg(const scihpc_solve_opts &o, scihpc_solve_stats &st, bool zero, double bnorm, double scale,
+                bool learned_overlap = false) {
     ensure_fcg();
     auto &l0 = d[0];
     const long n = l0.n;
@@ -2152,8 +2155,8 @@ struct scihpc_mg {
     if (!mixed) std::swap(l0.rhs, fr_);
     double *r = mixed ? fr_ : l0.rhs;
     double rr = zero ? (bnorm > 0.0 ? 1.0 : 0.0) : std::sqrt(fetch(S_NORM)) / scale;
-    int it = 0, cur = 0;
-    while (rr > o.rel_tol && it < o.max_cycles) {
+    int it = 0, cur = 0, spent = 0;
+    while (rr > o.rel_tol && it + spent < o.max_cycles) {
       // z = B r; from the second iteration on the level-0 post kernel also
       // forms z . q_old (S_ZQ) for the flexible direction update
       OuterArgs oa{};
@@ -2187,7 +2190,30 @@ struct scihpc_mg {
       SCIHPC_CHECK_LAUNCH();
       rr = std::sqrt(host_sum(nb)) / scale;
       ++it;
+      // the guess is judged at the end of the first iteration after which
+      // its stream has finished (no wait): it replaces the iterate only if
+      // its FP64 residual is below both ||b|| and the iterate's
+      if (learned_overlap && hipEventQuery(lev) == hipSuccess) {
+        learned_overlap = false;
+        if (learned_decide(st, bnorm, std::min(rr, 1.0), false)) {
+          // restart from the guess: x = p0, r = b - A p0 (mixed: b is l0.rhs)
+          spent += it;
+          it = 0;
+          cur = 0;
+          SCIHPC_HIP(hipMemcpyAsync(x, l0.sol, sizeof(double) * n, hipMemcpyDeviceToDevice, stream));
+          reduce_residual(l0, true);
+          k_copy_cast<<<nb, kBlock, 0, stream>>>(l0.res, fr_, f[0].rhs, n);
+          SCIHPC_CHECK_LAUNCH();
+          rr = std::sqrt(fetch(S_NORM)) / scale;
+        }
+      }
     }
+    if (learned_overlap) {  // converged before the guess was ready: unused
+      SCIHPC_HIP(hipEventSynchronize(lev));
+      learned_pending = false;
+      st.learned_rejected++;
+    }
+    it += spent;
     last_rr_ = rr;
     if (!mixed) std::swap(l0.rhs, fr_);  // restore b
     SCIHPC_HIP(hipMemcpyAsync(l0.sol, x, sizeof(double) * n, hipMemcpyDeviceToDevice, stream));
@@ -2209,17 +2235,22 @@ struct scihpc_mg {
     int cycles = 0;
     double rr;
     bool zero = o.zero_guess != 0;
+    bool overlap = false;
     if (o.use_learned && learned.active() && bnorm > 0.0) {
       // use_learned == 2 keeps the guess unjudged (ablation measurements)
-      if (learned_initial_guess(st, bnorm, o.use_learned == 2)) {
-        zero = false;
-      } else {  // fall back to plain multigrid from p = 0
-        SCIHPC_HIP(hipMemsetAsync(l0.sol, 0, sizeof(double) * l0.n, stream));
-        zero = true;
+      overlap = o.use_learned == 1 && o.accel == 1 && o.precision == SCIHPC_PREC_MIXED && o.max_cycles > 0;
+      learned_enqueue(bnorm);
+      if (!overlap) {
+        if (learned_decide(st, bnorm, 1.0, o.use_learned == 2)) {
+          zero = false;
+        } else {  // fall back to plain multigrid from p = 0
+          SCIHPC_HIP(hipMemsetAsync(l0.sol, 0, sizeof(double) * l0.n, stream));
+          zero = true;
+        }
       }
     }
     if (o.accel == 1) {
-      solve_fcg(o, st, zero, bnorm, scale);
+      solve_fcg(o, st, zero, bnorm, scale, overlap);
       cycles = st.cycles;
       rr = last_rr_;
     } else if (o.precision == SCIHPC_PREC_FP64) {
@@ -2293,69 +2324,127 @@ struct scihpc_mg {
     SCIHPC_CHECK_LAUNCH();
   }
 
-  // learned initial guess: p0 = alpha * S P S N(b, ln rho) where N runs on
-  // the hook's level, S is two red-black sweeps on a level (against the
-  // child-averaged b), P prolongs one level up, and alpha = (p.b)/(p.A p) is
-  // the energy-optimal scale.  It is kept only if ||b - A p0|| < ||b|| in
-  // FP64 (provisional: kept unjudged whenever alpha is finite); otherwise the
-  // solve starts from p = 0 (plain multigrid).
-  bool learned_initial_guess(scihpc_solve_stats &st, double bnorm, bool provisional) {
-    auto &l0 = d[0];
-    const int lvl = learned.level;
-    auto &a = d[lvl];
-    const long hw = (long)a.nx * a.ny;
-    if (!lfeat) {
-      lfeat = dalloc<float>(2 * hw, stream);
-      lout = dalloc<float>(hw, stream);
-    }
-    const double sigma = bnorm / std::sqrt((double)l0.n);
-    learned_features(lvl, sigma, lfeat, nullptr);
-    unet_infer(learned.net, lfeat, lout, stream);
-    // b on the levels down to the network's (child averages of b, sol = 0)
-    SCIHPC_HIP(hipMemsetAsync(l0.sol, 0, sizeof(double) * l0.n, stream));
-    for (int l = 0; l < lvl; ++l) residual_restrict(d[l], d[l + 1]);
-    k_net_to_level<<<ceil_div(hw, kBlock), kBlock, 0, stream>>>(lout, a.sol, hw,
-                                                                 sigma * (double)unet_out_scale(learned.net));
-    SCIHPC_CHECK_LAUNCH();
-    // two red-black sweeps on every level from the network's up to level 0
-    // remove the network's pixel-scale noise and the interpolation's kinks
-    sweep(a, 0, 0);
-    sweep(a, 0, 0);
-    for (int l = lvl - 1; l >= 0; --l) {
-      SCIHPC_HIP(hipMemsetAsync(d[l].sol, 0, sizeof(double) * d[l].n, stream));
-      prolong_add(d[l + 1], d[l]);
-      sweep(d[l], 0, 0);
-      sweep(d[l], 0, 0);
-    }
-    if (bc == SCIHPC_MG_NEUMANN) sub_mean(l0.sol, l0.n);
-    const unsigned nb = ceil_div(l0.n, kBlock);
-    if (ndim == 2) {
-      if (variable) k_guess_dots<2, true><<<nb, kBlock, 0, stream>>>(view(l0), l0.sol, partial, (int)nb);
-      else k_guess_dots<2, false><<<nb, kBlock, 0, stream>>>(view(l0), l0.sol, partial, (int)nb);
-    } else {
-      if (variable) k_guess_dots<3, true><<<nb, kBlock, 0, stream>>>(view(l0), l0.sol, partial, (int)nb);
-      else k_guess_dots<3, false><<<nb, kBlock, 0, stream>>>(view(l0), l0.sol, partial, (int)nb);
-    }
-    SCIHPC_CHECK_LAUNCH();
-    SCIHPC_HIP(hipMemcpyAsync(host_part, partial, sizeof(double) * 4 * nb, hipMemcpyDeviceToHost, stream));
+  // ------------------------------------------------ learned guess
+  // p0 = alpha * S P S N(b, ln rho): N runs on the hook's level, S is two
+  // red-black sweeps on a level (against the child-averaged b), P prolongs
+  // one level up, alpha = (p.b)/(p.A p) is the energy-optimal scale.  The
+  // pipeline is enqueued on its own stream with its own reduction buffers,
+  // so in the mixed flexible-CG solve (whose V-cycle works on the FP32
+  // levels) it runs while the iterations from p = 0 run; when it is ready the
+  // guess replaces the iterate only if its FP64 residual is below both ||b||
+  // and the iterate's.  Otherwise (FP64 V-cycles share the FP64 levels) it
+  // runs first and is kept only if its residual is below ||b||.
+  hipStream_t lstream = nullptr;
+  hipEvent_t lev = nullptr;
+  double *lpartial = nullptr, *lpartial2 = nullptr, *lscal = nullptr, *lhost = nullptr;
+  bool learned_pending = false;
+
+  void ensure_learned_buffers() {
+    if (lstream) return;
+    SCIHPC_HIP(hipStreamCreateWithFlags(&lstream, hipStreamNonBlocking));
+    SCIHPC_HIP(hipEventCreateWithFlags(&lev, hipEventDisableTiming));
+    const long nb = (long)ceil_div(d[0].n, kBlock);
+    lpartial = dalloc<double>(4 * (nb + 64), stream);
+    lpartial2 = dalloc<double>(4 * ((long)ceil_div(d[0].n, (long)kBlock * kBlock) + 64), stream);
+    lscal = dalloc<double>(S_COUNT, stream);
+    SCIHPC_HIP(hipHostMalloc(reinterpret_cast<void **>(&lhost), sizeof(double) * 4 * (nb + 64),
+                             hipHostMallocDefault));
     SCIHPC_HIP(hipStreamSynchronize(stream));
+  }
+  void ensure_learned_io() {
+    if (lfeat) return;
+    const long hw = (long)d[learned.level].nx * d[learned.level].ny;
+    lfeat = dalloc<float>(2 * hw, stream);
+    lout = dalloc<float>(hw, stream);
+    SCIHPC_HIP(hipStreamSynchronize(stream));
+  }
+
+  // enqueue the guess into l0.sol (and its four dots into lhost) on lstream
+  void learned_enqueue(double bnorm) {
+    ensure_learned_buffers();
+    ensure_learned_io();
+    SCIHPC_HIP(hipEventRecord(lev, stream));  // b as the main stream left it
+    SCIHPC_HIP(hipStreamWaitEvent(lstream, lev, 0));
+    auto swap_all = [&] {
+      std::swap(stream, lstream);
+      std::swap(partial, lpartial);
+      std::swap(partial2, lpartial2);
+      std::swap(scal, lscal);
+      std::swap(host_part, lhost);
+    };
+    swap_all();  // every helper below now enqueues on lstream with its buffers
+    try {
+      auto &l0 = d[0];
+      const int lvl = learned.level;
+      auto &a = d[lvl];
+      const long hw = (long)a.nx * a.ny;
+      const double sigma = bnorm / std::sqrt((double)l0.n);
+      learned_features(lvl, sigma, lfeat, nullptr);
+      unet_infer(learned.net, lfeat, lout, stream);
+      // b on the levels down to the network's (child averages of b, sol = 0)
+      SCIHPC_HIP(hipMemsetAsync(l0.sol, 0, sizeof(double) * l0.n, stream));
+      for (int l = 0; l < lvl; ++l) residual_restrict(d[l], d[l + 1]);
+      k_net_to_level<<<ceil_div(hw, kBlock), kBlock, 0, stream>>>(lout, a.sol, hw,
+                                                                   sigma * (double)unet_out_scale(learned.net));
+      SCIHPC_CHECK_LAUNCH();
+      sweep(a, 0, 0);
+      sweep(a, 0, 0);
+      for (int l = lvl - 1; l >= 0; --l) {
+        SCIHPC_HIP(hipMemsetAsync(d[l].sol, 0, sizeof(double) * d[l].n, stream));
+        prolong_add(d[l + 1], d[l]);
+        sweep(d[l], 0, 0);
+        sweep(d[l], 0, 0);
+      }
+      if (bc == SCIHPC_MG_NEUMANN) sub_mean(l0.sol, l0.n);
+      const unsigned nb = ceil_div(l0.n, kBlock);
+      if (ndim == 2) {
+        if (variable) k_guess_dots<2, true><<<nb, kBlock, 0, stream>>>(view(l0), l0.sol, partial, (int)nb);
+        else k_guess_dots<2, false><<<nb, kBlock, 0, stream>>>(view(l0), l0.sol, partial, (int)nb);
+      } else {
+        if (variable) k_guess_dots<3, true><<<nb, kBlock, 0, stream>>>(view(l0), l0.sol, partial, (int)nb);
+        else k_guess_dots<3, false><<<nb, kBlock, 0, stream>>>(view(l0), l0.sol, partial, (int)nb);
+      }
+      SCIHPC_CHECK_LAUNCH();
+      SCIHPC_HIP(hipMemcpyAsync(host_part, partial, sizeof(double) * 4 * nb, hipMemcpyDeviceToHost, stream));
+    } catch (...) {
+      swap_all();
+      throw;
+    }
+    swap_all();
+    SCIHPC_HIP(hipEventRecord(lev, lstream));
+    learned_pending = true;
+  }
+
+  // judge the pending guess: kept if its relative FP64 residual is below
+  // `threshold` (or always when `force`); a kept guess is scaled by alpha in
+  // l0.sol on the main stream
+  bool learned_decide(scihpc_solve_stats &st, double bnorm, double threshold, bool force) {
+    SCIHPC_REQUIRE(learned_pending, "no learned guess pending");
+    learned_pending = false;
+    SCIHPC_HIP(hipEventSynchronize(lev));
+    const unsigned nb = ceil_div(d[0].n, kBlock);
     double s[4] = {0, 0, 0, 0};
     for (int k = 0; k < 4; ++k)
-      for (unsigned q = 0; q < nb; ++q) s[k] += host_part[k * nb + q];
+      for (unsigned q = 0; q < nb; ++q) s[k] += lhost[k * nb + q];
     const double b2 = bnorm * bnorm;
     const double alpha = s[1] != 0.0 ? s[0] / s[1] : 0.0;
     const double r2 = b2 - 2.0 * alpha * s[2] + alpha * alpha * s[3];
-    st.learned_residual = std::isfinite(r2) && b2 > 0.0 ? std::sqrt(std::fabs(r2) / b2) : -1.0;
-    if (std::isfinite(alpha) && std::isfinite(r2) && alpha != 0.0 && (provisional || r2 < b2)) {
-      k_scale<<<nb, kBlock, 0, stream>>>(l0.sol, l0.n, alpha);
+    const bool finite = std::isfinite(alpha) && std::isfinite(r2) && alpha != 0.0;
+    const double rel = finite && b2 > 0.0 ? std::sqrt(std::fabs(r2) / b2) : -1.0;
+    st.learned_residual = rel;
+    const bool keep = rel < threshold;
+    if (finite && (force || keep)) {
+      SCIHPC_HIP(hipStreamWaitEvent(stream, lev, 0));
+      k_scale<<<nb, kBlock, 0, stream>>>(d[0].sol, d[0].n, alpha);
       SCIHPC_CHECK_LAUNCH();
-      if (!provisional) st.learned_accepted++;
+      if (!force) st.learned_accepted++;
       return true;
     }
     st.learned_rejected++;
     return false;
   }
 };
+
 
 // ================================================================ C ABI
 namespace scihpc {
@@ -2471,6 +2560,7 @@ extern "C" int scihpc_mg_create(int ndim, int nx, int ny, int nz, double dx, dou
 extern "C" int scihpc_mg_destroy(scihpc_mg *h) {
   if (!h) return SCIHPC_OK;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
+  if (h->lstream) (void)hipStreamSynchronize(h->lstream);
   auto fr = [](void *p) {
     if (p) (void)hipFree(p);
   };
@@ -2510,6 +2600,12 @@ extern "C" int scihpc_mg_destroy(scihpc_mg *h) {
   fr(h->p_alt);
   fr(h->rho0);
   fr(h->rho0f);
+  fr(h->lpartial);
+  fr(h->lpartial2);
+  fr(h->lscal);
+  if (h->lhost) (void)hipHostFree(h->lhost);
+  if (h->lev) (void)hipEventDestroy(h->lev);
+  if (h->lstream) (void)hipStreamDestroy(h->lstream);
   fr(h->lfeat);
   fr(h->lout);
   if (h->host_scal) (void)hipHostFree(h->host_scal);
@@ -2769,6 +2865,7 @@ extern "C" int scihpc_mg_attach_learned(scihpc_mg *h, scihpc_unet *net, int leve
     SCIHPC_REQUIRE(a.nx * a.deg == h->d[0].nx && a.ny * a.deg == h->d[0].ny, "level is not a block coarsening");
     if (h->lfeat && (h->learned.level != level)) {
       SCIHPC_HIP(hipStreamSynchronize(h->stream));
+      if (h->lstream) SCIHPC_HIP(hipStreamSynchronize(h->lstream));
       SCIHPC_HIP(hipFree(h->lfeat));
       SCIHPC_HIP(hipFree(h->lout));
       h->lfeat = h->lout = nullptr;

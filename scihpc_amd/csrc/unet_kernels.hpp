@@ -71,26 +71,41 @@ __global__ void __launch_bounds__(kConvThreads)
 #pragma unroll
   for (int nb = 0; nb < NB; ++nb) acc[nb] = floatx4{0.f, 0.f, 0.f, 0.f};
   const int chunks = s.cin / 16;
-  for (int t = 0; t < 9; ++t) {
-    const int yy = row + t / 3 - 1, xx = px + t % 3 - 1;
-    const bool in = yy >= 0 && yy < s.h && xx >= 0 && xx < s.w;
-    const long pix = ((long)img * s.h + (in ? yy : 0)) * s.w + (in ? xx : 0);
-    for (int cc = 0; cc < chunks; ++cc) {
-      bf16x4 a = {0, 0, 0, 0};
-      if (in) {
-        a = *reinterpret_cast<const bf16x4 *>(x.p + pix * x.stride + x.off + cc * 16 + kq);
-        if (mask.p) {
-          const bf16_t *m = mask.p + pix * mask.stride + mask.off + cc * 16 + kq;
+  const int K = 9 * chunks;  // k steps of 16: tap-major, then input-channel chunk
+  // the operands of G consecutive k steps are loaded before their MFMAs, so
+  // G loads are in flight at once instead of one dependent load per step
+  constexpr int G = 4;
+  for (int k0 = 0; k0 < K; k0 += G) {
+    bf16x4 a[G], bw[G][NB];
 #pragma unroll
-          for (int q = 0; q < 4; ++q)
-            if (!(bf16_to_f(m[q]) > 0.f)) a[q] = 0;
+    for (int g = 0; g < G; ++g) {
+      const int kk = k0 + g;
+      a[g] = bf16x4{0, 0, 0, 0};
+      if (kk < K) {
+        const int t = kk / chunks, cc = kk - t * chunks;
+        const int yy = row + t / 3 - 1, xx = px + t % 3 - 1;
+        if (yy >= 0 && yy < s.h && xx >= 0 && xx < s.w) {
+          const long pix = ((long)img * s.h + yy) * s.w + xx;
+          a[g] = *reinterpret_cast<const bf16x4 *>(x.p + pix * x.stride + x.off + cc * 16 + kq);
+          if (mask.p) {
+            const bf16_t *m = mask.p + pix * mask.stride + mask.off + cc * 16 + kq;
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+              if (!(bf16_to_f(m[q]) > 0.f)) a[g][q] = 0;
+          }
         }
-      }
 #pragma unroll
-      for (int nb = 0; nb < NB; ++nb) {
-        const bf16x4 bb =
-            *reinterpret_cast<const bf16x4 *>(w + ((long)(nb * 16 + (lane & 15)) * 9 + t) * s.cin + cc * 16 + kq);
-        acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bb, acc[nb], 0, 0, 0);
+        for (int nb = 0; nb < NB; ++nb)
+          bw[g][nb] =
+              *reinterpret_cast<const bf16x4 *>(w + ((long)(nb * 16 + (lane & 15)) * 9 + t) * s.cin + cc * 16 + kq);
+      }
+    }
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      if (k0 + g < K) {
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb)
+          acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[g], bw[g][nb], acc[nb], 0, 0, 0);
       }
     }
   }
