@@ -147,6 +147,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-3d", action="store_true", help="skip the 256^3 3-D solves")
+    ap.add_argument("--only-3d", action="store_true", help="run only the 256^3 3-D solves (profiling)")
     ap.add_argument("--no-unet", action="store_true", help="skip the U-Net training-step timing")
     ap.add_argument("--kernel-only", action="store_true",
                     help="only launch the dominant kernel (and a 256 MiB copy for calibration): "
@@ -168,6 +169,9 @@ def main():
 
     if args.kernel_only:
         kernel_only(args)
+        return
+    if args.only_3d:
+        print(json.dumps(bench_3d(args)), flush=True)
         return
     log("bench: building the 512x1024 bubble workload")
     w = W.bubble_2d()

@@ -2188,12 +2188,21 @@ struct scihpc_mg {
       cur ^= 1;
       k_fcg_update<<<nb, kBlock, 0, stream>>>(n, x, r, fp_[cur], fq_, scal, mixed ? f[0].rhs : nullptr, partial);
       SCIHPC_CHECK_LAUNCH();
+      // the learned guess is enqueued while the first iteration runs
+      if (learned_overlap && !learned_pending) learned_enqueue(bnorm);
       rr = std::sqrt(host_sum(nb)) / scale;
       ++it;
       // the guess is judged at the end of the first iteration after which
       // its stream has finished (no wait): it replaces the iterate only if
       // its FP64 residual is below both ||b|| and the iterate's
-      if (learned_overlap && hipEventQuery(lev) == hipSuccess) {
+      bool guess_ready = false;
+      if (learned_overlap) {
+        const hipError_t q = hipEventQuery(lev);
+        if (q == hipErrorNotReady) (void)hipGetLastError();  // not an error: still running
+        else SCIHPC_HIP(q);
+        guess_ready = q == hipSuccess;
+      }
+      if (guess_ready) {
         learned_overlap = false;
         if (learned_decide(st, bnorm, std::min(rr, 1.0), false)) {
           // restart from the guess: x = p0, r = b - A p0 (mixed: b is l0.rhs)
@@ -2208,7 +2217,7 @@ struct scihpc_mg {
         }
       }
     }
-    if (learned_overlap) {  // converged before the guess was ready: unused
+    if (learned_overlap && learned_pending) {  // converged before the guess was ready: unused
       SCIHPC_HIP(hipEventSynchronize(lev));
       learned_pending = false;
       st.learned_rejected++;
@@ -2239,8 +2248,8 @@ struct scihpc_mg {
     if (o.use_learned && learned.active() && bnorm > 0.0) {
       // use_learned == 2 keeps the guess unjudged (ablation measurements)
       overlap = o.use_learned == 1 && o.accel == 1 && o.precision == SCIHPC_PREC_MIXED && o.max_cycles > 0;
-      learned_enqueue(bnorm);
       if (!overlap) {
+        learned_enqueue(bnorm);
         if (learned_decide(st, bnorm, 1.0, o.use_learned == 2)) {
           zero = false;
         } else {  // fall back to plain multigrid from p = 0
@@ -2360,11 +2369,12 @@ struct scihpc_mg {
   }
 
   // enqueue the guess into l0.sol (and its four dots into lhost) on lstream
+  // (b, rho and the FP64 faces are complete on the device when a solve
+  // starts: the host wrote them synchronously and ||b|| was read back; the
+  // main stream only reads them while the guess is formed)
   void learned_enqueue(double bnorm) {
     ensure_learned_buffers();
     ensure_learned_io();
-    SCIHPC_HIP(hipEventRecord(lev, stream));  // b as the main stream left it
-    SCIHPC_HIP(hipStreamWaitEvent(lstream, lev, 0));
     auto swap_all = [&] {
       std::swap(stream, lstream);
       std::swap(partial, lpartial);
