@@ -667,26 +667,33 @@ __global__ void __launch_bounds__(kFusedThreads)
 // ======================================================= 3-D z-streaming
 // One red-black Gauss-Seidel sweep of a 3-D level per launch.  A workgroup
 // owns a TX x TY column tile and a chunk of z planes and marches up the
-// chunk with a ring of four planes in LDS (the tile plus a 2-cell ring).
-// Step `pl` stages plane pl, updates the first colour on plane pl-1 over the
-// tile and a 1-cell ring (ring cells are computed redundantly by the
-// neighbouring tiles, from the same inputs), then the second colour on plane
-// pl-2 over the tile; plane pl-2 is then final and goes to `out`.  `in` is
-// never written, so tiles and chunks never read each other's new values.
-// The next plane is fetched into registers while the current step runs.
+// chunk with a ring of five planes in LDS (the tile plus a 2-cell ring) of
+// x and the coefficient source.  Step `pl` stages plane pl; then, in one
+// phase, it updates the first colour on plane pl-1 over the tile and a
+// 1-cell ring (ring cells are computed redundantly by the neighbouring
+// tiles, from the same inputs) and the second colour on plane pl-3 over the
+// tile -- the two read disjoint planes of the other's colour, so they need
+// no barrier between them -- and stores plane pl-3, now final, to `out`.
+// `in` is never written, so tiles and chunks never read each other's new
+// values.  Planes pl+1 and pl+2 are in flight in registers, and the right-
+// hand sides of the next step's cells are loaded one step ahead straight
+// into registers (b is read once per updated cell, so it needs no LDS).
 // A chunk also updates the first colour on the plane below and above it
 // (redundantly), which its second colour needs.
 // COEF 0: constant operator (ax, ay, az; Dirichlet wall terms as stencil());
-// COEF 1: face arrays cx, cy, cz; COEF 2: density rho with the face
-// 1 / (0.5 (rho_i + rho_j)) / h / h formed on the fly, the level-0 operator
-// of init_ppe_proj_coefficients (projection_method.cpp:404-465); cells
-// outside the domain hold rho = +inf, so wall faces vanish.
+// COEF 1: face arrays cx, cy, cz; COEF 2: density rho with the face formed
+// on the fly, the level-0 operator of init_ppe_proj_coefficients
+// (projection_method.cpp:404-465); cells outside the domain hold rho = +inf,
+// so wall faces vanish.
 constexpr int kZ3Threads = 256;
+constexpr int kZ3Ring = 5;
 
 template <int COEF>
 struct Z3 {
-  static constexpr int NA = COEF == 1 ? 5 : (COEF == 2 ? 3 : 2);  // sol, rhs, coefficient arrays
+  static constexpr int NA = COEF == 1 ? 4 : (COEF == 2 ? 2 : 1);  // x + coefficient arrays
 };
+
+__device__ __forceinline__ int z3_slot(int plane) { return (plane + 2 * kZ3Ring) % kZ3Ring; }
 
 // face between cells of density a and b, spacing h: 2 / (a + b) / h^2
 // (FP64: one IEEE division; equal to init_ppe_proj_coefficients'
@@ -733,7 +740,11 @@ __global__ void __launch_bounds__(kZ3Threads)
   constexpr int W = TX + 4, H = TY + 4, P = W * H;
   constexpr int NA = Z3<COEF>::NA;
   constexpr int PER = (P + kZ3Threads - 1) / kZ3Threads;
-  __shared__ T sm[NA][4][P];
+  // first-colour cells: tile + 1 ring, half of each row; second colour: tile
+  constexpr int RW = TX + 2, RH = TY + 2, HW0 = (RW + 1) / 2, N0 = RH * HW0;
+  constexpr int HW1 = (TX + 1) / 2, N1 = TY * HW1;
+  constexpr int M0 = (N0 + kZ3Threads - 1) / kZ3Threads, M1 = (N1 + kZ3Threads - 1) / kZ3Threads;
+  __shared__ T sm[NA][kZ3Ring][P];
   const int tiles_x = (F.nx + TX - 1) / TX;
   const int ntiles = tiles_x * ((F.ny + TY - 1) / TY);
   const int nblocks = ntiles * ((F.nz + zchunk - 1) / zchunk);
@@ -744,9 +755,6 @@ __global__ void __launch_bounds__(kZ3Threads)
   const int c0 = first, c1 = 1 - first;
   const long sxy = (long)F.nx * F.ny;
 
-  // two register buffers: plane pl+2 is in flight while plane pl+1 waits
-  // and plane pl is being computed
-  T reg0[NA][PER], reg1[NA][PER];
   auto fetch = [&](T (&reg)[NA][PER], int pl) {
 #pragma unroll
     for (int r = 0; r < PER; ++r) {
@@ -756,13 +764,12 @@ __global__ void __launch_bounds__(kZ3Threads)
       const bool inside = q < P && pl >= 0 && pl < F.nz && gx >= 0 && gx < F.nx && gy >= 0 && gy < F.ny;
       const long g = inside ? gx + (long)F.nx * gy + sxy * pl : 0;
       reg[0][r] = (inside && !zero_guess) ? in[g] : T(0);
-      reg[1][r] = inside ? F.rhs[g] : T(0);
       if constexpr (COEF == 1) {
-        reg[2][r] = inside ? F.cx[g] : T(0);
-        reg[3][r] = inside ? F.cy[g] : T(0);
-        reg[4][r] = inside ? F.cz[g] : T(0);
+        reg[1][r] = inside ? F.cx[g] : T(0);
+        reg[2][r] = inside ? F.cy[g] : T(0);
+        reg[3][r] = inside ? F.cz[g] : T(0);
       }
-      if constexpr (COEF == 2) reg[2][r] = inside ? rho[g] : T(INFINITY);
+      if constexpr (COEF == 2) reg[1][r] = inside ? rho[g] : T(INFINITY);
     }
   };
   auto commit = [&](const T (&reg)[NA][PER], int slot) {
@@ -775,81 +782,109 @@ __global__ void __launch_bounds__(kZ3Threads)
       }
     }
   };
-  auto update = [&](int q, int sc, int sb, int sf, int gx, int gy, int gz) {
+  // m-th first-colour cell of this thread on plane k (false if none)
+  auto cell0 = [&](int m, int k, int &gx, int &gy, int &q) {
+    const int t = threadIdx.x + m * kZ3Threads;
+    const int ry = t / HW0;
+    gy = y0 - 1 + ry;
+    const int rx = 2 * (t - ry * HW0) + ((c0 - (x0 - 1) - gy - k) & 1);
+    gx = x0 - 1 + rx;
+    q = (ry + 1) * W + rx + 1;
+    return t < N0 && rx < RW && gx >= 0 && gx < F.nx && gy >= 0 && gy < F.ny;
+  };
+  // m-th second-colour cell of this thread on plane k (false if none)
+  auto cell1 = [&](int m, int k, int &gx, int &gy, int &q) {
+    const int t = threadIdx.x + m * kZ3Threads;
+    const int ry = t / HW1;
+    gy = y0 + ry;
+    const int rx = 2 * (t - ry * HW1) + ((c1 - x0 - gy - k) & 1);
+    gx = x0 + rx;
+    q = (ry + 2) * W + rx + 2;
+    return t < N1 && rx < TX && gx < F.nx && gy < F.ny;
+  };
+  auto first_ok = [&](int k) { return k >= z0 - 1 && k <= z1 && k >= 0 && k < F.nz; };
+  auto second_ok = [&](int k) { return k >= z0 && k < z1; };
+  // right-hand sides of the cells a step updates (first colour on k0,
+  // second on k1), loaded ahead into registers
+  auto load_rhs = [&](T (&r0)[M0], T (&r1)[M1], int k0, int k1) {
+#pragma unroll
+    for (int m = 0; m < M0; ++m) {
+      int gx, gy, q;
+      r0[m] = (first_ok(k0) && cell0(m, k0, gx, gy, q)) ? F.rhs[gx + (long)F.nx * gy + sxy * k0] : T(0);
+    }
+#pragma unroll
+    for (int m = 0; m < M1; ++m) {
+      int gx, gy, q;
+      r1[m] = (second_ok(k1) && cell1(m, k1, gx, gy, q)) ? F.rhs[gx + (long)F.nx * gy + sxy * k1] : T(0);
+    }
+  };
+  auto update = [&](int q, int sc, int sb, int sf, int gx, int gy, int gz, T bq) {
     T cL = T(0), cR = T(0), cD = T(0), cU = T(0), cB = T(0), cF = T(0);
     if constexpr (COEF == 1) {
-      cL = sm[2][sc][q - 1];
-      cR = sm[2][sc][q];
-      cD = sm[3][sc][q - W];
-      cU = sm[3][sc][q];
-      cB = sm[4][sb][q];
-      cF = sm[4][sc][q];
+      cL = sm[1][sc][q - 1];
+      cR = sm[1][sc][q];
+      cD = sm[2][sc][q - W];
+      cU = sm[2][sc][q];
+      cB = sm[3][sb][q];
+      cF = sm[3][sc][q];
     }
     if constexpr (COEF == 2) {
-      const T rc = sm[2][sc][q];
-      cL = rho_face<T>(sm[2][sc][q - 1], rc, hx, F.ax);
-      cR = rho_face<T>(rc, sm[2][sc][q + 1], hx, F.ax);
-      cD = rho_face<T>(sm[2][sc][q - W], rc, hy, F.ay);
-      cU = rho_face<T>(rc, sm[2][sc][q + W], hy, F.ay);
-      cB = rho_face<T>(sm[2][sb][q], rc, hz, F.az);
-      cF = rho_face<T>(rc, sm[2][sf][q], hz, F.az);
+      const T rc = sm[1][sc][q];
+      cL = rho_face<T>(sm[1][sc][q - 1], rc, hx, F.ax);
+      cR = rho_face<T>(rc, sm[1][sc][q + 1], hx, F.ax);
+      cD = rho_face<T>(sm[1][sc][q - W], rc, hy, F.ay);
+      cU = rho_face<T>(rc, sm[1][sc][q + W], hy, F.ay);
+      cB = rho_face<T>(sm[1][sb][q], rc, hz, F.az);
+      cF = rho_face<T>(rc, sm[1][sf][q], hz, F.az);
     }
     T off, diag;
     z3_stencil<T, COEF>(F, gx, gy, gz, sm[0][sc][q - 1], sm[0][sc][q + 1], sm[0][sc][q - W], sm[0][sc][q + W],
                         sm[0][sb][q], sm[0][sf][q], cL, cR, cD, cU, cB, cF, off, diag);
-    sm[0][sc][q] = gs_div<T>(sm[1][sc][q] - off, diag);
+    sm[0][sc][q] = gs_div<T>(bq - off, diag);
   };
 
-  const int zs = z0 - 2, ze = z1 + 1;  // planes staged
-  auto step = [&](int pl) {
-      // first colour on plane k = pl - 1, tile + 1-cell ring
-      const int k = pl - 1;
-      if (k >= z0 - 1 && k >= 0 && k < F.nz) {
-        const int sc = (k + 4) & 3, sb = (k + 3) & 3, sf = (k + 5) & 3;
-        constexpr int RW = TX + 2, RH = TY + 2, HW = (RW + 1) / 2;
-        for (int t = threadIdx.x; t < RH * HW; t += kZ3Threads) {
-          const int ry = t / HW;
-          const int gy = y0 - 1 + ry;
-          const int rx = 2 * (t - ry * HW) + ((c0 - (x0 - 1) - gy - k) & 1);
-          const int gx = x0 - 1 + rx;
-          if (rx >= RW || gx < 0 || gx >= F.nx || gy < 0 || gy >= F.ny) continue;
-          update((ry + 1) * W + rx + 1, sc, sb, sf, gx, gy, k);
-        }
+  const int zs = z0 - 2;  // first plane staged
+  const int zl = z1 + 1;  // last plane staged
+  const int ze = z1 + 2;  // last step (second colour on z1 - 1)
+  auto step = [&](int pl, T (&reg)[NA][PER], const T (&r0)[M0], const T (&r1)[M1], T (&n0)[M0], T (&n1)[M1]) {
+    if (pl <= zl) commit(reg, z3_slot(pl));
+    __syncthreads();
+    if (pl + 2 <= zl) fetch(reg, pl + 2);
+    load_rhs(n0, n1, pl, pl - 2);  // the next step's cells
+    const int k = pl - 1, k3 = pl - 3;
+    if (first_ok(k)) {
+#pragma unroll
+      for (int m = 0; m < M0; ++m) {
+        int gx, gy, q;
+        if (cell0(m, k, gx, gy, q)) update(q, z3_slot(k), z3_slot(k - 1), z3_slot(k + 1), gx, gy, k, r0[m]);
       }
-      __syncthreads();
-      // second colour on plane k1 = pl - 2, tile only; then plane k1 is final
-      const int k1 = pl - 2;
-      if (k1 >= z0 && k1 < z1) {
-        const int sc = (k1 + 4) & 3, sb = (k1 + 3) & 3, sf = (k1 + 5) & 3;
-        constexpr int HW = (TX + 1) / 2;
-        for (int t = threadIdx.x; t < TY * HW; t += kZ3Threads) {
-          const int ry = t / HW;
-          const int gy = y0 + ry;
-          const int rx = 2 * (t - ry * HW) + ((c1 - x0 - gy - k1) & 1);
-          const int gx = x0 + rx;
-          if (rx >= TX || gx >= F.nx || gy >= F.ny) continue;
-          update((ry + 2) * W + rx + 2, sc, sb, sf, gx, gy, k1);
-        }
-        __syncthreads();
-        for (int t = threadIdx.x; t < TX * TY; t += kZ3Threads) {
-          const int ry = t / TX, rx = t - ry * TX;
-          const int gx = x0 + rx, gy = y0 + ry;
-          if (gx < F.nx && gy < F.ny) out[gx + (long)F.nx * gy + sxy * k1] = sm[0][sc][(ry + 2) * W + rx + 2];
-        }
+    }
+    if (second_ok(k3)) {
+#pragma unroll
+      for (int m = 0; m < M1; ++m) {
+        int gx, gy, q;
+        if (cell1(m, k3, gx, gy, q)) update(q, z3_slot(k3), z3_slot(k3 - 1), z3_slot(k3 + 1), gx, gy, k3, r1[m]);
       }
+    }
+    __syncthreads();
+    if (second_ok(k3)) {
+      const int sc = z3_slot(k3);
+      for (int t = threadIdx.x; t < TX * TY; t += kZ3Threads) {
+        const int ry = t / TX, rx = t - ry * TX;
+        const int gx = x0 + rx, gy = y0 + ry;
+        if (gx < F.nx && gy < F.ny) out[gx + (long)F.nx * gy + sxy * k3] = sm[0][sc][(ry + 2) * W + rx + 2];
+      }
+    }
   };
+  T reg0[NA][PER], reg1[NA][PER];
+  T ra0[M0], ra1[M1], rb0[M0], rb1[M1];
   fetch(reg0, zs);
   fetch(reg1, zs + 1);
+  load_rhs(ra0, ra1, zs - 1, zs - 3);
   for (int pl = zs; pl <= ze; pl += 2) {
-    commit(reg0, (pl + 4) & 3);
-    __syncthreads();
-    if (pl + 2 <= ze) fetch(reg0, pl + 2);
-    step(pl);
+    step(pl, reg0, ra0, ra1, rb0, rb1);
     if (pl + 1 > ze) break;
-    commit(reg1, (pl + 5) & 3);
-    __syncthreads();
-    if (pl + 3 <= ze) fetch(reg1, pl + 3);
-    step(pl + 1);
+    step(pl + 1, reg1, rb0, rb1, ra0, ra1);
   }
 }
 
