@@ -167,6 +167,45 @@ SCIHPC_API int scihpc_mg_time_kernel(scihpc_mg *h, int which, int iters, double 
                           double *bytes);
 
 /* ------------------------------------------------------------------ */
+/* Projection step on padded fields (scalar_data layout), around a
+ * multigrid handle of the same interior grid.  Velocities are the MAC face
+ * components u, v, w; their BCs are applied after the correction as
+ * wrapper::apply_vel_bc (wrapper.cpp:21-26, SLIP made NO_SLIP for the
+ * wall-normal component, wrapper.cpp:111-125); the pressure gets Neumann
+ * BCs.  Host arrays; calls synchronise.                                 */
+typedef struct scihpc_proj scihpc_proj;
+SCIHPC_API int scihpc_proj_create(int ndim, int nx, int ny, int nz, double dx, double dy,
+                                  double dz, const int vel_bc_types[6], scihpc_proj **out);
+SCIHPC_API int scihpc_proj_destroy(scihpc_proj *h);
+
+typedef struct {
+  double dt, dt_old, dt_old2; /* problem_parameters (global.h:25-54) */
+  double rho12;               /* density_ratio */
+  double ppe_tol;             /* absolute residual of the multigrid loop */
+  int max_cycles;             /* the reference caps the loop at 100 */
+} scihpc_fpc_params;
+
+/* One FPC step: pressure_guess_solution (projection_method.cpp:803-815),
+ * init_ppe_fpc_coefficients (:467-541) and fast_pressure_correction
+ * (:659-753): p_hat from p_old, p_old2; MG rhs = rho12/dt div u + sum C
+ * p_hat; full cycles of the reference-faithful multigrid (a Neumann
+ * constant-coefficient handle whose level-0 sol is the warm start, as
+ * SciHPC's persistent at[0]->sol) until ||r|| <= ppe_tol; p, velocity
+ * correction, BCs.  Writes u, v, w, p, p_hat and the cycle count. */
+SCIHPC_API int scihpc_proj_fpc_step(scihpc_proj *h, scihpc_mg *mg, const scihpc_fpc_params *prm,
+                                    const double *rho, const double *p_old,
+                                    const double *p_old2, double *u, double *v, double *w,
+                                    double *p, double *phat, int *cycles);
+
+/* The variable-density projection (projection_method.cpp:544-657) with the
+ * SOR replaced by scihpc_mg_solve on A(rho): rhs = div u / dt (mean
+ * removed), A(rho) p = rhs, u -= dt / (rho_face h) dp, BCs.  `mg` must be a
+ * Neumann handle; rho is padded. */
+SCIHPC_API int scihpc_proj_step(scihpc_proj *h, scihpc_mg *mg, double dt,
+                                const scihpc_solve_opts *opts, const double *rho, double *u,
+                                double *v, double *w, double *p, scihpc_solve_stats *stats);
+
+/* ------------------------------------------------------------------ */
 /* Learned pressure guess (U-Net).  No reference counterpart: SciHPC has no
  * learned component.  A 4-level U-Net (16/32/64/128 channels, 3x3
  * convolutions, BF16 weights and activations, FP32 accumulation on the

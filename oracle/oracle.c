@@ -1458,3 +1458,47 @@ void orc_pressure_guess(long n, double dt_old, double dt_old2,
     phat[i] = old[i] + dt_old * dpdt;
   }
 }
+
+int orc_fpc_step(int ndim, int nx, int ny, int nz, double dx, double dy,
+                 double dz, double dt, double dt_old, double dt_old2,
+                 double rho12, double ppe_tol, int max_cycles,
+                 const int *vel_types, const double *rho, const double *p_old,
+                 const double *p_old2, double *u, double *v, double *w,
+                 double *pr, double *phat) {
+  /* projection_method.cpp:803-815 (guess), :467-541 (coefficients),
+   * :659-753 (rhs, full-cycle loop, copy, pressure BCs, velocity, velocity
+   * BCs = wrapper::apply_vel_bc, wrapper.cpp:21-26) */
+  orc_pf p = pf_make(ndim, nx, ny, nz);
+  const long tot = (long)p.N[0] * p.N[1] * p.N[2];
+  orc_pressure_guess(tot, dt_old, dt_old2, p_old, p_old2, phat);
+  double *C = (double *)calloc((size_t)8 * tot, sizeof(double));
+  orc_fpc_coefficients(ndim, nx, ny, nz, dx, dy, dz, rho12, rho, phat, C, C + tot, C + 2 * tot, C + 3 * tot,
+                       C + 4 * tot, C + 5 * tot, C + 6 * tot, C + 7 * tot);
+  void *m = orc_mg_create(ndim, nx, ny, ndim > 2 ? nz : 1, dx, dy, dz, 0);
+  orc_fpc_rhs(ndim, nx, ny, nz, dx, dy, dz, dt, rho12, u, v, w, C + 7 * tot, orc_mg_vec(m, 0, 0));
+  double err = orc_mg_residual(m, 0);
+  int iter = 0;
+  while (err > ppe_tol && iter < max_cycles) {
+    ++iter;
+    orc_mg_fullcycle_ref(m);
+    err = orc_mg_residual(m, 0);
+  }
+  orc_mg_to_padded(ndim, nx, ny, nz, orc_mg_vec(m, 0, 1), pr);
+  const int nt[6] = {0, 0, 0, 0, 0, 0};
+  const double zero[6] = {0, 0, 0, 0, 0, 0};
+  orc_bc_apply(ndim, nx, ny, nz, dx, dy, dz, nt, zero, -1, pr);
+  orc_fpc_velocity(ndim, nx, ny, nz, dx, dy, dz, dt, rho12, rho, pr, phat, u, v, w);
+  /* the velocity wrapper turns SLIP into NO_SLIP for the wall-normal
+   * component (wrapper.cpp:111-125 and the 3-D constructor) */
+  double *comp[3] = {u, v, w};
+  for (int a = 0; a < 3; ++a) {
+    int t[6];
+    for (int q = 0; q < 6; ++q) t[q] = vel_types[q];
+    for (int q = 2 * a; q < 2 * a + 2; ++q)
+      if (t[q] == 4) t[q] = 3;
+    orc_bc_apply(ndim, nx, ny, nz, dx, dy, dz, t, zero, a, comp[a]);
+  }
+  orc_mg_destroy(m);
+  free(C);
+  return iter;
+}

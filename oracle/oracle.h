@@ -156,13 +156,14 @@ int orc_projection_sor(int ndim, int nx, int ny, int nz, double omega,
                        double *tmp);
 
 /* projection_method.cpp:467-541 FPC coefficients and RHS_fpc = sum C*p_hat
- * (p_hat = pressure->scalar->flux).                            [unpinned] */
+ * (p_hat = pressure->scalar->flux).                            [golden] */
 void orc_fpc_coefficients(int ndim, int nx, int ny, int nz, double dx,
                           double dy, double dz, double rho12,
                           const double *rho, const double *phat, double *CC,
                           double *CR, double *CL, double *CU, double *CD,
                           double *CF, double *CB, double *RHS);
-/* projection_method.cpp:667-692 MG rhs (x-fastest) = rho12/dt*div u + RHS */
+/* projection_method.cpp:667-692 MG rhs (x-fastest) = rho12/dt*div u + RHS
+ *                                                 [golden via orc_fpc_step] */
 void orc_fpc_rhs(int ndim, int nx, int ny, int nz, double dx, double dy,
                  double dz, double dt, double rho12, const double *u,
                  const double *v, const double *w, const double *RHS,
@@ -172,14 +173,28 @@ void orc_mg_to_padded(int ndim, int nx, int ny, int nz, const double *sol,
                       double *p);
 void orc_padded_to_mg(int ndim, int nx, int ny, int nz, const double *p,
                       double *sol);
-/* projection_method.cpp:723-750 FPC velocity correction          [unpinned] */
+/* projection_method.cpp:723-750 FPC velocity correction [golden via orc_fpc_step] */
 void orc_fpc_velocity(int ndim, int nx, int ny, int nz, double dx, double dy,
                       double dz, double dt, double rho12, const double *rho,
                       const double *p, const double *phat, double *u,
                       double *v, double *w);
-/* projection_method.cpp:803-815 pressure guess p_hat           [unpinned] */
+/* projection_method.cpp:803-815 pressure guess p_hat  [golden via orc_fpc_step] */
 void orc_pressure_guess(long n, double dt_old, double dt_old2,
                         const double *old, const double *old2, double *phat);
+/* One FPC projection step: pressure_guess_solution, init_ppe_fpc_coefficients,
+ * fast_pressure_correction (projection_method.cpp:467-541, 659-753, 803-815)
+ * with the reference-faithful multigrid started from p = 0 (full cycles
+ * until the absolute residual <= ppe_tol, at most max_cycles), Neumann
+ * pressure BCs and staggered velocity BCs of vel_types (SLIP becomes
+ * NO_SLIP for the wall-normal component, as the velocity wrapper sets it,
+ * wrapper.cpp:111-125).  Returns cycles.
+ *                                                              [golden] */
+int orc_fpc_step(int ndim, int nx, int ny, int nz, double dx, double dy,
+                 double dz, double dt, double dt_old, double dt_old2,
+                 double rho12, double ppe_tol, int max_cycles,
+                 const int *vel_types, const double *rho, const double *p_old,
+                 const double *p_old2, double *u, double *v, double *w,
+                 double *pr, double *phat);
 
 #ifdef __cplusplus
 }

@@ -91,6 +91,7 @@ class Oracle:
         _sig(L, "orc_fpc_velocity", None,
              [_i, _i, _i, _i, _d, _d, _d, _d, _d, _dp, _dp, _dp, _dp, _dp, _dp])
         _sig(L, "orc_pressure_guess", None, [_l, _d, _d, _dp, _dp, _dp])
+        _sig(L, "orc_fpc_step", _i, [_i, _i, _i, _i, _d, _d, _d, _d, _d, _d, _d, _d, _i, _ip] + [_dp] * 8)
 
     # -------------------------------------------------------------- MG
     def levels(self, ndim, nx, ny, nz=1):
@@ -209,6 +210,7 @@ class Reference:
         _sig(L, "ref_projection", None,
              [_i, _i, _i, _i, _dp, _d, _i, _d, _d, _dp, _dp, _dp, _dp, _dp, _dp])
         _sig(L, "ref_fpc_coefficients", None, [_i, _i, _i, _i, _dp, _d, _dp, _dp, _dp])
+        _sig(L, "ref_fpc_step", _i, [_i, _i, _i, _i, _dp, _d, _d, _d, _d, _d, _ip] + [_dp] * 8)
 
     def mg(self, ndim, nx, ny, nz, ext, bc="neumann"):
         return RefMG(self, ndim, nx, ny, nz, ext, BC[bc])

@@ -332,4 +332,53 @@ void ref_fpc_coefficients(int ndim, int nx, int ny, int nz, const double *ext, d
   for (int a = 0; a < 8; ++a) from_ref(s, arrs[a], coef + a * tot);
 }
 
+// -------------------------------------------------------------- FPC step
+// pressure_guess_solution + init_ppe_fpc_coefficients +
+// fast_pressure_correction on a fresh pressure wrapper (Neumann BCs, its
+// multigrid starting from p = 0) and face velocities with the given BC
+// types.  Outputs the pressure, p_hat (the pressure's flux array), the
+// corrected velocities (all padded) and the multigrid cycles.
+int ref_fpc_step(int ndim, int nx, int ny, int nz, const double *ext, double dt, double dt_old,
+                 double dt_old2, double rho12, double ppe_tol, const int *vel_types, const double *rho,
+                 const double *p_old, const double *p_old2, double *u, double *v, double *w, double *p,
+                 double *phat) {
+  structured_grid *geo = make_grid(ndim, nx, ny, nz, ext);
+  int nt[6] = {0, 0, 0, 0, 0, 0};
+  double vv[6] = {0, 0, 0, 0, 0, 0};
+  wrapper *pressure = make_wrapper(true, geo, ndim, nt, vv);
+  wrapper *lsf = make_wrapper(true, geo, ndim, nt, vv);
+  wrapper *vel = make_wrapper(false, geo, ndim, vel_types, vv);
+  auto *param = new problem_parameters();
+  param->dt = dt;
+  param->dt_old = dt_old;
+  param->dt_old2 = dt_old2;
+  param->density_ratio = rho12;
+  param->ppe_tol = ppe_tol;
+  auto *dummy = dummy_data_alloc(lsf->scalar);
+  auto *solvers = SharedSolvers_alloc(pressure->scalar, geo);
+  shared_solvers_mg_init_Neumann(solvers);
+  for (wrapper *x : {pressure, lsf, vel}) {
+    x->link_params(param);
+    x->link_dummy(dummy);
+  }
+  pressure->link_solvers(solvers);
+  scalar_data *s = pressure->scalar;
+  to_ref(s, dummy->density, rho);
+  to_ref(s, s->old, p_old);
+  to_ref(s, s->old2, p_old2);
+  to_ref(s, vel->vector->x.data, u);
+  to_ref(s, vel->vector->y.data, v);
+  to_ref(s, vel->vector->z.data, w);
+  projection_method pm(s);
+  pm.pressure_guess_solution(pressure);
+  pm.init_ppe_fpc_coefficients(pressure);
+  pm.fast_pressure_correction(pressure, lsf, vel);
+  from_ref(s, s->data, p);
+  from_ref(s, s->flux, phat);
+  from_ref(s, vel->vector->x.data, u);
+  from_ref(s, vel->vector->y.data, v);
+  from_ref(s, vel->vector->z.data, w);
+  return 0;
+}
+
 }  // extern "C"

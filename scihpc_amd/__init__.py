@@ -14,9 +14,10 @@ import ctypes as C
 import numpy as np
 
 from ._capi import (BC_CODES, MG_DIRICHLET, MG_NEUMANN, NOT_CONVERGED, OK, PREC_FP64,
-                    PREC_MIXED, ScihpcError, SolveOpts, SolveStats, check, lib)
+                    PREC_MIXED, FpcParams, ScihpcError, SolveOpts, SolveStats, check, lib)
 
-__all__ = ["Multigrid", "LevelSet", "UNet", "ScihpcError", "PREC_FP64", "PREC_MIXED", "padded_shape"]
+__all__ = ["Multigrid", "LevelSet", "Projection", "UNet", "ScihpcError", "PREC_FP64", "PREC_MIXED",
+           "padded_shape"]
 
 GHC = 3
 
@@ -329,3 +330,53 @@ class LevelSet:
         ms, b = C.c_double(), C.c_double()
         check(self._lib.scihpc_ls_time_step(self.h, dt, iters, C.byref(ms), C.byref(b)))
         return ms.value, b.value
+
+
+class Projection:
+    """The projection step on padded fields around a Multigrid of the same
+    grid: the FPC step of projection_method (fast_pressure_correction,
+    projection_method.cpp:659-753, with pressure_guess_solution :803-815 and
+    init_ppe_fpc_coefficients :467-541) and the variable-density projection
+    (projection_method.cpp:544-657) with the A(rho) solve."""
+
+    def __init__(self, ndim, nx, ny, nz=1, dx=1.0, dy=1.0, dz=1.0, vel_bc_types=None):
+        self._lib = lib()
+        types = [BC_CODES[t] if isinstance(t, str) else int(t) for t in (vel_bc_types or ["no_slip"] * 6)]
+        types = np.ascontiguousarray((types + [0] * 6)[:6], np.int32)
+        h = C.c_void_p()
+        check(self._lib.scihpc_proj_create(ndim, nx, ny, nz, dx, dy, dz, types, C.byref(h)))
+        self.h = h
+        self.shape = padded_shape(ndim, nx, ny, nz)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._lib.scihpc_proj_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def fpc_step(self, mg, rho, p_old, p_old2, u, v, w, dt, dt_old, dt_old2, rho12, ppe_tol=1e-4,
+                 max_cycles=100):
+        """Returns (u, v, w, p, p_hat, cycles) as padded arrays."""
+        u, v, w = (_f64(a).copy() for a in (u, v, w))
+        p = np.zeros_like(u)
+        ph = np.zeros_like(u)
+        prm = FpcParams(dt, dt_old, dt_old2, rho12, ppe_tol, max_cycles)
+        cyc = C.c_int()
+        check(self._lib.scihpc_proj_fpc_step(self.h, mg.h, C.byref(prm), _f64(rho), _f64(p_old), _f64(p_old2),
+                                             u, v, w, p, ph, C.byref(cyc)))
+        return u, v, w, p, ph, cyc.value
+
+    def step(self, mg, rho, u, v, w, dt, rel_tol=1e-8, precision=PREC_FP64, accel=1, max_cycles=200):
+        """Variable-density projection; returns (u, v, w, p, stats)."""
+        u, v, w = (_f64(a).copy() for a in (u, v, w))
+        p = np.zeros_like(u)
+        o = SolveOpts(rel_tol, max_cycles, 2, 2, precision, 0, 1, accel)
+        s = SolveStats()
+        rc = check(self._lib.scihpc_proj_step(self.h, mg.h, dt, C.byref(o), _f64(rho), u, v, w, p, C.byref(s)),
+                   allow=(OK, NOT_CONVERGED))
+        return u, v, w, p, {"converged": rc == OK, "cycles": s.cycles, "rel_residual": s.rel_residual}

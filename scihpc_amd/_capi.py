@@ -36,6 +36,11 @@ class SolveStats(C.Structure):
                 ("learned_residual", C.c_double)]
 
 
+class FpcParams(C.Structure):
+    _fields_ = [("dt", C.c_double), ("dt_old", C.c_double), ("dt_old2", C.c_double),
+                ("rho12", C.c_double), ("ppe_tol", C.c_double), ("max_cycles", C.c_int)]
+
+
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
@@ -105,6 +110,10 @@ SIGNATURES = {
     "scihpc_unet_sync": [_vp],
     "scihpc_unet_time": [_vp, _i, _i, _i, _pd, _pd],
     "scihpc_conv3x3_forward": [_u16p, _i, _i, _i, _i, _u16p, _fpn, _i, _i, _u16p],
+    "scihpc_proj_create": [_i, _i, _i, _i, _d, _d, _d, _ip, C.POINTER(_vp)],
+    "scihpc_proj_destroy": [_vp],
+    "scihpc_proj_fpc_step": [_vp, _vp, C.POINTER(FpcParams)] + [_dp] * 8 + [_pi],
+    "scihpc_proj_step": [_vp, _vp, _d, C.POINTER(SolveOpts)] + [_dp] * 5 + [C.POINTER(SolveStats)],
     "scihpc_ls_create": [_i, _i, _i, _i, _d, _d, _d, _ip, _dp, C.POINTER(_vp)],
     "scihpc_ls_destroy": [_vp],
     "scihpc_ls_set_phi": [_vp, _dp],

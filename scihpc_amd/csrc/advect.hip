@@ -281,6 +281,152 @@ __global__ void __launch_bounds__(kLsBlock) k_density(const double *__restrict__
   rho[c] = positive_ref ? h + (1.0 - h) * ratio : (1.0 - h) + h * ratio;
 }
 
+// ------------------------------------------------- projection step
+// The projection kernels round like the reference (no fused multiply-add
+// contraction), so their results are bit-identical for the same inputs.
+// padded index of interior cell (I, J, K)
+__device__ __forceinline__ long pidx(const Grid &g, int I, int J, int K) {
+  return (long)(I + g.off[0]) * g.s[0] + (long)(J + g.off[1]) * g.s[1] + (K + g.off[2]);
+}
+__device__ __forceinline__ void interior_ijk(const Grid &g, long t, int &I, int &J, int &K) {
+  // MG order: x fastest
+  I = (int)(t % g.n[0]);
+  J = (int)((t / g.n[0]) % g.n[1]);
+  K = (int)(t / ((long)g.n[0] * g.n[1]));
+}
+
+// pressure_guess_solution (projection_method.cpp:803-815), every padded cell
+__global__ void __launch_bounds__(kLsBlock) k_pressure_guess(const double *__restrict__ old,
+                                                             const double *__restrict__ old2, double *__restrict__ phat,
+                                                             long n, double dt_old, double dt_old2) {
+#pragma clang fp contract(off)  // the reference rounds every product (no FMA)
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const double dpdt = (old[c] - old2[c]) / dt_old2;
+  phat[c] = old[c] + dt_old * dpdt;
+}
+
+// MG right-hand side of the FPC step (projection_method.cpp:667-692) with
+// RHS = sum C p_hat of init_ppe_fpc_coefficients (:467-541) formed in place;
+// one thread per interior cell, written in the multigrid layout
+template <int DIM>
+__global__ void __launch_bounds__(kLsBlock) k_fpc_rhs(Grid g, const double *__restrict__ rho,
+                                                      const double *__restrict__ f, const double *__restrict__ u,
+                                                      const double *__restrict__ v, const double *__restrict__ w,
+                                                      double dt, double rho12, double *__restrict__ mg_rhs) {
+#pragma clang fp contract(off)  // the reference rounds every product (no FMA)
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.interior) return;
+  int I, J, K;
+  interior_ijk(g, t, I, J, K);
+  const long c = pidx(g, I, J, K), sx = g.s[0], sy = g.s[1], sz = g.s[2];
+  const double dx = g.h[0], dy = g.h[1], dz = g.h[2];
+  const double R = 0.5 * (rho[c] + rho[c + sx]), L = 0.5 * (rho[c] + rho[c - sx]);
+  const double U = 0.5 * (rho[c] + rho[c + sy]), D = 0.5 * (rho[c] + rho[c - sy]);
+  double CR = (1.0 - rho12 / R) / dx / dx, CL = (1.0 - rho12 / L) / dx / dx;
+  double CU = (1.0 - rho12 / U) / dy / dy, CD = (1.0 - rho12 / D) / dy / dy;
+  if (I == 0) CL = 0.0;
+  if (I == g.n[0] - 1) CR = 0.0;
+  if (J == 0) CD = 0.0;
+  if (J == g.n[1] - 1) CU = 0.0;
+  double CF = 0.0, CB = 0.0;
+  if constexpr (DIM > 2) {
+    const double Fz = 0.5 * (rho[c] + rho[c + sz]), B = 0.5 * (rho[c] + rho[c - sz]);
+    CF = (1.0 - rho12 / Fz) / dz / dz;
+    CB = (1.0 - rho12 / B) / dz / dz;
+    if (K == 0) CB = 0.0;
+    if (K == g.n[2] - 1) CF = 0.0;
+  }
+  double CC = -(CR + CL + CU + CD);
+  if constexpr (DIM > 2) CC -= (CF + CB);
+  double RHS = CC * f[c] + CR * f[c + sx] + CL * f[c - sx] + CU * f[c + sy] + CD * f[c - sy];
+  if constexpr (DIM > 2) RHS += CF * f[c + sz] + CB * f[c - sz];
+  double r = (u[c] - u[c - sx]) / dx + (v[c] - v[c - sy]) / dy;
+  if constexpr (DIM > 2) r += (w[c] - w[c - sz]) / dz;
+  r *= rho12 / dt;
+  r += RHS;
+  mg_rhs[t] = r;
+}
+
+// MG right-hand side of the variable-density projection: div(u*) / dt
+// (projection_method.cpp:546-568) in the multigrid layout; part[block] = sum
+template <int DIM>
+__global__ void __launch_bounds__(kLsBlock) k_div_rhs(Grid g, const double *__restrict__ u,
+                                                      const double *__restrict__ v, const double *__restrict__ w,
+                                                      double dt, double *__restrict__ mg_rhs,
+                                                      double *__restrict__ part) {
+#pragma clang fp contract(off)  // the reference rounds every product (no FMA)
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double r = 0.0;
+  if (t < g.interior) {
+    int I, J, K;
+    interior_ijk(g, t, I, J, K);
+    const long c = pidx(g, I, J, K);
+    r = (u[c] - u[c - g.s[0]]) / g.h[0] + (v[c] - v[c - g.s[1]]) / g.h[1];
+    if constexpr (DIM > 2) r += (w[c] - w[c - g.s[2]]) / g.h[2];
+    r /= dt;
+    mg_rhs[t] = r;
+  }
+  r = block_sum(r);
+  if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
+__global__ void __launch_bounds__(kLsBlock) k_sub_const(double *__restrict__ x, long n, double m) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) x[t] -= m;
+}
+
+// p (padded interior) <- multigrid solution (projection_method.cpp:708-720)
+__global__ void __launch_bounds__(kLsBlock) k_mg_to_padded(Grid g, const double *__restrict__ sol,
+                                                           double *__restrict__ p) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.interior) return;
+  int I, J, K;
+  interior_ijk(g, t, I, J, K);
+  p[pidx(g, I, J, K)] = sol[t];
+}
+
+// velocity corrections over the padded range i < Nx-1, j < Ny-1, k < Nz:
+// FPC (projection_method.cpp:723-750) when phat != null, else the
+// variable-density projection (projection_method.cpp:630-654)
+template <int DIM>
+__global__ void __launch_bounds__(kLsBlock) k_correct_velocity(Grid g, const double *__restrict__ rho,
+                                                               const double *__restrict__ p,
+                                                               const double *__restrict__ ph, double dt,
+                                                               double rho12, double *__restrict__ u,
+                                                               double *__restrict__ v, double *__restrict__ w) {
+#pragma clang fp contract(off)  // the reference rounds every product (no FMA)
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int Ni = g.N[0] - 1, Nj = g.N[1] - 1, Nk = g.N[2];
+  if (t >= (long)Ni * Nj * Nk) return;
+  const int k = (int)(t % Nk), j = (int)((t / Nk) % Nj), i = (int)(t / ((long)Nk * Nj));
+  const long c = (long)i * g.s[0] + (long)j * g.s[1] + k;
+  const long sx = g.s[0], sy = g.s[1], sz = g.s[2];
+  const double dx = g.h[0], dy = g.h[1], dz = g.h[2];
+  if (ph) {
+    const double px = (p[c + sx] - p[c]) / dx, px2 = (ph[c + sx] - ph[c]) / dx;
+    double density = 0.5 * (rho[c] + rho[c + sx]);
+    u[c] -= dt * ((px - px2) / rho12 + px2 / density);
+    const double py = (p[c + sy] - p[c]) / dy, py2 = (ph[c + sy] - ph[c]) / dy;
+    density = 0.5 * (rho[c] + rho[c + sy]);
+    v[c] -= dt * ((py - py2) / rho12 + py2 / density);
+    if (DIM > 2 && k < Nk - 1) {
+      const double pz = (p[c + sz] - p[c]) / dz, pz2 = (ph[c + sz] - ph[c]) / dz;
+      density = 0.5 * (rho[c] + rho[c + sz]);
+      w[c] -= dt * ((pz - pz2) / rho12 + pz2 / density);
+    }
+  } else {
+    double density = 0.5 * (rho[c] + rho[c + sx]);
+    u[c] -= dt / dx / density * (p[c + sx] - p[c]);
+    density = 0.5 * (rho[c] + rho[c + sy]);
+    v[c] -= dt / dy / density * (p[c + sy] - p[c]);
+    if (DIM > 2 && k < Nk - 1) {
+      density = 0.5 * (rho[c] + rho[c + sz]);
+      w[c] -= dt / dz / density * (p[c + sz] - p[c]);
+    }
+  }
+}
+
 }  // namespace scihpc
 
 using namespace scihpc;
@@ -507,4 +653,210 @@ extern "C" int scihpc_ls_time_step(scihpc_ls *h, double dt, int iters, double *m
     // minimal Shu-Osher traffic per step: (7 + 3 d) * 8 B per interior cell
     if (bytes) *bytes = (7.0 + 3.0 * h->g.ndim) * 8.0 * (double)h->g.interior;
   });
+}
+
+// ================================================= projection step (C ABI)
+// One projection step on the padded fields of one grid, around a multigrid
+// handle of the same interior grid: the FPC step of SciHPC
+// (pressure_guess_solution, init_ppe_fpc_coefficients,
+// fast_pressure_correction; projection_method.cpp:467-541, 659-753,
+// 803-815) with the reference-faithful full cycles, and the
+// variable-density projection (projection_method.cpp:544-657) with the SOR
+// replaced by the A(rho) multigrid solve.
+struct scihpc_proj {
+  Grid g{};
+  BcSpec pbc{};     // pressure: Neumann (the pressure wrapper of the drivers)
+  BcSpec vbc[3]{};  // velocity components: SLIP -> NO_SLIP wall-normal (wrapper.cpp:111-125)
+  hipStream_t stream = nullptr;
+  double *rho = nullptr, *pold = nullptr, *pold2 = nullptr, *phat = nullptr, *p = nullptr;
+  double *u = nullptr, *v = nullptr, *w = nullptr;
+  double *mg_rhs = nullptr, *sol = nullptr, *part = nullptr, *hpart = nullptr;
+
+  double *alloc(long n) {
+    void *q = nullptr;
+    SCIHPC_HIP(hipMalloc(&q, sizeof(double) * n));
+    SCIHPC_HIP(hipMemsetAsync(q, 0, sizeof(double) * n, stream));
+    return static_cast<double *>(q);
+  }
+  void bc(double *f, const BcSpec &b, int stag) {
+    for (int a = 0; a < g.ndim; ++a) {
+      const int t1 = (a + 1) % 3, t2 = (a + 2) % 3;
+      const long lines = (long)g.n[t1] * g.n[t2];
+      k_bc_axis<<<ceil_div(lines, kLsBlock), kLsBlock, 0, stream>>>(g, f, b, a, stag);
+      SCIHPC_CHECK_LAUNCH();
+    }
+  }
+  void h2d(double *dst, const double *src) {
+    SCIHPC_HIP(hipMemcpyAsync(dst, src, sizeof(double) * g.total, hipMemcpyHostToDevice, stream));
+  }
+  void d2h(double *dst, const double *src) {
+    SCIHPC_HIP(hipMemcpyAsync(dst, src, sizeof(double) * g.total, hipMemcpyDeviceToHost, stream));
+  }
+  void check_mg(scihpc_mg *mg) {
+    SCIHPC_REQUIRE(mg, "null multigrid handle");
+    int a = 0, b = 0, c = 0, deg = 0;
+    const int rc = scihpc_mg_level_shape(mg, 0, &a, &b, &c, &deg);
+    SCIHPC_REQUIRE(rc == SCIHPC_OK, "bad multigrid handle");
+    SCIHPC_REQUIRE(a == g.n[0] && b == g.n[1] && c == (g.ndim > 2 ? g.n[2] : 1), "multigrid grid differs");
+  }
+  void finish_pressure_and_velocity(scihpc_mg *mg, const double *ph, double dt, double rho12) {
+    SCIHPC_REQUIRE(scihpc_mg_get_sol_device(mg, sol) == SCIHPC_OK, scihpc_last_error());
+    k_mg_to_padded<<<ceil_div(g.interior, kLsBlock), kLsBlock, 0, stream>>>(g, sol, p);
+    SCIHPC_CHECK_LAUNCH();
+    bc(p, pbc, -1);
+    const long nv = (long)(g.N[0] - 1) * (g.N[1] - 1) * g.N[2];
+    if (g.ndim == 2)
+      k_correct_velocity<2><<<ceil_div(nv, kLsBlock), kLsBlock, 0, stream>>>(g, rho, p, ph, dt, rho12, u, v, w);
+    else
+      k_correct_velocity<3><<<ceil_div(nv, kLsBlock), kLsBlock, 0, stream>>>(g, rho, p, ph, dt, rho12, u, v, w);
+    SCIHPC_CHECK_LAUNCH();
+    bc(u, vbc[0], 0);
+    bc(v, vbc[1], 1);
+    bc(w, vbc[2], 2);
+  }
+};
+
+extern "C" int scihpc_proj_create(int ndim, int nx, int ny, int nz, double dx, double dy, double dz,
+                                  const int *vel_bc_types, scihpc_proj **out) {
+  return guarded([&] {
+    SCIHPC_REQUIRE(out, "out is null");
+    *out = nullptr;
+    SCIHPC_REQUIRE(ndim == 2 || ndim == 3, "ndim must be 2 or 3");
+    SCIHPC_REQUIRE(nx > 2 && ny > 2 && (ndim < 3 || nz > 2), "each axis needs > 2 cells");
+    SCIHPC_REQUIRE(dx > 0 && dy > 0 && (ndim < 3 || dz > 0), "spacings must be positive");
+    SCIHPC_REQUIRE(vel_bc_types, "vel_bc_types is null");
+    auto *h = new scihpc_proj;
+    try {
+      h->g = make_grid(ndim, nx, ny, nz, dx, dy, dz);
+      for (int q = 0; q < 6; ++q) {
+        const int t = vel_bc_types[q];
+        SCIHPC_REQUIRE(q >= 2 * ndim || (t >= 0 && t <= 4 && t != SCIHPC_BC_PERIODIC), "unsupported bc type");
+        h->pbc.type[q] = SCIHPC_BC_NEUMANN;
+        for (int a = 0; a < 3; ++a) {
+          const bool normal = q / 2 == a;
+          h->vbc[a].type[q] = (normal && t == SCIHPC_BC_SLIP) ? SCIHPC_BC_NO_SLIP : t;
+        }
+      }
+      SCIHPC_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+      const long T = h->g.total;
+      h->rho = h->alloc(T);
+      h->pold = h->alloc(T);
+      h->pold2 = h->alloc(T);
+      h->phat = h->alloc(T);
+      h->p = h->alloc(T);
+      h->u = h->alloc(T);
+      h->v = h->alloc(T);
+      h->w = h->alloc(T);
+      h->mg_rhs = h->alloc(h->g.interior);
+      h->sol = h->alloc(h->g.interior);
+      h->part = h->alloc(ceil_div(h->g.interior, kLsBlock));
+      SCIHPC_HIP(hipHostMalloc(reinterpret_cast<void **>(&h->hpart),
+                               sizeof(double) * ceil_div(h->g.interior, kLsBlock), hipHostMallocDefault));
+      SCIHPC_HIP(hipStreamSynchronize(h->stream));
+    } catch (...) {
+      scihpc_proj_destroy(h);
+      throw;
+    }
+    *out = h;
+  });
+}
+
+extern "C" int scihpc_proj_destroy(scihpc_proj *h) {
+  if (!h) return SCIHPC_OK;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  for (double *q : {h->rho, h->pold, h->pold2, h->phat, h->p, h->u, h->v, h->w, h->mg_rhs, h->sol, h->part})
+    if (q) (void)hipFree(q);
+  if (h->hpart) (void)hipHostFree(h->hpart);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return SCIHPC_OK;
+}
+
+extern "C" int scihpc_proj_fpc_step(scihpc_proj *h, scihpc_mg *mg, const scihpc_fpc_params *prm, const double *rho,
+                                    const double *p_old, const double *p_old2, double *u, double *v, double *w,
+                                    double *p, double *phat, int *cycles) {
+  return guarded([&] {
+    SCIHPC_REQUIRE(h && prm && rho && p_old && p_old2 && u && v && w && p && phat, "null argument");
+    SCIHPC_REQUIRE(prm->dt > 0 && prm->dt_old2 != 0 && prm->rho12 > 0 && prm->max_cycles >= 0, "bad parameters");
+    h->check_mg(mg);
+    const Grid &g = h->g;
+    h->h2d(h->rho, rho);
+    h->h2d(h->pold, p_old);
+    h->h2d(h->pold2, p_old2);
+    h->h2d(h->u, u);
+    h->h2d(h->v, v);
+    h->h2d(h->w, w);
+    k_pressure_guess<<<ceil_div(g.total, kLsBlock), kLsBlock, 0, h->stream>>>(h->pold, h->pold2, h->phat, g.total,
+                                                                             prm->dt_old, prm->dt_old2);
+    if (g.ndim == 2)
+      k_fpc_rhs<2><<<ceil_div(g.interior, kLsBlock), kLsBlock, 0, h->stream>>>(g, h->rho, h->phat, h->u, h->v, h->w,
+                                                                             prm->dt, prm->rho12, h->mg_rhs);
+    else
+      k_fpc_rhs<3><<<ceil_div(g.interior, kLsBlock), kLsBlock, 0, h->stream>>>(g, h->rho, h->phat, h->u, h->v, h->w,
+                                                                             prm->dt, prm->rho12, h->mg_rhs);
+    SCIHPC_CHECK_LAUNCH();
+    SCIHPC_HIP(hipStreamSynchronize(h->stream));
+    // the loop of fast_pressure_correction (projection_method.cpp:694-706):
+    // full cycles from the handle's current solution until the absolute
+    // residual is <= ppe_tol
+    SCIHPC_REQUIRE(scihpc_mg_set_rhs_device(mg, h->mg_rhs) == SCIHPC_OK, scihpc_last_error());
+    double err = 0.0;
+    SCIHPC_REQUIRE(scihpc_mg_residual(mg, 0, &err) == SCIHPC_OK, scihpc_last_error());
+    int it = 0;
+    while (err > prm->ppe_tol && it < prm->max_cycles) {
+      ++it;
+      SCIHPC_REQUIRE(scihpc_mg_fullcycle_ref(mg) == SCIHPC_OK, scihpc_last_error());
+      SCIHPC_REQUIRE(scihpc_mg_residual(mg, 0, &err) == SCIHPC_OK, scihpc_last_error());
+    }
+    h->finish_pressure_and_velocity(mg, h->phat, prm->dt, prm->rho12);
+    h->d2h(u, h->u);
+    h->d2h(v, h->v);
+    h->d2h(w, h->w);
+    h->d2h(p, h->p);
+    h->d2h(phat, h->phat);
+    SCIHPC_HIP(hipStreamSynchronize(h->stream));
+    if (cycles) *cycles = it;
+  });
+}
+
+extern "C" int scihpc_proj_step(scihpc_proj *h, scihpc_mg *mg, double dt, const scihpc_solve_opts *opts,
+                                const double *rho, double *u, double *v, double *w, double *p,
+                                scihpc_solve_stats *stats) {
+  int rc = SCIHPC_OK;
+  const int g_rc = guarded([&] {
+    SCIHPC_REQUIRE(h && opts && rho && u && v && w && p && dt > 0, "null or bad argument");
+    h->check_mg(mg);
+    const Grid &g = h->g;
+    SCIHPC_REQUIRE(scihpc_mg_set_density_padded(mg, rho) == SCIHPC_OK, scihpc_last_error());
+    h->h2d(h->rho, rho);
+    h->h2d(h->u, u);
+    h->h2d(h->v, v);
+    h->h2d(h->w, w);
+    const unsigned nb = ceil_div(g.interior, kLsBlock);
+    if (g.ndim == 2)
+      k_div_rhs<2><<<nb, kLsBlock, 0, h->stream>>>(g, h->u, h->v, h->w, dt, h->mg_rhs, h->part);
+    else
+      k_div_rhs<3><<<nb, kLsBlock, 0, h->stream>>>(g, h->u, h->v, h->w, dt, h->mg_rhs, h->part);
+    SCIHPC_CHECK_LAUNCH();
+    // compatibility of the all-Neumann problem: remove the mean of div u / dt
+    SCIHPC_HIP(hipMemcpyAsync(h->hpart, h->part, sizeof(double) * nb, hipMemcpyDeviceToHost, h->stream));
+    SCIHPC_HIP(hipStreamSynchronize(h->stream));
+    double sum = 0.0;
+    for (unsigned b = 0; b < nb; ++b) sum += h->hpart[b];
+    k_sub_const<<<nb, kLsBlock, 0, h->stream>>>(h->mg_rhs, g.interior, sum / (double)g.interior);
+    SCIHPC_CHECK_LAUNCH();
+    SCIHPC_HIP(hipStreamSynchronize(h->stream));
+    SCIHPC_REQUIRE(scihpc_mg_set_rhs_device(mg, h->mg_rhs) == SCIHPC_OK, scihpc_last_error());
+    scihpc_solve_stats st{};
+    rc = scihpc_mg_solve(mg, opts, &st);
+    SCIHPC_REQUIRE(rc == SCIHPC_OK || rc == SCIHPC_NOT_CONVERGED, scihpc_last_error());
+    h->finish_pressure_and_velocity(mg, nullptr, dt, 1.0);
+    h->d2h(u, h->u);
+    h->d2h(v, h->v);
+    h->d2h(w, h->w);
+    h->d2h(p, h->p);
+    SCIHPC_HIP(hipStreamSynchronize(h->stream));
+    if (stats) *stats = st;
+  });
+  return g_rc != SCIHPC_OK ? g_rc : rc;
 }

@@ -187,9 +187,43 @@ def field_cases(ref):
     return out
 
 
+def fpc_cases(ref):
+    """One FPC projection step (pressure guess, FPC coefficients, rhs,
+    full-cycle multigrid from p = 0, velocity correction, velocity BCs)."""
+    out = {}
+    rng = np.random.default_rng(23)
+    for ndim, n, vt in [(2, (16, 16, 1), (3, 3, 3, 3, 3, 3)), (3, (8, 8, 8), (3, 3, 4, 4, 3, 3))]:
+        shp = padded_shape(ndim, *n)
+        ext = ext_for(ndim, n, (1.0, 1.0, 1.0))
+        tot = int(np.prod(shp))
+        phi = rng.uniform(-0.3, 0.3, tot)
+        rho = np.zeros(tot)
+        ref.lib.ref_density(ndim, *n, ext, phi, 0.1, 1000.0, 1, rho)
+        p_old, p_old2 = rng.standard_normal(tot), rng.standard_normal(tot)
+        u, v, w = (rng.standard_normal(tot) for _ in range(3))
+        types = np.array(vt, np.int32)
+        # walls as the velocity wrapper leaves them, so that div u sums to zero
+        # and the Neumann pressure problem is compatible
+        for stag, arr in enumerate((u, v, w)):
+            ref.lib.ref_bc_apply(ndim, *n, ext, types, np.zeros(6), stag, arr)
+        dt, dt_old, dt_old2, rho12, tol = 0.01, 0.009, 0.011, 0.001, 1e-8
+        uu, vv, ww = u.copy(), v.copy(), w.copy()
+        pp, ph = np.zeros(tot), np.zeros(tot)
+        ref.lib.ref_fpc_step(ndim, *n, ext, dt, dt_old, dt_old2, rho12, tol, types, rho, p_old, p_old2,
+                             uu, vv, ww, pp, ph)
+        out[f"fpc{ndim}d"] = dict(ndim=ndim, n=np.array(n), ext=ext, types=types, dt=dt, dt_old=dt_old,
+                                  dt_old2=dt_old2, rho12=rho12, tol=tol, rho=rho, p_old=p_old, p_old2=p_old2,
+                                  u=u, v=v, w=w, u_out=uu, v_out=vv, w_out=ww, p_out=pp, phat_out=ph)
+    return out
+
+
 def main():
     ref = Reference()
-    groups = {"mg": mg_cases, "bc": bc_cases, "weno": weno_cases, "rk3": rk3_cases, "fields": field_cases}
+    groups = {"mg": mg_cases, "bc": bc_cases, "weno": weno_cases, "rk3": rk3_cases, "fields": field_cases,
+              "fpc": fpc_cases}
+    only = sys.argv[1:]
+    if only:
+        groups = {k: v for k, v in groups.items() if k in only}
     for gname, fn in groups.items():
         cases = fn(ref)
         flat = {}

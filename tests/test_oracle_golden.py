@@ -128,3 +128,27 @@ def test_density_coefficients_projection(oracle, name):
         if ndim == 2 and a in (5, 6):
             continue
         assert np.array_equal(F[a], fref[a]), a
+
+
+FPC = load("fpc")
+
+
+@pytest.mark.parametrize("case", sorted(FPC), ids=str)
+def test_fpc_step_matches_reference(oracle, case):
+    """One FPC projection step (pressure guess, FPC coefficients, rhs,
+    reference-faithful full cycles from p = 0, pressure BCs, velocity
+    correction, velocity BCs) against fast_pressure_correction itself."""
+    c = FPC[case]
+    ndim, n = int(c["ndim"]), [int(x) for x in c["n"]]
+    dx, dy, dz = spacing(c)
+    u, v, w = c["u"].copy(), c["v"].copy(), c["w"].copy()
+    p, ph = np.zeros_like(u), np.zeros_like(u)
+    cycles = oracle.lib.orc_fpc_step(ndim, *n, dx, dy, dz, float(c["dt"]), float(c["dt_old"]), float(c["dt_old2"]),
+                                     float(c["rho12"]), float(c["tol"]), 100,
+                                     np.ascontiguousarray(c["types"], np.int32), c["rho"], c["p_old"], c["p_old2"],
+                                     u, v, w, p, ph)
+    assert cycles >= 1
+    np.testing.assert_array_equal(ph, c["phat_out"])
+    for got, key in [(p, "p_out"), (u, "u_out"), (v, "v_out"), (w, "w_out")]:
+        ref = c[key]
+        assert np.linalg.norm(got - ref) <= 1e-12 * np.linalg.norm(ref), key
