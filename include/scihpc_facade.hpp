@@ -9,6 +9,7 @@
 //   multigrid         multigrid.h:13-25         (level_num, at[], v_cycle, full_cycle)
 //   weno_solver       weno_solver.h:17-48       (weno5_find_derivatives)
 //   runge_kutta       runge_kutta.h:12-23       (tvd_rk3 of the WENO5 level-set convection)
+//   projection_method projection_method.h:14-51 (fast_pressure_correction, projection)
 // so a driver written against SciHPC compiles against this header with
 // `using namespace scihpc_gpu;`.  Every operation runs on the GPU through
 // include/scihpc_hip.h; host vectors are copied to the device before a call
@@ -103,7 +104,7 @@ class scalar_data {
  public:
   int ghc = 3;
   int Nx, Ny, Nz, nx, ny, nz, ndim;
-  DataType ***data, ***flux, ***fx, ***fy, ***fz;
+  DataType ***data, ***flux, ***old, ***old2, ***fx, ***fy, ***fz;
 
   explicit scalar_data(int _nx) : scalar_data(_nx, 1, 1, 1) {}
   scalar_data(int _nx, int _ny) : scalar_data(_nx, _ny, 1, 2) {}
@@ -123,7 +124,7 @@ class scalar_data {
     std::vector<DataType *> rows;
     std::vector<DataType **> planes;
   };
-  field f_[5];
+  field f_[7];
   DataType ***make(field &f) {
     f.v.assign((size_t)Nx * Ny * Nz, 0.0);
     f.rows.resize((size_t)Nx * Ny);
@@ -140,9 +141,18 @@ class scalar_data {
     Nz = nz > 1 ? nz + 2 * ghc : 1;
     data = make(f_[0]);
     flux = make(f_[1]);
-    fx = make(f_[2]);
-    fy = make(f_[3]);
-    fz = make(f_[4]);
+    old = make(f_[2]);
+    old2 = make(f_[3]);
+    fx = make(f_[4]);
+    fy = make(f_[5]);
+    fz = make(f_[6]);
+  }
+
+ public:
+  // scalar_data::store (scalar_data.cpp): old2 <- old <- data
+  void store() {
+    std::copy(f_[2].v.begin(), f_[2].v.end(), f_[3].v.begin());
+    std::copy(f_[0].v.begin(), f_[0].v.end(), f_[2].v.begin());
   }
 };
 
@@ -354,6 +364,68 @@ enum bc_type { NEUMANN = SCIHPC_BC_NEUMANN, DIRICHLET = SCIHPC_BC_DIRICHLET, PER
 struct bc_info {
   bc_type type;
   DataType value = 0.0;
+};
+
+// problem_parameters (global.h:25-54): the fields the projection step reads
+struct problem_parameters {
+  DataType dt = 1.0, dt_old = 1.0, dt_old2 = 1.0;
+  DataType density_ratio = 1.0;
+  DataType ppe_tol = 1e-4;
+};
+
+// projection_method (projection_method.h:14-51): the projection step of
+// one grid on the GPU.  The multigrid is the caller's (its level-0 solution
+// is the warm start of the FPC loop, as SciHPC's persistent at[0]->sol).
+class projection_method {
+ public:
+  projection_method(scalar_data *f, structured_grid *geo, const bc_info vel_faces[6]) {
+    int types[6];
+    for (int q = 0; q < 6; ++q) types[q] = vel_faces[q].type;
+    check(scihpc_proj_create(f->ndim, f->nx, f->ny, f->nz, geo->dx, geo->dy, f->ndim == 3 ? geo->dz : 1.0, types,
+                             &h_));
+  }
+  ~projection_method() {
+    if (h_) scihpc_proj_destroy(h_);
+  }
+  projection_method(const projection_method &) = delete;
+  projection_method &operator=(const projection_method &) = delete;
+
+  // pressure_guess_solution + init_ppe_fpc_coefficients +
+  // fast_pressure_correction (projection_method.cpp:467-541, 659-753,
+  // 803-815): pressure->old/old2 in, pressure->data and ->flux (p_hat) out,
+  // vel corrected in place.  Returns the multigrid cycles.
+  int fast_pressure_correction(multigrid &mg, scalar_data *pressure, const scalar_data *density, vector_data *vel,
+                               const problem_parameters &prm, int max_cycles = 100) {
+    scihpc_fpc_params fp{prm.dt, prm.dt_old, prm.dt_old2, prm.density_ratio, prm.ppe_tol, max_cycles};
+    std::vector<double> zero;
+    double *w = vel_w(vel, pressure, zero);
+    int cycles = 0;
+    check(scihpc_proj_fpc_step(h_, mg.handle(), &fp, base(density->data), base(pressure->old), base(pressure->old2),
+                               base(vel->x->data), base(vel->y->data), w, base(pressure->data),
+                               base(pressure->flux), &cycles));
+    return cycles;
+  }
+  // projection() (projection_method.cpp:544-657) with the SOR replaced by the
+  // A(rho) multigrid solve to rel_tol
+  scihpc_solve_stats projection(multigrid &mg, scalar_data *pressure, const scalar_data *density, vector_data *vel,
+                                DataType dt, double rel_tol = 1e-8, int precision = SCIHPC_PREC_MIXED) {
+    scihpc_solve_opts o{rel_tol, 200, 2, 2, precision, 0, 1, 1};
+    scihpc_solve_stats st{};
+    std::vector<double> zero;
+    double *w = vel_w(vel, pressure, zero);
+    check(scihpc_proj_step(h_, mg.handle(), dt, &o, base(density->data), base(vel->x->data), base(vel->y->data), w,
+                           base(pressure->data), &st));
+    return st;
+  }
+
+ private:
+  scihpc_proj *h_ = nullptr;
+  static double *base(DataType ***f) { return &f[0][0][0]; }
+  static double *vel_w(vector_data *vel, const scalar_data *like, std::vector<double> &zero) {
+    if (vel->z) return base(vel->z->data);
+    zero.assign(like->size(), 0.0);
+    return zero.data();
+  }
 };
 
 // the level-set transport of one scalar field on the GPU

@@ -6,7 +6,9 @@ GPU: the reference-faithful V-cycle loop of the Dirichlet driver converges,
 its error against sin(pi x) sin(pi y) is the first-order value of the
 reference operator (about 3.7e-2 at 64^2: 9.29e-3 at 256^2, doubling per
 halving of n), the red-black mixed solve reaches the same discrete solution,
-the variable-density solve converges, and the level set is transported.
+the variable-density solve converges, the level set is transported, and the
+projection step (A(rho) projection and FPC step) runs through
+projection_method and leaves the velocity divergence-free.
 """
 import os
 import subprocess
@@ -46,3 +48,5 @@ def test_facade_driver_runs():
     assert v["solve_vs_vcycle_maxdiff"] < 1e-6
     assert v["bubble_rel_residual"] <= 1e-8
     assert v["levelset_cells"] > 100 and v["levelset_max_error_near_interface"] < 0.02
+    assert v["projection_rel_residual"] <= 1e-10 and v["projection_div_ratio"] < 1e-8
+    assert 1 <= v["fpc_cycles"] <= 100
