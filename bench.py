@@ -139,6 +139,70 @@ def bench_3d(args):
     return res
 
 
+def bench_snapshot(torch, nx=512, ny=1024, reps=5):
+    """Legacy-VTK snapshot of the 512x1024 grid as the reference's drivers
+    write it (three scalars and one vector per file, vtkWriter.cpp:18-111):
+    the reorder + byte-swap kernel alone against HBM and the whole snapshot
+    from device-resident fields to a closed file.  The fields and the file's
+    bytes ride along (underscore keys) for the CPU baseline leg."""
+    import tempfile
+
+    import numpy as np
+
+    from scihpc_amd import VtkWriter, padded_shape
+
+    shp = padded_shape(2, nx, ny)
+    cells = nx * ny
+    h = (1.0 / nx, 2.0 / ny)
+    dev = [torch.randn(int(np.prod(shp)), dtype=torch.float64, device="cuda") for _ in range(6)]
+    torch.cuda.synchronize()
+    out = {"grid": [nx, ny], "fields": "phi, density, pressure (scalars) + nvel (vector)"}
+    with tempfile.TemporaryDirectory(dir="/tmp") as d:
+        wr = VtkWriter(2, (nx, ny), (0.0, 0.0), h, os.path.join(d, "snap"))
+        ms1 = wr.time_pack(1, iters=50)
+        ms3 = wr.time_pack(3, iters=50)
+        out["pack_scalar_us"] = round(ms1 * 1e3, 2)
+        out["pack_scalar_GBps"] = round(16.0 * cells / (ms1 * 1e-3) / 1e9, 1)
+        out["pack_vector_us"] = round(ms3 * 1e3, 2)
+        out["pack_vector_GBps"] = round(48.0 * cells / (ms3 * 1e-3) / 1e9, 1)
+        file_bytes = 0
+        t0 = time.perf_counter()
+        for r in range(reps):
+            path = wr.begin(r)
+            for name, t in zip(("phi", "density", "pressure"), dev[:3]):
+                wr.add_scalar_device(name, t.data_ptr())
+            wr.add_vector_device("nvel", *(t.data_ptr() for t in dev[3:]))
+            wr.close()
+            file_bytes = os.path.getsize(path)
+        ms_snap = (time.perf_counter() - t0) / reps * 1e3
+        out["snapshot_ms"] = round(ms_snap, 3)
+        out["file_MB"] = round(file_bytes / 1e6, 3)
+        out["snapshot_MBps"] = round(file_bytes / (ms_snap * 1e-3) / 1e6, 1)
+        out["_fields"] = [t.cpu().numpy() for t in dev]
+        out["_file"] = open(path, "rb").read()
+        wr.close_handle()
+    return out
+
+
+def cpu_snapshot_baseline(snap):
+    """The oracle's C writer (one core) on the same fields: ms per file and
+    whether its bytes equal the GPU writer's file."""
+    import tempfile
+
+    from oracle.pyoracle import Oracle
+
+    nx, ny = snap["grid"]
+    f = snap["_fields"]
+    with tempfile.TemporaryDirectory(dir="/tmp") as d:
+        path = os.path.join(d, "cpu.vtk")
+        t0 = time.perf_counter()
+        Oracle().vtk_write(path, 2, (nx, ny), (0.0, 0.0, 0.0), (1.0 / nx, 2.0 / ny, 0.0),
+                           list(zip(("phi", "density", "pressure"), f[:3])), [("nvel", tuple(f[3:]))])
+        ms = (time.perf_counter() - t0) * 1e3
+        same = open(path, "rb").read() == snap["_file"]
+    return {"ms_per_file": round(ms, 3), "cores": 1, "identical_to_gpu_file": same}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -149,6 +213,7 @@ def main():
     ap.add_argument("--no-3d", action="store_true", help="skip the 256^3 3-D solves")
     ap.add_argument("--only-3d", action="store_true", help="run only the 256^3 3-D solves (profiling)")
     ap.add_argument("--no-unet", action="store_true", help="skip the U-Net training-step timing")
+    ap.add_argument("--no-snapshot", action="store_true", help="skip the VTK snapshot timing")
     ap.add_argument("--kernel-only", action="store_true",
                     help="only launch the dominant kernel (and a 256 MiB copy for calibration): "
                          "the command profiled with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE")
@@ -262,6 +327,13 @@ def main():
                 "mfma_frac_train": round(fl_tr / (ms_tr * 1e-3) / 1e12 / MFMA_BF16_PEAK, 4)}
         nt.close()
 
+    snapshot = None
+    if not args.no_snapshot:
+        log("bench: VTK snapshot of the 512x1024 grid")
+        snapshot = bench_snapshot(torch)
+        log(f"bench: snapshot {snapshot['snapshot_ms']} ms ({snapshot['snapshot_MBps']} MB/s), "
+            f"pack {snapshot['pack_scalar_GBps']} GB/s")
+
     out = {
         "metric": METRIC,
         "value": round(ms_ml, 4),
@@ -296,12 +368,18 @@ def main():
                      "kernel": "k_pre2d<float,true> level 0 (2 RB-GS sweeps + residual + restriction)",
                      "bytes_per_launch": kbytes, "us_per_launch": round(kms * 1e3, 3)},
         "unet": unet,
+        "snapshot": snapshot,
     }
     if not args.no_3d:
         out["plane_3d_256"] = bench_3d(args)
     if not args.no_cpu_baseline:
         log("bench: cpu baseline (oracle port, 1 core)")
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_seconds)
+        if snapshot is not None:
+            out["cpu_baseline"]["snapshot_writer"] = cpu_snapshot_baseline(snapshot)
+    if snapshot is not None:
+        snapshot.pop("_fields")
+        snapshot.pop("_file")
     print(json.dumps(out), flush=True)
 
 

@@ -487,4 +487,45 @@ class runge_kutta : public level_set_solver {
   }
 };
 
+// vtkWriter.h:15-35: legacy VTK snapshots, byte-identical to the
+// reference's files; create(id) opens <filename><id>.vtk.  The device does
+// the reorder and byte swap, a background thread writes the file, close()
+// waits for it.  A 2-D vector_data of this facade has no z component: its
+// VECTORS get zeros there, as the reference's zero-filled z field.
+class vtkWriter {
+ public:
+  vtkWriter(structured_grid *geo, std::string filename) : grid(geo), filename(std::move(filename)) {
+    const int n[3] = {geo->x_axis.n, geo->ndim > 1 ? geo->y_axis.n : 1, geo->ndim > 2 ? geo->z_axis.n : 1};
+    const double org[3] = {geo->x_axis.start, geo->y_axis.start, geo->z_axis.start};
+    const double sp[3] = {geo->dx, geo->dy, geo->dz};
+    check(scihpc_vtk_create(geo->ndim, n[0], n[1], n[2], org, sp, this->filename.c_str(), &h_));
+  }
+  ~vtkWriter() { scihpc_vtk_destroy(h_); }
+  vtkWriter(const vtkWriter &) = delete;
+  vtkWriter &operator=(const vtkWriter &) = delete;
+
+  structured_grid *grid;
+  std::string filename;
+
+  void create(unsigned int id) { check(scihpc_vtk_begin(h_, id)); }
+  void add_scalar(scalar_data *data, const std::string &name) { add_scalar(data->data, data, name); }
+  void add_scalar(DataType ***f, scalar_data *, const std::string &name) {
+    check(scihpc_vtk_add_scalar(h_, name.c_str(), &f[0][0][0], 0));
+  }
+  void add_vector(vector_data *data, const std::string &name) {
+    add_vector(data->x->data, data->y ? data->y->data : nullptr, data->z ? data->z->data : nullptr, data, name);
+  }
+  void add_vector(DataType ***u, DataType ***v, DataType ***w, vector_data *data, const std::string &name) {
+    std::vector<DataType> zero;
+    if (!u || !v || !w) zero.assign(data->x->size(), 0.0);
+    auto ptr = [&](DataType ***f) -> const DataType * { return f ? &f[0][0][0] : zero.data(); };
+    const DataType *pu = ptr(u), *pv = ptr(v), *pw = ptr(w);
+    check(scihpc_vtk_add_vector(h_, name.c_str(), pu, pv, pw, 0));
+  }
+  void close() { check(scihpc_vtk_close(h_)); }
+
+ private:
+  scihpc_vtk *h_ = nullptr;
+};
+
 }  // namespace scihpc_gpu

@@ -36,7 +36,8 @@ enum {
   SCIHPC_NOT_CONVERGED = 1,
   SCIHPC_BAD_ARG = 2,
   SCIHPC_HIP_ERROR = 3,
-  SCIHPC_FELL_BACK = 4
+  SCIHPC_FELL_BACK = 4,
+  SCIHPC_IO_ERROR = 5
 };
 
 /* multigrid boundary operator: init_NeumannBC / init_DirichletBC
@@ -311,6 +312,38 @@ SCIHPC_API int scihpc_ls_density(scihpc_ls *h, double ls_width, double ratio,
  * and algorithmic HBM bytes per step. */
 SCIHPC_API int scihpc_ls_time_step(scihpc_ls *h, double dt, int iters, double *ms,
                         double *bytes);
+
+/* ------------------------------------------------------------------ */
+/* Snapshots: vtkWriter (vtkWriter.h:15-35, vtkWriter.cpp:18-111).  Files are
+ * byte-identical to the reference's legacy VTK (ASCII header, BINARY
+ * STRUCTURED_POINTS, CELL_DATA, big-endian FP64, x fastest).  Fields are
+ * padded (scalar_data layout) host arrays (on_device = 0) or device arrays
+ * (on_device = 1, complete before the call; the device does the reorder
+ * and byte swap).  add_* return once the field has been read; the file is
+ * written by a background thread and is complete after scihpc_vtk_close
+ * (or the next scihpc_vtk_begin), which also report write failures as
+ * SCIHPC_IO_ERROR.                                                       */
+typedef struct scihpc_vtk scihpc_vtk;
+/* vtkWriter(structured_grid*, filename): origin = axis starts, spacing =
+ * dx, dy, dz of the grid (entries beyond ndim are ignored and written as 0,
+ * as structured_grid.cpp:7-84 sets them). */
+SCIHPC_API int scihpc_vtk_create(int ndim, int nx, int ny, int nz, const double origin[3],
+                                 const double spacing[3], const char *prefix, scihpc_vtk **out);
+SCIHPC_API int scihpc_vtk_destroy(scihpc_vtk *h);
+/* vtkWriter::create(id): opens prefix<id>.vtk and writes the header */
+SCIHPC_API int scihpc_vtk_begin(scihpc_vtk *h, unsigned int id);
+/* vtkWriter::add_scalar(scalar_data*, name) / add_scalar(DataType***, ...) */
+SCIHPC_API int scihpc_vtk_add_scalar(scihpc_vtk *h, const char *name, const double *field,
+                                     int on_device);
+/* vtkWriter::add_vector(vector_data*, name) / add_vector(u, v, w, ...) */
+SCIHPC_API int scihpc_vtk_add_vector(scihpc_vtk *h, const char *name, const double *u,
+                                     const double *v, const double *w, int on_device);
+/* vtkWriter::close(): waits for the writer thread and closes the file */
+SCIHPC_API int scihpc_vtk_close(scihpc_vtk *h);
+/* Times the reorder + byte-swap kernel alone (ncomp 1: a scalar, 3: a
+ * vector) on the handle's own device staging fields: average ms per launch
+ * over `iters`.  Algorithmic bytes per launch: 16 * ncomp * cells. */
+SCIHPC_API int scihpc_vtk_time_pack(scihpc_vtk *h, int ncomp, int iters, double *ms);
 
 #ifdef __cplusplus
 }

@@ -1502,3 +1502,59 @@ int orc_fpc_step(int ndim, int nx, int ny, int nz, double dx, double dy,
   free(C);
   return iter;
 }
+
+/* ================================================================== */
+/* legacy VTK snapshot: vtkWriter::create / add_scalar / add_vector /   */
+/* close (vtkWriter.cpp:18-111), header doubles as ostream's default   */
+/* "%g", values big-endian (SwapEnd, vtkWriter.cpp:7-11), k-j-i order.  */
+#include <stdio.h>
+
+static void vtk_put_be(FILE *f, double v) {
+  unsigned char b[8], r[8];
+  memcpy(b, &v, 8);
+  for (int q = 0; q < 8; q++) r[q] = b[7 - q];
+  fwrite(r, 1, 8, f);
+}
+
+int orc_vtk_write(const char *path, int ndim, int nx, int ny, int nz,
+                  const double *origin, const double *spacing, int nscalar,
+                  const char *const *snames, const double *const *scalars,
+                  int nvector, const char *const *vnames,
+                  const double *const *vectors) {
+  orc_pf p = pf_make(ndim, nx, ny, nz);
+  FILE *f = fopen(path, "wb");
+  if (!f) return -1;
+  int dim[3];
+  double org[3], sp[3];
+  for (int a = 0; a < 3; a++) {
+    dim[a] = a < ndim ? p.n[a] + 1 : 1;       /* structured_grid.cpp:30-36 */
+    org[a] = a < ndim ? origin[a] : 0.0;
+    sp[a] = a < ndim ? spacing[a] : 0.0;
+  }
+  fprintf(f, "# vtk DataFile Version 3.0\nvtk output\nBINARY\n");
+  fprintf(f, "DATASET STRUCTURED_POINTS\n");
+  fprintf(f, "DIMENSIONS %d %d %d\n", dim[0], dim[1], dim[2]);
+  fprintf(f, "ORIGIN %g %g %g\n", org[0], org[1], org[2]);
+  fprintf(f, "SPACING %g %g %g\n", sp[0], sp[1], sp[2]);
+  long size = 1;
+  for (int a = 0; a < 3; a++)
+    if (dim[a] > 1) size *= dim[a] - 1;
+  fprintf(f, "CELL_DATA %ld\n", size);
+  for (int q = 0; q < nscalar; q++) {
+    fprintf(f, "SCALARS %s double 1\nLOOKUP_TABLE default\n", snames[q]);
+    for (int k = 0; k < p.n[2]; k++)
+      for (int j = 0; j < p.n[1]; j++)
+        for (int i = 0; i < p.n[0]; i++)
+          vtk_put_be(f, scalars[q][PIDX(p, i + p.off[0], j + p.off[1], k + p.off[2])]);
+  }
+  for (int q = 0; q < nvector; q++) {
+    fprintf(f, "VECTORS %s double\n", vnames[q]);
+    for (int k = 0; k < p.n[2]; k++)
+      for (int j = 0; j < p.n[1]; j++)
+        for (int i = 0; i < p.n[0]; i++) {
+          const long id = PIDX(p, i + p.off[0], j + p.off[1], k + p.off[2]);
+          for (int c = 0; c < 3; c++) vtk_put_be(f, vectors[3 * q + c][id]);
+        }
+  }
+  return fclose(f) == 0 ? 0 : -1;
+}

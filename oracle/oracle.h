@@ -196,6 +196,17 @@ int orc_fpc_step(int ndim, int nx, int ny, int nz, double dx, double dy,
                  const double *p_old2, double *u, double *v, double *w,
                  double *pr, double *phat);
 
+/* ------------------------------------------------------------------ */
+/* legacy VTK snapshot file: vtkWriter::create, add_scalar (nscalar
+ * padded fields), add_vector (nvector padded u,v,w triples, vectors[3q+c]),
+ * close (vtkWriter.cpp:18-111).  Returns 0, or -1 on an I/O failure.
+ *                                                              [golden] */
+int orc_vtk_write(const char *path, int ndim, int nx, int ny, int nz,
+                  const double *origin, const double *spacing, int nscalar,
+                  const char *const *snames, const double *const *scalars,
+                  int nvector, const char *const *vnames,
+                  const double *const *vectors);
+
 #ifdef __cplusplus
 }
 #endif

@@ -31,6 +31,28 @@ def padded_shape(ndim, nx, ny=1, nz=1):
     return tuple(n[a] + 2 * GHC if a < ndim else 1 for a in range(3))
 
 
+def _vtk_args(scalars, vectors):
+    """(nscalar, names, ptrs, nvector, names, ptrs) for the VTK writers; the
+    arrays are kept alive on the returned tuple's ctypes objects."""
+    keep = []
+
+    def ptrs(arrs):
+        arrs = [np.ascontiguousarray(a, np.float64) for a in arrs]
+        keep.extend(arrs)
+        out = (C.c_void_p * max(1, len(arrs)))(*[a.ctypes.data for a in arrs])
+        out._keep = arrs
+        return out
+
+    def names(ns):
+        out = (C.c_char_p * max(1, len(ns)))(*[x.encode() for x in ns])
+        return out
+
+    sv = [a for _, a in scalars]
+    vv = [c for _, t in vectors for c in t]
+    return (len(scalars), names([nm for nm, _ in scalars]), ptrs(sv),
+            len(vectors), names([nm for nm, _ in vectors]), ptrs(vv))
+
+
 def _sig(lib, name, res, args):
     f = getattr(lib, name)
     f.restype = res
@@ -92,6 +114,8 @@ class Oracle:
              [_i, _i, _i, _i, _d, _d, _d, _d, _d, _dp, _dp, _dp, _dp, _dp, _dp])
         _sig(L, "orc_pressure_guess", None, [_l, _d, _d, _dp, _dp, _dp])
         _sig(L, "orc_fpc_step", _i, [_i, _i, _i, _i, _d, _d, _d, _d, _d, _d, _d, _d, _i, _ip] + [_dp] * 8)
+        _sig(L, "orc_vtk_write", _i, [C.c_char_p, _i, _i, _i, _i, _dp, _dp, _i, C.c_void_p, C.c_void_p,
+                                      _i, C.c_void_p, C.c_void_p])
 
     # -------------------------------------------------------------- MG
     def levels(self, ndim, nx, ny, nz=1):
@@ -122,6 +146,15 @@ class Oracle:
         fx, fy, fz = (np.zeros_like(phi) for _ in range(3))
         self.lib.orc_tvd_rk3_weno(ndim, *n, *h, dt, t, vals, phi, u, v, w, fx, fy, fz, nsteps)
         return phi
+
+    def vtk_write(self, path, ndim, n, origin, spacing, scalars=(), vectors=()):
+        """Legacy VTK file of padded fields, as vtkWriter writes it."""
+        args = _vtk_args(scalars, vectors)
+        n = list(n) + [1] * (3 - len(n))
+        rc = self.lib.orc_vtk_write(path.encode(), ndim, *n, np.ascontiguousarray(origin, np.float64),
+                                    np.ascontiguousarray(spacing, np.float64), *args)
+        if rc != 0:
+            raise OSError(f"orc_vtk_write failed for {path}")
 
     def density(self, phi, ls_width, ratio, positive_ref=True):
         rho = np.zeros_like(phi)
@@ -211,6 +244,16 @@ class Reference:
              [_i, _i, _i, _i, _dp, _d, _i, _d, _d, _dp, _dp, _dp, _dp, _dp, _dp])
         _sig(L, "ref_fpc_coefficients", None, [_i, _i, _i, _i, _dp, _d, _dp, _dp, _dp])
         _sig(L, "ref_fpc_step", _i, [_i, _i, _i, _i, _dp, _d, _d, _d, _d, _d, _ip] + [_dp] * 8)
+        _sig(L, "ref_vtk_write", _i, [C.c_char_p, C.c_uint, _i, _i, _i, _i, _dp, _i, C.c_void_p, C.c_void_p,
+                                      _i, C.c_void_p, C.c_void_p])
+
+    def vtk_write(self, prefix, id, ndim, n, ext, scalars=(), vectors=()):
+        """vtkWriter(grid, prefix): create(id), add_scalar per (name, padded),
+        add_vector per (name, (u, v, w)), close().  Returns the file path."""
+        args = _vtk_args(scalars, vectors)
+        n = list(n) + [1] * (3 - len(n))
+        self.lib.ref_vtk_write(prefix.encode(), id, ndim, *n, np.ascontiguousarray(ext, np.float64), *args)
+        return f"{prefix}{id}.vtk"
 
     def mg(self, ndim, nx, ny, nz, ext, bc="neumann"):
         return RefMG(self, ndim, nx, ny, nz, ext, BC[bc])

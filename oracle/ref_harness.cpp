@@ -14,6 +14,7 @@
 #include "multigrid.h"
 #include "projection_method.h"
 #include "runge_kutta.h"
+#include "vtkWriter.h"
 #include "scalar_data.h"
 #include "shared_solvers.h"
 #include "structured_grid.h"
@@ -378,6 +379,33 @@ int ref_fpc_step(int ndim, int nx, int ny, int nz, const double *ext, double dt,
   from_ref(s, vel->vector->x.data, u);
   from_ref(s, vel->vector->y.data, v);
   from_ref(s, vel->vector->z.data, w);
+  return 0;
+}
+
+// vtkWriter(grid, prefix).create(id), add_scalar per padded scalar field,
+// add_vector per padded (u, v, w) triple (vectors[3q+c]), close().
+int ref_vtk_write(const char *prefix, unsigned id, int ndim, int nx, int ny, int nz,
+                  const double *ext, int nscalar, const char *const *snames,
+                  const double *const *scalars, int nvector, const char *const *vnames,
+                  const double *const *vectors) {
+  structured_grid *geo = make_grid(ndim, nx, ny, nz, ext);
+  vtkWriter vtk(geo, prefix);
+  vtk.create(id);
+  for (int q = 0; q < nscalar; ++q) {
+    scalar_data *f = make_scalar(ndim, nx, ny, nz);
+    to_ref(f, f->data, scalars[q]);
+    vtk.add_scalar(f, snames[q]);
+  }
+  for (int q = 0; q < nvector; ++q) {
+    vector_data *v = ndim == 1 ? new vector_data(nx)
+                     : ndim == 2 ? new vector_data(nx, ny)
+                                 : new vector_data(nx, ny, nz);
+    to_ref(&v->x, v->x.data, vectors[3 * q]);
+    to_ref(&v->x, v->y.data, vectors[3 * q + 1]);
+    to_ref(&v->x, v->z.data, vectors[3 * q + 2]);
+    vtk.add_vector(v, vnames[q]);
+  }
+  vtk.close();
   return 0;
 }
 

@@ -16,8 +16,8 @@ import numpy as np
 from ._capi import (BC_CODES, MG_DIRICHLET, MG_NEUMANN, NOT_CONVERGED, OK, PREC_FP64,
                     PREC_MIXED, FpcParams, ScihpcError, SolveOpts, SolveStats, check, lib)
 
-__all__ = ["Multigrid", "LevelSet", "Projection", "UNet", "ScihpcError", "PREC_FP64", "PREC_MIXED",
-           "padded_shape"]
+__all__ = ["Multigrid", "LevelSet", "Projection", "UNet", "VtkWriter", "ScihpcError", "PREC_FP64",
+           "PREC_MIXED", "padded_shape", "read_vtk"]
 
 GHC = 3
 
@@ -380,3 +380,110 @@ class Projection:
         rc = check(self._lib.scihpc_proj_step(self.h, mg.h, dt, C.byref(o), _f64(rho), u, v, w, p, C.byref(s)),
                    allow=(OK, NOT_CONVERGED))
         return u, v, w, p, {"converged": rc == OK, "cycles": s.cycles, "rel_residual": s.rel_residual}
+
+
+class VtkWriter:
+    """vtkWriter (vtkWriter.h:15-35): legacy VTK snapshots of padded fields,
+    byte-identical to the reference's files.  The device reorders and
+    byte-swaps each field; a background thread writes the file.
+
+        w = VtkWriter(2, (nx, ny), origin=(0, 0), spacing=(dx, dy), prefix="out/dambreak")
+        w.begin(3); w.add_scalar("phi", phi); w.add_vector("nvel", u, v, w_); w.close()
+    """
+
+    def __init__(self, ndim, n, origin, spacing, prefix):
+        self._lib = lib()
+        n = list(n) + [1] * (3 - len(n))
+        org = np.zeros(3)
+        sp = np.zeros(3)
+        org[:ndim] = origin[:ndim]
+        sp[:ndim] = spacing[:ndim]
+        h = C.c_void_p()
+        check(self._lib.scihpc_vtk_create(ndim, n[0], n[1], n[2], org, sp, prefix.encode(), C.byref(h)))
+        self.h = h
+        self.prefix = prefix
+        self.shape = padded_shape(ndim, *n)
+
+    def close_handle(self):
+        if getattr(self, "h", None):
+            self._lib.scihpc_vtk_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close_handle()
+        except Exception:
+            pass
+
+    def begin(self, id):
+        check(self._lib.scihpc_vtk_begin(self.h, int(id)))
+        return f"{self.prefix}{int(id)}.vtk"
+
+    def _host(self, a):
+        a = _f64(a)
+        if a.size != int(np.prod(self.shape)):
+            raise ValueError(f"field has {a.size} values, padded grid has {int(np.prod(self.shape))}")
+        return a
+
+    def add_scalar(self, name, field):
+        f = self._host(field)
+        check(self._lib.scihpc_vtk_add_scalar(self.h, name.encode(), f.ctypes.data, 0))
+
+    def add_vector(self, name, u, v, w):
+        u, v, w = (self._host(a) for a in (u, v, w))
+        check(self._lib.scihpc_vtk_add_vector(self.h, name.encode(), u.ctypes.data, v.ctypes.data,
+                                              w.ctypes.data, 0))
+
+    def add_scalar_device(self, name, dptr):
+        """Device pointer (int) to a padded field, complete before the call."""
+        check(self._lib.scihpc_vtk_add_scalar(self.h, name.encode(), int(dptr), 1))
+
+    def add_vector_device(self, name, du, dv, dw):
+        check(self._lib.scihpc_vtk_add_vector(self.h, name.encode(), int(du), int(dv), int(dw), 1))
+
+    def close(self):
+        check(self._lib.scihpc_vtk_close(self.h))
+
+    def time_pack(self, ncomp=1, iters=50):
+        ms = C.c_double()
+        check(self._lib.scihpc_vtk_time_pack(self.h, ncomp, iters, C.byref(ms)))
+        return ms.value
+
+
+def read_vtk(path):
+    """Reads a legacy BINARY STRUCTURED_POINTS file written by vtkWriter (or
+    VtkWriter) into {"dims", "origin", "spacing", "cells", <name>: array}.
+    Scalars come back with shape (nz, ny, nx), vectors (nz, ny, nx, 3), in
+    the file's big-endian FP64 converted to native floats."""
+    raw = open(path, "rb").read()
+    pos = 0
+    out = {}
+
+    def line():
+        nonlocal pos
+        e = raw.index(b"\n", pos)
+        t = raw[pos:e].decode()
+        pos = e + 1
+        return t
+
+    for _ in range(4):
+        line()
+    out["dims"] = tuple(int(x) for x in line().split()[1:])
+    out["origin"] = tuple(float(x) for x in line().split()[1:])
+    out["spacing"] = tuple(float(x) for x in line().split()[1:])
+    cells = int(line().split()[1])
+    out["cells"] = cells
+    shape = tuple(max(d - 1, 1) for d in out["dims"])[::-1]
+    while pos < len(raw):
+        head = line().split()
+        if head[0] == "SCALARS":
+            line()  # LOOKUP_TABLE
+            a = np.frombuffer(raw, ">f8", cells, pos).astype(np.float64).reshape(shape)
+            pos += 8 * cells
+        elif head[0] == "VECTORS":
+            a = np.frombuffer(raw, ">f8", 3 * cells, pos).astype(np.float64).reshape(shape + (3,))
+            pos += 24 * cells
+        else:
+            raise ValueError(f"unexpected VTK section {head[0]!r} in {path}")
+        out[head[1]] = a
+    return out

@@ -12,7 +12,7 @@ import numpy as np
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libscihpc_hip.so")
 
-OK, NOT_CONVERGED, BAD_ARG, HIP_ERROR, FELL_BACK = 0, 1, 2, 3, 4
+OK, NOT_CONVERGED, BAD_ARG, HIP_ERROR, FELL_BACK, IO_ERROR = 0, 1, 2, 3, 4, 5
 MG_NEUMANN, MG_DIRICHLET = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
 BC_CODES = {"neumann": 0, "dirichlet": 1, "periodic": 2, "no_slip": 3, "slip": 4}
@@ -124,6 +124,13 @@ SIGNATURES = {
     "scihpc_ls_apply_bc": [_vp, _dp, _i],
     "scihpc_ls_density": [_vp, _d, _d, _i, _dp],
     "scihpc_ls_time_step": [_vp, _d, _i, _pd, _pd],
+    "scihpc_vtk_create": [_i, _i, _i, _i, _dp, _dp, C.c_char_p, C.POINTER(_vp)],
+    "scihpc_vtk_destroy": [_vp],
+    "scihpc_vtk_begin": [_vp, C.c_uint],
+    "scihpc_vtk_add_scalar": [_vp, C.c_char_p, _vp, _i],
+    "scihpc_vtk_add_vector": [_vp, C.c_char_p, _vp, _vp, _vp, _i],
+    "scihpc_vtk_close": [_vp],
+    "scihpc_vtk_time_pack": [_vp, _i, _i, _pd],
 }
 _RET = {"scihpc_last_error": C.c_char_p}
 

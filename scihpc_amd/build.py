@@ -17,7 +17,7 @@ PKG = os.path.join(ROOT, "scihpc_amd")
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libscihpc_hip.so")
 
-SOURCES = ["mg.hip", "advect.hip", "unet.hip"]
+SOURCES = ["mg.hip", "advect.hip", "unet.hip", "snapshot.hip"]
 DRIVERS = ["facade_demo"]
 
 HIPCC_FLAGS = [

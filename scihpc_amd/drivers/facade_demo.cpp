@@ -107,6 +107,16 @@ int main(int argc, char **argv) {
     }
   std::printf("levelset_max_error_near_interface %.6e\n", e);
   std::printf("levelset_cells %d\n", cnt);
+  if (argc > 2) {
+    // snapshot of the transported level set and its velocity, as the
+    // reference's drivers write them (benchmarks_2d_dambreak.cpp:94-98)
+    vtkWriter vtk(&geo, argv[2]);
+    vtk.create(10);
+    vtk.add_scalar(&ls, "phi");
+    vtk.add_vector(&vel, "nvel");
+    vtk.close();
+    std::printf("vtk_file %s10.vtk\n", argv[2]);
+  }
 
   // (5) projection steps on the bubble grid: FPC (fast_pressure_correction
   // with the reference-faithful full cycles) and the A(rho) projection; both

@@ -217,10 +217,30 @@ def fpc_cases(ref):
     return out
 
 
+def vtk_cases(ref):
+    """vtkWriter files (create, two add_scalar, one add_vector, close) of
+    seeded padded fields; the fixture keeps the file's bytes."""
+    import tempfile
+    out = {}
+    rng = np.random.default_rng(29)
+    for ndim, n, ext in [(1, (9, 1, 1), [0.5, 2.0, 0, 0, 0, 0]),
+                         (2, (12, 8, 1), [0.0, 1.0, -0.5, 2.0, 0, 0]),
+                         (3, (6, 5, 4), [0.0, 1.0, -0.5, 2.0, 0.25, 3.0])]:
+        shp = padded_shape(ndim, *n)
+        phi, p, u, v, w = (rng.standard_normal(shp).ravel() for _ in range(5))
+        with tempfile.TemporaryDirectory() as d:
+            path = ref.vtk_write(os.path.join(d, "snap"), 3, ndim, n, ext, [("phi", phi), ("pressure", p)],
+                                 [("nvel", (u, v, w))])
+            data = np.frombuffer(open(path, "rb").read(), np.uint8)
+        out[f"vtk{ndim}d"] = dict(ndim=ndim, n=np.array(n), ext=np.array(ext), phi=phi, p=p, u=u, v=v, w=w,
+                                  file=data)
+    return out
+
+
 def main():
     ref = Reference()
     groups = {"mg": mg_cases, "bc": bc_cases, "weno": weno_cases, "rk3": rk3_cases, "fields": field_cases,
-              "fpc": fpc_cases}
+              "fpc": fpc_cases, "vtk": vtk_cases}
     only = sys.argv[1:]
     if only:
         groups = {k: v for k, v in groups.items() if k in only}

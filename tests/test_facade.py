@@ -13,6 +13,8 @@ projection_method and leaves the velocity divergence-free.
 import os
 import subprocess
 
+import numpy as np
+
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -32,15 +34,15 @@ def test_facade_driver_compiles(tmp_path):
 
 
 @pytest.mark.gpu
-def test_facade_driver_runs():
+def test_facade_driver_runs(tmp_path):
     assert os.path.exists(DEMO), "build() compiles scihpc_amd/drivers/facade_demo"
     print("facade_demo 64", flush=True)
-    res = subprocess.run([DEMO, "64"], capture_output=True, text=True, timeout=120)
+    res = subprocess.run([DEMO, "64", str(tmp_path / "ls")], capture_output=True, text=True, timeout=120)
     assert res.returncode == 0, res.stderr
     v = {}
     for line in res.stdout.splitlines():
         k, val = line.split()
-        v[k] = float(val)
+        v[k] = val if k == "vtk_file" else float(val)
     assert v["vcycle_rel_residual"] <= 1e-8 and v["vcycle_cycles"] < 200
     assert 0.02 < v["vcycle_error"] < 0.06
     assert v["solve_rel_residual"] <= 1e-10
@@ -50,3 +52,14 @@ def test_facade_driver_runs():
     assert v["levelset_cells"] > 100 and v["levelset_max_error_near_interface"] < 0.02
     assert v["projection_rel_residual"] <= 1e-10 and v["projection_div_ratio"] < 1e-8
     assert 1 <= v["fpc_cycles"] <= 100
+    # the driver's vtkWriter snapshot: the transported circle and its velocity
+    from scihpc_amd import read_vtk
+    snap = read_vtk(v["vtk_file"])
+    n, h = 64, 1.0 / 64
+    assert snap["dims"] == (n + 1, n + 1, 1) and snap["spacing"] == (h, h, 0.0)
+    dt = 0.25 * h
+    xc = (np.arange(n) + 0.5) * h
+    exact = 0.2 - np.hypot(xc[None, :] - 0.4 - 10 * dt, xc[:, None] - 0.4 - 5 * dt)  # (y, x)
+    near = np.abs(exact) < 0.1
+    assert np.max(np.abs(snap["phi"][0][near] - exact[near])) < 0.02
+    assert np.all(snap["nvel"][0] == np.array([1.0, 0.5, 0.0]))
