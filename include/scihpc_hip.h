@@ -133,7 +133,10 @@ typedef struct {
   int nu1, nu2;      /* smoothing sweeps down / up */
   int precision;     /* SCIHPC_PREC_FP64 or SCIHPC_PREC_MIXED */
   int use_learned;   /* 1: learned initial guess (if attached), judged by the
-                        FP64 residual; 2: the guess kept unjudged (ablation) */
+                        FP64 residual, with back-off: after the k-th
+                        rejection in a row the next 2^k - 1 solves (at most
+                        63) skip the network; 3: judged on every solve;
+                        2: the guess kept unjudged (ablation) */
   int zero_guess;    /* 1: start from p = 0, else from the current level-0 sol */
   int accel;         /* 0: stationary defect correction (one V-cycle per
                         correction); 1: flexible CG (FCG(1)) with the V-cycle
@@ -147,6 +150,7 @@ typedef struct {
   double rel_residual;   /* final ||b - A p|| / ||b||, FP64 */
   double ms;             /* device time of the solve (HIP events) */
   double learned_residual; /* ||b - A p0|| / ||b|| of the learned guess (0 if none) */
+  int learned_skipped;     /* 1: the back-off skipped the network on this solve */
 } scihpc_solve_stats;
 
 /* Solve A p = b on level 0 to opts->rel_tol (the call behind

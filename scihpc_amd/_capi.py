@@ -33,7 +33,7 @@ class SolveOpts(C.Structure):
 class SolveStats(C.Structure):
     _fields_ = [("cycles", C.c_int), ("learned_accepted", C.c_int),
                 ("learned_rejected", C.c_int), ("rel_residual", C.c_double), ("ms", C.c_double),
-                ("learned_residual", C.c_double)]
+                ("learned_residual", C.c_double), ("learned_skipped", C.c_int)]
 
 
 class FpcParams(C.Structure):

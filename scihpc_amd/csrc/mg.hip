@@ -2270,6 +2270,7 @@ struct scihpc_mg {
     SCIHPC_REQUIRE(o.rel_tol > 0 && o.max_cycles >= 0 && o.nu1 >= 0 && o.nu2 >= 0, "bad solve options");
     SCIHPC_REQUIRE(o.nu1 <= 2 && o.nu2 <= 2, "nu1, nu2 <= 2 (tiled kernels stage a 5-cell halo)");
     SCIHPC_REQUIRE(o.accel == 0 || o.accel == 1, "accel must be 0 (stationary) or 1 (flexible CG)");
+    SCIHPC_REQUIRE(o.use_learned >= 0 && o.use_learned <= 3, "use_learned must be 0..3");
     auto &l0 = d[0];
     const unsigned nb = ceil_div(l0.n, kBlock);
     SCIHPC_HIP(hipEventRecord(ev0, stream));
@@ -2280,9 +2281,15 @@ struct scihpc_mg {
     double rr;
     bool zero = o.zero_guess != 0;
     bool overlap = false;
-    if (o.use_learned && learned.active() && bnorm > 0.0) {
-      // use_learned == 2 keeps the guess unjudged (ablation measurements)
-      overlap = o.use_learned == 1 && o.accel == 1 && o.precision == SCIHPC_PREC_MIXED && o.max_cycles > 0;
+    bool gated = false;
+    if (o.use_learned == 1 && learned.active() && learned_skip > 0) {
+      --learned_skip;
+      st.learned_skipped = 1;
+    } else if (o.use_learned && learned.active() && bnorm > 0.0) {
+      // use_learned == 2 keeps the guess unjudged (ablation measurements);
+      // 1 and 3 judge it (1 with the back-off)
+      gated = o.use_learned == 1;
+      overlap = o.use_learned != 2 && o.accel == 1 && o.precision == SCIHPC_PREC_MIXED && o.max_cycles > 0;
       if (!overlap) {
         learned_enqueue(bnorm);
         if (learned_decide(st, bnorm, 1.0, o.use_learned == 2)) {
@@ -2355,6 +2362,7 @@ struct scihpc_mg {
     st.cycles = cycles;
     st.rel_residual = rr;
     st.ms = ms;
+    if (gated) learned_note(st);
     return rr <= o.rel_tol ? SCIHPC_OK : SCIHPC_NOT_CONVERGED;
   }
 
@@ -2382,6 +2390,20 @@ struct scihpc_mg {
   hipEvent_t lev = nullptr;
   double *lpartial = nullptr, *lpartial2 = nullptr, *lscal = nullptr, *lhost = nullptr;
   bool learned_pending = false;
+  // back-off of a guess that keeps losing (use_learned == 1): after the
+  // k-th rejection in a row the next 2^k - 1 solves (at most 63) skip the
+  // network; an accepted guess resets it
+  int learned_streak = 0, learned_skip = 0;
+
+  void learned_note(const scihpc_solve_stats &st) {
+    if (st.learned_accepted > 0) {
+      learned_streak = 0;
+      learned_skip = 0;
+    } else if (st.learned_rejected > 0) {
+      learned_streak = std::min(learned_streak + 1, 6);
+      learned_skip = (1 << learned_streak) - 1;
+    }
+  }
 
   void ensure_learned_buffers() {
     if (lstream) return;
@@ -2900,6 +2922,7 @@ extern "C" int scihpc_mg_attach_learned(scihpc_mg *h, scihpc_unet *net, int leve
     check_level(h, 0);
     if (!net) {
       h->learned = scihpc::LearnedHook{};
+      h->learned_streak = h->learned_skip = 0;
       return;
     }
     SCIHPC_REQUIRE(h->ndim == 2, "the learned guess is 2-D");
@@ -2917,6 +2940,7 @@ extern "C" int scihpc_mg_attach_learned(scihpc_mg *h, scihpc_unet *net, int leve
     }
     h->learned.net = net;
     h->learned.level = level;
+    h->learned_streak = h->learned_skip = 0;
   });
 }
 

@@ -133,14 +133,14 @@ def held_out_gain(mg: Multigrid, configs, rel_tol=1e-8, precision=PREC_MIXED, ac
         mg.set_density(w["rho"])
         mg.set_rhs(w["rhs"])
         res = {}
-        for learned in (False, True):
+        for learned in (0, 3):  # 3: judged on every solve (no back-off)
             best = None
             for _ in range(reps):
                 st = mg.solve(rel_tol=rel_tol, max_cycles=100, precision=precision, accel=accel,
                               use_learned=learned)
                 best = st if best is None or st["ms"] < best["ms"] else best
             res[learned] = best
-        rows.append(dict(cfg=cfg, plain=res[False], learned=res[True]))
+        rows.append(dict(cfg=cfg, plain=res[0], learned=res[3]))
     return rows
 
 

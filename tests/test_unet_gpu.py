@@ -188,17 +188,47 @@ def test_learned_guess_fallback_on_bad_network():
     p[:] = np.nan
     net.set_params(p)
     mg.attach_learned(net, 1)
-    st = mg.solve(rel_tol=1e-8, precision=PREC_MIXED, accel=1, use_learned=True)
+    st = mg.solve(rel_tol=1e-8, precision=PREC_MIXED, accel=1, use_learned=3)
     assert st["converged"] and st["learned_rejected"] == 1 and st["learned_accepted"] == 0
     assert st["cycles"] == plain["cycles"]
     assert rel(mg.sol(0), x_plain) < 1e-12
     # a zero network (guess 0) is rejected as well: it cannot lower ||b||
     net.set_params(np.zeros_like(p))
-    st = mg.solve(rel_tol=1e-8, precision=PREC_FP64, accel=0, use_learned=True)
+    st = mg.solve(rel_tol=1e-8, precision=PREC_FP64, accel=0, use_learned=3)
     assert st["converged"] and st["learned_rejected"] == 1
     mg.attach_learned(None)
     st = mg.solve(rel_tol=1e-8, precision=PREC_FP64, accel=0, use_learned=True)
     assert st["learned_rejected"] == 0 and st["learned_accepted"] == 0
+
+
+def test_learned_guess_backoff_after_rejections():
+    """use_learned=1: after the k-th rejection in a row the next 2^k - 1
+    solves skip the network (the result is the plain solve's); attaching a
+    network again resets the back-off."""
+    w = W.bubble_2d(64, 128, seed=13)
+    mg = Multigrid(2, 64, 128, 1, 1 / 64, 2 / 128, 1.0, "neumann")
+    mg.set_density(w["rho"])
+    mg.set_rhs(w["rhs"])
+    plain = mg.solve(rel_tol=1e-8, precision=PREC_MIXED, accel=1)
+    x_plain = mg.sol(0)
+    H, Wd = mg.shapes[1][1], mg.shapes[1][0]
+    net = UNet(H, Wd, 1)
+    net.set_params(np.zeros(net.num_params, np.float32))  # guess 0: always rejected
+    mg.attach_learned(net, 1)
+    tried = []
+    for s in range(12):
+        st = mg.solve(rel_tol=1e-8, precision=PREC_MIXED, accel=1, use_learned=1)
+        assert st["converged"] and st["cycles"] == plain["cycles"]
+        assert rel(mg.sol(0), x_plain) < 1e-12
+        assert st["learned_rejected"] + st["learned_skipped"] == 1
+        if st["learned_rejected"]:
+            tried.append(s)
+    assert tried == [0, 2, 6], tried
+    mg.attach_learned(net, 1)
+    st = mg.solve(rel_tol=1e-8, precision=PREC_MIXED, accel=1, use_learned=1)
+    assert st["learned_rejected"] == 1 and st["learned_skipped"] == 0
+    st = mg.solve(rel_tol=1e-8, precision=PREC_MIXED, accel=1, use_learned=3)
+    assert st["learned_rejected"] == 1 and st["learned_skipped"] == 0
 
 
 def test_learned_features_match_block_means():
@@ -260,7 +290,7 @@ def test_learned_guess_trained_gains():
         e -= e.mean()
         err = np.sqrt((e @ mg.apply(e)) / (xs @ mg.apply(xs)))  # both energies are negative
         errs.append(err)
-        st = mg.solve(rel_tol=1e-8, precision=PREC_MIXED, accel=1, use_learned=True)
+        st = mg.solve(rel_tol=1e-8, precision=PREC_MIXED, accel=1, use_learned=3)
         print(f"r={cfg['radius']:.3f} energy err {err:.3f} guess residual {st['learned_residual']:.3e} "
               f"accepted {st['learned_accepted']} cycles plain {plain['cycles']} learned {st['cycles']}", flush=True)
         assert st["converged"] and st["rel_residual"] <= 1e-8
