@@ -210,6 +210,44 @@ SCIHPC_API int scihpc_proj_step(scihpc_proj *h, scihpc_mg *mg, double dt,
                                 const scihpc_solve_opts *opts, const double *rho, double *u,
                                 double *v, double *w, double *p, scihpc_solve_stats *stats);
 
+/* Momentum predictor of ab_solve_sec (projection_method.cpp:779-783).
+ * Flow parameters of problem_parameters (global.h:25-54); froude <= 0
+ * disables gravity and weber <= 0 surface tension, as in the reference. */
+typedef struct {
+  double dt;
+  double reynolds, froude, weber;
+  double ls_width;                       /* interface half width (1.5 h in the drivers) */
+  double density_ratio, viscosity_ratio; /* second fluid / first fluid */
+  int positive_ref;                      /* 1: phi > 0 is the reference fluid */
+} scihpc_flow_params;
+
+/* projection_method::find_source_sec (:314-382) + find_intermediate_velocity
+ * (:384-402): Heaviside density/viscosity and delta of phi, second-order
+ * level-set derivatives, per direction the face transfer all_to_face_*
+ * (wrapper_func.cpp:177-262) into nvel with its BCs, convection_sec
+ * (source.cpp:42-73) and add_stress_*, gravity, surface tension (curvature,
+ * wrapper_func.cpp:115-172), then AB2 u += dt (1.5 s - 0.5 s_old) and the
+ * velocity BCs.  The sources u_src, v_src, w_src are handle state (zero
+ * after create, as the reference's arrays; see set/get_sources).  phi, u,
+ * v, w, nu, nv, nw padded host arrays; u, v, w, nu, nv, nw are updated;
+ * rho (nullable) receives find_density (the pressure step's density).
+ * Velocity BCs: the handle's (both for vel and nvel, as the drivers). */
+SCIHPC_API int scihpc_proj_momentum_step(scihpc_proj *h, const scihpc_flow_params *prm,
+                                         const double *phi, double *u, double *v, double *w,
+                                         double *nu, double *nv, double *nw, double *rho);
+/* projection_method::u_src/v_src/w_src (projection_method.h:18) */
+SCIHPC_API int scihpc_proj_set_sources(scihpc_proj *h, const double *su, const double *sv,
+                                       const double *sw);
+SCIHPC_API int scihpc_proj_get_sources(scihpc_proj *h, double *su, double *sv, double *sw);
+/* node_from_face + nvel->apply_nvel_bc (wrapper_func.cpp:264-293,
+ * wrapper.cpp:28-33): cell-centred velocities after the pressure step */
+SCIHPC_API int scihpc_proj_node_velocity(scihpc_proj *h, const double *u, const double *v,
+                                         const double *w, double *nu, double *nv, double *nw);
+/* Times the device part of scihpc_proj_momentum_step on the resident
+ * fields: average ms per step over `iters`. */
+SCIHPC_API int scihpc_proj_time_momentum(scihpc_proj *h, const scihpc_flow_params *prm, int iters,
+                                         double *ms);
+
 /* ------------------------------------------------------------------ */
 /* Learned pressure guess (U-Net).  No reference counterpart: SciHPC has no
  * learned component.  A 4-level U-Net (16/32/64/128 channels, 3x3

@@ -1558,3 +1558,324 @@ int orc_vtk_write(const char *path, int ndim, int nx, int ny, int nz,
   }
   return fclose(f) == 0 ? 0 : -1;
 }
+
+/* ================================================================== */
+/* momentum predictor: projection_method::find_source_sec (:314-382)   */
+/* with add_stress_x/y/z (:31-181), add_body_force (:183-203),         */
+/* add_surface_force (:205-240), then find_intermediate_velocity       */
+/* (:384-402).  Level-set helpers: Heaviside/Delta (lsm.cpp:8-30),     */
+/* find_delta / find_density / find_viscosity / find_curvature         */
+/* (wrapper_func.cpp:20-29, 115-172, 387-416), identity_with_          */
+/* extrapolation[_face_x/y/z] (flux.cpp:98-375), second_order_solver   */
+/* (second_order_solver.cpp:13-242), convection_sec (source.cpp:42-73),*/
+/* all_to_face_x/y/z and node_from_face (wrapper_func.cpp:177-293).    */
+/* Every array is a padded field; derivative arrays that the reference */
+/* only writes on part of the padded range are zero elsewhere (they are*/
+/* zero-initialised, init_array, and nothing else writes them).        */
+
+static const double orc_pi = 3.14159265358979323846; /* global.h:13 */
+
+static double orc_delta(double x, double w) {
+  if (x < -w) return 0;
+  if (x > w) return 0;
+  return 0.5 * (1 + cos(orc_pi * x / w)) / w;
+}
+
+static double orc_heav(double x, double w) {
+  if (x < -w) return 0;
+  if (x > w) return 1;
+  return 0.5 * (1 + x / w + sin(orc_pi * x / w) / orc_pi);
+}
+
+/* identity_with_extrapolation[_face_a]: flux = data, then ghosts copied
+ * axis by axis (x, y, z) over every index of the other axes; on the face
+ * axis `fa` the low side copies the wall face (index off-1) into the
+ * ghosts below it and keeps the face itself (flux.cpp:166-375). fa = -1:
+ * plain (flux.cpp:98-149). */
+static void orc_extrap(const orc_pf *p, const double *data, double *flux, int fa) {
+  const long tot = (long)p->N[0] * p->N[1] * p->N[2];
+  memcpy(flux, data, sizeof(double) * tot);
+  for (int a = 0; a < p->ndim; a++) {
+    const int t1 = (a + 1) % 3, t2 = (a + 2) % 3;
+    const int lo = p->off[a], hi = p->off[a] + p->n[a] - 1;
+    for (int u = 0; u < p->N[t1]; u++)
+      for (int v = 0; v < p->N[t2]; v++) {
+        const long base = (long)u * p->s[t1] + (long)v * p->s[t2];
+        for (int q = 1; q <= ORC_GHC; q++) flux[base + (hi + q) * p->s[a]] = flux[base + hi * p->s[a]];
+        if (a == fa) {
+          for (int q = 2; q <= ORC_GHC; q++) flux[base + (lo - q) * p->s[a]] = flux[base + (lo - 1) * p->s[a]];
+        } else {
+          for (int q = 1; q <= ORC_GHC; q++) flux[base + (lo - q) * p->s[a]] = flux[base + lo * p->s[a]];
+        }
+      }
+  }
+}
+
+/* second_order_solver::find_fx/fy/fz(f) on the whole padded range, one-
+ * sided at the array ends (second_order_solver.cpp:13-88) */
+static void orc_sec_full(const orc_pf *p, int a, double h, const double *f, double *d1, double *d2) {
+  const long st = p->s[a];
+  for (int I = 0; I < p->N[0]; I++)
+    for (int J = 0; J < p->N[1]; J++)
+      for (int K = 0; K < p->N[2]; K++) {
+        const int idx[3] = {I, J, K};
+        const long c = PIDX(*p, I, J, K);
+        const int q = idx[a];
+        if (q == 0) {
+          d1[c] = (-1.5 * f[c] + 2 * f[c + st] - 0.5 * f[c + 2 * st]) / h;
+          d2[c] = (2 * f[c] - 5 * f[c + st] + 4 * f[c + 2 * st] - f[c + 3 * st]) / (h * h);
+        } else if (q == p->N[a] - 1) {
+          d1[c] = (1.5 * f[c] - 2 * f[c - st] + 0.5 * f[c - 2 * st]) / h;
+          d2[c] = (2 * f[c] - 5 * f[c - st] + 4 * f[c - 2 * st] - f[c - 3 * st]) / (h * h);
+        } else {
+          d1[c] = (f[c + st] - f[c - st]) / (2 * h);
+          d2[c] = (f[c + st] - 2 * f[c] + f[c - st]) / (h * h);
+        }
+      }
+}
+
+/* second_order_solver::mixed_xy / mixed_yz / mixed_zx: d/d(axis a) of g,
+ * one-sided at the array ends (second_order_solver.cpp:173-230) */
+static void orc_sec_mixed(const orc_pf *p, int a, double h, const double *g, double *out) {
+  const long st = p->s[a];
+  for (int I = 0; I < p->N[0]; I++)
+    for (int J = 0; J < p->N[1]; J++)
+      for (int K = 0; K < p->N[2]; K++) {
+        const int idx[3] = {I, J, K};
+        const long c = PIDX(*p, I, J, K);
+        const int q = idx[a];
+        if (q == 0) out[c] = (-1.5 * g[c] + 2 * g[c + st] - 0.5 * g[c + 2 * st]) / h;
+        else if (q == p->N[a] - 1) out[c] = (1.5 * g[c] - 2 * g[c - st] + 0.5 * g[c - 2 * st]) / h;
+        else out[c] = (g[c + st] - g[c - st]) / (2 * h);
+      }
+}
+
+/* second_order_solver::find_fx/fy/fz(f, vel): upwind by vel_a > 0 on the
+ * interior of axis a, every index of the other axes
+ * (second_order_solver.cpp:90-148) */
+static void orc_sec_upwind(const orc_pf *p, int a, double h, const double *f, const double *va,
+                           double *d1, double *d2) {
+  const long st = p->s[a];
+  for (int I = 0; I < p->N[0]; I++)
+    for (int J = 0; J < p->N[1]; J++)
+      for (int K = 0; K < p->N[2]; K++) {
+        const int idx[3] = {I, J, K};
+        const int q = idx[a];
+        if (q < p->off[a] || q >= p->off[a] + p->n[a]) continue;
+        const long c = PIDX(*p, I, J, K);
+        if (va[c] > 0.0) d1[c] = (1.5 * f[c] - 2 * f[c - st] + 0.5 * f[c - 2 * st]) / h;
+        else d1[c] = (-1.5 * f[c] + 2 * f[c + st] - 0.5 * f[c + 2 * st]) / h;
+        d2[c] = (f[c + st] - 2 * f[c] + f[c - st]) / (h * h);
+      }
+}
+
+typedef struct {
+  double dt, Re, Fr, We, ls_width, density_ratio, viscosity_ratio;
+  int positive_ref;
+} orc_flow;
+
+static void orc_vel_bc(int ndim, const int *n, const double *h, const int *types, int comp, int stag, double *f) {
+  /* the component's BC factory: wall-normal SLIP made NO_SLIP (wrapper.cpp:111-125) */
+  int t[6];
+  for (int q = 0; q < 6; q++) t[q] = (q / 2 == comp && types[q] == 4) ? 3 : types[q];
+  double z[6] = {0, 0, 0, 0, 0, 0};
+  orc_bc_apply(ndim, n[0], n[1], n[2], h[0], h[1], h[2], t, z, stag, f);
+}
+
+int orc_momentum_step(int ndim, int nx, int ny, int nz, double dx, double dy, double dz,
+                      const double *prm8, const int *vel_types, const double *phi, double *vel0,
+                      double *vel1, double *vel2, double *nv0, double *nv1, double *nv2,
+                      double *src0, double *src1, double *src2, double *rho_out) {
+  const orc_pf p = pf_make(ndim, nx, ny, nz);
+  const int n[3] = {nx, ny, nz};
+  const double hh[3] = {dx, dy, dz};
+  orc_flow F = {prm8[0], prm8[1], prm8[2], prm8[3], prm8[4], prm8[5], prm8[6], (int)prm8[7]};
+  const long T = (long)p.N[0] * p.N[1] * p.N[2];
+  double *vel[3] = {vel0, vel1, vel2}, *nv[3] = {nv0, nv1, nv2}, *src[3] = {src0, src1, src2};
+  /* scratch: 3 old sources, delta, density, viscosity, curvature, level-set
+   * flux and 9 derivatives, flux + 6 derivatives of one velocity component,
+   * flux + 2 derivatives of one nvel component */
+  const int NS = 3 + 4 + 10 + 7 + 3;
+  double *mem = calloc((size_t)NS * T, sizeof(double));
+  if (!mem) return -1;
+  double *old[3] = {mem, mem + T, mem + 2 * T};
+  double *dl = mem + 3 * T, *rho = mem + 4 * T, *mu = mem + 5 * T, *curv = mem + 6 * T;
+  double *lfl = mem + 7 * T, *lf[3] = {mem + 8 * T, mem + 9 * T, mem + 10 * T};
+  double *lff[3] = {mem + 11 * T, mem + 12 * T, mem + 13 * T};
+  double *lfxy = mem + 14 * T, *lfyz = mem + 15 * T, *lfzx = mem + 16 * T;
+  double *vfl = mem + 17 * T, *vd1[3] = {mem + 18 * T, mem + 19 * T, mem + 20 * T};
+  double *vd2[3] = {mem + 21 * T, mem + 22 * T, mem + 23 * T};
+  double *nfl = mem + 24 * T, *nd1 = mem + 25 * T, *nd2 = mem + 26 * T;
+  const double ref = F.positive_ref ? 1.0 : -1.0;
+
+  for (int a = 0; a < 3; a++) memcpy(old[a], src[a], sizeof(double) * T);
+  /* find_delta, find_density, find_viscosity (heaviside itself is unused) */
+  for (long c = 0; c < T; c++) {
+    dl[c] = orc_delta(phi[c], F.ls_width);
+    const double H = orc_heav(phi[c], F.ls_width);
+    rho[c] = F.positive_ref ? H + (1.0 - H) * F.density_ratio : (1.0 - H) + H * F.density_ratio;
+    mu[c] = F.positive_ref ? H + (1.0 - H) * F.viscosity_ratio : (1.0 - H) + H * F.viscosity_ratio;
+  }
+  /* identity_with_extrapolation + secSol->find_derivatives_all (lsf) */
+  orc_extrap(&p, phi, lfl, -1);
+  for (int a = 0; a < ndim; a++) orc_sec_full(&p, a, hh[a], lfl, lf[a], lff[a]);
+  if (ndim > 1) orc_sec_mixed(&p, 1, dy, lf[0], lfxy);
+  if (ndim > 2) {
+    orc_sec_mixed(&p, 2, dz, lf[1], lfyz);
+    orc_sec_mixed(&p, 0, dx, lf[2], lfzx);
+  }
+
+  for (int a = 0; a < ndim; a++) {
+    const long sa = p.s[a];
+    /* all_to_face_a (wrapper_func.cpp:177-262): nvel_a = vel_a, the other
+     * components averaged onto face a from their own faces */
+    const int lo2 = ndim == 2 ? 0 : 1, hi2 = ndim == 2 ? p.N[2] : p.N[2] - 1;
+    for (int I = 1; I < p.N[0] - 1; I++)
+      for (int J = 1; J < p.N[1] - 1; J++)
+        for (int K = lo2; K < hi2; K++) {
+          const long c = PIDX(p, I, J, K);
+          nv[a][c] = vel[a][c];
+          for (int b = 0; b < ndim; b++) {
+            if (b == a) continue;
+            const long sb = p.s[b];
+            if (b < a)  /* components below a: shifted +a, -b (x onto y: u[i][j], u[i][j+1], u[i-1][j], u[i-1][j+1]) */
+              nv[b][c] = 0.25 * (vel[b][c] + vel[b][c + sa] + vel[b][c - sb] + vel[b][c - sb + sa]);
+            else        /* components above a: vel_b[c], [c+a], [c-b], [c+a-b] */
+              nv[b][c] = 0.25 * (vel[b][c] + vel[b][c + sa] + vel[b][c - sb] + vel[b][c + sa - sb]);
+          }
+        }
+    /* nvel->apply_vel_a_bc (wrapper.cpp:40-59) */
+    for (int b = 0; b < 3; b++) orc_vel_bc(ndim, n, hh, vel_types, b, a, nv[b]);
+    /* convection_sec(vel_a, nvel, src_a, identity_with_extrapolation_face_a) */
+    orc_extrap(&p, vel[a], vfl, a);
+    for (int q = 0; q < 3; q++) {
+      memset(vd1[q], 0, sizeof(double) * T);
+      memset(vd2[q], 0, sizeof(double) * T);
+    }
+    for (int q = 0; q < ndim; q++) orc_sec_upwind(&p, q, hh[q], vfl, nv[q], vd1[q], vd2[q]);
+    for (long c = 0; c < T; c++) src[a][c] = -nv[0][c] * vd1[0][c];
+    if (ndim > 1)
+      for (long c = 0; c < T; c++) src[a][c] -= nv[1][c] * vd1[1][c];
+    if (ndim > 2)
+      for (long c = 0; c < T; c++) src[a][c] -= nv[2][c] * vd1[2][c];
+    /* add_stress_a: derivatives along a of the other nvel components, then
+     * the interior stress term */
+    double *cross[3] = {NULL, NULL, NULL};
+    double *cross_mem = calloc((size_t)3 * T, sizeof(double));
+    if (!cross_mem) {
+      free(mem);
+      return -1;
+    }
+    for (int b = 0; b < ndim; b++) {
+      if (b == a) continue;
+      cross[b] = cross_mem + b * T;
+      orc_extrap(&p, nv[b], nfl, a);
+      memset(nd2, 0, sizeof(double) * T);
+      orc_sec_upwind(&p, a, hh[a], nfl, nv[a], cross[b], nd2);
+    }
+    (void)nd1;
+    for (int I = 0; I < n[0]; I++)
+      for (int J = 0; J < (ndim > 1 ? n[1] : 1); J++)
+        for (int K = 0; K < (ndim > 2 ? n[2] : 1); K++) {
+          const long c = PIDX(p, I + p.off[0], J + p.off[1], K + p.off[2]);
+          const double delta = 0.5 * (dl[c] + dl[c + sa]);
+          const double m = 0.5 * (mu[c] + mu[c + sa]);
+          const double r = 0.5 * (rho[c] + rho[c + sa]);
+          double ph[3] = {0, 0, 0};
+          for (int q = 0; q < ndim; q++) ph[q] = 0.5 * (lf[q][c] + lf[q][c + sa]);
+          double p1, p2;
+          if (a == 0) {
+            p1 = vd2[0][c] + vd2[1][c];
+            p2 = 2.0 * ph[0] * vd1[0][c] + ph[1] * (vd1[1][c] + cross[1][c]);
+            if (ndim > 2) {
+              p1 += vd2[2][c];
+              p2 += ph[2] * (vd1[2][c] + cross[2][c]);
+            }
+          } else if (a == 1) {
+            p1 = vd2[0][c] + vd2[1][c];
+            p2 = 2.0 * ph[1] * vd1[1][c] + ph[0] * (vd1[0][c] + cross[0][c]);
+            if (ndim > 2) {
+              p1 += vd2[2][c];
+              p2 += ph[2] * (vd1[2][c] + cross[2][c]);
+            }
+          } else { /* add_stress_z has no w_zz term (projection_method.cpp:171) */
+            p1 = vd2[0][c] + vd2[1][c];
+            p2 = 2.0 * ph[2] * vd1[2][c] + ph[0] * (vd1[0][c] + cross[0][c]) + ph[1] * (vd1[1][c] + cross[1][c]);
+          }
+          p1 *= m;
+          p2 *= ref * delta * (1.0 - F.viscosity_ratio);
+          src[a][c] += (p1 + p2) / F.Re / r;
+        }
+    free(cross_mem);
+  }
+  /* add_body_force: gravity along y (2-D) or z (3-D) on every padded cell */
+  if (F.Fr > 0.0) {
+    double *g = src[ndim == 2 ? 1 : 2];
+    if (ndim >= 2)
+      for (long c = 0; c < T; c++) g[c] += -1.0 / F.Fr;
+  }
+  /* add_surface_force */
+  if (F.We > 0.0) {
+    for (long c = 0; c < T; c++) {
+      double k;
+      if (ndim == 2) {
+        const double fx = lf[0][c], fy = lf[1][c], fxx = lff[0][c], fyy = lff[1][c], fxy = lfxy[c];
+        k = (fxx * fy * fy + fyy * fx * fx - 2.0 * fxy * fx * fy) / pow(fx * fx + fy * fy, 1.5);
+        if (k > 0.0) k = 1.0 / (1.0 / k + phi[c]);
+        else k = -1.0 / (-1.0 / k + phi[c]);
+      } else {
+        const double fx = lf[0][c], fy = lf[1][c], fz = lf[2][c];
+        const double fxx = lff[0][c], fyy = lff[1][c], fzz = lff[2][c];
+        const double fxy = lfxy[c], fxz = lfzx[c], fyz = lfyz[c];
+        k = (fx * fx * (fyy + fzz) + fy * fy * (fxx + fzz) + fz * fz * (fxx + fyy) -
+             2.0 * (fx * fy * fxy + fx * fz * fxz + fy * fz * fyz)) /
+            pow(fx * fx + fy * fy + fz * fz, 1.5);
+        if (k > 0.0) k = 1.0 / (2.0 / k + phi[c]);
+        else k = -1.0 / (-2.0 / k + phi[c]);
+      }
+      curv[c] = k;
+    }
+    for (int I = 0; I < p.N[0] - 1; I++)
+      for (int J = 0; J < p.N[1] - 1; J++)
+        for (int K = 0; K < p.N[2]; K++) {
+          const long c = PIDX(p, I, J, K);
+          for (int a = 0; a < ndim; a++) {
+            const long sa = p.s[a];
+            if (a == 2 && K >= p.N[2] - 1) continue;
+            const double cv = 0.5 * (curv[c] + curv[c + sa]);
+            const double delta = 0.5 * (dl[c] + dl[c + sa]);
+            const double pa = 0.5 * (lf[a][c] + lf[a][c + sa]);
+            const double r = 0.5 * (rho[c] + rho[c + sa]);
+            src[a][c] -= cv * delta * pa / (F.We * r);
+          }
+        }
+  }
+  /* find_intermediate_velocity: AB2 on every padded cell, then the BCs */
+  for (int a = 0; a < 3; a++)
+    for (long c = 0; c < T; c++) vel[a][c] += F.dt * (1.5 * src[a][c] - 0.5 * old[a][c]);
+  for (int a = 0; a < 3; a++) orc_vel_bc(ndim, n, hh, vel_types, a, a, vel[a]);
+  if (rho_out) memcpy(rho_out, rho, sizeof(double) * T);
+  free(mem);
+  return 0;
+}
+
+/* node_from_face + nvel->apply_nvel_bc (wrapper_func.cpp:264-293,
+ * wrapper.cpp:28-33): cell-centred velocities from the face values */
+void orc_node_velocity(int ndim, int nx, int ny, int nz, double dx, double dy, double dz,
+                       const int *vel_types, const double *u, const double *v, const double *w,
+                       double *nu, double *nv, double *nw) {
+  const orc_pf p = pf_make(ndim, nx, ny, nz);
+  const int n[3] = {nx, ny, nz};
+  const double hh[3] = {dx, dy, dz};
+  const int klo = ndim == 2 ? 0 : 1;
+  for (int I = 1; I < p.N[0] - 1; I++)
+    for (int J = 1; J < p.N[1] - 1; J++)
+      for (int K = klo; K < p.N[2]; K++) {
+        const long c = PIDX(p, I, J, K);
+        nu[c] = 0.5 * (u[c] + u[c - p.s[0]]);
+        nv[c] = 0.5 * (v[c] + v[c - p.s[1]]);
+        if (ndim == 3) nw[c] = 0.5 * (w[c] + w[c - p.s[2]]);
+      }
+  double *f[3] = {nu, nv, nw};
+  for (int a = 0; a < 3; a++) orc_vel_bc(ndim, n, hh, vel_types, a, -1, f[a]);
+}

@@ -207,6 +207,26 @@ int orc_vtk_write(const char *path, int ndim, int nx, int ny, int nz,
                   int nvector, const char *const *vnames,
                   const double *const *vectors);
 
+/* ------------------------------------------------------------------ */
+/* Momentum predictor: projection_method::find_source_sec
+ * (projection_method.cpp:314-382: find_delta/density/viscosity, level-set
+ * second-order derivatives, per direction all_to_face + nvel BCs +
+ * convection_sec + add_stress, add_body_force, add_surface_force) and
+ * find_intermediate_velocity (:384-402, AB2 + velocity BCs).
+ * prm8 = {dt, Re, Fr, We, ls_width, density_ratio, viscosity_ratio,
+ * positive_ref}.  vel* in/out; nv* (nvel) in/out; src* in = previous
+ * sources (u_src, v_src, w_src), out = new sources; rho_out (nullable) =
+ * find_density.  ndim 2 or 3.  Returns 0 (-1 if out of memory). [golden] */
+int orc_momentum_step(int ndim, int nx, int ny, int nz, double dx, double dy, double dz,
+                      const double *prm8, const int *vel_types, const double *phi, double *vel0,
+                      double *vel1, double *vel2, double *nv0, double *nv1, double *nv2,
+                      double *src0, double *src1, double *src2, double *rho_out);
+/* node_from_face + apply_nvel_bc (wrapper_func.cpp:264-293,
+ * wrapper.cpp:28-33)                                           [golden] */
+void orc_node_velocity(int ndim, int nx, int ny, int nz, double dx, double dy, double dz,
+                       const int *vel_types, const double *u, const double *v, const double *w,
+                       double *nu, double *nv, double *nw);
+
 #ifdef __cplusplus
 }
 #endif

@@ -114,6 +114,8 @@ class Oracle:
              [_i, _i, _i, _i, _d, _d, _d, _d, _d, _dp, _dp, _dp, _dp, _dp, _dp])
         _sig(L, "orc_pressure_guess", None, [_l, _d, _d, _dp, _dp, _dp])
         _sig(L, "orc_fpc_step", _i, [_i, _i, _i, _i, _d, _d, _d, _d, _d, _d, _d, _d, _i, _ip] + [_dp] * 8)
+        _sig(L, "orc_momentum_step", _i, [_i, _i, _i, _i, _d, _d, _d, _dp, _ip, _dp] + [_dp] * 10)
+        _sig(L, "orc_node_velocity", None, [_i, _i, _i, _i, _d, _d, _d, _ip] + [_dp] * 6)
         _sig(L, "orc_vtk_write", _i, [C.c_char_p, _i, _i, _i, _i, _dp, _dp, _i, C.c_void_p, C.c_void_p,
                                       _i, C.c_void_p, C.c_void_p])
 
@@ -244,6 +246,8 @@ class Reference:
              [_i, _i, _i, _i, _dp, _d, _i, _d, _d, _dp, _dp, _dp, _dp, _dp, _dp])
         _sig(L, "ref_fpc_coefficients", None, [_i, _i, _i, _i, _dp, _d, _dp, _dp, _dp])
         _sig(L, "ref_fpc_step", _i, [_i, _i, _i, _i, _dp, _d, _d, _d, _d, _d, _ip] + [_dp] * 8)
+        _sig(L, "ref_momentum_step", _i, [_i, _i, _i, _i, _dp, _dp, _ip, _dp] + [_dp] * 10)
+        _sig(L, "ref_node_velocity", _i, [_i, _i, _i, _i, _dp, _ip] + [_dp] * 6)
         _sig(L, "ref_vtk_write", _i, [C.c_char_p, C.c_uint, _i, _i, _i, _i, _dp, _i, C.c_void_p, C.c_void_p,
                                       _i, C.c_void_p, C.c_void_p])
 

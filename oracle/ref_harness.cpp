@@ -409,4 +409,86 @@ int ref_vtk_write(const char *prefix, unsigned id, int ndim, int nx, int ny, int
   return 0;
 }
 
+// projection_method::find_source_sec + find_intermediate_velocity on fresh
+// wrappers: vel and nvel with the given BC types (both), lsf with phi,
+// parameters prm8 = {dt, Re, Fr, We, ls_width, density_ratio,
+// viscosity_ratio, positive_ref}.  src_* are u_src/v_src/w_src on entry
+// (the previous step's sources) and on exit; nvel_* are the node
+// velocities on entry and the last face transfer on exit.
+int ref_momentum_step(int ndim, int nx, int ny, int nz, const double *ext, const double *prm8,
+                      const int *vel_types, const double *phi, double *u, double *v, double *w,
+                      double *nu, double *nv, double *nw, double *su, double *sv, double *sw,
+                      double *rho_out) {
+  structured_grid *geo = make_grid(ndim, nx, ny, nz, ext);
+  int nt[6] = {0, 0, 0, 0, 0, 0};
+  double vv[6] = {0, 0, 0, 0, 0, 0};
+  wrapper *lsf = make_wrapper(true, geo, ndim, nt, vv);
+  wrapper *vel = make_wrapper(false, geo, ndim, vel_types, vv);
+  wrapper *nvel = make_wrapper(false, geo, ndim, vel_types, vv);
+  auto *param = new problem_parameters();
+  param->dt = prm8[0];
+  param->Reynolds_number = prm8[1];
+  param->Froude_number = prm8[2];
+  param->Weber_number = prm8[3];
+  param->ls_width = prm8[4];
+  param->density_ratio = prm8[5];
+  param->viscosity_ratio = prm8[6];
+  param->positive_ref = prm8[7] != 0.0;
+  auto *dummy = dummy_data_alloc(lsf->scalar);
+  auto *solvers = SharedSolvers_alloc(lsf->scalar, geo);
+  for (wrapper *x : {lsf, vel, nvel}) {
+    x->link_params(param);
+    x->link_dummy(dummy);
+    x->link_solvers(solvers);
+  }
+  scalar_data *s = lsf->scalar;
+  to_ref(s, s->data, phi);
+  to_ref(s, vel->vector->x.data, u);
+  to_ref(s, vel->vector->y.data, v);
+  to_ref(s, vel->vector->z.data, w);
+  to_ref(s, nvel->vector->x.data, nu);
+  to_ref(s, nvel->vector->y.data, nv);
+  to_ref(s, nvel->vector->z.data, nw);
+  projection_method pm(s);
+  to_ref(s, pm.u_src, su);
+  to_ref(s, pm.v_src, sv);
+  to_ref(s, pm.w_src, sw);
+  pm.find_source_sec(vel, nvel, lsf);
+  pm.find_intermediate_velocity(vel);
+  from_ref(s, vel->vector->x.data, u);
+  from_ref(s, vel->vector->y.data, v);
+  from_ref(s, vel->vector->z.data, w);
+  from_ref(s, nvel->vector->x.data, nu);
+  from_ref(s, nvel->vector->y.data, nv);
+  from_ref(s, nvel->vector->z.data, nw);
+  from_ref(s, pm.u_src, su);
+  from_ref(s, pm.v_src, sv);
+  from_ref(s, pm.w_src, sw);
+  from_ref(s, dummy->density, rho_out);
+  return 0;
+}
+
+// node_from_face + nvel->apply_nvel_bc (the end of ab_solve_sec)
+int ref_node_velocity(int ndim, int nx, int ny, int nz, const double *ext, const int *vel_types,
+                      const double *u, const double *v, const double *w, double *nu, double *nv,
+                      double *nw) {
+  structured_grid *geo = make_grid(ndim, nx, ny, nz, ext);
+  double vv[6] = {0, 0, 0, 0, 0, 0};
+  wrapper *vel = make_wrapper(false, geo, ndim, vel_types, vv);
+  wrapper *nvel = make_wrapper(false, geo, ndim, vel_types, vv);
+  scalar_data *s = &vel->vector->x;
+  to_ref(s, vel->vector->x.data, u);
+  to_ref(s, vel->vector->y.data, v);
+  to_ref(s, vel->vector->z.data, w);
+  to_ref(s, nvel->vector->x.data, nu);
+  to_ref(s, nvel->vector->y.data, nv);
+  to_ref(s, nvel->vector->z.data, nw);
+  node_from_face(vel, nvel);
+  nvel->apply_nvel_bc();
+  from_ref(s, nvel->vector->x.data, nu);
+  from_ref(s, nvel->vector->y.data, nv);
+  from_ref(s, nvel->vector->z.data, nw);
+  return 0;
+}
+
 }  // extern "C"

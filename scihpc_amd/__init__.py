@@ -14,7 +14,7 @@ import ctypes as C
 import numpy as np
 
 from ._capi import (BC_CODES, MG_DIRICHLET, MG_NEUMANN, NOT_CONVERGED, OK, PREC_FP64,
-                    PREC_MIXED, FpcParams, ScihpcError, SolveOpts, SolveStats, check, lib)
+                    PREC_MIXED, FlowParams, FpcParams, ScihpcError, SolveOpts, SolveStats, check, lib)
 
 __all__ = ["Multigrid", "LevelSet", "Projection", "UNet", "VtkWriter", "ScihpcError", "PREC_FP64",
            "PREC_MIXED", "padded_shape", "read_vtk"]
@@ -373,6 +373,45 @@ class Projection:
         check(self._lib.scihpc_proj_fpc_step(self.h, mg.h, C.byref(prm), _f64(rho), _f64(p_old), _f64(p_old2),
                                              u, v, w, p, ph, C.byref(cyc)))
         return u, v, w, p, ph, cyc.value
+
+    @staticmethod
+    def flow_params(dt, reynolds, froude=-1.0, weber=-1.0, ls_width=None, density_ratio=1e-3,
+                    viscosity_ratio=1e-2, positive_ref=True, h=None):
+        """problem_parameters of the momentum step (global.h:25-54);
+        ls_width defaults to 1.5 h as the drivers set it."""
+        if ls_width is None:
+            ls_width = 1.5 * h
+        return FlowParams(dt, reynolds, froude, weber, ls_width, density_ratio, viscosity_ratio, int(positive_ref))
+
+    def momentum_step(self, prm, phi, u, v, w, nu, nv, nw, want_rho=False):
+        """find_source_sec + find_intermediate_velocity (projection_method.cpp:
+        314-402).  Returns (u, v, w, nu, nv, nw[, rho]) as new padded arrays;
+        the sources stay in the handle for the next step's AB2."""
+        out = [_f64(a).copy() for a in (u, v, w, nu, nv, nw)]
+        rho = np.zeros_like(out[0]) if want_rho else None
+        check(self._lib.scihpc_proj_momentum_step(self.h, C.byref(prm), _f64(phi), *out,
+                                                  rho.ctypes.data if rho is not None else None))
+        return tuple(out) + ((rho,) if want_rho else ())
+
+    def set_sources(self, su, sv, sw):
+        check(self._lib.scihpc_proj_set_sources(self.h, _f64(su), _f64(sv), _f64(sw)))
+
+    def sources(self):
+        n = int(np.prod(self.shape))
+        s = [np.zeros(n) for _ in range(3)]
+        check(self._lib.scihpc_proj_get_sources(self.h, *s))
+        return tuple(s)
+
+    def node_velocity(self, u, v, w, nu, nv, nw):
+        """node_from_face + apply_nvel_bc; returns (nu, nv, nw)."""
+        out = [_f64(a).copy() for a in (nu, nv, nw)]
+        check(self._lib.scihpc_proj_node_velocity(self.h, _f64(u), _f64(v), _f64(w), *out))
+        return tuple(out)
+
+    def time_momentum(self, prm, iters=20):
+        ms = C.c_double()
+        check(self._lib.scihpc_proj_time_momentum(self.h, C.byref(prm), iters, C.byref(ms)))
+        return ms.value
 
     def step(self, mg, rho, u, v, w, dt, rel_tol=1e-8, precision=PREC_FP64, accel=1, max_cycles=200):
         """Variable-density projection; returns (u, v, w, p, stats)."""

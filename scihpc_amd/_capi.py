@@ -41,6 +41,12 @@ class FpcParams(C.Structure):
                 ("rho12", C.c_double), ("ppe_tol", C.c_double), ("max_cycles", C.c_int)]
 
 
+class FlowParams(C.Structure):
+    _fields_ = [("dt", C.c_double), ("reynolds", C.c_double), ("froude", C.c_double), ("weber", C.c_double),
+                ("ls_width", C.c_double), ("density_ratio", C.c_double), ("viscosity_ratio", C.c_double),
+                ("positive_ref", C.c_int)]
+
+
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
@@ -114,6 +120,11 @@ SIGNATURES = {
     "scihpc_proj_destroy": [_vp],
     "scihpc_proj_fpc_step": [_vp, _vp, C.POINTER(FpcParams)] + [_dp] * 8 + [_pi],
     "scihpc_proj_step": [_vp, _vp, _d, C.POINTER(SolveOpts)] + [_dp] * 5 + [C.POINTER(SolveStats)],
+    "scihpc_proj_momentum_step": [_vp, C.POINTER(FlowParams), _dp] + [_dp] * 6 + [_vp],
+    "scihpc_proj_set_sources": [_vp, _dp, _dp, _dp],
+    "scihpc_proj_get_sources": [_vp, _dp, _dp, _dp],
+    "scihpc_proj_node_velocity": [_vp] + [_dp] * 6,
+    "scihpc_proj_time_momentum": [_vp, C.POINTER(FlowParams), _i, _pd],
     "scihpc_ls_create": [_i, _i, _i, _i, _d, _d, _d, _ip, _dp, C.POINTER(_vp)],
     "scihpc_ls_destroy": [_vp],
     "scihpc_ls_set_phi": [_vp, _dp],

@@ -13,6 +13,7 @@
 // reference's order (x, y, z; right side before left side).  Corner ghosts
 // are never written, as in the reference.
 #include <cmath>
+#include <utility>
 #include <vector>
 
 #include "common.hpp"
@@ -427,6 +428,300 @@ __global__ void __launch_bounds__(kLsBlock) k_correct_velocity(Grid g, const dou
   }
 }
 
+// ------------------------------------------------- momentum predictor
+// find_source_sec (projection_method.cpp:314-382) and
+// find_intermediate_velocity (:384-402) on padded fields, fused per
+// direction.  Ghost extrapolations (identity_with_extrapolation[_face_a],
+// flux.cpp:98-375) are index clamps: copying the boundary value outward
+// axis by axis over every index of the other axes leaves flux(i,j,k) =
+// data(clamp(i), clamp(j), clamp(k)), the face variant clamping its face
+// axis to [off-1, off+n-1] (the wall face is kept).  Derivative arrays the
+// reference writes only on the interior of their axis are zero elsewhere
+// (zero-initialised and never written there), so those terms are 0.  Every
+// product is rounded (no FMA), as in the reference.
+
+struct FlowParams {
+  double dt, re, fr, we, w, ratio_rho, ratio_mu;
+  int positive_ref;
+};
+
+__device__ __forceinline__ void padded_ijk(const Grid &g, long c, int q[3]) {
+  q[0] = (int)(c / g.s[0]);
+  q[1] = (int)((c / g.s[1]) % g.N[1]);
+  q[2] = (int)(c % g.N[2]);
+}
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// flux index of padded position r: clamp every real axis to its interior;
+// `fa` (or -1) is the face axis whose low clamp is the wall face off-1
+template <int DIM>
+__device__ __forceinline__ long ext_index(const Grid &g, const int r[3], int fa) {
+  long c = 0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    int v = r[a];
+    if (a < DIM) v = clampi(v, g.off[a] - (a == fa ? 1 : 0), g.off[a] + g.n[a] - 1);
+    c += (long)v * g.s[a];
+  }
+  return c;
+}
+
+// level-set derivatives on the whole padded range (second_order_solver.cpp:
+// 13-88 and mixed_xy/yz/zx :173-230), one-sided at the array ends
+template <int DIM>
+__device__ __forceinline__ double ls_at(const Grid &g, const double *__restrict__ phi, const int q[3], int a, int d) {
+  int r[3] = {q[0], q[1], q[2]};
+  r[a] += d;
+  return phi[ext_index<DIM>(g, r, -1)];
+}
+
+template <int DIM>
+__device__ double ls_d1(const Grid &g, const double *__restrict__ phi, const int q[3], int a) {
+#pragma clang fp contract(off)
+  const double h = g.h[a];
+  if (q[a] == 0) return (-1.5 * ls_at<DIM>(g, phi, q, a, 0) + 2 * ls_at<DIM>(g, phi, q, a, 1) -
+                         0.5 * ls_at<DIM>(g, phi, q, a, 2)) / h;
+  if (q[a] == g.N[a] - 1) return (1.5 * ls_at<DIM>(g, phi, q, a, 0) - 2 * ls_at<DIM>(g, phi, q, a, -1) +
+                                  0.5 * ls_at<DIM>(g, phi, q, a, -2)) / h;
+  return (ls_at<DIM>(g, phi, q, a, 1) - ls_at<DIM>(g, phi, q, a, -1)) / (2 * h);
+}
+
+template <int DIM>
+__device__ double ls_d2(const Grid &g, const double *__restrict__ phi, const int q[3], int a) {
+#pragma clang fp contract(off)
+  const double h = g.h[a];
+  if (q[a] == 0) return (2 * ls_at<DIM>(g, phi, q, a, 0) - 5 * ls_at<DIM>(g, phi, q, a, 1) +
+                         4 * ls_at<DIM>(g, phi, q, a, 2) - ls_at<DIM>(g, phi, q, a, 3)) / (h * h);
+  if (q[a] == g.N[a] - 1) return (2 * ls_at<DIM>(g, phi, q, a, 0) - 5 * ls_at<DIM>(g, phi, q, a, -1) +
+                                  4 * ls_at<DIM>(g, phi, q, a, -2) - ls_at<DIM>(g, phi, q, a, -3)) / (h * h);
+  return (ls_at<DIM>(g, phi, q, a, 1) - 2 * ls_at<DIM>(g, phi, q, a, 0) + ls_at<DIM>(g, phi, q, a, -1)) / (h * h);
+}
+
+// d/d(axis b) of the first derivative along a
+template <int DIM>
+__device__ double ls_mixed(const Grid &g, const double *__restrict__ phi, const int q[3], int a, int b) {
+#pragma clang fp contract(off)
+  const double h = g.h[b];
+  auto G = [&](int d) {
+    int r[3] = {q[0], q[1], q[2]};
+    r[b] += d;
+    return ls_d1<DIM>(g, phi, r, a);
+  };
+  if (q[b] == 0) return (-1.5 * G(0) + 2 * G(1) - 0.5 * G(2)) / h;
+  if (q[b] == g.N[b] - 1) return (1.5 * G(0) - 2 * G(-1) + 0.5 * G(-2)) / h;
+  return (G(1) - G(-1)) / (2 * h);
+}
+
+// find_delta, find_density, find_viscosity (wrapper_func.cpp:20-29,
+// 387-416, lsm.cpp:8-30), the level-set gradient and, with surface tension,
+// find_curvature (wrapper_func.cpp:115-172), on every padded cell
+template <int DIM, bool ST>
+__global__ void __launch_bounds__(kLsBlock)
+    k_mom_props(Grid g, FlowParams fp, const double *__restrict__ phi, double *__restrict__ dl,
+                double *__restrict__ rho, double *__restrict__ mu, double *__restrict__ lf0,
+                double *__restrict__ lf1, double *__restrict__ lf2, double *__restrict__ curv) {
+#pragma clang fp contract(off)
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.total) return;
+  const double pi = 3.14159265358979323846;
+  const double x = phi[c], w = fp.w;
+  double d, H;
+  if (x < -w) d = 0;
+  else if (x > w) d = 0;
+  else d = 0.5 * (1 + cos(pi * x / w)) / w;
+  if (x < -w) H = 0;
+  else if (x > w) H = 1;
+  else H = 0.5 * (1 + x / w + sin(pi * x / w) / pi);
+  dl[c] = d;
+  rho[c] = fp.positive_ref ? H + (1.0 - H) * fp.ratio_rho : (1.0 - H) + H * fp.ratio_rho;
+  mu[c] = fp.positive_ref ? H + (1.0 - H) * fp.ratio_mu : (1.0 - H) + H * fp.ratio_mu;
+  int q[3];
+  padded_ijk(g, c, q);
+  const double fx = ls_d1<DIM>(g, phi, q, 0), fy = ls_d1<DIM>(g, phi, q, 1);
+  lf0[c] = fx;
+  lf1[c] = fy;
+  double fz = 0.0;
+  if constexpr (DIM == 3) {
+    fz = ls_d1<DIM>(g, phi, q, 2);
+    lf2[c] = fz;
+  }
+  if constexpr (ST) {
+    const double fxx = ls_d2<DIM>(g, phi, q, 0), fyy = ls_d2<DIM>(g, phi, q, 1);
+    const double fxy = ls_mixed<DIM>(g, phi, q, 0, 1);
+    double k;
+    if constexpr (DIM == 2) {
+      k = (fxx * fy * fy + fyy * fx * fx - 2.0 * fxy * fx * fy) / pow(fx * fx + fy * fy, 1.5);
+      if (k > 0.0) k = 1.0 / (1.0 / k + x);
+      else k = -1.0 / (-1.0 / k + x);
+    } else {
+      const double fzz = ls_d2<DIM>(g, phi, q, 2);
+      const double fyz = ls_mixed<DIM>(g, phi, q, 1, 2), fxz = ls_mixed<DIM>(g, phi, q, 2, 0);
+      k = (fx * fx * (fyy + fzz) + fy * fy * (fxx + fzz) + fz * fz * (fxx + fyy) -
+           2.0 * (fx * fy * fxy + fx * fz * fxz + fy * fz * fyz)) /
+          pow(fx * fx + fy * fy + fz * fz, 1.5);
+      if (k > 0.0) k = 1.0 / (2.0 / k + x);
+      else k = -1.0 / (-2.0 / k + x);
+    }
+    curv[c] = k;
+  }
+}
+
+// all_to_face_a (wrapper_func.cpp:177-262) over the padded range 1..N-2
+// (every k in 2-D)
+template <int DIM>
+__global__ void __launch_bounds__(kLsBlock)
+    k_to_face(Grid g, int A, const double *__restrict__ v0, const double *__restrict__ v1,
+              const double *__restrict__ v2, double *__restrict__ n0, double *__restrict__ n1,
+              double *__restrict__ n2) {
+#pragma clang fp contract(off)
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.total) return;
+  int q[3];
+  padded_ijk(g, c, q);
+  if (q[0] < 1 || q[0] > g.N[0] - 2 || q[1] < 1 || q[1] > g.N[1] - 2) return;
+  if (DIM == 3 && (q[2] < 1 || q[2] > g.N[2] - 2)) return;
+  const double *vel[3] = {v0, v1, v2};
+  double *nv[3] = {n0, n1, n2};
+  const long sa = g.s[A];
+  nv[A][c] = vel[A][c];
+#pragma unroll
+  for (int b = 0; b < DIM; ++b) {
+    if (b == A) continue;
+    const long sb = g.s[b];
+    const double *f = vel[b];
+    nv[b][c] = 0.25 * (f[c] + f[c + sa] + f[c - sb] + f[c + sa - sb]);
+  }
+}
+
+// upwind first and central second derivative along axis a of the
+// face-extrapolated field f (second_order_solver.cpp:90-148)
+template <int DIM>
+__device__ __forceinline__ void upwind(const Grid &g, const double *__restrict__ f, int fa, const int q[3], int a,
+                                       double vel, double &d1, double &d2) {
+#pragma clang fp contract(off)
+  auto F = [&](int d) {
+    int r[3] = {q[0], q[1], q[2]};
+    r[a] += d;
+    return f[ext_index<DIM>(g, r, fa)];
+  };
+  const double h = g.h[a];
+  const double f0 = F(0);
+  if (vel > 0.0) d1 = (1.5 * f0 - 2 * F(-1) + 0.5 * F(-2)) / h;
+  else d1 = (-1.5 * f0 + 2 * F(1) - 0.5 * F(2)) / h;
+  d2 = (F(1) - 2 * f0 + F(-1)) / (h * h);
+}
+
+// the new source of velocity component A on every padded cell:
+// convection_sec (source.cpp:42-73), add_stress_A (projection_method.cpp:
+// 31-181, interior), add_body_force (:183-203), add_surface_force (:205-240)
+template <int DIM, int A>
+__global__ void __launch_bounds__(kLsBlock)
+    k_mom_src(Grid g, FlowParams fp, const double *__restrict__ va, const double *__restrict__ n0,
+              const double *__restrict__ n1, const double *__restrict__ n2, const double *__restrict__ dl,
+              const double *__restrict__ rho, const double *__restrict__ mu, const double *__restrict__ lf0,
+              const double *__restrict__ lf1, const double *__restrict__ lf2, const double *__restrict__ curv,
+              double *__restrict__ src) {
+#pragma clang fp contract(off)
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.total) return;
+  int q[3];
+  padded_ijk(g, c, q);
+  const double *nv[3] = {n0, n1, n2};
+  bool in[3], all = true;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    in[a] = a >= DIM || (q[a] >= g.off[a] && q[a] < g.off[a] + g.n[a]);
+    all = all && in[a];
+  }
+  double d1[3] = {0, 0, 0}, d2[3] = {0, 0, 0};
+#pragma unroll
+  for (int a = 0; a < DIM; ++a)
+    if (in[a]) upwind<DIM>(g, va, A, q, a, nv[a][c], d1[a], d2[a]);
+  double s = -nv[0][c] * d1[0];
+  s -= nv[1][c] * d1[1];
+  if constexpr (DIM > 2) s -= nv[2][c] * d1[2];
+  const long sa = g.s[A];
+  if (all) {
+    // derivatives along A of the other transferred components, upwinded by
+    // the transferred A component
+    double cr[3] = {0, 0, 0}, unused;
+#pragma unroll
+    for (int b = 0; b < DIM; ++b)
+      if (b != A) upwind<DIM>(g, nv[b], A, q, A, nv[A][c], cr[b], unused);
+    const double delta = 0.5 * (dl[c] + dl[c + sa]);
+    const double m = 0.5 * (mu[c] + mu[c + sa]);
+    const double r = 0.5 * (rho[c] + rho[c + sa]);
+    const double ph0 = 0.5 * (lf0[c] + lf0[c + sa]), ph1 = 0.5 * (lf1[c] + lf1[c + sa]);
+    const double ph2 = DIM > 2 ? 0.5 * (lf2[c] + lf2[c + sa]) : 0.0;
+    double p1, p2;
+    if constexpr (A == 0) {
+      p1 = d2[0] + d2[1];
+      p2 = 2.0 * ph0 * d1[0] + ph1 * (d1[1] + cr[1]);
+      if constexpr (DIM > 2) {
+        p1 += d2[2];
+        p2 += ph2 * (d1[2] + cr[2]);
+      }
+    } else if constexpr (A == 1) {
+      p1 = d2[0] + d2[1];
+      p2 = 2.0 * ph1 * d1[1] + ph0 * (d1[0] + cr[0]);
+      if constexpr (DIM > 2) {
+        p1 += d2[2];
+        p2 += ph2 * (d1[2] + cr[2]);
+      }
+    } else {  // add_stress_z carries no w_zz term (projection_method.cpp:171)
+      p1 = d2[0] + d2[1];
+      p2 = 2.0 * ph2 * d1[2] + ph0 * (d1[0] + cr[0]) + ph1 * (d1[1] + cr[1]);
+    }
+    const double ref = fp.positive_ref ? 1.0 : -1.0;
+    p1 *= m;
+    p2 *= ref * delta * (1.0 - fp.ratio_mu);
+    s += (p1 + p2) / fp.re / r;
+  }
+  if (fp.fr > 0.0 && A == (DIM == 2 ? 1 : 2)) s += -1.0 / fp.fr;
+  if (fp.we > 0.0 && q[0] < g.N[0] - 1 && q[1] < g.N[1] - 1 && (A != 2 || q[2] < g.N[2] - 1)) {
+    const double cv = 0.5 * (curv[c] + curv[c + sa]);
+    const double delta = 0.5 * (dl[c] + dl[c + sa]);
+    const double *lf = A == 0 ? lf0 : (A == 1 ? lf1 : lf2);
+    const double pa = 0.5 * (lf[c] + lf[c + sa]);
+    const double r = 0.5 * (rho[c] + rho[c + sa]);
+    s -= cv * delta * pa / (fp.we * r);
+  }
+  src[c] = s;
+}
+
+// find_intermediate_velocity: AB2 on every padded cell of u, v, w
+__global__ void __launch_bounds__(kLsBlock)
+    k_ab2(long n, double dt, double *__restrict__ u, double *__restrict__ v, double *__restrict__ w,
+          const double *__restrict__ su, const double *__restrict__ sv, const double *__restrict__ sw,
+          const double *__restrict__ ou, const double *__restrict__ ov, const double *__restrict__ ow) {
+#pragma clang fp contract(off)
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  u[c] += dt * (1.5 * su[c] - 0.5 * ou[c]);
+  v[c] += dt * (1.5 * sv[c] - 0.5 * ov[c]);
+  w[c] += dt * (1.5 * sw[c] - 0.5 * ow[c]);
+}
+
+// node_from_face (wrapper_func.cpp:264-293): i, j in 1..N-2; k every
+// index in 2-D, 1..Nz-1 in 3-D
+template <int DIM>
+__global__ void __launch_bounds__(kLsBlock)
+    k_node_from_face(Grid g, const double *__restrict__ u, const double *__restrict__ v,
+                     const double *__restrict__ w, double *__restrict__ nu, double *__restrict__ nv,
+                     double *__restrict__ nw) {
+#pragma clang fp contract(off)
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.total) return;
+  int q[3];
+  padded_ijk(g, c, q);
+  if (q[0] < 1 || q[0] > g.N[0] - 2 || q[1] < 1 || q[1] > g.N[1] - 2) return;
+  if (DIM == 3 && q[2] < 1) return;
+  nu[c] = 0.5 * (u[c] + u[c - g.s[0]]);
+  nv[c] = 0.5 * (v[c] + v[c - g.s[1]]);
+  if constexpr (DIM == 3) nw[c] = 0.5 * (w[c] + w[c - g.s[2]]);
+}
+
 }  // namespace scihpc
 
 using namespace scihpc;
@@ -699,6 +994,83 @@ struct scihpc_proj {
     SCIHPC_REQUIRE(rc == SCIHPC_OK, "bad multigrid handle");
     SCIHPC_REQUIRE(a == g.n[0] && b == g.n[1] && c == (g.ndim > 2 ? g.n[2] : 1), "multigrid grid differs");
   }
+  // momentum predictor state and scratch (allocated on first use): phi, the
+  // transferred velocities nvel, sources (current / other, ping-pong per
+  // component), delta, viscosity, the level-set gradient and curvature
+  double *phi = nullptr, *nvel[3] = {nullptr, nullptr, nullptr};
+  double *src[3] = {nullptr, nullptr, nullptr}, *src_alt[3] = {nullptr, nullptr, nullptr};
+  double *dl = nullptr, *mu = nullptr, *lf[3] = {nullptr, nullptr, nullptr}, *curv = nullptr;
+
+  void ensure_momentum() {
+    if (phi) return;
+    const long T = g.total;
+    phi = alloc(T);
+    dl = alloc(T);
+    mu = alloc(T);
+    curv = alloc(T);
+    for (int a = 0; a < 3; ++a) {
+      nvel[a] = alloc(T);
+      src[a] = alloc(T);
+      src_alt[a] = alloc(T);
+      lf[a] = alloc(T);
+    }
+  }
+
+  // find_source_sec + find_intermediate_velocity on the device-resident
+  // phi, u, v, w, nvel and sources; rho <- find_density
+  void momentum(const FlowParams &fp) {
+    const unsigned nb = ceil_div(g.total, kLsBlock);
+    const bool st = fp.we > 0.0;
+#define SCIHPC_PROPS(D, S) \
+  k_mom_props<D, S><<<nb, kLsBlock, 0, stream>>>(g, fp, phi, dl, rho, mu, lf[0], lf[1], lf[2], curv)
+    if (g.ndim == 2) { if (st) SCIHPC_PROPS(2, true); else SCIHPC_PROPS(2, false); }
+    else { if (st) SCIHPC_PROPS(3, true); else SCIHPC_PROPS(3, false); }
+#undef SCIHPC_PROPS
+    SCIHPC_CHECK_LAUNCH();
+    double *vel[3] = {u, v, w};
+    for (int a = 0; a < g.ndim; ++a) {
+      if (g.ndim == 2)
+        k_to_face<2><<<nb, kLsBlock, 0, stream>>>(g, a, u, v, w, nvel[0], nvel[1], nvel[2]);
+      else
+        k_to_face<3><<<nb, kLsBlock, 0, stream>>>(g, a, u, v, w, nvel[0], nvel[1], nvel[2]);
+      SCIHPC_CHECK_LAUNCH();
+      // nvel->apply_vel_a_bc (wrapper.cpp:40-59)
+      for (int b = 0; b < 3; ++b) bc(nvel[b], vbc[b], a);
+#define SCIHPC_SRC(D, A)                                                                                 \
+  k_mom_src<D, A><<<nb, kLsBlock, 0, stream>>>(g, fp, vel[A], nvel[0], nvel[1], nvel[2], dl, rho, mu, lf[0], \
+                                               lf[1], lf[2], curv, src_alt[A])
+      if (g.ndim == 2) {
+        if (a == 0) SCIHPC_SRC(2, 0); else SCIHPC_SRC(2, 1);
+      } else {
+        if (a == 0) SCIHPC_SRC(3, 0); else if (a == 1) SCIHPC_SRC(3, 1); else SCIHPC_SRC(3, 2);
+      }
+#undef SCIHPC_SRC
+      SCIHPC_CHECK_LAUNCH();
+    }
+    // components beyond ndim keep their source (2-D: w_src is never written)
+    const double *sn[3], *so[3];
+    for (int a = 0; a < 3; ++a) {
+      so[a] = src[a];
+      sn[a] = a < g.ndim ? src_alt[a] : src[a];
+    }
+    k_ab2<<<nb, kLsBlock, 0, stream>>>(g.total, fp.dt, u, v, w, sn[0], sn[1], sn[2], so[0], so[1], so[2]);
+    SCIHPC_CHECK_LAUNCH();
+    for (int a = 0; a < g.ndim; ++a) std::swap(src[a], src_alt[a]);
+    bc(u, vbc[0], 0);
+    bc(v, vbc[1], 1);
+    bc(w, vbc[2], 2);
+  }
+
+  // node_from_face + nvel->apply_nvel_bc (wrapper_func.cpp:264-293,
+  // wrapper.cpp:28-33)
+  void node_velocity() {
+    const unsigned nb = ceil_div(g.total, kLsBlock);
+    if (g.ndim == 2) k_node_from_face<2><<<nb, kLsBlock, 0, stream>>>(g, u, v, w, nvel[0], nvel[1], nvel[2]);
+    else k_node_from_face<3><<<nb, kLsBlock, 0, stream>>>(g, u, v, w, nvel[0], nvel[1], nvel[2]);
+    SCIHPC_CHECK_LAUNCH();
+    for (int a = 0; a < 3; ++a) bc(nvel[a], vbc[a], -1);
+  }
+
   void finish_pressure_and_velocity(scihpc_mg *mg, const double *ph, double dt, double rho12) {
     SCIHPC_REQUIRE(scihpc_mg_get_sol_device(mg, sol) == SCIHPC_OK, scihpc_last_error());
     k_mg_to_padded<<<ceil_div(g.interior, kLsBlock), kLsBlock, 0, stream>>>(g, sol, p);
@@ -764,8 +1136,12 @@ extern "C" int scihpc_proj_create(int ndim, int nx, int ny, int nz, double dx, d
 extern "C" int scihpc_proj_destroy(scihpc_proj *h) {
   if (!h) return SCIHPC_OK;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
-  for (double *q : {h->rho, h->pold, h->pold2, h->phat, h->p, h->u, h->v, h->w, h->mg_rhs, h->sol, h->part})
+  for (double *q : {h->rho, h->pold, h->pold2, h->phat, h->p, h->u, h->v, h->w, h->mg_rhs, h->sol, h->part,
+                    h->phi, h->dl, h->mu, h->curv})
     if (q) (void)hipFree(q);
+  for (int a = 0; a < 3; ++a)
+    for (double *q : {h->nvel[a], h->src[a], h->src_alt[a], h->lf[a]})
+      if (q) (void)hipFree(q);
   if (h->hpart) (void)hipHostFree(h->hpart);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
@@ -859,4 +1235,101 @@ extern "C" int scihpc_proj_step(scihpc_proj *h, scihpc_mg *mg, double dt, const 
     if (stats) *stats = st;
   });
   return g_rc != SCIHPC_OK ? g_rc : rc;
+}
+
+static FlowParams flow_params(const scihpc_flow_params *p) {
+  SCIHPC_REQUIRE(p, "null parameters");
+  SCIHPC_REQUIRE(p->dt > 0 && p->reynolds > 0 && p->ls_width > 0 && p->density_ratio > 0 &&
+                     p->viscosity_ratio > 0,
+                 "dt, reynolds, ls_width and the ratios must be positive");
+  return FlowParams{p->dt, p->reynolds, p->froude, p->weber, p->ls_width, p->density_ratio, p->viscosity_ratio,
+                    p->positive_ref != 0};
+}
+
+extern "C" int scihpc_proj_momentum_step(scihpc_proj *h, const scihpc_flow_params *prm, const double *phi,
+                                         double *u, double *v, double *w, double *nu, double *nv, double *nw,
+                                         double *rho) {
+  return guarded([&] {
+    SCIHPC_REQUIRE(h && phi && u && v && w && nu && nv && nw, "null argument");
+    const FlowParams fp = flow_params(prm);
+    h->ensure_momentum();
+    h->h2d(h->phi, phi);
+    h->h2d(h->u, u);
+    h->h2d(h->v, v);
+    h->h2d(h->w, w);
+    h->h2d(h->nvel[0], nu);
+    h->h2d(h->nvel[1], nv);
+    h->h2d(h->nvel[2], nw);
+    h->momentum(fp);
+    h->d2h(u, h->u);
+    h->d2h(v, h->v);
+    h->d2h(w, h->w);
+    h->d2h(nu, h->nvel[0]);
+    h->d2h(nv, h->nvel[1]);
+    h->d2h(nw, h->nvel[2]);
+    if (rho) h->d2h(rho, h->rho);
+    SCIHPC_HIP(hipStreamSynchronize(h->stream));
+  });
+}
+
+extern "C" int scihpc_proj_set_sources(scihpc_proj *h, const double *su, const double *sv, const double *sw) {
+  return guarded([&] {
+    SCIHPC_REQUIRE(h && su && sv && sw, "null argument");
+    h->ensure_momentum();
+    h->h2d(h->src[0], su);
+    h->h2d(h->src[1], sv);
+    h->h2d(h->src[2], sw);
+    SCIHPC_HIP(hipStreamSynchronize(h->stream));
+  });
+}
+
+extern "C" int scihpc_proj_get_sources(scihpc_proj *h, double *su, double *sv, double *sw) {
+  return guarded([&] {
+    SCIHPC_REQUIRE(h && su && sv && sw, "null argument");
+    h->ensure_momentum();
+    h->d2h(su, h->src[0]);
+    h->d2h(sv, h->src[1]);
+    h->d2h(sw, h->src[2]);
+    SCIHPC_HIP(hipStreamSynchronize(h->stream));
+  });
+}
+
+extern "C" int scihpc_proj_node_velocity(scihpc_proj *h, const double *u, const double *v, const double *w,
+                                         double *nu, double *nv, double *nw) {
+  return guarded([&] {
+    SCIHPC_REQUIRE(h && u && v && w && nu && nv && nw, "null argument");
+    h->ensure_momentum();
+    h->h2d(h->u, u);
+    h->h2d(h->v, v);
+    h->h2d(h->w, w);
+    h->h2d(h->nvel[0], nu);
+    h->h2d(h->nvel[1], nv);
+    h->h2d(h->nvel[2], nw);
+    h->node_velocity();
+    h->d2h(nu, h->nvel[0]);
+    h->d2h(nv, h->nvel[1]);
+    h->d2h(nw, h->nvel[2]);
+    SCIHPC_HIP(hipStreamSynchronize(h->stream));
+  });
+}
+
+extern "C" int scihpc_proj_time_momentum(scihpc_proj *h, const scihpc_flow_params *prm, int iters, double *ms) {
+  return guarded([&] {
+    SCIHPC_REQUIRE(h && ms && iters > 0, "bad argument");
+    const FlowParams fp = flow_params(prm);
+    h->ensure_momentum();
+    hipEvent_t a, b;
+    SCIHPC_HIP(hipEventCreate(&a));
+    SCIHPC_HIP(hipEventCreate(&b));
+    h->momentum(fp);
+    SCIHPC_HIP(hipEventRecord(a, h->stream));
+    for (int it = 0; it < iters; ++it) h->momentum(fp);
+    SCIHPC_HIP(hipEventRecord(b, h->stream));
+    SCIHPC_HIP(hipEventSynchronize(b));
+    float t = 0;
+    SCIHPC_HIP(hipEventElapsedTime(&t, a, b));
+    *ms = t / iters;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+  });
 }
