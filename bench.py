@@ -184,6 +184,81 @@ def bench_snapshot(torch, nx=512, ny=1024, reps=5):
     return out
 
 
+MOM_SHAPE = (1024, 512)
+
+
+def momentum_workload(nx=MOM_SHAPE[0], ny=MOM_SHAPE[1]):
+    """2-D dam break on [0,4] x [0,2] (1024 x 512 cells): phi = signed distance
+    to the water column [0,1] x [0,2], air/water ratios, gravity, no slip,
+    a seeded smooth velocity field with its wall values."""
+    import numpy as np
+
+    from scihpc_amd import padded_shape
+
+    h = 4.0 / nx
+    shp = padded_shape(2, nx, ny)
+    I, J = np.meshgrid(np.arange(shp[0]), np.arange(shp[1]), indexing="ij")
+    x, y = (I - 3 + 0.5) * h, (J - 3 + 0.5) * h
+    phi = (1.0 - x).ravel()  # column of full height: distance to x = 1
+    rng = np.random.default_rng(41)
+    a = rng.standard_normal(4)
+    u = (0.1 * (a[0] * np.sin(np.pi * x / 4) * np.cos(np.pi * y / 2) + a[1])).ravel()
+    v = (0.1 * (a[2] * np.cos(np.pi * x / 4) * np.sin(np.pi * y / 2) + a[3])).ravel()
+    L = 0.0571
+    Uc = np.sqrt(9.81 * L)
+    prm = dict(dt=0.01 * h, reynolds=1000.0 * Uc * L / 1.137e-3, froude=1.0, weber=-1.0, ls_width=1.5 * h,
+               density_ratio=1.226e-3, viscosity_ratio=1.78e-5 / 1.137e-3)
+    return dict(shape=(nx, ny), h=h, phi=phi, u=u, v=v, w=np.zeros_like(u), prm=prm)
+
+
+# algorithmic bytes per padded cell of one 2-D predictor step without surface
+# tension: level-set properties 6 arrays, per direction the face transfer 4
+# and the source 9, AB2 12 (8-byte values, each array once per pass)
+MOM_BYTES_PER_CELL_2D = 8 * (6 + 2 * (4 + 9) + 12)
+
+
+def bench_momentum(iters=20):
+    """find_source_sec + find_intermediate_velocity at the dam-break shape:
+    the device time of one step on resident fields against HBM."""
+    import numpy as np
+
+    from scihpc_amd import Projection
+
+    wl = momentum_workload()
+    nx, ny = wl["shape"]
+    pj = Projection(2, nx, ny, 1, wl["h"], wl["h"], 1.0, ["no_slip"] * 6)
+    prm = Projection.flow_params(**wl["prm"])
+    z = np.zeros_like(wl["u"])
+    pj.momentum_step(prm, wl["phi"], wl["u"], wl["v"], wl["w"], z, z, z)
+    ms = pj.time_momentum(prm, iters=iters)
+    cells = int(np.prod(pj.shape))
+    byts = MOM_BYTES_PER_CELL_2D * cells
+    out = {"workload": f"dam break {nx}x{ny}, air/water, gravity, no slip", "ms_per_step": round(ms, 4),
+           "padded_cells": cells, "bytes_per_step": byts, "GBps": round(byts / (ms * 1e-3) / 1e9, 1),
+           "hbm_frac": round(byts / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    pj.close()
+    return out
+
+
+def cpu_momentum_baseline():
+    """The oracle's momentum step (one core) at the same shape."""
+    import numpy as np
+
+    from oracle.pyoracle import Oracle
+
+    wl = momentum_workload()
+    nx, ny = wl["shape"]
+    p = wl["prm"]
+    prm8 = np.array([p["dt"], p["reynolds"], p["froude"], p["weber"], p["ls_width"], p["density_ratio"],
+                     p["viscosity_ratio"], 1.0])
+    arrs = [wl["u"].copy(), wl["v"].copy(), wl["w"].copy()] + [np.zeros_like(wl["u"]) for _ in range(7)]
+    o = Oracle()
+    t0 = time.perf_counter()
+    o.lib.orc_momentum_step(2, nx, ny, 1, wl["h"], wl["h"], 1.0, prm8, np.full(6, 3, np.int32), wl["phi"], *arrs)
+    return {"ms_per_step": round((time.perf_counter() - t0) * 1e3, 2), "cores": 1, "kind": "port",
+            "sample": f"one predictor step at {nx}x{ny}"}
+
+
 def cpu_snapshot_baseline(snap):
     """The oracle's C writer (one core) on the same fields: ms per file and
     whether its bytes equal the GPU writer's file."""
@@ -214,6 +289,7 @@ def main():
     ap.add_argument("--only-3d", action="store_true", help="run only the 256^3 3-D solves (profiling)")
     ap.add_argument("--no-unet", action="store_true", help="skip the U-Net training-step timing")
     ap.add_argument("--no-snapshot", action="store_true", help="skip the VTK snapshot timing")
+    ap.add_argument("--no-momentum", action="store_true", help="skip the momentum predictor timing")
     ap.add_argument("--kernel-only", action="store_true",
                     help="only launch the dominant kernel (and a 256 MiB copy for calibration): "
                          "the command profiled with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE")
@@ -347,6 +423,12 @@ def main():
                 "mfma_frac_train": round(fl_tr / (ms_tr * 1e-3) / 1e12 / MFMA_BF16_PEAK, 4)}
         nt.close()
 
+    momentum = None
+    if not args.no_momentum:
+        log("bench: momentum predictor at the 1024x512 dam-break shape")
+        momentum = bench_momentum()
+        log(f"bench: momentum {momentum['ms_per_step']} ms/step, {momentum['GBps']} GB/s")
+
     snapshot = None
     if not args.no_snapshot:
         log("bench: VTK snapshot of the 512x1024 grid")
@@ -389,6 +471,7 @@ def main():
                      "kernel": "k_pre2d<float,true> level 0 (2 RB-GS sweeps + residual + restriction)",
                      "bytes_per_launch": kbytes, "us_per_launch": round(kms * 1e3, 3)},
         "unet": unet,
+        "momentum": momentum,
         "snapshot": snapshot,
     }
     if not args.no_3d:
@@ -398,6 +481,8 @@ def main():
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_seconds)
         if snapshot is not None:
             out["cpu_baseline"]["snapshot_writer"] = cpu_snapshot_baseline(snapshot)
+        if momentum is not None:
+            out["cpu_baseline"]["momentum"] = cpu_momentum_baseline()
     if snapshot is not None:
         snapshot.pop("_fields")
         snapshot.pop("_file")

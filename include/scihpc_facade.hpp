@@ -371,6 +371,12 @@ struct problem_parameters {
   DataType dt = 1.0, dt_old = 1.0, dt_old2 = 1.0;
   DataType density_ratio = 1.0;
   DataType ppe_tol = 1e-4;
+  // momentum predictor (global.h:25-54; <= 0 disables gravity / surface tension)
+  DataType Reynolds_number = 1.0, Froude_number = -1.0, Weber_number = -1.0;
+  DataType ls_width = 0.0, viscosity_ratio = 1.0;
+  bool positive_ref = true;
+  int iter = 0;        // time step counter (the first 5 steps use projection())
+  int ppe_initer = 1;  // pressure iterations per step
 };
 
 // projection_method (projection_method.h:14-51): the projection step of
@@ -403,6 +409,44 @@ class projection_method {
     check(scihpc_proj_fpc_step(h_, mg.handle(), &fp, base(density->data), base(pressure->old), base(pressure->old2),
                                base(vel->x->data), base(vel->y->data), w, base(pressure->data),
                                base(pressure->flux), &cycles));
+    return cycles;
+  }
+  // find_source_sec + find_intermediate_velocity (projection_method.cpp:
+  // 314-402): vel predicted in place (AB2 with the sources kept on the
+  // device between calls, as the reference's u_src), nvel holds the last
+  // face transfer, density (optional) receives find_density
+  void find_source_sec(vector_data *vel, vector_data *nvel, const scalar_data *lsf, const problem_parameters &prm,
+                       scalar_data *density = nullptr) {
+    scihpc_flow_params fp{prm.dt, prm.Reynolds_number, prm.Froude_number, prm.Weber_number, prm.ls_width,
+                          prm.density_ratio, prm.viscosity_ratio, prm.positive_ref ? 1 : 0};
+    std::vector<double> z1, z2;
+    double *w = vel_w(vel, lsf, z1), *nw = vel_w(nvel, lsf, z2);
+    check(scihpc_proj_momentum_step(h_, &fp, base(lsf->data), base(vel->x->data), base(vel->y->data), w,
+                                    base(nvel->x->data), base(nvel->y->data), nw,
+                                    density ? base(density->data) : nullptr));
+  }
+  // node_from_face + nvel->apply_nvel_bc (wrapper_func.cpp:264-293)
+  void node_from_face(vector_data *vel, vector_data *nvel, const scalar_data *like) {
+    std::vector<double> z1, z2;
+    double *w = vel_w(vel, like, z1), *nw = vel_w(nvel, like, z2);
+    check(scihpc_proj_node_velocity(h_, base(vel->x->data), base(vel->y->data), w, base(nvel->x->data),
+                                    base(nvel->y->data), nw));
+  }
+  // ab_solve_sec (projection_method.cpp:779-800): pressure history,
+  // momentum predictor, projection() for the first 5 steps (here the A(rho)
+  // solve) then the FPC step, node velocities.  Returns the last pressure
+  // step's multigrid cycles.
+  int ab_solve_sec(multigrid &mg_const, multigrid &mg_rho, vector_data *vel, vector_data *nvel,
+                   scalar_data *pressure, const scalar_data *lsf, scalar_data *density,
+                   const problem_parameters &prm) {
+    pressure->store();
+    find_source_sec(vel, nvel, lsf, prm, density);
+    int cycles = 0;
+    for (int cnt = 0; cnt < prm.ppe_initer; ++cnt) {
+      if (prm.iter < 5) cycles = projection(mg_rho, pressure, density, vel, prm.dt).cycles;
+      else cycles = fast_pressure_correction(mg_const, pressure, density, vel, prm);
+    }
+    node_from_face(vel, nvel, lsf);
     return cycles;
   }
   // projection() (projection_method.cpp:544-657) with the SOR replaced by the

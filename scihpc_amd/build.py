@@ -18,7 +18,7 @@ CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libscihpc_hip.so")
 
 SOURCES = ["mg.hip", "advect.hip", "unet.hip", "snapshot.hip"]
-DRIVERS = ["facade_demo"]
+DRIVERS = ["facade_demo", "dambreak_demo"]
 
 HIPCC_FLAGS = [
     "-O3",
