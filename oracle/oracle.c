@@ -1879,3 +1879,456 @@ void orc_node_velocity(int ndim, int nx, int ny, int nz, double dx, double dy, d
   double *f[3] = {nu, nv, nw};
   for (int a = 0; a < 3; a++) orc_vel_bc(ndim, n, hh, vel_types, a, -1, f[a]);
 }
+
+/* ================================================================== */
+/* compact schemes: twin (2x2 block) tridiagonal decomposition and back */
+/* substitution (matrix_solver.cpp:21-83), coefficients of the CCD     */
+/* (ccd_base.cpp:43-104) and UCCD (uccd_base.cpp:16-117) line systems, */
+/* and the derivatives along padded lines (ccd_solver.cpp:14-142,      */
+/* uccd_solver.cpp:14-206).  A coefficient set is 12 arrays of n:      */
+/* a0 a1 a2 b0 b1 b2 aa0 aa1 aa2 bb0 bb1 bb2.                          */
+
+static void orc_twin_dec(double *c, int n) {
+  double *a[3] = {c, c + n, c + 2 * n}, *b[3] = {c + 3 * n, c + 4 * n, c + 5 * n};
+  double *aa[3] = {c + 6 * n, c + 7 * n, c + 8 * n}, *bb[3] = {c + 9 * n, c + 10 * n, c + 11 * n};
+  a[0][0] = 0.0;
+  aa[0][0] = 0.0;
+  b[0][0] = 1.0;
+  bb[0][0] = 1.0;
+  a[2][n - 1] = 0.0;
+  aa[2][n - 1] = 0.0;
+  b[2][n - 1] = 1.0;
+  bb[2][n - 1] = 1.0;
+  for (int i = 1; i < n; i++) {
+    const double den = a[1][i - 1] * bb[1][i - 1] - aa[1][i - 1] * b[1][i - 1];
+    double sc1 = -(aa[1][i - 1] * b[0][i] - a[0][i] * bb[1][i - 1]) / den;
+    double sc2 = (b[0][i] * a[1][i - 1] - b[1][i - 1] * a[0][i]) / den;
+    a[0][i] = sc1;
+    a[1][i] = a[1][i] - (sc1 * a[2][i - 1] + sc2 * aa[2][i - 1]);
+    b[0][i] = sc2;
+    b[1][i] = b[1][i] - (sc1 * b[2][i - 1] + sc2 * bb[2][i - 1]);
+    sc1 = -(aa[1][i - 1] * bb[0][i] - aa[0][i] * bb[1][i - 1]) / den;
+    sc2 = (bb[0][i] * a[1][i - 1] - b[1][i - 1] * aa[0][i]) / den;
+    aa[0][i] = sc1;
+    aa[1][i] = aa[1][i] - (sc1 * a[2][i - 1] + sc2 * aa[2][i - 1]);
+    bb[0][i] = sc2;
+    bb[1][i] = bb[1][i] - (sc1 * b[2][i - 1] + sc2 * bb[2][i - 1]);
+  }
+}
+
+/* s, ss: right-hand sides along the line (stride st), overwritten */
+static void orc_twin_bks(const double *c, int n, double *s, double *ss, long st) {
+  const double *a[3] = {c, c + n, c + 2 * n}, *b[3] = {c + 3 * n, c + 4 * n, c + 5 * n};
+  const double *aa[3] = {c + 6 * n, c + 7 * n, c + 8 * n}, *bb[3] = {c + 9 * n, c + 10 * n, c + 11 * n};
+#define S(i) s[(long)(i) * st]
+#define SS(i) ss[(long)(i) * st]
+  for (int i = 1; i < n; i++) {
+    S(i) = S(i) - (a[0][i] * S(i - 1) + b[0][i] * SS(i - 1));
+    SS(i) = SS(i) - (aa[0][i] * S(i - 1) + bb[0][i] * SS(i - 1));
+  }
+  double den = a[1][n - 1] * bb[1][n - 1] - aa[1][n - 1] * b[1][n - 1];
+  double sols = -(b[1][n - 1] * SS(n - 1) - bb[1][n - 1] * S(n - 1)) / den;
+  double solss = (a[1][n - 1] * SS(n - 1) - aa[1][n - 1] * S(n - 1)) / den;
+  S(n - 1) = sols;
+  SS(n - 1) = solss;
+  for (int i = n - 2; i > -1; --i) {
+    S(i) = S(i) - (a[2][i] * S(i + 1) + b[2][i] * SS(i + 1));
+    SS(i) = SS(i) - (aa[2][i] * S(i + 1) + bb[2][i] * SS(i + 1));
+    den = a[1][i] * bb[1][i] - aa[1][i] * b[1][i];
+    sols = -(b[1][i] * SS(i) - bb[1][i] * S(i)) / den;
+    solss = (a[1][i] * SS(i) - aa[1][i] * S(i)) / den;
+    S(i) = sols;
+    SS(i) = solss;
+  }
+#undef S
+#undef SS
+}
+
+/* the boundary closure shared by CCD and UCCD (ccd_base.cpp:63-101) */
+static void orc_compact_ends(double *c, int n, double h) {
+  double *a[3] = {c, c + n, c + 2 * n}, *b[3] = {c + 3 * n, c + 4 * n, c + 5 * n};
+  double *aa[3] = {c + 6 * n, c + 7 * n, c + 8 * n}, *bb[3] = {c + 9 * n, c + 10 * n, c + 11 * n};
+  const double alpha_1 = 2.0, beta_1 = -1.0, alpha_2 = 5.0, beta_2 = -6.0;
+  a[0][0] = 0.0; a[1][0] = 1.0; a[2][0] = alpha_1;
+  b[0][0] = 0.0; b[1][0] = 0.0; b[2][0] = beta_1 * h;
+  aa[0][0] = 0.0; aa[1][0] = 0.0; aa[2][0] = beta_2 / h;
+  bb[0][0] = 0.0; bb[1][0] = 1.0; bb[2][0] = alpha_2;
+  a[0][n - 1] = alpha_1; a[1][n - 1] = 1.0; a[2][n - 1] = 0.0;
+  b[0][n - 1] = -beta_1 * h; b[1][n - 1] = 0.0; b[2][n - 1] = 0.0;
+  aa[0][n - 1] = -beta_2 / h; aa[1][n - 1] = 0.0; aa[2][n - 1] = 0.0;
+  bb[0][n - 1] = alpha_2; bb[1][n - 1] = 1.0; bb[2][n - 1] = 0.0;
+}
+
+void orc_ccd_coefficients(int n, double h, double *c) {
+  for (int i = 0; i < n; i++) {
+    c[i] = 7.0 / 16.0; c[n + i] = 1.0; c[2 * n + i] = 7.0 / 16.0;
+    c[3 * n + i] = h / 16.0; c[4 * n + i] = 0.0; c[5 * n + i] = -h / 16.0;
+    c[6 * n + i] = -9.0 / 8.0 / h; c[7 * n + i] = 0.0; c[8 * n + i] = 9.0 / 8.0 / h;
+    c[9 * n + i] = -1.0 / 8.0; c[10 * n + i] = 1.0; c[11 * n + i] = -1.0 / 8.0;
+  }
+  orc_compact_ends(c, n, h);
+  orc_twin_dec(c, n);
+}
+
+void orc_uccd_coefficients(int n, double h, double *up, double *dn) {
+  const double a1 = 0.875, b1 = 0.1251282341599089, b2 = -0.2487176584009104, b3 = 0.0001282341599089;
+  for (int i = 0; i < n; i++) {
+    up[i] = a1; up[n + i] = 1.0; up[2 * n + i] = 0.0;
+    dn[i] = 0.0; dn[n + i] = 1.0; dn[2 * n + i] = a1;
+    up[3 * n + i] = b1 * h; up[4 * n + i] = b2 * h; up[5 * n + i] = b3 * h;
+    dn[3 * n + i] = -b3 * h; dn[4 * n + i] = -b2 * h; dn[5 * n + i] = -b1 * h;
+    up[6 * n + i] = -9.0 / 8.0 / h; up[7 * n + i] = 0.0; up[8 * n + i] = 9.0 / 8.0 / h;
+    dn[6 * n + i] = -9.0 / 8.0 / h; dn[7 * n + i] = 0.0; dn[8 * n + i] = 9.0 / 8.0 / h;
+    up[9 * n + i] = -1.0 / 8.0; up[10 * n + i] = 1.0; up[11 * n + i] = -1.0 / 8.0;
+    dn[9 * n + i] = -1.0 / 8.0; dn[10 * n + i] = 1.0; dn[11 * n + i] = -1.0 / 8.0;
+  }
+  orc_compact_ends(up, n, h);
+  orc_compact_ends(dn, n, h);
+  orc_twin_dec(up, n);
+  orc_twin_dec(dn, n);
+}
+
+/* right-hand sides s, ss (contiguous) of one padded line f (stride st,
+ * length n): the one-sided ends and the interior stencil of the scheme;
+ * kind 0 CCD, 1 UCCD upwind, 2 UCCD downwind */
+static void orc_compact_rhs(const double *f, long st, int n, double h, int kind, double *s, double *ss) {
+  const double bda1 = -3.5, bdb1 = 4.0, bdc1 = -0.5, bda2 = 9.0, bdb2 = -12.0, bdc2 = 3.0;
+  const double c3 = -0.06096119008109, c2 = 1.99692238016218, c1 = -1.93596119008109;
+#define F(i) f[(long)(i) * st]
+  s[0] = (bda1 * F(0) + bdb1 * F(1) + bdc1 * F(2)) / h;
+  ss[0] = (bda2 * F(0) + bdb2 * F(1) + bdc2 * F(2)) / h / h;
+  s[n - 1] = -(bda1 * F(n - 1) + bdb1 * F(n - 2) + bdc1 * F(n - 3)) / h;
+  ss[n - 1] = (bda2 * F(n - 1) + bdb2 * F(n - 2) + bdc2 * F(n - 3)) / h / h;
+  for (int i = 1; i < n - 1; i++) {
+    double v;
+    if (kind == 0) v = 15.0 / 16.0 * (F(i + 1) - F(i - 1)) / h;
+    else if (kind == 1) v = (c1 * F(i - 1) + c2 * F(i) + c3 * F(i + 1)) / h;
+    else v = -(c3 * F(i - 1) + c2 * F(i) + c1 * F(i + 1)) / h;
+    s[i] = v;
+    ss[i] = (3.0 * F(i - 1) - 6.0 * F(i) + 3.0 * F(i + 1)) / h / h;
+  }
+#undef F
+}
+
+/* every padded line along axis a: CCD (vel = NULL) -> d1 = s, d2 = ss;
+ * UCCD -> d1 = upwind or downwind by vel > 0, d2 = mean of the two ss */
+static void orc_compact_axis(const orc_pf *p, int a, double h, const double *f, const double *vel,
+                             double *d1, double *d2) {
+  const int n = p->N[a];
+  const long st = p->s[a];
+  double *cu = malloc(sizeof(double) * 12 * n), *cd = malloc(sizeof(double) * 12 * n);
+  double *su = malloc(sizeof(double) * 4 * n);
+  double *ssu = su + n, *sd = su + 2 * n, *ssd = su + 3 * n;
+  if (vel) orc_uccd_coefficients(n, h, cu, cd);
+  else orc_ccd_coefficients(n, h, cu);
+  const int t1 = (a + 1) % 3, t2 = (a + 2) % 3;
+  for (int u = 0; u < p->N[t1]; u++)
+    for (int v = 0; v < p->N[t2]; v++) {
+      const long base = (long)u * p->s[t1] + (long)v * p->s[t2];
+      orc_compact_rhs(f + base, st, n, h, vel ? 1 : 0, su, ssu);
+      if (vel) {
+        orc_compact_rhs(f + base, st, n, h, 2, sd, ssd);
+        /* the downwind ends equal the upwind ones (uccd_solver.cpp:47-50) */
+        sd[0] = su[0]; ssd[0] = ssu[0]; sd[n - 1] = su[n - 1]; ssd[n - 1] = ssu[n - 1];
+      }
+      orc_twin_bks(cu, n, su, ssu, 1);
+      if (vel) orc_twin_bks(cd, n, sd, ssd, 1);
+      for (int i = 0; i < n; i++) {
+        const long c = base + (long)i * st;
+        if (vel) {
+          d1[c] = vel[c] > 0 ? su[i] : sd[i];
+          d2[c] = 0.5 * (ssu[i] + ssd[i]);
+        } else {
+          d1[c] = su[i];
+          d2[c] = ssu[i];
+        }
+      }
+    }
+  free(cu);
+  free(cd);
+  free(su);
+}
+
+/* uccd_solver::find_derivatives(f, vel) and ccd_solver::find_derivatives(f)
+ * on the padded flux (vel_x/y/z = NULL for CCD) */
+void orc_compact_derivatives(int ndim, int nx, int ny, int nz, double dx, double dy, double dz, const double *flux,
+                             const double *u, const double *v, const double *w, int upwind, double *fx,
+                             double *fy, double *fz, double *fxx, double *fyy, double *fzz) {
+  const orc_pf p = pf_make(ndim, nx, ny, nz);
+  const double hh[3] = {dx, dy, dz};
+  const double *vel[3] = {u, v, w};
+  double *d1[3] = {fx, fy, fz}, *d2[3] = {fxx, fyy, fzz};
+  for (int a = 0; a < ndim; a++) orc_compact_axis(&p, a, hh[a], flux, upwind ? vel[a] : NULL, d1[a], d2[a]);
+}
+
+/* runge_kutta::tvd_rk3(phi, nvel, identity_flux, convection) with the UCCD
+ * convection term (source.cpp:8-40, uccd_solver.cpp:14-215) and the
+ * scalar BCs after each stage (runge_kutta.cpp:13-49)                  */
+void orc_tvd_rk3_uccd(int ndim, int nx, int ny, int nz, double dx, double dy, double dz, double dt,
+                      const int *types, const double *values, double *phi, const double *u, const double *v,
+                      const double *w, int nsteps) {
+  const orc_pf p = pf_make(ndim, nx, ny, nz);
+  const long T = (long)p.N[0] * p.N[1] * p.N[2];
+  double *m = calloc((size_t)9 * T, sizeof(double));
+  double *s[3] = {m, m + T, m + 2 * T}, *fx = m + 3 * T, *fy = m + 4 * T, *fz = m + 5 * T;
+  double *fxx = m + 6 * T, *fyy = m + 7 * T, *fzz = m + 8 * T;
+  for (int step = 0; step < nsteps; step++)
+    for (int stage = 0; stage < 3; stage++) {
+      orc_compact_derivatives(ndim, nx, ny, nz, dx, dy, dz, phi, u, v, w, 1, fx, fy, fz, fxx, fyy, fzz);
+      double *r = s[stage];
+      for (long c = 0; c < T; c++) r[c] = -u[c] * fx[c];
+      if (ndim > 1)
+        for (long c = 0; c < T; c++) r[c] -= v[c] * fy[c];
+      if (ndim > 2)
+        for (long c = 0; c < T; c++) r[c] -= w[c] * fz[c];
+      for (long c = 0; c < T; c++) {
+        if (stage == 0) phi[c] += dt * s[0][c];
+        else if (stage == 1) phi[c] += dt * (s[1][c] - 3.0 * s[0][c]) / 4.0;
+        else phi[c] += dt * (-s[0][c] - s[1][c] + 8.0 * s[2][c]) / 12.0;
+      }
+      orc_bc_apply(ndim, nx, ny, nz, dx, dy, dz, types, values, -1, phi);
+    }
+  free(m);
+}
+
+/* lsf_mass (wrapper_func.cpp:450-511): density-weighted Heaviside mass,
+ * 24-weight (2-D) / 77-weight (3-D) stencil on interior cells away from
+ * the walls; the 3-D branch multiplies by dv twice, as the reference does */
+double orc_lsf_mass(int ndim, int nx, int ny, int nz, double dv, double ls_width, double ratio, int positive_ref,
+                    const double *phi) {
+  const orc_pf p = pf_make(ndim, nx, ny, nz);
+  const long T = (long)p.N[0] * p.N[1] * p.N[2];
+  double *dn = malloc(sizeof(double) * 2 * T), *hv = dn + T;
+  for (long c = 0; c < T; c++) {
+    const double H = orc_heav(phi[c], ls_width);
+    hv[c] = H;
+    dn[c] = positive_ref ? H + (1.0 - H) * ratio : (1.0 - H) + H * ratio;
+  }
+  double mass = 0.0;
+  for (int I = 0; I < p.n[0]; I++)
+    for (int J = 0; J < p.n[1]; J++)
+      for (int K = 0; K < p.n[2]; K++) {
+        const long c = PIDX(p, I + p.off[0], J + p.off[1], K + p.off[2]);
+        if (ndim == 2) {
+          if (I != 0 && I != p.n[0] - 1 && J != 0 && J != p.n[1] - 1) {
+            double lm = 15.0 * dn[c] * hv[c];
+            for (int ii = -1; ii < 2; ii++)
+              for (int jj = -1; jj < 2; jj++) {
+                const long q = c + ii * p.s[0] + jj * p.s[1];
+                lm += dn[q] * hv[q];
+              }
+            mass += lm / 24.0;
+          } else {
+            mass += dn[c] * hv[c];
+          }
+        } else {
+          if (I != 0 && I != p.n[0] - 1 && J != 0 && J != p.n[1] - 1 && K != 0 && K != p.n[2] - 1) {
+            double lm = 50.0 * dn[c] * hv[c];
+            for (int ii = -1; ii < 2; ii++)
+              for (int jj = -1; jj < 2; jj++)
+                for (int kk = -1; kk < 2; kk++) {
+                  const long q = c + ii * p.s[0] + jj * p.s[1] + kk * p.s[2];
+                  lm += dn[q] * hv[q];
+                }
+            mass += lm * dv / 77.0;
+          } else {
+            mass += dn[c] * hv[c] * dv;
+          }
+        }
+      }
+  free(dn);
+  return mass * dv;
+}
+
+/* gradient |grad phi| of the CCD derivatives (find_gradient,
+ * wrapper_func.cpp:42-57) */
+static double orc_grad_mag(int ndim, double fx, double fy, double fz) {
+  double g = pow(fx, 2);
+  if (ndim > 1) g += pow(fy, 2);
+  if (ndim > 2) g += pow(fz, 2);
+  return sqrt(g);
+}
+
+/* `count` runge_kutta::euler steps with the mpls right-hand side
+ * (source.cpp:109-146): identity_with_extrapolation + CCD derivatives of
+ * phi, lsf_mass, Heaviside, delta and gradient, the global factor
+ * eta = (mass0 - mass) / (sum g delta |grad| dv dt), s = eta delta |grad|;
+ * phi += dt s and the scalar BCs.  delta_out (nullable): the last delta
+ * (dummy->delta after the call).                                       */
+void orc_mpls_euler(int ndim, int nx, int ny, int nz, double dx, double dy, double dz, double dt, double ls_width,
+                    double ratio, int positive_ref, double mass0, const int *types, const double *values,
+                    double *phi, int count, double *delta_out) {
+  const orc_pf p = pf_make(ndim, nx, ny, nz);
+  const long T = (long)p.N[0] * p.N[1] * p.N[2];
+  const double dv = ndim == 2 ? dx * dy : dx * dy * dz;
+  double *m = calloc((size_t)9 * T, sizeof(double));
+  double *fl = m, *d1[3] = {m + T, m + 2 * T, m + 3 * T}, *d2[3] = {m + 4 * T, m + 5 * T, m + 6 * T};
+  double *dl = m + 7 * T, *gr = m + 8 * T;
+  for (int it = 0; it < count; it++) {
+    orc_extrap(&p, phi, fl, -1);
+    orc_compact_derivatives(ndim, nx, ny, nz, dx, dy, dz, fl, NULL, NULL, NULL, 0, d1[0], d1[1], d1[2], d2[0],
+                            d2[1], d2[2]);
+    const double mass = orc_lsf_mass(ndim, nx, ny, nz, dv, ls_width, ratio, positive_ref, phi);
+    for (long c = 0; c < T; c++) {
+      dl[c] = orc_delta(phi[c], ls_width);
+      gr[c] = orc_grad_mag(ndim, d1[0][c], d1[1][c], d1[2][c]);
+    }
+    double eta = 0.0;
+    for (int I = 0; I < p.n[0]; I++)
+      for (int J = 0; J < p.n[1]; J++)
+        for (int K = 0; K < p.n[2]; K++) {
+          const long c = PIDX(p, I + p.off[0], J + p.off[1], K + p.off[2]);
+          const double H = orc_heav(phi[c], ls_width), d = dl[c];
+          double g = d * (2.0 * (1.0 - ratio) * H + ratio);
+          if (!positive_ref) g = d * (2.0 * (ratio - 1.0) * H + 1.0);
+          eta += g * d * gr[c];
+        }
+    eta = (mass0 - mass) / (eta * dv * dt);
+    for (long c = 0; c < T; c++) phi[c] += dt * (eta * dl[c] * gr[c]);
+    orc_bc_apply(ndim, nx, ny, nz, dx, dy, dz, types, values, -1, phi);
+  }
+  if (delta_out) memcpy(delta_out, dl, sizeof(double) * T);
+  free(m);
+}
+
+/* weno5_for_godunov (godunov_gradient.cpp:184-201) */
+static double orc_weno_godunov(double a, double b, double c, double d) {
+  const double eps = orc_eps;
+  const double is0 = 13.0 * pow(a - b, 2.0) + 3.0 * pow(a - 3.0 * b, 2.0);
+  const double is1 = 13.0 * pow(b - c, 2.0) + 3.0 * pow(b + c, 2.0);
+  const double is2 = 13.0 * pow(c - d, 2.0) + 3.0 * pow(3.0 * c - d, 2.0);
+  const double alp0 = 1.0 / pow(eps + is0, 2.0);
+  const double alp1 = 6.0 / pow(eps + is1, 2.0);
+  const double alp2 = 3.0 / pow(eps + is2, 2.0);
+  const double w0 = alp0 / (alp0 + alp1 + alp2);
+  const double w2 = alp2 / (alp0 + alp1 + alp2);
+  return w0 / 3.0 * (a - 2.0 * b + c) + (w2 - 0.5) / 6.0 * (b - 2.0 * c + d);
+}
+
+/* godunov_gradient (godunov_gradient.cpp:7-152): Godunov upwind |grad phi|
+ * on the interior from WENO5 one-sided differences, chosen by sign; 0 on
+ * the ghosts */
+static void orc_godunov(const orc_pf *p, const double *hh, const double *phi, const double *sign, double *grad) {
+  const long T = (long)p->N[0] * p->N[1] * p->N[2];
+  for (long c = 0; c < T; c++) grad[c] = 0.0;
+  for (int a = 0; a < p->ndim; a++) {
+    const long s = p->s[a];
+    const double h = hh[a];
+    for (int I = 0; I < p->n[0]; I++)
+      for (int J = 0; J < p->n[1]; J++)
+        for (int K = 0; K < p->n[2]; K++) {
+          const long c = PIDX(*p, I + p->off[0], J + p->off[1], K + p->off[2]);
+#define P(q) phi[c + (q) * s]
+          const double v = 1.0 / (12.0 * h) *
+                           (-(P(-1) - P(-2)) + 7.0 * (P(0) - P(-1)) + 7.0 * (P(1) - P(0)) - (P(2) - P(1)));
+          const double up = v + 1.0 / h *
+                                    orc_weno_godunov(P(3) - 2.0 * P(2) + P(1), P(2) - 2.0 * P(1) + P(0),
+                                                     P(1) - 2.0 * P(0) + P(-1), P(0) - 2.0 * P(-1) + P(-2));
+          const double um = v - 1.0 / h *
+                                    orc_weno_godunov(P(-3) - 2.0 * P(-2) + P(-1), P(-2) - 2.0 * P(-1) + P(0),
+                                                     P(-1) - 2.0 * P(0) + P(1), P(0) - 2.0 * P(1) + P(2));
+#undef P
+          double g;
+          if (sign[c] > 0.0) {
+            const double upm = -fmin(up, 0.0), ump = fmax(um, 0.0);
+            g = pow(fmax(upm, ump), 2);
+          } else {
+            const double upp = fmax(up, 0.0), upm = -fmin(um, 0.0);
+            g = pow(fmax(upp, upm), 2);
+          }
+          if (a == 0) grad[c] = g;
+          else grad[c] += g;
+        }
+  }
+  for (long c = 0; c < T; c++) grad[c] = sqrt(grad[c]);
+}
+
+/* zero_order_extrapolation (simple_bc.cpp:7-43) over the padded array */
+static void orc_zero_extrap(const orc_pf *p, double *f) {
+  const int g = ORC_GHC;
+  for (int a = 0; a < 3; a++) {
+    if (a > 0 && p->N[a] <= 1) continue;
+    const int t1 = (a + 1) % 3, t2 = (a + 2) % 3, N = p->N[a];
+    for (int u = 0; u < p->N[t1]; u++)
+      for (int v = 0; v < p->N[t2]; v++) {
+        const long base = (long)u * p->s[t1] + (long)v * p->s[t2];
+        for (int i = 0; i < g; i++) {
+          f[base + i * p->s[a]] = f[base + g * p->s[a]];
+          f[base + (N - 1 - i) * p->s[a]] = f[base + (N - 1 - g) * p->s[a]];
+        }
+      }
+  }
+}
+
+/* Sign (lsm.cpp:33-43) of every padded cell (find_sign) */
+void orc_find_sign(long n, const double *phi, double ls_width, double *sign) {
+  for (long c = 0; c < n; c++) {
+    const double x = phi[c];
+    if (x < -ls_width) sign[c] = -1;
+    else if (x > ls_width) sign[c] = 1;
+    else sign[c] = x / sqrt(x * x + ls_width * ls_width);
+  }
+}
+
+/* `nsteps` runge_kutta::tvd_rk3 steps with lsf_redistance_lambda
+ * (source.cpp:148-240): s = -sign (|grad| - 1) rdt/dt with the Godunov
+ * gradient, plus the interior mass-constraint term a/b delta |grad|
+ * rdt/dt from the zero-order-extrapolated a = sign delta (|grad| - 1),
+ * b = |grad| delta^2 (16-weight centre + 4 diagonal neighbours in 2-D,
+ * 51 + 8 in 3-D), |b| > 1e-8.  sign and delta are the dummy arrays the
+ * reference reads (find_sign / find_delta of earlier calls).            */
+void orc_redistance_rk3(int ndim, int nx, int ny, int nz, double dx, double dy, double dz, double dt, double rdt,
+                        const double *sign, const double *delta, const int *types, const double *values,
+                        double *phi, int nsteps) {
+  const orc_pf p = pf_make(ndim, nx, ny, nz);
+  const long T = (long)p.N[0] * p.N[1] * p.N[2];
+  const double hh[3] = {dx, dy, dz};
+  double *m = calloc((size_t)6 * T, sizeof(double));
+  double *s[3] = {m, m + T, m + 2 * T}, *gr = m + 3 * T, *A = m + 4 * T, *B = m + 5 * T;
+  for (int step = 0; step < nsteps; step++)
+    for (int stage = 0; stage < 3; stage++) {
+      double *r = s[stage];
+      orc_godunov(&p, hh, phi, sign, gr);
+      for (long c = 0; c < T; c++) r[c] = -sign[c] * (gr[c] - 1.0) * rdt / dt;
+      for (long c = 0; c < T; c++) {
+        A[c] = sign[c] * delta[c] * (gr[c] - 1.0);
+        B[c] = gr[c] * delta[c] * delta[c];
+      }
+      orc_zero_extrap(&p, A);
+      orc_zero_extrap(&p, B);
+      for (int I = 0; I < p.n[0]; I++)
+        for (int J = 0; J < p.n[1]; J++)
+          for (int K = 0; K < p.n[2]; K++) {
+            const long c = PIDX(p, I + p.off[0], J + p.off[1], K + p.off[2]);
+            double a, b;
+            if (ndim == 2) {
+              a = 16.0 * A[c];
+              b = 16.0 * B[c];
+              for (int ii = -1; ii < 2; ii++)
+                for (int jj = -1; jj < 2; jj++)
+                  if (ii != 0 && jj != 0) {
+                    a += A[c + ii * p.s[0] + jj * p.s[1]];
+                    b += B[c + ii * p.s[0] + jj * p.s[1]];
+                  }
+            } else {
+              a = 51.0 * A[c];
+              b = 51.0 * B[c];
+              for (int ii = -1; ii < 2; ii++)
+                for (int jj = -1; jj < 2; jj++)
+                  for (int kk = -1; kk < 2; kk++)
+                    if (ii != 0 && jj != 0 && kk != 0) {
+                      a += A[c + ii * p.s[0] + jj * p.s[1] + kk * p.s[2]];
+                      b += B[c + ii * p.s[0] + jj * p.s[1] + kk * p.s[2]];
+                    }
+            }
+            if (fabs(b) > 1e-8) r[c] += a / b * delta[c] * gr[c] * rdt / dt;
+          }
+      for (long c = 0; c < T; c++) {
+        if (stage == 0) phi[c] += dt * s[0][c];
+        else if (stage == 1) phi[c] += dt * (s[1][c] - 3.0 * s[0][c]) / 4.0;
+        else phi[c] += dt * (-s[0][c] - s[1][c] + 8.0 * s[2][c]) / 12.0;
+      }
+      orc_bc_apply(ndim, nx, ny, nz, dx, dy, dz, types, values, -1, phi);
+    }
+  free(m);
+}

@@ -227,6 +227,37 @@ void orc_node_velocity(int ndim, int nx, int ny, int nz, double dx, double dy, d
                        const int *vel_types, const double *u, const double *v, const double *w,
                        double *nu, double *nv, double *nw);
 
+/* ------------------------------------------------------------------ */
+/* Interface maintenance (the level set between flow steps).
+ * CCD / UCCD line derivatives (ccd_solver.cpp:14-142, uccd_solver.cpp:
+ * 14-206, coefficients ccd_base.cpp:43-104 / uccd_base.cpp:16-117, twin
+ * block solver matrix_solver.cpp:21-83) on the padded flux; upwind = 1
+ * selects UCCD by the sign of u/v/w.                          [golden] */
+void orc_compact_derivatives(int ndim, int nx, int ny, int nz, double dx, double dy, double dz,
+                             const double *flux, const double *u, const double *v, const double *w,
+                             int upwind, double *fx, double *fy, double *fz, double *fxx, double *fyy,
+                             double *fzz);
+void orc_ccd_coefficients(int n, double h, double *c);
+void orc_uccd_coefficients(int n, double h, double *up, double *dn);
+/* tvd_rk3 with the UCCD convection term (source.cpp:8-40)     [golden] */
+void orc_tvd_rk3_uccd(int ndim, int nx, int ny, int nz, double dx, double dy, double dz, double dt,
+                      const int *types, const double *values, double *phi, const double *u,
+                      const double *v, const double *w, int nsteps);
+/* lsf_mass (wrapper_func.cpp:450-511)                         [golden] */
+double orc_lsf_mass(int ndim, int nx, int ny, int nz, double dv, double ls_width, double ratio,
+                    int positive_ref, const double *phi);
+/* `count` Euler steps with mpls (source.cpp:109-146)          [golden] */
+void orc_mpls_euler(int ndim, int nx, int ny, int nz, double dx, double dy, double dz, double dt,
+                    double ls_width, double ratio, int positive_ref, double mass0, const int *types,
+                    const double *values, double *phi, int count, double *delta_out);
+/* find_sign (lsm.cpp:33-43)                                   [golden] */
+void orc_find_sign(long n, const double *phi, double ls_width, double *sign);
+/* `nsteps` tvd_rk3 steps with lsf_redistance_lambda (source.cpp:148-240,
+ * godunov_gradient.cpp:7-201)                                 [golden] */
+void orc_redistance_rk3(int ndim, int nx, int ny, int nz, double dx, double dy, double dz, double dt,
+                        double rdt, const double *sign, const double *delta, const int *types,
+                        const double *values, double *phi, int nsteps);
+
 #ifdef __cplusplus
 }
 #endif

@@ -116,6 +116,12 @@ class Oracle:
         _sig(L, "orc_fpc_step", _i, [_i, _i, _i, _i, _d, _d, _d, _d, _d, _d, _d, _d, _i, _ip] + [_dp] * 8)
         _sig(L, "orc_momentum_step", _i, [_i, _i, _i, _i, _d, _d, _d, _dp, _ip, _dp] + [_dp] * 10)
         _sig(L, "orc_node_velocity", None, [_i, _i, _i, _i, _d, _d, _d, _ip] + [_dp] * 6)
+        _sig(L, "orc_compact_derivatives", None, [_i, _i, _i, _i, _d, _d, _d, _dp, _vp, _vp, _vp, _i] + [_dp] * 6)
+        _sig(L, "orc_tvd_rk3_uccd", None, [_i, _i, _i, _i, _d, _d, _d, _d, _ip, _dp, _dp, _dp, _dp, _dp, _i])
+        _sig(L, "orc_lsf_mass", _d, [_i, _i, _i, _i, _d, _d, _d, _i, _dp])
+        _sig(L, "orc_mpls_euler", None, [_i, _i, _i, _i, _d, _d, _d, _d, _d, _d, _i, _d, _ip, _dp, _dp, _i, _vp])
+        _sig(L, "orc_find_sign", None, [_l, _dp, _d, _dp])
+        _sig(L, "orc_redistance_rk3", None, [_i, _i, _i, _i, _d, _d, _d, _d, _d, _dp, _dp, _ip, _dp, _dp, _i])
         _sig(L, "orc_vtk_write", _i, [C.c_char_p, _i, _i, _i, _i, _dp, _dp, _i, C.c_void_p, C.c_void_p,
                                       _i, C.c_void_p, C.c_void_p])
 
@@ -248,6 +254,12 @@ class Reference:
         _sig(L, "ref_fpc_step", _i, [_i, _i, _i, _i, _dp, _d, _d, _d, _d, _d, _ip] + [_dp] * 8)
         _sig(L, "ref_momentum_step", _i, [_i, _i, _i, _i, _dp, _dp, _ip, _dp] + [_dp] * 10)
         _sig(L, "ref_node_velocity", _i, [_i, _i, _i, _i, _dp, _ip] + [_dp] * 6)
+        _sig(L, "ref_compact_derivatives", None, [_i, _i, _i, _i, _dp, _dp, _vp, _vp, _vp, _i] + [_dp] * 6)
+        _sig(L, "ref_tvd_rk3_uccd", None, [_i, _i, _i, _i, _dp, _d, _ip, _dp, _dp, _dp, _dp, _dp, _i])
+        _sig(L, "ref_lsf_mass", _d, [_i, _i, _i, _i, _dp, _d, _d, _i, _dp])
+        _sig(L, "ref_mpls_euler", None, [_i, _i, _i, _i, _dp, _d, _d, _d, _i, _d, _ip, _dp, _dp, _i, _vp])
+        _sig(L, "ref_find_sign", None, [_i, _i, _i, _i, _dp, _d, _dp, _dp])
+        _sig(L, "ref_redistance_rk3", None, [_i, _i, _i, _i, _dp, _d, _d, _dp, _dp, _ip, _dp, _dp, _i])
         _sig(L, "ref_vtk_write", _i, [C.c_char_p, C.c_uint, _i, _i, _i, _i, _dp, _i, C.c_void_p, C.c_void_p,
                                       _i, C.c_void_p, C.c_void_p])
 

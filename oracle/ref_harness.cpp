@@ -14,6 +14,8 @@
 #include "multigrid.h"
 #include "projection_method.h"
 #include "runge_kutta.h"
+#include "lsm.h"
+#include "source.h"
 #include "vtkWriter.h"
 #include "scalar_data.h"
 #include "shared_solvers.h"
@@ -489,6 +491,127 @@ int ref_node_velocity(int ndim, int nx, int ny, int nz, const double *ext, const
   from_ref(s, nvel->vector->y.data, nv);
   from_ref(s, nvel->vector->z.data, nw);
   return 0;
+}
+
+// ------------------------------------------------- interface maintenance
+namespace {
+struct LsSetup {
+  structured_grid *geo;
+  wrapper *f, *vel;
+  problem_parameters *param;
+  dummy_data *dummy;
+};
+LsSetup ls_setup(int ndim, int nx, int ny, int nz, const double *ext, const int *types, const double *values,
+                 const double *phi, const double *u, const double *v, const double *w) {
+  LsSetup L;
+  L.geo = make_grid(ndim, nx, ny, nz, ext);
+  L.f = make_wrapper(true, L.geo, ndim, types, values);
+  int vt[6] = {0, 0, 0, 0, 0, 0};
+  double vv[6] = {0, 0, 0, 0, 0, 0};
+  L.vel = make_wrapper(false, L.geo, ndim, vt, vv);
+  L.param = new problem_parameters();
+  auto *solvers = SharedSolvers_alloc(L.f->scalar, L.geo);
+  L.dummy = dummy_data_alloc(L.f->scalar);
+  for (wrapper *x : {L.f, L.vel}) {
+    x->link_params(L.param);
+    x->link_solvers(solvers);
+    x->link_dummy(L.dummy);
+  }
+  scalar_data *s = L.f->scalar;
+  to_ref(s, s->data, phi);
+  if (u) to_ref(s, L.vel->vector->x.data, u);
+  if (v) to_ref(s, L.vel->vector->y.data, v);
+  if (w) to_ref(s, L.vel->vector->z.data, w);
+  return L;
+}
+}  // namespace
+
+// uccd_solver::find_derivatives(f, vel) (upwind = 1) or
+// ccd_solver::find_derivatives(f) on the flux array
+void ref_compact_derivatives(int ndim, int nx, int ny, int nz, const double *ext, const double *flux,
+                             const double *u, const double *v, const double *w, int upwind, double *fx,
+                             double *fy, double *fz, double *fxx, double *fyy, double *fzz) {
+  int t[6] = {0, 0, 0, 0, 0, 0};
+  double vv[6] = {0, 0, 0, 0, 0, 0};
+  LsSetup L = ls_setup(ndim, nx, ny, nz, ext, t, vv, flux, u, v, w);
+  scalar_data *s = L.f->scalar;
+  to_ref(s, s->flux, flux);
+  if (upwind) L.f->solvers->uccd->find_derivatives(s, L.vel->vector);
+  else L.f->solvers->ccd->find_derivatives(s);
+  from_ref(s, s->fx, fx);
+  from_ref(s, s->fxx, fxx);
+  if (ndim > 1) {
+    from_ref(s, s->fy, fy);
+    from_ref(s, s->fyy, fyy);
+  }
+  if (ndim > 2) {
+    from_ref(s, s->fz, fz);
+    from_ref(s, s->fzz, fzz);
+  }
+}
+
+// runge_kutta::tvd_rk3(phi, nvel, identity_flux, convection) (UCCD)
+void ref_tvd_rk3_uccd(int ndim, int nx, int ny, int nz, const double *ext, double dt, const int *types,
+                      const double *values, double *phi, const double *u, const double *v, const double *w,
+                      int nsteps) {
+  LsSetup L = ls_setup(ndim, nx, ny, nz, ext, types, values, phi, u, v, w);
+  L.param->dt = dt;
+  runge_kutta rk(L.f->scalar->Nx, L.f->scalar->Ny, L.f->scalar->Nz);
+  for (int s = 0; s < nsteps; ++s) rk.tvd_rk3(L.f, L.vel, &identity_flux, &convection);
+  from_ref(L.f->scalar, L.f->scalar->data, phi);
+}
+
+double ref_lsf_mass(int ndim, int nx, int ny, int nz, const double *ext, double ls_width, double ratio,
+                    int positive_ref, const double *phi) {
+  int t[6] = {0, 0, 0, 0, 0, 0};
+  double vv[6] = {0, 0, 0, 0, 0, 0};
+  LsSetup L = ls_setup(ndim, nx, ny, nz, ext, t, vv, phi, nullptr, nullptr, nullptr);
+  L.param->ls_width = ls_width;
+  L.param->density_ratio = ratio;
+  L.param->positive_ref = positive_ref != 0;
+  return lsf_mass(L.f);
+}
+
+// `count` runge_kutta::euler(phi, nvel, identity_flux, mpls) steps
+void ref_mpls_euler(int ndim, int nx, int ny, int nz, const double *ext, double dt, double ls_width,
+                    double ratio, int positive_ref, double mass0, const int *types, const double *values,
+                    double *phi, int count, double *delta_out) {
+  LsSetup L = ls_setup(ndim, nx, ny, nz, ext, types, values, phi, nullptr, nullptr, nullptr);
+  L.param->dt = dt;
+  L.param->ls_width = ls_width;
+  L.param->density_ratio = ratio;
+  L.param->positive_ref = positive_ref != 0;
+  L.param->lsf_mass0 = mass0;
+  runge_kutta rk(L.f->scalar->Nx, L.f->scalar->Ny, L.f->scalar->Nz);
+  for (int s = 0; s < count; ++s) rk.euler(L.f, L.vel, &identity_flux, &mpls);
+  from_ref(L.f->scalar, L.f->scalar->data, phi);
+  if (delta_out) from_ref(L.f->scalar, L.dummy->delta, delta_out);
+}
+
+// find_sign (wrapper_func.cpp:31-40)
+void ref_find_sign(int ndim, int nx, int ny, int nz, const double *ext, double ls_width, const double *phi,
+                   double *sign) {
+  int t[6] = {0, 0, 0, 0, 0, 0};
+  double vv[6] = {0, 0, 0, 0, 0, 0};
+  LsSetup L = ls_setup(ndim, nx, ny, nz, ext, t, vv, phi, nullptr, nullptr, nullptr);
+  L.param->ls_width = ls_width;
+  find_sign(L.f);
+  from_ref(L.f->scalar, L.dummy->sign, sign);
+}
+
+// `nsteps` runge_kutta::tvd_rk3(phi, nvel, identity_flux,
+// lsf_redistance_lambda) with dummy->sign and dummy->delta given
+void ref_redistance_rk3(int ndim, int nx, int ny, int nz, const double *ext, double dt, double rdt,
+                        const double *sign, const double *delta, const int *types, const double *values,
+                        double *phi, int nsteps) {
+  LsSetup L = ls_setup(ndim, nx, ny, nz, ext, types, values, phi, nullptr, nullptr, nullptr);
+  L.param->dt = dt;
+  L.param->rdt = rdt;
+  to_ref(L.f->scalar, L.dummy->sign, sign);
+  to_ref(L.f->scalar, L.dummy->delta, delta);
+  runge_kutta rk(L.f->scalar->Nx, L.f->scalar->Ny, L.f->scalar->Nz);
+  for (int s = 0; s < nsteps; ++s) rk.tvd_rk3(L.f, L.vel, &identity_flux, &lsf_redistance_lambda);
+  from_ref(L.f->scalar, L.f->scalar->data, phi);
 }
 
 }  // extern "C"

@@ -284,10 +284,60 @@ def momentum_cases(ref):
     return out
 
 
+def interface_cases(ref):
+    """Interface maintenance: CCD and UCCD line derivatives, tvd_rk3 with the
+    UCCD convection term, lsf_mass, Euler steps with mpls, find_sign and
+    tvd_rk3 with lsf_redistance_lambda, from a perturbed circle/sphere
+    level set and seeded velocities."""
+    out = {}
+    rng = np.random.default_rng(37)
+    lens = (1.0, 2.0, 1.5)
+    for ndim, n, types in [(2, (20, 14, 1), (0, 0, 0, 0, 0, 0)), (3, (9, 8, 7), (0, 0, 0, 0, 0, 0))]:
+        shp = padded_shape(ndim, *n)
+        T = int(np.prod(shp))
+        ext = ext_for(ndim, n, lens)
+        h = [lens[a] / n[a] if a < ndim else 1.0 for a in range(3)]
+        idx = np.indices(shp).astype(float)
+        X = [(idx[a] - 3 + 0.5) * h[a] for a in range(3)]
+        ctr = (0.45, 0.8, 0.6)
+        r0 = np.sqrt(sum((X[a] - ctr[a]) ** 2 for a in range(ndim)))
+        phi = (0.3 - r0 + 0.02 * np.sin(5 * X[0]) * np.cos(3 * X[1])).ravel()
+        u, v, w = (rng.standard_normal(T) for _ in range(3))
+        types = np.array(types, np.int32)
+        vals = np.zeros(6)
+        d = dict(ndim=ndim, n=np.array(n), ext=ext, types=types, phi=phi, u=u, v=v, w=w)
+        for up in (0, 1):
+            outs = [np.zeros(T) for _ in range(6)]
+            vp = [x.ctypes.data for x in (u, v, w)]
+            ref.lib.ref_compact_derivatives(ndim, *n, ext, phi, *vp, up, *outs)
+            tag = "uccd" if up else "ccd"
+            for k, x in zip(("fx", "fy", "fz", "fxx", "fyy", "fzz"), outs):
+                d[f"{tag}_{k}"] = x
+        dt, lw, ratio = 0.004, 1.5 * h[0], 1e-3
+        a = phi.copy()
+        ref.lib.ref_tvd_rk3_uccd(ndim, *n, ext, dt, types, vals, a, u, v, w, 2)
+        d["rk3_phi"] = a
+        mass = ref.lib.ref_lsf_mass(ndim, *n, ext, lw, ratio, 1, phi)
+        d["mass"] = mass
+        a = phi.copy()
+        delta = np.zeros(T)
+        ref.lib.ref_mpls_euler(ndim, *n, ext, dt, lw, ratio, 1, mass * 1.002, types, vals, a, 3, delta.ctypes.data)
+        d.update(mpls_mass0=mass * 1.002, mpls_phi=a, mpls_delta=delta)
+        sign = np.zeros(T)
+        ref.lib.ref_find_sign(ndim, *n, ext, lw, phi, sign)
+        d["sign"] = sign
+        a = 1.3 * phi
+        ref.lib.ref_redistance_rk3(ndim, *n, ext, dt, 0.25 * h[0], sign, delta, types, vals, a, 3)
+        d.update(redist_rdt=0.25 * h[0], redist_phi=a, dt=dt, ls_width=lw, ratio=ratio)
+        out[f"iface{ndim}d"] = d
+    return out
+
+
 def main():
     ref = Reference()
     groups = {"mg": mg_cases, "bc": bc_cases, "weno": weno_cases, "rk3": rk3_cases, "fields": field_cases,
-              "fpc": fpc_cases, "vtk": vtk_cases, "momentum": momentum_cases}
+              "fpc": fpc_cases, "vtk": vtk_cases, "momentum": momentum_cases,
+              "interface": interface_cases}
     only = sys.argv[1:]
     if only:
         groups = {k: v for k, v in groups.items() if k in only}

@@ -250,3 +250,112 @@ def test_momentum_step_rest_state(oracle):
     assert np.all(u[3:15, 3:13] == 0.0)
     np.testing.assert_allclose(v[3:15, 3:12], 0.01 * 1.5 * (-1.0 / 2.0), rtol=1e-15)
     assert np.all(z[7].reshape(shp) == -0.5)  # v_src on every padded cell
+
+
+IFACE = load("interface")
+
+
+@pytest.mark.parametrize("case", sorted(IFACE), ids=str)
+def test_interface_maintenance_matches_reference(oracle, case):
+    """CCD and UCCD derivatives, tvd_rk3 with UCCD convection, lsf_mass,
+    mpls Euler steps, find_sign and the redistancing RK3 restated in C equal
+    the compiled reference bit for bit."""
+    c = IFACE[case]
+    ndim, n = int(c["ndim"]), [int(x) for x in c["n"]]
+    h = spacing(c)
+    T = c["phi"].size
+    types = np.ascontiguousarray(c["types"], np.int32)
+    vals = np.zeros(6)
+    for up, tag in ((0, "ccd"), (1, "uccd")):
+        outs = [np.zeros(T) for _ in range(6)]
+        vp = [c[k].ctypes.data for k in ("u", "v", "w")]
+        oracle.lib.orc_compact_derivatives(ndim, *n, *h, c["phi"], *vp, up, *outs)
+        for k, got in zip(("fx", "fy", "fz", "fxx", "fyy", "fzz"), outs):
+            np.testing.assert_array_equal(got, c[f"{tag}_{k}"], err_msg=f"{tag} {k}")
+    dt, lw, ratio = float(c["dt"]), float(c["ls_width"]), float(c["ratio"])
+    a = c["phi"].copy()
+    oracle.lib.orc_tvd_rk3_uccd(ndim, *n, *h, dt, types, vals, a, c["u"], c["v"], c["w"], 2)
+    np.testing.assert_array_equal(a, c["rk3_phi"])
+    dv = h[0] * h[1] * (h[2] if ndim == 3 else 1.0)
+    assert oracle.lib.orc_lsf_mass(ndim, *n, dv, lw, ratio, 1, c["phi"]) == float(c["mass"])
+    a = c["phi"].copy()
+    delta = np.zeros(T)
+    oracle.lib.orc_mpls_euler(ndim, *n, *h, dt, lw, ratio, 1, float(c["mpls_mass0"]), types, vals, a, 3,
+                              delta.ctypes.data)
+    np.testing.assert_array_equal(a, c["mpls_phi"])
+    np.testing.assert_array_equal(delta, c["mpls_delta"])
+    sign = np.zeros(T)
+    oracle.lib.orc_find_sign(T, c["phi"], lw, sign)
+    np.testing.assert_array_equal(sign, c["sign"])
+    a = 1.3 * c["phi"]
+    oracle.lib.orc_redistance_rk3(ndim, *n, *h, dt, float(c["redist_rdt"]), c["sign"], c["mpls_delta"], types, vals,
+                                  a, 3)
+    np.testing.assert_array_equal(a, c["redist_phi"])
+
+
+def _line_field(n, h, f):
+    """2-D padded field whose every row along x is f(x_c) (ghosts included)."""
+    from scihpc_amd import padded_shape
+    shp = padded_shape(2, n, 6)
+    x = (np.arange(shp[0]) - 3 + 0.5) * h
+    return shp, np.repeat(f(x)[:, None], shp[1], axis=1).ravel()
+
+
+def test_compact_derivatives_converge(oracle):
+    """Away from the one-sided line ends, CCD and UCCD first derivatives of a
+    smooth function converge at high order: halving h cuts the interior
+    error by more than 16 (fourth order or better)."""
+    errs = {0: [], 1: []}
+    for n in (32, 64):
+        h = 1.0 / n
+        shp, f = _line_field(n, h, np.sin)
+        T = f.size
+        x = np.repeat(((np.arange(shp[0]) - 3 + 0.5) * h)[:, None], shp[1], axis=1)
+        for up in (0, 1):
+            outs = [np.zeros(T) for _ in range(6)]
+            ones = np.ones(T)
+            oracle.lib.orc_compact_derivatives(2, n, 6, 1, h, h, 1.0, f, ones.ctypes.data, ones.ctypes.data,
+                                               None, up, *outs)
+            fx = outs[0].reshape(shp[:2])
+            core = slice(8, shp[0] - 8)
+            errs[up].append(np.max(np.abs(fx[core] - np.cos(x[core]))))
+    for up in (0, 1):
+        assert errs[up][0] / errs[up][1] > 16.0, (up, errs[up])
+
+
+def test_mpls_moves_mass_toward_target(oracle):
+    """One mpls Euler step moves the level-set mass toward mass0 (its
+    purpose, source.cpp:109-146): the gap shrinks by more than half."""
+    c = IFACE["iface2d"]
+    n = [int(x) for x in c["n"]]
+    h = spacing(c)
+    dv = h[0] * h[1]
+    lw, ratio, dt = float(c["ls_width"]), float(c["ratio"]), float(c["dt"])
+    m0 = oracle.lib.orc_lsf_mass(2, *n, dv, lw, ratio, 1, c["phi"])
+    target = m0 * 1.002
+    a = c["phi"].copy()
+    oracle.lib.orc_mpls_euler(2, *n, *h, dt, lw, ratio, 1, target, np.zeros(6, np.int32), np.zeros(6), a, 1, None)
+    m1 = oracle.lib.orc_lsf_mass(2, *n, dv, lw, ratio, 1, a)
+    assert abs(target - m1) < 0.5 * abs(target - m0)
+
+
+def test_redistancing_restores_unit_gradient(oracle):
+    """phi scaled by 1.3 has |grad phi| = 1.3; redistancing steps bring the
+    interior gradient near the interface back toward 1."""
+    c = IFACE["iface2d"]
+    n = [int(x) for x in c["n"]]
+    h = spacing(c)
+    shp = (n[0] + 6, n[1] + 6, 1)
+    a = 1.3 * c["phi"]
+
+    def grad_err(p):
+        q = p.reshape(shp)[:, :, 0]
+        gx = (q[4:-2, 3:-3] - q[2:-4, 3:-3]) / (2 * h[0])
+        gy = (q[3:-3, 4:-2] - q[3:-3, 2:-4]) / (2 * h[1])
+        band = np.abs(q[3:-3, 3:-3]) < 0.1
+        return np.mean(np.abs(np.hypot(gx, gy)[band] - 1.0))
+
+    e0 = grad_err(a)
+    oracle.lib.orc_redistance_rk3(2, *n, *h, 0.004, 0.25 * h[0], c["sign"], c["mpls_delta"],
+                                  np.zeros(6, np.int32), np.zeros(6), a, 20)
+    assert grad_err(a) < 0.5 * e0
