@@ -355,6 +355,40 @@ SCIHPC_API int scihpc_ls_density(scihpc_ls *h, double ls_width, double ratio,
 SCIHPC_API int scihpc_ls_time_step(scihpc_ls *h, double dt, int iters, double *ms,
                         double *bytes);
 
+/* Interface maintenance (the level set between flow steps). */
+enum { SCIHPC_LS_WENO5 = 0, SCIHPC_LS_UCCD = 1 };
+/* Convection scheme of scihpc_ls_tvd_rk3: WENO5 (weno_solver.cpp, the
+ * default) or UCCD, the reference's own convection (source.cpp:8-40 with
+ * uccd_solver.cpp:14-215: compact upwind/downwind line systems chosen by the
+ * sign of the node velocity), updated on every padded cell as the reference. */
+SCIHPC_API int scihpc_ls_set_scheme(scihpc_ls *h, int scheme);
+/* uccd_solver::find_derivatives(f, vel) (upwind = 1, the handle's
+ * velocity) or ccd_solver::find_derivatives(f) (ccd_solver.cpp:14-142) of
+ * a padded host array: first and second derivatives along every axis. */
+SCIHPC_API int scihpc_ls_compact_derivatives(scihpc_ls *h, const double *f, int upwind, double *fx,
+                                             double *fy, double *fz, double *fxx, double *fyy,
+                                             double *fzz);
+/* lsf_mass (wrapper_func.cpp:450-511) of the current phi (2-D, 3-D). */
+SCIHPC_API int scihpc_ls_mass(scihpc_ls *h, double ls_width, double ratio, int positive_ref,
+                              double *mass);
+/* `count` runge_kutta::euler steps with mpls (source.cpp:109-146): phi
+ * moved along its normal by eta delta |grad phi| so that lsf_mass returns
+ * toward mass0 (params->lsf_mass0); BCs after each step.  Leaves delta of
+ * the last pre-update phi (the reference's dummy->delta). */
+SCIHPC_API int scihpc_ls_mpls(scihpc_ls *h, double dt, double ls_width, double ratio,
+                              int positive_ref, double mass0, int count);
+/* find_sign / find_delta (wrapper_func.cpp:20-40) of the current phi into
+ * the handle's sign / delta (the dummy arrays the redistancing reads). */
+SCIHPC_API int scihpc_ls_find_sign(scihpc_ls *h, double ls_width);
+SCIHPC_API int scihpc_ls_find_delta(scihpc_ls *h, double ls_width);
+enum { SCIHPC_LS_SIGN = 0, SCIHPC_LS_DELTA = 1 };
+SCIHPC_API int scihpc_ls_set_field(scihpc_ls *h, int which, const double *src);
+SCIHPC_API int scihpc_ls_get_field(scihpc_ls *h, int which, double *dst);
+/* `nsteps` runge_kutta::tvd_rk3 steps with lsf_redistance_lambda
+ * (source.cpp:148-240, godunov_gradient.cpp:7-201): pseudo-time rdt,
+ * the stored sign and delta (not recomputed, as in the reference). */
+SCIHPC_API int scihpc_ls_redistance(scihpc_ls *h, double dt, double rdt, int nsteps);
+
 /* ------------------------------------------------------------------ */
 /* Snapshots: vtkWriter (vtkWriter.h:15-35, vtkWriter.cpp:18-111).  Files are
  * byte-identical to the reference's legacy VTK (ASCII header, BINARY

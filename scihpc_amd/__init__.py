@@ -334,6 +334,44 @@ class LevelSet:
         check(self._lib.scihpc_ls_time_step(self.h, dt, iters, C.byref(ms), C.byref(b)))
         return ms.value, b.value
 
+    # ------------------------------------------- interface maintenance
+    def set_scheme(self, scheme):
+        """'weno5' (default) or 'uccd' (the reference's convection term)."""
+        code = {"weno5": 0, "uccd": 1}[scheme] if isinstance(scheme, str) else int(scheme)
+        check(self._lib.scihpc_ls_set_scheme(self.h, code))
+
+    def compact_derivatives(self, f, upwind=True):
+        """UCCD (by the handle's velocity) or CCD derivatives of a padded
+        field: (fx, fy, fz, fxx, fyy, fzz)."""
+        outs = [np.zeros(self.size) for _ in range(6)]
+        check(self._lib.scihpc_ls_compact_derivatives(self.h, _f64(f), int(upwind), *outs))
+        return tuple(o.reshape(self.shape) for o in outs)
+
+    def mass(self, ls_width, ratio, positive_ref=True):
+        m = C.c_double()
+        check(self._lib.scihpc_ls_mass(self.h, ls_width, ratio, int(positive_ref), C.byref(m)))
+        return m.value
+
+    def mpls(self, dt, ls_width, ratio, mass0, count=1, positive_ref=True):
+        check(self._lib.scihpc_ls_mpls(self.h, dt, ls_width, ratio, int(positive_ref), mass0, count))
+
+    def find_sign(self, ls_width):
+        check(self._lib.scihpc_ls_find_sign(self.h, ls_width))
+
+    def find_delta(self, ls_width):
+        check(self._lib.scihpc_ls_find_delta(self.h, ls_width))
+
+    def set_field(self, which, a):
+        check(self._lib.scihpc_ls_set_field(self.h, {"sign": 0, "delta": 1}[which], _f64(a)))
+
+    def field(self, which):
+        out = np.zeros(self.size)
+        check(self._lib.scihpc_ls_get_field(self.h, {"sign": 0, "delta": 1}[which], out))
+        return out.reshape(self.shape)
+
+    def redistance(self, dt, rdt, nsteps=1):
+        check(self._lib.scihpc_ls_redistance(self.h, dt, rdt, nsteps))
+
 
 class Projection:
     """The projection step on padded fields around a Multigrid of the same

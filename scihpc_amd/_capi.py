@@ -135,6 +135,15 @@ SIGNATURES = {
     "scihpc_ls_apply_bc": [_vp, _dp, _i],
     "scihpc_ls_density": [_vp, _d, _d, _i, _dp],
     "scihpc_ls_time_step": [_vp, _d, _i, _pd, _pd],
+    "scihpc_ls_set_scheme": [_vp, _i],
+    "scihpc_ls_compact_derivatives": [_vp, _dp, _i] + [_dp] * 6,
+    "scihpc_ls_mass": [_vp, _d, _d, _i, _pd],
+    "scihpc_ls_mpls": [_vp, _d, _d, _d, _i, _d, _i],
+    "scihpc_ls_find_sign": [_vp, _d],
+    "scihpc_ls_find_delta": [_vp, _d],
+    "scihpc_ls_set_field": [_vp, _i, _dp],
+    "scihpc_ls_get_field": [_vp, _i, _dp],
+    "scihpc_ls_redistance": [_vp, _d, _d, _i],
     "scihpc_vtk_create": [_i, _i, _i, _i, _dp, _dp, C.c_char_p, C.POINTER(_vp)],
     "scihpc_vtk_destroy": [_vp],
     "scihpc_vtk_begin": [_vp, C.c_uint],

@@ -722,6 +722,461 @@ __global__ void __launch_bounds__(kLsBlock)
   if constexpr (DIM == 3) nw[c] = 0.5 * (w[c] + w[c - g.s[2]]);
 }
 
+// ------------------------------------------------- interface maintenance
+// Compact line derivatives (ccd_solver.cpp:14-142, uccd_solver.cpp:14-206):
+// along every padded line of axis a the right-hand sides of the scheme are
+// formed and the 2x2-block tridiagonal system is solved by the reference's
+// twin back substitution (matrix_solver.cpp:58-83) with the factorised
+// coefficients of the axis (twin_dec, :21-56, done once on the host).  One
+// thread per line; the forward sweep keeps the previous unknowns in
+// registers and parks the line in scratch arrays for the backward sweep.
+// Lines are enumerated with the contiguous k axis fastest where the line
+// does not run along k.  Coefficient set: 12 arrays of n (a0 a1 a2 b0 b1 b2
+// aa0 aa1 aa2 bb0 bb1 bb2).
+
+__device__ __forceinline__ bool line_base(const Grid &g, int a, long t, long &base) {
+  long lines;
+  if (a == 0) {
+    lines = (long)g.N[1] * g.N[2];
+    if (t >= lines) return false;
+    base = (t / g.N[2]) * g.s[1] + (t % g.N[2]);
+  } else if (a == 1) {
+    lines = (long)g.N[0] * g.N[2];
+    if (t >= lines) return false;
+    base = (t / g.N[2]) * g.s[0] + (t % g.N[2]);
+  } else {
+    lines = (long)g.N[0] * g.N[1];
+    if (t >= lines) return false;
+    base = (t / g.N[1]) * g.s[0] + (t % g.N[1]) * g.s[1];
+  }
+  return true;
+}
+
+// right-hand side at position i of one line; KIND 0 CCD, 1 UCCD upwind,
+// 2 UCCD downwind (whose ends are the upwind ones)
+template <int KIND>
+__device__ __forceinline__ void compact_rhs(const double *__restrict__ f, long st, int n, double h, int i,
+                                            double &s, double &ss) {
+#pragma clang fp contract(off)
+  const double bda1 = -3.5, bdb1 = 4.0, bdc1 = -0.5, bda2 = 9.0, bdb2 = -12.0, bdc2 = 3.0;
+  const double c3 = -0.06096119008109, c2 = 1.99692238016218, c1 = -1.93596119008109;
+#define F(q) f[(long)(q) * st]
+  if (i == 0) {
+    s = (bda1 * F(0) + bdb1 * F(1) + bdc1 * F(2)) / h;
+    ss = (bda2 * F(0) + bdb2 * F(1) + bdc2 * F(2)) / h / h;
+  } else if (i == n - 1) {
+    s = -(bda1 * F(n - 1) + bdb1 * F(n - 2) + bdc1 * F(n - 3)) / h;
+    ss = (bda2 * F(n - 1) + bdb2 * F(n - 2) + bdc2 * F(n - 3)) / h / h;
+  } else {
+    if (KIND == 0) s = 15.0 / 16.0 * (F(i + 1) - F(i - 1)) / h;
+    else if (KIND == 1) s = (c1 * F(i - 1) + c2 * F(i) + c3 * F(i + 1)) / h;
+    else s = -(c3 * F(i - 1) + c2 * F(i) + c1 * F(i + 1)) / h;
+    ss = (3.0 * F(i - 1) - 6.0 * F(i) + 3.0 * F(i + 1)) / h / h;
+  }
+#undef F
+}
+
+// twin_bks of one line: rhs formed on the fly in the forward sweep; S/SS
+// scratch (same layout as the fields); the solution is left in S/SS
+template <int KIND>
+__device__ __forceinline__ void compact_solve(const double *__restrict__ f, const double *__restrict__ c, long st,
+                                              int n, double h, double *__restrict__ S, double *__restrict__ SS) {
+#pragma clang fp contract(off)
+  const double *a0 = c, *a1 = c + n, *a2 = c + 2 * n, *b0 = c + 3 * n, *b1 = c + 4 * n, *b2 = c + 5 * n;
+  const double *aa0 = c + 6 * n, *aa1 = c + 7 * n, *aa2 = c + 8 * n, *bb0 = c + 9 * n, *bb1 = c + 10 * n,
+               *bb2 = c + 11 * n;
+  double sp, ssp;
+  compact_rhs<KIND == 2 ? 1 : KIND>(f, st, n, h, 0, sp, ssp);
+  S[0] = sp;
+  SS[0] = ssp;
+  for (int i = 1; i < n; ++i) {
+    double si, ssi;
+    if (KIND == 2 && i == n - 1) compact_rhs<1>(f, st, n, h, i, si, ssi);
+    else compact_rhs<KIND>(f, st, n, h, i, si, ssi);
+    si = si - (a0[i] * sp + b0[i] * ssp);
+    ssi = ssi - (aa0[i] * sp + bb0[i] * ssp);
+    S[(long)i * st] = si;
+    SS[(long)i * st] = ssi;
+    sp = si;
+    ssp = ssi;
+  }
+  double den = a1[n - 1] * bb1[n - 1] - aa1[n - 1] * b1[n - 1];
+  double sols = -(b1[n - 1] * ssp - bb1[n - 1] * sp) / den;
+  double solss = (a1[n - 1] * ssp - aa1[n - 1] * sp) / den;
+  S[(long)(n - 1) * st] = sols;
+  SS[(long)(n - 1) * st] = solss;
+  double sn = sols, ssn = solss;
+  for (int i = n - 2; i > -1; --i) {
+    double si = S[(long)i * st], ssi = SS[(long)i * st];
+    si = si - (a2[i] * sn + b2[i] * ssn);
+    ssi = ssi - (aa2[i] * sn + bb2[i] * ssn);
+    den = a1[i] * bb1[i] - aa1[i] * b1[i];
+    sols = -(b1[i] * ssi - bb1[i] * si) / den;
+    solss = (a1[i] * ssi - aa1[i] * si) / den;
+    S[(long)i * st] = sols;
+    SS[(long)i * st] = solss;
+    sn = sols;
+    ssn = solss;
+  }
+}
+
+// d1 (and d2 when not null) along axis a of every padded line of f; UP:
+// UCCD picked by vel > 0, d2 the mean of both second derivatives; else CCD
+template <bool UP>
+__global__ void __launch_bounds__(kLsBlock)
+    k_compact_line(Grid g, int a, const double *__restrict__ f, const double *__restrict__ vel,
+                   const double *__restrict__ cu, const double *__restrict__ cd, double *__restrict__ su,
+                   double *__restrict__ ssu, double *__restrict__ sd, double *__restrict__ ssd,
+                   double *__restrict__ d1, double *__restrict__ d2) {
+#pragma clang fp contract(off)
+  long base;
+  if (!line_base(g, a, (long)blockIdx.x * blockDim.x + threadIdx.x, base)) return;
+  const int n = g.N[a];
+  const long st = g.s[a];
+  const double h = g.h[a];
+  if (UP) {
+    compact_solve<1>(f + base, cu, st, n, h, su + base, ssu + base);
+    compact_solve<2>(f + base, cd, st, n, h, sd + base, ssd + base);
+    for (int i = 0; i < n; ++i) {
+      const long c = base + (long)i * st;
+      d1[c] = vel[c] > 0 ? su[c] : sd[c];
+      if (d2) d2[c] = 0.5 * (ssu[c] + ssd[c]);
+    }
+  } else {
+    compact_solve<0>(f + base, cu, st, n, h, su + base, ssu + base);
+    for (int i = 0; i < n; ++i) {
+      const long c = base + (long)i * st;
+      d1[c] = su[c];
+      if (d2) d2[c] = ssu[c];
+    }
+  }
+}
+
+// convection (source.cpp:8-40) and the TVD-RK3 stage update
+// (runge_kutta.cpp:13-49) on every padded cell, in place
+template <int DIM, int STAGE>
+__global__ void __launch_bounds__(kLsBlock)
+    k_uccd_stage(long n, double *__restrict__ phi, const double *__restrict__ u, const double *__restrict__ v,
+                 const double *__restrict__ w, const double *__restrict__ fx, const double *__restrict__ fy,
+                 const double *__restrict__ fz, double *__restrict__ s1, double *__restrict__ s2, double dt) {
+#pragma clang fp contract(off)
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  double s = -u[c] * fx[c];
+  if (DIM > 1) s -= v[c] * fy[c];
+  if (DIM > 2) s -= w[c] * fz[c];
+  if (STAGE == 1) {
+    s1[c] = s;
+    phi[c] += dt * s;
+  } else if (STAGE == 2) {
+    s2[c] = s;
+    phi[c] += dt * (s - 3.0 * s1[c]) / 4.0;
+  } else {
+    phi[c] += dt * (-s1[c] - s2[c] + 8.0 * s) / 12.0;
+  }
+}
+
+__device__ __forceinline__ double ls_heav(double x, double w) {
+#pragma clang fp contract(off)
+  const double pi = 3.14159265358979323846;
+  if (x < -w) return 0;
+  if (x > w) return 1;
+  return 0.5 * (1 + x / w + sin(pi * x / w) / pi);
+}
+
+__device__ __forceinline__ double ls_delta(double x, double w) {
+#pragma clang fp contract(off)
+  const double pi = 3.14159265358979323846;
+  if (x < -w) return 0;
+  if (x > w) return 0;
+  return 0.5 * (1 + cos(pi * x / w)) / w;
+}
+
+// identity_with_extrapolation: flux(c) = phi(clamped c) on every padded cell
+template <int DIM>
+__global__ void __launch_bounds__(kLsBlock) k_extrap(Grid g, const double *__restrict__ phi,
+                                                     double *__restrict__ flux) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.total) return;
+  int q[3];
+  padded_ijk(g, c, q);
+  flux[c] = phi[ext_index<DIM>(g, q, -1)];
+}
+
+__device__ __forceinline__ double grad_mag(int dim, double fx, double fy, double fz) {
+#pragma clang fp contract(off)
+  double gr = fx * fx;
+  if (dim > 1) gr += fy * fy;
+  if (dim > 2) gr += fz * fz;
+  return sqrt(gr);
+}
+
+// per interior cell: the lsf_mass term (wrapper_func.cpp:450-511) and the
+// mpls denominator term g delta |grad| (source.cpp:119-138); part[2b],
+// part[2b+1] = block sums
+template <int DIM>
+__global__ void __launch_bounds__(kLsBlock)
+    k_mpls_sums(Grid g, const double *__restrict__ phi, const double *__restrict__ fx,
+                const double *__restrict__ fy, const double *__restrict__ fz, double w, double ratio,
+                int positive_ref, double dv, double *__restrict__ part) {
+#pragma clang fp contract(off)
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double m = 0.0, e = 0.0;
+  if (t < g.interior) {
+    int I, J, K;
+    interior_ijk(g, t, I, J, K);
+    const long c = pidx(g, I, J, K);
+    auto dens = [&](long q, double &H) {
+      H = ls_heav(phi[q], w);
+      return positive_ref ? H + (1.0 - H) * ratio : (1.0 - H) + H * ratio;
+    };
+    double Hc;
+    const double dc = dens(c, Hc);
+    const bool inner = I != 0 && I != g.n[0] - 1 && J != 0 && J != g.n[1] - 1 &&
+                       (DIM == 2 || (K != 0 && K != g.n[2] - 1));
+    if (DIM == 2) {
+      if (inner) {
+        double lm = 15.0 * dc * Hc;
+        for (int ii = -1; ii < 2; ++ii)
+          for (int jj = -1; jj < 2; ++jj) {
+            double H;
+            const long q = c + ii * g.s[0] + jj * g.s[1];
+            const double d = dens(q, H);
+            lm += d * H;
+          }
+        m = lm / 24.0;
+      } else {
+        m = dc * Hc;
+      }
+    } else {
+      if (inner) {
+        double lm = 50.0 * dc * Hc;
+        for (int ii = -1; ii < 2; ++ii)
+          for (int jj = -1; jj < 2; ++jj)
+            for (int kk = -1; kk < 2; ++kk) {
+              double H;
+              const long q = c + ii * g.s[0] + jj * g.s[1] + kk * g.s[2];
+              const double d = dens(q, H);
+              lm += d * H;
+            }
+        m = lm * dv / 77.0;
+      } else {
+        m = dc * Hc * dv;
+      }
+    }
+    const double dl = ls_delta(phi[c], w);
+    const double gr = grad_mag(DIM, fx[c], fy[c], DIM > 2 ? fz[c] : 0.0);
+    double gg = dl * (2.0 * (1.0 - ratio) * Hc + ratio);
+    if (!positive_ref) gg = dl * (2.0 * (ratio - 1.0) * Hc + 1.0);
+    e = gg * dl * gr;
+  }
+  m = block_sum(m);
+  e = block_sum(e);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = m;
+    part[2 * blockIdx.x + 1] = e;
+  }
+}
+
+// runge_kutta::euler with s = eta delta |grad| on every padded cell; the
+// delta of the pre-update phi is kept (dummy->delta)
+template <int DIM>
+__global__ void __launch_bounds__(kLsBlock)
+    k_mpls_update(long n, double *__restrict__ phi, const double *__restrict__ fx, const double *__restrict__ fy,
+                  const double *__restrict__ fz, double w, double eta, double dt, double *__restrict__ delta) {
+#pragma clang fp contract(off)
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const double dl = ls_delta(phi[c], w);
+  const double gr = grad_mag(DIM, fx[c], fy[c], DIM > 2 ? fz[c] : 0.0);
+  delta[c] = dl;
+  phi[c] += dt * (eta * dl * gr);
+}
+
+__global__ void __launch_bounds__(kLsBlock)
+    k_sign_delta(long n, const double *__restrict__ phi, double w, double *__restrict__ sign,
+                 double *__restrict__ delta) {
+#pragma clang fp contract(off)
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const double x = phi[c];
+  if (sign) {
+    double sg;
+    if (x < -w) sg = -1;
+    else if (x > w) sg = 1;
+    else sg = x / sqrt(x * x + w * w);
+    sign[c] = sg;
+  }
+  if (delta) delta[c] = ls_delta(x, w);
+}
+
+// weno5_for_godunov (godunov_gradient.cpp:184-201)
+__device__ __forceinline__ double weno_godunov(double a, double b, double c, double d) {
+#pragma clang fp contract(off)
+  const double eps = 1e-8;
+  const double t0 = a - b, t1 = a - 3.0 * b, t2 = b - c, t3 = b + c, t4 = c - d, t5 = 3.0 * c - d;
+  const double is0 = 13.0 * (t0 * t0) + 3.0 * (t1 * t1);
+  const double is1 = 13.0 * (t2 * t2) + 3.0 * (t3 * t3);
+  const double is2 = 13.0 * (t4 * t4) + 3.0 * (t5 * t5);
+  const double e0 = eps + is0, e1 = eps + is1, e2 = eps + is2;
+  const double alp0 = 1.0 / (e0 * e0), alp1 = 6.0 / (e1 * e1), alp2 = 3.0 / (e2 * e2);
+  const double w0 = alp0 / (alp0 + alp1 + alp2);
+  const double w2 = alp2 / (alp0 + alp1 + alp2);
+  return w0 / 3.0 * (a - 2.0 * b + c) + (w2 - 0.5) / 6.0 * (b - 2.0 * c + d);
+}
+
+// godunov_gradient (godunov_gradient.cpp:7-152) on every padded cell (0 on
+// the ghosts)
+template <int DIM>
+__global__ void __launch_bounds__(kLsBlock)
+    k_godunov(Grid g, const double *__restrict__ phi, const double *__restrict__ sign, double *__restrict__ grad) {
+#pragma clang fp contract(off)
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.total) return;
+  int q[3];
+  padded_ijk(g, c, q);
+  bool in = true;
+#pragma unroll
+  for (int a = 0; a < DIM; ++a) in = in && q[a] >= g.off[a] && q[a] < g.off[a] + g.n[a];
+  double gsum = 0.0;
+  if (in) {
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) {
+      const long s = g.s[a];
+      const double h = g.h[a];
+#define P(k) phi[c + (k) * s]
+      const double v = 1.0 / (12.0 * h) *
+                       (-(P(-1) - P(-2)) + 7.0 * (P(0) - P(-1)) + 7.0 * (P(1) - P(0)) - (P(2) - P(1)));
+      const double up = v + 1.0 / h *
+                                weno_godunov(P(3) - 2.0 * P(2) + P(1), P(2) - 2.0 * P(1) + P(0),
+                                             P(1) - 2.0 * P(0) + P(-1), P(0) - 2.0 * P(-1) + P(-2));
+      const double um = v - 1.0 / h *
+                                weno_godunov(P(-3) - 2.0 * P(-2) + P(-1), P(-2) - 2.0 * P(-1) + P(0),
+                                             P(-1) - 2.0 * P(0) + P(1), P(0) - 2.0 * P(1) + P(2));
+#undef P
+      double m;
+      if (sign[c] > 0.0) m = fmax(-fmin(up, 0.0), fmax(um, 0.0));
+      else m = fmax(fmax(up, 0.0), -fmin(um, 0.0));
+      gsum = a == 0 ? m * m : gsum + m * m;
+    }
+  }
+  grad[c] = sqrt(gsum);
+}
+
+// lsf_redistance_lambda (source.cpp:148-240) and the RK3 stage update on
+// every padded cell, in place; a and b at the diagonal neighbours are read
+// at clamped positions (zero_order_extrapolation, simple_bc.cpp:7-43)
+template <int DIM, int STAGE>
+__global__ void __launch_bounds__(kLsBlock)
+    k_redist_stage(Grid g, double *__restrict__ phi, const double *__restrict__ sign,
+                   const double *__restrict__ delta, const double *__restrict__ grad, double *__restrict__ s1,
+                   double *__restrict__ s2, double dt, double rdt) {
+#pragma clang fp contract(off)
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.total) return;
+  int q[3];
+  padded_ijk(g, c, q);
+  double s = -sign[c] * (grad[c] - 1.0) * rdt / dt;
+  bool in = true;
+#pragma unroll
+  for (int a = 0; a < DIM; ++a) in = in && q[a] >= g.off[a] && q[a] < g.off[a] + g.n[a];
+  if (in && DIM >= 2) {
+    auto AB = [&](int di, int dj, int dk, double &A, double &B) {
+      int r[3] = {q[0] + di, q[1] + dj, q[2] + dk};
+      const long p = ext_index<DIM>(g, r, -1);
+      A = sign[p] * delta[p] * (grad[p] - 1.0);
+      B = grad[p] * delta[p] * delta[p];
+    };
+    double A, B;
+    AB(0, 0, 0, A, B);
+    double a, b;
+    if (DIM == 2) {
+      a = 16.0 * A;
+      b = 16.0 * B;
+      for (int ii = -1; ii < 2; ii += 2)
+        for (int jj = -1; jj < 2; jj += 2) {
+          AB(ii, jj, 0, A, B);
+          a += A;
+          b += B;
+        }
+    } else {
+      a = 51.0 * A;
+      b = 51.0 * B;
+      for (int ii = -1; ii < 2; ii += 2)
+        for (int jj = -1; jj < 2; jj += 2)
+          for (int kk = -1; kk < 2; kk += 2) {
+            AB(ii, jj, kk, A, B);
+            a += A;
+            b += B;
+          }
+    }
+    if (fabs(b) > 1e-8) s += a / b * delta[c] * grad[c] * rdt / dt;
+  }
+  if (STAGE == 1) {
+    s1[c] = s;
+    phi[c] += dt * s;
+  } else if (STAGE == 2) {
+    s2[c] = s;
+    phi[c] += dt * (s - 3.0 * s1[c]) / 4.0;
+  } else {
+    phi[c] += dt * (-s1[c] - s2[c] + 8.0 * s) / 12.0;
+  }
+}
+
+// Coefficients of one line system, factorised: ccd_base::init_coefficients
+// (ccd_base.cpp:43-104, kind 0) and uccd_base::init_coefficients
+// (uccd_base.cpp:16-117, kind 1 upwind, 2 downwind), then twin_dec
+// (matrix_solver.cpp:21-56).  12 arrays of n.
+static std::vector<double> compact_coefficients(int kind, int n, double h) {
+#pragma clang fp contract(off)
+  std::vector<double> c(12 * (size_t)n, 0.0);
+  double *a[3] = {&c[0], &c[n], &c[2 * n]}, *b[3] = {&c[3 * n], &c[4 * n], &c[5 * n]};
+  double *aa[3] = {&c[6 * n], &c[7 * n], &c[8 * n]}, *bb[3] = {&c[9 * n], &c[10 * n], &c[11 * n]};
+  const double a1 = 0.875, b1 = 0.1251282341599089, b2 = -0.2487176584009104, b3 = 0.0001282341599089;
+  for (int i = 0; i < n; ++i) {
+    if (kind == 0) {
+      a[0][i] = 7.0 / 16.0; a[1][i] = 1.0; a[2][i] = 7.0 / 16.0;
+      b[0][i] = h / 16.0; b[1][i] = 0.0; b[2][i] = -h / 16.0;
+    } else if (kind == 1) {
+      a[0][i] = a1; a[1][i] = 1.0; a[2][i] = 0.0;
+      b[0][i] = b1 * h; b[1][i] = b2 * h; b[2][i] = b3 * h;
+    } else {
+      a[0][i] = 0.0; a[1][i] = 1.0; a[2][i] = a1;
+      b[0][i] = -b3 * h; b[1][i] = -b2 * h; b[2][i] = -b1 * h;
+    }
+    aa[0][i] = -9.0 / 8.0 / h; aa[1][i] = 0.0; aa[2][i] = 9.0 / 8.0 / h;
+    bb[0][i] = -1.0 / 8.0; bb[1][i] = 1.0; bb[2][i] = -1.0 / 8.0;
+  }
+  const double alpha_1 = 2.0, beta_1 = -1.0, alpha_2 = 5.0, beta_2 = -6.0;
+  a[0][0] = 0.0; a[1][0] = 1.0; a[2][0] = alpha_1;
+  b[0][0] = 0.0; b[1][0] = 0.0; b[2][0] = beta_1 * h;
+  aa[0][0] = 0.0; aa[1][0] = 0.0; aa[2][0] = beta_2 / h;
+  bb[0][0] = 0.0; bb[1][0] = 1.0; bb[2][0] = alpha_2;
+  a[0][n - 1] = alpha_1; a[1][n - 1] = 1.0; a[2][n - 1] = 0.0;
+  b[0][n - 1] = -beta_1 * h; b[1][n - 1] = 0.0; b[2][n - 1] = 0.0;
+  aa[0][n - 1] = -beta_2 / h; aa[1][n - 1] = 0.0; aa[2][n - 1] = 0.0;
+  bb[0][n - 1] = alpha_2; bb[1][n - 1] = 1.0; bb[2][n - 1] = 0.0;
+  // twin_dec
+  a[0][0] = 0.0; aa[0][0] = 0.0; b[0][0] = 1.0; bb[0][0] = 1.0;
+  a[2][n - 1] = 0.0; aa[2][n - 1] = 0.0; b[2][n - 1] = 1.0; bb[2][n - 1] = 1.0;
+  for (int i = 1; i < n; ++i) {
+    const double den = a[1][i - 1] * bb[1][i - 1] - aa[1][i - 1] * b[1][i - 1];
+    double sc1 = -(aa[1][i - 1] * b[0][i] - a[0][i] * bb[1][i - 1]) / den;
+    double sc2 = (b[0][i] * a[1][i - 1] - b[1][i - 1] * a[0][i]) / den;
+    a[0][i] = sc1;
+    a[1][i] = a[1][i] - (sc1 * a[2][i - 1] + sc2 * aa[2][i - 1]);
+    b[0][i] = sc2;
+    b[1][i] = b[1][i] - (sc1 * b[2][i - 1] + sc2 * bb[2][i - 1]);
+    sc1 = -(aa[1][i - 1] * bb[0][i] - aa[0][i] * bb[1][i - 1]) / den;
+    sc2 = (bb[0][i] * a[1][i - 1] - b[1][i - 1] * aa[0][i]) / den;
+    aa[0][i] = sc1;
+    aa[1][i] = aa[1][i] - (sc1 * a[2][i - 1] + sc2 * aa[2][i - 1]);
+    bb[0][i] = sc2;
+    bb[1][i] = bb[1][i] - (sc1 * b[2][i - 1] + sc2 * bb[2][i - 1]);
+  }
+  return c;
+}
+
 }  // namespace scihpc
 
 using namespace scihpc;
@@ -755,8 +1210,143 @@ struct scihpc_ls {
     }
   }
 
+  // interface maintenance state (allocated on first use): line-system
+  // coefficients per axis (CCD; UCCD upwind and downwind), line scratch,
+  // derivatives, sign, delta, gradient, reduction partials
+  int scheme = 0;  // tvd_rk3 convection: 0 WENO5, 1 UCCD
+  double *coef_ccd[3] = {nullptr, nullptr, nullptr}, *coef_up[3] = {nullptr, nullptr, nullptr};
+  double *coef_dn[3] = {nullptr, nullptr, nullptr};
+  double *ls_scr[4] = {nullptr, nullptr, nullptr, nullptr};
+  double *fxx = nullptr, *fyy = nullptr, *fzz = nullptr;
+  double *sgn = nullptr, *dlt = nullptr, *grd = nullptr, *flux = nullptr;
+  double *lpart = nullptr, *lhost = nullptr;
+
+  void ensure_iface() {
+    if (sgn) return;
+    for (int a = 0; a < g.ndim; ++a) {
+      const int n = g.N[a];
+      const std::vector<double> cc = compact_coefficients(0, n, g.h[a]);
+      const std::vector<double> cu = compact_coefficients(1, n, g.h[a]);
+      const std::vector<double> cd = compact_coefficients(2, n, g.h[a]);
+      for (double **dst : {&coef_ccd[a], &coef_up[a], &coef_dn[a]})
+        SCIHPC_HIP(hipMalloc(reinterpret_cast<void **>(dst), sizeof(double) * 12 * n));
+      SCIHPC_HIP(hipMemcpyAsync(coef_ccd[a], cc.data(), sizeof(double) * 12 * n, hipMemcpyHostToDevice, stream));
+      SCIHPC_HIP(hipMemcpyAsync(coef_up[a], cu.data(), sizeof(double) * 12 * n, hipMemcpyHostToDevice, stream));
+      SCIHPC_HIP(hipMemcpyAsync(coef_dn[a], cd.data(), sizeof(double) * 12 * n, hipMemcpyHostToDevice, stream));
+      SCIHPC_HIP(hipStreamSynchronize(stream));  // the host vectors go out of scope
+    }
+    for (auto &q : ls_scr) q = alloc();
+    fxx = alloc();
+    fyy = alloc();
+    fzz = alloc();
+    sgn = alloc();
+    dlt = alloc();
+    grd = alloc();
+    flux = alloc();
+    const long nb = (long)ceil_div(g.interior, kLsBlock);
+    SCIHPC_HIP(hipMalloc(reinterpret_cast<void **>(&lpart), sizeof(double) * 2 * nb));
+    SCIHPC_HIP(hipHostMalloc(reinterpret_cast<void **>(&lhost), sizeof(double) * 2 * nb, hipHostMallocDefault));
+  }
+
+  // derivatives of f along every real axis: UCCD by (u, v, w) or CCD
+  void compact(const double *f, bool upwind, double *d1[3], double *d2[3]) {
+    const double *vel[3] = {u, v, w};
+    for (int a = 0; a < g.ndim; ++a) {
+      const int t1 = (a + 1) % 3, t2 = (a + 2) % 3;
+      const long lines = (long)g.N[t1] * g.N[t2];
+      const unsigned nb = ceil_div(lines, 64);
+      if (upwind)
+        k_compact_line<true><<<nb, 64, 0, stream>>>(g, a, f, vel[a], coef_up[a], coef_dn[a], ls_scr[0], ls_scr[1],
+                                                    ls_scr[2], ls_scr[3], d1[a], d2 ? d2[a] : nullptr);
+      else
+        k_compact_line<false><<<nb, 64, 0, stream>>>(g, a, f, nullptr, coef_ccd[a], nullptr, ls_scr[0], ls_scr[1],
+                                                     nullptr, nullptr, d1[a], d2 ? d2[a] : nullptr);
+      SCIHPC_CHECK_LAUNCH();
+    }
+  }
+
+  // tvd_rk3 with the UCCD convection term, phi[cur] in place
+  template <int DIM>
+  void uccd_step(double dt) {
+    ensure_iface();
+    double *d1[3] = {fx, fy, fz};
+    double *f = phi[cur];
+    const unsigned nb = ceil_div(g.total, kLsBlock);
+    compact(f, true, d1, nullptr);
+    k_uccd_stage<DIM, 1><<<nb, kLsBlock, 0, stream>>>(g.total, f, u, v, w, fx, fy, fz, s1, s2, dt);
+    SCIHPC_CHECK_LAUNCH();
+    apply_bc(f, -1);
+    compact(f, true, d1, nullptr);
+    k_uccd_stage<DIM, 2><<<nb, kLsBlock, 0, stream>>>(g.total, f, u, v, w, fx, fy, fz, s1, s2, dt);
+    SCIHPC_CHECK_LAUNCH();
+    apply_bc(f, -1);
+    compact(f, true, d1, nullptr);
+    k_uccd_stage<DIM, 3><<<nb, kLsBlock, 0, stream>>>(g.total, f, u, v, w, fx, fy, fz, s1, s2, dt);
+    SCIHPC_CHECK_LAUNCH();
+    apply_bc(f, -1);
+  }
+
+  // lsf_mass and the mpls denominator of the current phi (needs fx, fy, fz
+  // of its extrapolated CCD derivatives for the denominator)
+  template <int DIM>
+  void mpls_sums(double w_, double ratio, int positive_ref, double &mass, double &den) {
+    const unsigned nb = ceil_div(g.interior, kLsBlock);
+    const double dv = DIM == 2 ? g.h[0] * g.h[1] : g.h[0] * g.h[1] * g.h[2];
+    k_mpls_sums<DIM><<<nb, kLsBlock, 0, stream>>>(g, phi[cur], fx, fy, fz, w_, ratio, positive_ref, dv, lpart);
+    SCIHPC_CHECK_LAUNCH();
+    SCIHPC_HIP(hipMemcpyAsync(lhost, lpart, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost, stream));
+    SCIHPC_HIP(hipStreamSynchronize(stream));
+    double m = 0.0, e = 0.0;
+    for (unsigned b = 0; b < nb; ++b) {
+      m += lhost[2 * b];
+      e += lhost[2 * b + 1];
+    }
+    mass = m * dv;
+    den = e;
+  }
+
+  // runge_kutta::euler with mpls (source.cpp:109-146)
+  template <int DIM>
+  void mpls_step(double dt, double w_, double ratio, int positive_ref, double mass0) {
+    ensure_iface();
+    double *f = phi[cur];
+    const unsigned nb = ceil_div(g.total, kLsBlock);
+    k_extrap<DIM><<<nb, kLsBlock, 0, stream>>>(g, f, flux);
+    SCIHPC_CHECK_LAUNCH();
+    double *d1[3] = {fx, fy, fz};
+    compact(flux, false, d1, nullptr);
+    double mass, den;
+    mpls_sums<DIM>(w_, ratio, positive_ref, mass, den);
+    const double dv = DIM == 2 ? g.h[0] * g.h[1] : g.h[0] * g.h[1] * g.h[2];
+    const double eta = (mass0 - mass) / (den * dv * dt);
+    k_mpls_update<DIM><<<nb, kLsBlock, 0, stream>>>(g.total, f, fx, fy, fz, w_, eta, dt, dlt);
+    SCIHPC_CHECK_LAUNCH();
+    apply_bc(f, -1);
+  }
+
+  // tvd_rk3 with lsf_redistance_lambda on the stored sign and delta
+  template <int DIM>
+  void redistance_step(double dt, double rdt) {
+    double *f = phi[cur];
+    const unsigned nb = ceil_div(g.total, kLsBlock);
+#define SCIHPC_REDIST(ST)                                                                                  \
+  k_godunov<DIM><<<nb, kLsBlock, 0, stream>>>(g, f, sgn, grd);                                           \
+  SCIHPC_CHECK_LAUNCH();                                                                                   \
+  k_redist_stage<DIM, ST><<<nb, kLsBlock, 0, stream>>>(g, f, sgn, dlt, grd, s1, s2, dt, rdt);            \
+  SCIHPC_CHECK_LAUNCH();                                                                                   \
+  apply_bc(f, -1)
+    SCIHPC_REDIST(1);
+    SCIHPC_REDIST(2);
+    SCIHPC_REDIST(3);
+#undef SCIHPC_REDIST
+  }
+
   template <int DIM>
   void rk3_step(double dt) {
+    if (scheme == 1) {
+      uccd_step<DIM>(dt);
+      return;
+    }
     const unsigned nb = grid_interior();
     double *a = phi[cur], *b = phi[cur ^ 1];
     k_rk3_stage<DIM, 1><<<nb, kLsBlock, 0, stream>>>(g, a, b, u, v, w, s1, s2, fx, fy, fz, dt);
@@ -827,8 +1417,15 @@ extern "C" int scihpc_ls_create(int ndim, int nx, int ny, int nz, double dx, dou
 extern "C" int scihpc_ls_destroy(scihpc_ls *h) {
   if (!h) return SCIHPC_OK;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
-  for (double *p : {h->phi[0], h->phi[1], h->u, h->v, h->w, h->fx, h->fy, h->fz, h->s1, h->s2, h->tmp})
+  for (double *p : {h->phi[0], h->phi[1], h->u, h->v, h->w, h->fx, h->fy, h->fz, h->s1, h->s2, h->tmp, h->fxx,
+                    h->fyy, h->fzz, h->sgn, h->dlt, h->grd, h->flux, h->lpart})
     if (p) (void)hipFree(p);
+  for (int a = 0; a < 3; ++a)
+    for (double *p : {h->coef_ccd[a], h->coef_up[a], h->coef_dn[a]})
+      if (p) (void)hipFree(p);
+  for (double *p : h->ls_scr)
+    if (p) (void)hipFree(p);
+  if (h->lhost) (void)hipHostFree(h->lhost);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -905,6 +1502,118 @@ extern "C" int scihpc_ls_tvd_rk3(scihpc_ls *h, double dt, int nsteps) {
     ls_check(h);
     SCIHPC_REQUIRE(nsteps >= 0, "nsteps must be >= 0");
     for (int s = 0; s < nsteps; ++s) h->step(dt);
+    SCIHPC_HIP(hipStreamSynchronize(h->stream));
+  });
+}
+
+extern "C" int scihpc_ls_set_scheme(scihpc_ls *h, int scheme) {
+  return guarded([&] {
+    ls_check(h);
+    SCIHPC_REQUIRE(scheme == SCIHPC_LS_WENO5 || scheme == SCIHPC_LS_UCCD, "unknown scheme");
+    h->scheme = scheme;
+  });
+}
+
+extern "C" int scihpc_ls_compact_derivatives(scihpc_ls *h, const double *f, int upwind, double *fx, double *fy,
+                                             double *fz, double *fxx, double *fyy, double *fzz) {
+  return guarded([&] {
+    ls_check(h);
+    SCIHPC_REQUIRE(f && fx && fxx, "null argument");
+    SCIHPC_REQUIRE(h->g.ndim < 2 || (fy && fyy), "fy/fyy required for ndim >= 2");
+    SCIHPC_REQUIRE(h->g.ndim < 3 || (fz && fzz), "fz/fzz required for ndim 3");
+    h->ensure_iface();
+    h2d(h, h->tmp, f);
+    double *d1[3] = {h->fx, h->fy, h->fz}, *d2[3] = {h->fxx, h->fyy, h->fzz};
+    h->compact(h->tmp, upwind != 0, d1, d2);
+    double *o1[3] = {fx, fy, fz}, *o2[3] = {fxx, fyy, fzz};
+    for (int a = 0; a < h->g.ndim; ++a) {
+      d2h(h, o1[a], d1[a]);
+      d2h(h, o2[a], d2[a]);
+    }
+    SCIHPC_HIP(hipStreamSynchronize(h->stream));
+  });
+}
+
+extern "C" int scihpc_ls_mass(scihpc_ls *h, double ls_width, double ratio, int positive_ref, double *mass) {
+  return guarded([&] {
+    ls_check(h);
+    SCIHPC_REQUIRE(mass && ls_width > 0, "bad argument");
+    SCIHPC_REQUIRE(h->g.ndim >= 2, "lsf_mass is defined for 2-D and 3-D");
+    h->ensure_iface();
+    double den;
+    if (h->g.ndim == 2) h->mpls_sums<2>(ls_width, ratio, positive_ref, *mass, den);
+    else h->mpls_sums<3>(ls_width, ratio, positive_ref, *mass, den);
+  });
+}
+
+extern "C" int scihpc_ls_mpls(scihpc_ls *h, double dt, double ls_width, double ratio, int positive_ref,
+                              double mass0, int count) {
+  return guarded([&] {
+    ls_check(h);
+    SCIHPC_REQUIRE(dt > 0 && ls_width > 0 && count >= 0, "bad argument");
+    SCIHPC_REQUIRE(h->g.ndim >= 2, "mpls is defined for 2-D and 3-D");
+    for (int it = 0; it < count; ++it) {
+      if (h->g.ndim == 2) h->mpls_step<2>(dt, ls_width, ratio, positive_ref, mass0);
+      else h->mpls_step<3>(dt, ls_width, ratio, positive_ref, mass0);
+    }
+    SCIHPC_HIP(hipStreamSynchronize(h->stream));
+  });
+}
+
+extern "C" int scihpc_ls_find_sign(scihpc_ls *h, double ls_width) {
+  return guarded([&] {
+    ls_check(h);
+    SCIHPC_REQUIRE(ls_width > 0, "ls_width must be positive");
+    h->ensure_iface();
+    k_sign_delta<<<ceil_div(h->g.total, kLsBlock), kLsBlock, 0, h->stream>>>(h->g.total, h->phi[h->cur], ls_width,
+                                                                             h->sgn, nullptr);
+    SCIHPC_CHECK_LAUNCH();
+    SCIHPC_HIP(hipStreamSynchronize(h->stream));
+  });
+}
+
+extern "C" int scihpc_ls_find_delta(scihpc_ls *h, double ls_width) {
+  return guarded([&] {
+    ls_check(h);
+    SCIHPC_REQUIRE(ls_width > 0, "ls_width must be positive");
+    h->ensure_iface();
+    k_sign_delta<<<ceil_div(h->g.total, kLsBlock), kLsBlock, 0, h->stream>>>(h->g.total, h->phi[h->cur], ls_width,
+                                                                             nullptr, h->dlt);
+    SCIHPC_CHECK_LAUNCH();
+    SCIHPC_HIP(hipStreamSynchronize(h->stream));
+  });
+}
+
+extern "C" int scihpc_ls_set_field(scihpc_ls *h, int which, const double *src) {
+  return guarded([&] {
+    ls_check(h);
+    SCIHPC_REQUIRE(src && (which == SCIHPC_LS_SIGN || which == SCIHPC_LS_DELTA), "bad argument");
+    h->ensure_iface();
+    h2d(h, which == SCIHPC_LS_SIGN ? h->sgn : h->dlt, src);
+    SCIHPC_HIP(hipStreamSynchronize(h->stream));
+  });
+}
+
+extern "C" int scihpc_ls_get_field(scihpc_ls *h, int which, double *dst) {
+  return guarded([&] {
+    ls_check(h);
+    SCIHPC_REQUIRE(dst && (which == SCIHPC_LS_SIGN || which == SCIHPC_LS_DELTA), "bad argument");
+    h->ensure_iface();
+    d2h(h, dst, which == SCIHPC_LS_SIGN ? h->sgn : h->dlt);
+    SCIHPC_HIP(hipStreamSynchronize(h->stream));
+  });
+}
+
+extern "C" int scihpc_ls_redistance(scihpc_ls *h, double dt, double rdt, int nsteps) {
+  return guarded([&] {
+    ls_check(h);
+    SCIHPC_REQUIRE(dt > 0 && rdt > 0 && nsteps >= 0, "bad argument");
+    h->ensure_iface();
+    for (int it = 0; it < nsteps; ++it) {
+      if (h->g.ndim == 1) h->redistance_step<1>(dt, rdt);
+      else if (h->g.ndim == 2) h->redistance_step<2>(dt, rdt);
+      else h->redistance_step<3>(dt, rdt);
+    }
     SCIHPC_HIP(hipStreamSynchronize(h->stream));
   });
 }
