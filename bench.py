@@ -259,6 +259,73 @@ def cpu_momentum_baseline():
             "sample": f"one predictor step at {nx}x{ny}"}
 
 
+def bench_interface(reps=5):
+    """Interface maintenance at the dam-break shape (1024 x 512): one UCCD
+    TVD-RK3 step, the 5 mpls Euler steps of a time step and one
+    redistancing RK3 step, device-resident phi."""
+    import numpy as np
+
+    from scihpc_amd import LevelSet
+
+    wl = momentum_workload()
+    nx, ny = wl["shape"]
+    h = wl["h"]
+    ls = LevelSet(2, nx, ny, 1, h, h, 1.0, ["neumann"] * 6)
+    ls.set_phi(wl["phi"])
+    ls.set_velocity(wl["u"], wl["v"])
+    ls.set_scheme("uccd")
+    dt, lw, ratio = 0.01 * h, 1.5 * h, 1.226e-3
+    ms_rk3, _ = ls.time_step(dt, iters=reps)
+    m0 = ls.mass(lw, ratio)
+    ls.mpls(dt, lw, ratio, m0, count=1)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ls.mpls(dt, lw, ratio, m0, count=5)
+    ms_mpls = (time.perf_counter() - t0) / reps * 1e3
+    ls.find_sign(lw)
+    ls.find_delta(lw)
+    ls.redistance(dt, 0.25 * h, 1)
+    t0 = time.perf_counter()
+    ls.redistance(dt, 0.25 * h, reps)
+    ms_red = (time.perf_counter() - t0) / reps * 1e3
+    ls.close()
+    return {"workload": f"dam break {nx}x{ny} level set", "uccd_rk3_step_ms": round(ms_rk3, 4),
+            "mpls_5_steps_ms": round(ms_mpls, 4), "redistance_rk3_step_ms": round(ms_red, 4)}
+
+
+def cpu_interface_baseline():
+    """The oracle's UCCD RK3 step, one mpls step and one redistancing step
+    (one core) at the same shape."""
+    import numpy as np
+
+    from oracle.pyoracle import Oracle
+
+    wl = momentum_workload()
+    nx, ny = wl["shape"]
+    h = wl["h"]
+    o = Oracle()
+    T = wl["phi"].size
+    types, vals = np.zeros(6, np.int32), np.zeros(6)
+    dt, lw, ratio = 0.01 * h, 1.5 * h, 1.226e-3
+    phi = wl["phi"].copy()
+    t0 = time.perf_counter()
+    o.lib.orc_tvd_rk3_uccd(2, nx, ny, 1, h, h, 1.0, dt, types, vals, phi, wl["u"], wl["v"], wl["w"], 1)
+    t_rk3 = time.perf_counter() - t0
+    m0 = o.lib.orc_lsf_mass(2, nx, ny, 1, h * h, lw, ratio, 1, phi)
+    delta = np.zeros(T)
+    t0 = time.perf_counter()
+    o.lib.orc_mpls_euler(2, nx, ny, 1, h, h, 1.0, dt, lw, ratio, 1, m0, types, vals, phi, 1, delta.ctypes.data)
+    t_mpls = time.perf_counter() - t0
+    sign = np.zeros(T)
+    o.lib.orc_find_sign(T, phi, lw, sign)
+    t0 = time.perf_counter()
+    o.lib.orc_redistance_rk3(2, nx, ny, 1, h, h, 1.0, dt, 0.25 * h, sign, delta, types, vals, phi, 1)
+    t_red = time.perf_counter() - t0
+    return {"uccd_rk3_step_ms": round(t_rk3 * 1e3, 2), "mpls_5_steps_ms": round(5 * t_mpls * 1e3, 2),
+            "redistance_rk3_step_ms": round(t_red * 1e3, 2), "cores": 1, "kind": "port",
+            "sample": f"one step each at {nx}x{ny} (mpls: one step, x5)"}
+
+
 def cpu_snapshot_baseline(snap):
     """The oracle's C writer (one core) on the same fields: ms per file and
     whether its bytes equal the GPU writer's file."""
@@ -290,6 +357,7 @@ def main():
     ap.add_argument("--no-unet", action="store_true", help="skip the U-Net training-step timing")
     ap.add_argument("--no-snapshot", action="store_true", help="skip the VTK snapshot timing")
     ap.add_argument("--no-momentum", action="store_true", help="skip the momentum predictor timing")
+    ap.add_argument("--no-interface", action="store_true", help="skip the interface-maintenance timing")
     ap.add_argument("--kernel-only", action="store_true",
                     help="only launch the dominant kernel (and a 256 MiB copy for calibration): "
                          "the command profiled with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE")
@@ -429,6 +497,13 @@ def main():
         momentum = bench_momentum()
         log(f"bench: momentum {momentum['ms_per_step']} ms/step, {momentum['GBps']} GB/s")
 
+    interface = None
+    if not args.no_interface:
+        log("bench: interface maintenance at the 1024x512 dam-break shape")
+        interface = bench_interface()
+        log(f"bench: uccd rk3 {interface['uccd_rk3_step_ms']} ms, mpls x5 {interface['mpls_5_steps_ms']} ms, "
+            f"redistance {interface['redistance_rk3_step_ms']} ms")
+
     snapshot = None
     if not args.no_snapshot:
         log("bench: VTK snapshot of the 512x1024 grid")
@@ -472,6 +547,7 @@ def main():
                      "bytes_per_launch": kbytes, "us_per_launch": round(kms * 1e3, 3)},
         "unet": unet,
         "momentum": momentum,
+        "interface": interface,
         "snapshot": snapshot,
     }
     if not args.no_3d:
@@ -483,6 +559,8 @@ def main():
             out["cpu_baseline"]["snapshot_writer"] = cpu_snapshot_baseline(snapshot)
         if momentum is not None:
             out["cpu_baseline"]["momentum"] = cpu_momentum_baseline()
+        if interface is not None:
+            out["cpu_baseline"]["interface"] = cpu_interface_baseline()
     if snapshot is not None:
         snapshot.pop("_fields")
         snapshot.pop("_file")

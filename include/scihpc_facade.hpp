@@ -519,14 +519,49 @@ class weno_solver : public level_set_solver {
 };
 
 // runge_kutta.h:12-23: TVD-RK3 of d phi/dt = -u . grad(phi) with WENO5
-// derivatives of the identity flux and the solver's BCs after every stage
+// (default) or UCCD (set_scheme(SCIHPC_LS_UCCD): the reference's
+// `convection`) derivatives of the identity flux and the solver's BCs after
+// every stage; plus the reference's other right-hand sides run through
+// runge_kutta: euler with mpls (mass correction) and tvd_rk3 with
+// lsf_redistance_lambda (redistancing).  Each call copies phi in and out.
 class runge_kutta : public level_set_solver {
  public:
   using level_set_solver::level_set_solver;
+  void set_scheme(int scheme) { check(scihpc_ls_set_scheme(h_, scheme)); }
   void tvd_rk3(vector_data *vel, DataType dt, int steps = 1) {
     set_velocity(vel);
     check(scihpc_ls_set_phi(h_, &f_->data[0][0][0]));
     check(scihpc_ls_tvd_rk3(h_, dt, steps));
+    check(scihpc_ls_get_phi(h_, &f_->data[0][0][0]));
+  }
+  // lsf_mass (wrapper_func.cpp:450-511)
+  DataType lsf_mass(DataType ls_width, DataType density_ratio, bool positive_ref = true) {
+    double m = 0.0;
+    check(scihpc_ls_set_phi(h_, &f_->data[0][0][0]));
+    check(scihpc_ls_mass(h_, ls_width, density_ratio, positive_ref ? 1 : 0, &m));
+    return m;
+  }
+  // `count` x euler(phi, nvel, identity_flux, mpls) (source.cpp:109-146)
+  void euler_mpls(DataType dt, DataType ls_width, DataType density_ratio, DataType mass0, int count = 1,
+                  bool positive_ref = true) {
+    check(scihpc_ls_set_phi(h_, &f_->data[0][0][0]));
+    check(scihpc_ls_mpls(h_, dt, ls_width, density_ratio, positive_ref ? 1 : 0, mass0, count));
+    check(scihpc_ls_get_phi(h_, &f_->data[0][0][0]));
+  }
+  // find_sign / find_delta (wrapper_func.cpp:20-40) of the current phi
+  void find_sign(DataType ls_width) {
+    check(scihpc_ls_set_phi(h_, &f_->data[0][0][0]));
+    check(scihpc_ls_find_sign(h_, ls_width));
+  }
+  void find_delta(DataType ls_width) {
+    check(scihpc_ls_set_phi(h_, &f_->data[0][0][0]));
+    check(scihpc_ls_find_delta(h_, ls_width));
+  }
+  // `steps` x tvd_rk3(phi, nvel, identity_flux, lsf_redistance_lambda)
+  // with the stored sign and delta (source.cpp:148-240)
+  void tvd_rk3_redistance(DataType dt, DataType rdt, int steps = 1) {
+    check(scihpc_ls_set_phi(h_, &f_->data[0][0][0]));
+    check(scihpc_ls_redistance(h_, dt, rdt, steps));
     check(scihpc_ls_get_phi(h_, &f_->data[0][0][0]));
   }
 };

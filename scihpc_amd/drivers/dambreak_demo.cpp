@@ -4,13 +4,16 @@
 // predictor, the pressure step (projection() for the first 5 steps, then the
 // FPC step) and node velocities, with legacy-VTK snapshots.
 //
-// Differences from the reference driver, all outside this library's scope:
-// the level set starts as the exact signed distance to the water column
-// (the reference redistances a +-1 indicator first), it is advected with
-// the WENO5 convection term (the reference's UCCD scheme is not built), and
-// the mpls mass correction and periodic redistancing are not applied.
+// As the reference driver: the level set is advected with the UCCD
+// convection term, corrected by 5 mpls Euler steps per time step, and
+// redistanced every 10 steps (lsf_redistance_lambda with the sign found at
+// the start, as the reference's loop leaves it).  One difference: the level
+// set starts as the exact signed distance to the water column (the
+// reference redistances a +-1 indicator first).
 //
-//   dambreak_demo <n> <steps> [vtk prefix]     (n x n cells on [0,5]^2)
+//   dambreak_demo <n> <steps> [vtk prefix] [plain]   (n x n cells on [0,5]^2)
+// `plain` (any 4th argument) turns the interface maintenance off: WENO5
+// convection only.
 // prints key/value lines: the time, the water volume, the largest
 // velocity, the divergence after the pressure step and the front position.
 #include <cmath>
@@ -24,7 +27,8 @@ using namespace scihpc_gpu;
 int main(int argc, char **argv) {
   const int n = argc > 1 ? std::atoi(argv[1]) : 64;
   const int steps = argc > 2 ? std::atoi(argv[2]) : 100;
-  const char *prefix = argc > 3 ? argv[3] : nullptr;
+  const char *prefix = argc > 3 && argv[3][0] ? argv[3] : nullptr;
+  const bool maintain = argc <= 4;
 
   structured_grid geo(axis{0.0, 5.0, n}, axis{0.0, 5.0, n});
   scalar_data phi(n, n), pressure(n, n), density(n, n);
@@ -59,6 +63,8 @@ int main(int argc, char **argv) {
   prm.dt = prm.dt_old = prm.dt_old2 = dt;
 
   runge_kutta rk(&phi, &geo, neumann);
+  if (maintain) rk.set_scheme(SCIHPC_LS_UCCD);
+  const double rdt = 0.25 * geo.h;  // param->rdt of the driver
   projection_method flow(&pressure, &geo, walls);
   multigrid mg_const(&pressure, &geo);  // FPC: constant-coefficient Neumann hierarchy
   multigrid mg_rho(&pressure, &geo);    // projection(): A(rho)
@@ -104,6 +110,8 @@ int main(int argc, char **argv) {
   };
 
   const double v0 = volume(), f0 = front();
+  const double mass0 = maintain ? rk.lsf_mass(prm.ls_width, prm.density_ratio) : 0.0;
+  if (maintain) rk.find_sign(prm.ls_width);
   std::printf("volume0 %.6e\n", v0);
   std::printf("front0 %.6e\n", f0);
   if (prefix) {
@@ -119,8 +127,19 @@ int main(int argc, char **argv) {
     t += dt;
     prm.iter = s;
     rk.tvd_rk3(&nvel, dt, 1);
+    if (maintain) rk.euler_mpls(dt, prm.ls_width, prm.density_ratio, mass0, 5);
     cycles = flow.ab_solve_sec(mg_const, mg_rho, &vel, &nvel, &pressure, &phi, &density, prm);
+    if (maintain && s % 10 == 0) {
+      // the reference's redistancing loop (benchmarks_2d_dambreak.cpp:125-131):
+      // delta as find_source_sec left it, the initial sign
+      rk.find_delta(prm.ls_width);
+      int instep = 0;
+      int n_re = 0;
+      while (++instep * rdt < 5.0 * prm.ls_width) ++n_re;
+      rk.tvd_rk3_redistance(dt, rdt, n_re);
+    }
   }
+  if (maintain) std::printf("mass_ratio %.6e\n", rk.lsf_mass(prm.ls_width, prm.density_ratio) / mass0);
   std::printf("time %.6e\n", t);
   std::printf("steps %d\n", steps);
   std::printf("last_pressure_cycles %d\n", cycles);
