@@ -776,47 +776,106 @@ __device__ __forceinline__ void compact_rhs(const double *__restrict__ f, long s
 #undef F
 }
 
-// twin_bks of one line: rhs formed on the fly in the forward sweep; S/SS
-// scratch (same layout as the fields); the solution is left in S/SS
+// twin_bks of one line: rhs formed on the fly in the forward sweep;
+// S/SS scratch (same layout as the fields); the solution is left in S/SS.
+// Both sweeps work in chunks of kLineChunk positions: the chunk's field
+// values and coefficients are loaded first, then the recurrence runs on
+// registers, so the loads of a chunk overlap instead of serialising with
+// the recurrence.  Each position's arithmetic is the reference's, in its
+// order.
+constexpr int kLineChunk = 8;
+
 template <int KIND>
 __device__ __forceinline__ void compact_solve(const double *__restrict__ f, const double *__restrict__ c, long st,
                                               int n, double h, double *__restrict__ S, double *__restrict__ SS) {
 #pragma clang fp contract(off)
+  constexpr int B = kLineChunk;
   const double *a0 = c, *a1 = c + n, *a2 = c + 2 * n, *b0 = c + 3 * n, *b1 = c + 4 * n, *b2 = c + 5 * n;
   const double *aa0 = c + 6 * n, *aa1 = c + 7 * n, *aa2 = c + 8 * n, *bb0 = c + 9 * n, *bb1 = c + 10 * n,
                *bb2 = c + 11 * n;
+  const double c3 = -0.06096119008109, c2 = 1.99692238016218, c1 = -1.93596119008109;
   double sp, ssp;
   compact_rhs<KIND == 2 ? 1 : KIND>(f, st, n, h, 0, sp, ssp);
   S[0] = sp;
   SS[0] = ssp;
-  for (int i = 1; i < n; ++i) {
-    double si, ssi;
-    if (KIND == 2 && i == n - 1) compact_rhs<1>(f, st, n, h, i, si, ssi);
-    else compact_rhs<KIND>(f, st, n, h, i, si, ssi);
-    si = si - (a0[i] * sp + b0[i] * ssp);
-    ssi = ssi - (aa0[i] * sp + bb0[i] * ssp);
-    S[(long)i * st] = si;
-    SS[(long)i * st] = ssi;
-    sp = si;
-    ssp = ssi;
+  for (int i0 = 1; i0 < n; i0 += B) {
+    double fw[B + 2];
+#pragma unroll
+    for (int q = 0; q < B + 2; ++q) {
+      const int i = i0 - 1 + q;
+      fw[q] = i < n ? f[(long)i * st] : 0.0;
+    }
+    double r[B], rr[B], A0[B], B0[B], AA0[B], BB0[B];
+#pragma unroll
+    for (int q = 0; q < B; ++q) {
+      const int i = i0 + q;
+      if (i < n - 1) {
+        const double fm = fw[q], f0 = fw[q + 1], fp = fw[q + 2];
+        if (KIND == 0) r[q] = 15.0 / 16.0 * (fp - fm) / h;
+        else if (KIND == 1) r[q] = (c1 * fm + c2 * f0 + c3 * fp) / h;
+        else r[q] = -(c3 * fm + c2 * f0 + c1 * fp) / h;
+        rr[q] = (3.0 * fm - 6.0 * f0 + 3.0 * fp) / h / h;
+      } else if (i == n - 1) {
+        compact_rhs<1>(f, st, n, h, i, r[q], rr[q]);  // the one-sided end (same for every kind)
+      }
+      if (i < n) {
+        A0[q] = a0[i];
+        B0[q] = b0[i];
+        AA0[q] = aa0[i];
+        BB0[q] = bb0[i];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < B; ++q) {
+      const int i = i0 + q;
+      if (i < n) {
+        const double si = r[q] - (A0[q] * sp + B0[q] * ssp);
+        const double ssi = rr[q] - (AA0[q] * sp + BB0[q] * ssp);
+        S[(long)i * st] = si;
+        SS[(long)i * st] = ssi;
+        sp = si;
+        ssp = ssi;
+      }
+    }
   }
   double den = a1[n - 1] * bb1[n - 1] - aa1[n - 1] * b1[n - 1];
-  double sols = -(b1[n - 1] * ssp - bb1[n - 1] * sp) / den;
-  double solss = (a1[n - 1] * ssp - aa1[n - 1] * sp) / den;
-  S[(long)(n - 1) * st] = sols;
-  SS[(long)(n - 1) * st] = solss;
-  double sn = sols, ssn = solss;
-  for (int i = n - 2; i > -1; --i) {
-    double si = S[(long)i * st], ssi = SS[(long)i * st];
-    si = si - (a2[i] * sn + b2[i] * ssn);
-    ssi = ssi - (aa2[i] * sn + bb2[i] * ssn);
-    den = a1[i] * bb1[i] - aa1[i] * b1[i];
-    sols = -(b1[i] * ssi - bb1[i] * si) / den;
-    solss = (a1[i] * ssi - aa1[i] * si) / den;
-    S[(long)i * st] = sols;
-    SS[(long)i * st] = solss;
-    sn = sols;
-    ssn = solss;
+  double sn = -(b1[n - 1] * ssp - bb1[n - 1] * sp) / den;
+  double ssn = (a1[n - 1] * ssp - aa1[n - 1] * sp) / den;
+  S[(long)(n - 1) * st] = sn;
+  SS[(long)(n - 1) * st] = ssn;
+  for (int i0 = n - 2; i0 >= 0; i0 -= B) {
+    double Sv[B], SSv[B], A1[B], B1[B], AA1[B], BB1[B], A2[B], B2[B], AA2[B], BB2[B];
+#pragma unroll
+    for (int q = 0; q < B; ++q) {
+      const int i = i0 - q;
+      if (i >= 0) {
+        Sv[q] = S[(long)i * st];
+        SSv[q] = SS[(long)i * st];
+        A1[q] = a1[i];
+        B1[q] = b1[i];
+        AA1[q] = aa1[i];
+        BB1[q] = bb1[i];
+        A2[q] = a2[i];
+        B2[q] = b2[i];
+        AA2[q] = aa2[i];
+        BB2[q] = bb2[i];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < B; ++q) {
+      const int i = i0 - q;
+      if (i >= 0) {
+        const double si = Sv[q] - (A2[q] * sn + B2[q] * ssn);
+        const double ssi = SSv[q] - (AA2[q] * sn + BB2[q] * ssn);
+        den = A1[q] * BB1[q] - AA1[q] * B1[q];
+        const double sols = -(B1[q] * ssi - BB1[q] * si) / den;
+        const double solss = (A1[q] * ssi - AA1[q] * si) / den;
+        S[(long)i * st] = sols;
+        SS[(long)i * st] = solss;
+        sn = sols;
+        ssn = solss;
+      }
+    }
   }
 }
 
@@ -837,6 +896,7 @@ __global__ void __launch_bounds__(kLsBlock)
   if (UP) {
     compact_solve<1>(f + base, cu, st, n, h, su + base, ssu + base);
     compact_solve<2>(f + base, cd, st, n, h, sd + base, ssd + base);
+#pragma unroll 8
     for (int i = 0; i < n; ++i) {
       const long c = base + (long)i * st;
       d1[c] = vel[c] > 0 ? su[c] : sd[c];
@@ -844,6 +904,7 @@ __global__ void __launch_bounds__(kLsBlock)
     }
   } else {
     compact_solve<0>(f + base, cu, st, n, h, su + base, ssu + base);
+#pragma unroll 8
     for (int i = 0; i < n; ++i) {
       const long c = base + (long)i * st;
       d1[c] = su[c];
