@@ -358,6 +358,9 @@ def main():
     ap.add_argument("--no-snapshot", action="store_true", help="skip the VTK snapshot timing")
     ap.add_argument("--no-momentum", action="store_true", help="skip the momentum predictor timing")
     ap.add_argument("--no-interface", action="store_true", help="skip the interface-maintenance timing")
+    ap.add_argument("--only-rows", action="store_true",
+                    help="run only the rows around the solve (momentum predictor, interface maintenance, "
+                         "VTK snapshot) and print their JSON (profiling)")
     ap.add_argument("--kernel-only", action="store_true",
                     help="only launch the dominant kernel (and a 256 MiB copy for calibration): "
                          "the command profiled with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE")
@@ -381,6 +384,12 @@ def main():
         return
     if args.only_3d:
         print(json.dumps(bench_3d(args)), flush=True)
+        return
+    if args.only_rows:
+        rows = {"momentum": bench_momentum(), "interface": bench_interface(), "snapshot": bench_snapshot(torch)}
+        rows["snapshot"].pop("_fields")
+        rows["snapshot"].pop("_file")
+        print(json.dumps(rows), flush=True)
         return
     log("bench: building the 512x1024 bubble workload")
     w = W.bubble_2d()
