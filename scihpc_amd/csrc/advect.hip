@@ -776,49 +776,74 @@ __device__ __forceinline__ void compact_rhs(const double *__restrict__ f, long s
 #undef F
 }
 
-// twin_bks of one line: rhs formed on the fly in the forward sweep;
-// S/SS scratch (same layout as the fields); the solution is left in S/SS.
-// Both sweeps work in chunks of kLineChunk positions: the chunk's field
-// values and coefficients are loaded first, then the recurrence runs on
-// registers, so the loads of a chunk overlap instead of serialising with
-// the recurrence.  Each position's arithmetic is the reference's, in its
-// order.
+// the twin sweeps load a chunk of kLineChunk positions (right-hand sides
+// and coefficients) before running the recurrence on it, so the loads of a
+// chunk overlap instead of serialising with the recurrence
 constexpr int kLineChunk = 8;
 
-template <int KIND>
-__device__ __forceinline__ void compact_solve(const double *__restrict__ f, const double *__restrict__ c, long st,
-                                              int n, double h, double *__restrict__ S, double *__restrict__ SS) {
+// The line systems in three passes: the right-hand sides of every padded
+// cell (all divisions of the scheme's stencils, fully parallel), the twin
+// back substitution in place, one thread per (line, system) so UCCD's
+// upwind and downwind systems run side by side, and the selection of the
+// outputs (parallel again).  Same operations per value as the reference.
+template <bool UP>
+__global__ void __launch_bounds__(kLsBlock)
+    k_compact_rhs(Grid g, int a, const double *__restrict__ f, double *__restrict__ su, double *__restrict__ ssu,
+                  double *__restrict__ sd, double *__restrict__ ssd) {
+#pragma clang fp contract(off)
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.total) return;
+  int q[3];
+  padded_ijk(g, c, q);
+  const int i = q[a], n = g.N[a];
+  const long st = g.s[a];
+  const double *line = f + (c - (long)i * st);
+  const double h = g.h[a];
+  double s, ss;
+  if (UP) {
+    compact_rhs<1>(line, st, n, h, i, s, ss);
+    su[c] = s;
+    ssu[c] = ss;
+    if (i > 0 && i < n - 1) compact_rhs<2>(line, st, n, h, i, s, ss);  // the ends are the upwind ones
+    sd[c] = s;
+    ssd[c] = ss;
+  } else {
+    compact_rhs<0>(line, st, n, h, i, s, ss);
+    su[c] = s;
+    ssu[c] = ss;
+  }
+}
+
+// twin_bks (matrix_solver.cpp:58-83) in place on S/SS holding the
+// right-hand sides; thread t: line t % lines, system t / lines
+__global__ void __launch_bounds__(64)
+    k_twin_line(Grid g, int a, int nsys, const double *__restrict__ cu, const double *__restrict__ cd,
+                double *__restrict__ su, double *__restrict__ ssu, double *__restrict__ sd, double *__restrict__ ssd) {
 #pragma clang fp contract(off)
   constexpr int B = kLineChunk;
+  const int t1 = (a + 1) % 3, t2 = (a + 2) % 3;
+  const long lines = (long)g.N[t1] * g.N[t2];
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= lines * nsys) return;
+  const int sys = (int)(t / lines);
+  long base;
+  line_base(g, a, t % lines, base);
+  const int n = g.N[a];
+  const long st = g.s[a];
+  const double *c = sys == 0 ? cu : cd;
+  double *S = (sys == 0 ? su : sd) + base, *SS = (sys == 0 ? ssu : ssd) + base;
   const double *a0 = c, *a1 = c + n, *a2 = c + 2 * n, *b0 = c + 3 * n, *b1 = c + 4 * n, *b2 = c + 5 * n;
   const double *aa0 = c + 6 * n, *aa1 = c + 7 * n, *aa2 = c + 8 * n, *bb0 = c + 9 * n, *bb1 = c + 10 * n,
                *bb2 = c + 11 * n;
-  const double c3 = -0.06096119008109, c2 = 1.99692238016218, c1 = -1.93596119008109;
-  double sp, ssp;
-  compact_rhs<KIND == 2 ? 1 : KIND>(f, st, n, h, 0, sp, ssp);
-  S[0] = sp;
-  SS[0] = ssp;
+  double sp = S[0], ssp = SS[0];
   for (int i0 = 1; i0 < n; i0 += B) {
-    double fw[B + 2];
-#pragma unroll
-    for (int q = 0; q < B + 2; ++q) {
-      const int i = i0 - 1 + q;
-      fw[q] = i < n ? f[(long)i * st] : 0.0;
-    }
     double r[B], rr[B], A0[B], B0[B], AA0[B], BB0[B];
 #pragma unroll
     for (int q = 0; q < B; ++q) {
       const int i = i0 + q;
-      if (i < n - 1) {
-        const double fm = fw[q], f0 = fw[q + 1], fp = fw[q + 2];
-        if (KIND == 0) r[q] = 15.0 / 16.0 * (fp - fm) / h;
-        else if (KIND == 1) r[q] = (c1 * fm + c2 * f0 + c3 * fp) / h;
-        else r[q] = -(c3 * fm + c2 * f0 + c1 * fp) / h;
-        rr[q] = (3.0 * fm - 6.0 * f0 + 3.0 * fp) / h / h;
-      } else if (i == n - 1) {
-        compact_rhs<1>(f, st, n, h, i, r[q], rr[q]);  // the one-sided end (same for every kind)
-      }
       if (i < n) {
+        r[q] = S[(long)i * st];
+        rr[q] = SS[(long)i * st];
         A0[q] = a0[i];
         B0[q] = b0[i];
         AA0[q] = aa0[i];
@@ -879,37 +904,20 @@ __device__ __forceinline__ void compact_solve(const double *__restrict__ f, cons
   }
 }
 
-// d1 (and d2 when not null) along axis a of every padded line of f; UP:
-// UCCD picked by vel > 0, d2 the mean of both second derivatives; else CCD
 template <bool UP>
 __global__ void __launch_bounds__(kLsBlock)
-    k_compact_line(Grid g, int a, const double *__restrict__ f, const double *__restrict__ vel,
-                   const double *__restrict__ cu, const double *__restrict__ cd, double *__restrict__ su,
-                   double *__restrict__ ssu, double *__restrict__ sd, double *__restrict__ ssd,
-                   double *__restrict__ d1, double *__restrict__ d2) {
+    k_compact_select(long total, const double *__restrict__ vel, const double *__restrict__ su,
+                     const double *__restrict__ ssu, const double *__restrict__ sd, const double *__restrict__ ssd,
+                     double *__restrict__ d1, double *__restrict__ d2) {
 #pragma clang fp contract(off)
-  long base;
-  if (!line_base(g, a, (long)blockIdx.x * blockDim.x + threadIdx.x, base)) return;
-  const int n = g.N[a];
-  const long st = g.s[a];
-  const double h = g.h[a];
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= total) return;
   if (UP) {
-    compact_solve<1>(f + base, cu, st, n, h, su + base, ssu + base);
-    compact_solve<2>(f + base, cd, st, n, h, sd + base, ssd + base);
-#pragma unroll 8
-    for (int i = 0; i < n; ++i) {
-      const long c = base + (long)i * st;
-      d1[c] = vel[c] > 0 ? su[c] : sd[c];
-      if (d2) d2[c] = 0.5 * (ssu[c] + ssd[c]);
-    }
+    d1[c] = vel[c] > 0 ? su[c] : sd[c];
+    if (d2) d2[c] = 0.5 * (ssu[c] + ssd[c]);
   } else {
-    compact_solve<0>(f + base, cu, st, n, h, su + base, ssu + base);
-#pragma unroll 8
-    for (int i = 0; i < n; ++i) {
-      const long c = base + (long)i * st;
-      d1[c] = su[c];
-      if (d2) d2[c] = ssu[c];
-    }
+    d1[c] = su[c];
+    if (d2) d2[c] = ssu[c];
   }
 }
 
@@ -1315,13 +1323,23 @@ struct scihpc_ls {
     for (int a = 0; a < g.ndim; ++a) {
       const int t1 = (a + 1) % 3, t2 = (a + 2) % 3;
       const long lines = (long)g.N[t1] * g.N[t2];
-      const unsigned nb = ceil_div(lines, 64);
+      const unsigned nbc = ceil_div(g.total, kLsBlock);
+      const int nsys = upwind ? 2 : 1;
       if (upwind)
-        k_compact_line<true><<<nb, 64, 0, stream>>>(g, a, f, vel[a], coef_up[a], coef_dn[a], ls_scr[0], ls_scr[1],
-                                                    ls_scr[2], ls_scr[3], d1[a], d2 ? d2[a] : nullptr);
+        k_compact_rhs<true><<<nbc, kLsBlock, 0, stream>>>(g, a, f, ls_scr[0], ls_scr[1], ls_scr[2], ls_scr[3]);
       else
-        k_compact_line<false><<<nb, 64, 0, stream>>>(g, a, f, nullptr, coef_ccd[a], nullptr, ls_scr[0], ls_scr[1],
-                                                     nullptr, nullptr, d1[a], d2 ? d2[a] : nullptr);
+        k_compact_rhs<false><<<nbc, kLsBlock, 0, stream>>>(g, a, f, ls_scr[0], ls_scr[1], nullptr, nullptr);
+      SCIHPC_CHECK_LAUNCH();
+      k_twin_line<<<ceil_div(lines * nsys, 64), 64, 0, stream>>>(g, a, nsys, upwind ? coef_up[a] : coef_ccd[a],
+                                                                  coef_dn[a], ls_scr[0], ls_scr[1], ls_scr[2],
+                                                                  ls_scr[3]);
+      SCIHPC_CHECK_LAUNCH();
+      if (upwind)
+        k_compact_select<true><<<nbc, kLsBlock, 0, stream>>>(g.total, vel[a], ls_scr[0], ls_scr[1], ls_scr[2],
+                                                             ls_scr[3], d1[a], d2 ? d2[a] : nullptr);
+      else
+        k_compact_select<false><<<nbc, kLsBlock, 0, stream>>>(g.total, nullptr, ls_scr[0], ls_scr[1], nullptr,
+                                                              nullptr, d1[a], d2 ? d2[a] : nullptr);
       SCIHPC_CHECK_LAUNCH();
     }
   }
