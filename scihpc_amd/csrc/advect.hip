@@ -1052,14 +1052,36 @@ __global__ void __launch_bounds__(kLsBlock)
 template <int DIM>
 __global__ void __launch_bounds__(kLsBlock)
     k_mpls_update(long n, double *__restrict__ phi, const double *__restrict__ fx, const double *__restrict__ fy,
-                  const double *__restrict__ fz, double w, double eta, double dt, double *__restrict__ delta) {
+                  const double *__restrict__ fz, double w, const double *__restrict__ eta_p, double dt,
+                  double *__restrict__ delta) {
 #pragma clang fp contract(off)
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
+  const double eta = *eta_p;
   const double dl = ls_delta(phi[c], w);
   const double gr = grad_mag(DIM, fx[c], fy[c], DIM > 2 ? fz[c] : 0.0);
   delta[c] = dl;
   phi[c] += dt * (eta * dl * gr);
+}
+
+// eta = (mass0 - mass) / (den dv dt) from the block sums of k_mpls_sums,
+// on the device (one workgroup, fixed summation order): the Euler update
+// then needs no host round trip
+__global__ void __launch_bounds__(kLsBlock)
+    k_mpls_eta(const double *__restrict__ part, int nb, double mass0, double dv, double dt,
+               double *__restrict__ eta) {
+#pragma clang fp contract(off)
+  double m = 0.0, e = 0.0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    m += part[2 * b];
+    e += part[2 * b + 1];
+  }
+  m = block_sum(m);
+  e = block_sum(e);
+  if (threadIdx.x == 0) {
+    const double mass = m * dv;
+    eta[0] = (mass0 - mass) / (e * dv * dt);
+  }
 }
 
 __global__ void __launch_bounds__(kLsBlock)
@@ -1313,7 +1335,7 @@ struct scihpc_ls {
     grd = alloc();
     flux = alloc();
     const long nb = (long)ceil_div(g.interior, kLsBlock);
-    SCIHPC_HIP(hipMalloc(reinterpret_cast<void **>(&lpart), sizeof(double) * 2 * nb));
+    SCIHPC_HIP(hipMalloc(reinterpret_cast<void **>(&lpart), sizeof(double) * (2 * nb + 1)));
     SCIHPC_HIP(hipHostMalloc(reinterpret_cast<void **>(&lhost), sizeof(double) * 2 * nb, hipHostMallocDefault));
   }
 
@@ -1394,10 +1416,13 @@ struct scihpc_ls {
     SCIHPC_CHECK_LAUNCH();
     double *d1[3] = {fx, fy, fz};
     compact(flux, false, d1, nullptr);
-    double mass, den;
-    mpls_sums<DIM>(w_, ratio, positive_ref, mass, den);
+    const unsigned nbi = ceil_div(g.interior, kLsBlock);
     const double dv = DIM == 2 ? g.h[0] * g.h[1] : g.h[0] * g.h[1] * g.h[2];
-    const double eta = (mass0 - mass) / (den * dv * dt);
+    k_mpls_sums<DIM><<<nbi, kLsBlock, 0, stream>>>(g, f, fx, fy, fz, w_, ratio, positive_ref, dv, lpart);
+    SCIHPC_CHECK_LAUNCH();
+    double *eta = lpart + 2 * (long)nbi;  // one slot past the block sums
+    k_mpls_eta<<<1, kLsBlock, 0, stream>>>(lpart, (int)nbi, mass0, dv, dt, eta);
+    SCIHPC_CHECK_LAUNCH();
     k_mpls_update<DIM><<<nb, kLsBlock, 0, stream>>>(g.total, f, fx, fy, fz, w_, eta, dt, dlt);
     SCIHPC_CHECK_LAUNCH();
     apply_bc(f, -1);
