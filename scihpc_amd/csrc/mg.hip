@@ -128,13 +128,40 @@ __device__ __forceinline__ void cell_ijk(long c, int nx, int ny, int &i, int &j,
   k = (int)(r / uy);
 }
 
-// x / d: IEEE division in FP64 (parity with the oracle); in the FP32
-// V-cycle, which only preconditions the FP64 iteration, the reciprocal
-// instruction and one multiply
+// 1 / s for the smoothers and transfers.  FP32: the hardware reciprocal
+// (v_rcp_f32, about 1 ulp; the FP32 V-cycle only preconditions the FP64
+// iteration).  FP64: the hardware estimate refined by two Newton-Raphson
+// steps r <- r + r (1 - s r), which square the relative error each time and
+// end within an ulp or two of 1/s, in 5 instructions instead of the ~10 of
+// an IEEE division.  s = +inf gives 0 in both.
+template <typename T>
+__device__ __forceinline__ T smooth_rcp(T s) {
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_amdgcn_rcpf(s);
+  } else {
+    const double r0 = __builtin_amdgcn_rcp(s);
+    double e = __builtin_fma(-s, r0, 1.0);
+    double r = __builtin_fma(r0, e, r0);
+    e = __builtin_fma(-s, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    return r0 == 0.0 ? 0.0 : r;
+  }
+}
+
+// a / d for a Gauss-Seidel update.  FP32: a times the hardware reciprocal.
+// FP64: a times the refined reciprocal, then one residual correction
+// q + r (a - d q) (Markstein's quotient step), which rounds like the IEEE
+// division in nearly every case; the smoothers' iterates agree with the
+// oracle's to rounding, as the tests bound them.
 template <typename T>
 __device__ __forceinline__ T gs_div(T a, T d) {
-  if constexpr (sizeof(T) == 4) return __fdividef(a, d);
-  else return a / d;
+  if constexpr (sizeof(T) == 4) {
+    return a * __builtin_amdgcn_rcpf(d);
+  } else {
+    const double r = smooth_rcp<double>(d);
+    const double q = a * r;
+    return __builtin_fma(__builtin_fma(-d, q, a), r, q);
+  }
 }
 
 // ---------------------------------------------------- red-black smoother
@@ -695,13 +722,13 @@ struct Z3 {
 
 __device__ __forceinline__ int z3_slot(int plane) { return (plane + 2 * kZ3Ring) % kZ3Ring; }
 
-// face between cells of density a and b, spacing h: 2 / (a + b) / h^2
-// (FP64: one IEEE division; equal to init_ppe_proj_coefficients'
-// 1 / (0.5 (a + b)) / h / h up to rounding)
+// face between cells of density a and b: 2 / (a + b) / h^2, equal to
+// init_ppe_proj_coefficients' 1 / (0.5 (a + b)) / h / h
+// (projection_method.cpp:404-465) up to rounding; inv_h2 = 1 / h^2.  A
+// neighbour outside the domain carries rho = +inf and gives 0.
 template <typename T>
-__device__ __forceinline__ T rho_face(T a, T b, T h, T inv_h2) {
-  if constexpr (sizeof(T) == 4) return __fdividef(2.0f * inv_h2, a + b);
-  else return inv_h2 * (2.0 / (a + b));
+__device__ __forceinline__ T rho_face(T a, T b, T inv_h2) {
+  return inv_h2 * (T(2) * smooth_rcp<T>(a + b));
 }
 
 // off-diagonal sum and diagonal of cell (gx, gy, gz) from LDS neighbours;
@@ -830,12 +857,12 @@ __global__ void __launch_bounds__(kZ3Threads)
     }
     if constexpr (COEF == 2) {
       const T rc = sm[1][sc][q];
-      cL = rho_face<T>(sm[1][sc][q - 1], rc, hx, F.ax);
-      cR = rho_face<T>(rc, sm[1][sc][q + 1], hx, F.ax);
-      cD = rho_face<T>(sm[1][sc][q - W], rc, hy, F.ay);
-      cU = rho_face<T>(rc, sm[1][sc][q + W], hy, F.ay);
-      cB = rho_face<T>(sm[1][sb][q], rc, hz, F.az);
-      cF = rho_face<T>(rc, sm[1][sf][q], hz, F.az);
+      cL = rho_face<T>(sm[1][sc][q - 1], rc, F.ax);
+      cR = rho_face<T>(rc, sm[1][sc][q + 1], F.ax);
+      cD = rho_face<T>(sm[1][sc][q - W], rc, F.ay);
+      cU = rho_face<T>(rc, sm[1][sc][q + W], F.ay);
+      cB = rho_face<T>(sm[1][sb][q], rc, F.az);
+      cF = rho_face<T>(rc, sm[1][sf][q], F.az);
     }
     T off, diag;
     z3_stencil<T, COEF>(F, gx, gy, gz, sm[0][sc][q - 1], sm[0][sc][q + 1], sm[0][sc][q - W], sm[0][sc][q + W],
@@ -907,12 +934,12 @@ __global__ void __launch_bounds__(kBlock) k_rr3d(DLevel<T> F, DLevel<T> G, const
         T off, diag;
         if constexpr (COEF == 2) {
           const T inf = T(INFINITY), rc = rho[c];
-          const T cL = rho_face<T>(i > 0 ? rho[c - 1] : inf, rc, hx, F.ax);
-          const T cR = rho_face<T>(rc, i < F.nx - 1 ? rho[c + 1] : inf, hx, F.ax);
-          const T cD = rho_face<T>(j > 0 ? rho[c - sy] : inf, rc, hy, F.ay);
-          const T cU = rho_face<T>(rc, j < F.ny - 1 ? rho[c + sy] : inf, hy, F.ay);
-          const T cB = rho_face<T>(k > 0 ? rho[c - sz] : inf, rc, hz, F.az);
-          const T cF = rho_face<T>(rc, k < F.nz - 1 ? rho[c + sz] : inf, hz, F.az);
+          const T cL = rho_face<T>(i > 0 ? rho[c - 1] : inf, rc, F.ax);
+          const T cR = rho_face<T>(rc, i < F.nx - 1 ? rho[c + 1] : inf, F.ax);
+          const T cD = rho_face<T>(j > 0 ? rho[c - sy] : inf, rc, F.ay);
+          const T cU = rho_face<T>(rc, j < F.ny - 1 ? rho[c + sy] : inf, F.ay);
+          const T cB = rho_face<T>(k > 0 ? rho[c - sz] : inf, rc, F.az);
+          const T cF = rho_face<T>(rc, k < F.nz - 1 ? rho[c + sz] : inf, F.az);
           const T *x = F.sol;
           z3_stencil<T, 2>(F, i, j, k, i > 0 ? x[c - 1] : T(0), i < F.nx - 1 ? x[c + 1] : T(0),
                            j > 0 ? x[c - sy] : T(0), j < F.ny - 1 ? x[c + sy] : T(0), k > 0 ? x[c - sz] : T(0),
