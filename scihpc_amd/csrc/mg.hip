@@ -279,6 +279,52 @@ __global__ void __launch_bounds__(kBlock) k_prolong_add(DLevel<T> G, DLevel<T> F
   if (c < F.n) prolong_cell<T, DIM>(G, F, r, dir, c);
 }
 
+// 3-D prolong-add for factor-2 coarsening: prolong_cell with r = 2.  Each
+// thread owns the x-adjacent fine cells 2m, 2m+1 of one fine row; their x
+// weights are 1/4, 3/4 on coarse cells m-1, m and 3/4, 1/4 on m, m+1, so a
+// row pair reads three coarse values per coarse row and the fine pair moves
+// as one vector.  Block: 64 pairs x 4 fine rows; grid z: fine planes.
+constexpr int kP3X = 64, kP3Y = 4;
+template <typename T>
+__global__ void __launch_bounds__(kP3X *kP3Y) k_prolong3_r2(DLevel<T> G, DLevel<T> F, int dir) {
+  using T2 = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
+  const int m = blockIdx.x * kP3X + threadIdx.x;
+  const int j = blockIdx.y * kP3Y + threadIdx.y, k = blockIdx.z;
+  if (m >= G.nx || j >= F.ny) return;
+  const int J0 = (j - 1) >> 1, K0 = (k - 1) >> 1;  // floor((2j - 1) / 4)
+  const T ty = (j & 1) ? T(0.25) : T(0.75), tz = (k & 1) ? T(0.25) : T(0.75);
+  T a[2][2][3];  // [J - J0][K - K0][I - m + 1], ghost rule of coarse_at
+#pragma unroll
+  for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+    for (int kk = 0; kk < 2; ++kk) {
+      const int J = J0 + jj, K = K0 + kk;
+      const bool rout = J < 0 || J >= G.ny || K < 0 || K >= G.nz;
+      const T *row = G.sol + (long)G.nx * (min(max(J, 0), G.ny - 1) + (long)G.ny * min(max(K, 0), G.nz - 1));
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const int I = m - 1 + d;
+        const bool out = rout || I < 0 || I >= G.nx;
+        a[jj][kk][d] = (out && dir) ? T(0) : row[min(max(I, 0), G.nx - 1)];
+      }
+    }
+  auto interp = [&](T tx, int o) {
+    const T v00 = (T(1) - tx) * a[0][0][o] + tx * a[0][0][o + 1];
+    const T v10 = (T(1) - tx) * a[1][0][o] + tx * a[1][0][o + 1];
+    const T v0 = (T(1) - ty) * v00 + ty * v10;
+    const T v01 = (T(1) - tx) * a[0][1][o] + tx * a[0][1][o + 1];
+    const T v11 = (T(1) - tx) * a[1][1][o] + tx * a[1][1][o + 1];
+    const T v1 = (T(1) - ty) * v01 + ty * v11;
+    return (T(1) - tz) * v0 + tz * v1;
+  };
+  // F.nx = 2 G.nx is even, so the pair is aligned to two elements
+  T2 *p = reinterpret_cast<T2 *>(F.sol + 2 * m + (long)F.nx * (j + (long)F.ny * k));
+  T2 f = *p;
+  f.x += interp(T(0.75), 0);
+  f.y += interp(T(0.25), 1);
+  *p = f;
+}
+
 // --------------------------------------------- dense coarsest solve
 // x = Minv b in FP64 on one workgroup; Neumann: subtract the mean (the
 // pinned inverse maps a nonzero-mean b to x0 - mean(b)/s, see set-up).
@@ -915,42 +961,54 @@ __global__ void __launch_bounds__(kZ3Threads)
   }
 }
 
-// coarse.rhs = child average of (b - A x), coarse.sol = 0 on a 3-D level
-// whose operator is given by rho (COEF 2) or as stencil() (COEF 0/1)
+// coarse.rhs = child average of (b - A x), coarse.sol = 0 for factor-2
+// coarsening of a 3-D z-streaming level, with the 2 x 2 x 2 children of a
+// coarse cell spread over four threads: thread (m, jj, kk) of a 64 x 2 x 2
+// block takes the x-adjacent fine cells 2m, 2m+1 of fine row (2J + jj,
+// 2K + kk), and the block sums the four pair sums in LDS.  The residual of
+// a cell is k_rr3d's; only the order of the eight-term sum differs.
+constexpr int kRR3X = 64;
 template <typename T, int COEF>
-__global__ void __launch_bounds__(kBlock) k_rr3d(DLevel<T> F, DLevel<T> G, const T *__restrict__ rho, T hx, T hy,
-                                                 T hz) {
-  const long C = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (C >= G.n) return;
-  int I, J, K;
-  cell_ijk(C, G.nx, G.ny, I, J, K);
+__global__ void __launch_bounds__(kRR3X * 4) k_rr3d_pairs(DLevel<T> F, DLevel<T> G, const T *__restrict__ rho) {
+  __shared__ T part[4][kRR3X];
+  const int m = blockIdx.x * kRR3X + threadIdx.x;
+  const int J = blockIdx.y, K = blockIdx.z;
+  const int j = 2 * J + threadIdx.y, k = 2 * K + threadIdx.z;
   const long sy = F.nx, sz = (long)F.nx * F.ny;
   T s = T(0);
-  for (int kk = 0; kk < 2; ++kk)
-    for (int jj = 0; jj < 2; ++jj)
-      for (int ii = 0; ii < 2; ++ii) {
-        const int i = 2 * I + ii, j = 2 * J + jj, k = 2 * K + kk;
-        const long c = i + sy * j + sz * k;
-        T off, diag;
-        if constexpr (COEF == 2) {
-          const T inf = T(INFINITY), rc = rho[c];
-          const T cL = rho_face<T>(i > 0 ? rho[c - 1] : inf, rc, F.ax);
-          const T cR = rho_face<T>(rc, i < F.nx - 1 ? rho[c + 1] : inf, F.ax);
-          const T cD = rho_face<T>(j > 0 ? rho[c - sy] : inf, rc, F.ay);
-          const T cU = rho_face<T>(rc, j < F.ny - 1 ? rho[c + sy] : inf, F.ay);
-          const T cB = rho_face<T>(k > 0 ? rho[c - sz] : inf, rc, F.az);
-          const T cF = rho_face<T>(rc, k < F.nz - 1 ? rho[c + sz] : inf, F.az);
-          const T *x = F.sol;
-          z3_stencil<T, 2>(F, i, j, k, i > 0 ? x[c - 1] : T(0), i < F.nx - 1 ? x[c + 1] : T(0),
-                           j > 0 ? x[c - sy] : T(0), j < F.ny - 1 ? x[c + sy] : T(0), k > 0 ? x[c - sz] : T(0),
-                           k < F.nz - 1 ? x[c + sz] : T(0), cL, cR, cD, cU, cB, cF, off, diag);
-        } else {
-          stencil<T, 3, COEF == 1>(F, F.sol, i, j, k, c, off, diag);
-        }
-        s += F.rhs[c] - (off + diag * F.sol[c]);
+  if (m < G.nx) {
+#pragma unroll
+    for (int ii = 0; ii < 2; ++ii) {
+      const int i = 2 * m + ii;
+      const long c = i + sy * j + sz * k;
+      T off, diag;
+      if constexpr (COEF == 2) {
+        const T inf = T(INFINITY), rc = rho[c];
+        const T cL = rho_face<T>(i > 0 ? rho[c - 1] : inf, rc, F.ax);
+        const T cR = rho_face<T>(rc, i < F.nx - 1 ? rho[c + 1] : inf, F.ax);
+        const T cD = rho_face<T>(j > 0 ? rho[c - sy] : inf, rc, F.ay);
+        const T cU = rho_face<T>(rc, j < F.ny - 1 ? rho[c + sy] : inf, F.ay);
+        const T cB = rho_face<T>(k > 0 ? rho[c - sz] : inf, rc, F.az);
+        const T cF = rho_face<T>(rc, k < F.nz - 1 ? rho[c + sz] : inf, F.az);
+        const T *x = F.sol;
+        z3_stencil<T, 2>(F, i, j, k, i > 0 ? x[c - 1] : T(0), i < F.nx - 1 ? x[c + 1] : T(0),
+                         j > 0 ? x[c - sy] : T(0), j < F.ny - 1 ? x[c + sy] : T(0), k > 0 ? x[c - sz] : T(0),
+                         k < F.nz - 1 ? x[c + sz] : T(0), cL, cR, cD, cU, cB, cF, off, diag);
+      } else {
+        stencil<T, 3, COEF == 1>(F, F.sol, i, j, k, c, off, diag);
       }
-  G.rhs[C] = s / T(8);
-  G.sol[C] = T(0);
+      s += F.rhs[c] - (off + diag * F.sol[c]);
+    }
+  }
+  const int w = threadIdx.y + 2 * threadIdx.z;
+  part[w][threadIdx.x] = s;
+  __syncthreads();
+  if (w == 0 && m < G.nx) {
+    const long C = m + (long)G.nx * (J + (long)G.ny * K);
+    G.rhs[C] = (((part[0][threadIdx.x] + part[1][threadIdx.x]) + part[2][threadIdx.x]) + part[3][threadIdx.x]) /
+               T(8);
+    G.sol[C] = T(0);
+  }
 }
 
 // --------------------------------------------------- residual + norm
@@ -1577,7 +1635,11 @@ struct scihpc_mg {
     const int r = coarse.deg / fine.deg, dir = bc == SCIHPC_MG_DIRICHLET;
     if (ndim == 2)
       k_prolong_add<T, 2><<<ceil_div(fine.n, kBlock), kBlock, 0, stream>>>(view(coarse), view(fine), r, dir);
-    else
+    else if (r == 2) {
+      SCIHPC_REQUIRE(fine.nz <= 65535, "3-D level too tall for the prolongation grid");
+      const dim3 grid(ceil_div(coarse.nx, kP3X), ceil_div(fine.ny, kP3Y), fine.nz);
+      k_prolong3_r2<T><<<grid, dim3(kP3X, kP3Y), 0, stream>>>(view(coarse), view(fine), dir);
+    } else
       k_prolong_add<T, 3><<<ceil_div(fine.n, kBlock), kBlock, 0, stream>>>(view(coarse), view(fine), r, dir);
     SCIHPC_CHECK_LAUNCH();
   }
@@ -1697,14 +1759,12 @@ struct scihpc_mg {
   template <typename T>
   void rr3(std::vector<Level<T>> &lv, int l) {
     auto &a = lv[l], &b = lv[l + 1];
-    const unsigned g = ceil_div(b.n, kBlock);
     const int coef = z3_coef<T>(l);
-    if (coef == 2)
-      k_rr3d<T, 2><<<g, kBlock, 0, stream>>>(view(a), view(b), z3_rho<T>(), T(a.dx), T(a.dy), T(a.dz));
-    else if (coef == 1)
-      k_rr3d<T, 1><<<g, kBlock, 0, stream>>>(view(a), view(b), nullptr, T(a.dx), T(a.dy), T(a.dz));
-    else
-      k_rr3d<T, 0><<<g, kBlock, 0, stream>>>(view(a), view(b), nullptr, T(a.dx), T(a.dy), T(a.dz));
+    SCIHPC_REQUIRE(b.ny <= 65535 && b.nz <= 65535, "3-D level too tall for the restriction grid");
+    const dim3 grid(ceil_div(b.nx, kRR3X), b.ny, b.nz), block(kRR3X, 2, 2);
+    if (coef == 2) k_rr3d_pairs<T, 2><<<grid, block, 0, stream>>>(view(a), view(b), z3_rho<T>());
+    else if (coef == 1) k_rr3d_pairs<T, 1><<<grid, block, 0, stream>>>(view(a), view(b), nullptr);
+    else k_rr3d_pairs<T, 0><<<grid, block, 0, stream>>>(view(a), view(b), nullptr);
     SCIHPC_CHECK_LAUNCH();
   }
 
