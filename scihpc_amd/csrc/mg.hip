@@ -507,6 +507,11 @@ __device__ __forceinline__ int xcd_tile(int ntiles) {
   return (b % 8) * (ntiles / 8) + b / 8;
 }
 
+// blockIdx.x of a 1-D cell grid, remapped as xcd_tile does: each XCD gets
+// a contiguous run of blocks, so a stencil's neighbours one row or plane
+// away sit in the same L2.  Partial sums stay indexed by blockIdx.x.
+__device__ __forceinline__ unsigned xcd_block() { return (unsigned)xcd_tile((int)gridDim.x); }
+
 template <typename T, bool VAR, int TX, int TY>
 struct Fused {
   static constexpr int W = TX + 2 * kFH, H = TY + 2 * kFH;
@@ -1017,7 +1022,7 @@ template <typename T, int DIM, bool VAR>
 __global__ void __launch_bounds__(kBlock) k_residual(DLevel<T> L, int store, double *__restrict__ partial,
                                                      const double *__restrict__ flag) {
   if (flag && *flag != 0.0) return;
-  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long c = (long)xcd_block() * blockDim.x + threadIdx.x;
   double v = 0.0;
   if (c < L.n) {
     int i, j, k;
@@ -1038,7 +1043,7 @@ __global__ void __launch_bounds__(kBlock) k_apply_dot(DLevel<T> L, const T *__re
                                                       double *__restrict__ partial,
                                                       const double *__restrict__ flag) {
   if (flag && *flag != 0.0) return;
-  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long c = (long)xcd_block() * blockDim.x + threadIdx.x;
   double v = 0.0;
   if (c < L.n) {
     int i, j, k;
@@ -1218,7 +1223,7 @@ __global__ void __launch_bounds__(kBlock) k_outer_update(DLevel<double> L, const
                                                          const float *__restrict__ e, double *__restrict__ p_new,
                                                          float *__restrict__ r32, double *__restrict__ partial,
                                                          int first) {
-  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long c = (long)xcd_block() * blockDim.x + threadIdx.x;
   double v = 0.0;
   if (c < L.n) {
     double r;
@@ -1291,7 +1296,7 @@ __global__ void __launch_bounds__(kBlock)
     k_fcg_direction(DLevel<double> L, const T *__restrict__ z, const double *__restrict__ p_old,
                     double *__restrict__ p, double *__restrict__ q, const double *__restrict__ r,
                     const double *__restrict__ scal, int first, double *__restrict__ partial) {
-  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long c = (long)xcd_block() * blockDim.x + threadIdx.x;
   const double beta = first ? 0.0 : -scal[S_ZQ] / scal[S_PQ];
   double vpq = 0.0, vpr = 0.0;
   if (c < L.n) {
@@ -1478,7 +1483,7 @@ __global__ void __launch_bounds__(kBlock) k_net_to_level(const float *__restrict
 template <int DIM, bool VAR>
 __global__ void __launch_bounds__(kBlock) k_guess_dots(DLevel<double> L, const double *__restrict__ p,
                                                        double *__restrict__ partial, int nb) {
-  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long c = (long)xcd_block() * blockDim.x + threadIdx.x;
   double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
   if (c < L.n) {
     int i, j, k;
