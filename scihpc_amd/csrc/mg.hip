@@ -55,39 +55,58 @@ struct DLevel {
 enum { S_R2 = 0, S_RAR = 1, S_SUM = 2, S_FLAG = 3, S_NORM = 4, S_B2 = 5, S_ZQ = 8, S_PQ = 9, S_PR = 10, S_COUNT = 16 };
 
 // -------------------------------------------------------- stencil helpers
-// off = sum over present neighbours of coef*x, diag = A(c,c).  Loads of
-// neighbours are guarded (no padding).
+// off = sum over present neighbours of coef*x, diag = A(c,c).  There is no
+// padding: a missing neighbour's load is redirected to the cell itself and
+// its coefficient is 0, so every load issues unconditionally (one wait for
+// all of them) and adding the 0 * x term leaves the sum unchanged (it starts
+// at +0, so no -0 can appear).  The accumulation order is the reference's.
 template <typename T, int DIM, bool VAR>
 __device__ __forceinline__ void stencil(const DLevel<T> &L, const T *__restrict__ x, int i,
                                         int j, int k, long c, T &off, T &diag) {
   const int nx = L.nx, ny = L.ny, nz = L.nz;
   const long sy = nx, sz = (long)nx * ny;
+  const bool hl = i > 0, hr = i < nx - 1, hd = j > 0, hu = j < ny - 1;
+  const long cl = hl ? c - 1 : c, cr = hr ? c + 1 : c, cd = hd ? c - sy : c, cu = hu ? c + sy : c;
+  const T xl = x[cl], xr = x[cr], xd = x[cd], xu = x[cu];
   if constexpr (VAR) {
-    const T cR = L.cx[c], cL = i > 0 ? L.cx[c - 1] : T(0);
-    const T cU = L.cy[c], cD = j > 0 ? L.cy[c - sy] : T(0);
+    const T cR = L.cx[c], cLv = L.cx[cl], cU = L.cy[c], cDv = L.cy[cd];
+    const T cL = hl ? cLv : T(0), cD = hd ? cDv : T(0);
     T o = T(0);
-    if (i > 0) o += cL * x[c - 1];
-    if (i < nx - 1) o += cR * x[c + 1];
-    if (j > 0) o += cD * x[c - sy];
-    if (j < ny - 1) o += cU * x[c + sy];
+    o += cL * xl;
+    o += (hr ? cR : T(0)) * xr;
+    o += cD * xd;
+    o += (hu ? cU : T(0)) * xu;
     T d = -(cR + cL + cU + cD);
     if constexpr (DIM == 3) {
-      const T cF = L.cz[c], cB = k > 0 ? L.cz[c - sz] : T(0);
-      if (k > 0) o += cB * x[c - sz];
-      if (k < nz - 1) o += cF * x[c + sz];
+      const bool hb = k > 0, hf = k < nz - 1;
+      const long cb = hb ? c - sz : c, cf = hf ? c + sz : c;
+      const T cF = L.cz[c], cBv = L.cz[cb];
+      const T cB = hb ? cBv : T(0);
+      o += cB * x[cb];
+      o += (hf ? cF : T(0)) * x[cf];
       d -= (cF + cB);
     }
     off = o;
     diag = d;
   } else {
+    const T wx = L.rbwall ? L.wx : T(0), wy = L.rbwall ? L.wy : T(0);
     T o = T(0), cc = T(0);
-    if (i > 0) { o += L.ax * x[c - 1]; cc += L.ax; } else if (L.rbwall) cc += L.wx;
-    if (i < nx - 1) { o += L.ax * x[c + 1]; cc += L.ax; } else if (L.rbwall) cc += L.wx;
-    if (j > 0) { o += L.ay * x[c - sy]; cc += L.ay; } else if (L.rbwall) cc += L.wy;
-    if (j < ny - 1) { o += L.ay * x[c + sy]; cc += L.ay; } else if (L.rbwall) cc += L.wy;
+    o += (hl ? L.ax : T(0)) * xl;
+    cc += hl ? L.ax : wx;
+    o += (hr ? L.ax : T(0)) * xr;
+    cc += hr ? L.ax : wx;
+    o += (hd ? L.ay : T(0)) * xd;
+    cc += hd ? L.ay : wy;
+    o += (hu ? L.ay : T(0)) * xu;
+    cc += hu ? L.ay : wy;
     if constexpr (DIM == 3) {
-      if (k > 0) { o += L.az * x[c - sz]; cc += L.az; } else if (L.rbwall) cc += L.wz;
-      if (k < nz - 1) { o += L.az * x[c + sz]; cc += L.az; } else if (L.rbwall) cc += L.wz;
+      const T wz = L.rbwall ? L.wz : T(0);
+      const bool hb = k > 0, hf = k < nz - 1;
+      const long cb = hb ? c - sz : c, cf = hf ? c + sz : c;
+      o += (hb ? L.az : T(0)) * x[cb];
+      cc += hb ? L.az : wz;
+      o += (hf ? L.az : T(0)) * x[cf];
+      cc += hf ? L.az : wz;
     }
     off = o;
     diag = (L.dirichlet && !L.rbwall) ? L.dconst : -cc;
@@ -228,6 +247,12 @@ __global__ void __launch_bounds__(kBlock) k_residual_restrict(DLevel<T> F, DLeve
 // Cell-centred linear interpolation per axis: X = (2i+1-r)/(2r), exact
 // integer weights; ghost rule Neumann -> nearest interior, Dirichlet -> 0.
 __device__ __forceinline__ void interp_axis(int i, int r, int &I0, float &tf, double &td) {
+  if (r == 2) {  // num = 2i - 1, den = 4: the same values without the divisions
+    I0 = (i - 1) >> 1;
+    td = (i & 1) ? 0.25 : 0.75;
+    tf = float(td);
+    return;
+  }
   const int num = 2 * i + 1 - r, den = 2 * r;
   const int q = num >= 0 ? num / den : -((-num + den - 1) / den);
   I0 = q;
