@@ -382,13 +382,18 @@ __global__ void __launch_bounds__(kBlock) k_coarse_solve(DLevel<T> G, const doub
   coarse_solve_block(G, Minv, neumann);
 }
 
+// the one-workgroup kernels below run with exactly kBlockThreads threads; a
+// constant stride keeps blockDim.x (a load from the dispatch packet) out of
+// their loops
+constexpr int kBlockThreads = 1024;
+
 template <typename T, int DIM, bool VAR>
 __device__ void block_sweep(const DLevel<T> &L, int first, int zero_guess) {
   const int hx = (L.nx + 1) >> 1;
   const long half = (long)hx * L.ny * L.nz;
   for (int pass = 0; pass < 2; ++pass) {
     const int color = pass == 0 ? first : 1 - first;
-    for (long t = threadIdx.x; t < half; t += blockDim.x) {
+    for (long t = threadIdx.x; t < half; t += kBlockThreads) {
       int i, j, k;
       if (color_cell(t, L.nx, L.ny, L.nz, color, i, j, k))
         gs_cell<T, DIM, VAR>(L, i, j, k, pass == 0 ? zero_guess : 0);
@@ -407,7 +412,6 @@ __device__ void block_sweep(const DLevel<T> &L, int first, int zero_guess) {
 // than global-memory round trips.  Only the top level's iterate is written
 // back.
 constexpr int kMaxBlockLevels = 12;
-constexpr int kBlockThreads = 1024;
 
 template <typename T>
 struct BlockLevels {
@@ -446,7 +450,7 @@ __global__ void __launch_bounds__(kBlockThreads)
   for (int l = 0; l < L; ++l) {
     const DLevel<T> g = B->lv[l];
     const DLevel<T> S = lds_view<T, VAR, DIM>(B, l, smem);
-    for (long c = threadIdx.x; c < g.n; c += blockDim.x) {
+    for (long c = threadIdx.x; c < g.n; c += kBlockThreads) {
       S.sol[c] = (l == 0 && load_top && !zero_guess0) ? g.sol[c] : T(0);
       if (l == 0) S.rhs[c] = g.rhs[c];
       if constexpr (VAR) {
@@ -462,7 +466,7 @@ __global__ void __launch_bounds__(kBlockThreads)
     const DLevel<T> G = lds_view<T, VAR, DIM>(B, l + 1, smem);
     for (int s = 0; s < nu1; ++s) block_sweep<T, DIM, VAR>(F, 0, (l == 0 && s == 0) ? zero_guess0 : 0);
     const int r = B->ratio[l];
-    for (long C = threadIdx.x; C < G.n; C += blockDim.x) rr_cell<T, DIM, VAR>(F, G, r, C);
+    for (long C = threadIdx.x; C < G.n; C += kBlockThreads) rr_cell<T, DIM, VAR>(F, G, r, C);
     __syncthreads();
   }
   {  // dense coarsest solve, FP64, rows in registers
@@ -471,7 +475,7 @@ __global__ void __launch_bounds__(kBlockThreads)
     double xr[2] = {0.0, 0.0};
     double local = 0.0;
     for (int q = 0; q < 2; ++q) {
-      const int row = threadIdx.x + q * blockDim.x;
+      const int row = threadIdx.x + q * kBlockThreads;
       if (row < n) {
         const double *m = Minv + (long)row * n;
         double acc = 0.0;
@@ -485,7 +489,7 @@ __global__ void __launch_bounds__(kBlockThreads)
     if (threadIdx.x == 0) mean = neumann ? tot / n : 0.0;
     __syncthreads();
     for (int q = 0; q < 2; ++q) {
-      const int row = threadIdx.x + q * blockDim.x;
+      const int row = threadIdx.x + q * kBlockThreads;
       if (row < n) G.sol[row] = T(xr[q] - mean);
     }
     __syncthreads();
@@ -494,13 +498,13 @@ __global__ void __launch_bounds__(kBlockThreads)
     const DLevel<T> G = lds_view<T, VAR, DIM>(B, l, smem);
     const DLevel<T> F = lds_view<T, VAR, DIM>(B, l - 1, smem);
     const int r = B->ratio[l - 1];
-    for (long c = threadIdx.x; c < F.n; c += blockDim.x) prolong_cell<T, DIM>(G, F, r, dir, c);
+    for (long c = threadIdx.x; c < F.n; c += kBlockThreads) prolong_cell<T, DIM>(G, F, r, dir, c);
     __syncthreads();
     for (int s = 0; s < nu2; ++s) block_sweep<T, DIM, VAR>(F, 1, 0);
   }
   const DLevel<T> top = B->lv[0];
   const DLevel<T> S0 = lds_view<T, VAR, DIM>(B, 0, smem);
-  for (long c = threadIdx.x; c < top.n; c += blockDim.x) top.sol[c] = S0.sol[c];
+  for (long c = threadIdx.x; c < top.n; c += kBlockThreads) top.sol[c] = S0.sol[c];
 }
 
 // ---------------------------------------- fused 2-D fine-level kernels
