@@ -60,13 +60,14 @@ enum { S_R2 = 0, S_RAR = 1, S_SUM = 2, S_FLAG = 3, S_NORM = 4, S_B2 = 5, S_ZQ = 
 // its coefficient is 0, so every load issues unconditionally (one wait for
 // all of them) and adding the 0 * x term leaves the sum unchanged (it starts
 // at +0, so no -0 can appear).  The accumulation order is the reference's.
-template <typename T, int DIM, bool VAR>
+// I: index type (int for the LDS-resident levels of the one-workgroup cycle).
+template <typename T, int DIM, bool VAR, typename I = long>
 __device__ __forceinline__ void stencil(const DLevel<T> &L, const T *__restrict__ x, int i,
-                                        int j, int k, long c, T &off, T &diag) {
+                                        int j, int k, I c, T &off, T &diag) {
   const int nx = L.nx, ny = L.ny, nz = L.nz;
-  const long sy = nx, sz = (long)nx * ny;
+  const I sy = nx, sz = (I)nx * ny;
   const bool hl = i > 0, hr = i < nx - 1, hd = j > 0, hu = j < ny - 1;
-  const long cl = hl ? c - 1 : c, cr = hr ? c + 1 : c, cd = hd ? c - sy : c, cu = hu ? c + sy : c;
+  const I cl = hl ? c - 1 : c, cr = hr ? c + 1 : c, cd = hd ? c - sy : c, cu = hu ? c + sy : c;
   const T xl = x[cl], xr = x[cr], xd = x[cd], xu = x[cu];
   if constexpr (VAR) {
     const T cR = L.cx[c], cLv = L.cx[cl], cU = L.cy[c], cDv = L.cy[cd];
@@ -79,7 +80,7 @@ __device__ __forceinline__ void stencil(const DLevel<T> &L, const T *__restrict_
     T d = -(cR + cL + cU + cD);
     if constexpr (DIM == 3) {
       const bool hb = k > 0, hf = k < nz - 1;
-      const long cb = hb ? c - sz : c, cf = hf ? c + sz : c;
+      const I cb = hb ? c - sz : c, cf = hf ? c + sz : c;
       const T cF = L.cz[c], cBv = L.cz[cb];
       const T cB = hb ? cBv : T(0);
       o += cB * x[cb];
@@ -102,7 +103,7 @@ __device__ __forceinline__ void stencil(const DLevel<T> &L, const T *__restrict_
     if constexpr (DIM == 3) {
       const T wz = L.rbwall ? L.wz : T(0);
       const bool hb = k > 0, hf = k < nz - 1;
-      const long cb = hb ? c - sz : c, cf = hf ? c + sz : c;
+      const I cb = hb ? c - sz : c, cf = hf ? c + sz : c;
       o += (hb ? L.az : T(0)) * x[cb];
       cc += hb ? L.az : wz;
       o += (hf ? L.az : T(0)) * x[cf];
@@ -187,11 +188,11 @@ __device__ __forceinline__ T gs_div(T a, T d) {
 // Gauss-Seidel update of cell (i,j,k): x_c = (b_c - off) / diag.
 // zero_guess: neighbours read as 0 (first half-sweep from a zero initial
 // guess, which saves clearing x).
-template <typename T, int DIM, bool VAR>
+template <typename T, int DIM, bool VAR, typename I = long>
 __device__ __forceinline__ void gs_cell(const DLevel<T> &L, int i, int j, int k, int zero_guess) {
-  const long c = i + (long)L.nx * (j + (long)L.ny * k);
+  const I c = i + (I)L.nx * (j + (I)L.ny * k);
   T off, diag;
-  stencil<T, DIM, VAR>(L, L.sol, i, j, k, c, off, diag);
+  stencil<T, DIM, VAR, I>(L, L.sol, i, j, k, c, off, diag);
   if (zero_guess) off = T(0);
   L.sol[c] = gs_div<T>(L.rhs[c] - off, diag);
 }
@@ -218,8 +219,8 @@ __global__ void __launch_bounds__(kBlock) k_gs_color(DLevel<T> L, int color, int
 
 // ------------------------------------- fused residual + child average
 // coarse.rhs = mean over the r^d children of (b - A x), coarse.sol = 0.
-template <typename T, int DIM, bool VAR>
-__device__ __forceinline__ void rr_cell(const DLevel<T> &F, const DLevel<T> &G, int r, long C) {
+template <typename T, int DIM, bool VAR, typename Ix = long>
+__device__ __forceinline__ void rr_cell(const DLevel<T> &F, const DLevel<T> &G, int r, Ix C) {
   int I, J, K;
   cell_ijk(C, G.nx, G.ny, I, J, K);
   const int rz = DIM == 3 ? r : 1;
@@ -228,9 +229,9 @@ __device__ __forceinline__ void rr_cell(const DLevel<T> &F, const DLevel<T> &G, 
     for (int jj = 0; jj < r; ++jj)
       for (int ii = 0; ii < r; ++ii) {
         const int i = I * r + ii, j = J * r + jj, k = K * rz + kk;
-        const long c = i + (long)F.nx * (j + (long)F.ny * k);
+        const Ix c = i + (Ix)F.nx * (j + (Ix)F.ny * k);
         T off, diag;
-        stencil<T, DIM, VAR>(F, F.sol, i, j, k, c, off, diag);
+        stencil<T, DIM, VAR, Ix>(F, F.sol, i, j, k, c, off, diag);
         s += F.rhs[c] - (off + diag * F.sol[c]);
       }
   G.rhs[C] = s / T(r * r * rz);
@@ -260,18 +261,18 @@ __device__ __forceinline__ void interp_axis(int i, int r, int &I0, float &tf, do
   tf = float(td);
 }
 
-template <typename T>
+template <typename T, typename Ix = long>
 __device__ __forceinline__ T coarse_at(const DLevel<T> &G, int dir, int I, int J, int K) {
   const bool out = I < 0 || I >= G.nx || J < 0 || J >= G.ny || K < 0 || K >= G.nz;
   if (out && dir) return T(0);
   I = min(max(I, 0), G.nx - 1);
   J = min(max(J, 0), G.ny - 1);
   K = min(max(K, 0), G.nz - 1);
-  return G.sol[I + (long)G.nx * (J + (long)G.ny * K)];
+  return G.sol[I + (Ix)G.nx * (J + (Ix)G.ny * K)];
 }
 
-template <typename T, int DIM>
-__device__ __forceinline__ void prolong_cell(const DLevel<T> &G, const DLevel<T> &F, int r, int dir, long c) {
+template <typename T, int DIM, typename Ix = long>
+__device__ __forceinline__ void prolong_cell(const DLevel<T> &G, const DLevel<T> &F, int r, int dir, Ix c) {
   int i, j, k;
   cell_ijk(c, F.nx, F.ny, i, j, k);
   int I0, J0, K0 = 0;
@@ -280,18 +281,18 @@ __device__ __forceinline__ void prolong_cell(const DLevel<T> &G, const DLevel<T>
   interp_axis(i, r, I0, fx, dx_);
   interp_axis(j, r, J0, fy, dy_);
   const T tx = sizeof(T) == 8 ? T(dx_) : T(fx), ty = sizeof(T) == 8 ? T(dy_) : T(fy);
-  T v00 = (T(1) - tx) * coarse_at(G, dir, I0, J0, K0) + tx * coarse_at(G, dir, I0 + 1, J0, K0);
-  T v10 = (T(1) - tx) * coarse_at(G, dir, I0, J0 + 1, K0) + tx * coarse_at(G, dir, I0 + 1, J0 + 1, K0);
+  T v00 = (T(1) - tx) * coarse_at<T, Ix>(G, dir, I0, J0, K0) + tx * coarse_at<T, Ix>(G, dir, I0 + 1, J0, K0);
+  T v10 = (T(1) - tx) * coarse_at<T, Ix>(G, dir, I0, J0 + 1, K0) + tx * coarse_at<T, Ix>(G, dir, I0 + 1, J0 + 1, K0);
   T v = (T(1) - ty) * v00 + ty * v10;
   if constexpr (DIM == 3) {
     interp_axis(k, r, K0, fz, dz_);
     const T tz = sizeof(T) == 8 ? T(dz_) : T(fz);
-    v00 = (T(1) - tx) * coarse_at(G, dir, I0, J0, K0) + tx * coarse_at(G, dir, I0 + 1, J0, K0);
-    v10 = (T(1) - tx) * coarse_at(G, dir, I0, J0 + 1, K0) + tx * coarse_at(G, dir, I0 + 1, J0 + 1, K0);
+    v00 = (T(1) - tx) * coarse_at<T, Ix>(G, dir, I0, J0, K0) + tx * coarse_at<T, Ix>(G, dir, I0 + 1, J0, K0);
+    v10 = (T(1) - tx) * coarse_at<T, Ix>(G, dir, I0, J0 + 1, K0) + tx * coarse_at<T, Ix>(G, dir, I0 + 1, J0 + 1, K0);
     const T v0 = (T(1) - ty) * v00 + ty * v10;
-    const T v01 = (T(1) - tx) * coarse_at(G, dir, I0, J0, K0 + 1) + tx * coarse_at(G, dir, I0 + 1, J0, K0 + 1);
-    const T v11 = (T(1) - tx) * coarse_at(G, dir, I0, J0 + 1, K0 + 1) +
-                  tx * coarse_at(G, dir, I0 + 1, J0 + 1, K0 + 1);
+    const T v01 = (T(1) - tx) * coarse_at<T, Ix>(G, dir, I0, J0, K0 + 1) + tx * coarse_at<T, Ix>(G, dir, I0 + 1, J0, K0 + 1);
+    const T v11 = (T(1) - tx) * coarse_at<T, Ix>(G, dir, I0, J0 + 1, K0 + 1) +
+                  tx * coarse_at<T, Ix>(G, dir, I0 + 1, J0 + 1, K0 + 1);
     const T v1 = (T(1) - ty) * v01 + ty * v11;
     v = (T(1) - tz) * v0 + tz * v1;
   }
@@ -387,16 +388,24 @@ __global__ void __launch_bounds__(kBlock) k_coarse_solve(DLevel<T> G, const doub
 // their loops
 constexpr int kBlockThreads = 1024;
 
+// LDS-resident levels hold < 2^31 cells: 32-bit indices throughout
 template <typename T, int DIM, bool VAR>
 __device__ void block_sweep(const DLevel<T> &L, int first, int zero_guess) {
   const int hx = (L.nx + 1) >> 1;
-  const long half = (long)hx * L.ny * L.nz;
+  const int half = hx * L.ny * L.nz;
   for (int pass = 0; pass < 2; ++pass) {
     const int color = pass == 0 ? first : 1 - first;
-    for (long t = threadIdx.x; t < half; t += kBlockThreads) {
+    for (int t = threadIdx.x; t < half; t += kBlockThreads) {
       int i, j, k;
-      if (color_cell(t, L.nx, L.ny, L.nz, color, i, j, k))
-        gs_cell<T, DIM, VAR>(L, i, j, k, pass == 0 ? zero_guess : 0);
+      if constexpr (DIM == 2) {  // row j, colour cell t - j hx of it
+        j = (int)((unsigned)t / (unsigned)hx);
+        k = 0;
+        i = 2 * (t - j * hx) + ((j + color) & 1);
+        if (i >= L.nx) continue;
+      } else {
+        if (!color_cell(t, L.nx, L.ny, L.nz, color, i, j, k)) continue;
+      }
+      gs_cell<T, DIM, VAR, int>(L, i, j, k, pass == 0 ? zero_guess : 0);
     }
     __syncthreads();
   }
@@ -466,7 +475,7 @@ __global__ void __launch_bounds__(kBlockThreads)
     const DLevel<T> G = lds_view<T, VAR, DIM>(B, l + 1, smem);
     for (int s = 0; s < nu1; ++s) block_sweep<T, DIM, VAR>(F, 0, (l == 0 && s == 0) ? zero_guess0 : 0);
     const int r = B->ratio[l];
-    for (long C = threadIdx.x; C < G.n; C += kBlockThreads) rr_cell<T, DIM, VAR>(F, G, r, C);
+    for (int C = threadIdx.x; C < G.n; C += kBlockThreads) rr_cell<T, DIM, VAR, int>(F, G, r, C);
     __syncthreads();
   }
   {  // dense coarsest solve, FP64, rows in registers
@@ -498,7 +507,7 @@ __global__ void __launch_bounds__(kBlockThreads)
     const DLevel<T> G = lds_view<T, VAR, DIM>(B, l, smem);
     const DLevel<T> F = lds_view<T, VAR, DIM>(B, l - 1, smem);
     const int r = B->ratio[l - 1];
-    for (long c = threadIdx.x; c < F.n; c += kBlockThreads) prolong_cell<T, DIM>(G, F, r, dir, c);
+    for (int c = threadIdx.x; c < F.n; c += kBlockThreads) prolong_cell<T, DIM, int>(G, F, r, dir, c);
     __syncthreads();
     for (int s = 0; s < nu2; ++s) block_sweep<T, DIM, VAR>(F, 1, 0);
   }
