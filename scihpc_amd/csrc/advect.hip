@@ -781,57 +781,71 @@ __device__ __forceinline__ void compact_rhs(const double *__restrict__ f, long s
 // chunk overlap instead of serialising with the recurrence
 constexpr int kLineChunk = 8;
 
-// The line systems in three passes: the right-hand sides of every padded
-// cell (all divisions of the scheme's stencils, fully parallel), the twin
-// back substitution in place, one thread per (line, system) so UCCD's
-// upwind and downwind systems run side by side, and the selection of the
-// outputs (parallel again).  Same operations per value as the reference.
+// The line systems in three passes, each one launch for every axis: the
+// right-hand sides of every padded cell (all divisions of the scheme's
+// stencils, fully parallel), the twin back substitution in place, one thread
+// per (axis, line, system) so the axes' lines and UCCD's upwind and downwind
+// systems all run side by side (the sequential recurrences of one
+// derivative cost the longest line, not the sum over axes), and the
+// selection of the outputs (parallel again).  Same operations per value as
+// the reference.  Each axis has its own scratch.
+struct LineScratch {
+  double *su[3], *ssu[3], *sd[3], *ssd[3];
+};
+struct LineCoefs {
+  const double *cu[3], *cd[3];
+};
+
 template <bool UP>
 __global__ void __launch_bounds__(kLsBlock)
-    k_compact_rhs(Grid g, int a, const double *__restrict__ f, double *__restrict__ su, double *__restrict__ ssu,
-                  double *__restrict__ sd, double *__restrict__ ssd) {
+    k_compact_rhs(Grid g, const double *__restrict__ f, LineScratch sc) {
 #pragma clang fp contract(off)
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= g.total) return;
   int q[3];
   padded_ijk(g, c, q);
-  const int i = q[a], n = g.N[a];
-  const long st = g.s[a];
-  const double *line = f + (c - (long)i * st);
-  const double h = g.h[a];
-  double s, ss;
-  if (UP) {
-    compact_rhs<1>(line, st, n, h, i, s, ss);
-    su[c] = s;
-    ssu[c] = ss;
-    if (i > 0 && i < n - 1) compact_rhs<2>(line, st, n, h, i, s, ss);  // the ends are the upwind ones
-    sd[c] = s;
-    ssd[c] = ss;
-  } else {
-    compact_rhs<0>(line, st, n, h, i, s, ss);
-    su[c] = s;
-    ssu[c] = ss;
+  for (int a = 0; a < g.ndim; ++a) {
+    const int i = q[a], n = g.N[a];
+    const long st = g.s[a];
+    const double *line = f + (c - (long)i * st);
+    const double h = g.h[a];
+    double s, ss;
+    if (UP) {
+      compact_rhs<1>(line, st, n, h, i, s, ss);
+      sc.su[a][c] = s;
+      sc.ssu[a][c] = ss;
+      if (i > 0 && i < n - 1) compact_rhs<2>(line, st, n, h, i, s, ss);  // the ends are the upwind ones
+      sc.sd[a][c] = s;
+      sc.ssd[a][c] = ss;
+    } else {
+      compact_rhs<0>(line, st, n, h, i, s, ss);
+      sc.su[a][c] = s;
+      sc.ssu[a][c] = ss;
+    }
   }
 }
 
 // twin_bks (matrix_solver.cpp:58-83) in place on S/SS holding the
-// right-hand sides; thread t: line t % lines, system t / lines
-__global__ void __launch_bounds__(64)
-    k_twin_line(Grid g, int a, int nsys, const double *__restrict__ cu, const double *__restrict__ cd,
-                double *__restrict__ su, double *__restrict__ ssu, double *__restrict__ sd, double *__restrict__ ssd) {
+// right-hand sides; thread t: axis a, then line t % lines, system t / lines
+__global__ void __launch_bounds__(64) k_twin_line(Grid g, int nsys, LineCoefs co, LineScratch sc) {
 #pragma clang fp contract(off)
   constexpr int B = kLineChunk;
-  const int t1 = (a + 1) % 3, t2 = (a + 2) % 3;
-  const long lines = (long)g.N[t1] * g.N[t2];
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= lines * nsys) return;
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  int a = 0;
+  long lines = 0;
+  for (; a < g.ndim; ++a) {
+    lines = (long)g.N[(a + 1) % 3] * g.N[(a + 2) % 3];
+    if (t < lines * nsys) break;
+    t -= lines * nsys;
+  }
+  if (a >= g.ndim) return;
   const int sys = (int)(t / lines);
   long base;
   line_base(g, a, t % lines, base);
+  const double *c = sys == 0 ? co.cu[a] : co.cd[a];
+  double *S = (sys == 0 ? sc.su[a] : sc.sd[a]) + base, *SS = (sys == 0 ? sc.ssu[a] : sc.ssd[a]) + base;
   const int n = g.N[a];
   const long st = g.s[a];
-  const double *c = sys == 0 ? cu : cd;
-  double *S = (sys == 0 ? su : sd) + base, *SS = (sys == 0 ? ssu : ssd) + base;
   const double *a0 = c, *a1 = c + n, *a2 = c + 2 * n, *b0 = c + 3 * n, *b1 = c + 4 * n, *b2 = c + 5 * n;
   const double *aa0 = c + 6 * n, *aa1 = c + 7 * n, *aa2 = c + 8 * n, *bb0 = c + 9 * n, *bb1 = c + 10 * n,
                *bb2 = c + 11 * n;
@@ -906,18 +920,22 @@ __global__ void __launch_bounds__(64)
 
 template <bool UP>
 __global__ void __launch_bounds__(kLsBlock)
-    k_compact_select(long total, const double *__restrict__ vel, const double *__restrict__ su,
-                     const double *__restrict__ ssu, const double *__restrict__ sd, const double *__restrict__ ssd,
-                     double *__restrict__ d1, double *__restrict__ d2) {
+    k_compact_select(long total, int ndim, const double *__restrict__ u, const double *__restrict__ v,
+                     const double *__restrict__ w, LineScratch sc, double *d1x, double *d1y, double *d1z,
+                     double *d2x, double *d2y, double *d2z) {
 #pragma clang fp contract(off)
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= total) return;
-  if (UP) {
-    d1[c] = vel[c] > 0 ? su[c] : sd[c];
-    if (d2) d2[c] = 0.5 * (ssu[c] + ssd[c]);
-  } else {
-    d1[c] = su[c];
-    if (d2) d2[c] = ssu[c];
+  const double *vel[3] = {u, v, w};
+  double *d1[3] = {d1x, d1y, d1z}, *d2[3] = {d2x, d2y, d2z};
+  for (int a = 0; a < ndim; ++a) {
+    if (UP) {
+      d1[a][c] = vel[a][c] > 0 ? sc.su[a][c] : sc.sd[a][c];
+      if (d2[a]) d2[a][c] = 0.5 * (sc.ssu[a][c] + sc.ssd[a][c]);
+    } else {
+      d1[a][c] = sc.su[a][c];
+      if (d2[a]) d2[a][c] = sc.ssu[a][c];
+    }
   }
 }
 
@@ -1307,7 +1325,7 @@ struct scihpc_ls {
   int scheme = 0;  // tvd_rk3 convection: 0 WENO5, 1 UCCD
   double *coef_ccd[3] = {nullptr, nullptr, nullptr}, *coef_up[3] = {nullptr, nullptr, nullptr};
   double *coef_dn[3] = {nullptr, nullptr, nullptr};
-  double *ls_scr[4] = {nullptr, nullptr, nullptr, nullptr};
+  double *ls_scr[3][4] = {};  // line scratch per axis: su, ssu, sd, ssd
   double *fxx = nullptr, *fyy = nullptr, *fzz = nullptr;
   double *sgn = nullptr, *dlt = nullptr, *grd = nullptr, *flux = nullptr;
   double *lpart = nullptr, *lhost = nullptr;
@@ -1326,7 +1344,8 @@ struct scihpc_ls {
       SCIHPC_HIP(hipMemcpyAsync(coef_dn[a], cd.data(), sizeof(double) * 12 * n, hipMemcpyHostToDevice, stream));
       SCIHPC_HIP(hipStreamSynchronize(stream));  // the host vectors go out of scope
     }
-    for (auto &q : ls_scr) q = alloc();
+    for (int a = 0; a < g.ndim; ++a)
+      for (auto &q : ls_scr[a]) q = alloc();
     fxx = alloc();
     fyy = alloc();
     fzz = alloc();
@@ -1341,29 +1360,34 @@ struct scihpc_ls {
 
   // derivatives of f along every real axis: UCCD by (u, v, w) or CCD
   void compact(const double *f, bool upwind, double *d1[3], double *d2[3]) {
-    const double *vel[3] = {u, v, w};
+    LineScratch sc{};
+    LineCoefs co{};
+    long work = 0;
+    const int nsys = upwind ? 2 : 1;
     for (int a = 0; a < g.ndim; ++a) {
-      const int t1 = (a + 1) % 3, t2 = (a + 2) % 3;
-      const long lines = (long)g.N[t1] * g.N[t2];
-      const unsigned nbc = ceil_div(g.total, kLsBlock);
-      const int nsys = upwind ? 2 : 1;
-      if (upwind)
-        k_compact_rhs<true><<<nbc, kLsBlock, 0, stream>>>(g, a, f, ls_scr[0], ls_scr[1], ls_scr[2], ls_scr[3]);
-      else
-        k_compact_rhs<false><<<nbc, kLsBlock, 0, stream>>>(g, a, f, ls_scr[0], ls_scr[1], nullptr, nullptr);
-      SCIHPC_CHECK_LAUNCH();
-      k_twin_line<<<ceil_div(lines * nsys, 64), 64, 0, stream>>>(g, a, nsys, upwind ? coef_up[a] : coef_ccd[a],
-                                                                  coef_dn[a], ls_scr[0], ls_scr[1], ls_scr[2],
-                                                                  ls_scr[3]);
-      SCIHPC_CHECK_LAUNCH();
-      if (upwind)
-        k_compact_select<true><<<nbc, kLsBlock, 0, stream>>>(g.total, vel[a], ls_scr[0], ls_scr[1], ls_scr[2],
-                                                             ls_scr[3], d1[a], d2 ? d2[a] : nullptr);
-      else
-        k_compact_select<false><<<nbc, kLsBlock, 0, stream>>>(g.total, nullptr, ls_scr[0], ls_scr[1], nullptr,
-                                                              nullptr, d1[a], d2 ? d2[a] : nullptr);
-      SCIHPC_CHECK_LAUNCH();
+      sc.su[a] = ls_scr[a][0];
+      sc.ssu[a] = ls_scr[a][1];
+      sc.sd[a] = ls_scr[a][2];
+      sc.ssd[a] = ls_scr[a][3];
+      co.cu[a] = upwind ? coef_up[a] : coef_ccd[a];
+      co.cd[a] = coef_dn[a];
+      work += (long)g.N[(a + 1) % 3] * g.N[(a + 2) % 3] * nsys;
     }
+    const unsigned nbc = ceil_div(g.total, kLsBlock);
+    if (upwind) k_compact_rhs<true><<<nbc, kLsBlock, 0, stream>>>(g, f, sc);
+    else k_compact_rhs<false><<<nbc, kLsBlock, 0, stream>>>(g, f, sc);
+    SCIHPC_CHECK_LAUNCH();
+    k_twin_line<<<ceil_div(work, 64), 64, 0, stream>>>(g, nsys, co, sc);
+    SCIHPC_CHECK_LAUNCH();
+    double *e2[3] = {nullptr, nullptr, nullptr};
+    double **dd = d2 ? d2 : e2;
+    if (upwind)
+      k_compact_select<true><<<nbc, kLsBlock, 0, stream>>>(g.total, g.ndim, u, v, w, sc, d1[0], d1[1], d1[2], dd[0],
+                                                           dd[1], dd[2]);
+    else
+      k_compact_select<false><<<nbc, kLsBlock, 0, stream>>>(g.total, g.ndim, nullptr, nullptr, nullptr, sc, d1[0],
+                                                            d1[1], d1[2], dd[0], dd[1], dd[2]);
+    SCIHPC_CHECK_LAUNCH();
   }
 
   // tvd_rk3 with the UCCD convection term, phi[cur] in place
@@ -1527,8 +1551,9 @@ extern "C" int scihpc_ls_destroy(scihpc_ls *h) {
   for (int a = 0; a < 3; ++a)
     for (double *p : {h->coef_ccd[a], h->coef_up[a], h->coef_dn[a]})
       if (p) (void)hipFree(p);
-  for (double *p : h->ls_scr)
-    if (p) (void)hipFree(p);
+  for (auto &axis : h->ls_scr)
+    for (double *p : axis)
+      if (p) (void)hipFree(p);
   if (h->lhost) (void)hipHostFree(h->lhost);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
