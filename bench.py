@@ -450,6 +450,19 @@ def main():
         assert st_ml["converged"], st_ml
         xml = g.sol(0)
         _, st_forced = run(PREC_MIXED, 1, 0, accel=1, learned=2)
+        log("bench: mixed + learned coarse-level correction of the defect (FP64-judged)")
+        for _ in range(args.warmup):
+            g.solve(rel_tol=tol, max_cycles=200, precision=PREC_MIXED, accel=1, learned_correction=1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        corr_kept = 0
+        for _ in range(args.steps):
+            st_corr = g.solve(rel_tol=tol, max_cycles=200, precision=PREC_MIXED, accel=1, learned_correction=1)
+            corr_kept += st_corr["correction_accepted"]
+        torch.cuda.synchronize()
+        ms_corr = (time.perf_counter() - t0) / args.steps * 1e3
+        assert st_corr["converged"], st_corr
+        st_corr_forced = g.solve(rel_tol=tol, max_cycles=200, precision=PREC_MIXED, accel=1, learned_correction=2)
         f, t, _ = g.learned_features(meta["level"], with_target=True)
         net.set_samples(f[None], t[None])
         ms_inf, fl_inf = net.time(1, train=False, iters=20)
@@ -463,12 +476,19 @@ def main():
                                    "rejected": st_every["learned_rejected"],
                                    "guess_rel_residual": st_every["learned_residual"]},
                    "cycles_if_guess_forced": st_forced["cycles"],
+                   "correction": {"ms_per_solve": round(ms_corr, 4), "cycles": st_corr["cycles"],
+                                  "kept": corr_kept, "of": args.steps,
+                                  "defect_rel_residual": st_corr["correction_residual"],
+                                  "cycles_if_forced": st_corr_forced["cycles"]},
                    "rel_l2_vs_fp64": float(np.linalg.norm(xml - x64) / np.linalg.norm(x64)),
                    "inference_us": round(ms_inf * 1e3, 2),
                    "inference_mfma_tflops": round(fl_inf / (ms_inf * 1e-3) / 1e12, 3)}
         log(f"bench: mixed+learned {ms_ml:.3f} ms/solve (network on {args.steps - skipped} of {args.steps} "
             f"solves, {accepted} accepted); judged every solve {ms_every:.3f} ms, guess residual "
             f"{st_every['learned_residual']:.3e}")
+        log(f"bench: learned correction {ms_corr:.3f} ms/solve, {st_corr['cycles']} cycles, kept {corr_kept} of "
+            f"{args.steps}, ||r - alpha A c|| / ||r|| = {st_corr['correction_residual']:.3e}; forced: "
+            f"{st_corr_forced['cycles']} cycles")
         g.attach_learned(None)
 
     # level-0 kernels, HIP events on the solver's stream

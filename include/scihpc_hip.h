@@ -141,6 +141,15 @@ typedef struct {
   int accel;         /* 0: stationary defect correction (one V-cycle per
                         correction); 1: flexible CG (FCG(1)) with the V-cycle
                         as preconditioner, FP64 vectors and inner products */
+  int learned_correction; /* mixed flexible-CG solves with a network attached:
+                        1: after the first iteration the network maps the
+                        current FP64 defect r (its block means on the
+                        network's level, with ln rho) to a coarse-level
+                        correction c = alpha S P S N(r) (alpha energy-
+                        optimal); x += c is kept only if it lowers the FP64
+                        residual, and then the CG recurrence restarts from
+                        the corrected x; 2: kept unjudged (ablation);
+                        0: off.  No counterpart in the reference. */
 } scihpc_solve_opts;
 
 typedef struct {
@@ -151,6 +160,9 @@ typedef struct {
   double ms;             /* device time of the solve (HIP events) */
   double learned_residual; /* ||b - A p0|| / ||b|| of the learned guess (0 if none) */
   int learned_skipped;     /* 1: the back-off skipped the network on this solve */
+  int correction_accepted; /* learned coarse-level corrections kept (learned_correction) */
+  int correction_rejected; /* ... discarded by the FP64 residual test */
+  double correction_residual; /* ||r - alpha A c|| / ||r|| of the correction (0 if none) */
 } scihpc_solve_stats;
 
 /* Solve A p = b on level 0 to opts->rel_tol (the call behind

@@ -27,13 +27,15 @@ class ScihpcError(RuntimeError):
 class SolveOpts(C.Structure):
     _fields_ = [("rel_tol", C.c_double), ("max_cycles", C.c_int), ("nu1", C.c_int),
                 ("nu2", C.c_int), ("precision", C.c_int), ("use_learned", C.c_int),
-                ("zero_guess", C.c_int), ("accel", C.c_int)]
+                ("zero_guess", C.c_int), ("accel", C.c_int), ("learned_correction", C.c_int)]
 
 
 class SolveStats(C.Structure):
     _fields_ = [("cycles", C.c_int), ("learned_accepted", C.c_int),
                 ("learned_rejected", C.c_int), ("rel_residual", C.c_double), ("ms", C.c_double),
-                ("learned_residual", C.c_double), ("learned_skipped", C.c_int)]
+                ("learned_residual", C.c_double), ("learned_skipped", C.c_int),
+                ("correction_accepted", C.c_int), ("correction_rejected", C.c_int),
+                ("correction_residual", C.c_double)]
 
 
 class FpcParams(C.Structure):

@@ -1546,6 +1546,12 @@ __global__ void __launch_bounds__(kBlock) k_guess_dots(DLevel<double> L, const d
   }
 }
 
+// x += y
+__global__ void __launch_bounds__(kBlock) k_add(double *__restrict__ x, const double *__restrict__ y, long n) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < n) x[c] += y[c];
+}
+
 __global__ void __launch_bounds__(kBlock) k_scale(double *__restrict__ x, long n, double a) {
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c < n) x[c] *= a;
@@ -2321,6 +2327,7 @@ struct scihpc_mg {
     double *r = mixed ? fr_ : l0.rhs;
     double rr = zero ? (bnorm > 0.0 ? 1.0 : 0.0) : std::sqrt(fetch(S_NORM)) / scale;
     int it = 0, cur = 0, spent = 0;
+    bool corr_left = mixed && o.learned_correction != 0 && learned.active();
     while (rr > o.rel_tol && it + spent < o.max_cycles) {
       // z = B r; from the second iteration on the level-0 post kernel also
       // forms z . q_old (S_ZQ) for the flexible direction update
@@ -2381,6 +2388,17 @@ struct scihpc_mg {
           rr = std::sqrt(fetch(S_NORM)) / scale;
         }
       }
+      // the learned coarse-level correction, once, after the first
+      // iteration whose guess pipeline (if any) has been judged
+      if (corr_left && !learned_pending && rr > o.rel_tol && it + spent < o.max_cycles) {
+        corr_left = false;
+        if (learned_correct(o.learned_correction, st, rr * scale, x)) {
+          spent += it;
+          it = 0;
+          cur = 0;
+          rr = std::sqrt(fetch(S_NORM)) / scale;
+        }
+      }
     }
     if (learned_overlap && learned_pending) {  // converged before the guess was ready: unused
       SCIHPC_HIP(hipEventSynchronize(lev));
@@ -2401,6 +2419,9 @@ struct scihpc_mg {
     SCIHPC_REQUIRE(o.nu1 <= 2 && o.nu2 <= 2, "nu1, nu2 <= 2 (tiled kernels stage a 5-cell halo)");
     SCIHPC_REQUIRE(o.accel == 0 || o.accel == 1, "accel must be 0 (stationary) or 1 (flexible CG)");
     SCIHPC_REQUIRE(o.use_learned >= 0 && o.use_learned <= 3, "use_learned must be 0..3");
+    SCIHPC_REQUIRE(o.learned_correction >= 0 && o.learned_correction <= 2, "learned_correction must be 0..2");
+    SCIHPC_REQUIRE(o.learned_correction == 0 || (o.accel == 1 && o.precision == SCIHPC_PREC_MIXED),
+                   "learned_correction needs a mixed-precision flexible-CG solve (accel = 1)");
     auto &l0 = d[0];
     const unsigned nb = ceil_div(l0.n, kBlock);
     SCIHPC_HIP(hipEventRecord(ev0, stream));
@@ -2616,6 +2637,18 @@ struct scihpc_mg {
   // `threshold` (or always when `force`); a kept guess is scaled by alpha in
   // l0.sol on the main stream
   bool learned_decide(scihpc_solve_stats &st, double bnorm, double threshold, bool force) {
+    double rel = -1.0;
+    const bool keep = learned_judge(bnorm, threshold, force, rel);
+    st.learned_residual = rel;
+    if (keep && !force) st.learned_accepted++;
+    if (!keep) st.learned_rejected++;
+    return keep;
+  }
+  // the pending pipeline's output p against right-hand side g (||g|| =
+  // gnorm, g in d[0].rhs when it ran): alpha = (p.g)/(p.A p) and rel =
+  // ||g - alpha A p|| / ||g|| from its four inner products; when kept (rel <
+  // threshold, or force) alpha p is left in d[0].sol on the main stream
+  bool learned_judge(double gnorm, double threshold, bool force, double &rel) {
     SCIHPC_REQUIRE(learned_pending, "no learned guess pending");
     learned_pending = false;
     SCIHPC_HIP(hipEventSynchronize(lev));
@@ -2623,22 +2656,55 @@ struct scihpc_mg {
     double s[4] = {0, 0, 0, 0};
     for (int k = 0; k < 4; ++k)
       for (unsigned q = 0; q < nb; ++q) s[k] += lhost[k * nb + q];
-    const double b2 = bnorm * bnorm;
+    const double b2 = gnorm * gnorm;
     const double alpha = s[1] != 0.0 ? s[0] / s[1] : 0.0;
     const double r2 = b2 - 2.0 * alpha * s[2] + alpha * alpha * s[3];
     const bool finite = std::isfinite(alpha) && std::isfinite(r2) && alpha != 0.0;
-    const double rel = finite && b2 > 0.0 ? std::sqrt(std::fabs(r2) / b2) : -1.0;
-    st.learned_residual = rel;
-    const bool keep = rel < threshold;
-    if (finite && (force || keep)) {
+    rel = finite && b2 > 0.0 ? std::sqrt(std::fabs(r2) / b2) : -1.0;
+    if (finite && (force || rel < threshold)) {
       SCIHPC_HIP(hipStreamWaitEvent(stream, lev, 0));
       k_scale<<<nb, kBlock, 0, stream>>>(d[0].sol, d[0].n, alpha);
       SCIHPC_CHECK_LAUNCH();
-      if (!force) st.learned_accepted++;
       return true;
     }
-    st.learned_rejected++;
     return false;
+  }
+
+  // The network as a coarse-level correction inside the mixed flexible-CG
+  // refinement loop: the guess pipeline run on the current FP64 defect r
+  // (r stands in for b in d[0].rhs while it runs), c = alpha S P S N(r),
+  // judged by ||r - alpha A c|| < ||r||.  A kept correction moves x to
+  // x + c and restarts the recurrence from r = b - A x (the directions so
+  // far are conjugate for the old iterate only).  Returns true when kept.
+  bool learned_correct(int mode, scihpc_solve_stats &st, double rnorm, double *x) {
+    auto &l0 = d[0];
+    double *b = l0.rhs;
+    l0.rhs = fr_;
+    double rel = -1.0;
+    bool keep = false;
+    try {
+      learned_enqueue(rnorm);
+      keep = learned_judge(rnorm, 1.0, mode == 2, rel);
+    } catch (...) {
+      l0.rhs = b;
+      throw;
+    }
+    l0.rhs = b;
+    st.correction_residual = rel;
+    if (!keep) {
+      st.correction_rejected++;
+      return false;
+    }
+    if (mode != 2) st.correction_accepted++;
+    const long n = l0.n;
+    const unsigned nb = ceil_div(n, kBlock);
+    k_add<<<nb, kBlock, 0, stream>>>(x, l0.sol, n);
+    SCIHPC_CHECK_LAUNCH();
+    SCIHPC_HIP(hipMemcpyAsync(l0.sol, x, sizeof(double) * n, hipMemcpyDeviceToDevice, stream));
+    reduce_residual(l0, true);
+    k_copy_cast<<<nb, kBlock, 0, stream>>>(l0.res, fr_, f[0].rhs, n);
+    SCIHPC_CHECK_LAUNCH();
+    return true;
   }
 };
 

@@ -15,7 +15,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 from oracle import unet_oracle as U
-from scihpc_amd import PREC_FP64, PREC_MIXED, Multigrid, UNet
+from scihpc_amd import PREC_FP64, PREC_MIXED, Multigrid, ScihpcError, UNet
 from scihpc_amd import _capi
 from scihpc_amd import learned as LRN
 from scihpc_amd import workloads as W
@@ -199,6 +199,51 @@ def test_learned_guess_fallback_on_bad_network():
     mg.attach_learned(None)
     st = mg.solve(rel_tol=1e-8, precision=PREC_FP64, accel=0, use_learned=True)
     assert st["learned_rejected"] == 0 and st["learned_accepted"] == 0
+
+
+def test_learned_correction_judged_and_fallback():
+    """learned_correction: the network's coarse-level correction of the FP64
+    defect after the first flexible-CG iteration.  A correction the FP64
+    residual rejects (NaN weights) leaves the iteration untouched, so the
+    result is the plain solve's; a forced one (2) moves the iterate and
+    restarts the recurrence, and the solve still reaches the tolerance; a
+    judged one from random weights is counted either way.  Other solve
+    modes refuse the option."""
+    w = W.bubble_2d(64, 128, seed=17)
+    mg = Multigrid(2, 64, 128, 1, 1 / 64, 2 / 128, 1.0, "neumann")
+    mg.set_density(w["rho"])
+    mg.set_rhs(w["rhs"])
+    plain = mg.solve(rel_tol=1e-8, precision=PREC_MIXED, accel=1)
+    x_plain = mg.sol(0)
+    H, Wd = mg.shapes[1][1], mg.shapes[1][0]
+    net = UNet(H, Wd, 1)
+    n_par = net.num_params
+    net.set_params(np.full(n_par, np.nan, np.float32))
+    mg.attach_learned(net, 1)
+    st = mg.solve(rel_tol=1e-8, precision=PREC_MIXED, accel=1, learned_correction=1)
+    assert st["converged"] and st["correction_rejected"] == 1 and st["correction_accepted"] == 0
+    assert st["cycles"] == plain["cycles"]
+    np.testing.assert_array_equal(mg.sol(0), x_plain)
+    rng = np.random.default_rng(5)
+    net.set_params((0.05 * rng.standard_normal(n_par)).astype(np.float32))
+    st = mg.solve(rel_tol=1e-8, precision=PREC_MIXED, accel=1, learned_correction=2)
+    assert st["converged"] and st["rel_residual"] <= 1e-8
+    assert st["correction_accepted"] == 0 and st["correction_rejected"] == 0
+    st = mg.solve(rel_tol=1e-8, precision=PREC_MIXED, accel=1, learned_correction=1)
+    assert st["converged"] and st["rel_residual"] <= 1e-8
+    assert st["correction_accepted"] + st["correction_rejected"] == 1
+    if st["correction_rejected"]:
+        assert st["correction_residual"] >= 1.0 or st["correction_residual"] < 0
+        assert st["cycles"] == plain["cycles"]
+    else:
+        assert 0.0 <= st["correction_residual"] < 1.0
+    with pytest.raises(ScihpcError):
+        mg.solve(rel_tol=1e-8, precision=PREC_FP64, accel=1, learned_correction=1)
+    with pytest.raises(ScihpcError):
+        mg.solve(rel_tol=1e-8, precision=PREC_MIXED, accel=0, learned_correction=1)
+    mg.attach_learned(None)
+    st = mg.solve(rel_tol=1e-8, precision=PREC_MIXED, accel=1, learned_correction=1)
+    assert st["correction_accepted"] == 0 and st["correction_rejected"] == 0
 
 
 def test_learned_guess_backoff_after_rejections():
