@@ -574,7 +574,7 @@ def main():
                      "traffic": round(PRE2D_FP32_TRAFFIC_BYTES),
                      "kernel": "k_pre2d<float,true> level 0 (2 RB-GS sweeps + residual + restriction)",
                      "note": "largest HBM-facing kernel; the largest by time is the one-workgroup coarse-level "
-                             "cycle k_block_vcycle<float,2> (about 33 us per FCG iteration, on-chip, one CU)",
+                             "cycle k_block_vcycle<float,2> (about 25 us per FCG iteration, on-chip, one CU)",
                      "bytes_per_launch": kbytes, "us_per_launch": round(kms * 1e3, 3)},
         "unet": unet,
         "momentum": momentum,
