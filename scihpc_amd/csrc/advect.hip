@@ -268,6 +268,31 @@ __global__ void __launch_bounds__(kLsBlock) k_bc_axis(Grid g, double *f, BcSpec 
   bc_line(p, g.s[a], g.n[a], g.N[a], bc.type[2 * a], false, face, bc.value[2 * a], g.h[a]);
 }
 
+// k_bc_axis for up to three fields in one launch (thread t: field t /
+// lines, line t % lines).  A field's BC on one axis touches only that
+// field, so the fields of one axis are independent; the axes stay in
+// separate launches, in the reference's order.
+struct BcFields {
+  double *f[3];
+  BcSpec bc[3];
+  int stag[3];
+  int nf;
+};
+__global__ void __launch_bounds__(kLsBlock) k_bc_axis_fields(Grid g, BcFields m, int a) {
+  const int t1 = (a + 1) % 3, t2 = (a + 2) % 3;
+  const long lines = (long)g.n[t1] * g.n[t2];
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= lines * m.nf) return;
+  const int q = (int)(t / lines);
+  const long l = t - (long)q * lines;
+  const int u = (int)(l / g.n[t2]), v = (int)(l % g.n[t2]);
+  double *p = m.f[q] + (long)(u + g.off[t1]) * g.s[t1] + (long)(v + g.off[t2]) * g.s[t2];
+  const bool face = m.stag[q] == a;
+  const BcSpec &bc = m.bc[q];
+  bc_line(p, g.s[a], g.n[a], g.N[a], bc.type[2 * a + 1], true, face, bc.value[2 * a + 1], g.h[a]);
+  bc_line(p, g.s[a], g.n[a], g.N[a], bc.type[2 * a], false, face, bc.value[2 * a], g.h[a]);
+}
+
 // find_density: rho = H + (1-H) ratio (or mirrored), lsm.cpp:8-18
 __global__ void __launch_bounds__(kLsBlock) k_density(const double *__restrict__ phi, long n, double w, double ratio,
                                                       int positive_ref, double *__restrict__ rho) {
@@ -1819,6 +1844,17 @@ struct scihpc_proj {
       SCIHPC_CHECK_LAUNCH();
     }
   }
+  // the BCs of three fields (u, v, w or their node values), one launch per
+  // axis instead of one per field and axis; the same result as three bc()
+  void bc3(double *f0, double *f1, double *f2, const BcSpec *b, int s0, int s1, int s2) {
+    BcFields m{{f0, f1, f2}, {b[0], b[1], b[2]}, {s0, s1, s2}, 3};
+    for (int a = 0; a < g.ndim; ++a) {
+      const int t1 = (a + 1) % 3, t2 = (a + 2) % 3;
+      const long lines = (long)g.n[t1] * g.n[t2];
+      k_bc_axis_fields<<<ceil_div(3 * lines, kLsBlock), kLsBlock, 0, stream>>>(g, m, a);
+      SCIHPC_CHECK_LAUNCH();
+    }
+  }
   void h2d(double *dst, const double *src) {
     SCIHPC_HIP(hipMemcpyAsync(dst, src, sizeof(double) * g.total, hipMemcpyHostToDevice, stream));
   }
@@ -1873,7 +1909,7 @@ struct scihpc_proj {
         k_to_face<3><<<nb, kLsBlock, 0, stream>>>(g, a, u, v, w, nvel[0], nvel[1], nvel[2]);
       SCIHPC_CHECK_LAUNCH();
       // nvel->apply_vel_a_bc (wrapper.cpp:40-59)
-      for (int b = 0; b < 3; ++b) bc(nvel[b], vbc[b], a);
+      bc3(nvel[0], nvel[1], nvel[2], vbc, a, a, a);
 #define SCIHPC_SRC(D, A)                                                                                 \
   k_mom_src<D, A><<<nb, kLsBlock, 0, stream>>>(g, fp, vel[A], nvel[0], nvel[1], nvel[2], dl, rho, mu, lf[0], \
                                                lf[1], lf[2], curv, src_alt[A])
@@ -1894,9 +1930,7 @@ struct scihpc_proj {
     k_ab2<<<nb, kLsBlock, 0, stream>>>(g.total, fp.dt, u, v, w, sn[0], sn[1], sn[2], so[0], so[1], so[2]);
     SCIHPC_CHECK_LAUNCH();
     for (int a = 0; a < g.ndim; ++a) std::swap(src[a], src_alt[a]);
-    bc(u, vbc[0], 0);
-    bc(v, vbc[1], 1);
-    bc(w, vbc[2], 2);
+    bc3(u, v, w, vbc, 0, 1, 2);
   }
 
   // node_from_face + nvel->apply_nvel_bc (wrapper_func.cpp:264-293,
@@ -1906,7 +1940,7 @@ struct scihpc_proj {
     if (g.ndim == 2) k_node_from_face<2><<<nb, kLsBlock, 0, stream>>>(g, u, v, w, nvel[0], nvel[1], nvel[2]);
     else k_node_from_face<3><<<nb, kLsBlock, 0, stream>>>(g, u, v, w, nvel[0], nvel[1], nvel[2]);
     SCIHPC_CHECK_LAUNCH();
-    for (int a = 0; a < 3; ++a) bc(nvel[a], vbc[a], -1);
+    bc3(nvel[0], nvel[1], nvel[2], vbc, -1, -1, -1);
   }
 
   void finish_pressure_and_velocity(scihpc_mg *mg, const double *ph, double dt, double rho12) {
@@ -1920,9 +1954,7 @@ struct scihpc_proj {
     else
       k_correct_velocity<3><<<ceil_div(nv, kLsBlock), kLsBlock, 0, stream>>>(g, rho, p, ph, dt, rho12, u, v, w);
     SCIHPC_CHECK_LAUNCH();
-    bc(u, vbc[0], 0);
-    bc(v, vbc[1], 1);
-    bc(w, vbc[2], 2);
+    bc3(u, v, w, vbc, 0, 1, 2);
   }
 };
 
