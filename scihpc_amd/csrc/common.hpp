@@ -84,6 +84,48 @@ __device__ __forceinline__ T block_sum(T v) {
   return s;
 }
 
+// block_sum with the total returned to every thread
+template <typename T>
+__device__ __forceinline__ T block_allsum(T v) {
+  __shared__ T tot;
+  const T s = block_sum(v);
+  if (threadIdx.x == 0) tot = s;
+  __syncthreads();
+  const T r = tot;
+  __syncthreads();  // tot is reused by the next call
+  return r;
+}
+
+// two block_allsum in one pass (one LDS exchange instead of two; the same
+// summation order as block_sum for each)
+template <typename T>
+__device__ __forceinline__ void block_allsum2(T &a, T &b) {
+  __shared__ T pa[1024 / kWave], pb[1024 / kWave];
+  __shared__ T ta, tb;
+  const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+  a = wave_sum(a);
+  b = wave_sum(b);
+  if (lane == 0) {
+    pa[wid] = a;
+    pb[wid] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int nw = (blockDim.x + kWave - 1) / kWave;
+    T sa = 0, sb = 0;
+    for (int w = 0; w < nw; ++w) {
+      sa += pa[w];
+      sb += pb[w];
+    }
+    ta = sa;
+    tb = sb;
+  }
+  __syncthreads();
+  a = ta;
+  b = tb;
+  __syncthreads();  // ta, tb are reused by the next call
+}
+
 inline unsigned ceil_div(long a, long b) { return (unsigned)((a + b - 1) / b); }
 
 }  // namespace scihpc

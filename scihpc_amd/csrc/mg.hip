@@ -1328,14 +1328,29 @@ __global__ void __launch_bounds__(kBlock) k_dot_zq(const T *__restrict__ z, cons
 }
 
 // p = z + beta p_old (beta = 0 on the first iteration), q = A p, partial sums
-// of p.q and p.r
+// of p.q and p.r.  beta = -(z.q_old)/(p_old.q_old): z.q_old from scal[S_ZQ],
+// or, with zq_n > 0, reduced here from the post-smoothing kernel's zq_n
+// partials in k_finalize's order (every block gets the same sum), which
+// saves the separate reduction launch.
 template <typename T, int DIM, bool VAR>
 __global__ void __launch_bounds__(kBlock)
     k_fcg_direction(DLevel<double> L, const T *__restrict__ z, const double *__restrict__ p_old,
                     double *__restrict__ p, double *__restrict__ q, const double *__restrict__ r,
-                    const double *__restrict__ scal, int first, double *__restrict__ partial) {
+                    const double *__restrict__ scal, int first, double *__restrict__ partial,
+                    const double *__restrict__ zq_part, int zq_n) {
   const long c = (long)xcd_block() * blockDim.x + threadIdx.x;
-  const double beta = first ? 0.0 : -scal[S_ZQ] / scal[S_PQ];
+  double beta = 0.0;
+  if (!first) {
+    double zq;
+    if (zq_n > 0) {
+      double s = 0.0;
+      for (int b = threadIdx.x; b < zq_n; b += blockDim.x) s += zq_part[b];
+      zq = block_allsum(s);
+    } else {
+      zq = scal[S_ZQ];
+    }
+    beta = -zq / scal[S_PQ];
+  }
   double vpq = 0.0, vpr = 0.0;
   if (c < L.n) {
     int i, j, k;
@@ -1385,15 +1400,36 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // x += alpha p, r -= alpha q (alpha = p.r / p.q); r32 = float(r) when given;
-// partial sums of r.r
+// partial sums of r.r.  p.q and p.r from scal, or, with dir_nb > 0, reduced
+// here from k_fcg_direction's partials (p.q at [0, dir_nb), p.r after them)
+// in k_finalize2's order, block 0 storing them in scal for the next beta.
+// partial must not overlap dir_part.
 __global__ void __launch_bounds__(kBlock)
     k_fcg_update(long n, double *__restrict__ x, double *__restrict__ r, const double *__restrict__ p,
-                 const double *__restrict__ q, const double *__restrict__ scal, float *__restrict__ r32,
-                 double *__restrict__ partial) {
+                 const double *__restrict__ q, double *__restrict__ scal, float *__restrict__ r32,
+                 double *__restrict__ partial, const double *__restrict__ dir_part, int dir_nb) {
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double pq, pr;
+  if (dir_nb > 0) {
+    double a = 0.0, b = 0.0;
+    for (int k = threadIdx.x; k < dir_nb; k += blockDim.x) {
+      a += dir_part[k];
+      b += dir_part[dir_nb + k];
+    }
+    block_allsum2(a, b);
+    pq = a;
+    pr = b;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      scal[S_PQ] = pq;
+      scal[S_PR] = pr;
+    }
+  } else {
+    pq = scal[S_PQ];
+    pr = scal[S_PR];
+  }
   double v = 0.0;
   if (c < n) {
-    const double alpha = scal[S_PR] / scal[S_PQ];
+    const double alpha = pr / pq;
     x[c] += alpha * p[c];
     const double rc = r[c] - alpha * q[c];
     r[c] = rc;
@@ -2342,11 +2378,18 @@ struct scihpc_mg {
         np = vcycle(d, o.nu1, o.nu2, true, it > 0 ? 3 : 0, oa);
         z = l0.sol;
       }
-      if (it > 0) finalize(np, S_ZQ);
+      // small reductions are folded into the consumers: z.q_old into the
+      // direction kernel, p.q and p.r into the update (its own partials go
+      // to [0, nb), the direction's to [2 nb, 4 nb))
+      const bool fuse = nb <= kCompressAbove && np <= kCompressAbove;
+      if (it > 0 && !fuse) finalize(np, S_ZQ);
       double *pold = fp_[cur], *pnew = fp_[cur ^ 1];
+      double *dpart = fuse ? partial + 2 * (long)nb : partial;
+      const double *zq_part = (fuse && it > 0) ? partial : nullptr;
+      const int zq_n = (fuse && it > 0) ? np : 0;
 #define SCIHPC_FCG_DIR(T, D, V)                                                                                  \
   k_fcg_direction<T, D, V><<<nb, kBlock, 0, stream>>>(view(l0), (const T *)z, pold, pnew, fq_, r, scal, it == 0, \
-                                                       partial)
+                                                       dpart, zq_part, zq_n)
       if (mixed) {
         if (ndim == 2) { if (variable) SCIHPC_FCG_DIR(float, 2, true); else SCIHPC_FCG_DIR(float, 2, false); }
         else { if (variable) SCIHPC_FCG_DIR(float, 3, true); else SCIHPC_FCG_DIR(float, 3, false); }
@@ -2356,9 +2399,10 @@ struct scihpc_mg {
       }
 #undef SCIHPC_FCG_DIR
       SCIHPC_CHECK_LAUNCH();
-      finalize2(nb, S_PQ, S_PR);
+      if (!fuse) finalize2(nb, S_PQ, S_PR);
       cur ^= 1;
-      k_fcg_update<<<nb, kBlock, 0, stream>>>(n, x, r, fp_[cur], fq_, scal, mixed ? f[0].rhs : nullptr, partial);
+      k_fcg_update<<<nb, kBlock, 0, stream>>>(n, x, r, fp_[cur], fq_, scal, mixed ? f[0].rhs : nullptr, partial,
+                                              fuse ? dpart : nullptr, fuse ? (int)nb : 0);
       SCIHPC_CHECK_LAUNCH();
       // the learned guess is enqueued while the first iteration runs
       if (learned_overlap && !learned_pending) learned_enqueue(bnorm);
