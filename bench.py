@@ -217,6 +217,31 @@ def momentum_workload(nx=MOM_SHAPE[0], ny=MOM_SHAPE[1]):
 MOM_BYTES_PER_CELL_2D = 8 * (6 + 2 * (4 + 9) + 12)
 
 
+def bench_hbm_ceiling(torch, mib=256, reps=20):
+    """Measured streaming ceilings of this GPU, for reading the roofline
+    fractions: a device-to-device copy of `mib` MiB (read + write) and a
+    three-read one-write FP64 elementwise pass (torch.addcmul), HIP events."""
+    n = mib * 2 ** 20 // 8
+    a, b, c = (torch.randn(n, dtype=torch.float64, device="cuda") for _ in range(3))
+    o = torch.empty_like(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    out = {}
+    for name, fn, arrays in (("copy", lambda: o.copy_(a), 2), ("three_read_one_write", lambda: torch.addcmul(
+            a, b, c, out=o), 4)):
+        fn()
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        s = e0.elapsed_time(e1) * 1e-3 / reps
+        out[name + "_GBps"] = round(arrays * 8 * n / s / 1e9, 1)
+    out["bytes_per_array"] = 8 * n
+    del a, b, c, o
+    return out
+
+
 def bench_momentum(iters=20):
     """find_source_sec + find_intermediate_velocity at the dam-break shape:
     the device time of one step on resident fields against HBM."""
@@ -533,6 +558,10 @@ def main():
         log(f"bench: uccd rk3 {interface['uccd_rk3_step_ms']} ms, mpls x5 {interface['mpls_5_steps_ms']} ms, "
             f"redistance {interface['redistance_rk3_step_ms']} ms")
 
+    log("bench: streaming ceilings (device copy, three-read one-write pass)")
+    ceiling = bench_hbm_ceiling(torch)
+    log(f"bench: copy {ceiling['copy_GBps']} GB/s, 3R1W {ceiling['three_read_one_write_GBps']} GB/s")
+
     snapshot = None
     if not args.no_snapshot:
         log("bench: VTK snapshot of the 512x1024 grid")
@@ -575,7 +604,8 @@ def main():
                      "kernel": "k_pre2d<float,true> level 0 (2 RB-GS sweeps + residual + restriction)",
                      "note": "largest HBM-facing kernel; the largest by time is the one-workgroup coarse-level "
                              "cycle k_block_vcycle<float,2> (about 25 us per FCG iteration, on-chip, one CU)",
-                     "bytes_per_launch": kbytes, "us_per_launch": round(kms * 1e3, 3)},
+                     "bytes_per_launch": kbytes, "us_per_launch": round(kms * 1e3, 3),
+                     "measured_ceiling": ceiling},
         "unet": unet,
         "momentum": momentum,
         "interface": interface,
