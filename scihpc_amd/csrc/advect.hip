@@ -804,7 +804,7 @@ __device__ __forceinline__ void compact_rhs(const double *__restrict__ f, long s
 // the twin sweeps load a chunk of kLineChunk positions (right-hand sides
 // and coefficients) before running the recurrence on it, so the loads of a
 // chunk overlap instead of serialising with the recurrence
-constexpr int kLineChunk = 8;
+constexpr int kLineChunk = 16;
 
 // The line systems in three passes, each one launch for every axis: the
 // right-hand sides of every padded cell (all divisions of the scheme's
