@@ -237,6 +237,12 @@ def test_learned_correction_judged_and_fallback():
         assert st["cycles"] == plain["cycles"]
     else:
         assert 0.0 <= st["correction_residual"] < 1.0
+    # with the guess overlapping the first iterations as well: the correction
+    # waits until the guess is judged, and both verdicts are counted
+    st = mg.solve(rel_tol=1e-8, precision=PREC_MIXED, accel=1, use_learned=3, learned_correction=1)
+    assert st["converged"] and st["rel_residual"] <= 1e-8
+    assert st["learned_accepted"] + st["learned_rejected"] == 1
+    assert st["correction_accepted"] + st["correction_rejected"] <= 1
     with pytest.raises(ScihpcError):
         mg.solve(rel_tol=1e-8, precision=PREC_FP64, accel=1, learned_correction=1)
     with pytest.raises(ScihpcError):
