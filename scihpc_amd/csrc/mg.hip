@@ -1399,6 +1399,76 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+// 3-D k_fcg_direction marching in z: thread t of a kZmX x kZmY block owns
+// column (i, j) = (bx kZmX + t % kZmX, by kZmY + t / kZmX) of a chunk of
+// planes.  p = z + beta p_old of the plane below and above comes from a
+// register window (formed once per cell, as k_fcg_direction forms it), the
+// row and column neighbours through the caches, so each plane of z and
+// p_old leaves HBM about once.  Per-cell arithmetic as k_fcg_direction's;
+// the partial sums add a column's cells first (partial[b], partial[nb + b],
+// b the linear block index, nb the block count).
+constexpr int kZmX = 64, kZmY = 4;
+template <typename T, bool VAR>
+__global__ void __launch_bounds__(kZmX *kZmY)
+    k_fcg_direction_zm(DLevel<double> L, const T *__restrict__ z, const double *__restrict__ p_old,
+                       double *__restrict__ p, double *__restrict__ q, const double *__restrict__ r,
+                       const double *__restrict__ scal, int first, double *__restrict__ partial, int zchunk) {
+  const double beta = first ? 0.0 : -scal[S_ZQ] / scal[S_PQ];
+  const int i = blockIdx.x * kZmX + (int)threadIdx.x % kZmX, j = blockIdx.y * kZmY + (int)threadIdx.x / kZmX;
+  const int k0 = blockIdx.z * zchunk, k1 = min(k0 + zchunk, L.nz);
+  const long sy = L.nx, sz = (long)L.nx * L.ny;
+  auto P = [&](long g) { return double(z[g]) + (first ? 0.0 : beta * p_old[g]); };
+  double vpq = 0.0, vpr = 0.0;
+  if (i < L.nx && j < L.ny && k0 < k1) {
+    long c = i + sy * j + sz * k0;
+    double pb = k0 > 0 ? P(c - sz) : 0.0, pc = P(c);
+    double czb = 0.0;
+    if constexpr (VAR) czb = k0 > 0 ? L.cz[c - sz] : 0.0;
+    for (int k = k0; k < k1; ++k, c += sz) {
+      const double pf = k < L.nz - 1 ? P(c + sz) : 0.0;
+      double off = 0.0, diag;
+      if constexpr (VAR) {
+        const double cR = L.cx[c], cL = i > 0 ? L.cx[c - 1] : 0.0;
+        const double cU = L.cy[c], cD = j > 0 ? L.cy[c - sy] : 0.0;
+        if (i > 0) off += cL * P(c - 1);
+        if (i < L.nx - 1) off += cR * P(c + 1);
+        if (j > 0) off += cD * P(c - sy);
+        if (j < L.ny - 1) off += cU * P(c + sy);
+        diag = -(cR + cL + cU + cD);
+        const double cF = L.cz[c], cB = k > 0 ? czb : 0.0;
+        if (k > 0) off += cB * pb;
+        if (k < L.nz - 1) off += cF * pf;
+        diag -= (cF + cB);
+        czb = cF;
+      } else {
+        double cc = 0.0;
+        if (i > 0) { off += L.ax * P(c - 1); cc += L.ax; }
+        if (i < L.nx - 1) { off += L.ax * P(c + 1); cc += L.ax; }
+        if (j > 0) { off += L.ay * P(c - sy); cc += L.ay; }
+        if (j < L.ny - 1) { off += L.ay * P(c + sy); cc += L.ay; }
+        if (k > 0) { off += L.az * pb; cc += L.az; }
+        if (k < L.nz - 1) { off += L.az * pf; cc += L.az; }
+        diag = L.dirichlet ? L.dconst : -cc;
+      }
+      const double qc = off + diag * pc;
+      p[c] = pc;
+      q[c] = qc;
+      vpq += pc * qc;
+      vpr += pc * r[c];
+      pb = pc;
+      pc = pf;
+    }
+  }
+  vpq = block_sum(vpq);
+  vpr = block_sum(vpr);
+  if (threadIdx.x == 0) {
+    const unsigned nb = gridDim.x * gridDim.y * gridDim.z;
+    const unsigned b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    partial[b] = vpq;
+    partial[nb + b] = vpr;
+  }
+}
+
 // x += alpha p, r -= alpha q (alpha = p.r / p.q); r32 = float(r) when given;
 // partial sums of r.r.  p.q and p.r from scal, or, with dir_nb > 0, reduced
 // here from k_fcg_direction's partials (p.q at [0, dir_nb), p.r after them)
@@ -2364,6 +2434,20 @@ struct scihpc_mg {
     double rr = zero ? (bnorm > 0.0 ? 1.0 : 0.0) : std::sqrt(fetch(S_NORM)) / scale;
     int it = 0, cur = 0, spent = 0;
     bool corr_left = mixed && o.learned_correction != 0 && learned.active();
+    // 3-D levels of 2^18 cells and more: the direction kernel marches in z,
+    // about 16384 blocks (measured best of 1024..16384 at 256^3; never more
+    // partial sums than nb)
+    const bool zm = ndim == 3 && n >= (1L << 18);
+    dim3 zm_grid(1, 1, 1);
+    int zm_chunk = 1, ndir = (int)nb;
+    if (zm) {
+      const long bxy = (long)ceil_div(l0.nx, kZmX) * ceil_div(l0.ny, kZmY);
+      SCIHPC_REQUIRE(bxy <= (long)nb && (long)ceil_div(l0.ny, kZmY) <= 65535, "grid too flat for the z-march");
+      const long slabs = std::max(1L, std::min(16384L, (long)nb) / bxy);
+      zm_chunk = (int)ceil_div(l0.nz, slabs);
+      zm_grid = dim3(ceil_div(l0.nx, kZmX), ceil_div(l0.ny, kZmY), ceil_div(l0.nz, zm_chunk));
+      ndir = (int)(bxy * zm_grid.z);
+    }
     while (rr > o.rel_tol && it + spent < o.max_cycles) {
       // z = B r; from the second iteration on the level-0 post kernel also
       // forms z . q_old (S_ZQ) for the flexible direction update
@@ -2381,8 +2465,8 @@ struct scihpc_mg {
       // small reductions are folded into the consumers: z.q_old into the
       // direction kernel, p.q and p.r into the update (its own partials go
       // to [0, nb), the direction's to [2 nb, 4 nb))
-      const bool fuse = nb <= kCompressAbove && np <= kCompressAbove;
-      if (it > 0 && !fuse) finalize(np, S_ZQ);
+      const bool fuse = nb <= kCompressAbove && np <= kCompressAbove && ndir <= (int)nb;
+      if (it > 0 && (!fuse || zm)) finalize(np, S_ZQ);
       double *pold = fp_[cur], *pnew = fp_[cur ^ 1];
       double *dpart = fuse ? partial + 2 * (long)nb : partial;
       const double *zq_part = (fuse && it > 0) ? partial : nullptr;
@@ -2390,19 +2474,26 @@ struct scihpc_mg {
 #define SCIHPC_FCG_DIR(T, D, V)                                                                                  \
   k_fcg_direction<T, D, V><<<nb, kBlock, 0, stream>>>(view(l0), (const T *)z, pold, pnew, fq_, r, scal, it == 0, \
                                                        dpart, zq_part, zq_n)
-      if (mixed) {
+#define SCIHPC_FCG_ZM(T, V)                                                                              \
+  k_fcg_direction_zm<T, V><<<zm_grid, kZmX * kZmY, 0, stream>>>(view(l0), (const T *)z, pold, pnew, fq_, r, \
+                                                                 scal, it == 0, dpart, zm_chunk)
+      if (zm) {
+        if (mixed) { if (variable) SCIHPC_FCG_ZM(float, true); else SCIHPC_FCG_ZM(float, false); }
+        else { if (variable) SCIHPC_FCG_ZM(double, true); else SCIHPC_FCG_ZM(double, false); }
+      } else if (mixed) {
         if (ndim == 2) { if (variable) SCIHPC_FCG_DIR(float, 2, true); else SCIHPC_FCG_DIR(float, 2, false); }
         else { if (variable) SCIHPC_FCG_DIR(float, 3, true); else SCIHPC_FCG_DIR(float, 3, false); }
       } else {
         if (ndim == 2) { if (variable) SCIHPC_FCG_DIR(double, 2, true); else SCIHPC_FCG_DIR(double, 2, false); }
         else { if (variable) SCIHPC_FCG_DIR(double, 3, true); else SCIHPC_FCG_DIR(double, 3, false); }
       }
+#undef SCIHPC_FCG_ZM
 #undef SCIHPC_FCG_DIR
       SCIHPC_CHECK_LAUNCH();
-      if (!fuse) finalize2(nb, S_PQ, S_PR);
+      if (!fuse) finalize2(ndir, S_PQ, S_PR);
       cur ^= 1;
       k_fcg_update<<<nb, kBlock, 0, stream>>>(n, x, r, fp_[cur], fq_, scal, mixed ? f[0].rhs : nullptr, partial,
-                                              fuse ? dpart : nullptr, fuse ? (int)nb : 0);
+                                              fuse ? dpart : nullptr, fuse ? ndir : 0);
       SCIHPC_CHECK_LAUNCH();
       // the learned guess is enqueued while the first iteration runs
       if (learned_overlap && !learned_pending) learned_enqueue(bnorm);

@@ -276,3 +276,28 @@ def test_z3_sweeps_match_colour_kernels(oracle, case, precision, monkeypatch):
     assert stats[0]["converged"] and stats[1]["converged"]
     assert abs(stats[0]["cycles"] - stats[1]["cycles"]) <= (0 if precision == PREC_FP64 else 1)
     assert rel(sols[0][1], sols[1][1]) < (1e-10 if precision == PREC_FP64 else 1e-6)
+
+
+@pytest.mark.parametrize("variable", [True, False], ids=["var", "const"])
+def test_flexible_cg_3d_z_march(variable):
+    """3-D levels of 2^18 cells and more run the flexible-CG direction kernel
+    in its z-marching form: FP64 and mixed flexible CG reach the tolerance,
+    in no more iterations than the stationary solve, and agree with it."""
+    n = 64
+    if variable:
+        w = W.poisson_3d(n, variable=True)
+        g = Multigrid(3, n, n, n, 1 / n, 1 / n, 1 / n, "neumann")
+        g.set_density(w["rho"])
+    else:
+        w = W.poisson_3d(n, variable=False)
+        g = Multigrid(3, n, n, n, 1 / n, 1 / n, 1 / n, "dirichlet")
+    g.set_rhs(w["rhs"])
+    print(f"z-march FCG 3-D {n}^3 variable={variable}", flush=True)
+    ref = g.solve(rel_tol=1e-10, max_cycles=200, precision=PREC_FP64, accel=0)
+    assert ref["converged"]
+    x_ref = g.sol(0)
+    for prec in (PREC_FP64, PREC_MIXED):
+        st = g.solve(rel_tol=1e-10, max_cycles=200, precision=prec, accel=1)
+        assert st["converged"] and st["rel_residual"] <= 1e-10, st
+        assert st["cycles"] <= ref["cycles"]
+        assert rel(g.sol(0), x_ref) < 1e-8
