@@ -1469,6 +1469,74 @@ __global__ void __launch_bounds__(kZmX *kZmY)
   }
 }
 
+// 3-D k_residual marching in z (the layout of k_fcg_direction_zm): the
+// plane neighbours of x and the face below from a register window, the row
+// and column neighbours through the caches; each cell's residual rounds as
+// k_residual's (the same operations as stencil(), missing neighbours with a
+// zero coefficient), the partial sums add a column's cells first.
+template <bool VAR>
+__global__ void __launch_bounds__(kZmX *kZmY)
+    k_residual_zm(DLevel<double> L, int store, double *__restrict__ partial, const double *__restrict__ flag,
+                  int zchunk) {
+  if (flag && *flag != 0.0) return;
+  const int i = blockIdx.x * kZmX + (int)threadIdx.x % kZmX, j = blockIdx.y * kZmY + (int)threadIdx.x / kZmX;
+  const int k0 = blockIdx.z * zchunk, k1 = min(k0 + zchunk, L.nz);
+  const long sy = L.nx, sz = (long)L.nx * L.ny;
+  const double *x = L.sol;
+  double acc = 0.0;
+  if (i < L.nx && j < L.ny && k0 < k1) {
+    const bool hl = i > 0, hr = i < L.nx - 1, hd = j > 0, hu = j < L.ny - 1;
+    long c = i + sy * j + sz * k0;
+    double xb = k0 > 0 ? x[c - sz] : 0.0, xc = x[c];
+    double czb = 0.0;
+    if constexpr (VAR) czb = k0 > 0 ? L.cz[c - sz] : 0.0;
+    for (int k = k0; k < k1; ++k, c += sz) {
+      const bool hb = k > 0, hf = k < L.nz - 1;
+      const double xf = hf ? x[c + sz] : 0.0;
+      const long cl = hl ? c - 1 : c, cr = hr ? c + 1 : c, cd = hd ? c - sy : c, cu = hu ? c + sy : c;
+      const double xl = x[cl], xr = x[cr], xd = x[cd], xu = x[cu];
+      double off = 0.0, diag;
+      if constexpr (VAR) {
+        const double cR = L.cx[c], cLv = L.cx[cl], cU = L.cy[c], cDv = L.cy[cd];
+        const double cL = hl ? cLv : 0.0, cD = hd ? cDv : 0.0;
+        off += cL * xl;
+        off += (hr ? cR : 0.0) * xr;
+        off += cD * xd;
+        off += (hu ? cU : 0.0) * xu;
+        diag = -(cR + cL + cU + cD);
+        const double cF = L.cz[c], cB = hb ? czb : 0.0;
+        off += cB * xb;
+        off += (hf ? cF : 0.0) * xf;
+        diag -= (cF + cB);
+        czb = cF;
+      } else {
+        const double wx = L.rbwall ? L.wx : 0.0, wy = L.rbwall ? L.wy : 0.0, wz = L.rbwall ? L.wz : 0.0;
+        double cc = 0.0;
+        off += (hl ? L.ax : 0.0) * xl;
+        cc += hl ? L.ax : wx;
+        off += (hr ? L.ax : 0.0) * xr;
+        cc += hr ? L.ax : wx;
+        off += (hd ? L.ay : 0.0) * xd;
+        cc += hd ? L.ay : wy;
+        off += (hu ? L.ay : 0.0) * xu;
+        cc += hu ? L.ay : wy;
+        off += (hb ? L.az : 0.0) * xb;
+        cc += hb ? L.az : wz;
+        off += (hf ? L.az : 0.0) * xf;
+        cc += hf ? L.az : wz;
+        diag = (L.dirichlet && !L.rbwall) ? L.dconst : -cc;
+      }
+      const double rc = L.rhs[c] - (off + diag * xc);
+      if (store) L.res[c] = rc;
+      acc += rc * rc;
+      xb = xc;
+      xc = xf;
+    }
+  }
+  acc = block_sum(acc);
+  if (threadIdx.x == 0) partial[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = acc;
+}
+
 // x += alpha p, r -= alpha q (alpha = p.r / p.q); r32 = float(r) when given;
 // partial sums of r.r.  p.q and p.r from scal, or, with dir_nb > 0, reduced
 // here from k_fcg_direction's partials (p.q at [0, dir_nb), p.r after them)
@@ -2058,7 +2126,7 @@ struct scihpc_mg {
       return (int)nb;
     }
     if constexpr (std::is_same<T, double>::value) {
-      SCIHPC_DISPATCH(k_residual, double, nb, view(lv[0]), 0, partial, (const double *)nullptr);
+      return residual_partials(lv[0], false);
     } else {
       // e = lv[0].sol; write p_new and float(r) into oa.r32
 #define SCIHPC_OUTER(D, V)                                                                                     \
@@ -2132,10 +2200,39 @@ struct scihpc_mg {
     reduce_residual(l, store);
     return std::sqrt(fetch(S_NORM));
   }
-  void reduce_residual(const Level<double> &l, bool store, const double *flag = nullptr) {
+  // z-march layout of a 3-D level of 2^18 cells and more: about 16384
+  // blocks of kZmX x kZmY columns (never more than ceil(n / kBlock), the
+  // partial buffer's share); 0 when the level does not qualify
+  long zm_layout(const Level<double> &l, dim3 &grid, int &chunk) const {
+    if (ndim != 3 || l.n < (1L << 18)) return 0;
+    const long nb = ceil_div(l.n, kBlock);
+    const long bxy = (long)ceil_div(l.nx, kZmX) * ceil_div(l.ny, kZmY);
+    if (bxy > nb || ceil_div(l.ny, kZmY) > 65535) return 0;
+    const long slabs = std::max(1L, std::min(16384L, nb) / bxy);
+    chunk = (int)ceil_div(l.nz, slabs);
+    grid = dim3(ceil_div(l.nx, kZmX), ceil_div(l.ny, kZmY), ceil_div(l.nz, chunk));
+    return bxy * grid.z;
+  }
+  // b - A x on an FP64 level (res stored when asked), r.r partials in
+  // partial; returns their number
+  int residual_partials(const Level<double> &l, bool store, const double *flag = nullptr) {
+    dim3 grid;
+    int chunk = 1;
+    const long nz = zm_layout(l, grid, chunk);
+    if (nz > 0) {
+      if (variable) k_residual_zm<true><<<grid, kZmX * kZmY, 0, stream>>>(view(l, false), store ? 1 : 0, partial,
+                                                                          flag, chunk);
+      else k_residual_zm<false><<<grid, kZmX * kZmY, 0, stream>>>(view(l, false), store ? 1 : 0, partial, flag,
+                                                                  chunk);
+      SCIHPC_CHECK_LAUNCH();
+      return (int)nz;
+    }
     const unsigned nb = ceil_div(l.n, kBlock);
     SCIHPC_DISPATCH(k_residual, double, nb, view(l, false), store ? 1 : 0, partial, flag);
-    finalize(nb, S_NORM);
+    return (int)nb;
+  }
+  void reduce_residual(const Level<double> &l, bool store, const double *flag = nullptr) {
+    finalize(residual_partials(l, store, flag), S_NORM);
   }
 
   double vec_norm2(const double *x, long n) {
@@ -2434,20 +2531,13 @@ struct scihpc_mg {
     double rr = zero ? (bnorm > 0.0 ? 1.0 : 0.0) : std::sqrt(fetch(S_NORM)) / scale;
     int it = 0, cur = 0, spent = 0;
     bool corr_left = mixed && o.learned_correction != 0 && learned.active();
-    // 3-D levels of 2^18 cells and more: the direction kernel marches in z,
-    // about 16384 blocks (measured best of 1024..16384 at 256^3; never more
-    // partial sums than nb)
-    const bool zm = ndim == 3 && n >= (1L << 18);
+    // 3-D levels of 2^18 cells and more: the direction kernel marches in z
+    // (about 16384 blocks, the best of 1024..16384 measured at 256^3)
     dim3 zm_grid(1, 1, 1);
-    int zm_chunk = 1, ndir = (int)nb;
-    if (zm) {
-      const long bxy = (long)ceil_div(l0.nx, kZmX) * ceil_div(l0.ny, kZmY);
-      SCIHPC_REQUIRE(bxy <= (long)nb && (long)ceil_div(l0.ny, kZmY) <= 65535, "grid too flat for the z-march");
-      const long slabs = std::max(1L, std::min(16384L, (long)nb) / bxy);
-      zm_chunk = (int)ceil_div(l0.nz, slabs);
-      zm_grid = dim3(ceil_div(l0.nx, kZmX), ceil_div(l0.ny, kZmY), ceil_div(l0.nz, zm_chunk));
-      ndir = (int)(bxy * zm_grid.z);
-    }
+    int zm_chunk = 1;
+    const long nzm = zm_layout(l0, zm_grid, zm_chunk);
+    const bool zm = nzm > 0;
+    const int ndir = zm ? (int)nzm : (int)nb;
     while (rr > o.rel_tol && it + spent < o.max_cycles) {
       // z = B r; from the second iteration on the level-0 post kernel also
       // forms z . q_old (S_ZQ) for the flexible direction update
