@@ -1407,7 +1407,7 @@ __global__ void __launch_bounds__(kBlock)
 // p_old leaves HBM about once.  Per-cell arithmetic as k_fcg_direction's;
 // the partial sums add a column's cells first (partial[b], partial[nb + b],
 // b the linear block index, nb the block count).
-constexpr int kZmX = 64, kZmY = 4;
+constexpr int kZmX = 64, kZmY = 8;
 template <typename T, bool VAR>
 __global__ void __launch_bounds__(kZmX *kZmY)
     k_fcg_direction_zm(DLevel<double> L, const T *__restrict__ z, const double *__restrict__ p_old,
@@ -2200,15 +2200,16 @@ struct scihpc_mg {
     reduce_residual(l, store);
     return std::sqrt(fetch(S_NORM));
   }
-  // z-march layout of a 3-D level of 2^18 cells and more: about 16384
-  // blocks of kZmX x kZmY columns (never more than ceil(n / kBlock), the
-  // partial buffer's share); 0 when the level does not qualify
+  // z-march layout of a 3-D level of 2^18 cells and more: about 8192
+  // blocks of kZmX x kZmY columns (the best of 1024..16384 blocks and 4 or
+  // 8 rows measured at 256^3; never more than ceil(n / kBlock), the partial
+  // buffer's share); 0 when the level does not qualify
   long zm_layout(const Level<double> &l, dim3 &grid, int &chunk) const {
     if (ndim != 3 || l.n < (1L << 18)) return 0;
     const long nb = ceil_div(l.n, kBlock);
     const long bxy = (long)ceil_div(l.nx, kZmX) * ceil_div(l.ny, kZmY);
     if (bxy > nb || ceil_div(l.ny, kZmY) > 65535) return 0;
-    const long slabs = std::max(1L, std::min(16384L, nb) / bxy);
+    const long slabs = std::max(1L, std::min(8192L, nb) / bxy);
     chunk = (int)ceil_div(l.nz, slabs);
     grid = dim3(ceil_div(l.nx, kZmX), ceil_div(l.ny, kZmY), ceil_div(l.nz, chunk));
     return bxy * grid.z;
@@ -2532,7 +2533,6 @@ struct scihpc_mg {
     int it = 0, cur = 0, spent = 0;
     bool corr_left = mixed && o.learned_correction != 0 && learned.active();
     // 3-D levels of 2^18 cells and more: the direction kernel marches in z
-    // (about 16384 blocks, the best of 1024..16384 measured at 256^3)
     dim3 zm_grid(1, 1, 1);
     int zm_chunk = 1;
     const long nzm = zm_layout(l0, zm_grid, zm_chunk);
