@@ -1537,6 +1537,71 @@ __global__ void __launch_bounds__(kZmX *kZmY)
   if (threadIdx.x == 0) partial[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = acc;
 }
 
+// 3-D residual + child-average restriction on the A(rho) level 0, marching
+// in z (the layout of k_residual_zm): thread (lx, ly) owns fine column (i, j)
+// and takes two fine planes per coarse plane, x and rho of the plane below
+// and above in a register window, row and column neighbours through the
+// caches.  Each residual rounds as k_rr3d_pairs' (rho_face, z3_stencil);
+// the eight children are added as: the two planes in the thread, the x pair
+// by a lane exchange, the y pair (two rows of the block) through LDS.
+// zchunk is even; nx, ny, nz are even (factor-2 coarsening).
+template <typename T>
+__global__ void __launch_bounds__(kZmX *kZmY)
+    k_rr3d_zm(DLevel<T> F, DLevel<T> G, const T *__restrict__ rho, int zchunk) {
+  __shared__ T part[kZmY][kZmX];
+  const int lx = (int)threadIdx.x % kZmX, ly = (int)threadIdx.x / kZmX;
+  const int i = blockIdx.x * kZmX + lx, j = blockIdx.y * kZmY + ly;
+  const int k0 = blockIdx.z * zchunk, k1 = min(k0 + zchunk, F.nz);
+  const bool in = i < F.nx && j < F.ny;
+  const long sy = F.nx, sz = (long)F.nx * F.ny;
+  const T inf = T(INFINITY);
+  const T *x = F.sol;
+  long c = in ? i + sy * j + sz * k0 : 0;
+  T xb = T(0), xc = T(0), rb = inf, rcv = inf;
+  if (in) {
+    xb = k0 > 0 ? x[c - sz] : T(0);
+    xc = x[c];
+    rb = k0 > 0 ? rho[c - sz] : inf;
+    rcv = rho[c];
+  }
+  for (int k = k0; k < k1; k += 2) {
+    T s = T(0);
+#pragma unroll
+    for (int kk = 0; kk < 2; ++kk) {
+      const int kf = k + kk;
+      if (in) {
+        const bool hf = kf < F.nz - 1;
+        const T xf = hf ? x[c + sz] : T(0), rf = hf ? rho[c + sz] : inf;
+        const T cL = rho_face<T>(i > 0 ? rho[c - 1] : inf, rcv, F.ax);
+        const T cR = rho_face<T>(rcv, i < F.nx - 1 ? rho[c + 1] : inf, F.ax);
+        const T cD = rho_face<T>(j > 0 ? rho[c - sy] : inf, rcv, F.ay);
+        const T cU = rho_face<T>(rcv, j < F.ny - 1 ? rho[c + sy] : inf, F.ay);
+        const T cB = rho_face<T>(rb, rcv, F.az);
+        const T cF = rho_face<T>(rcv, rf, F.az);
+        T off, diag;
+        z3_stencil<T, 2>(F, i, j, kf, i > 0 ? x[c - 1] : T(0), i < F.nx - 1 ? x[c + 1] : T(0),
+                         j > 0 ? x[c - sy] : T(0), j < F.ny - 1 ? x[c + sy] : T(0), xb, xf, cL, cR, cD, cU, cB,
+                         cF, off, diag);
+        s += F.rhs[c] - (off + diag * xc);
+        xb = xc;
+        xc = xf;
+        rb = rcv;
+        rcv = rf;
+        c += sz;
+      }
+    }
+    s += __shfl_xor(s, 1, kWave);
+    part[ly][lx] = s;
+    __syncthreads();
+    if (in && (ly & 1) == 0 && (lx & 1) == 0) {
+      const long C = (i >> 1) + (long)G.nx * ((j >> 1) + (long)G.ny * (k >> 1));
+      G.rhs[C] = (part[ly][lx] + part[ly + 1][lx]) / T(8);
+      G.sol[C] = T(0);
+    }
+    __syncthreads();
+  }
+}
+
 // x += alpha p, r -= alpha q (alpha = p.r / p.q); r32 = float(r) when given;
 // partial sums of r.r.  p.q and p.r from scal, or, with dir_nb > 0, reduced
 // here from k_fcg_direction's partials (p.q at [0, dir_nb), p.r after them)
@@ -1983,6 +2048,16 @@ struct scihpc_mg {
   void rr3(std::vector<Level<T>> &lv, int l) {
     auto &a = lv[l], &b = lv[l + 1];
     const int coef = z3_coef<T>(l);
+    if (coef == 2 && a.n >= (1L << 18) && ceil_div(a.ny, kZmY) <= 65535) {
+      // z-march on the A(rho) level: about 8192 blocks, even plane chunks
+      const long bxy = (long)ceil_div(a.nx, kZmX) * ceil_div(a.ny, kZmY);
+      const long slabs = std::max(1L, 8192L / bxy);
+      const int chunk = 2 * (int)ceil_div(a.nz / 2, slabs);
+      const dim3 grid(ceil_div(a.nx, kZmX), ceil_div(a.ny, kZmY), ceil_div(a.nz, chunk));
+      k_rr3d_zm<T><<<grid, kZmX * kZmY, 0, stream>>>(view(a), view(b), z3_rho<T>(), chunk);
+      SCIHPC_CHECK_LAUNCH();
+      return;
+    }
     SCIHPC_REQUIRE(b.ny <= 65535 && b.nz <= 65535, "3-D level too tall for the restriction grid");
     const dim3 grid(ceil_div(b.nx, kRR3X), b.ny, b.nz), block(kRR3X, 2, 2);
     if (coef == 2) k_rr3d_pairs<T, 2><<<grid, block, 0, stream>>>(view(a), view(b), z3_rho<T>());
