@@ -1602,6 +1602,77 @@ __global__ void __launch_bounds__(kZmX *kZmY)
   }
 }
 
+// 3-D factor-2 prolong-add marching in z: thread (lx, ly) owns the fine
+// x pair 2m, 2m+1 (m = bx kZmX + lx) of fine row j for an even chunk of
+// planes, with the three coarse planes the next two fine planes need in a
+// register window (fine planes 2K', 2K'+1 read coarse planes K'-1, K', K'+1),
+// so each coarse plane is loaded once per column instead of four times.
+// Same weights, ghost rule and operation order as k_prolong3_r2.
+template <typename T>
+__global__ void __launch_bounds__(kZmX *kZmY) k_prolong3_zm(DLevel<T> G, DLevel<T> F, int dir, int zchunk) {
+  using T2 = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
+  const int lx = (int)threadIdx.x % kZmX, ly = (int)threadIdx.x / kZmX;
+  const int m = blockIdx.x * kZmX + lx, j = blockIdx.y * kZmY + ly;
+  const int k0 = blockIdx.z * zchunk, k1 = min(k0 + zchunk, F.nz);
+  if (m >= G.nx || j >= F.ny) return;  // no barriers below
+  const int J0 = (j - 1) >> 1;
+  const T ty = (j & 1) ? T(0.25) : T(0.75);
+  auto load = [&](T (&a)[2][3], int K) {  // coarse plane K, ghost rule of coarse_at
+    const bool kout = K < 0 || K >= G.nz;
+    const int Kc = min(max(K, 0), G.nz - 1);
+#pragma unroll
+    for (int jj = 0; jj < 2; ++jj) {
+      const int J = J0 + jj;
+      const bool rout = kout || J < 0 || J >= G.ny;
+      const T *row = G.sol + (long)G.nx * (min(max(J, 0), G.ny - 1) + (long)G.ny * Kc);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const int I = m - 1 + d;
+        const bool out = rout || I < 0 || I >= G.nx;
+        a[jj][d] = (out && dir) ? T(0) : row[min(max(I, 0), G.nx - 1)];
+      }
+    }
+  };
+  T pa[2][3], pb[2][3], pc[2][3];
+  int Kp = k0 >> 1;
+  load(pa, Kp - 1);
+  load(pb, Kp);
+  load(pc, Kp + 1);
+  T *fine = F.sol + 2 * m + (long)F.nx * (j + (long)F.ny * k0);
+  const long sz = (long)F.nx * F.ny;
+  for (int k = k0; k < k1; k += 2, ++Kp) {
+#pragma unroll
+    for (int kk = 0; kk < 2; ++kk) {
+      const T tz = kk ? T(0.25) : T(0.75);
+      const T(&A)[2][3] = kk ? pb : pa;
+      const T(&B)[2][3] = kk ? pc : pb;
+      auto interp = [&](T tx, int o) {
+        const T v00 = (T(1) - tx) * A[0][o] + tx * A[0][o + 1];
+        const T v10 = (T(1) - tx) * A[1][o] + tx * A[1][o + 1];
+        const T v0 = (T(1) - ty) * v00 + ty * v10;
+        const T v01 = (T(1) - tx) * B[0][o] + tx * B[0][o + 1];
+        const T v11 = (T(1) - tx) * B[1][o] + tx * B[1][o + 1];
+        const T v1 = (T(1) - ty) * v01 + ty * v11;
+        return (T(1) - tz) * v0 + tz * v1;
+      };
+      T2 *p = reinterpret_cast<T2 *>(fine + (kk ? sz : 0));
+      T2 f = *p;
+      f.x += interp(T(0.75), 0);
+      f.y += interp(T(0.25), 1);
+      *p = f;
+    }
+    fine += 2 * sz;
+#pragma unroll
+    for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        pa[jj][d] = pb[jj][d];
+        pb[jj][d] = pc[jj][d];
+      }
+    if (k + 2 < k1) load(pc, Kp + 2);
+  }
+}
+
 // x += alpha p, r -= alpha q (alpha = p.r / p.q); r32 = float(r) when given;
 // partial sums of r.r.  p.q and p.r from scal, or, with dir_nb > 0, reduced
 // here from k_fcg_direction's partials (p.q at [0, dir_nb), p.r after them)
@@ -1923,7 +1994,14 @@ struct scihpc_mg {
     const int r = coarse.deg / fine.deg, dir = bc == SCIHPC_MG_DIRICHLET;
     if (ndim == 2)
       k_prolong_add<T, 2><<<ceil_div(fine.n, kBlock), kBlock, 0, stream>>>(view(coarse), view(fine), r, dir);
-    else if (r == 2) {
+    else if (r == 2 && fine.n >= (1L << 18) && ceil_div(fine.ny, kZmY) <= 65535) {
+      // z-march: about 2048 blocks, chunks of at least 8 (even) planes
+      const long bxy = (long)ceil_div(coarse.nx, kZmX) * ceil_div(fine.ny, kZmY);
+      const long slabs = std::max(1L, 2048L / bxy);
+      const int chunk = std::max(8, 2 * (int)ceil_div(fine.nz / 2, slabs));
+      const dim3 grid(ceil_div(coarse.nx, kZmX), ceil_div(fine.ny, kZmY), ceil_div(fine.nz, chunk));
+      k_prolong3_zm<T><<<grid, kZmX * kZmY, 0, stream>>>(view(coarse), view(fine), dir, chunk);
+    } else if (r == 2) {
       SCIHPC_REQUIRE(fine.nz <= 65535, "3-D level too tall for the prolongation grid");
       const dim3 grid(ceil_div(coarse.nx, kP3X), ceil_div(fine.ny, kP3Y), fine.nz);
       k_prolong3_r2<T><<<grid, dim3(kP3X, kP3Y), 0, stream>>>(view(coarse), view(fine), dir);
