@@ -150,6 +150,16 @@ typedef struct {
                         residual, and then the CG recurrence restarts from
                         the corrected x; 2: kept unjudged (ablation);
                         0: off.  No counterpart in the reference. */
+  int lagged_check;  /* 1: the host reads the residual only from one
+                        iteration before the count that the last converged
+                        solve on this handle (same path, tolerance and
+                        nu1/nu2) needed, and on the last allowed iteration;
+                        earlier iterations are enqueued without a host round
+                        trip.  A solve may then run past the tolerance by a
+                        few iterations (the count self-corrects); the result
+                        still meets rel_tol.  0: read after every
+                        iteration, as the reference's loop does.  No
+                        counterpart in the reference. */
 } scihpc_solve_opts;
 
 typedef struct {
@@ -178,8 +188,10 @@ SCIHPC_API int scihpc_mg_solve(scihpc_mg *h, const scihpc_solve_opts *opts,
  * level-0 kernel between HIP events on the handle stream; which = 0 FP64
  * pre-smoothing (nu1 = 2 sweeps + residual + restriction), 1 the same in
  * FP32, 2 FP32 post-smoothing fused with the FP64 outer update, 3 FP64
- * post-smoothing with the residual norm.  Returns the mean milliseconds per
- * launch and the algorithmic HBM bytes of one launch. */
+ * post-smoothing with the residual norm, 4 the FP64 prolongation of level 1
+ * added to level 0 alone, 5 the FP64 level-0 residual pass (partial sums of
+ * the norm) alone.  Returns the mean milliseconds per launch and the
+ * algorithmic HBM bytes of one launch. */
 SCIHPC_API int scihpc_mg_time_kernel(scihpc_mg *h, int which, int iters, double *ms,
                           double *bytes);
 

@@ -134,14 +134,17 @@ class Multigrid:
         check(self._lib.scihpc_mg_vcycle(self.h, nu1, nu2))
 
     def solve(self, rel_tol=1e-8, max_cycles=200, nu1=2, nu2=2, precision=PREC_FP64,
-              use_learned=False, zero_guess=True, accel=0, learned_correction=0):
+              use_learned=False, zero_guess=True, accel=0, learned_correction=0, lagged_check=0):
         """accel 0: stationary defect correction; 1: flexible CG around the V-cycle.
         use_learned: 1 judged guess with back-off after rejections, 3 judged
         on every solve, 2 forced (ablation).  learned_correction (mixed
         flexible CG): 1 the network's coarse-level correction of the defect
-        after the first iteration, judged by the FP64 residual; 2 forced."""
+        after the first iteration, judged by the FP64 residual; 2 forced.
+        lagged_check 1: the residual is read only from one iteration before
+        the count the last converged solve of the same path needed (fewer
+        host round trips; a solve may run a few iterations past rel_tol)."""
         o = SolveOpts(rel_tol, max_cycles, nu1, nu2, precision, int(use_learned), int(zero_guess),
-                      int(accel), int(learned_correction))
+                      int(accel), int(learned_correction), int(lagged_check))
         s = SolveStats()
         rc = check(self._lib.scihpc_mg_solve(self.h, C.byref(o), C.byref(s)), allow=(OK, NOT_CONVERGED))
         return {"converged": rc == OK, "cycles": s.cycles, "rel_residual": s.rel_residual,

@@ -27,7 +27,8 @@ class ScihpcError(RuntimeError):
 class SolveOpts(C.Structure):
     _fields_ = [("rel_tol", C.c_double), ("max_cycles", C.c_int), ("nu1", C.c_int),
                 ("nu2", C.c_int), ("precision", C.c_int), ("use_learned", C.c_int),
-                ("zero_guess", C.c_int), ("accel", C.c_int), ("learned_correction", C.c_int)]
+                ("zero_guess", C.c_int), ("accel", C.c_int), ("learned_correction", C.c_int),
+                ("lagged_check", C.c_int)]
 
 
 class SolveStats(C.Structure):

@@ -1349,7 +1349,8 @@ __global__ void __launch_bounds__(kBlock)
     } else {
       zq = scal[S_ZQ];
     }
-    beta = -zq / scal[S_PQ];
+    const double pq = scal[S_PQ];
+    beta = pq != 0.0 ? -zq / pq : 0.0;  // 0 only after exact convergence
   }
   double vpq = 0.0, vpr = 0.0;
   if (c < L.n) {
@@ -1413,7 +1414,8 @@ __global__ void __launch_bounds__(kZmX *kZmY)
     k_fcg_direction_zm(DLevel<double> L, const T *__restrict__ z, const double *__restrict__ p_old,
                        double *__restrict__ p, double *__restrict__ q, const double *__restrict__ r,
                        const double *__restrict__ scal, int first, double *__restrict__ partial, int zchunk) {
-  const double beta = first ? 0.0 : -scal[S_ZQ] / scal[S_PQ];
+  const double pq_old = first ? 0.0 : scal[S_PQ];
+  const double beta = pq_old != 0.0 ? -scal[S_ZQ] / pq_old : 0.0;  // 0 only after exact convergence
   const int i = blockIdx.x * kZmX + (int)threadIdx.x % kZmX, j = blockIdx.y * kZmY + (int)threadIdx.x / kZmX;
   const int k0 = blockIdx.z * zchunk, k1 = min(k0 + zchunk, L.nz);
   const long sy = L.nx, sz = (long)L.nx * L.ny;
@@ -1703,7 +1705,7 @@ __global__ void __launch_bounds__(kBlock)
   }
   double v = 0.0;
   if (c < n) {
-    const double alpha = pr / pq;
+    const double alpha = pq != 0.0 ? pr / pq : 0.0;  // 0 only after exact convergence
     x[c] += alpha * p[c];
     const double rc = r[c] - alpha * q[c];
     r[c] = rc;
@@ -2645,6 +2647,13 @@ struct scihpc_mg {
   // FP64 work vectors of the flexible-CG solve
   double *fx_ = nullptr, *fr_ = nullptr, *fp_[2] = {nullptr, nullptr}, *fq_ = nullptr;
   double last_rr_ = 0.0;  // final relative residual of the last flexible-CG solve
+  // lagged convergence checks (opts.lagged_check): per path (stationary or
+  // flexible CG, FP64 or mixed) the iteration count of the last converged
+  // solve, with the tolerance and smoothing counts it ran with
+  int lag_iters_[4] = {0, 0, 0, 0};
+  double lag_tol_[4] = {0.0, 0.0, 0.0, 0.0};
+  int lag_nu_[4] = {-1, -1, -1, -1};
+  int first_check_ = 0;  // first iteration whose residual the last solve read
   void ensure_fcg() {
     if (fx_) return;
     const long n = d[0].n;
@@ -2660,7 +2669,7 @@ struct scihpc_mg {
   // learned: a guess is being formed on lstream; after the first iteration
   // it replaces that iteration if its FP64 residual is lower
   int solve_fcg(const scihpc_solve_opts &o, scihpc_solve_stats &st, bool zero, double bnorm, double scale,
-                bool learned_overlap = false) {
+                bool learned_overlap, int check_from) {
     ensure_fcg();
     auto &l0 = d[0];
     const long n = l0.n;
@@ -2740,7 +2749,13 @@ struct scihpc_mg {
       SCIHPC_CHECK_LAUNCH();
       // the learned guess is enqueued while the first iteration runs
       if (learned_overlap && !learned_pending) learned_enqueue(bnorm);
-      rr = std::sqrt(host_sum(nb)) / scale;
+      // lagged checks: iterations before check_from run without reading
+      // the residual (the host keeps enqueueing); never while a learned
+      // step is pending, and always on the last allowed iteration
+      if (learned_overlap || corr_left || it + spent + 1 >= check_from || it + spent + 1 >= o.max_cycles) {
+        if (!first_check_) first_check_ = it + spent + 1;
+        rr = std::sqrt(host_sum(nb)) / scale;
+      }
       ++it;
       // the guess is judged at the end of the first iteration after which
       // its stream has finished (no wait): it replaces the iterate only if
@@ -2796,6 +2811,7 @@ struct scihpc_mg {
     SCIHPC_REQUIRE(o.rel_tol > 0 && o.max_cycles >= 0 && o.nu1 >= 0 && o.nu2 >= 0, "bad solve options");
     SCIHPC_REQUIRE(o.nu1 <= 2 && o.nu2 <= 2, "nu1, nu2 <= 2 (tiled kernels stage a 5-cell halo)");
     SCIHPC_REQUIRE(o.accel == 0 || o.accel == 1, "accel must be 0 (stationary) or 1 (flexible CG)");
+    SCIHPC_REQUIRE(o.lagged_check == 0 || o.lagged_check == 1, "lagged_check must be 0 or 1");
     SCIHPC_REQUIRE(o.use_learned >= 0 && o.use_learned <= 3, "use_learned must be 0..3");
     SCIHPC_REQUIRE(o.learned_correction >= 0 && o.learned_correction <= 2, "learned_correction must be 0..2");
     SCIHPC_REQUIRE(o.learned_correction == 0 || (o.accel == 1 && o.precision == SCIHPC_PREC_MIXED),
@@ -2811,6 +2827,13 @@ struct scihpc_mg {
     bool zero = o.zero_guess != 0;
     bool overlap = false;
     bool gated = false;
+    // lagged checks: the residual is first read one iteration before the
+    // count that the last converged solve of the same path needed
+    const int key = (o.accel == 1 ? 2 : 0) + (o.precision == SCIHPC_PREC_MIXED ? 1 : 0);
+    const int nu = o.nu1 * 16 + o.nu2;
+    const int check_from =
+        o.lagged_check && lag_tol_[key] == o.rel_tol && lag_nu_[key] == nu ? lag_iters_[key] - 1 : 0;
+    first_check_ = 0;
     if (o.use_learned == 1 && learned.active() && learned_skip > 0) {
       --learned_skip;
       st.learned_skipped = 1;
@@ -2830,14 +2853,17 @@ struct scihpc_mg {
       }
     }
     if (o.accel == 1) {
-      solve_fcg(o, st, zero, bnorm, scale, overlap);
+      solve_fcg(o, st, zero, bnorm, scale, overlap, check_from);
       cycles = st.cycles;
       rr = last_rr_;
     } else if (o.precision == SCIHPC_PREC_FP64) {
       rr = zero ? (bnorm > 0.0 ? 1.0 : 0.0) : residual_norm(0, false) / scale;
       while (rr > o.rel_tol && cycles < o.max_cycles) {
         const int np = vcycle(d, o.nu1, o.nu2, zero && cycles == 0, 1);
-        rr = std::sqrt(host_sum(np)) / scale;
+        if (cycles + 1 >= check_from || cycles + 1 >= o.max_cycles) {
+          if (!first_check_) first_check_ = cycles + 1;
+          rr = std::sqrt(host_sum(np)) / scale;
+        }
         ++cycles;
       }
       if (zero && cycles == 0) SCIHPC_HIP(hipMemsetAsync(l0.sol, 0, sizeof(double) * l0.n, stream));
@@ -2875,7 +2901,10 @@ struct scihpc_mg {
         const int np = vcycle(f, o.nu1, o.nu2, true, 2, oa);
         std::swap(f0.rhs, f0.rhs_alt);
         std::swap(p, q);
-        rr = std::sqrt(host_sum(np)) / scale;
+        if (cycles + 1 >= check_from || cycles + 1 >= o.max_cycles) {
+          if (!first_check_) first_check_ = cycles + 1;
+          rr = std::sqrt(host_sum(np)) / scale;
+        }
         ++cycles;
       }
       if (p != l0.sol) {
@@ -2891,6 +2920,13 @@ struct scihpc_mg {
     st.cycles = cycles;
     st.rel_residual = rr;
     st.ms = ms;
+    if (o.lagged_check && rr <= o.rel_tol && cycles > 0) {
+      // converged at the first residual read: the count may overshoot, so
+      // the next solve reads one iteration earlier
+      lag_iters_[key] = first_check_ == cycles && check_from > 0 ? cycles - 1 : cycles;
+      lag_tol_[key] = o.rel_tol;
+      lag_nu_[key] = nu;
+    }
     if (gated) learned_note(st);
     return rr <= o.rel_tol ? SCIHPC_OK : SCIHPC_NOT_CONVERGED;
   }
@@ -3558,7 +3594,7 @@ extern "C" int scihpc_mg_learned_features(scihpc_mg *h, int level, float *feat, 
 extern "C" int scihpc_mg_time_kernel(scihpc_mg *h, int which, int iters, double *ms, double *bytes) {
   return guarded([&] {
     check_level(h, 0);
-    SCIHPC_REQUIRE(iters > 0 && which >= 0 && which <= 3, "bad timing request");
+    SCIHPC_REQUIRE(iters > 0 && which >= 0 && which <= 5, "bad timing request");
     SCIHPC_REQUIRE(h->L > 1, "timing needs at least two levels");
     h->ensure_fp32();
     auto &l0 = h->d[0];
@@ -3608,12 +3644,20 @@ extern "C" int scihpc_mg_time_kernel(scihpc_mg *h, int which, int iters, double 
           if (tiled) h->post2d_mode<double, 1>(h->d, 0, 2, oa);
           else post(h->d, 1);
           break;
+        case 4:  // FP64 prolongation of level 1 added to level 0, alone
+          h->prolong_add(h->d[1], l0);
+          break;
+        case 5:  // FP64 level-0 residual partial sums (the norm's pass over the grid), alone
+          h->residual_partials(l0, false);
+          break;
       }
     };
     // algorithmic bytes: every array the launch touches, once per read and
     // once per write, at its stored precision
     double b = 0;
-    if (tiled) {  // one fused kernel per launch
+    if (which == 4) b = 2.0 * 8.0 * n + 8.0 * nc;  // fine read + write, coarse read
+    else if (which == 5) b = (2.0 + nf) * 8.0 * n;  // solution, right-hand side, faces
+    else if (tiled) {  // one fused kernel per launch
       switch (which) {
         case 0: b = (3.0 + nf) * 8.0 * n + 2.0 * 8.0 * nc; break;
         case 1: b = (3.0 + nf) * 4.0 * n + 2.0 * 4.0 * nc; break;

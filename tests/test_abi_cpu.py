@@ -45,3 +45,28 @@ def test_library_loads_and_reports_errors_without_gpu_work():
                               ctypes.c_double(1.0), 0, ctypes.byref(h))
     assert rc == 2
     assert b"ndim" in lib.scihpc_last_error()
+
+
+def test_ctypes_structs_match_header(tmp_path):
+    """The ctypes mirrors of the option and statistics structs have the
+    header's size and field offsets (compiled with the host C compiler)."""
+    from scihpc_amd import _capi
+
+    pairs = [("scihpc_solve_opts", _capi.SolveOpts), ("scihpc_solve_stats", _capi.SolveStats),
+             ("scihpc_fpc_params", _capi.FpcParams), ("scihpc_flow_params", _capi.FlowParams)]
+    lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "scihpc_hip.h"', "int main(void) {"]
+    for cname, py in pairs:
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'  printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l.strip()}
+    for cname, py in pairs:
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert got[(cname, f)] == getattr(py, f).offset, (cname, f)

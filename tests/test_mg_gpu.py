@@ -301,3 +301,76 @@ def test_flexible_cg_3d_z_march(variable):
         assert st["converged"] and st["rel_residual"] <= 1e-10, st
         assert st["cycles"] <= ref["cycles"]
         assert rel(g.sol(0), x_ref) < 1e-8
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_timing_hooks(ndim):
+    """The level-0 timing hooks run every mode, report the algorithmic bytes
+    of the prolongation and residual passes, reject unknown modes, and leave
+    the handle able to solve."""
+    if ndim == 2:
+        n = (128, 256, 1)
+        w = W.bubble_2d(*n[:2])
+        g = Multigrid(2, *n, 1 / n[0], 2 / n[1], 1.0, "neumann")
+    else:
+        n = (64, 64, 64)
+        w = W.poisson_3d(64, variable=True)
+        g = Multigrid(3, *n, 1 / 64, 1 / 64, 1 / 64, "neumann")
+    g.set_density(w["rho"])
+    g.set_rhs(w["rhs"])
+    print(f"timing hooks {ndim}-D {n}", flush=True)
+    assert g.solve(rel_tol=1e-8, max_cycles=100, precision=PREC_MIXED, accel=1)["converged"]
+    cells = int(np.prod(n))
+    coarse = int(np.prod([max(x // 2, 1) for x in n]))
+    for which in range(6):
+        ms, b = g.time_kernel(which, iters=2)
+        assert math.isfinite(ms) and ms > 0 and b > 0
+    assert g.time_kernel(4, iters=1)[1] == 16.0 * cells + 8.0 * coarse
+    assert g.time_kernel(5, iters=1)[1] == (2 + ndim) * 8.0 * cells
+    with pytest.raises(ScihpcError):
+        g.time_kernel(6, iters=1)
+    st = g.solve(rel_tol=1e-8, max_cycles=100, precision=PREC_MIXED, accel=1)
+    assert st["converged"], st
+
+
+@pytest.mark.parametrize("precision,accel", [(PREC_FP64, 0), (PREC_MIXED, 0), (PREC_MIXED, 1)],
+                         ids=["fp64-stationary", "mixed-stationary", "mixed-fcg"])
+def test_lagged_check(precision, accel):
+    """lagged_check: the same answer as reading the residual every iteration
+    once the count is known; after a harder solve, an easier start (a
+    half-converged guess) still meets the tolerance, overshooting by the
+    lag, and the count comes back down over the next solves."""
+    nx, ny = 128, 256
+    w = W.bubble_2d(nx, ny)
+    g = Multigrid(2, nx, ny, 1, 1 / nx, 2 / ny, 1.0, "neumann")
+    g.set_density(w["rho"])
+    g.set_rhs(w["rhs"])
+    print(f"lagged check {precision} accel={accel}", flush=True)
+    kw = dict(rel_tol=1e-9, max_cycles=200, precision=precision, accel=accel)
+    ref = g.solve(**kw)
+    assert ref["converged"]
+    x_ref = g.sol(0)
+    for _ in range(3):
+        st = g.solve(lagged_check=1, **kw)
+        assert st["converged"] and st["rel_residual"] <= 1e-9, st
+        assert st["cycles"] == ref["cycles"]
+        assert rel(g.sol(0), x_ref) < 1e-12
+    # an easier problem: the same b from a half-converged start
+    g.solve(**{**kw, "max_cycles": ref["cycles"] // 2})
+    x_half = g.sol(0)
+    g.set_sol(x_half)
+    want = g.solve(zero_guess=False, **kw)
+    assert want["converged"] and want["cycles"] < ref["cycles"] - 2
+    g.solve(lagged_check=1, **kw)  # the count of the zero start again
+    counts = []
+    for _ in range(ref["cycles"] - want["cycles"] + 2):
+        g.set_sol(x_half)
+        st = g.solve(zero_guess=False, lagged_check=1, **kw)
+        assert st["converged"] and st["rel_residual"] <= 1e-9, st
+        assert rel(g.sol(0), x_ref) < 1e-8
+        counts.append(st["cycles"])
+    assert counts[0] == ref["cycles"] - 1  # read first one iteration before the old count
+    assert counts[-1] == want["cycles"]
+    assert all(b <= a for a, b in zip(counts, counts[1:]))
+    with pytest.raises(ScihpcError):
+        g.solve(lagged_check=2, **kw)
