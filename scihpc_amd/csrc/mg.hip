@@ -1327,13 +1327,18 @@ __global__ void __launch_bounds__(kBlock) k_dot_zq(const T *__restrict__ z, cons
   if (threadIdx.x == 0) partial[blockIdx.x] = v;
 }
 
+// Workgroup size of the two FCG vector kernels: every workgroup reduces
+// the producer's partial sums itself, so fewer, larger workgroups read
+// fewer of them (512 x 1024 threads at 512 x 1024 cells).
+constexpr int kFcgThreads = 1024;
+
 // p = z + beta p_old (beta = 0 on the first iteration), q = A p, partial sums
 // of p.q and p.r.  beta = -(z.q_old)/(p_old.q_old): z.q_old from scal[S_ZQ],
 // or, with zq_n > 0, reduced here from the post-smoothing kernel's zq_n
-// partials in k_finalize's order (every block gets the same sum), which
-// saves the separate reduction launch.
+// partials (every workgroup sums them in the same order, so all get the
+// same value), which saves the separate reduction launch.
 template <typename T, int DIM, bool VAR>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kFcgThreads)
     k_fcg_direction(DLevel<double> L, const T *__restrict__ z, const double *__restrict__ p_old,
                     double *__restrict__ p, double *__restrict__ q, const double *__restrict__ r,
                     const double *__restrict__ scal, int first, double *__restrict__ partial,
@@ -1677,10 +1682,11 @@ __global__ void __launch_bounds__(kZmX *kZmY) k_prolong3_zm(DLevel<T> G, DLevel<
 
 // x += alpha p, r -= alpha q (alpha = p.r / p.q); r32 = float(r) when given;
 // partial sums of r.r.  p.q and p.r from scal, or, with dir_nb > 0, reduced
-// here from k_fcg_direction's partials (p.q at [0, dir_nb), p.r after them)
-// in k_finalize2's order, block 0 storing them in scal for the next beta.
+// here from k_fcg_direction's partials (p.q at [0, dir_nb), p.r after them),
+// in one order for every workgroup, block 0 storing them in scal for the
+// next beta.
 // partial must not overlap dir_part.
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kFcgThreads)
     k_fcg_update(long n, double *__restrict__ x, double *__restrict__ r, const double *__restrict__ p,
                  const double *__restrict__ q, double *__restrict__ scal, float *__restrict__ r32,
                  double *__restrict__ partial, const double *__restrict__ dir_part, int dir_nb) {
@@ -2699,7 +2705,8 @@ struct scihpc_mg {
     int zm_chunk = 1;
     const long nzm = zm_layout(l0, zm_grid, zm_chunk);
     const bool zm = nzm > 0;
-    const int ndir = zm ? (int)nzm : (int)nb;
+    const unsigned nbf = ceil_div(n, kFcgThreads);  // workgroups of the FCG vector kernels
+    const int ndir = zm ? (int)nzm : (int)nbf;
     while (rr > o.rel_tol && it + spent < o.max_cycles) {
       // z = B r; from the second iteration on the level-0 post kernel also
       // forms z . q_old (S_ZQ) for the flexible direction update
@@ -2716,15 +2723,15 @@ struct scihpc_mg {
       }
       // small reductions are folded into the consumers: z.q_old into the
       // direction kernel, p.q and p.r into the update (its own partials go
-      // to [0, nb), the direction's to [2 nb, 4 nb))
-      const bool fuse = nb <= kCompressAbove && np <= kCompressAbove && ndir <= (int)nb;
+      // to [0, nbf), the direction's to [2 nb, 2 nb + 2 ndir))
+      const bool fuse = nbf <= kCompressAbove && np <= kCompressAbove && ndir <= (int)nb;
       if (it > 0 && (!fuse || zm)) finalize(np, S_ZQ);
       double *pold = fp_[cur], *pnew = fp_[cur ^ 1];
       double *dpart = fuse ? partial + 2 * (long)nb : partial;
       const double *zq_part = (fuse && it > 0) ? partial : nullptr;
       const int zq_n = (fuse && it > 0) ? np : 0;
 #define SCIHPC_FCG_DIR(T, D, V)                                                                                  \
-  k_fcg_direction<T, D, V><<<nb, kBlock, 0, stream>>>(view(l0), (const T *)z, pold, pnew, fq_, r, scal, it == 0, \
+  k_fcg_direction<T, D, V><<<nbf, kFcgThreads, 0, stream>>>(view(l0), (const T *)z, pold, pnew, fq_, r, scal, it == 0, \
                                                        dpart, zq_part, zq_n)
 #define SCIHPC_FCG_ZM(T, V)                                                                              \
   k_fcg_direction_zm<T, V><<<zm_grid, kZmX * kZmY, 0, stream>>>(view(l0), (const T *)z, pold, pnew, fq_, r, \
@@ -2744,7 +2751,7 @@ struct scihpc_mg {
       SCIHPC_CHECK_LAUNCH();
       if (!fuse) finalize2(ndir, S_PQ, S_PR);
       cur ^= 1;
-      k_fcg_update<<<nb, kBlock, 0, stream>>>(n, x, r, fp_[cur], fq_, scal, mixed ? f[0].rhs : nullptr, partial,
+      k_fcg_update<<<nbf, kFcgThreads, 0, stream>>>(n, x, r, fp_[cur], fq_, scal, mixed ? f[0].rhs : nullptr, partial,
                                               fuse ? dpart : nullptr, fuse ? ndir : 0);
       SCIHPC_CHECK_LAUNCH();
       // the learned guess is enqueued while the first iteration runs
@@ -2754,7 +2761,7 @@ struct scihpc_mg {
       // step is pending, and always on the last allowed iteration
       if (learned_overlap || corr_left || it + spent + 1 >= check_from || it + spent + 1 >= o.max_cycles) {
         if (!first_check_) first_check_ = it + spent + 1;
-        rr = std::sqrt(host_sum(nb)) / scale;
+        rr = std::sqrt(host_sum(nbf)) / scale;
       }
       ++it;
       // the guess is judged at the end of the first iteration after which
