@@ -678,6 +678,145 @@ __global__ void __launch_bounds__(kFusedThreads)
   }
 }
 
+// p.q and p.r from k_fcg_direction's partial sums (p.q at [0, nb), p.r
+// after them), summed by the first wave in one fixed order, whatever the
+// workgroup size, and returned to every thread: the FCG update and the fused
+// update + pre-smoothing kernel form the same alpha.
+__device__ __forceinline__ void fcg_dir_sums(const double *__restrict__ part, int nb, double &pq, double &pr) {
+  __shared__ double ta, tb;
+  if (threadIdx.x < kWave) {
+    double a = 0.0, b = 0.0;
+    for (int k = threadIdx.x; k < nb; k += kWave) {
+      a += part[k];
+      b += part[nb + k];
+    }
+    a = wave_sum(a);
+    b = wave_sum(b);
+    if (threadIdx.x == 0) {
+      ta = a;
+      tb = b;
+    }
+  }
+  __syncthreads();
+  pq = ta;
+  pr = tb;
+}
+
+// The FCG update of the mixed solve fused into the next V-cycle's level-0
+// pre-smoothing (k_pre2d on the FP32 level 0 from a zero guess).  alpha =
+// p.r / p.q from the direction kernel's partial sums; the staged right-hand
+// side is float(r - alpha q), formed for the halo cells too, from the
+// residual before the update (r_in, read-only here); the tile's own cells
+// get x += alpha p, r_out = r - alpha q, the FP32 level-0 right-hand side
+// float(r_out) and a per-tile sum of r_out^2.  Workgroup 0 stores p.q and
+// p.r for the next beta.  Every value is the one k_fcg_update followed by
+// k_pre2d would produce.
+struct FcgUpdate {
+  double *x;
+  const double *r_in;
+  double *r_out;
+  const double *p, *q;
+  double *scal;
+  const double *dir_part;
+  int dir_nb;
+  double *r2_part;  // indexed by workgroup
+};
+
+template <bool VAR, int TX, int TY>
+__global__ void __launch_bounds__(kFusedThreads)
+    k_pre2d_fcg(DLevel<float> F, DLevel<float> G, float *__restrict__ out, int nu1, FcgUpdate u) {
+  constexpr int W = TX + 2 * kFH, H = TY + 2 * kFH;
+  __shared__ float xs[H][W];
+  __shared__ float bs[H][W];
+  __shared__ float cxs[VAR ? H : 1][W];
+  __shared__ float cys[VAR ? H : 1][W];
+  const int tiles_x = (F.nx + TX - 1) / TX, ntiles = tiles_x * ((F.ny + TY - 1) / TY);
+  const int tile = xcd_tile(ntiles);
+  if (tile >= ntiles) return;  // never taken: the grid has exactly ntiles blocks
+  const int x0 = (tile % tiles_x) * TX, y0 = (tile / tiles_x) * TY;
+  // r and q of the staged region are loaded before alpha's reduction (the
+  // two latencies overlap), then combined once alpha is known; the largest
+  // tile loads them afterwards (its registers would cost occupancy)
+  constexpr int NS0 = (W * H + kFusedThreads - 1) / kFusedThreads;
+  constexpr bool early = NS0 <= 6;
+  constexpr int NS = early ? NS0 : 1;
+  double rv[NS], qv[NS];
+#pragma unroll
+  for (int t = 0; t < NS0; ++t) {
+    const int q = threadIdx.x + t * kFusedThreads;
+    if (q < W * H) {
+      const int ly = q / W, lx = q % W;
+      const int gx = x0 - kFH + lx, gy = y0 - kFH + ly;
+      const bool in = gx >= 0 && gx < F.nx && gy >= 0 && gy < F.ny;
+      const long g = gx + (long)F.nx * gy;
+      if constexpr (early) {
+        rv[t] = in ? u.r_in[g] : 0.0;
+        qv[t] = in ? u.q[g] : 0.0;
+      }
+      xs[ly][lx] = 0.0f;
+      if constexpr (VAR) {
+        cxs[ly][lx] = in ? F.cx[g] : 0.0f;
+        cys[ly][lx] = in ? F.cy[g] : 0.0f;
+      }
+    }
+  }
+  double pq, pr;
+  fcg_dir_sums(u.dir_part, u.dir_nb, pq, pr);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    u.scal[S_PQ] = pq;
+    u.scal[S_PR] = pr;
+  }
+  const double alpha = pq != 0.0 ? pr / pq : 0.0;  // 0 only after exact convergence
+#pragma unroll
+  for (int t = 0; t < NS0; ++t) {
+    const int q = threadIdx.x + t * kFusedThreads;
+    if (q < W * H) {
+      const int ly = q / W, lx = q % W;
+      const int gx = x0 - kFH + lx, gy = y0 - kFH + ly;
+      const bool in = gx >= 0 && gx < F.nx && gy >= 0 && gy < F.ny;
+      if constexpr (early) {
+        bs[ly][lx] = in ? float(rv[t] - alpha * qv[t]) : 0.0f;
+      } else {
+        const long g = gx + (long)F.nx * gy;
+        bs[ly][lx] = in ? float(u.r_in[g] - alpha * u.q[g]) : 0.0f;
+      }
+    }
+  }
+  double r2 = 0.0;
+  for (int q = threadIdx.x; q < TX * TY; q += blockDim.x) {
+    const int gx = x0 + q % TX, gy = y0 + q / TX;
+    if (gx < F.nx && gy < F.ny) {
+      const long g = gx + (long)F.nx * gy;
+      u.x[g] += alpha * u.p[g];
+      const double rc = u.r_in[g] - alpha * u.q[g];
+      u.r_out[g] = rc;
+      F.rhs[g] = float(rc);
+      r2 += rc * rc;
+    }
+  }
+  r2 = block_sum(r2);  // its barriers also order the staging above
+  if (threadIdx.x == 0) u.r2_part[blockIdx.x] = r2;
+  fused_sweeps<float, VAR, TX, TY>(F, x0, y0, nu1, 0, xs, bs, cxs, cys);
+  // residual + child average (ratio 2, tiles are even-aligned)
+  for (int q = threadIdx.x; q < (TX / 2) * (TY / 2); q += blockDim.x) {
+    const int I = x0 / 2 + q % (TX / 2), J = y0 / 2 + q / (TX / 2);
+    if (I >= G.nx || J >= G.ny) continue;
+    float sum = 0.0f;
+    for (int jj = 0; jj < 2; ++jj)
+      for (int ii = 0; ii < 2; ++ii) {
+        const int gx = 2 * I + ii, gy = 2 * J + jj;
+        sum += fused_res<float, VAR, TX>(F, gx, gy, gx - x0 + kFH, gy - y0 + kFH, xs, bs, cxs, cys);
+      }
+    const long C = I + (long)G.nx * J;
+    G.rhs[C] = sum / 4.0f;
+    G.sol[C] = 0.0f;
+  }
+  for (int q = threadIdx.x; q < TX * TY; q += blockDim.x) {
+    const int gx = x0 + q % TX, gy = y0 + q / TX;
+    if (gx < F.nx && gy < F.ny) out[gx + (long)F.nx * gy] = xs[gy - y0 + kFH][gx - x0 + kFH];
+  }
+}
+
 struct OuterArgs {  // level-0 extras of the post-smoothing kernel
   DLevel<double> P;         // MODE 2: rhs = b, faces / constants of the FP64 operator
   const double *p_old;      // MODE 2: current FP64 iterate
@@ -1693,14 +1832,7 @@ __global__ void __launch_bounds__(kFcgThreads)
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   double pq, pr;
   if (dir_nb > 0) {
-    double a = 0.0, b = 0.0;
-    for (int k = threadIdx.x; k < dir_nb; k += blockDim.x) {
-      a += dir_part[k];
-      b += dir_part[dir_nb + k];
-    }
-    block_allsum2(a, b);
-    pq = a;
-    pr = b;
+    fcg_dir_sums(dir_part, dir_nb, pq, pr);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       scal[S_PQ] = pq;
       scal[S_PR] = pr;
@@ -1936,6 +2068,8 @@ struct scihpc_mg {
   int num_cu = 256;           // compute units (queried at create)
   bool tiled = true;          // LDS-tiled 2-D fine-level kernels (SCIHPC_MG_TILED=0 disables)
   bool tiled3d = true;        // z-streaming 3-D sweeps (SCIHPC_MG_TILED3D=0 disables)
+  bool fused_update = true;   // mixed 2-D FCG: update inside the level-0 pre-smoothing
+                              // (SCIHPC_MG_FUSED_UPDATE=0 disables)
   int z3_chunk = 0;           // z planes per 3-D workgroup (0: chosen per level)
   float *rho0f = nullptr;     // FP32 copy of rho0 for the mixed V-cycle
   void *blk_dev[2] = {nullptr, nullptr};   // device copies of BlockLevels<double>/<float>
@@ -2061,6 +2195,28 @@ struct scihpc_mg {
       if (k == 0) pre2d_k<T, false, 0>(lv, l, nu1, zero_guess);
       else if (k == 1) pre2d_k<T, false, 1>(lv, l, nu1, zero_guess);
       else pre2d_k<T, false, 2>(lv, l, nu1, zero_guess);
+    }
+  }
+
+  // the FCG update fused into the FP32 level-0 pre-smoothing (k_pre2d_fcg)
+  template <bool VAR, int K>
+  void pre2d_fcg_k(int nu1, const FcgUpdate &u) {
+    constexpr int TX = TileCfg<float>::TX[K], TY = TileCfg<float>::TY[K];
+    auto &a = f[0];
+    k_pre2d_fcg<VAR, TX, TY><<<tiles2d(a), kFusedThreads, 0, stream>>>(view(a), view(f[1]), a.sol_alt, nu1, u);
+    SCIHPC_CHECK_LAUNCH();
+    std::swap(a.sol, a.sol_alt);
+  }
+  void pre2d_fcg(int nu1, const FcgUpdate &u) {
+    const int k = tile_kind(f[0]);
+    if (variable) {
+      if (k == 0) pre2d_fcg_k<true, 0>(nu1, u);
+      else if (k == 1) pre2d_fcg_k<true, 1>(nu1, u);
+      else pre2d_fcg_k<true, 2>(nu1, u);
+    } else {
+      if (k == 0) pre2d_fcg_k<false, 0>(nu1, u);
+      else if (k == 1) pre2d_fcg_k<false, 1>(nu1, u);
+      else pre2d_fcg_k<false, 2>(nu1, u);
     }
   }
 
@@ -2225,14 +2381,16 @@ struct scihpc_mg {
   // 0 nothing, 1 FP64 residual norm (partials), 2 mixed outer step (oa).
   // Returns the number of residual partials written (0 when post0 == 0).
   template <typename T>
+  // pre0_done: the level-0 pre-smoothing has already run (k_pre2d_fcg)
   int vcycle(std::vector<Level<T>> &lv, int nu1, int nu2, bool zero_guess0, int post0 = 0,
-             const OuterArgs &oa = OuterArgs{}) {
+             const OuterArgs &oa = OuterArgs{}, bool pre0_done = false) {
     const int lb = block_level(lv);
     if (nu1 == 0 && zero_guess0) SCIHPC_HIP(hipMemsetAsync(lv[0].sol, 0, sizeof(T) * lv[0].n, stream));
     const bool zg = zero_guess0 && nu1 > 0;
     const bool can_tile = nu1 <= 2 && nu2 <= 2;  // the tiles stage a 5-cell halo
     const int top = std::min(lb, L - 1);  // deepest level reached by the fine loop
-    for (int l = 0; l < top; ++l) {
+    SCIHPC_REQUIRE(!pre0_done || (top >= 1 && can_tile && fused_level(lv, 0)), "internal: level 0 not tiled");
+    for (int l = pre0_done ? 1 : 0; l < top; ++l) {
       const int z = (l == 0 && zg) ? 1 : 0;
       if (can_tile && fused_level(lv, l)) {
         pre2d(lv, l, nu1, z);
@@ -2652,6 +2810,7 @@ struct scihpc_mg {
   // ------------------------------------------------------- solve
   // FP64 work vectors of the flexible-CG solve
   double *fx_ = nullptr, *fr_ = nullptr, *fp_[2] = {nullptr, nullptr}, *fq_ = nullptr;
+  double *fr2_ = nullptr;  // second residual buffer of the fused update (lazy)
   double last_rr_ = 0.0;  // final relative residual of the last flexible-CG solve
   // lagged convergence checks (opts.lagged_check): per path (stationary or
   // flexible CG, FP64 or mixed) the iteration count of the last converged
@@ -2707,6 +2866,14 @@ struct scihpc_mg {
     const bool zm = nzm > 0;
     const unsigned nbf = ceil_div(n, kFcgThreads);  // workgroups of the FCG vector kernels
     const int ndir = zm ? (int)nzm : (int)nbf;
+    // mixed 2-D solves on a tiled level 0: the update of x and r runs inside
+    // the next V-cycle's level-0 pre-smoothing (k_pre2d_fcg), r alternating
+    // between two buffers; never while a learned step is pending
+    const bool fuse_upd = mixed && fused_update && !zm && o.nu1 > 0 && o.nu1 <= 2 && o.nu2 <= 2 &&
+                          fused_level(f, 0) && block_level(f) > 0 &&
+                          (long)tiles2d(f[0]) <= 2 * (long)nb;
+    if (fuse_upd && !fr2_) fr2_ = dalloc<double>(n, stream);
+    bool pre0_done = false;
     while (rr > o.rel_tol && it + spent < o.max_cycles) {
       // z = B r; from the second iteration on the level-0 post kernel also
       // forms z . q_old (S_ZQ) for the flexible direction update
@@ -2715,7 +2882,7 @@ struct scihpc_mg {
       const void *z;
       int np;
       if (mixed) {
-        np = vcycle(f, o.nu1, o.nu2, true, it > 0 ? 3 : 0, oa);
+        np = vcycle(f, o.nu1, o.nu2, true, it > 0 ? 3 : 0, oa, pre0_done);
         z = f[0].sol;
       } else {
         np = vcycle(d, o.nu1, o.nu2, true, it > 0 ? 3 : 0, oa);
@@ -2751,17 +2918,26 @@ struct scihpc_mg {
       SCIHPC_CHECK_LAUNCH();
       if (!fuse) finalize2(ndir, S_PQ, S_PR);
       cur ^= 1;
-      k_fcg_update<<<nbf, kFcgThreads, 0, stream>>>(n, x, r, fp_[cur], fq_, scal, mixed ? f[0].rhs : nullptr, partial,
-                                              fuse ? dpart : nullptr, fuse ? ndir : 0);
-      SCIHPC_CHECK_LAUNCH();
+      const bool last = it + spent + 1 >= o.max_cycles;
+      const bool fused_now = fuse_upd && fuse && !learned_overlap && !corr_left && !last;
+      if (fused_now) {
+        FcgUpdate u{x, r, r == fr_ ? fr2_ : fr_, fp_[cur], fq_, scal, dpart, ndir, partial};
+        pre2d_fcg(o.nu1, u);
+        r = u.r_out;
+      } else {
+        k_fcg_update<<<nbf, kFcgThreads, 0, stream>>>(n, x, r, fp_[cur], fq_, scal, mixed ? f[0].rhs : nullptr,
+                                                      partial, fuse ? dpart : nullptr, fuse ? ndir : 0);
+        SCIHPC_CHECK_LAUNCH();
+      }
+      pre0_done = fused_now;
       // the learned guess is enqueued while the first iteration runs
       if (learned_overlap && !learned_pending) learned_enqueue(bnorm);
       // lagged checks: iterations before check_from run without reading
       // the residual (the host keeps enqueueing); never while a learned
       // step is pending, and always on the last allowed iteration
-      if (learned_overlap || corr_left || it + spent + 1 >= check_from || it + spent + 1 >= o.max_cycles) {
+      if (learned_overlap || corr_left || it + spent + 1 >= check_from || last) {
         if (!first_check_) first_check_ = it + spent + 1;
-        rr = std::sqrt(host_sum(nbf)) / scale;
+        rr = std::sqrt(host_sum(fused_now ? (int)tiles2d(f[0]) : (int)nbf)) / scale;
       }
       ++it;
       // the guess is judged at the end of the first iteration after which
@@ -2781,6 +2957,8 @@ struct scihpc_mg {
           spent += it;
           it = 0;
           cur = 0;
+          r = fr_;
+          pre0_done = false;
           SCIHPC_HIP(hipMemcpyAsync(x, l0.sol, sizeof(double) * n, hipMemcpyDeviceToDevice, stream));
           reduce_residual(l0, true);
           k_copy_cast<<<nb, kBlock, 0, stream>>>(l0.res, fr_, f[0].rhs, n);
@@ -2796,6 +2974,8 @@ struct scihpc_mg {
           spent += it;
           it = 0;
           cur = 0;
+          r = fr_;
+          pre0_done = false;
           rr = std::sqrt(fetch(S_NORM)) / scale;
         }
       }
@@ -3182,6 +3362,7 @@ extern "C" int scihpc_mg_create(int ndim, int nx, int ny, int nz, double dx, dou
       if (const char *e = std::getenv("SCIHPC_MG_BLOCK_MAX")) h->block_max = std::atol(e);
       if (const char *e = std::getenv("SCIHPC_MG_TILED")) h->tiled = std::atoi(e) != 0;
       if (const char *e = std::getenv("SCIHPC_MG_TILED3D")) h->tiled3d = std::atoi(e) != 0;
+      if (const char *e = std::getenv("SCIHPC_MG_FUSED_UPDATE")) h->fused_update = std::atoi(e) != 0;
       if (const char *e = std::getenv("SCIHPC_MG_Z3_CHUNK")) h->z3_chunk = std::atoi(e);
       std::vector<int> lx, ly, lz, ld;
       h->L = g_levels(ndim, nx, ny, nz, lx, ly, lz, ld);
@@ -3280,6 +3461,7 @@ extern "C" int scihpc_mg_destroy(scihpc_mg *h) {
   fr(h->fp_[0]);
   fr(h->fp_[1]);
   fr(h->fq_);
+  fr(h->fr2_);
   fr(h->scal);
   fr(h->p_alt);
   fr(h->rho0);

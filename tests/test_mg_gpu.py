@@ -374,3 +374,34 @@ def test_lagged_check(precision, accel):
     assert all(b <= a for a, b in zip(counts, counts[1:]))
     with pytest.raises(ScihpcError):
         g.solve(lagged_check=2, **kw)
+
+
+@pytest.mark.parametrize("lagged", [0, 1], ids=["checked", "lagged"])
+def test_fused_fcg_update_matches_separate_kernels(monkeypatch, lagged):
+    """The mixed 2-D FCG update fused into the next level-0 pre-smoothing
+    (the default) gives the iterates of the separate update kernel: the same
+    iteration counts and a bit-identical pressure, also when max_cycles stops
+    the solve."""
+    nx, ny = 128, 256
+    w = W.bubble_2d(nx, ny)
+    out = {}
+    for fused in ("1", "0"):
+        monkeypatch.setenv("SCIHPC_MG_FUSED_UPDATE", fused)
+        print(f"fused FCG update={fused} lagged={lagged}", flush=True)
+        g = Multigrid(2, nx, ny, 1, 1 / nx, 2 / ny, 1.0, "neumann")
+        g.set_density(w["rho"])
+        g.set_rhs(w["rhs"])
+        kw = dict(rel_tol=1e-9, precision=PREC_MIXED, accel=1, lagged_check=lagged)
+        sts = [g.solve(max_cycles=200, **kw) for _ in range(2)]
+        x_conv = g.sol(0)
+        capped = g.solve(max_cycles=3, **kw)
+        out[fused] = (sts, x_conv, capped, g.sol(0))
+        g.close()
+    (s1, x1, c1, y1), (s0, x0, c0, y0) = out["1"], out["0"]
+    for a, b in zip(s1, s0):
+        assert a["converged"] and b["converged"] and a["rel_residual"] <= 1e-9
+        assert a["cycles"] == b["cycles"]
+        assert abs(a["rel_residual"] - b["rel_residual"]) <= 1e-12 * b["rel_residual"]
+    np.testing.assert_array_equal(x1, x0)
+    assert not c1["converged"] and c1["cycles"] == c0["cycles"] == 3
+    np.testing.assert_array_equal(y1, y0)
