@@ -36,6 +36,10 @@ MFMA_BF16_PEAK = 2500.0  # TFLOP/s, dense
 # counter reads for a 256 MiB device copy (FETCH x 2.0, WRITE x 1.0):
 # profiles/pmc_traffic_k_pre2d_fp32_bubble_512x1024.md
 PRE2D_FP32_TRAFFIC_BYTES = (3977.4 * 2.0 + 3072.0 * 1.0) * 1024
+# the same for the FCG update fused into that kernel (the headline solve's
+# level-0 pre-smoothing from the second iteration on):
+# profiles/pmc_traffic_k_pre2d_fcg_bubble_512x1024.md
+FCG_PRE2D_TRAFFIC_BYTES = (10766.5 * 2.0 + 13344.0 * 1.0) * 1024
 
 
 def log(msg):
@@ -69,8 +73,9 @@ def cpu_baseline(w, seconds=12.0):
 
 
 def kernel_only(args):
-    """20 launches of the FP32 level-0 pre-smoothing kernel of the bubble
-    solve (after one solve), then 5 copies of 256 MiB with torch: reading
+    """21 launches each of the FP32 level-0 pre-smoothing kernel of the
+    bubble solve and of its form with the FCG update fused in (after one
+    mixed FCG solve), then 5 copies of 256 MiB with torch: reading
     and writing a known byte count calibrates the FETCH_SIZE / WRITE_SIZE
     counters."""
     import torch
@@ -86,12 +91,14 @@ def kernel_only(args):
     g.set_rhs(w["rhs"])
     g.solve(rel_tol=1e-8, precision=PREC_MIXED, accel=1)
     ms, b = g.time_kernel(1, iters=20)
+    ms6, b6 = g.time_kernel(6, iters=20)
     a = torch.empty(64 * 2 ** 20, dtype=torch.float32, device="cuda")
     c = torch.ones_like(a)
     for _ in range(5):
         a.copy_(c)
     torch.cuda.synchronize()
     print(json.dumps({"kernel": "k_pre2d<float,true,32,16>", "us": ms * 1e3, "bytes": b,
+                      "fused_kernel": "k_pre2d_fcg<true,32,16>", "fused_us": ms6 * 1e3, "fused_bytes": b6,
                       "calibration_copy_bytes_each_way": a.numel() * 4}), flush=True)
 
 
@@ -550,12 +557,16 @@ def main():
     # level-0 kernels, HIP events on the solver's stream
     kernels = {}
     for which, name in [(0, "fp64_pre_smooth_restrict"), (1, "fp32_pre_smooth_restrict"),
-                        (2, "fp32_post_smooth_fp64_outer"), (3, "fp64_post_smooth_norm")]:
+                        (2, "fp32_post_smooth_fp64_outer"), (3, "fp64_post_smooth_norm"),
+                        (6, "fcg_update_fp32_pre_smooth_restrict")]:
         ms_k, b_k = g.time_kernel(which, iters=50)
         kernels[name] = {"us": round(ms_k * 1e3, 2), "bytes": b_k, "GBps": round(b_k / (ms_k * 1e-3) / 1e9, 1)}
-    # dominant kernel of the headline solve: the FP32 level-0 pre-smoothing
-    # kernel (two red-black sweeps + residual + restriction)
-    kms, kbytes = kernels["fp32_pre_smooth_restrict"]["us"] * 1e-3, kernels["fp32_pre_smooth_restrict"]["bytes"]
+    # dominant kernel of the headline solve: the FCG update fused into the
+    # FP32 level-0 pre-smoothing (two red-black sweeps + residual +
+    # restriction), which runs on every iteration after the first
+    kernels["fp32_pre_smooth_restrict"]["traffic"] = round(PRE2D_FP32_TRAFFIC_BYTES)
+    dom = kernels["fcg_update_fp32_pre_smooth_restrict"]
+    kms, kbytes = dom["us"] * 1e-3, dom["bytes"]
     achieved = kbytes / (kms * 1e-3) / 1e9
 
     # U-Net training throughput on the matrix cores (level-1 grid, batch 4)
@@ -638,8 +649,9 @@ def main():
         "kernels": kernels,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": round(PRE2D_FP32_TRAFFIC_BYTES),
-                     "kernel": "k_pre2d<float,true> level 0 (2 RB-GS sweeps + residual + restriction)",
+                     "traffic": round(FCG_PRE2D_TRAFFIC_BYTES),
+                     "kernel": "k_pre2d_fcg<true,32,16> level 0 (FCG update of x and r + 2 RB-GS sweeps + "
+                               "residual + restriction)",
                      "note": "largest HBM-facing kernel; the largest by time is the one-workgroup coarse-level "
                              "cycle k_block_vcycle<float,2> (about 25 us per FCG iteration, on-chip, one CU)",
                      "bytes_per_launch": kbytes, "us_per_launch": round(kms * 1e3, 3),

@@ -190,8 +190,10 @@ SCIHPC_API int scihpc_mg_solve(scihpc_mg *h, const scihpc_solve_opts *opts,
  * FP32, 2 FP32 post-smoothing fused with the FP64 outer update, 3 FP64
  * post-smoothing with the residual norm, 4 the FP64 prolongation of level 1
  * added to level 0 alone, 5 the FP64 level-0 residual pass (partial sums of
- * the norm) alone.  Returns the mean milliseconds per launch and the
- * algorithmic HBM bytes of one launch. */
+ * the norm) alone, 6 the mixed flexible-CG update fused into the FP32
+ * level-0 pre-smoothing (2-D; after a mixed flexible-CG solve, on its
+ * vectors).  Returns the mean milliseconds per launch and the algorithmic
+ * HBM bytes of one launch. */
 SCIHPC_API int scihpc_mg_time_kernel(scihpc_mg *h, int which, int iters, double *ms,
                           double *bytes);
 

@@ -3783,7 +3783,7 @@ extern "C" int scihpc_mg_learned_features(scihpc_mg *h, int level, float *feat, 
 extern "C" int scihpc_mg_time_kernel(scihpc_mg *h, int which, int iters, double *ms, double *bytes) {
   return guarded([&] {
     check_level(h, 0);
-    SCIHPC_REQUIRE(iters > 0 && which >= 0 && which <= 5, "bad timing request");
+    SCIHPC_REQUIRE(iters > 0 && which >= 0 && which <= 6, "bad timing request");
     SCIHPC_REQUIRE(h->L > 1, "timing needs at least two levels");
     h->ensure_fp32();
     auto &l0 = h->d[0];
@@ -3815,6 +3815,16 @@ extern "C" int scihpc_mg_time_kernel(scihpc_mg *h, int which, int iters, double 
       }
       h->post0_generic(lv, mode, oa);
     };
+    // which = 6 runs on the vectors of the last mixed flexible-CG solve (x
+    // moves by alpha p on every launch; the next solve starts afresh)
+    FcgUpdate fu{};
+    if (which == 6) {
+      SCIHPC_REQUIRE(tiled && h->fr2_ != nullptr,
+                     "which = 6 needs a tiled 2-D level 0 and a mixed flexible-CG solve first");
+      const long nb = ceil_div(l0.n, kBlock);
+      fu = FcgUpdate{h->fx_, h->fr_, h->fr2_, h->fp_[0], h->fq_, h->scal, h->partial + 2 * nb,
+                     (int)ceil_div(l0.n, kFcgThreads), h->partial};
+    }
     auto launch = [&] {
       switch (which) {
         case 0:  // FP64 level-0 pre-smoothing (2 sweeps + residual + restriction)
@@ -3839,12 +3849,16 @@ extern "C" int scihpc_mg_time_kernel(scihpc_mg *h, int which, int iters, double 
         case 5:  // FP64 level-0 residual partial sums (the norm's pass over the grid), alone
           h->residual_partials(l0, false);
           break;
+        case 6:  // mixed FCG update fused into the FP32 level-0 pre-smoothing
+          h->pre2d_fcg(2, fu);
+          break;
       }
     };
     // algorithmic bytes: every array the launch touches, once per read and
     // once per write, at its stored precision
     double b = 0;
-    if (which == 4) b = 2.0 * 8.0 * n + 8.0 * nc;  // fine read + write, coarse read
+    if (which == 6) b = (4.0 * nf + 56.0) * n + 8.0 * nc;  // faces, FP64 r q x p, FP32 outputs
+    else if (which == 4) b = 2.0 * 8.0 * n + 8.0 * nc;  // fine read + write, coarse read
     else if (which == 5) b = (2.0 + nf) * 8.0 * n;  // solution, right-hand side, faces
     else if (tiled) {  // one fused kernel per launch
       switch (which) {

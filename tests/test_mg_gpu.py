@@ -327,8 +327,14 @@ def test_timing_hooks(ndim):
         assert math.isfinite(ms) and ms > 0 and b > 0
     assert g.time_kernel(4, iters=1)[1] == 16.0 * cells + 8.0 * coarse
     assert g.time_kernel(5, iters=1)[1] == (2 + ndim) * 8.0 * cells
+    if ndim == 2:  # the fused FCG update + pre-smoothing, on the last mixed FCG solve's vectors
+        ms, b = g.time_kernel(6, iters=2)
+        assert math.isfinite(ms) and ms > 0 and b == (4 * ndim + 56) * cells + 8.0 * coarse
+    else:
+        with pytest.raises(ScihpcError):
+            g.time_kernel(6, iters=1)
     with pytest.raises(ScihpcError):
-        g.time_kernel(6, iters=1)
+        g.time_kernel(7, iters=1)
     st = g.solve(rel_tol=1e-8, max_cycles=100, precision=PREC_MIXED, accel=1)
     assert st["converged"], st
 
