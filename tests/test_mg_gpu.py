@@ -339,8 +339,8 @@ def test_timing_hooks(ndim):
     assert st["converged"], st
 
 
-@pytest.mark.parametrize("precision,accel", [(PREC_FP64, 0), (PREC_MIXED, 0), (PREC_MIXED, 1)],
-                         ids=["fp64-stationary", "mixed-stationary", "mixed-fcg"])
+@pytest.mark.parametrize("precision,accel", [(PREC_FP64, 0), (PREC_MIXED, 0), (PREC_FP64, 1), (PREC_MIXED, 1)],
+                         ids=["fp64-stationary", "mixed-stationary", "fp64-fcg", "mixed-fcg"])
 def test_lagged_check(precision, accel):
     """lagged_check: the same answer as reading the residual every iteration
     once the count is known; after a harder solve, an easier start (a
