@@ -2037,11 +2037,11 @@ template <typename T>
 struct TileCfg;
 template <>
 struct TileCfg<double> {  // largest: 4 x 74 x 22 x 8 B + coarse tile = 57 KiB of LDS
-  static constexpr int TX[3] = {64, 32, 16}, TY[3] = {12, 16, 16};
+  static constexpr int TX[4] = {64, 32, 16, 8}, TY[4] = {12, 16, 16, 8};
 };
 template <>
 struct TileCfg<float> {  // largest: 4 x 74 x 42 x 4 B + coarse tile = 54 KiB of LDS
-  static constexpr int TX[3] = {64, 32, 16}, TY[3] = {32, 16, 16};
+  static constexpr int TX[4] = {64, 32, 16, 8}, TY[4] = {32, 16, 16, 8};
 };
 
 }  // namespace scihpc
@@ -2164,14 +2164,18 @@ struct scihpc_mg {
   }
   // tile shape index for a level (see TileCfg) and its tile count
   template <typename T>
-  int tile_kind(const Level<T> &l) const {
-    for (int t = 0; t < 2; ++t)
+  int tile_kind(const Level<T> &l, int level) const {
+    // the largest tile that still gives two workgroups per CU; 8 x 8 on the
+    // small latency-bound coarse levels (shorter chains per workgroup).
+    // Level 0 keeps at most one tile per 256 cells: its per-tile partial
+    // sums share the partial buffer with the FCG kernels'.
+    for (int t = 0; t < 3; ++t)
       if ((long)ceil_div(l.nx, TileCfg<T>::TX[t]) * ceil_div(l.ny, TileCfg<T>::TY[t]) >= 2L * num_cu) return t;
-    return 2;
+    return level == 0 ? 2 : 3;
   }
   template <typename T>
-  unsigned tiles2d(const Level<T> &l) const {
-    const int t = tile_kind(l);
+  unsigned tiles2d(const Level<T> &l, int level) const {
+    const int t = tile_kind(l, level);
     return ceil_div(l.nx, TileCfg<T>::TX[t]) * ceil_div(l.ny, TileCfg<T>::TY[t]);
   }
 
@@ -2179,22 +2183,24 @@ struct scihpc_mg {
   void pre2d_k(std::vector<Level<T>> &lv, int l, int nu1, int zero_guess) {
     constexpr int TX = TileCfg<T>::TX[K], TY = TileCfg<T>::TY[K];
     auto &a = lv[l];
-    k_pre2d<T, VAR, TX, TY><<<tiles2d(a), kFusedThreads, 0, stream>>>(view(a), view(lv[l + 1]), a.sol_alt, nu1,
+    k_pre2d<T, VAR, TX, TY><<<tiles2d(a, l), kFusedThreads, 0, stream>>>(view(a), view(lv[l + 1]), a.sol_alt, nu1,
                                                                         zero_guess);
     SCIHPC_CHECK_LAUNCH();
     std::swap(a.sol, a.sol_alt);
   }
   template <typename T>
   void pre2d(std::vector<Level<T>> &lv, int l, int nu1, int zero_guess) {
-    const int k = tile_kind(lv[l]);
+    const int k = tile_kind(lv[l], l);
     if (variable) {
       if (k == 0) pre2d_k<T, true, 0>(lv, l, nu1, zero_guess);
       else if (k == 1) pre2d_k<T, true, 1>(lv, l, nu1, zero_guess);
-      else pre2d_k<T, true, 2>(lv, l, nu1, zero_guess);
+      else if (k == 2) pre2d_k<T, true, 2>(lv, l, nu1, zero_guess);
+      else pre2d_k<T, true, 3>(lv, l, nu1, zero_guess);
     } else {
       if (k == 0) pre2d_k<T, false, 0>(lv, l, nu1, zero_guess);
       else if (k == 1) pre2d_k<T, false, 1>(lv, l, nu1, zero_guess);
-      else pre2d_k<T, false, 2>(lv, l, nu1, zero_guess);
+      else if (k == 2) pre2d_k<T, false, 2>(lv, l, nu1, zero_guess);
+      else pre2d_k<T, false, 3>(lv, l, nu1, zero_guess);
     }
   }
 
@@ -2203,20 +2209,20 @@ struct scihpc_mg {
   void pre2d_fcg_k(int nu1, const FcgUpdate &u) {
     constexpr int TX = TileCfg<float>::TX[K], TY = TileCfg<float>::TY[K];
     auto &a = f[0];
-    k_pre2d_fcg<VAR, TX, TY><<<tiles2d(a), kFusedThreads, 0, stream>>>(view(a), view(f[1]), a.sol_alt, nu1, u);
+    k_pre2d_fcg<VAR, TX, TY><<<tiles2d(a, 0), kFusedThreads, 0, stream>>>(view(a), view(f[1]), a.sol_alt, nu1, u);
     SCIHPC_CHECK_LAUNCH();
     std::swap(a.sol, a.sol_alt);
   }
   void pre2d_fcg(int nu1, const FcgUpdate &u) {
-    const int k = tile_kind(f[0]);
+    const int k = tile_kind(f[0], 0);
     if (variable) {
       if (k == 0) pre2d_fcg_k<true, 0>(nu1, u);
       else if (k == 1) pre2d_fcg_k<true, 1>(nu1, u);
-      else pre2d_fcg_k<true, 2>(nu1, u);
+      else pre2d_fcg_k<true, 2>(nu1, u);  // level 0: never the 8 x 8 kind
     } else {
       if (k == 0) pre2d_fcg_k<false, 0>(nu1, u);
       else if (k == 1) pre2d_fcg_k<false, 1>(nu1, u);
-      else pre2d_fcg_k<false, 2>(nu1, u);
+      else pre2d_fcg_k<false, 2>(nu1, u);  // level 0: never the 8 x 8 kind
     }
   }
 
@@ -2225,22 +2231,24 @@ struct scihpc_mg {
     constexpr int TX = TileCfg<T>::TX[K], TY = TileCfg<T>::TY[K];
     auto &a = lv[l];
     const int dir = bc == SCIHPC_MG_DIRICHLET;
-    k_post2d<T, VAR, TX, TY, MODE><<<tiles2d(a), kFusedThreads, 0, stream>>>(view(lv[l + 1]), view(a), a.sol_alt,
+    k_post2d<T, VAR, TX, TY, MODE><<<tiles2d(a, l), kFusedThreads, 0, stream>>>(view(lv[l + 1]), view(a), a.sol_alt,
                                                                                nu2, dir, partial, oa);
     SCIHPC_CHECK_LAUNCH();
     if (MODE != 2) std::swap(a.sol, a.sol_alt);
   }
   template <typename T, int MODE>
   void post2d_mode(std::vector<Level<T>> &lv, int l, int nu2, const OuterArgs &oa) {
-    const int k = tile_kind(lv[l]);
+    const int k = tile_kind(lv[l], l);
     if (variable) {
       if (k == 0) post2d_k<T, true, 0, MODE>(lv, l, nu2, oa);
       else if (k == 1) post2d_k<T, true, 1, MODE>(lv, l, nu2, oa);
-      else post2d_k<T, true, 2, MODE>(lv, l, nu2, oa);
+      else if (k == 2) post2d_k<T, true, 2, MODE>(lv, l, nu2, oa);
+      else post2d_k<T, true, 3, MODE>(lv, l, nu2, oa);
     } else {
       if (k == 0) post2d_k<T, false, 0, MODE>(lv, l, nu2, oa);
       else if (k == 1) post2d_k<T, false, 1, MODE>(lv, l, nu2, oa);
-      else post2d_k<T, false, 2, MODE>(lv, l, nu2, oa);
+      else if (k == 2) post2d_k<T, false, 2, MODE>(lv, l, nu2, oa);
+      else post2d_k<T, false, 3, MODE>(lv, l, nu2, oa);
     }
   }
 
@@ -2419,7 +2427,7 @@ struct scihpc_mg {
         else post2d_mode<T, 3>(lv, f, nu2, oa);
         if (f == 0) {
           post0_done = true;
-          partials = mode != 0 ? (int)tiles2d(lv[0]) : 0;
+          partials = mode != 0 ? (int)tiles2d(lv[0], 0) : 0;
         }
       } else if (z3_level(lv, f)) {
         prolong_add(lv[l], lv[f]);
@@ -2871,7 +2879,7 @@ struct scihpc_mg {
     // between two buffers; never while a learned step is pending
     const bool fuse_upd = mixed && fused_update && !zm && o.nu1 > 0 && o.nu1 <= 2 && o.nu2 <= 2 &&
                           fused_level(f, 0) && block_level(f) > 0 &&
-                          (long)tiles2d(f[0]) <= 2 * (long)nb;
+                          (long)tiles2d(f[0], 0) <= 2 * (long)nb;
     if (fuse_upd && !fr2_) fr2_ = dalloc<double>(n, stream);
     bool pre0_done = false;
     while (rr > o.rel_tol && it + spent < o.max_cycles) {
@@ -2891,7 +2899,8 @@ struct scihpc_mg {
       // small reductions are folded into the consumers: z.q_old into the
       // direction kernel, p.q and p.r into the update (its own partials go
       // to [0, nbf), the direction's to [2 nb, 2 nb + 2 ndir))
-      const bool fuse = nbf <= kCompressAbove && np <= kCompressAbove && ndir <= (int)nb;
+      // (z.q_old partials at [0, np) must end before the direction's)
+      const bool fuse = nbf <= kCompressAbove && np <= kCompressAbove && np <= 2 * (int)nb && ndir <= (int)nb;
       if (it > 0 && (!fuse || zm)) finalize(np, S_ZQ);
       double *pold = fp_[cur], *pnew = fp_[cur ^ 1];
       double *dpart = fuse ? partial + 2 * (long)nb : partial;
@@ -2937,7 +2946,7 @@ struct scihpc_mg {
       // step is pending, and always on the last allowed iteration
       if (learned_overlap || corr_left || it + spent + 1 >= check_from || last) {
         if (!first_check_) first_check_ = it + spent + 1;
-        rr = std::sqrt(host_sum(fused_now ? (int)tiles2d(f[0]) : (int)nbf)) / scale;
+        rr = std::sqrt(host_sum(fused_now ? (int)tiles2d(f[0], 0) : (int)nbf)) / scale;
       }
       ++it;
       // the guess is judged at the end of the first iteration after which
