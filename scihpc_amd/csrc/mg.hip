@@ -2170,7 +2170,9 @@ struct scihpc_mg {
     // Level 0 keeps at most one tile per 256 cells: its per-tile partial
     // sums share the partial buffer with the FCG kernels'.
     for (int t = 0; t < 3; ++t)
-      if ((long)ceil_div(l.nx, TileCfg<T>::TX[t]) * ceil_div(l.ny, TileCfg<T>::TY[t]) >= 2L * num_cu) return t;
+      if ((long)ceil_div(l.nx, TileCfg<T>::TX[t]) * ceil_div(l.ny, TileCfg<T>::TY[t]) >=
+          (t == 2 && level > 0 ? 4L : 2L) * num_cu)
+        return t;
     return level == 0 ? 2 : 3;
   }
   template <typename T>
