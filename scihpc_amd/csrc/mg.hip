@@ -537,7 +537,7 @@ __global__ void __launch_bounds__(kBlockThreads)
 // Tiles are ordered so the workgroups that land on one XCD (round robin
 // over 8) own a contiguous band of rows and share halos in that XCD's L2.
 constexpr int kFH = 5;  // staged halo: 2*nu + 1 with nu <= 2
-constexpr int kFusedThreads = 256;
+constexpr int kFusedThreads = 512;
 
 __device__ __forceinline__ int xcd_tile(int ntiles) {
   const int b = blockIdx.x;
@@ -2185,7 +2185,9 @@ struct scihpc_mg {
   void pre2d_k(std::vector<Level<T>> &lv, int l, int nu1, int zero_guess) {
     constexpr int TX = TileCfg<T>::TX[K], TY = TileCfg<T>::TY[K];
     auto &a = lv[l];
-    k_pre2d<T, VAR, TX, TY><<<tiles2d(a, l), kFusedThreads, 0, stream>>>(view(a), view(lv[l + 1]), a.sol_alt, nu1,
+    // 8 x 8 tiles (18 x 18 staged cells) run with four waves
+    constexpr int threads = TX * TY <= 64 ? kFusedThreads / 2 : kFusedThreads;
+    k_pre2d<T, VAR, TX, TY><<<tiles2d(a, l), threads, 0, stream>>>(view(a), view(lv[l + 1]), a.sol_alt, nu1,
                                                                         zero_guess);
     SCIHPC_CHECK_LAUNCH();
     std::swap(a.sol, a.sol_alt);
@@ -2233,7 +2235,8 @@ struct scihpc_mg {
     constexpr int TX = TileCfg<T>::TX[K], TY = TileCfg<T>::TY[K];
     auto &a = lv[l];
     const int dir = bc == SCIHPC_MG_DIRICHLET;
-    k_post2d<T, VAR, TX, TY, MODE><<<tiles2d(a, l), kFusedThreads, 0, stream>>>(view(lv[l + 1]), view(a), a.sol_alt,
+    constexpr int threads = TX * TY <= 64 ? kFusedThreads / 2 : kFusedThreads;
+    k_post2d<T, VAR, TX, TY, MODE><<<tiles2d(a, l), threads, 0, stream>>>(view(lv[l + 1]), view(a), a.sol_alt,
                                                                                nu2, dir, partial, oa);
     SCIHPC_CHECK_LAUNCH();
     if (MODE != 2) std::swap(a.sol, a.sol_alt);
