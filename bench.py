@@ -35,11 +35,11 @@ MFMA_BF16_PEAK = 2500.0  # TFLOP/s, dense
 # (FETCH_SIZE, WRITE_SIZE) of `bench.py --kernel-only`, each scaled by what the
 # counter reads for a 256 MiB device copy (FETCH x 2.0, WRITE x 1.0):
 # profiles/pmc_traffic_k_pre2d_fp32_bubble_512x1024.md
-PRE2D_FP32_TRAFFIC_BYTES = (3977.4 * 2.0 + 3072.0 * 1.0) * 1024
+PRE2D_FP32_TRAFFIC_BYTES = (4362.8 * 2.0 + 3072.0 * 1.0) * 1024
 # the same for the FCG update fused into that kernel (the headline solve's
 # level-0 pre-smoothing from the second iteration on):
 # profiles/pmc_traffic_k_pre2d_fcg_bubble_512x1024.md
-FCG_PRE2D_TRAFFIC_BYTES = (10766.5 * 2.0 + 13344.0 * 1.0) * 1024
+FCG_PRE2D_TRAFFIC_BYTES = (10757.0 * 2.0 + 13344.0 * 1.0) * 1024
 
 
 def log(msg):
