@@ -103,6 +103,21 @@ SCIHPC_API int scihpc_mg_set_nocompat(scihpc_mg *h, int level, int flag);
 /* multigrid_base::residual() (multigrid_base.cpp:69-72): res = rhs - A sol,
  * returns ||res||_2 (absolute) */
 SCIHPC_API int scihpc_mg_residual(scihpc_mg *h, int level, double *norm);
+/* No counterpart in the reference: a variant of scihpc_mg_residual for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  No handle and no
+ * allocation: res = b - A x on the level-0 grid of caller device arrays (MG
+ * layout) and *norm = ||res||_2, both written to device memory, enqueued on
+ * `stream` (a hipStream_t; NULL = the default stream) without a host sync.
+ * A = A(rho) (face 1/(0.5*(rho_i+rho_j))/h^2, zero at walls, as
+ * scihpc_mg_set_density; bc must be SCIHPC_MG_NEUMANN) when rho is given,
+ * else the constant operator of `bc`.  res may be NULL.  The per-block
+ * partial sums live in one fixed device array per GPU (512.5 KiB), which
+ * limits the grid to 2^24 cells (256^3); larger grids return SCIHPC_BAD_ARG
+ * before any launch.  Calls that overlap on one GPU race on that array. */
+SCIHPC_API int scihpc_mg_residual_noalloc(int ndim, int nx, int ny, int nz, double dx, double dy,
+                                          double dz, int bc, const double *rho, const double *x,
+                                          const double *b, double *res, double *norm, void *stream);
 /* multigrid_base::Ax (multigrid_base.cpp:388-390) on a host vector */
 SCIHPC_API int scihpc_mg_apply(scihpc_mg *h, int level, const double *x, double *y);
 /* multigrid_base::compatibility_condition (multigrid_base.cpp:47-67) on

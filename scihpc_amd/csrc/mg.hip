@@ -22,6 +22,8 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
+#include <string>
 #include <type_traits>
 #include <vector>
 
@@ -3611,6 +3613,152 @@ extern "C" int scihpc_mg_residual(scihpc_mg *h, int level, double *norm) {
     check_level(h, level);
     const double r = h->residual_norm(level, true);
     if (norm) *norm = r;
+  });
+}
+
+// ------------------------------------------- residual without allocation
+// scihpc_mg_residual_noalloc: the level-0 residual of scihpc_mg_residual on
+// caller device arrays, with no handle and no allocation.  Its only device
+// memory beyond the inputs and outputs is `scratch`: the per-block partial
+// sums of ||r||^2 (one per kBlock cells) and their first-stage reduction.
+// Every call on a GPU shares that GPU's copy; each call writes every entry
+// it reads before reading it, so nothing has to start at zero.
+namespace scihpc {
+constexpr long kNoAllocPartials = 65536;                 // blocks: n <= 2^24 cells
+constexpr long kNoAllocCells = kNoAllocPartials * kBlock;
+constexpr int kNoAllocDevices = 64;
+}  // namespace scihpc
+
+static __device__ __align__(256) double scratch[scihpc::kNoAllocPartials + 512];
+
+namespace scihpc {
+// b - A x on level 0: A(rho) with the face coefficients formed on the fly
+// (the expression and order of k_faces_from_rho, summed in stencil's order),
+// or the constant operator of dv (rho == nullptr); res (nullable) written,
+// partial[block] = sum of r^2 over the block's cells
+template <int DIM, bool VAR>
+__global__ void __launch_bounds__(kBlock)
+    k_residual_free(DLevel<double> L, const double *__restrict__ rho, const double *__restrict__ x,
+                    const double *__restrict__ b, double *__restrict__ res, double dx, double dy, double dz,
+                    double *__restrict__ partial) {
+  const long c = (long)xcd_block() * blockDim.x + threadIdx.x;
+  double v = 0.0;
+  if (c < L.n) {
+    int i, j, k;
+    cell_ijk(c, L.nx, L.ny, i, j, k);
+    double off, diag;
+    if constexpr (VAR) {
+      const long sy = L.nx, sz = (long)L.nx * L.ny;
+      const bool hl = i > 0, hr = i < L.nx - 1, hd = j > 0, hu = j < L.ny - 1;
+      const double rc = rho[c];
+      const double cR = hr ? 1.0 / (0.5 * (rc + rho[c + 1])) / dx / dx : 0.0;
+      const double cL = hl ? 1.0 / (0.5 * (rho[c - 1] + rc)) / dx / dx : 0.0;
+      const double cU = hu ? 1.0 / (0.5 * (rc + rho[c + sy])) / dy / dy : 0.0;
+      const double cD = hd ? 1.0 / (0.5 * (rho[c - sy] + rc)) / dy / dy : 0.0;
+      double o = 0.0;
+      o += cL * x[hl ? c - 1 : c];
+      o += cR * x[hr ? c + 1 : c];
+      o += cD * x[hd ? c - sy : c];
+      o += cU * x[hu ? c + sy : c];
+      double d = -(cR + cL + cU + cD);
+      if constexpr (DIM == 3) {
+        const bool hb = k > 0, hf = k < L.nz - 1;
+        const double cF = hf ? 1.0 / (0.5 * (rc + rho[c + sz])) / dz / dz : 0.0;
+        const double cB = hb ? 1.0 / (0.5 * (rho[c - sz] + rc)) / dz / dz : 0.0;
+        o += cB * x[hb ? c - sz : c];
+        o += cF * x[hf ? c + sz : c];
+        d -= (cF + cB);
+      }
+      off = o;
+      diag = d;
+    } else {
+      stencil<double, DIM, false>(L, x, i, j, k, c, off, diag);
+    }
+    const double r = b[c] - (off + diag * x[c]);
+    if (res) res[c] = r;
+    v = r * r;
+  }
+  v = block_sum(v);
+  if (threadIdx.x == 0) partial[blockIdx.x] = v;
+}
+
+// *out = sqrt(sum of partial[0..nb)), summed as k_finalize sums
+__global__ void k_norm_out(const double *__restrict__ partial, int nb, double *__restrict__ out) {
+  double s = 0.0;
+  for (int q = threadIdx.x; q < nb; q += blockDim.x) s += partial[q];
+  s = block_sum(s);
+  if (threadIdx.x == 0) *out = sqrt(s);
+}
+
+// device address of `scratch` on the current device, looked up once per
+// device (the array's host-side name is not a device address)
+static double *noalloc_scratch() {
+  static std::mutex mu;
+  static double *ptr[kNoAllocDevices] = {};
+  int dev = 0;
+  SCIHPC_HIP(hipGetDevice(&dev));
+  SCIHPC_REQUIRE(dev >= 0 && dev < kNoAllocDevices, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(mu);
+  if (!ptr[dev]) {
+    void *p = nullptr;
+    SCIHPC_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    SCIHPC_REQUIRE(p != nullptr, "hipGetSymbolAddress returned no address for the scratch array");
+    ptr[dev] = static_cast<double *>(p);
+  }
+  return ptr[dev];
+}
+}  // namespace scihpc
+
+extern "C" SCIHPC_API int scihpc_mg_residual_noalloc(int ndim, int nx, int ny, int nz, double dx, double dy, double dz,
+                                          int bc, const double *rho, const double *x, const double *b,
+                                          double *res, double *norm, void *stream) {
+  return guarded([&] {
+    SCIHPC_REQUIRE(ndim == 2 || ndim == 3, "ndim must be 2 or 3");
+    if (ndim == 2) {
+      nz = 1;
+      dz = 1.0;
+    }
+    SCIHPC_REQUIRE(nx > 0 && ny > 0 && nz > 0, "grid sizes must be positive");
+    SCIHPC_REQUIRE(dx > 0.0 && dy > 0.0 && dz > 0.0, "spacings must be positive");
+    SCIHPC_REQUIRE(bc == SCIHPC_MG_NEUMANN || bc == SCIHPC_MG_DIRICHLET, "bc must be NEUMANN or DIRICHLET");
+    SCIHPC_REQUIRE(rho == nullptr || bc == SCIHPC_MG_NEUMANN, "A(rho) needs SCIHPC_MG_NEUMANN walls");
+    SCIHPC_REQUIRE(x && b && norm, "x, b and norm must be device pointers");
+    const long n = (long)nx * ny * nz;
+    if (n > kNoAllocCells)
+      throw Error(SCIHPC_BAD_ARG, "scihpc_mg_residual_noalloc: " + std::to_string(n) + " cells exceed the " +
+                                      std::to_string(kNoAllocCells) + " its fixed scratch array serves");
+    double *part = noalloc_scratch();
+    DLevel<double> L{};
+    L.nx = nx;
+    L.ny = ny;
+    L.nz = nz;
+    L.n = n;
+    L.ax = 1.0 / dx / dx;
+    L.ay = 1.0 / dy / dy;
+    L.az = ndim == 3 ? 1.0 / dz / dz : 0.0;
+    L.dirichlet = bc == SCIHPC_MG_DIRICHLET;
+    L.dconst = ndim == 2 ? -2.0 / dx / dx - 2.0 / dy / dy : -2.0 / dx / dx - 2.0 / dy / dy - 2.0 / dz / dz;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const unsigned nb = ceil_div(n, kBlock);
+    if (ndim == 2) {
+      if (rho) k_residual_free<2, true><<<nb, kBlock, 0, s>>>(L, rho, x, b, res, dx, dy, dz, part);
+      else k_residual_free<2, false><<<nb, kBlock, 0, s>>>(L, rho, x, b, res, dx, dy, dz, part);
+    } else {
+      if (rho) k_residual_free<3, true><<<nb, kBlock, 0, s>>>(L, rho, x, b, res, dx, dy, dz, part);
+      else k_residual_free<3, false><<<nb, kBlock, 0, s>>>(L, rho, x, b, res, dx, dy, dz, part);
+    }
+    SCIHPC_CHECK_LAUNCH();
+    const double *src = part;
+    int cnt = (int)nb;
+    if (nb > 2048) {  // first stage into the tail of scratch (at most 256 sums)
+      const int m = (int)ceil_div(nb, kBlock);
+      k_compress<<<m, kBlock, 0, s>>>(part, (int)nb, kBlock, part + kNoAllocPartials);
+      SCIHPC_CHECK_LAUNCH();
+      src = part + kNoAllocPartials;
+      cnt = m;
+    }
+    k_norm_out<<<1, kBlock, 0, s>>>(src, cnt, norm);
+    SCIHPC_CHECK_LAUNCH();
   });
 }
 
