@@ -17,6 +17,9 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define ORC_MAXL 32
 #define ORC_GHC 3
@@ -64,6 +67,19 @@ int orc_mg_levels(int ndim, int nx, int ny, int nz, int maxl, int *lnx,
   return L;
 }
 
+/* OpenMP threads of the parallel loops (no-op in a serial build).  Only
+ * loops whose iterations write disjoint cells run in parallel; every
+ * reduction stays serial, so results do not depend on the thread count. */
+int orc_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+#else
+  (void)n;
+  return 1;
+#endif
+}
+
 /* ================================================================== */
 /* multigrid object                                                    */
 typedef struct {
@@ -89,6 +105,9 @@ typedef struct {
   int rkind; /* restriction: 0 child average, 1 full weighting (P^T/r^d) */
   int pkind; /* prolongation: 0 (bi/tri)linear, 1 piecewise constant     */
   orc_lvl lv[ORC_MAXL];
+  /* flexible-CG vectors of level 0 (orc_mg_solve_fcg): x, r, p, q, p_old,
+   * q_old, z, b; allocated on first use                                 */
+  double *fcg[8];
 } orc_mg;
 
 void orc_mg_set_transfer(void *h, int rkind, int pkind) {
@@ -218,6 +237,7 @@ void orc_mg_destroy(void *h) {
     free(v->lu_rb);
     free(v->piv_rb);
   }
+  for (int q = 0; q < 8; q++) free(m->fcg[q]);
   free(m);
 }
 
@@ -345,6 +365,7 @@ void orc_mg_get_operator(void *h, int level, double *diag, double *cx,
 static void lvl_apply_d(const orc_lvl *l, const double *diag, const double *x, double *y) {
   const int nx = l->nx, ny = l->ny, nz = l->nz;
   const long sxy = (long)nx * ny;
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 0; k < nz; k++)
     for (int j = 0; j < ny; j++)
       for (int i = 0; i < nx; i++) {
@@ -374,6 +395,7 @@ static double lvl_residual(orc_lvl *l) {
   const int nx = l->nx, ny = l->ny, nz = l->nz;
   const long sxy = (long)nx * ny;
   const double *x = l->sol;
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 0; k < nz; k++)
     for (int j = 0; j < ny; j++)
       for (int i = 0; i < nx; i++) {
@@ -421,6 +443,7 @@ static void lvl_relax(orc_lvl *l, int iter) {
     double alpha = 0.0;
     for (int c = 0; c < l->n; c++) alpha += l->res[c] * l->buf[c];
     alpha = rdot / alpha;
+#pragma omp parallel for schedule(static)
     for (int c = 0; c < l->n; c++) l->sol[c] += alpha * l->res[c];
     lvl_compat(l, l->sol);
   }
@@ -440,6 +463,7 @@ void orc_mg_restriction(void *h, int level) {
   lvl_residual(f);
   const int r = g->deg / f->deg;
   if (m->ndim == 2) {
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < g->nx; ++i)
       for (int j = 0; j < g->ny; ++j) {
         long C = i + (long)g->nx * j;
@@ -451,6 +475,7 @@ void orc_mg_restriction(void *h, int level) {
         g->sol[C] = 0.0;
       }
   } else {
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < g->nx; ++i)
       for (int j = 0; j < g->ny; ++j)
         for (int k = 0; k < g->nz; ++k) {
@@ -478,6 +503,7 @@ void orc_mg_prolongation(void *h, int level) {
   const long sxy = (long)nx * ny;
   const double *s = c->sol;
   if (m->ndim == 2) {
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < nx; ++i)
       for (int j = 0; j < ny; ++j) {
         long o = i + (long)nx * j;
@@ -503,6 +529,7 @@ void orc_mg_prolongation(void *h, int level) {
           }
       }
   } else {
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < nx; ++i)
       for (int j = 0; j < ny; ++j)
         for (int k = 0; k < nz; ++k) {
@@ -685,10 +712,11 @@ static void lvl_gs_color(const orc_lvl *l, int color) {
   const int nx = l->nx, ny = l->ny, nz = l->nz;
   const long sxy = (long)nx * ny;
   double *x = l->sol;
+  /* cells of one colour only couple to the other colour: rows in parallel */
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 0; k < nz; k++)
     for (int j = 0; j < ny; j++)
-      for (int i = 0; i < nx; i++) {
-        if (((i + j + k) & 1) != color) continue;
+      for (int i = (color + j + k) & 1; i < nx; i += 2) {
         long c = i + (long)nx * (j + (long)ny * k);
         double t = 0.0;
         if (i > 0) t += l->cx[c - 1] * x[c - 1];
@@ -719,12 +747,14 @@ void orc_mg_restrict_rb(void *h, int level) {
   orc_mg *m = (orc_mg *)h;
   orc_lvl *f = &m->lv[level], *g = &m->lv[level + 1];
   lvl_apply_d(f, f->diag_rb, f->sol, f->buf);
+#pragma omp parallel for schedule(static)
   for (int c = 0; c < f->n; c++) f->res[c] = f->rhs[c] - f->buf[c];
   const int r = g->deg / f->deg;
   const int dir = m->bc == 1;
   const double inv = 1.0 / (double)(m->ndim == 3 ? r * r * r : r * r);
   if (m->rkind == 0) {
     const int rz = (m->ndim == 3) ? r : 1;
+#pragma omp parallel for collapse(2) schedule(static)
     for (int K = 0; K < g->nz; K++)
       for (int J = 0; J < g->ny; J++)
         for (int I = 0; I < g->nx; I++) {
@@ -826,6 +856,7 @@ void orc_mg_prolong_rb(void *h, int level) {
               g->sol[i / r + (long)g->nx * (j / r + (long)g->ny * (m->ndim == 3 ? k / r : 0))];
     return;
   }
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 0; k < f->nz; k++)
     for (int j = 0; j < f->ny; j++)
       for (int i = 0; i < f->nx; i++) {
@@ -902,6 +933,87 @@ int orc_mg_solve_rb(void *h, double rel_tol, int max_cycles, int nu1, int nu2,
     for (int i = 0; i < l->n; i++) l->sol[i] -= s;
   }
   return c;
+}
+
+static double vec_dot(const double *a, const double *b, long n) {
+  double s = 0.0;
+  for (long i = 0; i < n; i++) s += a[i] * b[i];
+  return s;
+}
+
+/* Flexible CG, FCG(1) (Notay, SIAM J. Sci. Comput. 22 (2000) 1444-1460),
+ * preconditioned by one red-black V(nu1,nu2) from zero on level 0:
+ *   x = 0, r = b
+ *   repeat: z = V(r); p = z (first) or z + beta p_old with
+ *           beta = -(z.q_old)/(p_old.q_old); q = A p;
+ *           alpha = (p.r)/(p.q); x += alpha p; r -= alpha q
+ *   until ||r||/||b|| <= rel_tol (the recurrence residual) or max_iter.
+ * beta and alpha are 0 when their denominator is exactly 0 (only after
+ * exact convergence).  Neumann: the result is made mean-free.  b is
+ * level 0's rhs (restored on return), the result goes to level 0's sol.
+ * The build's outer loop (no reference counterpart; the closest analogue is
+ * the FPC loop, projection_method.cpp:694-706).                          */
+int orc_mg_solve_fcg(void *h, double rel_tol, int max_iter, int nu1, int nu2,
+                     double *hist) {
+  orc_mg *m = (orc_mg *)h;
+  orc_lvl *l = &m->lv[0];
+  const long n = l->n;
+  for (int q = 0; q < 8; q++)
+    if (!m->fcg[q]) m->fcg[q] = (double *)calloc(n, sizeof(double));
+  double *x = m->fcg[0], *r = m->fcg[1], *p = m->fcg[2], *q = m->fcg[3];
+  double *po = m->fcg[4], *qo = m->fcg[5], *z = m->fcg[6], *b = m->fcg[7];
+  memcpy(b, l->rhs, sizeof(double) * n);
+  memcpy(r, b, sizeof(double) * n);
+  memset(x, 0, sizeof(double) * n);
+  double bn = vec_norm(b, n);
+  const double scale = bn > 0.0 ? bn : 1.0;
+  double rr = bn > 0.0 ? 1.0 : 0.0;
+  int it = 0;
+  while (rr > rel_tol && it < max_iter) {
+    memcpy(l->rhs, r, sizeof(double) * n);
+    memset(l->sol, 0, sizeof(double) * n);
+    orc_mg_vcycle_rb(h, nu1, nu2);
+    memcpy(z, l->sol, sizeof(double) * n);
+    if (it == 0) {
+      memcpy(p, z, sizeof(double) * n);
+    } else {
+      double t;
+      memcpy(po, p, sizeof(double) * n);
+      memcpy(qo, q, sizeof(double) * n);
+      const double pq_old = vec_dot(po, qo, n);
+      const double beta = pq_old != 0.0 ? -vec_dot(z, qo, n) / pq_old : 0.0;
+      for (long c = 0; c < n; c++) {
+        t = z[c] + beta * po[c];
+        p[c] = t;
+      }
+    }
+    lvl_apply(l, p, q);
+    const double pq = vec_dot(p, q, n);
+    const double alpha = pq != 0.0 ? vec_dot(p, r, n) / pq : 0.0;
+    for (long c = 0; c < n; c++) {
+      x[c] += alpha * p[c];
+      r[c] -= alpha * q[c];
+    }
+    rr = vec_norm(r, n) / scale;
+    if (hist) hist[it] = rr;
+    it++;
+  }
+  memcpy(l->rhs, b, sizeof(double) * n);
+  memcpy(l->sol, x, sizeof(double) * n);
+  if (m->bc == 0) {
+    double s = 0.0;
+    for (long i = 0; i < n; i++) s += l->sol[i];
+    s /= (double)n;
+    for (long i = 0; i < n; i++) l->sol[i] -= s;
+  }
+  return it;
+}
+
+/* the flexible-CG vectors after orc_mg_solve_fcg: 0 x (before the mean
+ * removal), 1 r, 2 p, 3 q, 4 p_old, 5 q_old, 6 z of the last iteration */
+double *orc_mg_fcg_vec(void *h, int which) {
+  orc_mg *m = (orc_mg *)h;
+  return (which >= 0 && which < 8) ? m->fcg[which] : NULL;
 }
 
 int orc_mg_dense_solve_level0(void *h, const double *b, double *x) {
@@ -2277,6 +2389,21 @@ void orc_find_sign(long n, const double *phi, double ls_width, double *sign) {
  * b = |grad| delta^2 (16-weight centre + 4 diagonal neighbours in 2-D,
  * 51 + 8 in 3-D), |b| > 1e-8.  sign and delta are the dummy arrays the
  * reference reads (find_sign / find_delta of earlier calls).            */
+/* stabilized_upon_gradient (godunov_gradient.cpp:159-181): the Godunov
+ * gradient with the stored sign, its maximum over every padded cell (a
+ * running fmax from 0.0) and phi /= that maximum on every cell.        */
+double orc_stabilize_gradient(int ndim, int nx, int ny, int nz, double dx, double dy, double dz,
+                              const double *sign, double *phi, double *grad) {
+  const orc_pf p = pf_make(ndim, nx, ny, nz);
+  const long T = (long)p.N[0] * p.N[1] * p.N[2];
+  const double hh[3] = {dx, dy, dz};
+  orc_godunov(&p, hh, phi, sign, grad);
+  double mx = 0.0;
+  for (long c = 0; c < T; c++) mx = fmax(mx, grad[c]);
+  for (long c = 0; c < T; c++) phi[c] /= mx;
+  return mx;
+}
+
 void orc_redistance_rk3(int ndim, int nx, int ny, int nz, double dx, double dy, double dz, double dt, double rdt,
                         const double *sign, const double *delta, const int *types, const double *values,
                         double *phi, int nsteps) {

@@ -29,6 +29,10 @@
 extern "C" {
 #endif
 
+/* OpenMP threads of the loops that write disjoint cells (every reduction
+ * stays serial, so results do not depend on n); returns the count in use. */
+int orc_set_threads(int n);
+
 /* ------------------------------------------------------------------ */
 /* multigrid hierarchy (multigrid.cpp:8-41, 81-122)          [golden]   */
 int orc_mg_levels(int ndim, int nx, int ny, int nz, int maxl,
@@ -86,6 +90,19 @@ void orc_mg_vcycle_rb(void *h, int nu1, int nu2);
  * used, writes the relative residual after each cycle into hist[0..]    */
 int orc_mg_solve_rb(void *h, double rel_tol, int max_cycles, int nu1, int nu2,
                     double *hist);
+/* flexible CG, FCG(1) (Notay 2000), with one V(nu1,nu2) of
+ * orc_mg_vcycle_rb from zero as the preconditioner; b = level 0's rhs, x0 =
+ * 0, stops on the recurrence residual ||r||/||b|| <= rel_tol; result in
+ * level 0's sol (mean-free for Neumann); hist[k] = relative residual after
+ * iteration k.  Returns iterations.  Build's outer loop (closest analogue:
+ * projection_method.cpp:694-706).                                [math]
+ * Pinned by the dense solve, the A-conjugacy p.q_old = 0 and the
+ * orthogonality p.r = 0 after each step (tests/test_oracle_math.py). */
+int orc_mg_solve_fcg(void *h, double rel_tol, int max_iter, int nu1, int nu2,
+                     double *hist);
+/* its level-0 vectors after the solve: 0 x (before the mean removal), 1 r,
+ * 2 p, 3 q, 4 p_old, 5 q_old, 6 z of the last iteration              */
+double *orc_mg_fcg_vec(void *h, int which);
 /* dense direct solve of level 0 (n <= 4096): Dirichlet -> plain LU;
  * Neumann -> pinned system, mean-free result.                   [math]   */
 int orc_mg_dense_solve_level0(void *h, const double *b, double *x);
@@ -257,6 +274,13 @@ void orc_find_sign(long n, const double *phi, double ls_width, double *sign);
 void orc_redistance_rk3(int ndim, int nx, int ny, int nz, double dx, double dy, double dz, double dt,
                         double rdt, const double *sign, const double *delta, const int *types,
                         const double *values, double *phi, int nsteps);
+
+/* stabilized_upon_gradient (godunov_gradient.cpp:159-181): phi /= the
+ * largest Godunov gradient (godunov_gradient.cpp:7-152, stored sign) over
+ * the padded array; grad receives the gradient; returns the maximum.
+ *                                                             [golden] */
+double orc_stabilize_gradient(int ndim, int nx, int ny, int nz, double dx, double dy, double dz,
+                              const double *sign, double *phi, double *grad);
 
 #ifdef __cplusplus
 }

@@ -65,6 +65,7 @@ class Oracle:
         if not os.path.exists(path):
             raise FileNotFoundError(f"{path} missing: run `make -C oracle` first")
         L = self.lib = C.CDLL(path)
+        _sig(L, "orc_set_threads", _i, [_i])
         _sig(L, "orc_mg_levels", _i, [_i, _i, _i, _i, _i, _ip, _ip, _ip, _ip])
         _sig(L, "orc_mg_create", _vp, [_i, _i, _i, _i, _d, _d, _d, _i])
         _sig(L, "orc_mg_destroy", None, [_vp])
@@ -90,6 +91,8 @@ class Oracle:
         _sig(L, "orc_mg_vcycle_rb", None, [_vp, _i, _i])
         _sig(L, "orc_mg_solve_rb", _i, [_vp, _d, _i, _i, _i, _dp])
         _sig(L, "orc_mg_dense_solve_level0", _i, [_vp, _dp, _dp])
+        _sig(L, "orc_mg_solve_fcg", _i, [_vp, _d, _i, _i, _i, _dp])
+        _sig(L, "orc_mg_fcg_vec", C.POINTER(C.c_double), [_vp, _i])
         _sig(L, "orc_bc_apply", None, [_i, _i, _i, _i, _d, _d, _d, _ip, _dp, _i, _dp])
         _sig(L, "orc_weno5_derivatives", None,
              [_i, _i, _i, _i, _d, _d, _d, _dp, _dp, _dp, _dp, _dp, _dp, _dp])
@@ -122,6 +125,7 @@ class Oracle:
         _sig(L, "orc_mpls_euler", None, [_i, _i, _i, _i, _d, _d, _d, _d, _d, _d, _i, _d, _ip, _dp, _dp, _i, _vp])
         _sig(L, "orc_find_sign", None, [_l, _dp, _d, _dp])
         _sig(L, "orc_redistance_rk3", None, [_i, _i, _i, _i, _d, _d, _d, _d, _d, _dp, _dp, _ip, _dp, _dp, _i])
+        _sig(L, "orc_stabilize_gradient", _d, [_i, _i, _i, _i, _d, _d, _d, _dp, _dp, _dp])
         _sig(L, "orc_vtk_write", _i, [C.c_char_p, _i, _i, _i, _i, _dp, _dp, _i, C.c_void_p, C.c_void_p,
                                       _i, C.c_void_p, C.c_void_p])
 
@@ -214,6 +218,19 @@ class OracleMG:
         c = self.lib.orc_mg_solve_rb(self.h, rel_tol, max_cycles, nu1, nu2, hist)
         return c, hist[:c]
 
+    def solve_fcg(self, rel_tol, max_iter=100, nu1=2, nu2=2):
+        """FCG(1) around the red-black V-cycle from x = 0 (orc_mg_solve_fcg);
+        returns (iterations, relative residual after each)."""
+        hist = np.zeros(max_iter + 1)
+        c = self.lib.orc_mg_solve_fcg(self.h, rel_tol, max_iter, nu1, nu2, hist)
+        return c, hist[:c]
+
+    def fcg_vec(self, which):
+        """x, r, p, q, p_old, q_old or z of the last orc_mg_solve_fcg (a copy)."""
+        k = ["x", "r", "p", "q", "p_old", "q_old", "z"].index(which)
+        n = self.shapes[0][4]
+        return np.ctypeslib.as_array(self.lib.orc_mg_fcg_vec(self.h, k), shape=(n,)).copy()
+
     def dense_solve(self, b):
         x = np.zeros(self.shapes[0][4])
         rc = self.lib.orc_mg_dense_solve_level0(self.h, np.ascontiguousarray(b, np.float64), x)
@@ -260,6 +277,7 @@ class Reference:
         _sig(L, "ref_mpls_euler", None, [_i, _i, _i, _i, _dp, _d, _d, _d, _i, _d, _ip, _dp, _dp, _i, _vp])
         _sig(L, "ref_find_sign", None, [_i, _i, _i, _i, _dp, _d, _dp, _dp])
         _sig(L, "ref_redistance_rk3", None, [_i, _i, _i, _i, _dp, _d, _d, _dp, _dp, _ip, _dp, _dp, _i])
+        _sig(L, "ref_stabilize_gradient", None, [_i, _i, _i, _i, _dp, _dp, _dp, _dp])
         _sig(L, "ref_vtk_write", _i, [C.c_char_p, C.c_uint, _i, _i, _i, _i, _dp, _i, C.c_void_p, C.c_void_p,
                                       _i, C.c_void_p, C.c_void_p])
 

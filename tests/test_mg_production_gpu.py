@@ -1,0 +1,283 @@
+"""Iterate-level parity of the production multigrid kernels against the
+oracle, at sizes that reach them.
+
+The small cases of test_mg_gpu.py fit the whole hierarchy into the
+one-workgroup cycle (k_block_vcycle); the kernels that carry the bench
+workloads only run on larger levels:
+  * 2-D: the LDS-tiled k_pre2d / k_post2d on every level above the block
+    cycle's top, with tile shapes 64x12, 32x16, 16x16 and 8x8 (FP64) chosen
+    per level from the level size and the CU count (256 on MI355X), and the
+    mixed flexible-CG update fused into the level-0 pre-smoothing
+    (k_pre2d_fcg);
+  * 3-D: on levels of 2^18 cells and more the z-marching k_rr3d_zm
+    (A(rho) level 0), k_prolong3_zm, k_residual_zm and k_fcg_direction_zm.
+SCIHPC_MG_BLOCK_MAX caps the block cycle's top level, so the tiled kernels
+also run on small levels with partial tiles.
+
+Bars (FP64): residual 1e-13 relative; V-cycle and flexible-CG iterates
+1e-10 relative L2.  The FP32 V-cycle of the mixed path is compared with the
+oracle's FP64 cycle at 2e-5 relative L2 and 1e-4 of the largest value per
+cell (FP32 rounding; a single wrong cell of a 512x1024 level moves the
+relative L2 norm by about 1e-3).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from scihpc_amd import PREC_FP64, PREC_MIXED, Multigrid
+from scihpc_amd import workloads as W
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def cellmax(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def _blob_rho(nx, ny, ly, seed):
+    rng = np.random.default_rng(seed)
+    i, j = np.meshgrid(np.arange(nx), np.arange(ny), indexing="xy")
+    x, y = (i + 0.5) / nx, (j + 0.5) * ly / ny
+    phi = 0.2 - np.hypot(x - rng.uniform(0.3, 0.7), y - rng.uniform(0.3, 0.7) * ly)
+    return W.density(phi.ravel(), 1.5 / nx, 1000.0)
+
+
+# name, (nx, ny), ly, bc, variable, block_max (None: default)
+CASES_2D = [
+    ("bubble-512x1024", (512, 1024), 2.0, "neumann", True, None),  # FP64 tiles 64x12 (level 0), 8x8
+    ("var-384x768", (384, 768), 2.0, "neumann", True, None),       # 32x16 on level 0
+    ("const-256x512", (256, 512), 2.0, "neumann", False, None),    # 16x16 on level 0, 8x8 above
+    ("dir-256x256", (256, 256), 1.0, "dirichlet", False, None),
+    ("var-200x328-ragged", (200, 328), 1.64, "neumann", True, None),  # partial tiles, odd coarse sizes
+    ("var-96x160-block64", (96, 160), 1.0, "neumann", True, 64),   # tiles down to 6x10
+    ("dir-64x64-block64", (64, 64), 1.0, "dirichlet", False, 64),
+    ("var-40x72-block16", (40, 72), 1.8, "neumann", True, 16),     # 5x9 level in 8x8 tiles
+]
+
+
+def _pair_2d(oracle, case, monkeypatch):
+    name, (nx, ny), ly, bc, var, bmax = case
+    if bmax is not None:
+        monkeypatch.setenv("SCIHPC_MG_BLOCK_MAX", str(bmax))
+    else:
+        monkeypatch.delenv("SCIHPC_MG_BLOCK_MAX", raising=False)
+    dx, dy = 1.0 / nx, ly / ny
+    g = Multigrid(2, nx, ny, 1, dx, dy, 1.0, bc)
+    o = oracle.mg(2, nx, ny, 1, dx, dy, 1.0, bc)
+    if var:
+        rho = _blob_rho(nx, ny, ly, 17)
+        g.set_density(rho)
+        o.set_rho(rho)
+    b = np.random.default_rng(18).standard_normal(nx * ny)
+    if bc == "neumann":
+        b -= b.mean()
+    return g, o, b
+
+
+@pytest.mark.parametrize("case", CASES_2D, ids=lambda c: c[0])
+def test_tiled_vcycle_iterates_match_oracle(oracle, case, monkeypatch):
+    """Three V(2,2) cycles through the tiled 2-D kernels follow the oracle's
+    red-black cycles within 1e-10, and the residual of the same iterate
+    within 1e-13 of the stencil terms' size."""
+    g, o, b = _pair_2d(oracle, case, monkeypatch)
+    print(f"tiled V-cycle {case[0]} levels={g.shapes}", flush=True)
+    g.set_rhs(b)
+    g.set_sol(np.zeros_like(b))
+    o.vec(0, "rhs")[:] = b
+    o.vec(0, "sol")[:] = 0
+    for c in range(3):
+        g.v_cycle(2, 2)
+        oracle.lib.orc_mg_vcycle_rb(o.h, 2, 2)
+        e = rel(g.sol(0), o.vec(0, "sol"))
+        print(f"  cycle {c + 1}: rel {e:.2e}", flush=True)
+        assert e < 1e-10, (c, e)
+    # the residual of the same iterate (the GPU's) on both sides: both sum
+    # the stencil's terms, each about diag * x, in their own order, so the
+    # bar is 1e-13 of ||diag x||
+    x = g.sol(0)
+    o.vec(0, "sol")[:] = x
+    rg, ro = g.residual(0), o.residual(0)
+    scale = np.linalg.norm(o.operator(0)[0] * x)
+    assert np.linalg.norm(g.res(0) - o.vec(0, "res")) < 1e-13 * scale
+    assert abs(rg - ro) < 1e-13 * scale
+
+
+@pytest.mark.parametrize("case", [c for c in CASES_2D if c[4] or c[3] == "dirichlet"], ids=lambda c: c[0])
+def test_fp32_vcycle_matches_oracle(oracle, case, monkeypatch):
+    """One mixed-precision correction from p = 0 is the FP32 V-cycle of b
+    (p1 = V32(b)): the FP32 tiled kernels stay within FP32 rounding of the
+    oracle's FP64 V-cycle."""
+    g, o, b = _pair_2d(oracle, case, monkeypatch)
+    g.set_rhs(b)
+    st = g.solve(rel_tol=1e-30, max_cycles=1, precision=PREC_MIXED, accel=0)
+    assert st["cycles"] == 1
+    o.vec(0, "rhs")[:] = b
+    o.vec(0, "sol")[:] = 0
+    oracle.lib.orc_mg_vcycle_rb(o.h, 2, 2)
+    x = o.vec(0, "sol").copy()
+    if case[3] == "neumann":
+        x -= x.mean()
+    e, m = rel(g.sol(0), x), cellmax(g.sol(0), x)
+    print(f"FP32 V-cycle {case[0]}: rel {e:.2e}, cell max {m:.2e}", flush=True)
+    assert e < 2e-5 and m < 1e-4, (e, m)
+
+
+FCG_CASES = [c for c in CASES_2D if c[0] in ("bubble-512x1024", "var-384x768", "var-96x160-block64",
+                                             "dir-256x256")]
+
+
+@pytest.mark.parametrize("case", FCG_CASES, ids=lambda c: c[0])
+def test_fcg_iterates_match_oracle(oracle, case, monkeypatch):
+    """FP64 flexible CG: the iterates after 1..4 iterations follow the
+    oracle's FCG(1) within 1e-10, and both need the same iteration count to
+    1e-9.  The mixed path (FP32 V-cycle, FCG update fused into the level-0
+    pre-smoothing from the second iteration on) tracks the same iterates
+    within FP32 rounding and needs at most one iteration more."""
+    g, o, b = _pair_2d(oracle, case, monkeypatch)
+    g.set_rhs(b)
+    print(f"FCG {case[0]}", flush=True)
+    for k in range(1, 5):
+        g.solve(rel_tol=1e-30, max_cycles=k, precision=PREC_FP64, accel=1)
+        xg = g.sol(0)
+        o.vec(0, "rhs")[:] = b
+        it, _ = o.solve_fcg(1e-30, k)
+        assert it == k
+        x = o.vec(0, "sol").copy()
+        e = rel(xg, x)
+        g.solve(rel_tol=1e-30, max_cycles=k, precision=PREC_MIXED, accel=1)
+        em, mm = rel(g.sol(0), x), cellmax(g.sol(0), x)
+        print(f"  iteration {k}: fp64 rel {e:.2e}; mixed rel {em:.2e}, cell max {mm:.2e}", flush=True)
+        assert e < 1e-10, (k, e)
+        assert em < 1e-4 and mm < 1e-3, (k, em, mm)
+    st = g.solve(rel_tol=1e-9, max_cycles=100, precision=PREC_FP64, accel=1)
+    o.vec(0, "rhs")[:] = b
+    it, hist = o.solve_fcg(1e-9, 100)
+    assert st["converged"] and st["cycles"] == it, (st, it, hist)
+    assert rel(g.sol(0), o.vec(0, "sol")) < 1e-10
+    sm = g.solve(rel_tol=1e-9, max_cycles=100, precision=PREC_MIXED, accel=1)
+    assert sm["converged"] and sm["cycles"] <= it + 1, (sm, it)
+
+
+# 3-D: z-marching kernels on levels of 2^18 cells and more
+CASES_3D = [
+    ("var-plane-64^3", (64, 64, 64), "neumann", True),
+    ("var-plane-64x64x128", (64, 64, 128), "neumann", True),
+    ("dir-64x64x128", (64, 64, 128), "dirichlet", False),
+    ("neu-const-64^3", (64, 64, 64), "neumann", False),
+]
+
+
+def _pair_3d(oracle, case):
+    name, (nx, ny, nz), bc, var = case
+    h = 1.0 / nx
+    g = Multigrid(3, nx, ny, nz, h, h, h, bc)
+    o = oracle.mg(3, nx, ny, nz, h, h, h, bc)
+    if var:
+        xc = (np.arange(nx) + 0.5) * h
+        zc = (np.arange(nz) + 0.5) * h
+        surf = 0.5 + 0.05 * np.sin(2 * np.pi * xc[None, :]) * np.sin(2 * np.pi * xc[:ny, None])
+        phi = zc[:, None, None] - nz * h / 2 * surf[None, :, :] * 2
+        rho = W.density(phi.ravel(), 1.5 * h, 1000.0)
+        g.set_density(rho)
+        o.set_rho(rho)
+    b = np.random.default_rng(19).standard_normal(nx * ny * nz)
+    if bc == "neumann":
+        b -= b.mean()
+    return g, o, b
+
+
+@pytest.mark.parametrize("case", CASES_3D, ids=lambda c: c[0])
+def test_zmarch_kernels_match_oracle(oracle, case):
+    """Residual (k_residual_zm) within 1e-13; two V(2,2) cycles (z-streaming
+    sweeps, k_rr3d_zm / k_rr3d_pairs, k_prolong3_zm) within 1e-10; FP64
+    flexible-CG iterates (k_fcg_direction_zm) within 1e-10; the FP32
+    V-cycle within FP32 rounding."""
+    g, o, b = _pair_3d(oracle, case)
+    print(f"3-D {case[0]} levels={g.shapes}", flush=True)
+    rng = np.random.default_rng(20)
+    x0 = rng.standard_normal(b.size)
+    g.set_rhs(b)
+    g.set_sol(x0)
+    o.vec(0, "rhs")[:] = b
+    o.vec(0, "sol")[:] = x0
+    rg, ro = g.residual(0), o.residual(0)
+    assert abs(rg - ro) / ro < 1e-13
+    assert rel(g.res(0), o.vec(0, "res")) < 1e-13
+    # the FP64 residual pass of the solver (k_residual_zm): warm start from x0
+    st = g.solve(rel_tol=1e-30, max_cycles=0, precision=PREC_FP64, accel=0, zero_guess=False)
+    assert abs(st["rel_residual"] - ro / np.linalg.norm(b)) < 1e-13 * st["rel_residual"]
+    g.set_sol(np.zeros_like(b))
+    o.vec(0, "sol")[:] = 0
+    for c in range(2):
+        g.v_cycle(2, 2)
+        oracle.lib.orc_mg_vcycle_rb(o.h, 2, 2)
+        e = rel(g.sol(0), o.vec(0, "sol"))
+        print(f"  cycle {c + 1}: rel {e:.2e}", flush=True)
+        assert e < 1e-10, (c, e)
+    for k in (1, 2, 3):
+        g.solve(rel_tol=1e-30, max_cycles=k, precision=PREC_FP64, accel=1)
+        o.vec(0, "rhs")[:] = b
+        assert o.solve_fcg(1e-30, k)[0] == k
+        e = rel(g.sol(0), o.vec(0, "sol"))
+        print(f"  FCG iteration {k}: rel {e:.2e}", flush=True)
+        assert e < 1e-10, (k, e)
+    g.solve(rel_tol=1e-30, max_cycles=1, precision=PREC_MIXED, accel=0)
+    o.vec(0, "rhs")[:] = b
+    o.vec(0, "sol")[:] = 0
+    oracle.lib.orc_mg_vcycle_rb(o.h, 2, 2)
+    x = o.vec(0, "sol").copy()
+    if case[2] == "neumann":
+        x -= x.mean()
+    e, m = rel(g.sol(0), x), cellmax(g.sol(0), x)
+    print(f"  FP32 V-cycle: rel {e:.2e}, cell max {m:.2e}", flush=True)
+    assert e < 2e-5 and m < 1e-4, (e, m)
+
+
+def test_c2_bubble_converged_vs_oracle(oracle):
+    """The 512x1024 bubble (config 2): GPU FP64 stationary and flexible-CG
+    solves against the oracle's solves of the same algorithms at equal
+    tolerances.  1e-10 is the FP64 floor of this 1000:1 operator (the
+    relative residual stalls there), so at equal tolerance the pressures
+    agree to the accuracy that tolerance fixes; the measured differences are
+    printed beside the asserted bars."""
+    w = W.bubble_2d()
+    nx, ny, _ = w["shape"]
+    dx, dy, _ = w["spacing"]
+    g = Multigrid(2, nx, ny, 1, dx, dy, 1.0, "neumann")
+    g.set_density(w["rho"])
+    g.set_rhs(w["rhs"])
+    o = oracle.mg(2, nx, ny, 1, dx, dy, 1.0, "neumann")
+    o.set_rho(w["rho"])
+    o.vec(0, "rhs")[:] = w["rhs"]
+    out = {}
+    for tol in (1e-8, 1e-10):
+        o.vec(0, "sol")[:] = 0
+        c_rb, _ = o.solve_rb(tol, 100)
+        x_rb = o.vec(0, "sol").copy()
+        c_fcg, _ = o.solve_fcg(tol, 100)
+        x_fcg = o.vec(0, "sol").copy()
+        s = g.solve(rel_tol=tol, max_cycles=100, precision=PREC_FP64, accel=0)
+        e_rb = rel(g.sol(0), x_rb)
+        f = g.solve(rel_tol=tol, max_cycles=100, precision=PREC_FP64, accel=1)
+        e_fcg = rel(g.sol(0), x_fcg)
+        m = g.solve(rel_tol=tol, max_cycles=100, precision=PREC_MIXED, accel=1)
+        e_mix = rel(g.sol(0), x_fcg)
+        print(f"C2 tol {tol:.0e}: stationary {s['cycles']}/{c_rb} cycles rel {e_rb:.2e}; "
+              f"FCG {f['cycles']}/{c_fcg} rel {e_fcg:.2e}; mixed FCG {m['cycles']} rel {e_mix:.2e}; "
+              f"oracle stationary vs FCG {rel(x_rb, x_fcg):.2e}", flush=True)
+        assert s["converged"] and f["converged"] and m["converged"]
+        if tol > 1e-10:
+            # above the floor the counts are the oracle's; at the floor the
+            # stationary residual hovers about 1e-10, and the cycle that
+            # first reads below it can differ by rounding
+            assert s["cycles"] == c_rb and f["cycles"] == c_fcg
+        out[tol] = (e_rb, e_fcg, e_mix, rel(x_rb, x_fcg))
+    # same algorithm, same tolerance: the iterates are the oracle's to rounding
+    assert out[1e-10][0] < 1e-10 and out[1e-10][1] < 1e-10
+    assert out[1e-8][0] < 1e-10 and out[1e-8][1] < 1e-10
+    # a different algorithm (or FP32 inside) at the same tolerance agrees to
+    # what the tolerance fixes
+    assert out[1e-10][2] < 1e-6 and out[1e-10][3] < 1e-6
