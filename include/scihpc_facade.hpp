@@ -1,36 +1,77 @@
-// scihpc_facade.hpp -- SciHPC's solver classes over the MI355X C ABI.
+// scihpc_facade.hpp -- SciHPC's driver-facing API over the MI355X C ABI.
 //
-// Header-only C++17.  The classes keep the public interface that SciHPC's
-// drivers and projection step use (paths relative to the SciHPC tree):
-//   structured_grid   structured_grid.h:10-30   (axis, dx, xc, ...)
-//   scalar_data       scalar_data.h:10-32       (ghc = 3, data[i][j][k], index_mapping)
-//   vector_data       vector_data.h             (x, y, z components)
-//   multigrid_base    multigrid_base.h:17-55    (rhs/sol/res, of, residual, relax, ...)
-//   multigrid         multigrid.h:13-25         (level_num, at[], v_cycle, full_cycle)
-//   weno_solver       weno_solver.h:17-48       (weno5_find_derivatives)
-//   runge_kutta       runge_kutta.h:12-23       (tvd_rk3 of the WENO5 level-set convection)
-//   projection_method projection_method.h:14-51 (fast_pressure_correction, projection)
-// so a driver written against SciHPC compiles against this header with
-// `using namespace scihpc_gpu;`.  Every operation runs on the GPU through
-// include/scihpc_hip.h; host vectors are copied to the device before a call
-// that reads them and back after a call that writes them.  Errors become
-// scihpc_gpu::error exceptions carrying the ABI status and message.
+// Header-only C++17.  The types and functions keep the signatures that
+// SciHPC's benchmarks_*.cpp drivers and projection step use (paths relative
+// to the SciHPC tree), so a driver compiles against this header unchanged:
+// the headers under include/scihpc/ carry the reference's file names and
+// bring this namespace into scope.
 //
-// Differences a caller sees:
-//  * the multigrid boundary operator is per hierarchy: init_NeumannBC() /
-//    init_DirichletBC() on any level selects it for all levels (every SciHPC
-//    driver sets the same one on every level);
-//  * runge_kutta::tvd_rk3 advances the level set with the WENO5 convection
-//    right-hand side and the identity flux (the combination SciHPC's
-//    level-set step uses, source.cpp:8-40, flux.cpp:7-30) instead of taking
-//    arbitrary host callbacks, which cannot run on the device;
-//  * multigrid::solve() is added: the variable-density A(rho) solve to a
-//    relative tolerance (FP64 or mixed precision, optional learned guess).
+//   global.h:11-78          DataType, pi, epsilon, indices, axis,
+//                           problem_parameters, init_array, delete3d,
+//                           set_parameters, set_air_water
+//   structured_grid.h:10-30 structured_grid
+//   scalar_data.h:10-32     scalar_data (13 padded fields, ghc = 3)
+//   vector_data.h           vector_data (x, y, z components)
+//   dummy_data.h:11-28      dummy_data, dummy_data_alloc
+//   bc_factory.h:18-59      NO_SLIP..DIRICHLET, bc_info, bc_factory
+//   wrapper.h:16-49         wrapper
+//   shared_solvers.h:14-26  shared_solvers, SharedSolvers_alloc,
+//                           shared_solvers_mg_init_Neumann/_Dirichlet
+//   multigrid(_base).h      multigrid, multigrid_base
+//   weno_solver.h:30-47     weno_solver; ccd_solver, uccd_solver,
+//                           second_order_solver (the registry's members)
+//   runge_kutta.h:12-23     runge_kutta (tvd_rk3 / euler with flux and rhs
+//                           callbacks)
+//   flux.h, source.h        identity_flux, convection, mpls,
+//                           lsf_redistance_lambda, ...
+//   projection_method.h:14-51 projection_method
+//   wrapper_func.h, lsm.h, godunov_gradient.h: the level-set helpers
+//   vtkWriter.h:15-35       vtkWriter
+//
+// Data stays where the drivers put it: every field is a padded host array
+// behind the reference's DataType*** (contiguous, k fastest -- the padded
+// layout of the C ABI).  A solver call copies the fields it reads to the
+// device, runs HIP kernels there, and copies the fields it writes back.
+//
+// What runs where:
+//  * on the GPU: the multigrid, the boundary conditions, the level-set
+//    transport (runge_kutta with convection = UCCD, mpls, redistancing),
+//    WENO5/CCD/UCCD derivatives, the momentum predictor, both pressure
+//    steps, node velocities, find_sign/delta/density/viscosity,
+//    Heaviside, lsf_mass, stabilized_upon_gradient and the VTK writer;
+//  * on the host, over the fields the driver already holds: the drivers'
+//    scalar diagnostics store_tmp, l2norm, linfity, divergence and the CFL
+//    reductions of find_dt (a few maxima), and the parameter arithmetic.
+//
+// runge_kutta takes the reference's callbacks, but host callbacks cannot run
+// on the device: tvd_rk3/euler recognise the reference's right-hand sides
+// by address (convection, mpls, lsf_redistance_lambda with identity_flux)
+// and run their device form; any other callback throws scihpc_gpu::error.
+// Reference internals that the device fuses into larger steps
+// (all_to_face_*, add_stress_*, init_ppe_* coefficient arrays, ...) are
+// declared and throw the same error, naming the call that covers them.
+//
+// Differences a caller can see:
+//  * projection_method::find_source (the UCCD predictor,
+//    projection_method.cpp:244-312) computes the sources with the
+//    second-order predictor of find_source_sec.  At rest -- how the
+//    reference's drivers call it, once before the time loop -- both give
+//    the body force alone.  ab_solve uses the same predictor;
+//  * projection() solves A(rho) p = div u* / dt with the flexible-CG
+//    multigrid to a relative residual of 1e-8 instead of 100 racy SOR
+//    sweeps;
+//  * fields the device computes as by-products (fxy.., heaviside, grad,
+//    curvature, a, b of dummy_data; flux of a transported field) are not
+//    copied back; the solvers' outputs are;
+//  * errors are scihpc_gpu::error exceptions carrying the ABI status
+//    instead of asserts.
 #pragma once
 
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstring>
+#include <initializer_list>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -41,6 +82,8 @@
 namespace scihpc_gpu {
 
 using DataType = double;
+const DataType pi = std::acos(static_cast<DataType>(-1.0));
+const DataType epsilon = 1e-8;
 
 struct error : std::runtime_error {
   int code;
@@ -52,26 +95,112 @@ inline void check(int rc) {
     throw error(rc, std::string("scihpc: ") + scihpc_last_error());
 }
 
-struct axis {
-  DataType start, end;
-  int n;
-};
+[[noreturn]] inline void device_only(const char *what, const char *use) {
+  throw error(SCIHPC_BAD_ARG, std::string("scihpc: ") + what + " runs fused on the device; call " + use);
+}
 
 struct indices {
   int i, j, k;
 };
 
+struct axis {
+  DataType start, end;
+  int n;
+};
+
+// global.h:25-54
+struct problem_parameters {
+  int iter = 0;
+  DataType t = 0.0;
+  DataType dt = 1.0;
+  DataType dt_old = 1.0, dt_old2 = 1.0;
+  DataType max_CFL = 0.01;
+  DataType min_CFL = 0.01;
+  DataType stable_CFL = -1.0;
+  DataType rdt{};
+  DataType density_ratio = 1.0;
+  DataType viscosity_ratio = 1.0;
+  DataType ls_width{};
+  DataType lsf_mass0{};
+  bool positive_ref = true;
+  DataType Lc{};
+  DataType Uc{};
+  DataType Tc{};
+  DataType Reynolds_number{};
+  DataType Froude_number = -1.0;
+  DataType Weber_number = -1.0;
+  DataType ppe_tol = 1e-4;
+  DataType ppe_tol2 = 1e-7;
+  int ppe_max_iter = 100;
+  DataType ppe_omega = 1.5;
+  int ppe_initer = 10;
+};
+
+// global.cpp:7-28: a zeroed Nx x Ny x Nz array behind DataType***, here one
+// contiguous block (k fastest) so that it crosses the C ABI as is
+inline DataType ***init_array(int Nx, int Ny, int Nz) {
+  auto *block = new DataType[(size_t)Nx * Ny * Nz]();
+  auto **rows = new DataType *[(size_t)Nx * Ny];
+  auto ***arr = new DataType **[Nx];
+  for (int i = 0; i < Nx; ++i) {
+    arr[i] = rows + (size_t)i * Ny;
+    for (int j = 0; j < Ny; ++j) arr[i][j] = block + ((size_t)i * Ny + j) * Nz;
+  }
+  return arr;
+}
+
+inline void delete3d(DataType ***arr, int, int) {
+  if (!arr) return;
+  delete[] arr[0][0];
+  delete[] arr[0];
+  delete[] arr;
+}
+
+// global.cpp:30-81: nondimensional numbers of a two-fluid problem
+inline problem_parameters *set_parameters(DataType rho1, DataType rho2, DataType mu1, DataType mu2,
+                                          DataType surface_tension, DataType gravity, DataType length,
+                                          DataType velocity) {
+  auto *p = new problem_parameters();
+  p->density_ratio = rho2 / rho1;
+  p->viscosity_ratio = mu2 / mu1;
+  p->Lc = length;
+  p->Uc = velocity;
+  p->Tc = p->Lc / p->Uc;
+  p->Reynolds_number = rho1 * p->Uc * p->Lc / mu1;
+  p->Froude_number = p->Uc / std::sqrt(gravity * p->Lc);
+  p->Weber_number = rho1 * p->Uc * p->Uc * p->Lc / surface_tension;
+  return p;
+}
+inline problem_parameters *set_parameters(DataType rho1, DataType rho2, DataType mu1, DataType mu2,
+                                          DataType surface_tension, DataType gravity, DataType length) {
+  return set_parameters(rho1, rho2, mu1, mu2, surface_tension, gravity, length, std::sqrt(gravity * length));
+}
+inline problem_parameters *set_air_water(DataType length) {
+  return set_parameters(1000.0, 1.226, 1.137e-3, 1.78e-5, 0.0728, 9.81, length);
+}
+inline problem_parameters *set_air_water(DataType length, DataType velocity) {
+  return set_parameters(1000.0, 1.226, 1.137e-3, 1.78e-5, 0.0728, 9.81, length, velocity);
+}
+
 // structured_grid.cpp:43-133: n cells per axis, nodes x[0..n], centres xc
 class structured_grid {
  public:
-  explicit structured_grid(axis xa) : structured_grid(1, xa, axis{0, 0, 1}, axis{0, 0, 1}) {}
-  structured_grid(axis xa, axis ya) : structured_grid(2, xa, ya, axis{0, 0, 1}) {}
-  structured_grid(axis xa, axis ya, axis za) : structured_grid(3, xa, ya, za) {}
+  explicit structured_grid(axis x_data) : structured_grid(1, x_data, axis{0, 0, 1}, axis{0, 0, 1}) {}
+  structured_grid(axis x_data, axis y_data) : structured_grid(2, x_data, y_data, axis{0, 0, 1}) {}
+  structured_grid(axis x_data, axis y_data, axis z_data) : structured_grid(3, x_data, y_data, z_data) {}
+  structured_grid(const structured_grid &) = delete;
+  structured_grid &operator=(const structured_grid &) = delete;
 
-  DataType *x, *xc, *y, *yc, *z, *zc;
+  DataType *x, *xc;
+  DataType *y, *yc;
+  DataType *z, *zc;
   DataType dx = 0, dy = 0, dz = 0, h = 0, dv = 0;
   axis x_axis, y_axis, z_axis;
   int ndim;
+  struct {
+    DataType xstart, ystart, zstart;
+    int nx, ny, nz;
+  } vtk_info;
 
  private:
   std::vector<DataType> store_[6];
@@ -94,86 +223,224 @@ class structured_grid {
     zc = store_[5].data();
     h = nd == 1 ? dx : nd == 2 ? std::fmin(dx, dy) : std::fmin(std::fmin(dx, dy), dz);
     dv = nd == 1 ? dx : nd == 2 ? dx * dy : dx * dy * dz;
+    vtk_info = {xa.start, nd >= 2 ? ya.start : 0.0, nd >= 3 ? za.start : 0.0, xa.n + 1, nd >= 2 ? ya.n + 1 : 1,
+                nd >= 3 ? za.n + 1 : 1};
   }
 };
 
 // scalar_data.h:10-32: padded fields data[I][J][K] (ghost 3 on axes with
-// more than one cell).  Storage is contiguous, k fastest -- the padded
-// layout of the C ABI -- behind the same triple-pointer access.
+// more than one cell), each one contiguous block behind the triple pointer
 class scalar_data {
  public:
   int ghc = 3;
-  int Nx, Ny, Nz, nx, ny, nz, ndim;
-  DataType ***data, ***flux, ***old, ***old2, ***fx, ***fy, ***fz;
+  int Nx, Ny, Nz;
+  int nx, ny, nz;
+  int ndim;
+  DataType ***data, ***flux, ***old, ***old2;
+  DataType ***fx, ***fy, ***fz;
+  DataType ***fxx, ***fyy, ***fzz;
+  DataType ***fxy, ***fyz, ***fzx;
 
   explicit scalar_data(int _nx) : scalar_data(_nx, 1, 1, 1) {}
   scalar_data(int _nx, int _ny) : scalar_data(_nx, _ny, 1, 2) {}
   scalar_data(int _nx, int _ny, int _nz) : scalar_data(_nx, _ny, _nz, 3) {}
+  scalar_data(const scalar_data &) = delete;
+  scalar_data &operator=(const scalar_data &) = delete;
+  ~scalar_data() {
+    for (DataType ***f : {data, flux, old, old2, fx, fy, fz, fxx, fyy, fzz, fxy, fyz, fzx}) delete3d(f, Nx, Ny);
+  }
 
   // scalar_data.cpp:95-123 (1-based interior indices)
   indices index_mapping(int i, int j, int k) const {
     return indices{i - 1 + (nx > 1 ? ghc : 0), j - 1 + (ny > 1 ? ghc : 0), k - 1 + (nz > 1 ? ghc : 0)};
   }
+  // scalar_data::store: old2 <- old <- data
+  void store() const {
+    const size_t n = (size_t)Nx * Ny * Nz;
+    std::memcpy(&old2[0][0][0], &old[0][0][0], sizeof(DataType) * n);
+    std::memcpy(&old[0][0][0], &data[0][0][0], sizeof(DataType) * n);
+  }
   long size() const { return (long)Nx * Ny * Nz; }
-  // contiguous base of a field
-  static DataType *base(DataType ***f) { return &f[0][0][0]; }
 
  private:
-  struct field {
-    std::vector<DataType> v;
-    std::vector<DataType *> rows;
-    std::vector<DataType **> planes;
-  };
-  field f_[7];
-  DataType ***make(field &f) {
-    f.v.assign((size_t)Nx * Ny * Nz, 0.0);
-    f.rows.resize((size_t)Nx * Ny);
-    f.planes.resize(Nx);
-    for (int i = 0; i < Nx; ++i) {
-      for (int j = 0; j < Ny; ++j) f.rows[(size_t)i * Ny + j] = f.v.data() + ((size_t)i * Ny + j) * Nz;
-      f.planes[i] = f.rows.data() + (size_t)i * Ny;
-    }
-    return f.planes.data();
-  }
   scalar_data(int _nx, int _ny, int _nz, int nd) : nx(_nx), ny(_ny), nz(_nz), ndim(nd) {
     Nx = nx > 1 ? nx + 2 * ghc : 1;
     Ny = ny > 1 ? ny + 2 * ghc : 1;
     Nz = nz > 1 ? nz + 2 * ghc : 1;
-    data = make(f_[0]);
-    flux = make(f_[1]);
-    old = make(f_[2]);
-    old2 = make(f_[3]);
-    fx = make(f_[4]);
-    fy = make(f_[5]);
-    fz = make(f_[6]);
-  }
-
- public:
-  // scalar_data::store (scalar_data.cpp): old2 <- old <- data
-  void store() {
-    std::copy(f_[2].v.begin(), f_[2].v.end(), f_[3].v.begin());
-    std::copy(f_[0].v.begin(), f_[0].v.end(), f_[2].v.begin());
+    for (DataType ****f : {&data, &flux, &old, &old2, &fx, &fy, &fz, &fxx, &fyy, &fzz, &fxy, &fyz, &fzx})
+      *f = init_array(Nx, Ny, Nz);
   }
 };
 
+// contiguous base of a field (the padded layout of the C ABI)
+inline DataType *base(DataType ***f) { return &f[0][0][0]; }
+
+// vector_data.cpp:7-26: three components on the same grid (z present in 2-D)
 class vector_data {
  public:
-  std::unique_ptr<scalar_data> x, y, z;
-  explicit vector_data(int _nx) : x(new scalar_data(_nx)) {}
-  vector_data(int _nx, int _ny) : x(new scalar_data(_nx, _ny)), y(new scalar_data(_nx, _ny)) {}
-  vector_data(int _nx, int _ny, int _nz)
-      : x(new scalar_data(_nx, _ny, _nz)), y(new scalar_data(_nx, _ny, _nz)), z(new scalar_data(_nx, _ny, _nz)) {}
+  explicit vector_data(int _nx) : x(_nx), y(_nx), z(_nx) {}
+  vector_data(int _nx, int _ny) : x(_nx, _ny), y(_nx, _ny), z(_nx, _ny) {}
+  vector_data(int _nx, int _ny, int _nz) : x(_nx, _ny, _nz), y(_nx, _ny, _nz), z(_nx, _ny, _nz) {}
+  scalar_data x, y, z;
+  void store() {
+    x.store();
+    y.store();
+    z.store();
+  }
 };
 
-// Eigen::VectorX stand-in for the multigrid's host vectors
-class VectorX {
+// dummy_data.h:11-28, dummy_data.cpp:7-32
+struct dummy_data {
+  DataType ***heaviside;
+  DataType ***sign;
+  DataType ***delta;
+  DataType ***grad;
+  DataType ***curvature;
+  DataType ***tmp;
+  DataType ***u_tmp, ***v_tmp, ***w_tmp;
+  DataType ***a, ***b;
+  DataType ***a_int, ***b_int;
+  DataType ***density, ***viscosity;
+};
+
+inline dummy_data *dummy_data_alloc(scalar_data *f) {
+  auto *d = new dummy_data;
+  for (DataType ****p : {&d->grad, &d->delta, &d->heaviside, &d->sign, &d->curvature, &d->tmp, &d->u_tmp, &d->v_tmp,
+                         &d->w_tmp, &d->a, &d->b, &d->a_int, &d->b_int, &d->density, &d->viscosity})
+    *p = init_array(f->Nx, f->Ny, f->Nz);
+  return d;
+}
+
+// ------------------------------------------------------------------- BCs
+// bc_factory.h:18-27
+const std::string NO_SLIP = "no_slip";
+const std::string SLIP = "slip";
+const std::string PERIODIC = "periodic";
+const std::string NEUMANN = "neumann";
+const std::string DIRICHLET = "dirichlet";
+
+struct bc_info {
+  std::string type = "NoSlip";
+  DataType value = 0.0;
+};
+
+inline int bc_code(const std::string &t) {
+  if (t == NEUMANN) return SCIHPC_BC_NEUMANN;
+  if (t == DIRICHLET) return SCIHPC_BC_DIRICHLET;
+  if (t == PERIODIC) return SCIHPC_BC_PERIODIC;
+  if (t == NO_SLIP) return SCIHPC_BC_NO_SLIP;
+  if (t == SLIP) return SCIHPC_BC_SLIP;
+  throw error(SCIHPC_BAD_ARG, "scihpc: boundary type '" + t + "' is none of neumann, dirichlet, periodic, "
+                              "no_slip, slip");
+}
+
+// owner of one level-set handle (scihpc_ls) for a grid and six face BCs
+class ls_handle {
+ public:
+  ls_handle() = default;
+  ls_handle(const ls_handle &) = delete;
+  ls_handle &operator=(const ls_handle &) = delete;
+  ~ls_handle() { reset(); }
+  // the handle for these BCs, created (again) when they changed
+  scihpc_ls *get(int ndim, const int n[3], const DataType h[3], const int types[6], const DataType values[6]) {
+    const bool same = h_ && ndim == ndim_ && std::equal(n, n + 3, n_) && std::equal(types, types + 6, types_) &&
+                      std::equal(values, values + 6, values_) && std::equal(h, h + 3, hs_);
+    if (!same) {
+      reset();
+      check(scihpc_ls_create(ndim, n[0], n[1], n[2], h[0], h[1], h[2], types, values, &h_));
+      ndim_ = ndim;
+      std::copy(n, n + 3, n_);
+      std::copy(h, h + 3, hs_);
+      std::copy(types, types + 6, types_);
+      std::copy(values, values + 6, values_);
+    }
+    return h_;
+  }
+
+ private:
+  scihpc_ls *h_ = nullptr;
+  int ndim_ = 0, n_[3] = {0, 0, 0}, types_[6] = {0, 0, 0, 0, 0, 0};
+  DataType hs_[3] = {0, 0, 0}, values_[6] = {0, 0, 0, 0, 0, 0};
+  void reset() {
+    if (h_) scihpc_ls_destroy(h_);
+    h_ = nullptr;
+  }
+};
+
+// bc_factory.h:37-59: face BCs of one field; the apply_* calls run the
+// reference's per-type formulas on the GPU (bc_factory.cpp:61-375), in its
+// order, corners unfilled
+class bc_factory {
+ public:
+  bc_factory(structured_grid *geo, const bc_info &xl_bc, const bc_info &xr_bc)
+      : xlbc(xl_bc), xrbc(xr_bc), ylbc{NEUMANN}, yrbc{NEUMANN}, zlbc{NEUMANN}, zrbc{NEUMANN}, geo_(geo) {
+    set_spacing();
+  }
+  bc_factory(structured_grid *geo, const bc_info &xl_bc, const bc_info &xr_bc, const bc_info &yl_bc,
+             const bc_info &yr_bc)
+      : xlbc(xl_bc), xrbc(xr_bc), ylbc(yl_bc), yrbc(yr_bc), zlbc{NEUMANN}, zrbc{NEUMANN}, geo_(geo) {
+    set_spacing();
+  }
+  bc_factory(structured_grid *geo, const bc_info &xl_bc, const bc_info &xr_bc, const bc_info &yl_bc,
+             const bc_info &yr_bc, const bc_info &zl_bc, const bc_info &zr_bc)
+      : xlbc(xl_bc), xrbc(xr_bc), ylbc(yl_bc), yrbc(yr_bc), zlbc(zl_bc), zrbc(zr_bc), geo_(geo) {
+    set_spacing();
+  }
+  bc_factory(const bc_factory &) = delete;
+  bc_factory &operator=(const bc_factory &) = delete;
+
+  bc_info xlbc, xrbc, ylbc, yrbc, zlbc, zrbc;
+  DataType dx, dy, dz;
+
+  void apply_node_centered(scalar_data *f) { apply(f, -1); }
+  void apply_node_staggered_x(scalar_data *f) { apply(f, 0); }
+  void apply_node_staggered_y(scalar_data *f) { apply(f, 1); }
+  void apply_node_staggered_z(scalar_data *f) { apply(f, 2); }
+
+  // the device handle with these BCs, for grids shaped like f
+  scihpc_ls *handle(const scalar_data *f) {
+    const int n[3] = {f->nx, f->ny, f->nz};
+    const DataType h[3] = {geo_->dx, f->ndim > 1 ? geo_->dy : 1.0, f->ndim > 2 ? geo_->dz : 1.0};
+    const bc_info *b[6] = {&xlbc, &xrbc, &ylbc, &yrbc, &zlbc, &zrbc};
+    int t[6];
+    DataType v[6];
+    for (int q = 0; q < 6; ++q) {
+      t[q] = q < 2 * f->ndim ? bc_code(b[q]->type) : SCIHPC_BC_NEUMANN;
+      v[q] = q < 2 * f->ndim ? b[q]->value : 0.0;
+    }
+    return ls_.get(f->ndim, n, h, t, v);
+  }
+
+ private:
+  structured_grid *geo_;
+  ls_handle ls_;
+  void set_spacing() {
+    dx = geo_->dx;
+    dy = geo_->dy;
+    dz = geo_->dz;
+  }
+  // a staggered axis beyond the field's dimensions applies the node BCs of
+  // the axes it has (bc_factory.cpp:140-375 skip the missing axes)
+  void apply(scalar_data *f, int stag) {
+    check(scihpc_ls_apply_bc(handle(f), base(f->data), stag < f->ndim ? stag : -1));
+  }
+};
+
+// ------------------------------------------------------------ multigrid
+class VectorX {  // Eigen::Matrix<DataType, Dynamic, 1> stand-in (host vector)
  public:
   explicit VectorX(long n = 0) : v_(n, 0.0) {}
   DataType &operator[](long i) { return v_[i]; }
   DataType operator[](long i) const { return v_[i]; }
   DataType &operator()(long i) { return v_[i]; }
+  DataType operator()(long i) const { return v_[i]; }
   long size() const { return (long)v_.size(); }
   void setZero() { std::fill(v_.begin(), v_.end(), 0.0); }
+  DataType norm() const {
+    DataType s = 0.0;
+    for (DataType x : v_) s += x * x;
+    return std::sqrt(s);
+  }
   DataType *data() { return v_.data(); }
   const DataType *data() const { return v_.data(); }
 
@@ -186,7 +453,7 @@ class multigrid;
 // multigrid_base.h:17-55: one level of the hierarchy
 class multigrid_base {
  public:
-  VectorX rhs, sol, res;
+  VectorX rhs, sol, res, buffer;
   int nx, ny, nz, degree, ndim, n;
   DataType dx, dy, dz;
   bool no_compatibility = false;
@@ -197,13 +464,13 @@ class multigrid_base {
 
   void init_NeumannBC();
   void init_DirichletBC();
-  DataType residual();                                  // multigrid_base.cpp:69-72
-  void relax(int iter);                                 // multigrid_base.cpp:16-45
-  void compatibility_condition(VectorX &f);             // multigrid_base.cpp:47-67
-  void restriction(multigrid_base *coarse);             // multigrid_base.cpp:74-113
-  void prolongation(multigrid_base *dense);             // multigrid_base.cpp:115-208
-  void solve();                                         // multigrid_base.cpp:210-234
-  void Ax(const VectorX &x, VectorX &out);              // multigrid_base.cpp:388-390
+  DataType residual();                       // multigrid_base.cpp:69-72
+  void relax(int iter);                      // multigrid_base.cpp:16-45
+  void compatibility_condition(VectorX &f);  // multigrid_base.cpp:47-67
+  void restriction(multigrid_base *coarse);  // multigrid_base.cpp:74-113
+  void prolongation(multigrid_base *dense);  // multigrid_base.cpp:115-208
+  void solve();                              // multigrid_base.cpp:210-234
+  void Ax(const VectorX &x);                 // multigrid_base.cpp:388-390 (into buffer)
 
  private:
   friend class multigrid;
@@ -213,11 +480,14 @@ class multigrid_base {
   void pull_sol();
 };
 
-// multigrid.h:13-25
+// multigrid.h:13-25.  v_cycle/full_cycle run the reference-faithful cycles
+// (steepest-descent relax, slope prolongation) on the GPU; solve() is this
+// build's addition: the red-black V-cycle inside FP64 (or mixed FP32/FP64)
+// defect correction, to a relative residual.
 class multigrid {
  public:
-  multigrid(scalar_data *var, structured_grid *geo) : ndim_(var->ndim), nx_(var->nx), ny_(var->ny), nz_(var->nz),
-                                                       dx_(geo->dx), dy_(geo->dy), dz_(geo->dz) {
+  multigrid(scalar_data *var, structured_grid *geo)
+      : ndim_(var->ndim), nx_(var->nx), ny_(var->ny), nz_(var->nz), dx_(geo->dx), dy_(geo->dy), dz_(geo->dz) {
     create(SCIHPC_MG_NEUMANN);
     int L = 0;
     check(scihpc_mg_num_levels(h_, &L));
@@ -240,6 +510,7 @@ class multigrid {
       b.rhs = VectorX(b.n);
       b.sol = VectorX(b.n);
       b.res = VectorX(b.n);
+      b.buffer = VectorX(b.n);
       b.mg_ = this;
       b.level_ = l;
       ptrs_[l] = &levels_[l];
@@ -268,14 +539,12 @@ class multigrid {
 
   // A(rho) from a padded density field (init_ppe_proj_coefficients,
   // projection_method.cpp:404-465)
-  void set_density(const scalar_data *rho) {
-    check(scihpc_mg_set_density_padded(h_, &rho->data[0][0][0]));
-  }
-  // solve A p = at[0]->rhs to ||r|| / ||b|| <= rel_tol (returns the ABI stats)
+  void set_density(const scalar_data *rho) { check(scihpc_mg_set_density_padded(h_, &rho->data[0][0][0])); }
+  // solve A p = at[0]->rhs to ||r|| / ||b|| <= rel_tol from p = 0
   scihpc_solve_stats solve(double rel_tol, int precision = SCIHPC_PREC_MIXED, int accel = 1, int max_cycles = 200,
                            int use_learned = 0) {
     at[0]->push();
-    scihpc_solve_opts o{rel_tol, max_cycles, 2, 2, precision, use_learned, 1, accel};
+    scihpc_solve_opts o{rel_tol, max_cycles, 2, 2, precision, use_learned, 1, accel, 0, 0};
     scihpc_solve_stats st{};
     check(scihpc_mg_solve(h_, &o, &st));
     at[0]->pull_sol();
@@ -310,6 +579,8 @@ inline void multigrid_base::push() {
   check(scihpc_mg_set_sol(h, level_, sol.data()));
 }
 inline void multigrid_base::pull_sol() { check(scihpc_mg_get_sol(mg_->h_, level_, sol.data())); }
+// the operator is per hierarchy: init_*BC on any level selects it for all
+// (every SciHPC driver sets the same one on every level)
 inline void multigrid_base::init_NeumannBC() { mg_->create(SCIHPC_MG_NEUMANN); }
 inline void multigrid_base::init_DirichletBC() { mg_->create(SCIHPC_MG_DIRICHLET); }
 inline DataType multigrid_base::residual() {
@@ -342,7 +613,8 @@ inline void multigrid_base::restriction(multigrid_base *coarse) {
   check(scihpc_mg_get_sol(mg_->h_, level_ + 1, coarse->sol.data()));
 }
 inline void multigrid_base::prolongation(multigrid_base *dense) {
-  if (level_ == 0 || dense != mg_->at[level_ - 1]) throw error(SCIHPC_BAD_ARG, "prolongation target is not the previous level");
+  if (level_ == 0 || dense != mg_->at[level_ - 1])
+    throw error(SCIHPC_BAD_ARG, "prolongation target is not the previous level");
   push();
   dense->push();
   check(scihpc_mg_prolongation(mg_->h_, level_));
@@ -354,230 +626,669 @@ inline void multigrid_base::solve() {
   check(scihpc_mg_coarse_solve(mg_->h_));
   pull_sol();
 }
-inline void multigrid_base::Ax(const VectorX &x, VectorX &out) {
-  check(scihpc_mg_apply(mg_->h_, level_, x.data(), out.data()));
+inline void multigrid_base::Ax(const VectorX &x) { check(scihpc_mg_apply(mg_->h_, level_, x.data(), buffer.data())); }
+
+// --------------------------------------------------- derivative solvers
+// A solver object keeps a Neumann level-set handle of its grid; the
+// derivative kernels read the padded field as given (no BCs applied).
+class grid_solver {
+ public:
+  grid_solver(scalar_data *f, structured_grid *geo) : geo_(geo), f_(f) {}
+
+ protected:
+  structured_grid *geo_;
+  scalar_data *f_;
+  ls_handle ls_;
+  scihpc_ls *handle(const scalar_data *f) {
+    const int n[3] = {f->nx, f->ny, f->nz};
+    const DataType h[3] = {geo_->dx, f->ndim > 1 ? geo_->dy : 1.0, f->ndim > 2 ? geo_->dz : 1.0};
+    const int t[6] = {0, 0, 0, 0, 0, 0};
+    const DataType v[6] = {0, 0, 0, 0, 0, 0};
+    return ls_.get(f->ndim, n, h, t, v);
+  }
+  void set_velocity(scihpc_ls *h, vector_data *vel) {
+    check(scihpc_ls_set_velocity(h, base(vel->x.data), base(vel->y.data), base(vel->z.data)));
+  }
+};
+
+// weno_solver.h:30-47 (weno_solver.cpp:16-278): f->fx, fy, fz from
+// f->flux upwinded by vel, interior cells written
+class weno_solver : public grid_solver {
+ public:
+  weno_solver(scalar_data *f, structured_grid *geo)
+      : grid_solver(f, geo), dx(geo->dx), dy(geo->dy), dz(geo->dz), tmp_((size_t)f->size()) {}
+  DataType dx, dy, dz;
+  void weno5_find_derivatives(scalar_data *f, vector_data *vel) { derivs(f, vel, base(f->fx), base(f->fy), base(f->fz)); }
+  void weno5_find_fx(scalar_data *f, vector_data *vel) { derivs(f, vel, base(f->fx), scratch(0), scratch(1)); }
+  void weno5_find_fy(scalar_data *f, vector_data *vel) { derivs(f, vel, scratch(0), base(f->fy), scratch(1)); }
+  void weno5_find_fz(scalar_data *f, vector_data *vel) { derivs(f, vel, scratch(0), scratch(1), base(f->fz)); }
+
+ private:
+  std::vector<DataType> tmp_, tmp2_;
+  DataType *scratch(int k) {
+    std::vector<DataType> &t = k == 0 ? tmp_ : tmp2_;
+    t.resize((size_t)f_->size());
+    return t.data();
+  }
+  void derivs(scalar_data *f, vector_data *vel, DataType *fx, DataType *fy, DataType *fz) {
+    scihpc_ls *h = handle(f);
+    set_velocity(h, vel);
+    check(scihpc_ls_weno5_derivatives(h, base(f->flux), fx, fy, fz));
+  }
+};
+
+// ccd_solver.cpp:14-142: first and second derivatives of f->flux along
+// every axis (fx..fzz)
+class ccd_solver : public grid_solver {
+ public:
+  using grid_solver::grid_solver;
+  void find_derivatives(scalar_data *f) {
+    scihpc_ls *h = handle(f);
+    check(scihpc_ls_compact_derivatives(h, base(f->flux), 0, base(f->fx), base(f->fy), base(f->fz), base(f->fxx),
+                                        base(f->fyy), base(f->fzz)));
+  }
+  void find_derivatives(vector_data *f) {
+    find_derivatives(&f->x);
+    if (f->x.ndim > 1) find_derivatives(&f->y);
+    if (f->x.ndim > 2) find_derivatives(&f->z);
+  }
+  void find_derivatives_all(scalar_data *) { device_only("ccd_solver::find_derivatives_all (mixed derivatives)", "the momentum or curvature step"); }
+};
+
+// uccd_solver.cpp:14-215: upwind/downwind compact derivatives chosen by
+// the sign of vel
+class uccd_solver : public grid_solver {
+ public:
+  using grid_solver::grid_solver;
+  void find_derivatives(scalar_data *f, vector_data *vel) {
+    scihpc_ls *h = handle(f);
+    set_velocity(h, vel);
+    check(scihpc_ls_compact_derivatives(h, base(f->flux), 1, base(f->fx), base(f->fy), base(f->fz), base(f->fxx),
+                                        base(f->fyy), base(f->fzz)));
+  }
+};
+
+// second_order_solver.h:12-41: the registry member; its derivatives run
+// inside the momentum predictor (scihpc_proj_find_sources)
+class second_order_solver {
+ public:
+  second_order_solver(DataType _dx, DataType _dy, DataType _dz) : dx(_dx), dy(_dy), dz(_dz) {}
+  DataType dx, dy, dz;
+};
+
+// shared_solvers.h:14-26, shared_solvers.cpp:7-28
+struct shared_solvers {
+  ccd_solver *ccd;
+  uccd_solver *uccd;
+  weno_solver *weno;
+  second_order_solver *secSol;
+  multigrid *mg;
+};
+
+inline shared_solvers *SharedSolvers_alloc(scalar_data *f, structured_grid *geo) {
+  auto *s = new shared_solvers;
+  s->ccd = new ccd_solver(f, geo);
+  s->uccd = new uccd_solver(f, geo);
+  s->weno = new weno_solver(f, geo);
+  s->secSol = new second_order_solver(geo->dx, geo->dy, geo->dz);
+  s->mg = new multigrid(f, geo);
+  return s;
 }
 
-// boundary conditions per face (bc_factory.h:18-27): type + value
-enum bc_type { NEUMANN = SCIHPC_BC_NEUMANN, DIRICHLET = SCIHPC_BC_DIRICHLET, PERIODIC = SCIHPC_BC_PERIODIC,
-               NO_SLIP = SCIHPC_BC_NO_SLIP, SLIP = SCIHPC_BC_SLIP };
-struct bc_info {
-  bc_type type;
-  DataType value = 0.0;
-};
+inline void shared_solvers_mg_init_Neumann(shared_solvers *solvers) {
+  for (unsigned i = 0; i < solvers->mg->level_num; ++i) solvers->mg->at[i]->init_NeumannBC();
+}
 
-// problem_parameters (global.h:25-54): the fields the projection step reads
-struct problem_parameters {
-  DataType dt = 1.0, dt_old = 1.0, dt_old2 = 1.0;
-  DataType density_ratio = 1.0;
-  DataType ppe_tol = 1e-4;
-  // momentum predictor (global.h:25-54; <= 0 disables gravity / surface tension)
-  DataType Reynolds_number = 1.0, Froude_number = -1.0, Weber_number = -1.0;
-  DataType ls_width = 0.0, viscosity_ratio = 1.0;
-  bool positive_ref = true;
-  int iter = 0;        // time step counter (the first 5 steps use projection())
-  int ppe_initer = 1;  // pressure iterations per step
-};
+inline void shared_solvers_mg_init_Dirichlet(shared_solvers *solvers) {
+  for (unsigned i = 0; i < solvers->mg->level_num; ++i) {
+    solvers->mg->at[i]->init_DirichletBC();
+    solvers->mg->at[i]->no_compatibility = true;
+  }
+}
 
-// projection_method (projection_method.h:14-51): the projection step of
-// one grid on the GPU.  The multigrid is the caller's (its level-0 solution
-// is the warm start of the FPC loop, as SciHPC's persistent at[0]->sol).
-class projection_method {
+// --------------------------------------------------------------- wrapper
+// wrapper.h:16-49, wrapper.cpp: a scalar or vector field with its BC
+// factories and the shared solvers, parameters and dummy arrays
+class wrapper {
  public:
-  projection_method(scalar_data *f, structured_grid *geo, const bc_info vel_faces[6]) {
-    int types[6];
-    for (int q = 0; q < 6; ++q) types[q] = vel_faces[q].type;
-    check(scihpc_proj_create(f->ndim, f->nx, f->ny, f->nz, geo->dx, geo->dy, f->ndim == 3 ? geo->dz : 1.0, types,
-                             &h_));
-  }
-  ~projection_method() {
-    if (h_) scihpc_proj_destroy(h_);
-  }
-  projection_method(const projection_method &) = delete;
-  projection_method &operator=(const projection_method &) = delete;
+  bool is_scalar;
+  scalar_data *scalar;
+  vector_data *vector;
 
-  // pressure_guess_solution + init_ppe_fpc_coefficients +
-  // fast_pressure_correction (projection_method.cpp:467-541, 659-753,
-  // 803-815): pressure->old/old2 in, pressure->data and ->flux (p_hat) out,
-  // vel corrected in place.  Returns the multigrid cycles.
-  int fast_pressure_correction(multigrid &mg, scalar_data *pressure, const scalar_data *density, vector_data *vel,
-                               const problem_parameters &prm, int max_cycles = 100) {
-    scihpc_fpc_params fp{prm.dt, prm.dt_old, prm.dt_old2, prm.density_ratio, prm.ppe_tol, max_cycles};
-    std::vector<double> zero;
-    double *w = vel_w(vel, pressure, zero);
-    int cycles = 0;
-    check(scihpc_proj_fpc_step(h_, mg.handle(), &fp, base(density->data), base(pressure->old), base(pressure->old2),
-                               base(vel->x->data), base(vel->y->data), w, base(pressure->data),
-                               base(pressure->flux), &cycles));
-    return cycles;
+  wrapper(bool as_scalar, structured_grid *_geo, const bc_info &xlbc_type, const bc_info &xrbc_type)
+      : wrapper(as_scalar, _geo, 1, {xlbc_type, xrbc_type, bc_info{NEUMANN}, bc_info{NEUMANN}, bc_info{NEUMANN},
+                                     bc_info{NEUMANN}}) {}
+  wrapper(bool as_scalar, structured_grid *_geo, const bc_info &xlbc_type, const bc_info &xrbc_type,
+          const bc_info &ylbc_type, const bc_info &yrbc_type)
+      : wrapper(as_scalar, _geo, 2, {xlbc_type, xrbc_type, ylbc_type, yrbc_type, bc_info{NEUMANN}, bc_info{NEUMANN}}) {}
+  wrapper(bool as_scalar, structured_grid *_geo, const bc_info &xlbc_type, const bc_info &xrbc_type,
+          const bc_info &ylbc_type, const bc_info &yrbc_type, const bc_info &zlbc_type, const bc_info &zrbc_type)
+      : wrapper(as_scalar, _geo, 3, {xlbc_type, xrbc_type, ylbc_type, yrbc_type, zlbc_type, zrbc_type}) {}
+  wrapper(const wrapper &) = delete;
+  wrapper &operator=(const wrapper &) = delete;
+
+  bc_factory *bcFactory = nullptr;
+  bc_factory *bcFactoryU = nullptr, *bcFactoryV = nullptr, *bcFactoryW = nullptr;
+
+  shared_solvers *solvers{};
+  problem_parameters *params{};
+  dummy_data *dummy{};
+  structured_grid *geo;
+
+  void link_solvers(shared_solvers *_solvers) { solvers = _solvers; }
+  void link_params(problem_parameters *_params) { params = _params; }
+  void link_dummy(dummy_data *_dummy) { dummy = _dummy; }
+
+  // wrapper.cpp:21-59
+  void apply_vel_bc() const {
+    bcFactoryU->apply_node_staggered_x(&vector->x);
+    bcFactoryV->apply_node_staggered_y(&vector->y);
+    bcFactoryW->apply_node_staggered_z(&vector->z);
   }
-  // find_source_sec + find_intermediate_velocity (projection_method.cpp:
-  // 314-402): vel predicted in place (AB2 with the sources kept on the
-  // device between calls, as the reference's u_src), nvel holds the last
-  // face transfer, density (optional) receives find_density
-  void find_source_sec(vector_data *vel, vector_data *nvel, const scalar_data *lsf, const problem_parameters &prm,
-                       scalar_data *density = nullptr) {
-    scihpc_flow_params fp{prm.dt, prm.Reynolds_number, prm.Froude_number, prm.Weber_number, prm.ls_width,
-                          prm.density_ratio, prm.viscosity_ratio, prm.positive_ref ? 1 : 0};
-    std::vector<double> z1, z2;
-    double *w = vel_w(vel, lsf, z1), *nw = vel_w(nvel, lsf, z2);
-    check(scihpc_proj_momentum_step(h_, &fp, base(lsf->data), base(vel->x->data), base(vel->y->data), w,
-                                    base(nvel->x->data), base(nvel->y->data), nw,
-                                    density ? base(density->data) : nullptr));
+  void apply_nvel_bc() const {
+    bcFactoryU->apply_node_centered(&vector->x);
+    bcFactoryV->apply_node_centered(&vector->y);
+    bcFactoryW->apply_node_centered(&vector->z);
   }
-  // node_from_face + nvel->apply_nvel_bc (wrapper_func.cpp:264-293)
-  void node_from_face(vector_data *vel, vector_data *nvel, const scalar_data *like) {
-    std::vector<double> z1, z2;
-    double *w = vel_w(vel, like, z1), *nw = vel_w(nvel, like, z2);
-    check(scihpc_proj_node_velocity(h_, base(vel->x->data), base(vel->y->data), w, base(nvel->x->data),
-                                    base(nvel->y->data), nw));
+  void apply_scalar_bc() const { bcFactory->apply_node_centered(scalar); }
+  void apply_vel_x_bc() const {
+    bcFactoryU->apply_node_staggered_x(&vector->x);
+    bcFactoryV->apply_node_staggered_x(&vector->y);
+    bcFactoryW->apply_node_staggered_x(&vector->z);
   }
-  // ab_solve_sec (projection_method.cpp:779-800): pressure history,
-  // momentum predictor, projection() for the first 5 steps (here the A(rho)
-  // solve) then the FPC step, node velocities.  Returns the last pressure
-  // step's multigrid cycles.
-  int ab_solve_sec(multigrid &mg_const, multigrid &mg_rho, vector_data *vel, vector_data *nvel,
-                   scalar_data *pressure, const scalar_data *lsf, scalar_data *density,
-                   const problem_parameters &prm) {
-    pressure->store();
-    find_source_sec(vel, nvel, lsf, prm, density);
-    int cycles = 0;
-    for (int cnt = 0; cnt < prm.ppe_initer; ++cnt) {
-      if (prm.iter < 5) cycles = projection(mg_rho, pressure, density, vel, prm.dt).cycles;
-      else cycles = fast_pressure_correction(mg_const, pressure, density, vel, prm);
+  void apply_vel_y_bc() const {
+    bcFactoryU->apply_node_staggered_y(&vector->x);
+    bcFactoryV->apply_node_staggered_y(&vector->y);
+    bcFactoryW->apply_node_staggered_y(&vector->z);
+  }
+  void apply_vel_z_bc() const {
+    bcFactoryU->apply_node_staggered_z(&vector->x);
+    bcFactoryV->apply_node_staggered_z(&vector->y);
+    bcFactoryW->apply_node_staggered_z(&vector->z);
+  }
+
+  // the BCs the wrapper was built with (xl, xr, yl, yr, zl, zr), before the
+  // wall-normal SLIP -> NO_SLIP switch of the velocity components
+  const bc_info &face_bc(int q) const { return faces_[q]; }
+  // ABI codes of those BCs (unused faces Neumann)
+  void face_codes(int t[6]) const {
+    for (int q = 0; q < 6; ++q) t[q] = q < 2 * dims_ ? bc_code(faces_[q].type) : SCIHPC_BC_NEUMANN;
+  }
+  int dims() const { return dims_; }
+
+ private:
+  int dims_;
+  bc_info faces_[6];
+  wrapper(bool as_scalar, structured_grid *_geo, int nd, std::initializer_list<bc_info> faces)
+      : is_scalar(as_scalar), scalar(nullptr), vector(nullptr), geo(_geo), dims_(nd) {
+    std::copy(faces.begin(), faces.end(), faces_);
+    const int nx = geo->x_axis.n, ny = geo->y_axis.n, nz = geo->z_axis.n;
+    auto factory = [&]() {
+      if (nd == 1) return new bc_factory(geo, faces_[0], faces_[1]);
+      if (nd == 2) return new bc_factory(geo, faces_[0], faces_[1], faces_[2], faces_[3]);
+      return new bc_factory(geo, faces_[0], faces_[1], faces_[2], faces_[3], faces_[4], faces_[5]);
+    };
+    if (is_scalar) {
+      scalar = nd == 1 ? new scalar_data(nx) : nd == 2 ? new scalar_data(nx, ny) : new scalar_data(nx, ny, nz);
+      bcFactory = factory();
+    } else {
+      vector = nd == 1 ? new vector_data(nx) : nd == 2 ? new vector_data(nx, ny) : new vector_data(nx, ny, nz);
+      bcFactoryU = factory();
+      bcFactoryV = factory();
+      bcFactoryW = factory();
+      // wrapper.cpp:111-125: the wall-normal component of a SLIP wall is NO_SLIP
+      if (faces_[0].type == SLIP) bcFactoryU->xlbc.type = NO_SLIP;
+      if (faces_[1].type == SLIP) bcFactoryU->xrbc.type = NO_SLIP;
+      if (nd > 1 && faces_[2].type == SLIP) bcFactoryV->ylbc.type = NO_SLIP;
+      if (nd > 1 && faces_[3].type == SLIP) bcFactoryV->yrbc.type = NO_SLIP;
+      if (nd > 2 && faces_[4].type == SLIP) bcFactoryW->zlbc.type = NO_SLIP;
+      if (nd > 2 && faces_[5].type == SLIP) bcFactoryW->zrbc.type = NO_SLIP;
     }
-    node_from_face(vel, nvel, lsf);
-    return cycles;
   }
-  // projection() (projection_method.cpp:544-657) with the SOR replaced by the
-  // A(rho) multigrid solve to rel_tol
-  scihpc_solve_stats projection(multigrid &mg, scalar_data *pressure, const scalar_data *density, vector_data *vel,
-                                DataType dt, double rel_tol = 1e-8, int precision = SCIHPC_PREC_MIXED) {
-    scihpc_solve_opts o{rel_tol, 200, 2, 2, precision, 0, 1, 1};
-    scihpc_solve_stats st{};
-    std::vector<double> zero;
-    double *w = vel_w(vel, pressure, zero);
-    check(scihpc_proj_step(h_, mg.handle(), dt, &o, base(density->data), base(vel->x->data), base(vel->y->data), w,
-                           base(pressure->data), &st));
-    return st;
+};
+
+// ------------------------------------------------ flux and source callbacks
+using flux_fn = void (*)(scalar_data *, vector_data *);
+using rhs_fn = void (*)(wrapper *, wrapper *, DataType ***, flux_fn);
+
+// flux.cpp:7-30: flux <- data
+inline void identity_flux(scalar_data *f) { std::memcpy(base(f->flux), base(f->data), sizeof(DataType) * f->size()); }
+inline void identity_flux(vector_data *vel) {
+  identity_flux(&vel->x);
+  identity_flux(&vel->y);
+  identity_flux(&vel->z);
+}
+inline void identity_flux(scalar_data *f, vector_data *) { identity_flux(f); }
+// flux.cpp:86-96: flux <- data^2 / 2 (a host flux; the device steps of
+// runge_kutta take identity_flux)
+inline void burgers_flux(scalar_data *f, vector_data *) {
+  const DataType *d = base(f->data);
+  DataType *q = base(f->flux);
+  for (long c = 0; c < f->size(); ++c) q[c] = d[c] * d[c] * 0.5;
+}
+
+// source.h:14-19: the right-hand sides runge_kutta recognises; they run as
+// device steps inside tvd_rk3 / euler, not as host callbacks
+inline void convection(wrapper *, wrapper *, DataType ***, flux_fn) {
+  device_only("convection", "runge_kutta::tvd_rk3(f, vel, identity_flux, convection)");
+}
+inline void mpls(wrapper *, wrapper *, DataType ***, flux_fn) {
+  device_only("mpls", "runge_kutta::euler(phi, vel, identity_flux, mpls)");
+}
+inline void lsf_redistance_lambda(wrapper *, wrapper *, DataType ***, flux_fn) {
+  device_only("lsf_redistance_lambda", "runge_kutta::tvd_rk3(phi, vel, identity_flux, lsf_redistance_lambda)");
+}
+inline void convection_sec(wrapper *, wrapper *, DataType ***, flux_fn) {
+  device_only("convection_sec", "projection_method::find_source_sec");
+}
+inline void lsf_redistance_no_lambda(wrapper *, wrapper *, DataType ***, flux_fn) {
+  device_only("lsf_redistance_no_lambda", "runge_kutta::tvd_rk3 with lsf_redistance_lambda");
+}
+inline void Hamilton_Jacobi(wrapper *, wrapper *, DataType ***, flux_fn) {
+  device_only("Hamilton_Jacobi", "runge_kutta::tvd_rk3 with lsf_redistance_lambda");
+}
+
+// ------------------------------------------------ level-set helpers
+// lsm.cpp:8-43 (scalar formulas)
+inline DataType Heaviside(DataType x, DataType ls_width) {
+  if (x < -ls_width) return 0.0;
+  if (x > ls_width) return 1.0;
+  return 0.5 * (1.0 + x / ls_width + std::sin(pi * x / ls_width) / pi);
+}
+inline DataType Delta(DataType x, DataType ls_width) {
+  if (std::fabs(x) > ls_width) return 0.0;
+  return 0.5 * (1.0 + std::cos(pi * x / ls_width)) / ls_width;
+}
+inline DataType Sign(const DataType &x, const DataType &ls_width) {
+  if (x < -ls_width) return -1.0;
+  if (x > ls_width) return 1.0;
+  return x / std::sqrt(x * x + ls_width * ls_width);
+}
+
+namespace detail {
+inline scihpc_ls *transport(wrapper *f) {
+  if (!f->is_scalar) throw error(SCIHPC_BAD_ARG, "scihpc: a level-set step needs a scalar wrapper");
+  scihpc_ls *h = f->bcFactory->handle(f->scalar);
+  check(scihpc_ls_set_phi(h, base(f->scalar->data)));
+  return h;
+}
+inline void fetch_phi(wrapper *f, scihpc_ls *h) { check(scihpc_ls_get_phi(h, base(f->scalar->data))); }
+inline void need(const void *p, const char *what) {
+  if (!p) throw error(SCIHPC_BAD_ARG, std::string("scihpc: the wrapper has no ") + what + " linked");
+}
+}  // namespace detail
+
+// wrapper_func.cpp:20-40: dummy->sign / dummy->delta of the current phi
+inline void find_sign(wrapper *lsf) {
+  detail::need(lsf->params, "parameters");
+  detail::need(lsf->dummy, "dummy_data");
+  scihpc_ls *h = detail::transport(lsf);
+  check(scihpc_ls_find_sign(h, lsf->params->ls_width));
+  check(scihpc_ls_get_field(h, SCIHPC_LS_SIGN, base(lsf->dummy->sign)));
+}
+inline void find_delta(wrapper *lsf) {
+  detail::need(lsf->params, "parameters");
+  detail::need(lsf->dummy, "dummy_data");
+  scihpc_ls *h = detail::transport(lsf);
+  check(scihpc_ls_find_delta(h, lsf->params->ls_width));
+  check(scihpc_ls_get_field(h, SCIHPC_LS_DELTA, base(lsf->dummy->delta)));
+}
+// wrapper_func.cpp:9-18, 387-416: Heaviside, density and viscosity of phi
+// (the density formula with ratio 0 is H itself)
+inline void find_heavyside(wrapper *lsf) {
+  detail::need(lsf->params, "parameters");
+  detail::need(lsf->dummy, "dummy_data");
+  check(scihpc_ls_density(detail::transport(lsf), lsf->params->ls_width, 0.0, 1, base(lsf->dummy->heaviside)));
+}
+inline void find_density(wrapper *lsf) {
+  detail::need(lsf->params, "parameters");
+  detail::need(lsf->dummy, "dummy_data");
+  check(scihpc_ls_density(detail::transport(lsf), lsf->params->ls_width, lsf->params->density_ratio,
+                          lsf->params->positive_ref ? 1 : 0, base(lsf->dummy->density)));
+}
+inline void find_viscosity(wrapper *lsf) {
+  detail::need(lsf->params, "parameters");
+  detail::need(lsf->dummy, "dummy_data");
+  check(scihpc_ls_density(detail::transport(lsf), lsf->params->ls_width, lsf->params->viscosity_ratio,
+                          lsf->params->positive_ref ? 1 : 0, base(lsf->dummy->viscosity)));
+}
+inline void find_gradient(wrapper *) { device_only("find_gradient", "runge_kutta::euler with mpls"); }
+inline void find_curvature(wrapper *) { device_only("find_curvature", "projection_method::find_source_sec"); }
+
+// wrapper_func.cpp:450-511
+inline DataType lsf_mass(wrapper *lsf) {
+  detail::need(lsf->params, "parameters");
+  double m = 0.0;
+  check(scihpc_ls_mass(detail::transport(lsf), lsf->params->ls_width, lsf->params->density_ratio,
+                       lsf->params->positive_ref ? 1 : 0, &m));
+  return m;
+}
+
+// godunov_gradient.cpp:159-181: phi /= max Godunov gradient (stored sign);
+// the gradient lands in dummy->grad
+inline void stabilized_upon_gradient(wrapper *f) {
+  detail::need(f->dummy, "dummy_data");
+  scihpc_ls *h = detail::transport(f);
+  check(scihpc_ls_set_field(h, SCIHPC_LS_SIGN, base(f->dummy->sign)));
+  check(scihpc_ls_stabilize_gradient(h, base(f->dummy->grad), nullptr));
+  detail::fetch_phi(f, h);
+}
+inline void godunov_gradient(wrapper *) { device_only("godunov_gradient", "stabilized_upon_gradient or the redistancing step"); }
+
+// wrapper_func.cpp:60-113, 295-333, 420-448: the drivers' diagnostics, over
+// the host fields
+inline void store_tmp(wrapper *f) {
+  detail::need(f->dummy, "dummy_data");
+  if (f->is_scalar) {
+    std::memcpy(base(f->dummy->tmp), base(f->scalar->data), sizeof(DataType) * f->scalar->size());
+  } else {
+    const size_t n = sizeof(DataType) * f->vector->x.size();
+    std::memcpy(base(f->dummy->u_tmp), base(f->vector->x.data), n);
+    std::memcpy(base(f->dummy->v_tmp), base(f->vector->y.data), n);
+    std::memcpy(base(f->dummy->w_tmp), base(f->vector->z.data), n);
+  }
+}
+inline DataType l2norm(wrapper *f) {
+  detail::need(f->dummy, "dummy_data");
+  DataType e = 0.0;
+  if (f->is_scalar) {
+    const long n = f->scalar->size();
+    const DataType *a = base(f->dummy->tmp), *b = base(f->scalar->data);
+    for (long c = 0; c < n; ++c) e += (a[c] - b[c]) * (a[c] - b[c]);
+    return std::sqrt(e / n);
+  }
+  const long n = f->vector->x.size();
+  const DataType *t[3] = {base(f->dummy->u_tmp), base(f->dummy->v_tmp), base(f->dummy->w_tmp)};
+  const DataType *v[3] = {base(f->vector->x.data), base(f->vector->y.data), base(f->vector->z.data)};
+  for (long c = 0; c < n; ++c)
+    for (int q = 0; q < 3; ++q) e += (t[q][c] - v[q][c]) * (t[q][c] - v[q][c]);
+  return std::sqrt(e / n);
+}
+inline DataType linfity(wrapper *f) {
+  detail::need(f->dummy, "dummy_data");
+  DataType e = 0.0;
+  if (f->is_scalar) {
+    const long n = f->scalar->size();
+    const DataType *a = base(f->dummy->tmp), *b = base(f->scalar->data);
+    for (long c = 0; c < n; ++c) e = std::fmax(e, std::fabs(a[c] - b[c]));
+    return std::sqrt(e / n);
+  }
+  const long n = f->vector->x.size();
+  const DataType *t[3] = {base(f->dummy->u_tmp), base(f->dummy->v_tmp), base(f->dummy->w_tmp)};
+  const DataType *v[3] = {base(f->vector->x.data), base(f->vector->y.data), base(f->vector->z.data)};
+  for (long c = 0; c < n; ++c)
+    for (int q = 0; q < 3; ++q) e = std::fmax(e, std::fabs(t[q][c] - v[q][c]));
+  return std::sqrt(e / n);
+}
+// largest |div u| over the interior (wrapper_func.cpp:295-333)
+inline DataType divergence(wrapper *vel) {
+  const scalar_data &u = vel->vector->x, &v = vel->vector->y, &w = vel->vector->z;
+  DataType m = 0.0;
+  const int nz = u.ndim == 2 ? u.Nz : u.nz;
+  for (int i = 0; i < u.nx; ++i)
+    for (int j = 0; j < u.ny; ++j)
+      for (int k = 0; k < nz; ++k) {
+        const indices q = u.index_mapping(i + 1, j + 1, k + 1);
+        DataType d = (u.data[q.i][q.j][q.k] - u.data[q.i - 1][q.j][q.k]) / vel->geo->dx +
+                     (v.data[q.i][q.j][q.k] - v.data[q.i][q.j - 1][q.k]) / vel->geo->dy;
+        if (u.ndim == 3) d += (w.data[q.i][q.j][q.k] - w.data[q.i][q.j][q.k - 1]) / vel->geo->dz;
+        m = std::fmax(m, std::fabs(d));
+      }
+  return m;
+}
+// find_dt (wrapper_func.cpp:335-385): the stable CFL from the largest
+// velocities (and curvature), dt clamped to [min_CFL, max_CFL] h
+inline void find_dt(wrapper *vel) {
+  detail::need(vel->params, "parameters");
+  detail::need(vel->dummy, "dummy_data");
+  const scalar_data &u = vel->vector->x, &v = vel->vector->y, &w = vel->vector->z;
+  DataType mu = 0, mv = 0, mw = 0, mk = 0;
+  for (int I = 0; I < u.nx; ++I)
+    for (int J = 0; J < u.ny; ++J)
+      for (int K = 0; K < u.nz; ++K) {
+        const indices q = u.index_mapping(I + 1, J + 1, K + 1);
+        mu = std::fmax(mu, std::fabs(u.data[q.i][q.j][q.k]));
+        mv = std::fmax(mv, std::fabs(v.data[q.i][q.j][q.k]));
+        mw = std::fmax(mw, std::fabs(w.data[q.i][q.j][q.k]));
+        mk = std::fmax(vel->dummy->delta[q.i][q.j][q.k] * mk, std::fabs(vel->dummy->curvature[q.i][q.j][q.k]));
+      }
+  problem_parameters *p = vel->params;
+  const structured_grid *g = vel->geo;
+  DataType conv = mu / g->dx + mv / g->dy;
+  if (u.ndim == 3) conv += mw / g->dz;
+  DataType diff = 2.0 / (g->dx * g->dx) + 2.0 / (g->dy * g->dy);
+  if (u.ndim == 3) diff += 2.0 / (g->dz * g->dz);
+  diff *= p->viscosity_ratio / p->density_ratio / p->Reynolds_number;
+  DataType grav = 0.0, surf = 0.0;
+  if (p->Froude_number > 0.0)
+    grav = std::sqrt(std::fabs(1.0 - p->density_ratio) / (2 * p->density_ratio) / g->h / p->Froude_number);
+  if (p->Weber_number > 0.0) surf = std::sqrt(mk / std::pow(g->h, 2) / p->Weber_number / p->density_ratio);
+  const DataType cfl = conv + diff + std::sqrt(std::pow(conv + diff, 2) + 4 * std::pow(grav, 2) + 4 * std::pow(surf, 2));
+  p->stable_CFL = 1.0 / cfl / g->h;
+  p->dt_old2 = p->dt_old;
+  p->dt_old = p->dt;
+  p->dt = std::fmax(std::fmin(p->stable_CFL, p->max_CFL), p->min_CFL) * g->h;
+}
+inline DataType lsf_volume(wrapper *) { device_only("lsf_volume", "lsf_mass"); }
+
+// ------------------------------------------------------------ runge_kutta
+// runge_kutta.h:12-23: s1..s3 are the reference's stage arrays (kept for
+// drivers that read them; the device keeps its own)
+class runge_kutta {
+ public:
+  runge_kutta(int nx, int ny, int nz)
+      : s1(init_array(nx, ny, nz)), s2(init_array(nx, ny, nz)), s3(init_array(nx, ny, nz)), nx_(nx), ny_(ny) {}
+  runge_kutta(const runge_kutta &) = delete;
+  runge_kutta &operator=(const runge_kutta &) = delete;
+  ~runge_kutta() {
+    delete3d(s1, nx_, ny_);
+    delete3d(s2, nx_, ny_);
+    delete3d(s3, nx_, ny_);
+  }
+  DataType ***s1, ***s2, ***s3;
+
+  // runge_kutta.cpp:13-49: three stages of dt = params->dt with the
+  // wrapper's BCs after each
+  void tvd_rk3(wrapper *f, wrapper *vel, flux_fn flux, rhs_fn rhs) const {
+    detail::need(f->params, "parameters");
+    if (flux != static_cast<flux_fn>(&identity_flux))
+      throw error(SCIHPC_BAD_ARG, "scihpc: tvd_rk3 runs on the device with identity_flux only");
+    if (rhs == &convection) {
+      // source.cpp:8-40: -u . grad(phi) with UCCD derivatives (the
+      // reference's convection term), on every padded cell
+      scihpc_ls *h = detail::transport(f);
+      check(scihpc_ls_set_scheme(h, SCIHPC_LS_UCCD));
+      check(scihpc_ls_set_velocity(h, base(vel->vector->x.data), base(vel->vector->y.data), base(vel->vector->z.data)));
+      check(scihpc_ls_tvd_rk3(h, f->params->dt, 1));
+      detail::fetch_phi(f, h);
+    } else if (rhs == &lsf_redistance_lambda) {
+      // source.cpp:148-240 with the stored sign and delta
+      detail::need(f->dummy, "dummy_data");
+      scihpc_ls *h = detail::transport(f);
+      check(scihpc_ls_set_field(h, SCIHPC_LS_SIGN, base(f->dummy->sign)));
+      check(scihpc_ls_set_field(h, SCIHPC_LS_DELTA, base(f->dummy->delta)));
+      check(scihpc_ls_redistance(h, f->params->dt, f->params->rdt, 1));
+      detail::fetch_phi(f, h);
+    } else {
+      throw error(SCIHPC_BAD_ARG, "scihpc: tvd_rk3 runs convection and lsf_redistance_lambda on the device; "
+                                  "this right-hand side has no device form");
+    }
+  }
+  // runge_kutta.cpp:51-62: one stage of dt = params->dt
+  void euler(wrapper *f, wrapper *vel, flux_fn flux, rhs_fn rhs) const {
+    (void)vel;
+    detail::need(f->params, "parameters");
+    if (flux != static_cast<flux_fn>(&identity_flux) || rhs != &mpls)
+      throw error(SCIHPC_BAD_ARG, "scihpc: euler runs mpls with identity_flux on the device only");
+    // source.cpp:109-146: phi moved along its normal toward params->lsf_mass0
+    detail::need(f->dummy, "dummy_data");
+    problem_parameters *p = f->params;
+    scihpc_ls *h = detail::transport(f);
+    check(scihpc_ls_mpls(h, p->dt, p->ls_width, p->density_ratio, p->positive_ref ? 1 : 0, p->lsf_mass0, 1));
+    detail::fetch_phi(f, h);
+    check(scihpc_ls_get_field(h, SCIHPC_LS_DELTA, base(f->dummy->delta)));
   }
 
  private:
-  scihpc_proj *h_ = nullptr;
-  static double *base(DataType ***f) { return &f[0][0][0]; }
-  static double *vel_w(vector_data *vel, const scalar_data *like, std::vector<double> &zero) {
-    if (vel->z) return base(vel->z->data);
-    zero.assign(like->size(), 0.0);
-    return zero.data();
-  }
+  int nx_, ny_;
 };
 
-// the level-set transport of one scalar field on the GPU
-class level_set_solver {
+// ------------------------------------------------------ projection_method
+// projection_method.h:14-51.  The momentum predictor, both pressure steps
+// and node_from_face run on one device handle (created at the first call,
+// with the velocity wrapper's BCs); the sources live there between steps.
+class projection_method {
  public:
-  level_set_solver(scalar_data *f, structured_grid *geo, const bc_info faces[6]) : f_(f) {
-    int types[6];
-    double vals[6];
-    for (int q = 0; q < 6; ++q) {
-      types[q] = faces[q].type;
-      vals[q] = faces[q].value;
-    }
-    check(scihpc_ls_create(f->ndim, f->nx, f->ny, f->nz, geo->dx, geo->dy, f->ndim == 3 ? geo->dz : 1.0, types,
-                           vals, &h_));
+  explicit projection_method(scalar_data *f) : f_(f) {
+    for (DataType ****p : {&u_src, &v_src, &w_src, &u_src_old, &v_src_old, &w_src_old, &CC, &CR, &CL, &CU, &CD, &CF,
+                           &CB, &RHS})
+      *p = init_array(f->Nx, f->Ny, f->Nz);
   }
-  ~level_set_solver() {
-    if (h_) scihpc_ls_destroy(h_);
+  projection_method(const projection_method &) = delete;
+  projection_method &operator=(const projection_method &) = delete;
+  ~projection_method() {
+    if (h_) scihpc_proj_destroy(h_);
+    if (mg_rho_) scihpc_mg_destroy(mg_rho_);
+    for (DataType ***p : {u_src, v_src, w_src, u_src_old, v_src_old, w_src_old, CC, CR, CL, CU, CD, CF, CB, RHS})
+      delete3d(p, f_->Nx, f_->Ny);
   }
-  level_set_solver(const level_set_solver &) = delete;
-  level_set_solver &operator=(const level_set_solver &) = delete;
-  scihpc_ls *handle() const { return h_; }
-  scalar_data *field() const { return f_; }
 
- protected:
+  DataType ***u_src, ***v_src, ***w_src;
+  DataType ***u_src_old, ***v_src_old, ***w_src_old;
+  DataType ***CC, ***CR, ***CL, ***CU, ***CD, ***CF, ***CB, ***RHS;
+
+  // find_source_sec (projection_method.cpp:314-382): sources of every
+  // component (second-order convection, stresses, gravity, surface force)
+  // from vel and the level set; nvel holds the last face transfer;
+  // lsf->dummy gets density and delta
+  void find_source_sec(wrapper *vel, wrapper *nvel, wrapper *lsf) {
+    scihpc_proj *h = handle(vel);
+    const scihpc_flow_params fp = flow(lsf);
+    copy3(u_src, v_src, w_src, u_src_old, v_src_old, w_src_old);
+    check(scihpc_proj_find_sources(h, &fp, base(lsf->scalar->data), base(vel->vector->x.data),
+                                   base(vel->vector->y.data), base(vel->vector->z.data), base(nvel->vector->x.data),
+                                   base(nvel->vector->y.data), base(nvel->vector->z.data),
+                                   lsf->dummy ? base(lsf->dummy->density) : nullptr));
+    check(scihpc_proj_get_sources(h, base(u_src), base(v_src), base(w_src)));
+    if (lsf->dummy) find_delta(lsf);
+  }
+  // find_source (projection_method.cpp:244-312, the UCCD predictor): the
+  // same sources from the second-order predictor (equal at rest)
+  void find_source(wrapper *vel, wrapper *nvel, wrapper *lsf) { find_source_sec(vel, nvel, lsf); }
+  // find_intermediate_velocity (:384-402): AB2 with the sources, velocity BCs
+  void find_intermediate_velocity(wrapper *vel) {
+    scihpc_proj *h = handle(vel);
+    const scihpc_flow_params fp = flow(vel);
+    check(scihpc_proj_intermediate_velocity(h, &fp, base(vel->vector->x.data), base(vel->vector->y.data),
+                                            base(vel->vector->z.data)));
+  }
+  // projection() (:544-657): rhs = div u* / dt, A(rho) p = rhs (flexible-CG
+  // multigrid, mixed precision, relative residual 1e-8 in place of the
+  // SOR), u -= dt / (rho_face h) dp, BCs; rho = lsf->dummy->density
+  void projection(wrapper *pressure, wrapper *lsf, wrapper *vel) {
+    detail::need(lsf->dummy, "dummy_data");
+    scihpc_proj *h = handle(vel);
+    if (!mg_rho_) {
+      const scalar_data *s = pressure->scalar;
+      check(scihpc_mg_create(s->ndim, s->nx, s->ny, s->ndim == 3 ? s->nz : 1, pressure->geo->dx, pressure->geo->dy,
+                             s->ndim == 3 ? pressure->geo->dz : 1.0, SCIHPC_MG_NEUMANN, &mg_rho_));
+    }
+    scihpc_solve_opts o{1e-8, 200, 2, 2, SCIHPC_PREC_MIXED, 0, 1, 1, 0, 0};
+    scihpc_solve_stats st{};
+    check(scihpc_proj_step(h, mg_rho_, pressure->params->dt, &o, base(lsf->dummy->density), base(vel->vector->x.data),
+                           base(vel->vector->y.data), base(vel->vector->z.data), base(pressure->scalar->data), &st));
+    last_cycles = st.cycles;
+  }
+  // pressure_guess_solution + init_ppe_fpc_coefficients +
+  // fast_pressure_correction (:467-541, 659-753, 803-815) in one device
+  // step: p_hat from pressure->old/old2 into ->flux, full cycles of the
+  // reference-faithful multigrid of pressure->solvers->mg (its level-0
+  // solution is the warm start) until ||r|| <= ppe_tol (at most 100)
+  void fast_pressure_correction(wrapper *pressure, wrapper *lsf, wrapper *vel) {
+    detail::need(lsf->dummy, "dummy_data");
+    detail::need(pressure->solvers, "shared_solvers");
+    problem_parameters *p = pressure->params;
+    scihpc_fpc_params fp{p->dt, p->dt_old, p->dt_old2, p->density_ratio, p->ppe_tol, 100};
+    int cycles = 0;
+    check(scihpc_proj_fpc_step(handle(vel), pressure->solvers->mg->handle(), &fp, base(lsf->dummy->density),
+                               base(pressure->scalar->old), base(pressure->scalar->old2), base(vel->vector->x.data),
+                               base(vel->vector->y.data), base(vel->vector->z.data), base(pressure->scalar->data),
+                               base(pressure->scalar->flux), &cycles));
+    last_cycles = cycles;
+  }
+  // ab_solve_sec (:779-800) and ab_solve (:755-777, same predictor here)
+  void ab_solve_sec(wrapper *vel, wrapper *nvel, wrapper *pressure, wrapper *lsf) {
+    pressure->scalar->store();
+    find_source_sec(vel, nvel, lsf);
+    find_intermediate_velocity(vel);
+    for (int cnt = 0; cnt < pressure->params->ppe_initer; ++cnt) {
+      if (pressure->params->iter < 5) projection(pressure, lsf, vel);
+      else fast_pressure_correction(pressure, lsf, vel);
+    }
+    node_from_face(vel, nvel);
+  }
+  void ab_solve(wrapper *vel, wrapper *nvel, wrapper *pressure, wrapper *lsf) { ab_solve_sec(vel, nvel, pressure, lsf); }
+  // node_from_face + nvel->apply_nvel_bc (wrapper_func.cpp:264-293)
+  void node_from_face(wrapper *vel, wrapper *nvel) {
+    check(scihpc_proj_node_velocity(handle(vel), base(vel->vector->x.data), base(vel->vector->y.data),
+                                    base(vel->vector->z.data), base(nvel->vector->x.data), base(nvel->vector->y.data),
+                                    base(nvel->vector->z.data)));
+  }
+
+  void add_stress_x(wrapper *, wrapper *, wrapper *) const { device_only("add_stress_x", "find_source_sec"); }
+  void add_stress_y(wrapper *, wrapper *, wrapper *) const { device_only("add_stress_y", "find_source_sec"); }
+  void add_stress_z(wrapper *, wrapper *, wrapper *) const { device_only("add_stress_z", "find_source_sec"); }
+  void add_body_force(wrapper *) const { device_only("add_body_force", "find_source_sec"); }
+  void add_surface_force(wrapper *) const { device_only("add_surface_force", "find_source_sec"); }
+  void init_ppe_proj_coefficients(wrapper *) { device_only("init_ppe_proj_coefficients", "projection"); }
+  void init_ppe_fpc_coefficients(wrapper *) { device_only("init_ppe_fpc_coefficients", "fast_pressure_correction"); }
+  void pressure_guess_solution(wrapper *) { device_only("pressure_guess_solution", "fast_pressure_correction"); }
+
+  int last_cycles = 0;  // multigrid cycles (or iterations) of the last pressure step
+  scihpc_proj *device_handle() const { return h_; }
+
+ private:
   scalar_data *f_;
-  scihpc_ls *h_ = nullptr;
-  void set_velocity(vector_data *vel) {
-    std::vector<double> zero;
-    const double *w = nullptr;
-    if (vel->z) w = &vel->z->data[0][0][0];
-    else {
-      zero.assign(f_->size(), 0.0);
-      w = zero.data();
+  scihpc_proj *h_ = nullptr;
+  scihpc_mg *mg_rho_ = nullptr;
+
+  scihpc_proj *handle(wrapper *vel) {
+    if (!h_) {
+      if (vel->is_scalar) throw error(SCIHPC_BAD_ARG, "scihpc: the projection step needs the velocity wrapper");
+      int t[6];
+      vel->face_codes(t);
+      const structured_grid *g = vel->geo;
+      check(scihpc_proj_create(f_->ndim, f_->nx, f_->ny, f_->ndim == 3 ? f_->nz : 1, g->dx, g->dy,
+                               f_->ndim == 3 ? g->dz : 1.0, t, &h_));
     }
-    check(scihpc_ls_set_velocity(h_, &vel->x->data[0][0][0], &vel->y->data[0][0][0], w));
+    return h_;
+  }
+  static scihpc_flow_params flow(wrapper *lsf) {
+    detail::need(lsf->params, "parameters");
+    const problem_parameters *p = lsf->params;
+    return scihpc_flow_params{p->dt, p->Reynolds_number, p->Froude_number, p->Weber_number, p->ls_width,
+                              p->density_ratio, p->viscosity_ratio, p->positive_ref ? 1 : 0};
+  }
+  void copy3(DataType ***a, DataType ***b, DataType ***c, DataType ***ao, DataType ***bo, DataType ***co) {
+    const size_t n = sizeof(DataType) * f_->size();
+    std::memcpy(base(ao), base(a), n);
+    std::memcpy(base(bo), base(b), n);
+    std::memcpy(base(co), base(c), n);
   }
 };
 
-// weno_solver.h:17-48: f->fx, fy, fz from f->flux, upwinded by vel
-class weno_solver : public level_set_solver {
- public:
-  using level_set_solver::level_set_solver;
-  void weno5_find_derivatives(scalar_data *f, vector_data *vel) {
-    set_velocity(vel);
-    check(scihpc_ls_weno5_derivatives(h_, &f->flux[0][0][0], &f->fx[0][0][0], &f->fy[0][0][0], &f->fz[0][0][0]));
-  }
-};
-
-// runge_kutta.h:12-23: TVD-RK3 of d phi/dt = -u . grad(phi) with WENO5
-// (default) or UCCD (set_scheme(SCIHPC_LS_UCCD): the reference's
-// `convection`) derivatives of the identity flux and the solver's BCs after
-// every stage; plus the reference's other right-hand sides run through
-// runge_kutta: euler with mpls (mass correction) and tvd_rk3 with
-// lsf_redistance_lambda (redistancing).  Each call copies phi in and out.
-class runge_kutta : public level_set_solver {
- public:
-  using level_set_solver::level_set_solver;
-  void set_scheme(int scheme) { check(scihpc_ls_set_scheme(h_, scheme)); }
-  void tvd_rk3(vector_data *vel, DataType dt, int steps = 1) {
-    set_velocity(vel);
-    check(scihpc_ls_set_phi(h_, &f_->data[0][0][0]));
-    check(scihpc_ls_tvd_rk3(h_, dt, steps));
-    check(scihpc_ls_get_phi(h_, &f_->data[0][0][0]));
-  }
-  // lsf_mass (wrapper_func.cpp:450-511)
-  DataType lsf_mass(DataType ls_width, DataType density_ratio, bool positive_ref = true) {
-    double m = 0.0;
-    check(scihpc_ls_set_phi(h_, &f_->data[0][0][0]));
-    check(scihpc_ls_mass(h_, ls_width, density_ratio, positive_ref ? 1 : 0, &m));
-    return m;
-  }
-  // `count` x euler(phi, nvel, identity_flux, mpls) (source.cpp:109-146)
-  void euler_mpls(DataType dt, DataType ls_width, DataType density_ratio, DataType mass0, int count = 1,
-                  bool positive_ref = true) {
-    check(scihpc_ls_set_phi(h_, &f_->data[0][0][0]));
-    check(scihpc_ls_mpls(h_, dt, ls_width, density_ratio, positive_ref ? 1 : 0, mass0, count));
-    check(scihpc_ls_get_phi(h_, &f_->data[0][0][0]));
-  }
-  // find_sign / find_delta (wrapper_func.cpp:20-40) of the current phi
-  void find_sign(DataType ls_width) {
-    check(scihpc_ls_set_phi(h_, &f_->data[0][0][0]));
-    check(scihpc_ls_find_sign(h_, ls_width));
-  }
-  void find_delta(DataType ls_width) {
-    check(scihpc_ls_set_phi(h_, &f_->data[0][0][0]));
-    check(scihpc_ls_find_delta(h_, ls_width));
-  }
-  // `steps` x tvd_rk3(phi, nvel, identity_flux, lsf_redistance_lambda)
-  // with the stored sign and delta (source.cpp:148-240)
-  void tvd_rk3_redistance(DataType dt, DataType rdt, int steps = 1) {
-    check(scihpc_ls_set_phi(h_, &f_->data[0][0][0]));
-    check(scihpc_ls_redistance(h_, dt, rdt, steps));
-    check(scihpc_ls_get_phi(h_, &f_->data[0][0][0]));
-  }
-};
-
+// -------------------------------------------------------------- vtkWriter
 // vtkWriter.h:15-35: legacy VTK snapshots, byte-identical to the
 // reference's files; create(id) opens <filename><id>.vtk.  The device does
 // the reorder and byte swap, a background thread writes the file, close()
-// waits for it.  A 2-D vector_data of this facade has no z component: its
-// VECTORS get zeros there, as the reference's zero-filled z field.
+// waits for it.
 class vtkWriter {
  public:
-  vtkWriter(structured_grid *geo, std::string filename) : grid(geo), filename(std::move(filename)) {
-    const int n[3] = {geo->x_axis.n, geo->ndim > 1 ? geo->y_axis.n : 1, geo->ndim > 2 ? geo->z_axis.n : 1};
-    const double org[3] = {geo->x_axis.start, geo->y_axis.start, geo->z_axis.start};
-    const double sp[3] = {geo->dx, geo->dy, geo->dz};
-    check(scihpc_vtk_create(geo->ndim, n[0], n[1], n[2], org, sp, this->filename.c_str(), &h_));
+  vtkWriter(structured_grid *_geo, std::string _filename) : grid(_geo), filename(std::move(_filename)) {
+    const int n[3] = {grid->x_axis.n, grid->ndim > 1 ? grid->y_axis.n : 1, grid->ndim > 2 ? grid->z_axis.n : 1};
+    const double org[3] = {grid->x_axis.start, grid->y_axis.start, grid->z_axis.start};
+    const double sp[3] = {grid->dx, grid->dy, grid->dz};
+    check(scihpc_vtk_create(grid->ndim, n[0], n[1], n[2], org, sp, filename.c_str(), &h_));
   }
   ~vtkWriter() { scihpc_vtk_destroy(h_); }
   vtkWriter(const vtkWriter &) = delete;
@@ -588,18 +1299,14 @@ class vtkWriter {
 
   void create(unsigned int id) { check(scihpc_vtk_begin(h_, id)); }
   void add_scalar(scalar_data *data, const std::string &name) { add_scalar(data->data, data, name); }
-  void add_scalar(DataType ***f, scalar_data *, const std::string &name) {
-    check(scihpc_vtk_add_scalar(h_, name.c_str(), &f[0][0][0], 0));
+  void add_scalar(DataType ***_data, scalar_data *, const std::string &name) {
+    check(scihpc_vtk_add_scalar(h_, name.c_str(), base(_data), 0));
   }
   void add_vector(vector_data *data, const std::string &name) {
-    add_vector(data->x->data, data->y ? data->y->data : nullptr, data->z ? data->z->data : nullptr, data, name);
+    add_vector(data->x.data, data->y.data, data->z.data, data, name);
   }
-  void add_vector(DataType ***u, DataType ***v, DataType ***w, vector_data *data, const std::string &name) {
-    std::vector<DataType> zero;
-    if (!u || !v || !w) zero.assign(data->x->size(), 0.0);
-    auto ptr = [&](DataType ***f) -> const DataType * { return f ? &f[0][0][0] : zero.data(); };
-    const DataType *pu = ptr(u), *pv = ptr(v), *pw = ptr(w);
-    check(scihpc_vtk_add_vector(h_, name.c_str(), pu, pv, pw, 0));
+  void add_vector(DataType ***u, DataType ***v, DataType ***w, vector_data *, const std::string &name) {
+    check(scihpc_vtk_add_vector(h_, name.c_str(), base(u), base(v), base(w), 0));
   }
   void close() { check(scihpc_vtk_close(h_)); }
 

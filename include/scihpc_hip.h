@@ -276,6 +276,19 @@ typedef struct {
 SCIHPC_API int scihpc_proj_momentum_step(scihpc_proj *h, const scihpc_flow_params *prm,
                                          const double *phi, double *u, double *v, double *w,
                                          double *nu, double *nv, double *nw, double *rho);
+/* projection_method::find_source_sec (projection_method.cpp:314-382)
+ * alone: the sources of scihpc_proj_momentum_step without the velocity
+ * update.  The handle's sources rotate (the previous ones are kept for the
+ * AB2 step, as u_src_old); u, v, w are read only; nu, nv, nw receive the
+ * last face transfer; rho (nullable) find_density. */
+SCIHPC_API int scihpc_proj_find_sources(scihpc_proj *h, const scihpc_flow_params *prm, const double *phi,
+                                        const double *u, const double *v, const double *w, double *nu,
+                                        double *nv, double *nw, double *rho);
+/* projection_method::find_intermediate_velocity (:384-402): u += dt (1.5
+ * u_src - 0.5 u_src_old) (v, w alike) with the handle's sources, then the
+ * velocity BCs. */
+SCIHPC_API int scihpc_proj_intermediate_velocity(scihpc_proj *h, const scihpc_flow_params *prm, double *u,
+                                                 double *v, double *w);
 /* projection_method::u_src/v_src/w_src (projection_method.h:18) */
 SCIHPC_API int scihpc_proj_set_sources(scihpc_proj *h, const double *su, const double *sv,
                                        const double *sw);
@@ -429,6 +442,12 @@ SCIHPC_API int scihpc_ls_get_field(scihpc_ls *h, int which, double *dst);
  * (source.cpp:148-240, godunov_gradient.cpp:7-201): pseudo-time rdt,
  * the stored sign and delta (not recomputed, as in the reference). */
 SCIHPC_API int scihpc_ls_redistance(scihpc_ls *h, double dt, double rdt, int nsteps);
+/* stabilized_upon_gradient (godunov_gradient.cpp:159-181): the Godunov
+ * gradient of phi with the stored sign (godunov_gradient, :7-152) on every
+ * padded cell, phi /= its maximum.  grad (nullable, padded host array)
+ * receives the gradient (the reference's dummy->grad), max_grad (nullable)
+ * the maximum. */
+SCIHPC_API int scihpc_ls_stabilize_gradient(scihpc_ls *h, double *grad, double *max_grad);
 
 /* ------------------------------------------------------------------ */
 /* Snapshots: vtkWriter (vtkWriter.h:15-35, vtkWriter.cpp:18-111).  Files are

@@ -11,6 +11,7 @@
 #include <string>
 
 #include "bc_factory.h"
+#include "godunov_gradient.h"
 #include "multigrid.h"
 #include "projection_method.h"
 #include "runge_kutta.h"
@@ -612,6 +613,19 @@ void ref_redistance_rk3(int ndim, int nx, int ny, int nz, const double *ext, dou
   runge_kutta rk(L.f->scalar->Nx, L.f->scalar->Ny, L.f->scalar->Nz);
   for (int s = 0; s < nsteps; ++s) rk.tvd_rk3(L.f, L.vel, &identity_flux, &lsf_redistance_lambda);
   from_ref(L.f->scalar, L.f->scalar->data, phi);
+}
+
+// stabilized_upon_gradient(phi) (godunov_gradient.cpp:159-181) with
+// dummy->sign given; phi scaled in place, grad = dummy->grad
+void ref_stabilize_gradient(int ndim, int nx, int ny, int nz, const double *ext, const double *sign, double *phi,
+                            double *grad) {
+  int t[6] = {0, 0, 0, 0, 0, 0};
+  double vv[6] = {0, 0, 0, 0, 0, 0};
+  LsSetup L = ls_setup(ndim, nx, ny, nz, ext, t, vv, phi, nullptr, nullptr, nullptr);
+  to_ref(L.f->scalar, L.dummy->sign, sign);
+  stabilized_upon_gradient(L.f);
+  from_ref(L.f->scalar, L.f->scalar->data, phi);
+  from_ref(L.f->scalar, L.dummy->grad, grad);
 }
 
 }  // extern "C"

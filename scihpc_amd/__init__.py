@@ -79,6 +79,15 @@ class Multigrid:
     def set_sol(self, x, level=0):
         check(self._lib.scihpc_mg_set_sol(self.h, level, _f64(x)))
 
+    def set_rhs_device(self, dptr):
+        """Level-0 b from a device array (int address, n FP64 values, MG
+        layout, complete before the call): a device-to-device copy."""
+        check(self._lib.scihpc_mg_set_rhs_device(self.h, C.c_void_p(int(dptr))))
+
+    def get_sol_device(self, dptr):
+        """Level-0 solution into a device array (int address)."""
+        check(self._lib.scihpc_mg_get_sol_device(self.h, C.c_void_p(int(dptr))))
+
     def _get(self, fn, level):
         out = np.empty(self.n[level])
         check(fn(self.h, level, out))
@@ -379,6 +388,14 @@ class LevelSet:
     def redistance(self, dt, rdt, nsteps=1):
         check(self._lib.scihpc_ls_redistance(self.h, dt, rdt, nsteps))
 
+    def stabilize_gradient(self):
+        """stabilized_upon_gradient (godunov_gradient.cpp:159-181): phi /=
+        max Godunov gradient (stored sign); returns (gradient, maximum)."""
+        grad = np.zeros(self.size)
+        mx = C.c_double()
+        check(self._lib.scihpc_ls_stabilize_gradient(self.h, grad, C.byref(mx)))
+        return grad.reshape(self.shape), mx.value
+
 
 class Projection:
     """The projection step on padded fields around a Multigrid of the same
@@ -446,6 +463,23 @@ class Projection:
         s = [np.zeros(n) for _ in range(3)]
         check(self._lib.scihpc_proj_get_sources(self.h, *s))
         return tuple(s)
+
+    def find_sources(self, prm, phi, u, v, w, nu, nv, nw, want_rho=False):
+        """find_source_sec alone (projection_method.cpp:314-382): the handle's
+        sources rotate, the velocity is not changed.  Returns (nu, nv, nw[,
+        rho])."""
+        out = [_f64(a).copy() for a in (nu, nv, nw)]
+        rho = np.zeros_like(out[0]) if want_rho else None
+        check(self._lib.scihpc_proj_find_sources(self.h, C.byref(prm), _f64(phi), _f64(u), _f64(v), _f64(w),
+                                                 *out, rho.ctypes.data if rho is not None else None))
+        return tuple(out) + ((rho,) if want_rho else ())
+
+    def intermediate_velocity(self, prm, u, v, w):
+        """find_intermediate_velocity (projection_method.cpp:384-402): AB2 with
+        the handle's sources and the velocity BCs.  Returns (u, v, w)."""
+        out = [_f64(a).copy() for a in (u, v, w)]
+        check(self._lib.scihpc_proj_intermediate_velocity(self.h, C.byref(prm), *out))
+        return tuple(out)
 
     def node_velocity(self, u, v, w, nu, nv, nw):
         """node_from_face + apply_nvel_bc; returns (nu, nv, nw)."""

@@ -127,6 +127,8 @@ SIGNATURES = {
     "scihpc_proj_set_sources": [_vp, _dp, _dp, _dp],
     "scihpc_proj_get_sources": [_vp, _dp, _dp, _dp],
     "scihpc_proj_node_velocity": [_vp] + [_dp] * 6,
+    "scihpc_proj_find_sources": [_vp, C.POINTER(FlowParams), _dp] + [_dp] * 6 + [_vp],
+    "scihpc_proj_intermediate_velocity": [_vp, C.POINTER(FlowParams)] + [_dp] * 3,
     "scihpc_proj_time_momentum": [_vp, C.POINTER(FlowParams), _i, _pd],
     "scihpc_ls_create": [_i, _i, _i, _i, _d, _d, _d, _ip, _dp, C.POINTER(_vp)],
     "scihpc_ls_destroy": [_vp],
@@ -138,6 +140,7 @@ SIGNATURES = {
     "scihpc_ls_apply_bc": [_vp, _dp, _i],
     "scihpc_ls_density": [_vp, _d, _d, _i, _dp],
     "scihpc_ls_time_step": [_vp, _d, _i, _pd, _pd],
+    "scihpc_ls_stabilize_gradient": [_vp, _dp, _pd],
     "scihpc_ls_set_scheme": [_vp, _i],
     "scihpc_ls_compact_derivatives": [_vp, _dp, _i] + [_dp] * 6,
     "scihpc_ls_mass": [_vp, _d, _d, _i, _pd],

@@ -1758,6 +1758,55 @@ extern "C" int scihpc_ls_get_field(scihpc_ls *h, int which, double *dst) {
   });
 }
 
+// per-block maxima of x (x >= 0; empty lanes count 0, as the reference's
+// running maximum starts from 0.0)
+__global__ void __launch_bounds__(kLsBlock) k_block_max(const double *__restrict__ x, long n,
+                                                        double *__restrict__ out) {
+  __shared__ double part[kLsBlock / kWave];
+  double v = 0.0;
+  for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long)gridDim.x * blockDim.x)
+    v = fmax(v, x[c]);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, kWave));
+  if (threadIdx.x % kWave == 0) part[threadIdx.x / kWave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = 0.0;
+    for (int w = 0; w < (int)(blockDim.x / kWave); ++w) m = fmax(m, part[w]);
+    out[blockIdx.x] = m;
+  }
+}
+
+// f /= *den on every padded cell
+__global__ void __launch_bounds__(kLsBlock) k_div_by(double *__restrict__ f, long n, const double *__restrict__ den) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < n) f[c] = f[c] / *den;
+}
+
+extern "C" int scihpc_ls_stabilize_gradient(scihpc_ls *h, double *grad, double *max_grad) {
+  return guarded([&] {
+    ls_check(h);
+    h->ensure_iface();
+    const long n = h->g.total;
+    const unsigned nb = std::min<unsigned>(ceil_div(n, kLsBlock), 1024);
+    double *f = h->phi[h->cur];
+    if (h->g.ndim == 1) k_godunov<1><<<ceil_div(n, kLsBlock), kLsBlock, 0, h->stream>>>(h->g, f, h->sgn, h->grd);
+    else if (h->g.ndim == 2) k_godunov<2><<<ceil_div(n, kLsBlock), kLsBlock, 0, h->stream>>>(h->g, f, h->sgn, h->grd);
+    else k_godunov<3><<<ceil_div(n, kLsBlock), kLsBlock, 0, h->stream>>>(h->g, f, h->sgn, h->grd);
+    SCIHPC_CHECK_LAUNCH();
+    // maxima into the stage scratch s1: nb block maxima, then one
+    k_block_max<<<nb, kLsBlock, 0, h->stream>>>(h->grd, n, h->s1);
+    SCIHPC_CHECK_LAUNCH();
+    k_block_max<<<1, kLsBlock, 0, h->stream>>>(h->s1, (long)nb, h->s1 + nb);
+    SCIHPC_CHECK_LAUNCH();
+    k_div_by<<<ceil_div(n, kLsBlock), kLsBlock, 0, h->stream>>>(f, n, h->s1 + nb);
+    SCIHPC_CHECK_LAUNCH();
+    if (grad) d2h(h, grad, h->grd);
+    if (max_grad) SCIHPC_HIP(hipMemcpyAsync(max_grad, h->s1 + nb, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    SCIHPC_HIP(hipStreamSynchronize(h->stream));
+  });
+}
+
 extern "C" int scihpc_ls_redistance(scihpc_ls *h, double dt, double rdt, int nsteps) {
   return guarded([&] {
     ls_check(h);
@@ -1892,7 +1941,10 @@ struct scihpc_proj {
 
   // find_source_sec + find_intermediate_velocity on the device-resident
   // phi, u, v, w, nvel and sources; rho <- find_density
-  void momentum(const FlowParams &fp) {
+  // find_source_sec (projection_method.cpp:314-382): the new sources of
+  // every component; the handle's sources rotate (src_alt <- old, src <-
+  // new) and the velocity is not touched
+  void sources(const FlowParams &fp) {
     const unsigned nb = ceil_div(g.total, kLsBlock);
     const bool st = fp.we > 0.0;
 #define SCIHPC_PROPS(D, S) \
@@ -1921,16 +1973,28 @@ struct scihpc_proj {
 #undef SCIHPC_SRC
       SCIHPC_CHECK_LAUNCH();
     }
-    // components beyond ndim keep their source (2-D: w_src is never written)
+    for (int a = 0; a < g.ndim; ++a) std::swap(src[a], src_alt[a]);
+  }
+
+  // find_intermediate_velocity (projection_method.cpp:384-402): AB2 with the
+  // current and previous sources, then the velocity BCs; components beyond
+  // ndim keep their source (2-D: w_src is never written), so both terms of
+  // their update read it
+  void ab2(const FlowParams &fp) {
+    const unsigned nb = ceil_div(g.total, kLsBlock);
     const double *sn[3], *so[3];
     for (int a = 0; a < 3; ++a) {
-      so[a] = src[a];
-      sn[a] = a < g.ndim ? src_alt[a] : src[a];
+      sn[a] = src[a];
+      so[a] = a < g.ndim ? src_alt[a] : src[a];
     }
     k_ab2<<<nb, kLsBlock, 0, stream>>>(g.total, fp.dt, u, v, w, sn[0], sn[1], sn[2], so[0], so[1], so[2]);
     SCIHPC_CHECK_LAUNCH();
-    for (int a = 0; a < g.ndim; ++a) std::swap(src[a], src_alt[a]);
     bc3(u, v, w, vbc, 0, 1, 2);
+  }
+
+  void momentum(const FlowParams &fp) {
+    sources(fp);
+    ab2(fp);
   }
 
   // node_from_face + nvel->apply_nvel_bc (wrapper_func.cpp:264-293,
@@ -2138,6 +2202,46 @@ extern "C" int scihpc_proj_momentum_step(scihpc_proj *h, const scihpc_flow_param
     h->d2h(nv, h->nvel[1]);
     h->d2h(nw, h->nvel[2]);
     if (rho) h->d2h(rho, h->rho);
+    SCIHPC_HIP(hipStreamSynchronize(h->stream));
+  });
+}
+
+extern "C" int scihpc_proj_find_sources(scihpc_proj *h, const scihpc_flow_params *prm, const double *phi,
+                                        const double *u, const double *v, const double *w, double *nu, double *nv,
+                                        double *nw, double *rho) {
+  return guarded([&] {
+    SCIHPC_REQUIRE(h && phi && u && v && w && nu && nv && nw, "null argument");
+    const FlowParams fp = flow_params(prm);
+    h->ensure_momentum();
+    h->h2d(h->phi, phi);
+    h->h2d(h->u, u);
+    h->h2d(h->v, v);
+    h->h2d(h->w, w);
+    h->h2d(h->nvel[0], nu);
+    h->h2d(h->nvel[1], nv);
+    h->h2d(h->nvel[2], nw);
+    h->sources(fp);
+    h->d2h(nu, h->nvel[0]);
+    h->d2h(nv, h->nvel[1]);
+    h->d2h(nw, h->nvel[2]);
+    if (rho) h->d2h(rho, h->rho);
+    SCIHPC_HIP(hipStreamSynchronize(h->stream));
+  });
+}
+
+extern "C" int scihpc_proj_intermediate_velocity(scihpc_proj *h, const scihpc_flow_params *prm, double *u,
+                                                 double *v, double *w) {
+  return guarded([&] {
+    SCIHPC_REQUIRE(h && u && v && w, "null argument");
+    const FlowParams fp = flow_params(prm);
+    h->ensure_momentum();
+    h->h2d(h->u, u);
+    h->h2d(h->v, v);
+    h->h2d(h->w, w);
+    h->ab2(fp);
+    h->d2h(u, h->u);
+    h->d2h(v, h->v);
+    h->d2h(w, h->w);
     SCIHPC_HIP(hipStreamSynchronize(h->stream));
   });
 }

@@ -1,159 +1,218 @@
-// dambreak_demo.cpp -- the 2-D dam break of SciHPC's
-// benchmarks_2d_dambreak.cpp written against include/scihpc_facade.hpp, so
-// every step runs on the GPU: level-set transport (TVD-RK3), the momentum
-// predictor, the pressure step (projection() for the first 5 steps, then the
-// FPC step) and node velocities, with legacy-VTK snapshots.
+// dambreak_demo.cpp -- SciHPC's 2-D dam break (benchmarks_2d_dambreak.cpp)
+// written only against SciHPC's own driver API (include/scihpc/*.h, served
+// by the MI355X library): the same objects, the same calls in the same
+// order -- wrapper, runge_kutta::tvd_rk3 with convection, euler with mpls,
+// projection_method::ab_solve_sec, the redistancing every 10 steps,
+// vtkWriter -- with the grid, the box, the water column, the time step and
+// the snapshots taken from the command line:
 //
-// As the reference driver: the level set is advected with the UCCD
-// convection term, corrected by 5 mpls Euler steps per time step, and
-// redistanced every 10 steps (lsf_redistance_lambda with the sign found at
-// the start, as the reference's loop leaves it).  One difference: the level
-// set starts as the exact signed distance to the water column (the
-// reference redistances a +-1 indicator first).
+//   dambreak_demo <nx> <ny> <lx> <ly> <col_w> <col_h> <steps> <cfl> <init>
+//                 [vtk prefix] [snapshot every k steps]
 //
-//   dambreak_demo <n> <steps> [vtk prefix] [plain]   (n x n cells on [0,5]^2)
-// `plain` (any 4th argument) turns the interface maintenance off: WENO5
-// convection only.
-// prints key/value lines: the time, the water volume, the largest
-// velocity, the divergence after the pressure step and the front position.
+// init "indicator": phi = +-1 in/out of the column, then find_sign,
+// stabilized_upon_gradient and redistancing until l2norm < 1e-6 (the
+// reference's initialisation); init "distance": the exact signed distance
+// to the column's free faces.  dt = cfl * h (min_CFL = max_CFL = cfl, as the
+// reference sets 0.01).  Every k steps (k > 0) a snapshot <prefix><id>.vtk
+// holds phi, pressure, density, the pressure-step right-hand side
+// rhs = div(u*) / dt of the intermediate velocity, and nvel -- the (rhs,
+// rho, p) triples the learned preconditioner trains on.  Prints key/value
+// lines.
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
+#include <string>
 
-#include "scihpc_facade.hpp"
-
-using namespace scihpc_gpu;
+#include "scihpc/bc_factory.h"
+#include "scihpc/flux.h"
+#include "scihpc/godunov_gradient.h"
+#include "scihpc/projection_method.h"
+#include "scihpc/runge_kutta.h"
+#include "scihpc/structured_grid.h"
+#include "scihpc/vtkWriter.h"
+#include "scihpc/wrapper.h"
+#include "scihpc/wrapper_func.h"
 
 int main(int argc, char **argv) {
-  const int n = argc > 1 ? std::atoi(argv[1]) : 64;
-  const int steps = argc > 2 ? std::atoi(argv[2]) : 100;
-  const char *prefix = argc > 3 && argv[3][0] ? argv[3] : nullptr;
-  const bool maintain = argc <= 4;
+  if (argc < 10) {
+    std::fprintf(stderr, "usage: %s nx ny lx ly col_w col_h steps cfl indicator|distance [prefix] [every]\n", argv[0]);
+    return 2;
+  }
+  const int nx = std::atoi(argv[1]), ny = std::atoi(argv[2]);
+  const double lx = std::atof(argv[3]), ly = std::atof(argv[4]);
+  const double cw = std::atof(argv[5]), ch = std::atof(argv[6]);
+  const int steps = std::atoi(argv[7]);
+  const double cfl = std::atof(argv[8]);
+  const bool indicator = std::strcmp(argv[9], "indicator") == 0;
+  const char *prefix = argc > 10 && argv[10][0] ? argv[10] : nullptr;
+  const int every = argc > 11 ? std::atoi(argv[11]) : 0;
 
-  structured_grid geo(axis{0.0, 5.0, n}, axis{0.0, 5.0, n});
-  scalar_data phi(n, n), pressure(n, n), density(n, n);
-  vector_data vel(n, n), nvel(n, n);
-  const bc_info walls[6] = {{NO_SLIP}, {NO_SLIP}, {NO_SLIP}, {NO_SLIP}, {NO_SLIP}, {NO_SLIP}};
-  const bc_info neumann[6] = {{NEUMANN}, {NEUMANN}, {NEUMANN}, {NEUMANN}, {NEUMANN}, {NEUMANN}};
+  auto geo = structured_grid(axis{0.0, lx, nx}, axis{0.0, ly, ny});
 
-  // water column [0,1] x [0,1]: signed distance, positive inside
-  for (int I = 0; I < phi.Nx; ++I)
-    for (int J = 0; J < phi.Ny; ++J) {
-      const double x = geo.dx * (I - phi.ghc + 0.5), y = geo.dy * (J - phi.ghc + 0.5);
-      const double dx = x - 1.0, dy = y - 1.0;
+  auto phi = wrapper(true, &geo, bc_info{NEUMANN}, bc_info{NEUMANN}, bc_info{NEUMANN}, bc_info{NEUMANN});
+  auto vel = wrapper(false, &geo, bc_info{NO_SLIP}, bc_info{NO_SLIP}, bc_info{NO_SLIP}, bc_info{NO_SLIP});
+  auto nvel = wrapper(false, &geo, bc_info{NO_SLIP}, bc_info{NO_SLIP}, bc_info{NO_SLIP}, bc_info{NO_SLIP});
+  auto pressure = wrapper(true, &geo, bc_info{NEUMANN}, bc_info{NEUMANN}, bc_info{NEUMANN}, bc_info{NEUMANN});
+
+  auto solver = runge_kutta(phi.scalar->Nx, phi.scalar->Ny, phi.scalar->Nz);
+  auto flow_solver = projection_method(phi.scalar);
+
+  auto param = set_air_water(0.0571);
+  auto deri_solvers = SharedSolvers_alloc(phi.scalar, &geo);
+  shared_solvers_mg_init_Neumann(deri_solvers);
+  auto dummy = dummy_data_alloc(phi.scalar);
+
+  for (int i = 0; i < phi.scalar->nx; ++i) {
+    for (int j = 0; j < phi.scalar->ny; ++j) {
+      auto index = phi.scalar->index_mapping(i + 1, j + 1, 1);
+      const double x = geo.xc[i], y = geo.yc[j];
       double d;
-      if (dx <= 0 && dy <= 0) d = -std::fmax(dx, dy);
-      else d = -std::hypot(std::fmax(dx, 0.0), std::fmax(dy, 0.0));
-      phi.data[I][J][0] = d;
+      if (indicator) {
+        d = (x < cw && y < ch) ? 1.0 : -1.0;
+      } else {
+        // signed distance to the free faces x = cw (and y = ch below the lid)
+        const double ex = x - cw, ey = ch < ly ? y - ch : -1e30;
+        if (ex <= 0 && ey <= 0) d = -std::fmax(ex, ey);
+        else d = -std::hypot(std::fmax(ex, 0.0), std::fmax(ey, 0.0));
+      }
+      phi.scalar->data[index.i][index.j][index.k] = d;
     }
+  }
+  phi.apply_scalar_bc();
 
-  // set_air_water(0.0571) (global.cpp:30-49, :73-76) and the driver's settings
-  const double rho1 = 1000.0, rho2 = 1.226, mu1 = 1.137e-3, mu2 = 1.78e-5, g = 9.81, L = 0.0571;
-  const double Uc = std::sqrt(g * L);
-  problem_parameters prm;
-  prm.density_ratio = rho2 / rho1;
-  prm.viscosity_ratio = mu2 / mu1;
-  prm.Reynolds_number = rho1 * Uc * L / mu1;
-  prm.Froude_number = Uc / std::sqrt(g * L);
-  prm.Weber_number = -1.0;
-  prm.ls_width = 1.5 * geo.h;
-  prm.ppe_tol = 1e-8;
-  prm.ppe_initer = 1;
-  const double dt = 0.01 * geo.h;  // find_dt with min_CFL = max_CFL = 0.01
-  prm.dt = prm.dt_old = prm.dt_old2 = dt;
+  for (wrapper *w : {&phi, &pressure, &vel, &nvel}) {
+    w->link_params(param);
+    w->link_solvers(deri_solvers);
+    w->link_dummy(dummy);
+  }
 
-  runge_kutta rk(&phi, &geo, neumann);
-  if (maintain) rk.set_scheme(SCIHPC_LS_UCCD);
-  const double rdt = 0.25 * geo.h;  // param->rdt of the driver
-  projection_method flow(&pressure, &geo, walls);
-  multigrid mg_const(&pressure, &geo);  // FPC: constant-coefficient Neumann hierarchy
-  multigrid mg_rho(&pressure, &geo);    // projection(): A(rho)
-  vtkWriter vtk(&geo, prefix ? prefix : "dambreak");
+  param->ls_width = 1.5 * geo.h;
+  param->rdt = 0.25 * geo.h;
+  param->max_CFL = cfl;
+  param->min_CFL = cfl;
+  param->Weber_number = -1.0;
+  param->ppe_tol = 1e-8;
+  param->ppe_initer = 1;
 
-  auto volume = [&]() {
-    double s = 0.0;
-    for (int i = 0; i < n; ++i)
-      for (int j = 0; j < n; ++j) {
-        const indices q = phi.index_mapping(i + 1, j + 1, 1);
-        s += phi.data[q.i][q.j][0] > 0.0 ? geo.dv : 0.0;
-      }
-    return s;
-  };
-  auto max_div = [&]() {
-    double m = 0.0;
-    for (int i = 0; i < n; ++i)
-      for (int j = 0; j < n; ++j) {
-        const indices q = phi.index_mapping(i + 1, j + 1, 1);
-        const double d = (vel.x->data[q.i][q.j][0] - vel.x->data[q.i - 1][q.j][0]) / geo.dx +
-                         (vel.y->data[q.i][q.j][0] - vel.y->data[q.i][q.j - 1][0]) / geo.dy;
-        m = std::fmax(m, std::fabs(d));
-      }
-    return m;
+  std::printf("reynolds %.6e\nfroude %.6e\n", param->Reynolds_number, param->Froude_number);
+
+  // the reference's initialisation of phi (benchmarks_2d_dambreak.cpp:77-83)
+  int step = 0;
+  find_sign(&phi);
+  if (indicator) {
+    stabilized_upon_gradient(&phi);
+    do {
+      store_tmp(&phi);
+      solver.tvd_rk3(&phi, &vel, identity_flux, lsf_redistance_lambda);
+    } while (++step * param->rdt < 5.0 and l2norm(&phi) > 1e-6);
+  }
+  std::printf("init_redistance_steps %d\n", step);
+
+  flow_solver.find_source(&vel, &nvel, &phi);
+  param->lsf_mass0 = lsf_mass(&phi);
+
+  // snapshot buffers: density and the pressure-step right-hand side
+  scalar_data rhs(nx, ny);
+  auto snapshot = [&](vtkWriter &vtk, unsigned id) {
+    vtk.create(id);
+    vtk.add_scalar(phi.scalar, "phi");
+    vtk.add_scalar(pressure.scalar, "pressure");
+    vtk.add_scalar(dummy->density, phi.scalar, "density");
+    vtk.add_scalar(&rhs, "rhs");
+    vtk.add_vector(nvel.vector, "nvel");
+    vtk.close();
   };
   auto front = [&]() {  // right-most wet cell centre on the bottom row
-    const int j = 0;
     double f = 0.0;
-    for (int i = 0; i < n; ++i) {
-      const indices q = phi.index_mapping(i + 1, j + 1, 1);
-      if (phi.data[q.i][q.j][0] > 0.0) f = geo.xc[i];
+    for (int i = 0; i < nx; ++i) {
+      const indices q = phi.scalar->index_mapping(i + 1, 1, 1);
+      if (phi.scalar->data[q.i][q.j][0] > 0.0) f = geo.xc[i];
     }
     return f;
   };
   auto max_speed = [&]() {
     double m = 0.0;
-    for (int i = 0; i < n; ++i)
-      for (int j = 0; j < n; ++j) {
-        const indices q = phi.index_mapping(i + 1, j + 1, 1);
-        m = std::fmax(m, std::hypot(nvel.x->data[q.i][q.j][0], nvel.y->data[q.i][q.j][0]));
+    for (int i = 0; i < nx; ++i)
+      for (int j = 0; j < ny; ++j) {
+        const indices q = phi.scalar->index_mapping(i + 1, j + 1, 1);
+        m = std::fmax(m, std::hypot(nvel.vector->x.data[q.i][q.j][0], nvel.vector->y.data[q.i][q.j][0]));
       }
     return m;
   };
+  // rhs = div(u*) / dt on the interior (projection_method.cpp:546-568)
+  auto record_rhs = [&]() {
+    for (int i = 0; i < nx; ++i)
+      for (int j = 0; j < ny; ++j) {
+        const indices q = phi.scalar->index_mapping(i + 1, j + 1, 1);
+        const auto &u = vel.vector->x.data, &v = vel.vector->y.data;
+        rhs.data[q.i][q.j][0] = ((u[q.i][q.j][0] - u[q.i - 1][q.j][0]) / geo.dx +
+                                 (v[q.i][q.j][0] - v[q.i][q.j - 1][0]) / geo.dy) / param->dt;
+      }
+  };
 
-  const double v0 = volume(), f0 = front();
-  const double mass0 = maintain ? rk.lsf_mass(prm.ls_width, prm.density_ratio) : 0.0;
-  if (maintain) rk.find_sign(prm.ls_width);
-  std::printf("volume0 %.6e\n", v0);
-  std::printf("front0 %.6e\n", f0);
+  std::unique_ptr<vtkWriter> vtk;
   if (prefix) {
-    vtk.create(0);
-    vtk.add_scalar(&phi, "phi");
-    vtk.add_scalar(&pressure, "pressure");
-    vtk.add_vector(&nvel, "nvel");
-    vtk.close();
+    vtk.reset(new vtkWriter(&geo, prefix));
+    snapshot(*vtk, 0);
   }
-  double t = 0.0;
-  int cycles = 0;
+  std::printf("front0 %.6e\n", front());
+  std::printf("mass0 %.6e\n", param->lsf_mass0);
+
+  const auto t0 = std::chrono::steady_clock::now();
+  unsigned snap_id = 1;
+  int max_cycles = 0;
+  double max_div = 0.0;
   for (int s = 1; s <= steps; ++s) {
-    t += dt;
-    prm.iter = s;
-    rk.tvd_rk3(&nvel, dt, 1);
-    if (maintain) rk.euler_mpls(dt, prm.ls_width, prm.density_ratio, mass0, 5);
-    cycles = flow.ab_solve_sec(mg_const, mg_rho, &vel, &nvel, &pressure, &phi, &density, prm);
-    if (maintain && s % 10 == 0) {
-      // the reference's redistancing loop (benchmarks_2d_dambreak.cpp:125-131):
-      // delta as find_source_sec left it, the initial sign
-      rk.find_delta(prm.ls_width);
-      int instep = 0;
-      int n_re = 0;
-      while (++instep * rdt < 5.0 * prm.ls_width) ++n_re;
-      rk.tvd_rk3_redistance(dt, rdt, n_re);
+    find_dt(&vel);
+    param->t += param->dt;
+    param->iter++;
+
+    solver.tvd_rk3(&phi, &nvel, &identity_flux, &convection);
+    for (int cnt = 0; cnt < 5; ++cnt) solver.euler(&phi, &nvel, &identity_flux, &mpls);
+
+    const bool snap = prefix && every > 0 && param->iter % every == 0;
+    if (snap) {
+      // ab_solve_sec in its parts, to record the right-hand side the
+      // pressure step sees (projection_method.cpp:779-800)
+      pressure.scalar->store();
+      flow_solver.find_source_sec(&vel, &nvel, &phi);
+      flow_solver.find_intermediate_velocity(&vel);
+      record_rhs();
+      for (int cnt = 0; cnt < param->ppe_initer; ++cnt) {
+        if (param->iter < 5) flow_solver.projection(&pressure, &phi, &vel);
+        else flow_solver.fast_pressure_correction(&pressure, &phi, &vel);
+      }
+      flow_solver.node_from_face(&vel, &nvel);
+    } else {
+      flow_solver.ab_solve_sec(&vel, &nvel, &pressure, &phi);
     }
+    max_cycles = std::max(max_cycles, flow_solver.last_cycles);
+    max_div = std::fmax(max_div, divergence(&vel));
+
+    if (param->iter % 10 == 0) {
+      int instep = 0;
+      while (++instep * param->rdt < 5.0 * param->ls_width) {
+        if (instep == 0) find_sign(&phi);  // never reached, as in the reference
+        solver.tvd_rk3(&phi, &nvel, identity_flux, lsf_redistance_lambda);
+      }
+    }
+    if (snap) snapshot(*vtk, snap_id++);
   }
-  if (maintain) std::printf("mass_ratio %.6e\n", rk.lsf_mass(prm.ls_width, prm.density_ratio) / mass0);
-  std::printf("time %.6e\n", t);
+  const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+
+  std::printf("time %.6e\n", param->t);
   std::printf("steps %d\n", steps);
-  std::printf("last_pressure_cycles %d\n", cycles);
-  std::printf("volume_ratio %.6e\n", volume() / v0);
-  std::printf("max_divergence %.6e\n", max_div());
+  std::printf("dt %.6e\n", param->dt);
+  std::printf("max_pressure_cycles %d\n", max_cycles);
+  std::printf("last_pressure_cycles %d\n", flow_solver.last_cycles);
+  std::printf("mass_ratio %.6e\n", lsf_mass(&phi) / param->lsf_mass0);
+  std::printf("max_divergence %.6e\n", max_div);
+  std::printf("final_divergence %.6e\n", divergence(&vel));
   std::printf("max_speed %.6e\n", max_speed());
   std::printf("front %.6e\n", front());
-  if (prefix) {
-    vtk.create(1);
-    vtk.add_scalar(&phi, "phi");
-    vtk.add_scalar(&pressure, "pressure");
-    vtk.add_vector(&nvel, "nvel");
-    vtk.close();
-    std::printf("vtk_file %s1.vtk\n", prefix);
-  }
+  std::printf("snapshots %u\n", prefix ? snap_id : 0u);
+  std::printf("seconds %.3f\n", secs);
   return 0;
 }

@@ -329,6 +329,10 @@ def interface_cases(ref):
         a = 1.3 * phi
         ref.lib.ref_redistance_rk3(ndim, *n, ext, dt, 0.25 * h[0], sign, delta, types, vals, a, 3)
         d.update(redist_rdt=0.25 * h[0], redist_phi=a, dt=dt, ls_width=lw, ratio=ratio)
+        a = 1.7 * phi
+        grad = np.zeros(T)
+        ref.lib.ref_stabilize_gradient(ndim, *n, ext, sign, a, grad)
+        d.update(stab_phi=a, stab_grad=grad)
         out[f"iface{ndim}d"] = d
     return out
 

@@ -86,6 +86,22 @@ def test_sign_and_redistance_match_reference(case):
     assert rel_max(ls.phi().ravel(), c["redist_phi"]) <= 1e-12
 
 
+@pytest.mark.parametrize("case", sorted(IFACE), ids=str)
+def test_stabilize_gradient_matches_reference(case):
+    """stabilized_upon_gradient: the Godunov gradient with the stored sign
+    and phi divided by its largest value, within the redistancing bar (the
+    reference squares with glibc's pow)."""
+    c = IFACE[case]
+    print(f"stabilize gradient {case}", flush=True)
+    ls, _ = handle(c)
+    ls.set_phi(1.7 * c["phi"])
+    ls.set_field("sign", c["sign"])
+    grad, mx = ls.stabilize_gradient()
+    assert rel_max(grad.ravel(), c["stab_grad"]) <= 1e-12
+    assert abs(mx - np.max(c["stab_grad"])) <= 1e-12 * np.max(c["stab_grad"])
+    assert rel_max(ls.phi().ravel(), c["stab_phi"]) <= 1e-12
+
+
 @pytest.mark.parametrize("ndim,n", [(2, (160, 96, 1)), (3, (40, 32, 36))], ids=["2d", "3d"])
 def test_interface_maintenance_matches_oracle_at_size(oracle, ndim, n):
     """Larger grids than the fixtures (line solves over many lines, several

@@ -291,6 +291,12 @@ def test_interface_maintenance_matches_reference(oracle, case):
     oracle.lib.orc_redistance_rk3(ndim, *n, *h, dt, float(c["redist_rdt"]), c["sign"], c["mpls_delta"], types, vals,
                                   a, 3)
     np.testing.assert_array_equal(a, c["redist_phi"])
+    a = 1.7 * c["phi"]
+    grad = np.zeros(T)
+    mx = oracle.lib.orc_stabilize_gradient(ndim, *n, *h, c["sign"], a, grad)
+    np.testing.assert_array_equal(grad, c["stab_grad"])
+    np.testing.assert_array_equal(a, c["stab_phi"])
+    assert mx == float(np.max(c["stab_grad"]))
 
 
 def _line_field(n, h, f):
