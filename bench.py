@@ -4,7 +4,10 @@ Workload (one "step" = one full pressure solve from p = 0): the 2-D
 variable-density bubble pressure Poisson problem, 512x1024 cells in a 1x2
 box, rho 1000:1 across a circle of radius 0.25 at (0.5, 0.5) (smoothed
 Heaviside, width 1.5h), b = divergence of a seeded smooth velocity field,
-all-Neumann walls.  Inputs are resident in HBM before the timed region.
+all-Neumann walls.  Inputs are resident in HBM before the timed region; the
+timed solves rotate over four such right-hand sides (four seeds), each
+copied device to device into the solver at the start of its solve, as a
+time loop hands every solve a new b.
 
 `value` is the path the metric names: the FP32 V-cycle inside the FP64
 flexible-CG defect correction, with the trained U-Net guess evaluated and
@@ -13,9 +16,11 @@ Beside it: plain FP64 multigrid (stationary V-cycles), FP64 + flexible CG,
 and the mixed path without the network.  The roofline object is the
 dominant kernel timed with HIP events on the solver stream; the U-Net's
 MFMA rate is reported from its own timing hook; the 256^3 variable-density
-3-D solve (1000:1 jump across a wavy plane) is reported as well; and
-cpu_baseline is the oracle's C port of the same FP64 multigrid on one host
-core, on a bounded number of solves.
+3-D solves (the 1000:1 jump across a wavy plane and the constant-coefficient
+Dirichlet problem) and the WENO5 + TVD-RK3 level-set step are reported as
+well; and cpu_baseline is the oracle's C port of the same FP64 multigrid on
+the host's threads and on one thread, beside the reference-faithful FPC
+multigrid path, on a bounded number of solves.
 """
 from __future__ import annotations
 
@@ -46,9 +51,38 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(w, seconds=12.0):
-    """The oracle's FP64 red-black V(2,2) multigrid (C, one core) solving the
-    same system to 1e-8; repeated until `seconds` of CPU work."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_threads():
+    """Host threads for the multi-threaded CPU legs: OMP_NUM_THREADS when set
+    (the GPU box sets it to its CPU share), else the cores this process may
+    use, at most 16."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        return int(env)
+    return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+def cpu_baseline(w, rhs_list, seconds=12.0):
+    """CPU legs beside the GPU solve, on the host of the same run.
+
+    value: the oracle's C port (oracle/oracle.c, g++-compatible gcc -O3
+    -fopenmp) of the same FP64 red-black V(2,2) multigrid on the same A(rho)
+    system, solving the same rotating right-hand sides to 1e-8, with the
+    host's threads; beside it the same on one thread, and the
+    reference-faithful path of SciHPC's fast_pressure_correction
+    (steepest-descent relax, child-average restriction, slope
+    prolongation, direct coarse solve; full cycles until the relative
+    residual is 1e-8) on the constant-coefficient Neumann operator that
+    path solves (it diverges on the 1000:1 A(rho) operator)."""
     import numpy as np
 
     from oracle.pyoracle import Oracle
@@ -56,20 +90,90 @@ def cpu_baseline(w, seconds=12.0):
     nx, ny, _ = w["shape"]
     dx, dy, _ = w["spacing"]
     o = Oracle()
+    nthr = cpu_threads()
+
+    def rb_leg(threads, budget):
+        used = o.lib.orc_set_threads(threads)
+        m = o.mg(2, nx, ny, 1, dx, dy, 1.0, "neumann")
+        m.set_rho(w["rho"])
+        times, cycles = [], 0
+        t_start = time.perf_counter()
+        k = 0
+        while time.perf_counter() - t_start < budget or not times:
+            m.vec(0, "rhs")[:] = rhs_list[k % len(rhs_list)]
+            m.vec(0, "sol")[:] = 0.0
+            t0 = time.perf_counter()
+            cycles, _ = m.solve_rb(1e-8, 200, 2, 2)
+            times.append(time.perf_counter() - t0)
+            k += 1
+        return used, 1e3 * float(np.mean(times)), len(times), cycles
+
+    used, ms_n, cnt_n, cyc = rb_leg(nthr, 0.4 * seconds)
+    _, ms_1, cnt_1, cyc1 = rb_leg(1, 0.4 * seconds)
+    # reference-faithful FPC-mode solve (constant-coefficient Neumann)
+    o.lib.orc_set_threads(nthr)
     m = o.mg(2, nx, ny, 1, dx, dy, 1.0, "neumann")
-    m.set_rho(w["rho"])
-    times, cycles = [], 0
-    t_start = time.perf_counter()
-    while time.perf_counter() - t_start < seconds or not times:
-        m.vec(0, "rhs")[:] = w["rhs"]
-        m.vec(0, "sol")[:] = 0.0
-        t0 = time.perf_counter()
-        cycles, _ = m.solve_rb(1e-8, 200, 2, 2)
-        times.append(time.perf_counter() - t0)
-    ms = 1e3 * float(np.mean(times))
-    return {"value": round(ms, 2), "unit": "ms/solve", "cores": 1, "kind": "port",
-            "sample": f"{len(times)} full solves of the 512x1024 bubble system to 1e-8 "
-                      f"({cycles} V(2,2) cycles each) by oracle/oracle.c, 1 thread"}
+    m.vec(0, "rhs")[:] = rhs_list[0]
+    m.vec(0, "sol")[:] = 0.0
+    bn = float(np.linalg.norm(rhs_list[0]))
+    t0 = time.perf_counter()
+    ref_cycles = 0
+    while o.lib.orc_mg_residual(m.h, 0) / bn > 1e-8 and ref_cycles < 50:
+        o.lib.orc_mg_fullcycle_ref(m.h)
+        ref_cycles += 1
+    ms_ref = (time.perf_counter() - t0) * 1e3
+    ref_rr = o.lib.orc_mg_residual(m.h, 0) / bn
+    o.lib.orc_set_threads(1)
+    return {"value": round(ms_n, 2), "unit": "ms/solve", "cores": used, "kind": "port",
+            "cpu_model": cpu_model(), "threads": used,
+            "sample": f"{cnt_n} full solves of the 512x1024 bubble A(rho) system to 1e-8 ({cyc} V(2,2) cycles "
+                      f"each), rotating over {len(rhs_list)} right-hand sides, by oracle/oracle.c "
+                      f"(gcc -O3 -fopenmp, {used} threads)",
+            "one_thread": {"ms_per_solve": round(ms_1, 2), "solves": cnt_1, "cycles": cyc1},
+            "reference_faithful_fpc": {
+                "ms_per_solve": round(ms_ref, 1), "full_cycles": ref_cycles, "rel_residual": ref_rr,
+                "threads": used,
+                "path": "steepest-descent V(10,10) full cycles (multigrid.cpp:57-79, multigrid_base.cpp:16-234) "
+                        "on the constant-coefficient Neumann operator of the FPC solve, same b, to 1e-8"}}
+
+
+def bench_weno(reps=10):
+    """WENO5-Z level-set TVD-RK3 step (weno_solver.cpp, runge_kutta.cpp:13-49,
+    source.cpp:8-40 with the WENO call) at the 1024x512 dam-break shape:
+    device time per step on a resident level set against the minimal
+    Shu-Osher traffic (7 + 3d) * 8 B per cell."""
+    from scihpc_amd import LevelSet
+
+    wl = momentum_workload()
+    nx, ny = wl["shape"]
+    h = wl["h"]
+    ls = LevelSet(2, nx, ny, 1, h, h, 1.0, ["neumann"] * 6)
+    ls.set_phi(wl["phi"])
+    ls.set_velocity(wl["u"], wl["v"])
+    ms, byts = ls.time_step(0.25 * h, iters=reps)
+    ls.close()
+    gbs = byts / (ms * 1e-3) / 1e9
+    return {"workload": f"dam break {nx}x{ny} level set, WENO5-Z + TVD-RK3", "ms_per_step": round(ms, 4),
+            "bytes_per_step": byts, "GBps": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+
+
+def cpu_weno_baseline():
+    """The oracle's WENO5 TVD-RK3 step (one core) at the same shape."""
+    import numpy as np
+
+    from oracle.pyoracle import Oracle
+
+    wl = momentum_workload()
+    nx, ny = wl["shape"]
+    h = wl["h"]
+    o = Oracle()
+    o.lib.orc_set_threads(1)
+    phi = wl["phi"].copy()
+    t0 = time.perf_counter()
+    o.tvd_rk3(2, (nx, ny, 1), (h, h, 1.0), 0.25 * h, ["neumann"] * 4, [0.0] * 4, phi, wl["u"], wl["v"],
+              np.zeros_like(phi), 1)
+    return {"ms_per_step": round((time.perf_counter() - t0) * 1e3, 2), "cores": 1, "kind": "port",
+            "sample": f"one step at {nx}x{ny}"}
 
 
 def kernel_only(args):
@@ -151,6 +255,35 @@ def bench_3d(args):
                            "256 MB last-level cache between launches (fp64_prolong_add exceeds the device-copy "
                            "ceiling this way); inside a solve the arrays are evicted between uses")
     g.close()
+    del g
+    # the constant-coefficient Dirichlet half of the configuration: f from
+    # seeded random Fourier modes (k <= 8, N(0,1)/|k|^2)
+    log("bench: building the 256^3 Dirichlet workload")
+    wd = W.poisson_3d(n, variable=False)
+    gd = Multigrid(3, n, n, n, 1 / n, 1 / n, 1 / n, "dirichlet")
+    gd.set_rhs(wd["rhs"])
+    dres = {"workload": "poisson_3d 256^3 constant-coefficient Dirichlet, random Fourier f", "rel_tol": 1e-8}
+    xd = {}
+    for name, prec, acc in [("fp64_mg", PREC_FP64, 0), ("fp64_mg_fcg", PREC_FP64, 1), ("mixed", PREC_MIXED, 1)]:
+        gd.solve(rel_tol=1e-8, max_cycles=100, precision=prec, accel=acc)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        reps = 3
+        for _ in range(reps):
+            st = gd.solve(rel_tol=1e-8, max_cycles=100, precision=prec, accel=acc)
+        torch.cuda.synchronize()
+        ms = 1e3 * (time.perf_counter() - t0) / reps
+        assert st["converged"], st
+        xd[name] = gd.sol(0)
+        dres[name] = {"ms_per_solve": round(ms, 3), "cycles": st["cycles"], "rel_residual": st["rel_residual"]}
+        log(f"bench: 3-D Dirichlet {name} {ms:.2f} ms/solve, {st['cycles']} cycles")
+    gd.solve(rel_tol=1e-12, max_cycles=100, precision=PREC_FP64, accel=1)
+    x_ref = gd.sol(0)
+    for name in xd:
+        dres[name]["rel_l2_vs_fp64_1e-12"] = float(np.linalg.norm(xd[name] - x_ref) / np.linalg.norm(x_ref))
+    dres["speedup_mixed_vs_fp64"] = round(dres["fp64_mg"]["ms_per_solve"] / dres["mixed"]["ms_per_solve"], 3)
+    gd.close()
+    res["dirichlet_fourier"] = dres
     return res
 
 
@@ -200,6 +333,7 @@ def bench_snapshot(torch, nx=512, ny=1024, reps=5):
 
 
 MOM_SHAPE = (1024, 512)
+N_RHS = 4  # right-hand sides the timed 2-D solves rotate over
 
 
 def momentum_workload(nx=MOM_SHAPE[0], ny=MOM_SHAPE[1]):
@@ -398,6 +532,7 @@ def main():
     ap.add_argument("--no-snapshot", action="store_true", help="skip the VTK snapshot timing")
     ap.add_argument("--no-momentum", action="store_true", help="skip the momentum predictor timing")
     ap.add_argument("--no-interface", action="store_true", help="skip the interface-maintenance timing")
+    ap.add_argument("--no-weno", action="store_true", help="skip the WENO5 TVD-RK3 timing")
     ap.add_argument("--only-rows", action="store_true",
                     help="run only the rows around the solve (momentum predictor, interface maintenance, "
                          "VTK snapshot) and print their JSON (profiling)")
@@ -429,7 +564,8 @@ def main():
         print(json.dumps(out3), flush=True)
         return
     if args.only_rows:
-        rows = {"momentum": bench_momentum(), "interface": bench_interface(), "snapshot": bench_snapshot(torch)}
+        rows = {"momentum": bench_momentum(), "interface": bench_interface(), "weno_rk3": bench_weno(),
+                "snapshot": bench_snapshot(torch)}
         rows["snapshot"].pop("_fields")
         rows["snapshot"].pop("_file")
         print(json.dumps(rows), flush=True)
@@ -442,20 +578,39 @@ def main():
     g.set_density(w["rho"])
     g.set_rhs(w["rhs"])
     tol = 1e-8
+    # the timed solves rotate over right-hand sides of the same bubble
+    # (divergences of four seeded velocity fields), resident on the device:
+    # each solve starts by copying its b into the solver (device to
+    # device), as a time loop hands a new b to every solve, so lagged
+    # convergence checks pay their adaptation to changing iteration counts
+    rhs_host = [w["rhs"]] + [W.bubble_2d(seed=W.SEED + 2 + 100 * k)["rhs"] for k in range(1, N_RHS)]
+    rhs_dev = [torch.from_numpy(r).to("cuda") for r in rhs_host]
+    torch.cuda.synchronize()
+    turn = [0]
+
+    def next_rhs():
+        g.set_rhs_device(rhs_dev[turn[0] % N_RHS].data_ptr())
+        turn[0] += 1
 
     def run(precision, steps, warmup, accel=0, learned=False, lagged=0):
         stats = None
+        turn[0] = 0
         for _ in range(warmup):
+            next_rhs()
             stats = g.solve(rel_tol=tol, max_cycles=200, precision=precision, accel=accel, use_learned=learned,
                             lagged_check=lagged)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
+        cyc = []
         for _ in range(steps):
+            next_rhs()
             stats = g.solve(rel_tol=tol, max_cycles=200, precision=precision, accel=accel, use_learned=learned,
                             lagged_check=lagged)
+            assert stats["converged"], stats
+            cyc.append(stats["cycles"])
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t0) / steps
-        assert stats["converged"], stats
+        stats["cycles_per_solve"] = cyc
         return 1e3 * dt, stats
 
     log("bench: plain FP64 multigrid")
@@ -489,13 +644,16 @@ def main():
         ms_every, st_every = run(PREC_MIXED, args.steps, args.warmup, accel=1, learned=3)
         def backoff(lagged):
             g.attach_learned(net, meta["level"])  # fresh back-off state
+            turn[0] = 0
             for _ in range(args.warmup):
+                next_rhs()
                 g.solve(rel_tol=tol, max_cycles=200, precision=PREC_MIXED, accel=1, use_learned=1,
                         lagged_check=lagged)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             skipped = accepted = 0
             for _ in range(args.steps):
+                next_rhs()
                 st = g.solve(rel_tol=tol, max_cycles=200, precision=PREC_MIXED, accel=1, use_learned=1,
                              lagged_check=lagged)
                 skipped += st["learned_skipped"]
@@ -512,12 +670,15 @@ def main():
         xml = g.sol(0)
         _, st_forced = run(PREC_MIXED, 1, 0, accel=1, learned=2)
         log("bench: mixed + learned coarse-level correction of the defect (FP64-judged)")
+        turn[0] = 0
         for _ in range(args.warmup):
+            next_rhs()
             g.solve(rel_tol=tol, max_cycles=200, precision=PREC_MIXED, accel=1, learned_correction=1)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         corr_kept = 0
         for _ in range(args.steps):
+            next_rhs()
             st_corr = g.solve(rel_tol=tol, max_cycles=200, precision=PREC_MIXED, accel=1, learned_correction=1)
             corr_kept += st_corr["correction_accepted"]
         torch.cuda.synchronize()
@@ -554,20 +715,32 @@ def main():
             f"{st_corr_forced['cycles']} cycles")
         g.attach_learned(None)
 
-    # level-0 kernels, HIP events on the solver's stream
+    # the roofline kernel inside the headline solves: HIP events around every
+    # launch of the FCG update fused into the FP32 level-0 pre-smoothing (two
+    # red-black sweeps + residual + restriction; every iteration after the
+    # first), over a second timed pass of the same rotating solves
+    log("bench: in-solve launches of the roofline kernel (HIP events, second pass)")
+    g.probe_kernel(6, capacity=24 * (args.steps + args.warmup))
+    if learned.get("attached"):
+        g.attach_learned(net, meta["level"])
+        ms_probe, _, _, _ = backoff(1)
+        g.attach_learned(None)
+    else:
+        ms_probe, _ = run(PREC_MIXED, args.steps, args.warmup, accel=1, lagged=1)
+    launches, ev_total_ms, k_total_ms, kbytes = g.probe_result()
+    g.probe_kernel(-1, 0)
+    kms = ev_total_ms / max(launches, 1)       # HIP events around each launch (timing-only events)
+    span_ms = k_total_ms / max(launches, 1)    # its workgroups' first start to last end (device clock)
+    achieved = kbytes / (kms * 1e-3) / 1e9
+    # level-0 kernels relaunched back to back by the timing hook (HIP events)
     kernels = {}
     for which, name in [(0, "fp64_pre_smooth_restrict"), (1, "fp32_pre_smooth_restrict"),
-                        (2, "fp32_post_smooth_fp64_outer"), (3, "fp64_post_smooth_norm"),
-                        (6, "fcg_update_fp32_pre_smooth_restrict")]:
+                        (2, "fp32_post_smooth_fp64_outer"), (3, "fp64_post_smooth_norm")]:
         ms_k, b_k = g.time_kernel(which, iters=50)
         kernels[name] = {"us": round(ms_k * 1e3, 2), "bytes": b_k, "GBps": round(b_k / (ms_k * 1e-3) / 1e9, 1)}
-    # dominant kernel of the headline solve: the FCG update fused into the
-    # FP32 level-0 pre-smoothing (two red-black sweeps + residual +
-    # restriction), which runs on every iteration after the first
     kernels["fp32_pre_smooth_restrict"]["traffic"] = round(PRE2D_FP32_TRAFFIC_BYTES)
-    dom = kernels["fcg_update_fp32_pre_smooth_restrict"]
-    kms, kbytes = dom["us"] * 1e-3, dom["bytes"]
-    achieved = kbytes / (kms * 1e-3) / 1e9
+    kernels["note"] = ("back-to-back relaunches of one kernel over the same arrays (cache-warm); the roofline "
+                       "kernel is timed inside the solves instead")
 
     # U-Net training throughput on the matrix cores (level-1 grid, batch 4)
     unet = None
@@ -604,6 +777,12 @@ def main():
     ceiling = bench_hbm_ceiling(torch)
     log(f"bench: copy {ceiling['copy_GBps']} GB/s, 3R1W {ceiling['three_read_one_write_GBps']} GB/s")
 
+    weno = None
+    if not args.no_weno:
+        log("bench: WENO5 + TVD-RK3 level-set step at the 1024x512 dam-break shape")
+        weno = bench_weno()
+        log(f"bench: weno rk3 {weno['ms_per_step']} ms/step, {weno['GBps']} GB/s")
+
     snapshot = None
     if not args.no_snapshot:
         log("bench: VTK snapshot of the 512x1024 grid")
@@ -635,9 +814,10 @@ def main():
                         "rel_residual": st64k["rel_residual"]},
         "fp64_mg_lagged": {"ms_per_solve": round(ms64L, 4), "cycles": st64L["cycles"],
                            "rel_residual": st64L["rel_residual"], "accel": "stationary V-cycles",
-                           "lagged_check": True},
+                           "lagged_check": True, "cycles_per_solve": st64L["cycles_per_solve"]},
         "mixed_lagged": {"ms_per_solve": round(ms_mxL, 4), "cycles": st_mxL["cycles"],
-                         "rel_residual": st_mxL["rel_residual"], "rel_l2_vs_fp64": errL, "lagged_check": True},
+                         "rel_residual": st_mxL["rel_residual"], "rel_l2_vs_fp64": errL, "lagged_check": True,
+                         "cycles_per_solve": st_mxL["cycles_per_solve"]},
         "mixed": {"ms_per_solve": round(ms_mx, 4), "cycles": st_mx["cycles"],
                   "rel_residual": st_mx["rel_residual"], "rel_l2_vs_fp64": err,
                   "accel": "flexible CG (FCG(1)), FP64 vectors", "learned": False},
@@ -652,20 +832,28 @@ def main():
                      "traffic": round(FCG_PRE2D_TRAFFIC_BYTES),
                      "kernel": "k_pre2d_fcg<true,32,16> level 0 (FCG update of x and r + 2 RB-GS sweeps + "
                                "residual + restriction)",
+                     "timing": f"HIP events (timing-only, no system fence) around each of its {launches} "
+                               f"launches inside the headline solves (second timed pass, {round(ms_probe, 4)} "
+                               f"ms/solve) on the solver stream; us_per_launch_span: first workgroup start to "
+                               f"last workgroup end on the device wall clock",
+                     "us_per_launch_span": round(span_ms * 1e3, 3),
                      "note": "largest HBM-facing kernel; the largest by time is the one-workgroup coarse-level "
-                             "cycle k_block_vcycle<float,2> (about 25 us per FCG iteration, on-chip, one CU)",
+                             "cycle k_block_vcycle<float,2> (on-chip, one CU)",
                      "bytes_per_launch": kbytes, "us_per_launch": round(kms * 1e3, 3),
-                     "measured_ceiling": ceiling},
+                     "launches": launches, "measured_ceiling": ceiling},
         "unet": unet,
         "momentum": momentum,
         "interface": interface,
+        "weno_rk3": weno,
         "snapshot": snapshot,
     }
     if not args.no_3d:
         out["plane_3d_256"] = bench_3d(args)
     if not args.no_cpu_baseline:
         log("bench: cpu baseline (oracle port, 1 core)")
-        out["cpu_baseline"] = cpu_baseline(w, args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(w, rhs_host, args.cpu_seconds)
+        if weno is not None:
+            out["cpu_baseline"]["weno_rk3"] = cpu_weno_baseline()
         if snapshot is not None:
             out["cpu_baseline"]["snapshot_writer"] = cpu_snapshot_baseline(snapshot)
         if momentum is not None:

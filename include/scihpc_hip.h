@@ -94,7 +94,9 @@ SCIHPC_API int scihpc_mg_get_rhs(scihpc_mg *h, int level, double *dst);
 SCIHPC_API int scihpc_mg_set_sol(scihpc_mg *h, int level, const double *src);
 SCIHPC_API int scihpc_mg_get_sol(scihpc_mg *h, int level, double *dst);
 SCIHPC_API int scihpc_mg_get_res(scihpc_mg *h, int level, double *dst);
-/* device-pointer variants for level 0 (d2d copies on the handle stream) */
+/* device-pointer variants for level 0 (d2d copies on the handle stream;
+ * set_rhs_device returns without waiting: the copy is ordered before the
+ * handle's next work, and the source must stay valid until it has run) */
 SCIHPC_API int scihpc_mg_set_rhs_device(scihpc_mg *h, const double *dsrc);
 SCIHPC_API int scihpc_mg_get_sol_device(scihpc_mg *h, double *ddst);
 /* multigrid_base::no_compatibility (multigrid_base.h:33) */
@@ -211,6 +213,20 @@ SCIHPC_API int scihpc_mg_solve(scihpc_mg *h, const scihpc_solve_opts *opts,
  * HBM bytes of one launch. */
 SCIHPC_API int scihpc_mg_time_kernel(scihpc_mg *h, int which, int iters, double *ms,
                           double *bytes);
+
+/* Measurement hook (no reference counterpart): HIP events on the handle
+ * stream around every launch of one kernel inside the following solves, at
+ * most `capacity` launches; which = 6 the mixed flexible-CG update fused
+ * into the FP32 level-0 pre-smoothing (2-D), -1 disarms. */
+SCIHPC_API int scihpc_mg_probe_kernel(scihpc_mg *h, int which, int capacity);
+/* launches recorded since the probe was armed or last read; total_ms: the
+ * summed time between the HIP events around them (each interval also holds
+ * the dispatch of the launch); kernel_ms: the summed spans of the kernels
+ * themselves, earliest workgroup start to latest workgroup end on the
+ * device's wall clock; bytes: the algorithmic HBM bytes of one launch (as
+ * scihpc_mg_time_kernel).  Starts a new count. */
+SCIHPC_API int scihpc_mg_probe_result(scihpc_mg *h, int *launches, double *total_ms, double *kernel_ms,
+                                      double *bytes);
 
 /* ------------------------------------------------------------------ */
 /* Projection step on padded fields (scalar_data layout), around a

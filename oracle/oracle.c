@@ -948,16 +948,26 @@ static double vec_dot(const double *a, const double *b, long n) {
  *           beta = -(z.q_old)/(p_old.q_old); q = A p;
  *           alpha = (p.r)/(p.q); x += alpha p; r -= alpha q
  *   until ||r||/||b|| <= rel_tol (the recurrence residual) or max_iter.
- * beta and alpha are 0 when their denominator is exactly 0 (only after
- * exact convergence).  Neumann: the result is made mean-free.  b is
- * level 0's rhs (restored on return), the result goes to level 0's sol.
- * The build's outer loop (no reference counterpart; the closest analogue is
- * the FPC loop, projection_method.cpp:694-706).                          */
+ * On the singular all-Neumann operator solved below 1e-9, p.q and p.r are
+ * taken with p's constant component removed, (p - mean p).q and
+ * (p - mean p).r (A maps constants to zero; left in, that component grows
+ * at the FP64 floor and its product with the rounding-level sum of r drives
+ * the iteration away); the corrected p.q is also the next beta's
+ * denominator.  When the
+ * recurrence says converged, the true residual b - A x is formed; above
+ * rel_tol with iterations left, the iteration restarts from x with
+ * r = b - A x.  beta and alpha are 0 when their denominator is exactly 0
+ * (only after exact convergence).  hist[k] = the relative residual after
+ * iteration k (the recurrence's).  Neumann: the result is made mean-free.
+ * b is level 0's rhs (restored on return), the result goes to level 0's
+ * sol.  The build's outer loop (no reference counterpart; the closest
+ * analogue is the FPC loop, projection_method.cpp:694-706).             */
 int orc_mg_solve_fcg(void *h, double rel_tol, int max_iter, int nu1, int nu2,
                      double *hist) {
   orc_mg *m = (orc_mg *)h;
   orc_lvl *l = &m->lv[0];
   const long n = l->n;
+  const int corr = m->bc == 0 && rel_tol < 1e-9;
   for (int q = 0; q < 8; q++)
     if (!m->fcg[q]) m->fcg[q] = (double *)calloc(n, sizeof(double));
   double *x = m->fcg[0], *r = m->fcg[1], *p = m->fcg[2], *q = m->fcg[3];
@@ -968,35 +978,53 @@ int orc_mg_solve_fcg(void *h, double rel_tol, int max_iter, int nu1, int nu2,
   double bn = vec_norm(b, n);
   const double scale = bn > 0.0 ? bn : 1.0;
   double rr = bn > 0.0 ? 1.0 : 0.0;
-  int it = 0;
-  while (rr > rel_tol && it < max_iter) {
-    memcpy(l->rhs, r, sizeof(double) * n);
-    memset(l->sol, 0, sizeof(double) * n);
-    orc_mg_vcycle_rb(h, nu1, nu2);
-    memcpy(z, l->sol, sizeof(double) * n);
-    if (it == 0) {
-      memcpy(p, z, sizeof(double) * n);
-    } else {
-      double t;
-      memcpy(po, p, sizeof(double) * n);
-      memcpy(qo, q, sizeof(double) * n);
-      const double pq_old = vec_dot(po, qo, n);
-      const double beta = pq_old != 0.0 ? -vec_dot(z, qo, n) / pq_old : 0.0;
-      for (long c = 0; c < n; c++) {
-        t = z[c] + beta * po[c];
-        p[c] = t;
+  double pq_prev = 0.0;
+  int it = 0, first = 1;
+  for (;;) {
+    while (rr > rel_tol && it < max_iter) {
+      memcpy(l->rhs, r, sizeof(double) * n);
+      memset(l->sol, 0, sizeof(double) * n);
+      orc_mg_vcycle_rb(h, nu1, nu2);
+      memcpy(z, l->sol, sizeof(double) * n);
+      if (first) {
+        memcpy(p, z, sizeof(double) * n);
+      } else {
+        memcpy(po, p, sizeof(double) * n);
+        memcpy(qo, q, sizeof(double) * n);
+        const double beta = pq_prev != 0.0 ? -vec_dot(z, qo, n) / pq_prev : 0.0;
+        for (long c = 0; c < n; c++) p[c] = z[c] + beta * po[c];
       }
+      first = 0;
+      lvl_apply(l, p, q);
+      double pq = vec_dot(p, q, n), pr = vec_dot(p, r, n);
+      if (corr) {
+        double sp = 0.0, sr = 0.0, sq = 0.0;
+        for (long c = 0; c < n; c++) {
+          sp += p[c];
+          sr += r[c];
+          sq += q[c];
+        }
+        pq -= sp * sq / (double)n;
+        pr -= sp * sr / (double)n;
+      }
+      pq_prev = pq;
+      const double alpha = pq != 0.0 ? pr / pq : 0.0;
+      for (long c = 0; c < n; c++) {
+        x[c] += alpha * p[c];
+        r[c] -= alpha * q[c];
+      }
+      rr = vec_norm(r, n) / scale;
+      if (hist) hist[it] = rr;
+      it++;
     }
-    lvl_apply(l, p, q);
-    const double pq = vec_dot(p, q, n);
-    const double alpha = pq != 0.0 ? vec_dot(p, r, n) / pq : 0.0;
-    for (long c = 0; c < n; c++) {
-      x[c] += alpha * p[c];
-      r[c] -= alpha * q[c];
-    }
-    rr = vec_norm(r, n) / scale;
-    if (hist) hist[it] = rr;
-    it++;
+    /* the true residual of x */
+    const int recurrence_converged = rr <= rel_tol;
+    lvl_apply(l, x, z);
+    for (long c = 0; c < n; c++) z[c] = b[c] - z[c];
+    rr = vec_norm(z, n) / scale;
+    if (!recurrence_converged || rr <= rel_tol || it >= max_iter) break;
+    memcpy(r, z, sizeof(double) * n);
+    first = 1;
   }
   memcpy(l->rhs, b, sizeof(double) * n);
   memcpy(l->sol, x, sizeof(double) * n);
@@ -1010,7 +1038,7 @@ int orc_mg_solve_fcg(void *h, double rel_tol, int max_iter, int nu1, int nu2,
 }
 
 /* the flexible-CG vectors after orc_mg_solve_fcg: 0 x (before the mean
- * removal), 1 r, 2 p, 3 q, 4 p_old, 5 q_old, 6 z of the last iteration */
+ * removal), 1 r, 2 p, 3 q, 4 p_old, 5 q_old, 6 b - A x of the final x */
 double *orc_mg_fcg_vec(void *h, int which) {
   orc_mg *m = (orc_mg *)h;
   return (which >= 0 && which < 8) ? m->fcg[which] : NULL;

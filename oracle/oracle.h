@@ -92,16 +92,19 @@ int orc_mg_solve_rb(void *h, double rel_tol, int max_cycles, int nu1, int nu2,
                     double *hist);
 /* flexible CG, FCG(1) (Notay 2000), with one V(nu1,nu2) of
  * orc_mg_vcycle_rb from zero as the preconditioner; b = level 0's rhs, x0 =
- * 0, stops on the recurrence residual ||r||/||b|| <= rel_tol; result in
+ * 0, stops when the recurrence residual ||r||/||b|| <= rel_tol and the true
+ * one b - A x agrees (else it restarts from x); on all-Neumann operators
+ * the step sizes leave out the direction's constant component; result in
  * level 0's sol (mean-free for Neumann); hist[k] = relative residual after
  * iteration k.  Returns iterations.  Build's outer loop (closest analogue:
  * projection_method.cpp:694-706).                                [math]
  * Pinned by the dense solve, the A-conjugacy p.q_old = 0 and the
- * orthogonality p.r = 0 after each step (tests/test_oracle_math.py). */
+ * orthogonality p.r = 0 after each step, and the absence of drift below the
+ * FP64 floor (tests/test_oracle_math.py). */
 int orc_mg_solve_fcg(void *h, double rel_tol, int max_iter, int nu1, int nu2,
                      double *hist);
 /* its level-0 vectors after the solve: 0 x (before the mean removal), 1 r,
- * 2 p, 3 q, 4 p_old, 5 q_old, 6 z of the last iteration              */
+ * 2 p, 3 q, 4 p_old, 5 q_old, 6 b - A x of the final x            */
 double *orc_mg_fcg_vec(void *h, int which);
 /* dense direct solve of level 0 (n <= 4096): Dirichlet -> plain LU;
  * Neumann -> pinned system, mean-free result.                   [math]   */

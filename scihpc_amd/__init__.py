@@ -179,6 +179,20 @@ class Multigrid:
         check(self._lib.scihpc_mg_learned_features(self.h, level, feat, tgt, faces))
         return feat.reshape(ny * nx, 2), tgt, None if faces is None else faces.reshape(ny * nx, 2)
 
+    def probe_kernel(self, which=6, capacity=4096):
+        """Arm HIP events around every launch of kernel `which` inside the
+        following solves (6: the fused FCG update + FP32 level-0
+        pre-smoothing); -1 disarms."""
+        check(self._lib.scihpc_mg_probe_kernel(self.h, which, capacity))
+
+    def probe_result(self):
+        """(launches, summed ms between the HIP events around them, summed
+        ms of the kernels' own wall-clock spans, algorithmic bytes per
+        launch) since the probe was armed or last read."""
+        n, ms, kms, b = C.c_int(), C.c_double(), C.c_double(), C.c_double()
+        check(self._lib.scihpc_mg_probe_result(self.h, C.byref(n), C.byref(ms), C.byref(kms), C.byref(b)))
+        return n.value, ms.value, kms.value, b.value
+
     def time_kernel(self, which, iters=20):
         ms, b = C.c_double(), C.c_double()
         check(self._lib.scihpc_mg_time_kernel(self.h, which, iters, C.byref(ms), C.byref(b)))

@@ -101,6 +101,8 @@ SIGNATURES = {
     "scihpc_mg_vcycle": [_vp, _i, _i],
     "scihpc_mg_solve": [_vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)],
     "scihpc_mg_time_kernel": [_vp, _i, _i, _pd, _pd],
+    "scihpc_mg_probe_kernel": [_vp, _i, _i],
+    "scihpc_mg_probe_result": [_vp, _pi, _pd, _pd, _pd],
     "scihpc_mg_attach_learned": [_vp, _vp, _i],
     "scihpc_mg_learned_features": [_vp, _i, _fp, _fpn, _fpn],
     "scihpc_unet_create": [_i, _i, _i, C.c_ulonglong, C.POINTER(_vp)],

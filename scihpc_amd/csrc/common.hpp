@@ -126,6 +126,31 @@ __device__ __forceinline__ void block_allsum2(T &a, T &b) {
   __syncthreads();  // ta, tb are reused by the next call
 }
 
+// block_sum of N values in one LDS exchange (each summed in block_sum's
+// order); results valid in thread 0
+template <int N>
+__device__ __forceinline__ void block_sum_n(double (&v)[N]) {
+  __shared__ double part[N][1024 / kWave];
+  const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+#pragma unroll
+  for (int q = 0; q < N; ++q) v[q] = wave_sum(v[q]);
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < N; ++q) part[q][wid] = v[q];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int nw = (blockDim.x + kWave - 1) / kWave;
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+      double s = 0;
+      for (int w = 0; w < nw; ++w) s += part[q][w];
+      v[q] = s;
+    }
+  }
+  __syncthreads();
+}
+
 inline unsigned ceil_div(long a, long b) { return (unsigned)((a + b - 1) / b); }
 
 }  // namespace scihpc

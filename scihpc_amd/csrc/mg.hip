@@ -683,21 +683,43 @@ __global__ void __launch_bounds__(kFusedThreads)
 // p.q and p.r from k_fcg_direction's partial sums (p.q at [0, nb), p.r
 // after them), summed by the first wave in one fixed order, whatever the
 // workgroup size, and returned to every thread: the FCG update and the fused
-// update + pre-smoothing kernel form the same alpha.
-__device__ __forceinline__ void fcg_dir_sums(const double *__restrict__ part, int nb, double &pq, double &pr) {
+// update + pre-smoothing kernel form the same alpha.  corr_n > 0 (the cell
+// count of a singular all-Neumann operator): the sums of p, r and q follow
+// at [2 nb, 5 nb), and p.q, p.r become (p - mean p).q and (p - mean p).r.
+// The constant component of p, which A maps to zero, then cannot enter the
+// step sizes: left in, it grows once the residual is at rounding level and
+// its product with the rounding-level sum of r drives the iteration away.
+__device__ __forceinline__ void fcg_sums_of(double a, double b, double sp, double sr, double sq, double corr_n,
+                                            double &pq, double &pr) {
+  if (corr_n > 0.0) {
+    a -= sp * sq / corr_n;
+    b -= sp * sr / corr_n;
+  }
+  pq = a;
+  pr = b;
+}
+__device__ __forceinline__ void fcg_dir_sums(const double *__restrict__ part, int nb, double corr_n, double &pq,
+                                             double &pr) {
   __shared__ double ta, tb;
   if (threadIdx.x < kWave) {
-    double a = 0.0, b = 0.0;
+    double a = 0.0, b = 0.0, sp = 0.0, sr = 0.0, sq = 0.0;
     for (int k = threadIdx.x; k < nb; k += kWave) {
       a += part[k];
       b += part[nb + k];
+      if (corr_n > 0.0) {
+        sp += part[2 * nb + k];
+        sr += part[3 * nb + k];
+        sq += part[4 * nb + k];
+      }
     }
     a = wave_sum(a);
     b = wave_sum(b);
-    if (threadIdx.x == 0) {
-      ta = a;
-      tb = b;
+    if (corr_n > 0.0) {
+      sp = wave_sum(sp);
+      sr = wave_sum(sr);
+      sq = wave_sum(sq);
     }
+    if (threadIdx.x == 0) fcg_sums_of(a, b, sp, sr, sq, corr_n, ta, tb);
   }
   __syncthreads();
   pq = ta;
@@ -721,7 +743,12 @@ struct FcgUpdate {
   double *scal;
   const double *dir_part;
   int dir_nb;
+  double corr_n;    // fcg_dir_sums' null-space correction (0: none)
   double *r2_part;  // indexed by workgroup
+  // measurement (nullable): workgroup b stores its start and end on the
+  // device wall clock in clk[2 b], clk[2 b + 1] (the kernel's own span is
+  // the earliest start to the latest end)
+  unsigned long long *clk;
 };
 
 template <bool VAR, int TX, int TY>
@@ -736,6 +763,8 @@ __global__ void __launch_bounds__(kFusedThreads)
   const int tile = xcd_tile(ntiles);
   if (tile >= ntiles) return;  // never taken: the grid has exactly ntiles blocks
   const int x0 = (tile % tiles_x) * TX, y0 = (tile / tiles_x) * TY;
+  unsigned long long t_start = 0;
+  if (u.clk && threadIdx.x == 0) t_start = (unsigned long long)wall_clock64();
   // r and q of the staged region are loaded before alpha's reduction (the
   // two latencies overlap), then combined once alpha is known; the largest
   // tile loads them afterwards (its registers would cost occupancy)
@@ -763,7 +792,7 @@ __global__ void __launch_bounds__(kFusedThreads)
     }
   }
   double pq, pr;
-  fcg_dir_sums(u.dir_part, u.dir_nb, pq, pr);
+  fcg_dir_sums(u.dir_part, u.dir_nb, u.corr_n, pq, pr);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     u.scal[S_PQ] = pq;
     u.scal[S_PR] = pr;
@@ -816,6 +845,13 @@ __global__ void __launch_bounds__(kFusedThreads)
   for (int q = threadIdx.x; q < TX * TY; q += blockDim.x) {
     const int gx = x0 + q % TX, gy = y0 + q / TX;
     if (gx < F.nx && gy < F.ny) out[gx + (long)F.nx * gy] = xs[gy - y0 + kFH][gx - x0 + kFH];
+  }
+  if (u.clk) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      u.clk[2 * blockIdx.x] = t_start;
+      u.clk[2 * blockIdx.x + 1] = (unsigned long long)wall_clock64();
+    }
   }
 }
 
@@ -1280,6 +1316,18 @@ __global__ void k_finalize2(const double *__restrict__ partial, int nb, double *
   }
 }
 
+// scal[S_PQ], scal[S_PR] from the direction kernel's nb partial sums (two
+// sets, or five with the null-space correction of fcg_dir_sums)
+__global__ void __launch_bounds__(1024) k_fcg_dir_final(const double *__restrict__ partial, int nb,
+                                                        double *__restrict__ scal, double corr_n) {
+  double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  const int sets = corr_n > 0.0 ? 5 : 2;
+  for (int q = threadIdx.x; q < nb; q += blockDim.x)
+    for (int s = 0; s < sets; ++s) v[s] += partial[s * nb + q];
+  block_sum_n<5>(v);
+  if (threadIdx.x == 0) fcg_sums_of(v[0], v[1], v[2], v[3], v[4], corr_n, scal[S_PQ], scal[S_PR]);
+}
+
 // steepest-descent update sol += alpha res with alpha = rdot / (r^T A r);
 // partial[block] = sum of the updated sol (for the compatibility step)
 template <typename T>
@@ -1478,7 +1526,7 @@ constexpr int kFcgThreads = 1024;
 // or, with zq_n > 0, reduced here from the post-smoothing kernel's zq_n
 // partials (every workgroup sums them in the same order, so all get the
 // same value), which saves the separate reduction launch.
-template <typename T, int DIM, bool VAR>
+template <typename T, int DIM, bool VAR, bool SUMS>
 __global__ void __launch_bounds__(kFcgThreads)
     k_fcg_direction(DLevel<double> L, const T *__restrict__ z, const double *__restrict__ p_old,
                     double *__restrict__ p, double *__restrict__ q, const double *__restrict__ r,
@@ -1498,7 +1546,7 @@ __global__ void __launch_bounds__(kFcgThreads)
     const double pq = scal[S_PQ];
     beta = pq != 0.0 ? -zq / pq : 0.0;  // 0 only after exact convergence
   }
-  double vpq = 0.0, vpr = 0.0;
+  double vpq = 0.0, vpr = 0.0, s3[3] = {0.0, 0.0, 0.0};
   if (c < L.n) {
     int i, j, k;
     cell_ijk(c, L.nx, L.ny, i, j, k);
@@ -1533,16 +1581,28 @@ __global__ void __launch_bounds__(kFcgThreads)
     }
     const double pc = P(c);
     const double qc = off + diag * pc;
+    const double rc = r[c];
     p[c] = pc;
     q[c] = qc;
     vpq = pc * qc;
-    vpr = pc * r[c];
+    vpr = pc * rc;
+    if constexpr (SUMS) {
+      s3[0] = pc;
+      s3[1] = rc;
+      s3[2] = qc;
+    }
   }
   vpq = block_sum(vpq);
   vpr = block_sum(vpr);
+  if constexpr (SUMS) block_sum_n<3>(s3);
   if (threadIdx.x == 0) {
     partial[blockIdx.x] = vpq;
     partial[gridDim.x + blockIdx.x] = vpr;
+    if constexpr (SUMS) {
+      partial[2 * gridDim.x + blockIdx.x] = s3[0];
+      partial[3 * gridDim.x + blockIdx.x] = s3[1];
+      partial[4 * gridDim.x + blockIdx.x] = s3[2];
+    }
   }
 }
 
@@ -1555,7 +1615,7 @@ __global__ void __launch_bounds__(kFcgThreads)
 // the partial sums add a column's cells first (partial[b], partial[nb + b],
 // b the linear block index, nb the block count).
 constexpr int kZmX = 64, kZmY = 8;
-template <typename T, bool VAR>
+template <typename T, bool VAR, bool SUMS>
 __global__ void __launch_bounds__(kZmX *kZmY)
     k_fcg_direction_zm(DLevel<double> L, const T *__restrict__ z, const double *__restrict__ p_old,
                        double *__restrict__ p, double *__restrict__ q, const double *__restrict__ r,
@@ -1566,7 +1626,7 @@ __global__ void __launch_bounds__(kZmX *kZmY)
   const int k0 = blockIdx.z * zchunk, k1 = min(k0 + zchunk, L.nz);
   const long sy = L.nx, sz = (long)L.nx * L.ny;
   auto P = [&](long g) { return double(z[g]) + (first ? 0.0 : beta * p_old[g]); };
-  double vpq = 0.0, vpr = 0.0;
+  double vpq = 0.0, vpr = 0.0, s3[3] = {0.0, 0.0, 0.0};
   if (i < L.nx && j < L.ny && k0 < k1) {
     long c = i + sy * j + sz * k0;
     double pb = k0 > 0 ? P(c - sz) : 0.0, pc = P(c);
@@ -1599,21 +1659,33 @@ __global__ void __launch_bounds__(kZmX *kZmY)
         diag = L.dirichlet ? L.dconst : -cc;
       }
       const double qc = off + diag * pc;
+      const double rc = r[c];
       p[c] = pc;
       q[c] = qc;
       vpq += pc * qc;
-      vpr += pc * r[c];
+      vpr += pc * rc;
+      if constexpr (SUMS) {
+        s3[0] += pc;
+        s3[1] += rc;
+        s3[2] += qc;
+      }
       pb = pc;
       pc = pf;
     }
   }
   vpq = block_sum(vpq);
   vpr = block_sum(vpr);
+  if constexpr (SUMS) block_sum_n<3>(s3);
   if (threadIdx.x == 0) {
     const unsigned nb = gridDim.x * gridDim.y * gridDim.z;
     const unsigned b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
     partial[b] = vpq;
     partial[nb + b] = vpr;
+    if constexpr (SUMS) {
+      partial[2 * nb + b] = s3[0];
+      partial[3 * nb + b] = s3[1];
+      partial[4 * nb + b] = s3[2];
+    }
   }
 }
 
@@ -1830,11 +1902,11 @@ __global__ void __launch_bounds__(kZmX *kZmY) k_prolong3_zm(DLevel<T> G, DLevel<
 __global__ void __launch_bounds__(kFcgThreads)
     k_fcg_update(long n, double *__restrict__ x, double *__restrict__ r, const double *__restrict__ p,
                  const double *__restrict__ q, double *__restrict__ scal, float *__restrict__ r32,
-                 double *__restrict__ partial, const double *__restrict__ dir_part, int dir_nb) {
+                 double *__restrict__ partial, const double *__restrict__ dir_part, int dir_nb, double corr_n) {
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   double pq, pr;
   if (dir_nb > 0) {
-    fcg_dir_sums(dir_part, dir_nb, pq, pr);
+    fcg_dir_sums(dir_part, dir_nb, corr_n, pq, pr);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       scal[S_PQ] = pq;
       scal[S_PR] = pr;
@@ -2210,13 +2282,29 @@ struct scihpc_mg {
     }
   }
 
+  // measurement hook: HIP events around the launches of one kernel inside
+  // solves (probe_which 6: k_pre2d_fcg), at most probe_ev.size() / 2
+  int probe_which = -1, probe_used = 0;
+  std::vector<hipEvent_t> probe_ev;
+  unsigned long long *probe_clk = nullptr;  // per launch: start and end of every workgroup
+  long probe_stride = 0;                     // clock slots per launch
+  bool probe_on(int which) const { return probe_which == which && 2 * probe_used < (int)probe_ev.size(); }
+
   // the FCG update fused into the FP32 level-0 pre-smoothing (k_pre2d_fcg)
   template <bool VAR, int K>
   void pre2d_fcg_k(int nu1, const FcgUpdate &u) {
     constexpr int TX = TileCfg<float>::TX[K], TY = TileCfg<float>::TY[K];
     auto &a = f[0];
-    k_pre2d_fcg<VAR, TX, TY><<<tiles2d(a, 0), kFusedThreads, 0, stream>>>(view(a), view(f[1]), a.sol_alt, nu1, u);
+    const bool probe = probe_on(6);
+    FcgUpdate uc = u;
+    if (probe) {
+      SCIHPC_REQUIRE(2L * tiles2d(a, 0) <= probe_stride, "internal: probe clock slots");
+      uc.clk = probe_clk + probe_stride * probe_used;
+      SCIHPC_HIP(hipEventRecord(probe_ev[2 * probe_used], stream));
+    }
+    k_pre2d_fcg<VAR, TX, TY><<<tiles2d(a, 0), kFusedThreads, 0, stream>>>(view(a), view(f[1]), a.sol_alt, nu1, uc);
     SCIHPC_CHECK_LAUNCH();
+    if (probe) SCIHPC_HIP(hipEventRecord(probe_ev[2 * probe_used++ + 1], stream));
     std::swap(a.sol, a.sol_alt);
   }
   void pre2d_fcg(int nu1, const FcgUpdate &u) {
@@ -2857,21 +2945,39 @@ struct scihpc_mg {
     const bool mixed = o.precision == SCIHPC_PREC_MIXED;
     if (mixed) ensure_fp32();
     double *x = fx_;
-    // r = b - A x
-    if (zero) {
-      SCIHPC_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, stream));
-      k_copy_cast<<<nb, kBlock, 0, stream>>>(l0.rhs, fr_, mixed ? f[0].rhs : nullptr, n);
-      SCIHPC_CHECK_LAUNCH();
-    } else {
-      SCIHPC_HIP(hipMemcpyAsync(x, l0.sol, sizeof(double) * n, hipMemcpyDeviceToDevice, stream));
-      reduce_residual(l0, true);
-      k_copy_cast<<<nb, kBlock, 0, stream>>>(l0.res, fr_, mixed ? f[0].rhs : nullptr, n);
-      SCIHPC_CHECK_LAUNCH();
-    }
-    // FP64 preconditioner: the level-0 right-hand side *is* r during the solve
-    if (!mixed) std::swap(l0.rhs, fr_);
+    // a singular all-Neumann operator solved below 1e-9: the step sizes
+    // leave out the constant component of the direction (fcg_dir_sums),
+    // which otherwise drives the iteration away near the FP64 floor (above
+    // it the final check of b - A x below covers the drift)
+    const double corr_n = bc == SCIHPC_MG_NEUMANN && o.rel_tol < 1e-9 ? double(n) : 0.0;
+    const bool sums = corr_n > 0.0;
+    // r = b - A x (x = 0, or the current level-0 solution); the FP64
+    // preconditioner works in place on level 0, whose right-hand side *is*
+    // r during the solve (b waits in fr_)
+    bool swapped = false;
+    auto start = [&](bool from_zero) -> double {
+      if (swapped) {
+        std::swap(l0.rhs, fr_);
+        swapped = false;
+      }
+      if (from_zero) {
+        SCIHPC_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, stream));
+        k_copy_cast<<<nb, kBlock, 0, stream>>>(l0.rhs, fr_, mixed ? f[0].rhs : nullptr, n);
+        SCIHPC_CHECK_LAUNCH();
+      } else {
+        SCIHPC_HIP(hipMemcpyAsync(x, l0.sol, sizeof(double) * n, hipMemcpyDeviceToDevice, stream));
+        reduce_residual(l0, true);
+        k_copy_cast<<<nb, kBlock, 0, stream>>>(l0.res, fr_, mixed ? f[0].rhs : nullptr, n);
+        SCIHPC_CHECK_LAUNCH();
+      }
+      if (!mixed) {
+        std::swap(l0.rhs, fr_);
+        swapped = true;
+      }
+      return from_zero ? (bnorm > 0.0 ? 1.0 : 0.0) : std::sqrt(fetch(S_NORM)) / scale;
+    };
+    double rr = start(zero);
     double *r = mixed ? fr_ : l0.rhs;
-    double rr = zero ? (bnorm > 0.0 ? 1.0 : 0.0) : std::sqrt(fetch(S_NORM)) / scale;
     int it = 0, cur = 0, spent = 0;
     bool corr_left = mixed && o.learned_correction != 0 && learned.active();
     // 3-D levels of 2^18 cells and more: the direction kernel marches in z
@@ -2889,6 +2995,7 @@ struct scihpc_mg {
                           (long)tiles2d(f[0], 0) <= 2 * (long)nb;
     if (fuse_upd && !fr2_) fr2_ = dalloc<double>(n, stream);
     bool pre0_done = false;
+    for (;;) {
     while (rr > o.rel_tol && it + spent < o.max_cycles) {
       // z = B r; from the second iteration on the level-0 post kernel also
       // forms z . q_old (S_ZQ) for the flexible direction update
@@ -2913,12 +3020,22 @@ struct scihpc_mg {
       double *dpart = fuse ? partial + 2 * (long)nb : partial;
       const double *zq_part = (fuse && it > 0) ? partial : nullptr;
       const int zq_n = (fuse && it > 0) ? np : 0;
-#define SCIHPC_FCG_DIR(T, D, V)                                                                                  \
-  k_fcg_direction<T, D, V><<<nbf, kFcgThreads, 0, stream>>>(view(l0), (const T *)z, pold, pnew, fq_, r, scal, it == 0, \
-                                                       dpart, zq_part, zq_n)
-#define SCIHPC_FCG_ZM(T, V)                                                                              \
-  k_fcg_direction_zm<T, V><<<zm_grid, kZmX * kZmY, 0, stream>>>(view(l0), (const T *)z, pold, pnew, fq_, r, \
-                                                                 scal, it == 0, dpart, zm_chunk)
+#define SCIHPC_FCG_DIR_S(T, D, V, S)                                                                     \
+  k_fcg_direction<T, D, V, S><<<nbf, kFcgThreads, 0, stream>>>(view(l0), (const T *)z, pold, pnew, fq_, r, scal, \
+                                                               it == 0, dpart, zq_part, zq_n)
+#define SCIHPC_FCG_DIR(T, D, V)                   \
+  do {                                            \
+    if (sums) SCIHPC_FCG_DIR_S(T, D, V, true);    \
+    else SCIHPC_FCG_DIR_S(T, D, V, false);        \
+  } while (0)
+#define SCIHPC_FCG_ZM_S(T, V, S)                                                                            \
+  k_fcg_direction_zm<T, V, S><<<zm_grid, kZmX * kZmY, 0, stream>>>(view(l0), (const T *)z, pold, pnew, fq_, r, \
+                                                                    scal, it == 0, dpart, zm_chunk)
+#define SCIHPC_FCG_ZM(T, V)                    \
+  do {                                         \
+    if (sums) SCIHPC_FCG_ZM_S(T, V, true);     \
+    else SCIHPC_FCG_ZM_S(T, V, false);         \
+  } while (0)
       if (zm) {
         if (mixed) { if (variable) SCIHPC_FCG_ZM(float, true); else SCIHPC_FCG_ZM(float, false); }
         else { if (variable) SCIHPC_FCG_ZM(double, true); else SCIHPC_FCG_ZM(double, false); }
@@ -2930,19 +3047,24 @@ struct scihpc_mg {
         else { if (variable) SCIHPC_FCG_DIR(double, 3, true); else SCIHPC_FCG_DIR(double, 3, false); }
       }
 #undef SCIHPC_FCG_ZM
+#undef SCIHPC_FCG_ZM_S
 #undef SCIHPC_FCG_DIR
+#undef SCIHPC_FCG_DIR_S
       SCIHPC_CHECK_LAUNCH();
-      if (!fuse) finalize2(ndir, S_PQ, S_PR);
+      if (!fuse) {
+        k_fcg_dir_final<<<1, 1024, 0, stream>>>(dpart, ndir, scal, corr_n);
+        SCIHPC_CHECK_LAUNCH();
+      }
       cur ^= 1;
       const bool last = it + spent + 1 >= o.max_cycles;
       const bool fused_now = fuse_upd && fuse && !learned_overlap && !corr_left && !last;
       if (fused_now) {
-        FcgUpdate u{x, r, r == fr_ ? fr2_ : fr_, fp_[cur], fq_, scal, dpart, ndir, partial};
+        FcgUpdate u{x, r, r == fr_ ? fr2_ : fr_, fp_[cur], fq_, scal, dpart, ndir, corr_n, partial};
         pre2d_fcg(o.nu1, u);
         r = u.r_out;
       } else {
         k_fcg_update<<<nbf, kFcgThreads, 0, stream>>>(n, x, r, fp_[cur], fq_, scal, mixed ? f[0].rhs : nullptr,
-                                                      partial, fuse ? dpart : nullptr, fuse ? ndir : 0);
+                                                      partial, fuse ? dpart : nullptr, fuse ? ndir : 0, corr_n);
         SCIHPC_CHECK_LAUNCH();
       }
       pre0_done = fused_now;
@@ -2999,11 +3121,34 @@ struct scihpc_mg {
     if (learned_overlap && learned_pending) {  // converged before the guess was ready: unused
       SCIHPC_HIP(hipEventSynchronize(lev));
       learned_pending = false;
+      learned_overlap = false;
       st.learned_rejected++;
+    }
+    // r is a recurrence and drifts from b - A x by rounding: a solve it
+    // calls converged is checked against the FP64 residual of x, and
+    // continues from x (r = b - A x afresh) while that is above rel_tol and
+    // iterations remain; the reported residual is always b - A x's (one
+    // residual pass per solve)
+    if (swapped) {
+      std::swap(l0.rhs, fr_);
+      swapped = false;
+    }
+    const bool recurrence_converged = rr <= o.rel_tol;
+    SCIHPC_HIP(hipMemcpyAsync(l0.sol, x, sizeof(double) * n, hipMemcpyDeviceToDevice, stream));
+    reduce_residual(l0, false);
+    rr = std::sqrt(fetch(S_NORM)) / scale;
+    if (!recurrence_converged || rr <= o.rel_tol || it + spent >= o.max_cycles) break;
+    spent += it;
+    it = 0;
+    cur = 0;
+    pre0_done = false;
+    corr_left = false;
+    rr = start(false);
+    r = mixed ? fr_ : l0.rhs;
     }
     it += spent;
     last_rr_ = rr;
-    if (!mixed) std::swap(l0.rhs, fr_);  // restore b
+    if (swapped) std::swap(l0.rhs, fr_);  // restore b
     SCIHPC_HIP(hipMemcpyAsync(l0.sol, x, sizeof(double) * n, hipMemcpyDeviceToDevice, stream));
     if (bc == SCIHPC_MG_NEUMANN) sub_mean(l0.sol, n);
     st.cycles = it;
@@ -3405,7 +3550,7 @@ extern "C" int scihpc_mg_create(int ndim, int nx, int ny, int nz, double dx, dou
       }
       h->Minv = dalloc<double>((long)std::min<long>(h->d.back().n, kMaxCoarse) *
                                std::min<long>(h->d.back().n, kMaxCoarse), h->stream);
-      h->partial = dalloc<double>(4 * ((long)ceil_div(h->d[0].n, kBlock) + 64), h->stream);
+      h->partial = dalloc<double>(8 * ((long)ceil_div(h->d[0].n, kBlock) + 64), h->stream);
       h->partial2 = dalloc<double>(4 * ((long)ceil_div(h->d[0].n, (long)kBlock * kBlock) + 64), h->stream);
       SCIHPC_HIP(hipHostMalloc((void **)&h->host_part,
                                sizeof(double) * 4 * ((long)ceil_div(h->d[0].n, kBlock) + 64),
@@ -3442,6 +3587,8 @@ extern "C" int scihpc_mg_destroy(scihpc_mg *h) {
   if (!h) return SCIHPC_OK;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   if (h->lstream) (void)hipStreamSynchronize(h->lstream);
+  for (hipEvent_t e : h->probe_ev) (void)hipEventDestroy(e);
+  if (h->probe_clk) (void)hipFree(h->probe_clk);
   auto fr = [](void *p) {
     if (p) (void)hipFree(p);
   };
@@ -3588,7 +3735,6 @@ extern "C" int scihpc_mg_set_rhs_device(scihpc_mg *h, const double *dsrc) {
     check_level(h, 0);
     SCIHPC_REQUIRE(dsrc, "null source");
     SCIHPC_HIP(hipMemcpyAsync(h->d[0].rhs, dsrc, sizeof(double) * h->d[0].n, hipMemcpyDeviceToDevice, h->stream));
-    SCIHPC_HIP(hipStreamSynchronize(h->stream));
   });
 }
 
@@ -3942,6 +4088,73 @@ extern "C" int scihpc_mg_learned_features(scihpc_mg *h, int level, float *feat, 
   });
 }
 
+extern "C" int scihpc_mg_probe_kernel(scihpc_mg *h, int which, int capacity) {
+  return guarded([&] {
+    check_level(h, 0);
+    SCIHPC_REQUIRE(which == -1 || which == 6, "which must be 6 (k_pre2d_fcg) or -1");
+    SCIHPC_REQUIRE(capacity >= 0 && capacity <= 65536, "capacity must be 0..65536");
+    SCIHPC_HIP(hipStreamSynchronize(h->stream));
+    for (hipEvent_t e : h->probe_ev) SCIHPC_HIP(hipEventDestroy(e));
+    h->probe_ev.clear();
+    if (h->probe_clk) SCIHPC_HIP(hipFree(h->probe_clk));
+    h->probe_clk = nullptr;
+    h->probe_used = 0;
+    h->probe_which = which;
+    if (which < 0 || capacity == 0) return;
+    // timing-only events: no system-scope fence on record, whose L2
+    // writeback and invalidation would slow the bracketed kernel itself
+    h->probe_ev.resize(2 * (size_t)capacity);
+    for (auto &e : h->probe_ev) SCIHPC_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
+    // every tile kind of level 0 has at most one workgroup per 64 cells
+    h->probe_stride = 2 * (long)ceil_div(h->d[0].n, 64);
+    SCIHPC_HIP(hipMalloc(&h->probe_clk, sizeof(unsigned long long) * h->probe_stride * (size_t)capacity));
+  });
+}
+
+extern "C" int scihpc_mg_probe_result(scihpc_mg *h, int *launches, double *total_ms, double *kernel_ms,
+                                      double *bytes) {
+  return guarded([&] {
+    check_level(h, 0);
+    SCIHPC_REQUIRE(h->probe_which >= 0 && h->probe_clk, "no probe armed");
+    SCIHPC_HIP(hipStreamSynchronize(h->stream));
+    double t = 0.0;
+    for (int k = 0; k < h->probe_used; ++k) {
+      float ms = 0.f;
+      SCIHPC_HIP(hipEventElapsedTime(&ms, h->probe_ev[2 * k], h->probe_ev[2 * k + 1]));
+      t += ms;
+    }
+    // the kernels' own spans: earliest workgroup start to latest workgroup
+    // end on the wall clock (constant rate, kHz)
+    const long ntiles = (long)h->tiles2d(h->f[0], 0);
+    std::vector<unsigned long long> clk((size_t)h->probe_stride * h->probe_used);
+    if (h->probe_used)
+      SCIHPC_HIP(hipMemcpy(clk.data(), h->probe_clk, sizeof(unsigned long long) * clk.size(), hipMemcpyDeviceToHost));
+    int dev = 0, khz = 0;
+    SCIHPC_HIP(hipGetDevice(&dev));
+    SCIHPC_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+    SCIHPC_REQUIRE(khz > 0, "the device reports no wall-clock rate");
+    double span = 0.0;
+    for (int k = 0; k < h->probe_used; ++k) {
+      const unsigned long long *c = clk.data() + (size_t)h->probe_stride * k;
+      unsigned long long t0 = ~0ull, t1 = 0;
+      for (long b = 0; b < ntiles; ++b) {
+        t0 = std::min(t0, c[2 * b]);
+        t1 = std::max(t1, c[2 * b + 1]);
+      }
+      span += double(t1 - t0) / double(khz);
+    }
+    if (launches) *launches = h->probe_used;
+    if (total_ms) *total_ms = t;
+    if (kernel_ms) *kernel_ms = span;
+    if (bytes) {  // the algorithmic bytes of one launch, as scihpc_mg_time_kernel(which = 6)
+      const double n = (double)h->d[0].n, nc = h->L > 1 ? (double)h->d[1].n : 0.0;
+      const double nf = h->variable ? h->ndim : 0;
+      *bytes = (4.0 * nf + 56.0) * n + 8.0 * nc;
+    }
+    h->probe_used = 0;
+  });
+}
+
 extern "C" int scihpc_mg_time_kernel(scihpc_mg *h, int which, int iters, double *ms, double *bytes) {
   return guarded([&] {
     check_level(h, 0);
@@ -3985,7 +4198,8 @@ extern "C" int scihpc_mg_time_kernel(scihpc_mg *h, int which, int iters, double 
                      "which = 6 needs a tiled 2-D level 0 and a mixed flexible-CG solve first");
       const long nb = ceil_div(l0.n, kBlock);
       fu = FcgUpdate{h->fx_, h->fr_, h->fr2_, h->fp_[0], h->fq_, h->scal, h->partial + 2 * nb,
-                     (int)ceil_div(l0.n, kFcgThreads), h->partial};
+                     (int)ceil_div(l0.n, kFcgThreads), h->bc == SCIHPC_MG_NEUMANN ? double(l0.n) : 0.0,
+                     h->partial};
     }
     auto launch = [&] {
       switch (which) {

@@ -281,3 +281,63 @@ def test_c2_bubble_converged_vs_oracle(oracle):
     # a different algorithm (or FP32 inside) at the same tolerance agrees to
     # what the tolerance fixes
     assert out[1e-10][2] < 1e-6 and out[1e-10][3] < 1e-6
+
+
+@pytest.mark.parametrize("precision", [PREC_FP64, PREC_MIXED], ids=["fp64", "mixed"])
+def test_fcg_reports_the_true_residual(oracle, precision):
+    """The flexible CG updates r by recurrence, which drifts from b - A x by
+    rounding; near and below the FP64 floor of the 1000:1 bubble the gap
+    matters.  Every solve's reported residual is b - A x (re-checked here
+    with the oracle's FP64 operator), a converged solve meets the tolerance
+    with it, and asked for less than the floor the solve stays at the floor
+    instead of drifting away."""
+    nx, ny = 256, 512
+    w = W.bubble_2d(nx, ny)
+    g = Multigrid(2, nx, ny, 1, 1 / nx, 2 / ny, 1.0, "neumann")
+    g.set_density(w["rho"])
+    g.set_rhs(w["rhs"])
+    o = oracle.mg(2, nx, ny, 1, 1 / nx, 2 / ny, 1.0, "neumann")
+    o.set_rho(w["rho"])
+    o.vec(0, "rhs")[:] = w["rhs"]
+    bn = np.linalg.norm(w["rhs"])
+    for tol in (1e-8, 1e-10, 1e-11, 1e-12, 1e-14):
+        st = g.solve(rel_tol=tol, max_cycles=60, precision=precision, accel=1)
+        o.vec(0, "sol")[:] = g.sol(0)
+        r = o.residual(0) / bn
+        print(f"FCG precision={precision} tol={tol:.0e}: converged={st['converged']} {st['cycles']} iterations, "
+              f"reported {st['rel_residual']:.3e}, oracle {r:.3e}", flush=True)
+        # the two FP64 residuals of the same x sum the stencil in different
+        # orders: near the floor they differ by rounding, up to about 1e-12 of
+        # ||b|| on this 1000:1 operator
+        assert abs(st["rel_residual"] - r) <= 1e-3 * r + 2e-12
+        if st["converged"]:
+            assert r <= tol + 2e-12
+        if tol >= 1e-9:
+            assert st["converged"]
+        else:
+            assert r < 1e-9  # no drift away from the floor
+
+
+def test_probe_times_the_in_solve_launches():
+    """The measurement hook brackets every launch of the fused FCG update +
+    FP32 level-0 pre-smoothing inside mixed flexible-CG solves: one launch
+    per iteration after the first, kernel spans shorter than the event
+    intervals around them, the algorithmic bytes of the timing hook, and
+    solves unchanged by the probe."""
+    nx, ny = 512, 1024
+    w = W.bubble_2d(nx, ny)
+    g = Multigrid(2, nx, ny, 1, 1 / nx, 2 / ny, 1.0, "neumann")
+    g.set_density(w["rho"])
+    g.set_rhs(w["rhs"])
+    ref = g.solve(rel_tol=1e-8, precision=PREC_MIXED, accel=1)
+    x_ref = g.sol(0)
+    g.probe_kernel(6, capacity=64)
+    st = g.solve(rel_tol=1e-8, precision=PREC_MIXED, accel=1)
+    launches, ev_ms, k_ms, b = g.probe_result()
+    g.probe_kernel(-1, 0)
+    print(f"probe: {launches} launches, events {ev_ms / launches * 1e3:.2f} us, kernel {k_ms / launches * 1e3:.2f} us",
+          flush=True)
+    assert st["cycles"] == ref["cycles"] and np.array_equal(g.sol(0), x_ref)
+    assert st["cycles"] - 1 <= launches <= st["cycles"]
+    assert 0 < k_ms < ev_ms
+    assert b == (4 * 2 + 56) * nx * ny + 8.0 * (nx // 2) * (ny // 2)

@@ -138,29 +138,58 @@ def test_fcg_converges_to_dense(oracle, bc):
 
 def test_fcg_invariants(oracle):
     """After every FCG(1) step: the new direction is A-conjugate to the
-    previous one (p.q_old = 0), the updated residual is orthogonal to the
-    direction (p.r = 0), the recurrence residual equals b - A x, and the
-    first iterate is the energy-optimal multiple of z = V(b)."""
+    previous one, the updated residual is orthogonal to the direction (both
+    with the direction's constant component removed, the step sizes' form on
+    the all-Neumann operator), the recurrence residual equals b - A x, and
+    the first iterate is the energy-optimal multiple of z = V(b)."""
     nx, ny = 24, 40
     m = oracle.mg(2, nx, ny, 1, 1 / nx, 2 / ny, 1.0, "neumann")
     m.set_rho(_bubble_rho(nx, ny, 2.0))
     A = dense(m)
     b = np.random.default_rng(13).standard_normal(nx * ny)
     b -= b.mean()
+    # z = V(b) from the oracle's V-cycle alone
+    m.vec(0, "rhs")[:] = b
+    m.vec(0, "sol")[:] = 0
+    oracle.lib.orc_mg_vcycle_rb(m.h, 2, 2)
+    z1 = m.vec(0, "sol").copy()
     for k in range(1, 6):
         m.vec(0, "rhs")[:] = b
         it, _ = m.solve_fcg(1e-30, k)
         assert it == k
         x, r, p, q = (m.fcg_vec(w) for w in ("x", "r", "p", "q"))
+        pc = p - p.mean()
         assert np.allclose(q, A @ p, rtol=0, atol=1e-9 * np.abs(q).max())
-        assert abs(p @ r) < 1e-10 * np.linalg.norm(p) * np.linalg.norm(b)
+        assert abs(pc @ r) < 1e-10 * np.linalg.norm(pc) * np.linalg.norm(b)
         assert np.linalg.norm(r - (b - A @ x)) < 1e-9 * np.linalg.norm(b)
         if k == 1:
-            z = m.fcg_vec("z")
-            assert np.allclose(x, (z @ b) / (z @ (A @ z)) * z, rtol=1e-10, atol=1e-14)
+            zc = z1 - z1.mean()
+            assert np.allclose(x, (zc @ b) / (zc @ (A @ z1)) * z1, rtol=1e-10, atol=1e-14)
         else:
-            po, qo = m.fcg_vec("p_old"), m.fcg_vec("q_old")
-            assert abs(p @ qo) < 1e-10 * np.linalg.norm(p) * np.linalg.norm(qo)
+            qo = m.fcg_vec("q_old")
+            assert abs(pc @ qo) < 1e-10 * np.linalg.norm(pc) * np.linalg.norm(qo)
+
+
+def test_fcg_does_not_drift_below_the_floor(oracle):
+    """Asked for a residual below the FP64 floor, FCG(1) on the singular
+    all-Neumann operator keeps b - A x at the floor (without the direction's
+    constant component left out of the step sizes it grew back to about
+    1e-6 within 20 iterations), and a requested 1e-12 is met by the true
+    residual."""
+    nx, ny = 32, 48
+    m = oracle.mg(2, nx, ny, 1, 1 / nx, 1.5 / ny, 1.0, "neumann")
+    m.set_rho(_bubble_rho(nx, ny, 1.5))
+    A = dense(m)
+    b = np.random.default_rng(12).standard_normal(nx * ny)
+    b -= b.mean()
+    m.vec(0, "rhs")[:] = b
+    m.solve_fcg(1e-30, 40)
+    x = m.vec(0, "sol")
+    assert np.linalg.norm(b - A @ x) / np.linalg.norm(b) < 1e-12
+    m.vec(0, "rhs")[:] = b
+    it, _ = m.solve_fcg(1e-12, 100)
+    x = m.vec(0, "sol")
+    assert np.linalg.norm(b - A @ x) / np.linalg.norm(b) <= 1e-12 and it < 30
 
 
 def test_rb_transfers(oracle):
