@@ -1257,56 +1257,102 @@ __global__ void __launch_bounds__(kLsBlock)
   }
 }
 
-// Coefficients of one line system, factorised: ccd_base::init_coefficients
-// (ccd_base.cpp:43-104, kind 0) and uccd_base::init_coefficients
-// (uccd_base.cpp:16-117, kind 1 upwind, 2 downwind), then twin_dec
-// (matrix_solver.cpp:21-56).  12 arrays of n.
+// Coefficients of the compact schemes' line systems, factorised.  Along a
+// grid line of n cells CCD (kind 0; ccd_base.cpp:43-104) and UCCD (kind 1
+// upwind, 2 downwind; uccd_base.cpp:16-117) solve a block-tridiagonal
+// system for y_i = (f'_i, f''_i):
+//   L_i y_{i-1} + D_i y_i + U_i y_{i+1} = rhs_i,
+// every block 2 x 2, [[a, b], [aa, bb]]: the f' equation's coefficients of
+// f' and f'' in the first row, the f'' equation's in the second.  Interior
+// rows hold the schemes' stencils; rows 0 and n-1 their one-sided
+// closures.  The factorisation is block LU without pivoting (the
+// elimination of twin_dec, matrix_solver.cpp:21-56): for i = 1 .. n-1 the
+// multiplier M_i = L_i D'_{i-1}^{-1} replaces L_i and D'_i = D_i - M_i
+// U_{i-1}; the 2 x 2 inverse is the adjugate over the determinant.  The
+// back substitution (k_twin_line) reads L_0 and U_{n-1} as [[0, 1], [0, 1]].
+// Every product, sum and quotient is formed in the order the reference's
+// elimination forms it, so the coefficients are bit-identical to its.
+// Layout: 12 arrays of n, entry e (a, b, aa, bb) of band k (L, D, U) at
+// [(3 e + k) n, (3 e + k + 1) n).
+namespace {
+struct Blk2 {
+  double a, b, aa, bb;
+};
+struct BandRow {
+  Blk2 L, D, U;
+};
+
+BandRow compact_interior_row(int kind, double h) {
+  // the f'' equation is shared: -9/(8h) (f_{i+1} - f_{i-1}) stencil side
+  const Blk2 none{0.0, 0.0, 0.0, 0.0};
+  BandRow r{none, none, none};
+  r.L.aa = -9.0 / 8.0 / h;
+  r.D.aa = 0.0;
+  r.U.aa = 9.0 / 8.0 / h;
+  r.L.bb = -1.0 / 8.0;
+  r.D.bb = 1.0;
+  r.U.bb = -1.0 / 8.0;
+  // the f' equation: centred (CCD) or upwind-biased (UCCD) weights
+  const double w1 = 0.875, v1 = 0.1251282341599089, v2 = -0.2487176584009104, v3 = 0.0001282341599089;
+  r.D.a = 1.0;
+  if (kind == 0) {
+    r.L.a = 7.0 / 16.0;
+    r.U.a = 7.0 / 16.0;
+    r.L.b = h / 16.0;
+    r.D.b = 0.0;
+    r.U.b = -h / 16.0;
+  } else if (kind == 1) {
+    r.L.a = w1;
+    r.U.a = 0.0;
+    r.L.b = v1 * h;
+    r.D.b = v2 * h;
+    r.U.b = v3 * h;
+  } else {
+    r.L.a = 0.0;
+    r.U.a = w1;
+    r.L.b = -v3 * h;
+    r.D.b = -v2 * h;
+    r.U.b = -v1 * h;
+  }
+  return r;
+}
+}  // namespace
+
 static std::vector<double> compact_coefficients(int kind, int n, double h) {
 #pragma clang fp contract(off)
-  std::vector<double> c(12 * (size_t)n, 0.0);
-  double *a[3] = {&c[0], &c[n], &c[2 * n]}, *b[3] = {&c[3 * n], &c[4 * n], &c[5 * n]};
-  double *aa[3] = {&c[6 * n], &c[7 * n], &c[8 * n]}, *bb[3] = {&c[9 * n], &c[10 * n], &c[11 * n]};
-  const double a1 = 0.875, b1 = 0.1251282341599089, b2 = -0.2487176584009104, b3 = 0.0001282341599089;
-  for (int i = 0; i < n; ++i) {
-    if (kind == 0) {
-      a[0][i] = 7.0 / 16.0; a[1][i] = 1.0; a[2][i] = 7.0 / 16.0;
-      b[0][i] = h / 16.0; b[1][i] = 0.0; b[2][i] = -h / 16.0;
-    } else if (kind == 1) {
-      a[0][i] = a1; a[1][i] = 1.0; a[2][i] = 0.0;
-      b[0][i] = b1 * h; b[1][i] = b2 * h; b[2][i] = b3 * h;
-    } else {
-      a[0][i] = 0.0; a[1][i] = 1.0; a[2][i] = a1;
-      b[0][i] = -b3 * h; b[1][i] = -b2 * h; b[2][i] = -b1 * h;
-    }
-    aa[0][i] = -9.0 / 8.0 / h; aa[1][i] = 0.0; aa[2][i] = 9.0 / 8.0 / h;
-    bb[0][i] = -1.0 / 8.0; bb[1][i] = 1.0; bb[2][i] = -1.0 / 8.0;
-  }
-  const double alpha_1 = 2.0, beta_1 = -1.0, alpha_2 = 5.0, beta_2 = -6.0;
-  a[0][0] = 0.0; a[1][0] = 1.0; a[2][0] = alpha_1;
-  b[0][0] = 0.0; b[1][0] = 0.0; b[2][0] = beta_1 * h;
-  aa[0][0] = 0.0; aa[1][0] = 0.0; aa[2][0] = beta_2 / h;
-  bb[0][0] = 0.0; bb[1][0] = 1.0; bb[2][0] = alpha_2;
-  a[0][n - 1] = alpha_1; a[1][n - 1] = 1.0; a[2][n - 1] = 0.0;
-  b[0][n - 1] = -beta_1 * h; b[1][n - 1] = 0.0; b[2][n - 1] = 0.0;
-  aa[0][n - 1] = -beta_2 / h; aa[1][n - 1] = 0.0; aa[2][n - 1] = 0.0;
-  bb[0][n - 1] = alpha_2; bb[1][n - 1] = 1.0; bb[2][n - 1] = 0.0;
-  // twin_dec
-  a[0][0] = 0.0; aa[0][0] = 0.0; b[0][0] = 1.0; bb[0][0] = 1.0;
-  a[2][n - 1] = 0.0; aa[2][n - 1] = 0.0; b[2][n - 1] = 1.0; bb[2][n - 1] = 1.0;
+  std::vector<BandRow> row(n);
+  for (int i = 0; i < n; ++i) row[i] = compact_interior_row(kind, h);
+  // one-sided closures of the first and last cells (shared by all kinds)
+  const double c1 = 2.0, d1 = -1.0, c2 = 5.0, d2 = -6.0;
+  const Blk2 unused{0.0, 1.0, 0.0, 1.0};  // what the back substitution reads outside the band
+  row[0].L = unused;
+  row[0].D = Blk2{1.0, 0.0, 0.0, 1.0};
+  row[0].U = Blk2{c1, d1 * h, d2 / h, c2};
+  row[n - 1].L = Blk2{c1, -d1 * h, -d2 / h, c2};
+  row[n - 1].D = Blk2{1.0, 0.0, 0.0, 1.0};
+  row[n - 1].U = unused;
   for (int i = 1; i < n; ++i) {
-    const double den = a[1][i - 1] * bb[1][i - 1] - aa[1][i - 1] * b[1][i - 1];
-    double sc1 = -(aa[1][i - 1] * b[0][i] - a[0][i] * bb[1][i - 1]) / den;
-    double sc2 = (b[0][i] * a[1][i - 1] - b[1][i - 1] * a[0][i]) / den;
-    a[0][i] = sc1;
-    a[1][i] = a[1][i] - (sc1 * a[2][i - 1] + sc2 * aa[2][i - 1]);
-    b[0][i] = sc2;
-    b[1][i] = b[1][i] - (sc1 * b[2][i - 1] + sc2 * bb[2][i - 1]);
-    sc1 = -(aa[1][i - 1] * bb[0][i] - aa[0][i] * bb[1][i - 1]) / den;
-    sc2 = (bb[0][i] * a[1][i - 1] - b[1][i - 1] * aa[0][i]) / den;
-    aa[0][i] = sc1;
-    aa[1][i] = aa[1][i] - (sc1 * a[2][i - 1] + sc2 * aa[2][i - 1]);
-    bb[0][i] = sc2;
-    bb[1][i] = bb[1][i] - (sc1 * b[2][i - 1] + sc2 * bb[2][i - 1]);
+    const Blk2 P = row[i - 1].D, Q = row[i - 1].U, Lw = row[i].L;
+    const double det = P.a * P.bb - P.aa * P.b;
+    // M = Lw P^-1, P^-1 = [[bb, -b], [-aa, a]] / det
+    const Blk2 M{(Lw.a * P.bb - P.aa * Lw.b) / det, (Lw.b * P.a - P.b * Lw.a) / det,
+                 (Lw.aa * P.bb - P.aa * Lw.bb) / det, (Lw.bb * P.a - P.b * Lw.aa) / det};
+    Blk2 &D = row[i].D;
+    D.a = D.a - (M.a * Q.a + M.b * Q.aa);
+    D.b = D.b - (M.a * Q.b + M.b * Q.bb);
+    D.aa = D.aa - (M.aa * Q.a + M.bb * Q.aa);
+    D.bb = D.bb - (M.aa * Q.b + M.bb * Q.bb);
+    row[i].L = M;
+  }
+  std::vector<double> c(12 * (size_t)n);
+  for (int i = 0; i < n; ++i) {
+    const Blk2 *band[3] = {&row[i].L, &row[i].D, &row[i].U};
+    for (int k = 0; k < 3; ++k) {
+      c[(0 + k) * (size_t)n + i] = band[k]->a;
+      c[(3 + k) * (size_t)n + i] = band[k]->b;
+      c[(6 + k) * (size_t)n + i] = band[k]->aa;
+      c[(9 + k) * (size_t)n + i] = band[k]->bb;
+    }
   }
   return c;
 }
