@@ -320,19 +320,25 @@ SCIHPC_API int scihpc_proj_time_momentum(scihpc_proj *h, const scihpc_flow_param
 
 /* ------------------------------------------------------------------ */
 /* Learned pressure guess (U-Net).  No reference counterpart: SciHPC has no
- * learned component.  A 4-level U-Net (16/32/64/128 channels, 3x3
- * convolutions, BF16 weights and activations, FP32 accumulation on the
- * matrix cores) maps (b / sigma_b, ln rho / ln 1000) on one multigrid level
- * to p / (sigma_b * out_scale) there; the solve prolongs, smooths and scales
- * it and keeps it only if it lowers the FP64 residual.                    */
+ * learned component.  A 4-level U-Net (16/32/64/128 channels, 3x3 (2-D) or
+ * 3x3x3 (3-D) convolutions, BF16 weights and activations, FP32 accumulation
+ * on the matrix cores) maps (b / sigma_b, ln rho / ln 1000) on one
+ * multigrid level to p / (sigma_b * out_scale) there; the solve prolongs,
+ * smooths and scales it and keeps it only if it lowers the FP64 residual.
+ * Pixels are x fastest: (D x) H x W.                                      */
 typedef struct scihpc_unet scihpc_unet;
 
-/* height x width must be multiples of 8; max_batch images per pass */
+/* 2-D network: height x width must be multiples of 8; max_batch images per
+ * pass */
 SCIHPC_API int scihpc_unet_create(int height, int width, int max_batch,
                                   unsigned long long seed, scihpc_unet **out);
+/* 3-D network (3x3x3 convolutions, 2x2x2 pooling): depth x height x width,
+ * each a multiple of 8 */
+SCIHPC_API int scihpc_unet_create3(int depth, int height, int width, int max_batch,
+                                   unsigned long long seed, scihpc_unet **out);
 SCIHPC_API int scihpc_unet_destroy(scihpc_unet *h);
 SCIHPC_API int scihpc_unet_num_params(scihpc_unet *h, long long *n);
-/* layer 0..13: 3x3 convolutions (w [cout][3][3][cin] then b [cout] at
+/* layer 0..13: convolutions (w [cout][3][3](3)[cin] then b [cout] at
  * offset); layer 14: the 1x1 head (16 weights then 1 bias) */
 SCIHPC_API int scihpc_unet_layer(scihpc_unet *h, int layer, int *cin, int *cout,
                                  int *level, long long *offset);
@@ -341,8 +347,9 @@ SCIHPC_API int scihpc_unet_set_params(scihpc_unet *h, const float *host);
 SCIHPC_API int scihpc_unet_get_grads(scihpc_unet *h, float *host);
 SCIHPC_API int scihpc_unet_set_out_scale(scihpc_unet *h, float scale);
 SCIHPC_API int scihpc_unet_get_out_scale(scihpc_unet *h, float *scale);
-/* training samples kept on the device: feat [n][H*W][2], target [n][H*W],
- * faces [n][H*W][2] (host FP32, as scihpc_mg_learned_features writes them).
+/* training samples kept on the device: feat [n][pixels][2], target
+ * [n][pixels], faces [n][pixels][ndim] (host FP32, as
+ * scihpc_mg_learned_features writes them).
  * With faces the loss is the A(rho) energy norm of the error relative to the
  * target's, sum_f c_f (e_i - e_j)^2 / sum_f c_f (t_i - t_j)^2 (what flexible
  * CG reduces); with faces = NULL it is the mean squared error.  Set
@@ -350,7 +357,7 @@ SCIHPC_API int scihpc_unet_get_out_scale(scihpc_unet *h, float *scale);
 SCIHPC_API int scihpc_unet_set_samples(scihpc_unet *h, int n, const float *feat,
                                        const float *target, const float *faces);
 /* forward pass on stored samples; idx [n][2] = (sample, flips: bit 0 mirrors
- * x, bit 1 mirrors y); out (host [n][H*W]) may be NULL */
+ * x, bit 1 mirrors y, bit 2 mirrors z); out (host [n][pixels]) may be NULL */
 SCIHPC_API int scihpc_unet_forward(scihpc_unet *h, int n, const int *idx, float *out);
 /* loss of the last forward pass against target / out_scale, and all
  * parameter gradients (read with scihpc_unet_get_grads) */
@@ -358,6 +365,11 @@ SCIHPC_API int scihpc_unet_backward(scihpc_unet *h, double *loss);
 /* forward + backward + one Adam step (lr); loss may be NULL (no sync) */
 SCIHPC_API int scihpc_unet_train_step(scihpc_unet *h, int n, const int *idx, float lr,
                                       double *loss);
+/* energy loss of the smoothed error: E(M^sweeps e) / E(t) with M one
+ * damped-Jacobi sweep (weight omega in (0, 1]) of the network grid's A(rho),
+ * the error the multigrid smoother leaves of the network's output (0 sweeps:
+ * the plain energy norm, the default) */
+SCIHPC_API int scihpc_unet_set_loss_smoothing(scihpc_unet *h, int sweeps, float omega);
 SCIHPC_API int scihpc_unet_sync(scihpc_unet *h);
 /* device ms of one forward pass (which 0) or training step (which 1, with
  * a negligible learning rate) on stored samples 0..n-1, and its MFMA FLOPs */
@@ -369,6 +381,11 @@ SCIHPC_API int scihpc_unet_time(scihpc_unet *h, int n, int which, int iters, dou
 SCIHPC_API int scihpc_conv3x3_forward(const uint16_t *x, int n, int height, int width,
                                       int cin, const uint16_t *w, const float *b,
                                       int cout, int relu, uint16_t *y);
+/* single 3x3x3 convolution (NDHWC BF16 x [n][d][h][w][cin], w
+ * [cout][3][3][3][cin]), otherwise as scihpc_conv3x3_forward */
+SCIHPC_API int scihpc_conv3x3x3_forward(const uint16_t *x, int n, int depth, int height,
+                                        int width, int cin, const uint16_t *w,
+                                        const float *b, int cout, int relu, uint16_t *y);
 
 /* Use `net` (NULL detaches) for the learned initial guess of
  * scihpc_mg_solve(use_learned = 1); its size must equal multigrid level

@@ -170,14 +170,15 @@ class Multigrid:
         check(self._lib.scihpc_mg_attach_learned(self.h, net.h if net is not None else None, level))
 
     def learned_features(self, level=1, with_target=False, with_faces=False):
-        """(features [H*W, 2], target [H*W] or None, faces [H*W, 2] or None)
-        of the current b, rho and sol on `level`."""
-        nx, ny = self.shapes[level][0], self.shapes[level][1]
-        feat = np.empty(2 * nx * ny, np.float32)
-        tgt = np.empty(nx * ny, np.float32) if with_target else None
-        faces = np.empty(2 * nx * ny, np.float32) if with_faces else None
+        """(features [cells, 2], target [cells] or None, faces [cells, ndim]
+        or None) of the current b, rho and sol on `level` (cells x fastest)."""
+        nx, ny, nz = self.shapes[level][:3]
+        cells = nx * ny * (nz if self.ndim == 3 else 1)
+        feat = np.empty(2 * cells, np.float32)
+        tgt = np.empty(cells, np.float32) if with_target else None
+        faces = np.empty(self.ndim * cells, np.float32) if with_faces else None
         check(self._lib.scihpc_mg_learned_features(self.h, level, feat, tgt, faces))
-        return feat.reshape(ny * nx, 2), tgt, None if faces is None else faces.reshape(ny * nx, 2)
+        return feat.reshape(cells, 2), tgt, None if faces is None else faces.reshape(cells, self.ndim)
 
     def probe_kernel(self, which=6, capacity=4096):
         """Arm HIP events around every launch of kernel `which` inside the
@@ -201,17 +202,24 @@ class Multigrid:
 
 class UNet:
     """The learned pressure guess: 4-level U-Net (16/32/64/128 channels) with
-    BF16 MFMA convolutions; no SciHPC counterpart.  height x width is the
-    grid of the multigrid level it serves (multiples of 8)."""
+    BF16 MFMA convolutions; no SciHPC counterpart.  height x width (x depth
+    for the 3-D network: 3x3x3 convolutions, 2x2x2 pooling) is the grid of
+    the multigrid level it serves (multiples of 8)."""
 
-    N_LAYERS = 15  # 14 3x3 convolutions + the 1x1 head
+    N_LAYERS = 15  # 14 3x3 (3x3x3) convolutions + the 1x1 head
 
-    def __init__(self, height, width, max_batch=1, seed=0):
+    def __init__(self, height, width, max_batch=1, seed=0, depth=None):
         self._lib = lib()
         h = C.c_void_p()
-        check(self._lib.scihpc_unet_create(height, width, max_batch, seed, C.byref(h)))
+        if depth is None:
+            check(self._lib.scihpc_unet_create(height, width, max_batch, seed, C.byref(h)))
+        else:
+            check(self._lib.scihpc_unet_create3(depth, height, width, max_batch, seed, C.byref(h)))
         self.h = h
+        self.ndim = 2 if depth is None else 3
+        self.D = 1 if depth is None else depth
         self.H, self.W, self.max_batch = height, width, max_batch
+        self.pixels = self.D * height * width
         n = C.c_longlong()
         check(self._lib.scihpc_unet_num_params(h, C.byref(n)))
         self.num_params = n.value
@@ -258,16 +266,22 @@ class UNet:
         check(self._lib.scihpc_unet_set_out_scale(self.h, float(v)))
 
     def set_samples(self, feat, target, faces=None):
-        """feat [n, H*W, 2], target [n, H*W], faces [n, H*W, 2] or None (FP32,
-        host); faces select the energy-norm loss.  Set out_scale first."""
+        """feat [n, pixels, 2], target [n, pixels], faces [n, pixels, ndim] or
+        None (FP32, host; pixels x fastest); faces select the energy-norm
+        loss.  Set out_scale first."""
         feat = np.ascontiguousarray(feat, np.float32)
         target = np.ascontiguousarray(target, np.float32)
         n = feat.shape[0]
-        assert feat.size == n * self.H * self.W * 2 and target.size == n * self.H * self.W
+        assert feat.size == n * self.pixels * 2 and target.size == n * self.pixels
         if faces is not None:
             faces = np.ascontiguousarray(faces, np.float32).ravel()
-            assert faces.size == feat.size
+            assert faces.size == n * self.pixels * self.ndim
         check(self._lib.scihpc_unet_set_samples(self.h, n, feat.ravel(), target.ravel(), faces))
+
+    def set_loss_smoothing(self, sweeps=0, omega=0.8):
+        """Energy loss of the error after `sweeps` damped-Jacobi sweeps of the
+        network grid's A(rho) (what the multigrid smoother leaves of it)."""
+        check(self._lib.scihpc_unet_set_loss_smoothing(self.h, int(sweeps), float(omega)))
 
     @staticmethod
     def _idx(samples, flips=None):
@@ -277,9 +291,9 @@ class UNet:
 
     def forward(self, samples, flips=None):
         idx = self._idx(samples, flips)
-        out = np.empty(len(samples) * self.H * self.W, np.float32)
+        out = np.empty(len(samples) * self.pixels, np.float32)
         check(self._lib.scihpc_unet_forward(self.h, len(samples), idx, out))
-        return out.reshape(len(samples), self.H * self.W)
+        return out.reshape(len(samples), self.pixels)
 
     def backward(self):
         loss = C.c_double()

@@ -106,6 +106,8 @@ SIGNATURES = {
     "scihpc_mg_attach_learned": [_vp, _vp, _i],
     "scihpc_mg_learned_features": [_vp, _i, _fp, _fpn, _fpn],
     "scihpc_unet_create": [_i, _i, _i, C.c_ulonglong, C.POINTER(_vp)],
+    "scihpc_unet_create3": [_i, _i, _i, _i, C.c_ulonglong, C.POINTER(_vp)],
+    "scihpc_unet_set_loss_smoothing": [_vp, _i, C.c_float],
     "scihpc_unet_destroy": [_vp],
     "scihpc_unet_num_params": [_vp, C.POINTER(C.c_longlong)],
     "scihpc_unet_layer": [_vp, _i, _pi, _pi, _pi, C.POINTER(C.c_longlong)],
@@ -121,6 +123,7 @@ SIGNATURES = {
     "scihpc_unet_sync": [_vp],
     "scihpc_unet_time": [_vp, _i, _i, _i, _pd, _pd],
     "scihpc_conv3x3_forward": [_u16p, _i, _i, _i, _i, _u16p, _fpn, _i, _i, _u16p],
+    "scihpc_conv3x3x3_forward": [_u16p, _i, _i, _i, _i, _i, _u16p, _fpn, _i, _i, _u16p],
     "scihpc_proj_create": [_i, _i, _i, _i, _d, _d, _d, _ip, C.POINTER(_vp)],
     "scihpc_proj_destroy": [_vp],
     "scihpc_proj_fpc_step": [_vp, _vp, C.POINTER(FpcParams)] + [_dp] * 8 + [_pi],

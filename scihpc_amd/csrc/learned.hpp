@@ -10,8 +10,10 @@ struct scihpc_unet;
 
 namespace scihpc {
 
-// out[H*W] = network(feat[H*W][2]) in normalised units, stream ordered
+// out[pixels] = network(feat[pixels][2]) in normalised units, stream ordered
 void unet_infer(scihpc_unet *net, const float *feat, float *out, hipStream_t stream);
+int unet_ndim(const scihpc_unet *net);
+int unet_depth(const scihpc_unet *net);
 int unet_height(const scihpc_unet *net);
 int unet_width(const scihpc_unet *net);
 float unet_out_scale(const scihpc_unet *net);

@@ -29,60 +29,80 @@ constexpr int kLayers = 14;
 constexpr int kHeadC = 16;  // channels entering the 1x1 head
 
 // ------------------------------------------------------- elementwise
-// dst (coarse h x w) = 2x2 average of src (2h x 2w); thread per (pixel, channel)
-__global__ void __launch_bounds__(kEB) k_pool2(View src, View dst, int n, int h, int w, int c) {
-  const long e = (long)blockIdx.x * kEB + threadIdx.x;
-  if (e >= (long)n * h * w * c) return;
-  const int ch = (int)(e % c);
-  const long p = e / c;
-  const int x = (int)(p % w), y = (int)((p / w) % h), img = (int)(p / ((long)w * h));
-  const long f = ((long)img * 2 * h + 2 * y) * (2 * w) + 2 * x;
-  const long W2 = 2 * w;
-  const float s = bf16_to_f(src.p[f * src.stride + src.off + ch]) +
-                  bf16_to_f(src.p[(f + 1) * src.stride + src.off + ch]) +
-                  bf16_to_f(src.p[(f + W2) * src.stride + src.off + ch]) +
-                  bf16_to_f(src.p[(f + W2 + 1) * src.stride + src.off + ch]);
-  dst.p[p * dst.stride + dst.off + ch] = f_to_bf16(0.25f * s);
+// Grids of a level: d x h x w (d = 1 in 2-D); a coarse cell (z, y, x) has
+// the 2^DIM children (2z + a, 2y + b, 2x + c) on the finer level.
+template <int DIM>
+__device__ __forceinline__ long child0(long cp, int d, int h, int w) {  // first child of coarse pixel cp
+  const int x = (int)(cp % w), y = (int)((cp / w) % h);
+  const long r = cp / ((long)w * h);  // img * d + z
+  const int z = (int)(r % d), img = (int)(r / d);
+  const int D2 = DIM == 3 ? 2 * d : 1, z2 = DIM == 3 ? 2 * z : 0;
+  return (((long)img * D2 + z2) * (2 * h) + 2 * y) * (2 * w) + 2 * x;
+}
+template <int DIM>
+__device__ __forceinline__ long parent(long fp, int d, int h, int w) {  // coarse pixel of fine pixel fp
+  const int W2 = 2 * w, H2 = 2 * h, D2 = DIM == 3 ? 2 * d : 1;
+  const int x = (int)(fp % W2), y = (int)((fp / W2) % H2);
+  const long r = fp / ((long)W2 * H2);
+  const int z = (int)(r % D2), img = (int)(r / D2);
+  return (((long)img * d + (DIM == 3 ? z / 2 : 0)) * h + y / 2) * w + x / 2;
+}
+template <int DIM>
+__device__ __forceinline__ long child(long f0, int k, int h, int w) {  // k-th child (bits: x, y, z)
+  return f0 + (k & 1) + ((k >> 1) & 1) * (2L * w) + (DIM == 3 ? ((k >> 2) & 1) * (4L * h * w) : 0L);
 }
 
-// gf (fine, 2h x 2w) += gc[parent] / 4: gradient of k_pool2 added to the
-// gradient that the same tensor already received through its skip
-__global__ void __launch_bounds__(kEB) k_pool2_bwd_add(View gf, View gc, int n, int h, int w, int c) {
+// dst (coarse d x h x w) = mean of the children in src; thread per (pixel, channel)
+template <int DIM>
+__global__ void __launch_bounds__(kEB) k_pool2(View src, View dst, int n, int d, int h, int w, int c) {
   const long e = (long)blockIdx.x * kEB + threadIdx.x;
-  const long W2 = 2 * w, H2 = 2 * h;
-  if (e >= (long)n * H2 * W2 * c) return;
+  if (e >= (long)n * d * h * w * c) return;
   const int ch = (int)(e % c);
   const long p = e / c;
-  const int x = (int)(p % W2), y = (int)((p / W2) % H2), img = (int)(p / (W2 * H2));
-  const long pc = ((long)img * h + y / 2) * w + x / 2;
-  const float v = bf16_to_f(gf.p[p * gf.stride + gf.off + ch]) + 0.25f * bf16_to_f(gc.p[pc * gc.stride + gc.off + ch]);
+  const long f0 = child0<DIM>(p, d, h, w);
+  float s = 0.f;
+#pragma unroll
+  for (int k = 0; k < (1 << DIM); ++k) s += bf16_to_f(src.p[child<DIM>(f0, k, h, w) * src.stride + src.off + ch]);
+  dst.p[p * dst.stride + dst.off + ch] = f_to_bf16(s * (1.f / (1 << DIM)));
+}
+
+// gf (fine) += gc[parent] / 2^DIM: gradient of k_pool2 added to the
+// gradient that the same tensor already received through its skip
+template <int DIM>
+__global__ void __launch_bounds__(kEB) k_pool2_bwd_add(View gf, View gc, int n, int d, int h, int w, int c) {
+  const long e = (long)blockIdx.x * kEB + threadIdx.x;
+  const long nf = (long)n * (DIM == 3 ? 2 * d : 1) * (2 * h) * (2 * w);
+  if (e >= nf * c) return;
+  const int ch = (int)(e % c);
+  const long p = e / c;
+  const long pc = parent<DIM>(p, d, h, w);
+  const float v = bf16_to_f(gf.p[p * gf.stride + gf.off + ch]) +
+                  (1.f / (1 << DIM)) * bf16_to_f(gc.p[pc * gc.stride + gc.off + ch]);
   gf.p[p * gf.stride + gf.off + ch] = f_to_bf16(v);
 }
 
-// dst (fine 2h x 2w) = src[parent] (nearest upsampling)
-__global__ void __launch_bounds__(kEB) k_up2(View src, View dst, int n, int h, int w, int c) {
+// dst (fine) = src[parent] (nearest upsampling)
+template <int DIM>
+__global__ void __launch_bounds__(kEB) k_up2(View src, View dst, int n, int d, int h, int w, int c) {
   const long e = (long)blockIdx.x * kEB + threadIdx.x;
-  const long W2 = 2 * w, H2 = 2 * h;
-  if (e >= (long)n * H2 * W2 * c) return;
+  const long nf = (long)n * (DIM == 3 ? 2 * d : 1) * (2 * h) * (2 * w);
+  if (e >= nf * c) return;
   const int ch = (int)(e % c);
   const long p = e / c;
-  const int x = (int)(p % W2), y = (int)((p / W2) % H2), img = (int)(p / (W2 * H2));
-  const long pc = ((long)img * h + y / 2) * w + x / 2;
-  dst.p[p * dst.stride + dst.off + ch] = src.p[pc * src.stride + src.off + ch];
+  dst.p[p * dst.stride + dst.off + ch] = src.p[parent<DIM>(p, d, h, w) * src.stride + src.off + ch];
 }
 
-// gc (coarse) = sum of the 4 children of gf: gradient of k_up2
-__global__ void __launch_bounds__(kEB) k_up2_bwd(View gf, View gc, int n, int h, int w, int c) {
+// gc (coarse) = sum over the children of gf: gradient of k_up2
+template <int DIM>
+__global__ void __launch_bounds__(kEB) k_up2_bwd(View gf, View gc, int n, int d, int h, int w, int c) {
   const long e = (long)blockIdx.x * kEB + threadIdx.x;
-  if (e >= (long)n * h * w * c) return;
+  if (e >= (long)n * d * h * w * c) return;
   const int ch = (int)(e % c);
   const long p = e / c;
-  const int x = (int)(p % w), y = (int)((p / w) % h), img = (int)(p / ((long)w * h));
-  const long f = ((long)img * 2 * h + 2 * y) * (2 * w) + 2 * x;
-  const long W2 = 2 * w;
-  const float s = bf16_to_f(gf.p[f * gf.stride + gf.off + ch]) + bf16_to_f(gf.p[(f + 1) * gf.stride + gf.off + ch]) +
-                  bf16_to_f(gf.p[(f + W2) * gf.stride + gf.off + ch]) +
-                  bf16_to_f(gf.p[(f + W2 + 1) * gf.stride + gf.off + ch]);
+  const long f0 = child0<DIM>(p, d, h, w);
+  float s = 0.f;
+#pragma unroll
+  for (int k = 0; k < (1 << DIM); ++k) s += bf16_to_f(gf.p[child<DIM>(f0, k, h, w) * gf.stride + gf.off + ch]);
   gc.p[p * gc.stride + gc.off + ch] = f_to_bf16(s);
 }
 
@@ -113,36 +133,102 @@ __global__ void __launch_bounds__(kEB) k_mse(const float *__restrict__ out, cons
 
 // energy loss of one image: E(e) / E(t) with e = out - tgt and
 // E(v) = sum over faces c_f (v_i - v_j)^2 (the discrete A(rho) energy on the
-// network's grid; face[p] = (c to x+1, c to y+1), zero on walls);
-// dout = 2 sum_f c_f (e_p - e_nb) / E(t) / n
+// network's grid; face[p] = (c to x+1, c to y+1 (, c to z+1)), zero on
+// walls); dout = 2 sum_f c_f (e_p - e_nb) / E(t) / n
+template <int DIM>
 __global__ void __launch_bounds__(kEB) k_energy_loss(const float *__restrict__ out, const float *__restrict__ tgt,
                                                      const float *__restrict__ face, const float *__restrict__ et,
-                                                     float *__restrict__ dout, int n, int h, int w,
+                                                     float *__restrict__ dout, int n, int d, int h, int w,
                                                      double *__restrict__ part) {
   const long p = (long)blockIdx.x * kEB + threadIdx.x;
   double v = 0.0;
-  if (p < (long)n * h * w) {
-    const int x = (int)(p % w), y = (int)((p / w) % h), img = (int)(p / ((long)w * h));
-    const float e = out[p] - tgt[p];
+  if (p < (long)n * d * h * w) {
+    const int x = (int)(p % w), y = (int)((p / w) % h);
+    const long r = p / ((long)w * h);
+    const int z = (int)(r % d), img = (int)(r / d);
+    auto err = [&](long q) { return tgt ? out[q] - tgt[q] : out[q]; };
+    const float e = err(p);
     const float inv = 1.f / (et[img] * (float)n);
     float g = 0.f;
-    const float cR = face[2 * p], cU = face[2 * p + 1];
-    if (x + 1 < w) {
-      const float d = e - (out[p + 1] - tgt[p + 1]);
-      g += cR * d;
-      v += (double)cR * d * d * inv;
+    const int cnt[3] = {w, h, d};
+    const int pos[3] = {x, y, z};
+    const long step[3] = {1, (long)w, (long)w * h};
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) {
+      if (pos[a] + 1 < cnt[a]) {
+        const float c = face[DIM * p + a];
+        const float dd = e - err(p + step[a]);
+        g += c * dd;
+        v += (double)c * dd * dd * inv;
+      }
+      if (pos[a] > 0) g += face[DIM * (p - step[a]) + a] * (e - err(p - step[a]));
     }
-    if (y + 1 < h) {
-      const float d = e - (out[p + w] - tgt[p + w]);
-      g += cU * d;
-      v += (double)cU * d * d * inv;
-    }
-    if (x > 0) g += face[2 * (p - 1)] * (e - (out[p - 1] - tgt[p - 1]));
-    if (y > 0) g += face[2 * (p - w) + 1] * (e - (out[p - w] - tgt[p - w]));
     dout[p] = 2.f * g * inv;
   }
   v = block_sum(v);
   if (threadIdx.x == 0) part[blockIdx.x] = v;
+}
+
+// One damped-Jacobi sweep of the network grid's A(rho) (the energy form
+// E(v) = v.A v over the faces), applied to an error or, transposed, to a
+// gradient:  M e = e - omega D^-1 A e,  M^T g = g - omega A (D^-1 g),
+// (A v)_p = sum over p's faces c_f (v_p - v_nb), D_p = sum of p's faces.
+// With smoothing sweeps S the energy loss is E(M^S e) / E(t): the error the
+// multigrid smoother leaves of the network's output, so the network is not
+// asked for high-frequency accuracy that the smoother provides anyway.
+template <int DIM>
+__device__ __forceinline__ float diag_of(const float *__restrict__ face, long p, const int (&pos)[3],
+                                         const int (&cnt)[3], const long (&step)[3]) {
+  float dsum = 0.f;
+#pragma unroll
+  for (int a = 0; a < DIM; ++a) {
+    if (pos[a] + 1 < cnt[a]) dsum += face[DIM * p + a];
+    if (pos[a] > 0) dsum += face[DIM * (p - step[a]) + a];
+  }
+  return dsum;
+}
+template <int DIM>
+__global__ void __launch_bounds__(kEB) k_jacobi(const float *__restrict__ v, float *__restrict__ o,
+                                                const float *__restrict__ face, int n, int d, int h, int w,
+                                                float omega, int transpose) {
+  const long p = (long)blockIdx.x * kEB + threadIdx.x;
+  if (p >= (long)n * d * h * w) return;
+  const int pos[3] = {(int)(p % w), (int)((p / w) % h), (int)((p / ((long)w * h)) % d)};
+  const int cnt[3] = {w, h, d};
+  const long step[3] = {1, (long)w, (long)w * h};
+  const float dp = diag_of<DIM>(face, p, pos, cnt, step);
+  float acc = 0.f;
+  if (!transpose) {
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) {
+      if (pos[a] + 1 < cnt[a]) acc += face[DIM * p + a] * (v[p] - v[p + step[a]]);
+      if (pos[a] > 0) acc += face[DIM * (p - step[a]) + a] * (v[p] - v[p - step[a]]);
+    }
+    o[p] = dp > 0.f ? v[p] - omega * acc / dp : v[p];
+  } else {
+    const float up = dp > 0.f ? v[p] / dp : 0.f;
+#pragma unroll
+    for (int a = 0; a < DIM; ++a)
+#pragma unroll
+      for (int sgn = 0; sgn < 2; ++sgn) {
+        const bool has = sgn ? pos[a] > 0 : pos[a] + 1 < cnt[a];
+        if (!has) continue;
+        const long q = sgn ? p - step[a] : p + step[a];
+        int qpos[3] = {pos[0], pos[1], pos[2]};
+        qpos[a] += sgn ? -1 : 1;
+        const float dq = diag_of<DIM>(face, q, qpos, cnt, step);
+        const float uq = dq > 0.f ? v[q] / dq : 0.f;
+        acc += face[DIM * (sgn ? q : p) + a] * (up - uq);
+      }
+    o[p] = v[p] - omega * acc;
+  }
+}
+
+// e = out - tgt
+__global__ void __launch_bounds__(kEB) k_sub(const float *__restrict__ a, const float *__restrict__ b,
+                                             float *__restrict__ o, long n) {
+  const long p = (long)blockIdx.x * kEB + threadIdx.x;
+  if (p < n) o[p] = a[p] - b[p];
 }
 
 // head backward: gx[p][c] = dout[p] w[c] (BF16); part[block][c] = sum_p dout x[p][c], part[block][16] = sum dout
@@ -206,41 +292,48 @@ __global__ void __launch_bounds__(kEB) k_to_bf16(const float *__restrict__ a, bf
   if (i < n) b[i] = f_to_bf16(a[i]);
 }
 
-// Wt[ci][8 - t][co] = W[co][t][ci]: the backward-data convolution is the
+// Wt[ci][T-1-t][co] = W[co][t][ci]: the backward-data convolution is the
 // forward kernel on the gradient with these weights
 __global__ void __launch_bounds__(kEB) k_flip(const bf16_t *__restrict__ W, bf16_t *__restrict__ Wt, int cin,
-                                              int cout) {
+                                              int cout, int T) {
   const long e = (long)blockIdx.x * kEB + threadIdx.x;
-  if (e >= (long)cout * 9 * cin) return;
+  if (e >= (long)cout * T * cin) return;
   const int ci = (int)(e % cin);
-  const int t = (int)((e / cin) % 9);
-  const int co = (int)(e / ((long)cin * 9));
-  Wt[((long)ci * 9 + (8 - t)) * cout + co] = W[e];
+  const int t = (int)((e / cin) % T);
+  const int co = (int)(e / ((long)cin * T));
+  Wt[((long)ci * T + (T - 1 - t)) * cout + co] = W[e];
 }
 
 // network input: channels (feat0, feat1, 0 ... 0) from FP32 features
-// [sample][pixel][2]; optional sample indices and x/y flips per batch entry;
-// optional targets tgt = store_tgt[sample][pixel] * inv_scale and faces of the
-// mirrored grid (a face between cells x and x+1 maps to W-2-x under an x
-// mirror; the wall face stays zero) with the per-image energy norm
+// [sample][pixel][2]; optional sample indices and mirror bits per batch
+// entry (1 = x, 2 = y, 4 = z); optional targets tgt =
+// store_tgt[sample][pixel] * inv_scale and faces of the mirrored grid (a
+// face between cells x and x+1 maps to W-2-x under an x mirror; the wall
+// face stays zero) with the per-image energy norm
+template <int DIM>
 __global__ void __launch_bounds__(kEB) k_make_input(const float *__restrict__ feat, const float *__restrict__ tgt_store,
                                                     const float *__restrict__ face_store,
                                                     const float *__restrict__ e_store, const int *__restrict__ idx,
                                                     bf16_t *__restrict__ in, float *__restrict__ tgt,
-                                                    float *__restrict__ face, float *__restrict__ et, int n, int h,
-                                                    int w, float inv_scale) {
+                                                    float *__restrict__ face, float *__restrict__ et, int n, int d,
+                                                    int h, int w, float inv_scale) {
   const long p = (long)blockIdx.x * kEB + threadIdx.x;
-  if (p >= (long)n * h * w) return;
-  const int x = (int)(p % w), y = (int)((p / w) % h), img = (int)(p / ((long)w * h));
+  if (p >= (long)n * d * h * w) return;
+  const int x = (int)(p % w), y = (int)((p / w) % h);
+  const long r = p / ((long)w * h);
+  const int z = (int)(r % d), img = (int)(r / d);
   int s = img, fl = 0;
   if (idx) {
     s = idx[2 * img];
     fl = idx[2 * img + 1];
   }
-  const bool fx = fl & 1, fy = fl & 2;
-  const int sx = fx ? w - 1 - x : x, sy = fy ? h - 1 - y : y;
-  const long base = (long)s * h * w;
-  const long q = base + (long)sy * w + sx;
+  const int pos[3] = {x, y, z}, cnt[3] = {w, h, d};
+  int src[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) src[a] = (fl >> a) & 1 ? cnt[a] - 1 - pos[a] : pos[a];
+  const long base = (long)s * d * h * w;
+  auto at = [&](int sx, int sy, int sz) { return base + ((long)sz * h + sy) * w + sx; };
+  const long q = at(src[0], src[1], src[2]);
   bf16_t *o = in + p * 16;
   o[0] = f_to_bf16(feat[2 * q]);
   o[1] = f_to_bf16(feat[2 * q + 1]);
@@ -248,28 +341,50 @@ __global__ void __launch_bounds__(kEB) k_make_input(const float *__restrict__ fe
   for (int c = 2; c < 16; ++c) o[c] = 0;
   if (tgt) tgt[p] = tgt_store[q] * inv_scale;
   if (face) {
-    float cx = 0.f, cy = 0.f;
-    // x face to the right of (x, y) in the mirrored image
-    if (x + 1 < w) {
-      const int fxs = fx ? w - 2 - x : x;
-      cx = face_store[2 * (base + (long)sy * w + fxs)];
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) {
+      // face toward +1 along axis a of (x, y, z) in the mirrored image
+      float c = 0.f;
+      if (pos[a] + 1 < cnt[a]) {
+        int sf[3] = {src[0], src[1], src[2]};
+        if ((fl >> a) & 1) sf[a] = cnt[a] - 2 - pos[a];
+        c = face_store[DIM * at(sf[0], sf[1], sf[2]) + a];
+      }
+      face[DIM * p + a] = c;
     }
-    if (y + 1 < h) {
-      const int fys = fy ? h - 2 - y : y;
-      cy = face_store[2 * (base + (long)fys * w + sx) + 1];
-    }
-    face[2 * p] = cx;
-    face[2 * p + 1] = cy;
-    if (x == 0 && y == 0) et[img] = e_store[s];
+    if (x == 0 && y == 0 && z == 0) et[img] = e_store[s];
   }
 }
 
 inline unsigned nblk(long n) { return ceil_div(n, kEB); }
 
+// y = act(conv(x, w) + b) over s (2-D or 3-D); NB = s.cout / 16 output blocks
+template <int DIM>
+void launch_conv_dim(const ConvShape &s, View x, View mask, const bf16_t *w, const float *b, View y, int relu,
+                     hipStream_t st) {
+  const unsigned grid = (unsigned)((long)s.n * s.d * s.h * ((s.w + 63) / 64));
+  switch (s.cout / 16) {
+    case 1: k_conv<1, DIM><<<grid, kConvThreads, 0, st>>>(x, mask, w, b, y, s, relu); break;
+    case 2: k_conv<2, DIM><<<grid, kConvThreads, 0, st>>>(x, mask, w, b, y, s, relu); break;
+    case 3: k_conv<3, DIM><<<grid, kConvThreads, 0, st>>>(x, mask, w, b, y, s, relu); break;
+    case 4: k_conv<4, DIM><<<grid, kConvThreads, 0, st>>>(x, mask, w, b, y, s, relu); break;
+    case 6: k_conv<6, DIM><<<grid, kConvThreads, 0, st>>>(x, mask, w, b, y, s, relu); break;
+    case 8: k_conv<8, DIM><<<grid, kConvThreads, 0, st>>>(x, mask, w, b, y, s, relu); break;
+    case 12: k_conv<12, DIM><<<grid, kConvThreads, 0, st>>>(x, mask, w, b, y, s, relu); break;
+    default: throw Error(SCIHPC_BAD_ARG, "unsupported convolution width");
+  }
+  SCIHPC_CHECK_LAUNCH();
+}
+inline void launch_conv(int ndim, const ConvShape &s, View x, View mask, const bf16_t *w, const float *b, View y,
+                        int relu, hipStream_t st) {
+  if (ndim == 3) launch_conv_dim<3>(s, x, mask, w, b, y, relu, st);
+  else launch_conv_dim<2>(s, x, mask, w, b, y, relu, st);
+}
+
 struct Layer {
-  int cin, cout, lev;
-  long off;  // w at off, b at off + cout*9*cin
-  long nw() const { return (long)cout * 9 * cin; }
+  int cin, cout, lev, taps;
+  long off;  // w at off, b at off + cout*taps*cin
+  long nw() const { return (long)cout * taps * cin; }
   long size() const { return nw() + cout; }
 };
 
@@ -279,7 +394,8 @@ struct Layer {
 using namespace scihpc;
 
 struct scihpc_unet {
-  int H = 0, W = 0, nmax = 0;
+  int ndim = 2, T = 9;  // 2-D: 3x3 convolutions; 3-D: 3x3x3
+  int D = 1, H = 0, W = 0, nmax = 0;
   hipStream_t stream = nullptr;
   Layer L[kLayers];
   long head_off = 0, np = 0;
@@ -298,11 +414,15 @@ struct scihpc_unet {
   float *store_feat = nullptr, *store_tgt = nullptr, *store_face = nullptr, *store_e = nullptr;
   int store_n = 0;
   bool energy = false;  // loss: energy norm of the error (faces given) or mean squared error
+  int smooth = 0;       // damped-Jacobi sweeps applied to the error inside the energy loss
+  float omega = 0.8f;
+  float *sm0 = nullptr, *sm1 = nullptr;  // smoothed errors / their gradients
   float *bface = nullptr, *be = nullptr;  // faces and target energy of the current batch
   int n_fwd = 0;  // batch of the last forward pass
   std::vector<void *> allocs;
 
-  long pix(int lev, int n) const { return (long)n * (H >> lev) * (W >> lev); }
+  int dz(int lev) const { return ndim == 3 ? D >> lev : 1; }
+  long pix(int lev, int n) const { return (long)n * dz(lev) * (H >> lev) * (W >> lev); }
 
   template <typename T>
   T *alloc(long n) {
@@ -314,9 +434,15 @@ struct scihpc_unet {
   }
   bf16_t *act(int lev, int c) { return alloc<bf16_t>(pix(lev, nmax) * c); }
 
-  void create(int h, int w, int n, unsigned long long seed) {
-    SCIHPC_REQUIRE(h >= 8 && w >= 8 && h % 8 == 0 && w % 8 == 0, "U-Net size must be a multiple of 8, >= 8");
-    SCIHPC_REQUIRE(n >= 1 && (long)n * h * w <= (1L << 26), "batch x pixels must be in [1, 2^26]");
+  void create(int nd, int d, int h, int w, int n, unsigned long long seed) {
+    SCIHPC_REQUIRE(nd == 2 || nd == 3, "the U-Net is 2-D or 3-D");
+    if (nd == 2) d = 1;
+    SCIHPC_REQUIRE(h >= 8 && w >= 8 && h % 8 == 0 && w % 8 == 0 && (nd == 2 || (d >= 8 && d % 8 == 0)),
+                   "U-Net size must be a multiple of 8, >= 8");
+    SCIHPC_REQUIRE(n >= 1 && (long)n * d * h * w <= (1L << 26), "batch x pixels must be in [1, 2^26]");
+    ndim = nd;
+    T = nd == 3 ? 27 : 9;
+    D = d;
     H = h;
     W = w;
     nmax = n;
@@ -326,7 +452,7 @@ struct scihpc_unet {
                                   {96, 32, 1},  {32, 32, 1}, {48, 16, 0}, {16, 16, 0}};
     long off = 0;
     for (int i = 0; i < kLayers; ++i) {
-      L[i] = Layer{spec[i][0], spec[i][1], spec[i][2], off};
+      L[i] = Layer{spec[i][0], spec[i][1], spec[i][2], T, off};
       off += L[i].size();
     }
     head_off = off;
@@ -348,9 +474,11 @@ struct scihpc_unet {
     out = alloc<float>(pix(0, nmax));
     tgt = alloc<float>(pix(0, nmax));
     dout = alloc<float>(pix(0, nmax));
+    sm0 = alloc<float>(pix(0, nmax));
+    sm1 = alloc<float>(pix(0, nmax));
     lpart = alloc<double>(nblk(pix(0, nmax)));
     d_idx = alloc<int>(2 * nmax);
-    bface = alloc<float>(2 * pix(0, nmax));
+    bface = alloc<float>((long)ndim * pix(0, nmax));
     be = alloc<float>(nmax);
     part_n = (long)nblk(pix(0, nmax)) * (kHeadC + 1);
     for (int i = 0; i < kLayers; ++i) {
@@ -365,26 +493,26 @@ struct scihpc_unet {
   }
 
   void destroy() {
-    if (stream) hipStreamSynchronize(stream);
+    if (stream) (void)hipStreamSynchronize(stream);
     for (float *b : {store_feat, store_tgt, store_face, store_e})
-      if (b) hipFree(b);
+      if (b) (void)hipFree(b);
     store_feat = store_tgt = store_face = store_e = nullptr;
-    for (void *p : allocs) hipFree(p);
+    for (void *p : allocs) (void)hipFree(p);
     allocs.clear();
-    if (host_l) hipHostFree(host_l);
+    if (host_l) (void)hipHostFree(host_l);
     host_l = nullptr;
-    if (stream) hipStreamDestroy(stream);
+    if (stream) (void)hipStreamDestroy(stream);
     stream = nullptr;
   }
 
-  // He-normal 3x3 weights (fan-in 9 cin; the first layer's fan-in counts its
+  // He-normal weights (fan-in T cin; the first layer's fan-in counts its
   // two real input channels), zero biases, a small head
   void init_params(unsigned long long seed) {
     std::vector<float> h(np, 0.f);
     std::mt19937_64 rng(seed);
     std::normal_distribution<double> nd(0.0, 1.0);
     for (int i = 0; i < kLayers; ++i) {
-      const int fan = 9 * (i == 0 ? 2 : L[i].cin);
+      const int fan = T * (i == 0 ? 2 : L[i].cin);
       const double sd = std::sqrt(2.0 / fan);
       for (long e = 0; e < L[i].nw(); ++e) {
         const int ci = (int)(e % L[i].cin);
@@ -398,7 +526,7 @@ struct scihpc_unet {
   void refresh_derived() {
     k_to_bf16<<<nblk(np), kEB, 0, stream>>>(P, Pb, np);
     for (int i = 0; i < kLayers; ++i)
-      k_flip<<<nblk(L[i].nw()), kEB, 0, stream>>>(Pb + L[i].off, Pt + L[i].off, L[i].cin, L[i].cout);
+      k_flip<<<nblk(L[i].nw()), kEB, 0, stream>>>(Pb + L[i].off, Pt + L[i].off, L[i].cin, L[i].cout, T);
     SCIHPC_CHECK_LAUNCH();
   }
   void set_params(const float *host) {
@@ -411,48 +539,28 @@ struct scihpc_unet {
   }
 
   // ------------------------------------------------------- launches
-  ConvShape shape(int i, int n) const { return ConvShape{n, H >> L[i].lev, W >> L[i].lev, L[i].cin, L[i].cout}; }
+  ConvShape shape(int i, int n) const {
+    return ConvShape{n, dz(L[i].lev), H >> L[i].lev, W >> L[i].lev, L[i].cin, L[i].cout};
+  }
+  static unsigned conv_grid(const ConvShape &s) { return (unsigned)((long)s.n * s.d * s.h * ((s.w + 63) / 64)); }
 
   void conv_fwd(int i, int n, View x, View y, hipStream_t st) {
     const ConvShape s = shape(i, n);
-    const unsigned grid = (unsigned)((long)n * s.h * ((s.w + 63) / 64));
-    const View none{nullptr, 0, 0};
-    const bf16_t *w = Pb + L[i].off;
-    const float *b = P + L[i].off + L[i].nw();
-    switch (s.cout / 16) {
-      case 1: k_conv3x3<1><<<grid, kConvThreads, 0, st>>>(x, none, w, b, y, s, 1); break;
-      case 2: k_conv3x3<2><<<grid, kConvThreads, 0, st>>>(x, none, w, b, y, s, 1); break;
-      case 4: k_conv3x3<4><<<grid, kConvThreads, 0, st>>>(x, none, w, b, y, s, 1); break;
-      case 8: k_conv3x3<8><<<grid, kConvThreads, 0, st>>>(x, none, w, b, y, s, 1); break;
-      default: throw Error(SCIHPC_BAD_ARG, "unsupported output width");
-    }
-    SCIHPC_CHECK_LAUNCH();
+    launch_conv(ndim, s, x, View{nullptr, 0, 0}, Pb + L[i].off, P + L[i].off + L[i].nw(), y, 1, st);
   }
   // gx = conv^T(gy * (y > 0))
   void conv_bwd_data(int i, int n, View gy, View y, View gx) {
     ConvShape s = shape(i, n);
     std::swap(s.cin, s.cout);
-    const unsigned grid = (unsigned)((long)n * s.h * ((s.w + 63) / 64));
-    const bf16_t *w = Pt + L[i].off;
-    switch (s.cout / 16) {
-      case 1: k_conv3x3<1><<<grid, kConvThreads, 0, stream>>>(gy, y, w, nullptr, gx, s, 0); break;
-      case 2: k_conv3x3<2><<<grid, kConvThreads, 0, stream>>>(gy, y, w, nullptr, gx, s, 0); break;
-      case 3: k_conv3x3<3><<<grid, kConvThreads, 0, stream>>>(gy, y, w, nullptr, gx, s, 0); break;
-      case 4: k_conv3x3<4><<<grid, kConvThreads, 0, stream>>>(gy, y, w, nullptr, gx, s, 0); break;
-      case 6: k_conv3x3<6><<<grid, kConvThreads, 0, stream>>>(gy, y, w, nullptr, gx, s, 0); break;
-      case 8: k_conv3x3<8><<<grid, kConvThreads, 0, stream>>>(gy, y, w, nullptr, gx, s, 0); break;
-      case 12: k_conv3x3<12><<<grid, kConvThreads, 0, stream>>>(gy, y, w, nullptr, gx, s, 0); break;
-      default: throw Error(SCIHPC_BAD_ARG, "unsupported input width");
-    }
-    SCIHPC_CHECK_LAUNCH();
+    launch_conv(ndim, s, gy, y, Pt + L[i].off, nullptr, gx, 0, stream);
   }
   void wgrad_chunks(int i, int n, int &chunks, int &cpix) const {
     const long npix = pix(L[i].lev, n);
-    const int pairs = (L[i].cout / 16) * (L[i].cin / 16);
+    const int pairs = (L[i].cout / 16) * (L[i].cin / 16) * (T / 9);
     long c = std::max(1, std::min(64, (1024 + pairs - 1) / pairs));
-    c = std::min(c, std::max(1L, npix / 64));
+    c = std::min(c, std::max(1L, npix / 128));
     long cp = (npix + c - 1) / c;
-    cp = (cp + 63) / 64 * 64;
+    cp = (cp + 127) / 128 * 128;
     chunks = (int)((npix + cp - 1) / cp);
     cpix = (int)cp;
   }
@@ -460,8 +568,9 @@ struct scihpc_unet {
     const ConvShape s = shape(i, n);
     int chunks, cpix;
     wgrad_chunks(i, n, chunks, cpix);
-    const int pairs = (L[i].cout / 16) * (L[i].cin / 16);
-    k_conv3x3_wgrad<<<(unsigned)(pairs * chunks), kConvThreads, 0, stream>>>(x, gy, y, part, s, cpix);
+    const unsigned grid = (unsigned)((L[i].cout / 16) * (L[i].cin / 16) * (T / 9) * chunks);
+    if (ndim == 3) k_conv_wgrad<3><<<grid, kConvThreads, 0, stream>>>(x, gy, y, part, s, cpix);
+    else k_conv_wgrad<2><<<grid, kConvThreads, 0, stream>>>(x, gy, y, part, s, cpix);
     k_sum_rows<<<nblk(L[i].size()), kEB, 0, stream>>>(part, chunks, L[i].size(), G + L[i].off);
     SCIHPC_CHECK_LAUNCH();
   }
@@ -470,13 +579,31 @@ struct scihpc_unet {
     if (gx.p) conv_bwd_data(i, n, gy, y, gx);
   }
   void pool(View src, View dst, int lev, int n, int c, hipStream_t st) {  // lev = coarse level
-    const int h = H >> lev, w = W >> lev;
-    k_pool2<<<nblk((long)n * h * w * c), kEB, 0, st>>>(src, dst, n, h, w, c);
+    const int d = dz(lev), h = H >> lev, w = W >> lev;
+    const unsigned g = nblk((long)n * d * h * w * c);
+    if (ndim == 3) k_pool2<3><<<g, kEB, 0, st>>>(src, dst, n, d, h, w, c);
+    else k_pool2<2><<<g, kEB, 0, st>>>(src, dst, n, d, h, w, c);
     SCIHPC_CHECK_LAUNCH();
   }
   void up(View src, View dst, int lev, int n, int c, hipStream_t st) {  // lev = coarse level
-    const int h = H >> lev, w = W >> lev;
-    k_up2<<<nblk((long)n * 4 * h * w * c), kEB, 0, st>>>(src, dst, n, h, w, c);
+    const int d = dz(lev), h = H >> lev, w = W >> lev;
+    const unsigned g = nblk(pix(lev - 1, n) * c);
+    if (ndim == 3) k_up2<3><<<g, kEB, 0, st>>>(src, dst, n, d, h, w, c);
+    else k_up2<2><<<g, kEB, 0, st>>>(src, dst, n, d, h, w, c);
+    SCIHPC_CHECK_LAUNCH();
+  }
+  void up_bwd(View gf, View gc, int lev, int n, int c) {  // lev = coarse level
+    const int d = dz(lev), h = H >> lev, w = W >> lev;
+    const unsigned g = nblk(pix(lev, n) * c);
+    if (ndim == 3) k_up2_bwd<3><<<g, kEB, 0, stream>>>(gf, gc, n, d, h, w, c);
+    else k_up2_bwd<2><<<g, kEB, 0, stream>>>(gf, gc, n, d, h, w, c);
+    SCIHPC_CHECK_LAUNCH();
+  }
+  void pool_bwd_add(View gf, View gc, int lev, int n, int c) {  // lev = coarse level
+    const int d = dz(lev), h = H >> lev, w = W >> lev;
+    const unsigned g = nblk(pix(lev - 1, n) * c);
+    if (ndim == 3) k_pool2_bwd_add<3><<<g, kEB, 0, stream>>>(gf, gc, n, d, h, w, c);
+    else k_pool2_bwd_add<2><<<g, kEB, 0, stream>>>(gf, gc, n, d, h, w, c);
     SCIHPC_CHECK_LAUNCH();
   }
 
@@ -516,7 +643,7 @@ struct scihpc_unet {
     const long np0 = pix(0, n);
     const unsigned nb0 = nblk(np0);
     if (energy)
-      k_energy_loss<<<nb0, kEB, 0, stream>>>(out, tgt, bface, be, dout, n, H, W, lpart);
+      energy_loss(n, nb0);
     else
       k_mse<<<nb0, kEB, 0, stream>>>(out, tgt, dout, np0, lpart);
     k_head_bwd<<<nb0, kEB, 0, stream>>>(dout, v(t0d, 16), P + head_off, v(g_t0d, 16), part, np0);
@@ -525,22 +652,22 @@ struct scihpc_unet {
     const View none{nullptr, 0, 0};
     conv_back(13, n, v(t0c, 16), v(t0d, 16), v(g_t0d, 16), v(g_t0c, 16));
     conv_back(12, n, v(cat0, 48), v(t0c, 16), v(g_t0c, 16), v(g_cat0, 48));
-    k_up2_bwd<<<nblk(pix(1, n) * 32), kEB, 0, stream>>>(v(g_cat0, 48, 0), v(g_t1d, 32), n, H >> 1, W >> 1, 32);
+    up_bwd(v(g_cat0, 48, 0), v(g_t1d, 32), 1, n, 32);
     conv_back(11, n, v(t1c, 32), v(t1d, 32), v(g_t1d, 32), v(g_t1c, 32));
     conv_back(10, n, v(cat1, 96), v(t1c, 32), v(g_t1c, 32), v(g_cat1, 96));
-    k_up2_bwd<<<nblk(pix(2, n) * 64), kEB, 0, stream>>>(v(g_cat1, 96, 0), v(g_t2d, 64), n, H >> 2, W >> 2, 64);
+    up_bwd(v(g_cat1, 96, 0), v(g_t2d, 64), 2, n, 64);
     conv_back(9, n, v(t2c, 64), v(t2d, 64), v(g_t2d, 64), v(g_t2c, 64));
     conv_back(8, n, v(cat2, 192), v(t2c, 64), v(g_t2c, 64), v(g_cat2, 192));
-    k_up2_bwd<<<nblk(pix(3, n) * 128), kEB, 0, stream>>>(v(g_cat2, 192, 0), v(g_t3b, 128), n, H >> 3, W >> 3, 128);
+    up_bwd(v(g_cat2, 192, 0), v(g_t3b, 128), 3, n, 128);
     conv_back(7, n, v(t3a, 128), v(t3b, 128), v(g_t3b, 128), v(g_t3a, 128));
     conv_back(6, n, v(p3, 64), v(t3a, 128), v(g_t3a, 128), v(g_p3, 64));
-    k_pool2_bwd_add<<<nblk(pix(2, n) * 64), kEB, 0, stream>>>(v(g_cat2, 192, 128), v(g_p3, 64), n, H >> 3, W >> 3, 64);
+    pool_bwd_add(v(g_cat2, 192, 128), v(g_p3, 64), 3, n, 64);
     conv_back(5, n, v(t2a, 64), v(cat2, 192, 128), v(g_cat2, 192, 128), v(g_t2a, 64));
     conv_back(4, n, v(p2, 32), v(t2a, 64), v(g_t2a, 64), v(g_p2, 32));
-    k_pool2_bwd_add<<<nblk(pix(1, n) * 32), kEB, 0, stream>>>(v(g_cat1, 96, 64), v(g_p2, 32), n, H >> 2, W >> 2, 32);
+    pool_bwd_add(v(g_cat1, 96, 64), v(g_p2, 32), 2, n, 32);
     conv_back(3, n, v(t1a, 32), v(cat1, 96, 64), v(g_cat1, 96, 64), v(g_t1a, 32));
     conv_back(2, n, v(p1, 16), v(t1a, 32), v(g_t1a, 32), v(g_p1, 16));
-    k_pool2_bwd_add<<<nblk(pix(0, n) * 16), kEB, 0, stream>>>(v(g_cat0, 48, 32), v(g_p1, 16), n, H >> 1, W >> 1, 16);
+    pool_bwd_add(v(g_cat0, 48, 32), v(g_p1, 16), 1, n, 16);
     conv_back(1, n, v(t0a, 16), v(cat0, 48, 32), v(g_cat0, 48, 32), v(g_t0a, 16));
     conv_back(0, n, v(in0, 16), v(t0a, 16), v(g_t0a, 16), none);
     SCIHPC_CHECK_LAUNCH();
@@ -552,6 +679,41 @@ struct scihpc_unet {
     return s;
   }
 
+  // energy loss of the last forward pass into lpart and its gradient into
+  // dout, through `smooth` Jacobi sweeps of the error
+  void energy_loss(int n, unsigned nb0) {
+    const int d = dz(0);
+    auto jac = [&](const float *a, float *b, int tr) {
+      if (ndim == 3) k_jacobi<3><<<nb0, kEB, 0, stream>>>(a, b, bface, n, d, H, W, omega, tr);
+      else k_jacobi<2><<<nb0, kEB, 0, stream>>>(a, b, bface, n, d, H, W, omega, tr);
+    };
+    auto loss = [&](const float *o, const float *t, float *g) {
+      if (ndim == 3) k_energy_loss<3><<<nb0, kEB, 0, stream>>>(o, t, bface, be, g, n, d, H, W, lpart);
+      else k_energy_loss<2><<<nb0, kEB, 0, stream>>>(o, t, bface, be, g, n, d, H, W, lpart);
+    };
+    if (smooth == 0) {
+      loss(out, tgt, dout);
+      SCIHPC_CHECK_LAUNCH();
+      return;
+    }
+    // e_0 = out - tgt, e_{k+1} = M e_k (ping-pong through sm0 / sm1)
+    k_sub<<<nb0, kEB, 0, stream>>>(out, tgt, sm0, pix(0, n));
+    float *cur = sm0, *nxt = sm1;
+    for (int k = 0; k < smooth; ++k) {
+      jac(cur, nxt, 0);
+      std::swap(cur, nxt);
+    }
+    // g_S = dE(e_S)/de_S, g_k = M^T g_{k+1}; the last one lands in dout
+    loss(cur, nullptr, nxt);
+    std::swap(cur, nxt);  // cur = g_S
+    for (int k = 0; k < smooth; ++k) {
+      float *dst = k + 1 == smooth ? dout : (cur == sm0 ? sm1 : sm0);
+      jac(cur, dst, 1);
+      cur = dst;
+    }
+    SCIHPC_CHECK_LAUNCH();
+  }
+
   void adam(float lr) {
     ++step;
     const float b1 = 0.9f, b2 = 0.999f;
@@ -560,21 +722,30 @@ struct scihpc_unet {
     k_adam<<<nblk(np), kEB, 0, stream>>>(P, G, M, V, Pb, np, lr, b1, b2, 1e-8f, c1, c2);
     SCIHPC_CHECK_LAUNCH();
     for (int i = 0; i < kLayers; ++i)
-      k_flip<<<nblk(L[i].nw()), kEB, 0, stream>>>(Pb + L[i].off, Pt + L[i].off, L[i].cin, L[i].cout);
+      k_flip<<<nblk(L[i].nw()), kEB, 0, stream>>>(Pb + L[i].off, Pt + L[i].off, L[i].cin, L[i].cout, T);
     SCIHPC_CHECK_LAUNCH();
   }
 
-  // batch from the device sample store; idx = host [n][2] (sample, flip bits)
+  void make_input(const float *feat, const float *tstore, const float *fstore, const float *estore, const int *ix,
+                  float *t, float *f, int n, float inv_scale, hipStream_t st) {
+    if (ndim == 3)
+      k_make_input<3><<<nblk(pix(0, n)), kEB, 0, st>>>(feat, tstore, fstore, estore, ix, in0, t, f, be, n, D, H, W,
+                                                        inv_scale);
+    else
+      k_make_input<2><<<nblk(pix(0, n)), kEB, 0, st>>>(feat, tstore, fstore, estore, ix, in0, t, f, be, n, 1, H, W,
+                                                        inv_scale);
+  }
+  // batch from the device sample store; idx = host [n][2] (sample, mirror bits)
   void load_batch(int n, const int *idx) {
     SCIHPC_REQUIRE(store_feat && store_n > 0, "no training samples stored");
     SCIHPC_REQUIRE(n >= 1 && n <= nmax, "batch exceeds the U-Net's capacity");
     for (int b = 0; b < n; ++b)
-      SCIHPC_REQUIRE(idx[2 * b] >= 0 && idx[2 * b] < store_n && idx[2 * b + 1] >= 0 && idx[2 * b + 1] <= 3,
+      SCIHPC_REQUIRE(idx[2 * b] >= 0 && idx[2 * b] < store_n && idx[2 * b + 1] >= 0 &&
+                         idx[2 * b + 1] < (1 << ndim),
                      "bad sample index or flip code");
     SCIHPC_HIP(hipMemcpyAsync(d_idx, idx, sizeof(int) * 2 * n, hipMemcpyHostToDevice, stream));
-    k_make_input<<<nblk(pix(0, n)), kEB, 0, stream>>>(store_feat, store_tgt, energy ? store_face : nullptr,
-                                                       store_e, d_idx, in0, tgt, energy ? bface : nullptr, be, n,
-                                                       H, W, 1.f / out_scale);
+    make_input(store_feat, store_tgt, energy ? store_face : nullptr, store_e, d_idx, tgt,
+               energy ? bface : nullptr, n, 1.f / out_scale, stream);
     SCIHPC_CHECK_LAUNCH();
   }
 };
@@ -582,25 +753,28 @@ struct scihpc_unet {
 // ------------------------------------------------ multigrid hook
 namespace scihpc {
 void unet_infer(scihpc_unet *net, const float *feat, float *out, hipStream_t stream) {
-  k_make_input<<<nblk(net->pix(0, 1)), kEB, 0, stream>>>(feat, nullptr, nullptr, nullptr, nullptr, net->in0,
-                                                          nullptr, nullptr, nullptr, 1, net->H, net->W, 1.f);
+  net->make_input(feat, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 1, 1.f, stream);
   SCIHPC_CHECK_LAUNCH();
   net->forward(1, stream);
   SCIHPC_HIP(hipMemcpyAsync(out, net->out, sizeof(float) * net->pix(0, 1), hipMemcpyDeviceToDevice, stream));
 }
+int unet_ndim(const scihpc_unet *net) { return net->ndim; }
+int unet_depth(const scihpc_unet *net) { return net->D; }
 int unet_height(const scihpc_unet *net) { return net->H; }
 int unet_width(const scihpc_unet *net) { return net->W; }
 float unet_out_scale(const scihpc_unet *net) { return net->out_scale; }
 }  // namespace scihpc
 
 // ================================================================ C ABI
-extern "C" int scihpc_unet_create(int height, int width, int max_batch, unsigned long long seed, scihpc_unet **out) {
+namespace {
+int unet_create(int ndim, int depth, int height, int width, int max_batch, unsigned long long seed,
+                scihpc_unet **out) {
   return guarded([&] {
     SCIHPC_REQUIRE(out, "null output handle");
     *out = nullptr;
     auto *h = new scihpc_unet();
     try {
-      h->create(height, width, max_batch, seed);
+      h->create(ndim, depth, height, width, max_batch, seed);
     } catch (...) {
       h->destroy();
       delete h;
@@ -608,6 +782,16 @@ extern "C" int scihpc_unet_create(int height, int width, int max_batch, unsigned
     }
     *out = h;
   });
+}
+}  // namespace
+
+extern "C" int scihpc_unet_create(int height, int width, int max_batch, unsigned long long seed, scihpc_unet **out) {
+  return unet_create(2, 1, height, width, max_batch, seed, out);
+}
+
+extern "C" int scihpc_unet_create3(int depth, int height, int width, int max_batch, unsigned long long seed,
+                                   scihpc_unet **out) {
+  return unet_create(3, depth, height, width, max_batch, seed, out);
 }
 
 extern "C" int scihpc_unet_destroy(scihpc_unet *h) {
@@ -680,28 +864,33 @@ extern "C" int scihpc_unet_get_out_scale(scihpc_unet *h, float *s) {
   });
 }
 
-// features: host [n][H*W][2] FP32; targets: host [n][H*W]; faces: host
-// [n][H*W][2] (NULL: mean-squared-error loss, else energy-norm loss)
+// features: host [n][pixels][2] FP32; targets: host [n][pixels]; faces:
+// host [n][pixels][ndim] (NULL: mean-squared-error loss, else energy-norm
+// loss); pixels = (D x) H x W, x fastest
 extern "C" int scihpc_unet_set_samples(scihpc_unet *h, int n, const float *feat, const float *target,
                                        const float *faces) {
   return guarded([&] {
     SCIHPC_REQUIRE(h && feat && target && n >= 1, "bad sample arguments");
-    const long hw = (long)h->H * h->W;
+    const long hw = h->pix(0, 1);
+    const int nd = h->ndim;
     std::vector<float> e;
     if (faces) {
-      // energy norm of each target (flip invariant), FP64 on the host
+      // energy norm of each target (mirror invariant), FP64 on the host
       e.resize(n);
+      const int cnt[3] = {h->W, h->H, h->dz(0)};
+      const long step[3] = {1, (long)h->W, (long)h->W * h->H};
       for (int s = 0; s < n; ++s) {
         double acc = 0.0;
         const float *t = target + (long)s * hw;
-        const float *f = faces + (long)s * hw * 2;
-        for (int y = 0; y < h->H; ++y)
-          for (int x = 0; x < h->W; ++x) {
-            const long p = (long)y * h->W + x;
-            if (x + 1 < h->W) acc += (double)f[2 * p] * ((double)t[p] - t[p + 1]) * ((double)t[p] - t[p + 1]);
-            if (y + 1 < h->H)
-              acc += (double)f[2 * p + 1] * ((double)t[p] - t[p + h->W]) * ((double)t[p] - t[p + h->W]);
-          }
+        const float *f = faces + (long)s * hw * nd;
+        for (long p = 0; p < hw; ++p) {
+          const int pos[3] = {(int)(p % h->W), (int)((p / h->W) % h->H), (int)(p / ((long)h->W * h->H))};
+          for (int a = 0; a < nd; ++a)
+            if (pos[a] + 1 < cnt[a]) {
+              const double dd = (double)t[p] - t[p + step[a]];
+              acc += (double)f[nd * p + a] * dd * dd;
+            }
+        }
         SCIHPC_REQUIRE(acc > 0.0 && std::isfinite(acc), "a target has zero or non-finite energy");
         e[s] = (float)(acc / ((double)h->out_scale * h->out_scale));
       }
@@ -718,9 +907,10 @@ extern "C" int scihpc_unet_set_samples(scihpc_unet *h, int n, const float *feat,
     SCIHPC_HIP(hipMemcpyAsync(h->store_feat, feat, sizeof(float) * 2 * hw * n, hipMemcpyHostToDevice, h->stream));
     SCIHPC_HIP(hipMemcpyAsync(h->store_tgt, target, sizeof(float) * hw * n, hipMemcpyHostToDevice, h->stream));
     if (faces) {
-      SCIHPC_HIP(hipMalloc(reinterpret_cast<void **>(&h->store_face), sizeof(float) * 2 * hw * n));
+      SCIHPC_HIP(hipMalloc(reinterpret_cast<void **>(&h->store_face), sizeof(float) * nd * hw * n));
       SCIHPC_HIP(hipMalloc(reinterpret_cast<void **>(&h->store_e), sizeof(float) * n));
-      SCIHPC_HIP(hipMemcpyAsync(h->store_face, faces, sizeof(float) * 2 * hw * n, hipMemcpyHostToDevice, h->stream));
+      SCIHPC_HIP(
+          hipMemcpyAsync(h->store_face, faces, sizeof(float) * nd * hw * n, hipMemcpyHostToDevice, h->stream));
       SCIHPC_HIP(hipMemcpyAsync(h->store_e, e.data(), sizeof(float) * n, hipMemcpyHostToDevice, h->stream));
     }
     h->energy = faces != nullptr;
@@ -729,8 +919,8 @@ extern "C" int scihpc_unet_set_samples(scihpc_unet *h, int n, const float *feat,
   });
 }
 
-// forward on stored samples idx[n][2] = (sample, flip bits: 1 = x, 2 = y);
-// out (host [n][H*W], NULL to skip) receives the network output
+// forward on stored samples idx[n][2] = (sample, mirror bits: 1 = x, 2 = y,
+// 4 = z); out (host [n][pixels], NULL to skip) receives the network output
 extern "C" int scihpc_unet_forward(scihpc_unet *h, int n, const int *idx, float *out) {
   return guarded([&] {
     SCIHPC_REQUIRE(h && idx, "null argument");
@@ -766,6 +956,18 @@ extern "C" int scihpc_unet_train_step(scihpc_unet *h, int n, const int *idx, flo
   });
 }
 
+// energy loss through `sweeps` damped-Jacobi sweeps (weight omega) of the
+// error on the network's grid (0: the plain energy norm)
+extern "C" int scihpc_unet_set_loss_smoothing(scihpc_unet *h, int sweeps, float omega) {
+  return guarded([&] {
+    SCIHPC_REQUIRE(h, "null handle");
+    SCIHPC_REQUIRE(sweeps >= 0 && sweeps <= 16, "sweeps must be in [0, 16]");
+    SCIHPC_REQUIRE(std::isfinite(omega) && omega > 0.f && omega <= 1.f, "omega must be in (0, 1]");
+    h->smooth = sweeps;
+    h->omega = omega;
+  });
+}
+
 extern "C" int scihpc_unet_sync(scihpc_unet *h) {
   return guarded([&] {
     SCIHPC_REQUIRE(h, "null handle");
@@ -775,9 +977,9 @@ extern "C" int scihpc_unet_sync(scihpc_unet *h) {
 
 // device time of `iters` forward passes (which = 0) or training steps
 // (which = 1: forward, backward, Adam) on stored samples 0..n-1 (HIP events
-// on the network's stream), and the matrix-core FLOPs of one: 2 x 9 x cin x
-// cout per pixel for every 3x3 convolution, doubled in training for the
-// data and weight gradients (the first layer has no data gradient)
+// on the network's stream), and the matrix-core FLOPs of one: 2 x T x cin x
+// cout per pixel for every convolution of T taps, tripled in training for
+// the data and weight gradients (the first layer has no data gradient)
 extern "C" int scihpc_unet_time(scihpc_unet *h, int n, int which, int iters, double *ms, double *flops) {
   return guarded([&] {
     SCIHPC_REQUIRE(h && iters > 0 && (which == 0 || which == 1), "bad timing request");
@@ -789,7 +991,7 @@ extern "C" int scihpc_unet_time(scihpc_unet *h, int n, int which, int iters, dou
     }
     double f = 0.0;
     for (int i = 0; i < kLayers; ++i) {
-      const double c = 2.0 * 9.0 * h->L[i].cin * h->L[i].cout * (double)h->pix(h->L[i].lev, n);
+      const double c = 2.0 * h->T * h->L[i].cin * h->L[i].cout * (double)h->pix(h->L[i].lev, n);
       f += which == 0 ? c : (i == 0 ? 2.0 * c : 3.0 * c);
     }
     h->load_batch(n, idx.data());
@@ -817,20 +1019,22 @@ extern "C" int scihpc_unet_time(scihpc_unet *h, int n, int which, int iters, dou
   });
 }
 
-// ------------------------------------------------ single-kernel entry point
+// ------------------------------------------------ single-kernel entry points
 // (host arrays, copied to and from the device; used by the kernel tests)
-extern "C" int scihpc_conv3x3_forward(const uint16_t *x, int n, int height, int width, int cin, const uint16_t *w,
-                                      const float *b, int cout, int relu, uint16_t *y) {
+namespace {
+int conv_forward(int ndim, const uint16_t *x, int n, int depth, int height, int width, int cin, const uint16_t *w,
+                 const float *b, int cout, int relu, uint16_t *y) {
   return guarded([&] {
-    SCIHPC_REQUIRE(x && w && y && n > 0 && height > 0 && width > 0, "bad convolution arguments");
+    SCIHPC_REQUIRE(x && w && y && n > 0 && depth > 0 && height > 0 && width > 0, "bad convolution arguments");
     SCIHPC_REQUIRE(cin % 16 == 0 && cin >= 16 && cin <= 256, "cin must be a multiple of 16 in [16, 256]");
     const int nb = cout / 16;
     SCIHPC_REQUIRE(cout % 16 == 0 && (nb == 1 || nb == 2 || nb == 3 || nb == 4 || nb == 6 || nb == 8 || nb == 12),
                    "cout must be 16 x {1,2,3,4,6,8,12}");
-    const long npix = (long)n * height * width;
+    const long npix = (long)n * depth * height * width;
     SCIHPC_REQUIRE(npix <= (1L << 26), "too many pixels");
+    const int T = ndim == 3 ? 27 : 9;
     const size_t sx = sizeof(bf16_t) * npix * cin, sy = sizeof(bf16_t) * npix * cout;
-    const size_t sw = sizeof(bf16_t) * (size_t)cout * 9 * cin, sb = sizeof(float) * cout;
+    const size_t sw = sizeof(bf16_t) * (size_t)cout * T * cin, sb = sizeof(float) * cout;
     hipStream_t st = nullptr;
     void *dx = nullptr, *dy = nullptr, *dw = nullptr, *db = nullptr;
     auto cleanup = [&] {
@@ -848,21 +1052,10 @@ extern "C" int scihpc_conv3x3_forward(const uint16_t *x, int n, int height, int 
       SCIHPC_HIP(hipMemcpyAsync(dx, x, sx, hipMemcpyHostToDevice, st));
       SCIHPC_HIP(hipMemcpyAsync(dw, w, sw, hipMemcpyHostToDevice, st));
       if (b) SCIHPC_HIP(hipMemcpyAsync(db, b, sb, hipMemcpyHostToDevice, st));
-      const ConvShape s{n, height, width, cin, cout};
+      const ConvShape s{n, depth, height, width, cin, cout};
       const View xv{static_cast<bf16_t *>(dx), cin, 0}, yv{static_cast<bf16_t *>(dy), cout, 0}, none{nullptr, 0, 0};
-      const bf16_t *wp = static_cast<const bf16_t *>(dw);
-      const float *bp = b ? static_cast<const float *>(db) : nullptr;
-      const unsigned grid = (unsigned)((long)n * height * ((width + 63) / 64));
-      switch (nb) {
-        case 1: k_conv3x3<1><<<grid, kConvThreads, 0, st>>>(xv, none, wp, bp, yv, s, relu); break;
-        case 2: k_conv3x3<2><<<grid, kConvThreads, 0, st>>>(xv, none, wp, bp, yv, s, relu); break;
-        case 3: k_conv3x3<3><<<grid, kConvThreads, 0, st>>>(xv, none, wp, bp, yv, s, relu); break;
-        case 4: k_conv3x3<4><<<grid, kConvThreads, 0, st>>>(xv, none, wp, bp, yv, s, relu); break;
-        case 6: k_conv3x3<6><<<grid, kConvThreads, 0, st>>>(xv, none, wp, bp, yv, s, relu); break;
-        case 8: k_conv3x3<8><<<grid, kConvThreads, 0, st>>>(xv, none, wp, bp, yv, s, relu); break;
-        case 12: k_conv3x3<12><<<grid, kConvThreads, 0, st>>>(xv, none, wp, bp, yv, s, relu); break;
-      }
-      SCIHPC_CHECK_LAUNCH();
+      launch_conv(ndim, s, xv, none, static_cast<const bf16_t *>(dw), b ? static_cast<const float *>(db) : nullptr,
+                  yv, relu, st);
       SCIHPC_HIP(hipMemcpyAsync(y, dy, sy, hipMemcpyDeviceToHost, st));
       SCIHPC_HIP(hipStreamSynchronize(st));
     } catch (...) {
@@ -871,4 +1064,15 @@ extern "C" int scihpc_conv3x3_forward(const uint16_t *x, int n, int height, int 
     }
     cleanup();
   });
+}
+}  // namespace
+
+extern "C" int scihpc_conv3x3_forward(const uint16_t *x, int n, int height, int width, int cin, const uint16_t *w,
+                                      const float *b, int cout, int relu, uint16_t *y) {
+  return conv_forward(2, x, n, 1, height, width, cin, w, b, cout, relu, y);
+}
+
+extern "C" int scihpc_conv3x3x3_forward(const uint16_t *x, int n, int depth, int height, int width, int cin,
+                                        const uint16_t *w, const float *b, int cout, int relu, uint16_t *y) {
+  return conv_forward(3, x, n, depth, height, width, cin, w, b, cout, relu, y);
 }

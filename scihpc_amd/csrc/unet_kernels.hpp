@@ -1,18 +1,21 @@
 // unet_kernels.hpp -- BF16 convolution kernels of the learned pressure
-// preconditioner for gfx950.
+// preconditioner for gfx950, 2-D (3x3, 9 taps) and 3-D (3x3x3, 27 taps).
 //
-// Activations are NHWC BF16 tensors addressed through channel views (base
-// pointer, channel stride of the underlying tensor, channel offset), so a
-// skip connection is a slice of the decoder's concatenation buffer and no
-// concatenation copy exists.  The 3x3 convolution is an implicit GEMM on
-// the 16x16x16 BF16 matrix cores with FP32 accumulation:
+// Activations are N(D)HWC BF16 tensors addressed through channel views
+// (base pointer, channel stride of the underlying tensor, channel offset),
+// so a skip connection is a slice of the decoder's concatenation buffer and
+// no concatenation copy exists.  The convolution is an implicit GEMM on
+// gfx950's 16x16x32 BF16 matrix cores with FP32 accumulation:
 //   M = output pixels (a workgroup owns 64 pixels of one image row, each
 //       wavefront 16 of them), N = output channels (16-wide blocks, all held
-//       by the wavefront), K = 9 taps x input channels (16-channel chunks).
-// Operand layout of v_mfma_f32_16x16x16bf16_1k (lane l of 64):
-//   A (16 x 16, M x K): row l % 16, k = 4 (l / 16) .. +3
-//   B (16 x 16, K x N): k = 4 (l / 16) .. +3, column l % 16
+//       by the wavefront), K = taps x input channels, tap-major, walked in
+//       steps of 32 (the last step zero-padded).
+// Operand layout of v_mfma_f32_16x16x32_bf16 (lane l of 64):
+//   A (16 x 32, M x K): row l % 16, k = 8 (l / 16) .. +7
+//   B (32 x 16, K x N): k = 8 (l / 16) .. +7, column l % 16
 //   C (16 x 16, float4): rows 4 (l / 16) + j, column l % 16, j = 0..3
+// Every input-channel count is a multiple of 16, so a lane's 8 consecutive
+// k values are 8 consecutive channels of one tap: one 16-byte load.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -22,7 +25,7 @@
 namespace scihpc {
 
 typedef uint16_t bf16_t;
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 __host__ __device__ __forceinline__ float bf16_to_f(bf16_t b) {
@@ -38,105 +41,148 @@ __host__ __device__ __forceinline__ bf16_t f_to_bf16(float f) {
   u += 0x7fffu + ((u >> 16) & 1u);
   return (bf16_t)(u >> 16);
 }
+// a BF16 value is > 0 (finite, or +inf) iff its bits read as a positive int16
+__device__ __forceinline__ bool bf16_pos(bf16_t b) { return (int16_t)b > 0; }
 
-struct View {  // NHWC channel slice
+__device__ __forceinline__ floatx4 mfma32(uint4 a, uint4 b, floatx4 c) {
+  bf16x8 va, vb;
+  __builtin_memcpy(&va, &a, 16);
+  __builtin_memcpy(&vb, &b, 16);
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(va, vb, c, 0, 0, 0);
+}
+
+struct View {  // N(D)HWC channel slice
   bf16_t *p;
   int stride;  // channels of the underlying tensor
   int off;     // first channel of the slice
 };
 
 struct ConvShape {
-  int n, h, w;     // images, rows, columns
-  int cin, cout;   // channels (cin multiple of 16, cout multiple of 16)
+  int n, d, h, w;  // images, depth (1 in 2-D), rows, columns
+  int cin, cout;   // channels (multiples of 16)
+};
+
+template <int DIM>
+struct Taps {
+  static constexpr int T = DIM == 3 ? 27 : 9;
+  // tap t -> offsets (dz, dy, dx) in {-1, 0, 1}; z-major, then y, then x
+  static __device__ __forceinline__ void offsets(int t, int &dz, int &dy, int &dx) {
+    const int r = DIM == 3 ? t % 9 : t;
+    dz = DIM == 3 ? t / 9 - 1 : 0;
+    dy = r / 3 - 1;
+    dx = r % 3 - 1;
+  }
 };
 
 constexpr int kConvThreads = 256;  // 4 wavefronts x 16 pixels = 64 pixels per workgroup
 
-// y = act(conv3x3(x, w) + b); w layout [cout][9][cin] (BF16), b FP32 (may be
+// y = act(conv(x, w) + b); w layout [cout][T][cin] (BF16), b FP32 (may be
 // null).  mask: when given, x is read as x * (mask > 0) (ReLU derivative of
 // the layer whose output gradient x is, for the backward-data use).
-template <int NB>
+template <int NB, int DIM>
 __global__ void __launch_bounds__(kConvThreads)
-    k_conv3x3(View x, View mask, const bf16_t *__restrict__ w, const float *__restrict__ b, View y, ConvShape s,
-              int relu) {
+    k_conv(View x, View mask, const bf16_t *__restrict__ w, const float *__restrict__ b, View y, ConvShape s,
+           int relu) {
+  constexpr int T = Taps<DIM>::T;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int segs = (s.w + 63) / 64;
   int blk = blockIdx.x;
   const int seg = blk % segs;
   blk /= segs;
-  const int row = blk % s.h, img = blk / s.h;
+  const int row = blk % s.h;
+  blk /= s.h;
+  const int z = blk % s.d, img = blk / s.d;
   const int px = seg * 64 + wave * 16 + (lane & 15);  // pixel of this lane's A row
-  const int kq = 4 * (lane >> 4);                    // k offset of this lane inside a chunk
+  const int kl = 8 * (lane >> 4);                    // k offset of this lane inside a step
   floatx4 acc[NB];
 #pragma unroll
   for (int nb = 0; nb < NB; ++nb) acc[nb] = floatx4{0.f, 0.f, 0.f, 0.f};
-  const int chunks = s.cin / 16;
-  const int K = 9 * chunks;  // k steps of 16: tap-major, then input-channel chunk
+  const int K = T * s.cin;
   // the operands of G consecutive k steps are loaded before their MFMAs, so
   // G loads are in flight at once instead of one dependent load per step
-  constexpr int G = 4;
-  for (int k0 = 0; k0 < K; k0 += G) {
-    bf16x4 a[G], bw[G][NB];
+  constexpr int G = 2;
+  for (int k0 = 0; k0 < K; k0 += 32 * G) {
+    uint4 a[G], bw[G][NB];
 #pragma unroll
     for (int g = 0; g < G; ++g) {
-      const int kk = k0 + g;
-      a[g] = bf16x4{0, 0, 0, 0};
-      if (kk < K) {
-        const int t = kk / chunks, cc = kk - t * chunks;
-        const int yy = row + t / 3 - 1, xx = px + t % 3 - 1;
-        if (yy >= 0 && yy < s.h && xx >= 0 && xx < s.w) {
-          const long pix = ((long)img * s.h + yy) * s.w + xx;
-          a[g] = *reinterpret_cast<const bf16x4 *>(x.p + pix * x.stride + x.off + cc * 16 + kq);
-          if (mask.p) {
-            const bf16_t *m = mask.p + pix * mask.stride + mask.off + cc * 16 + kq;
+      const int kk = k0 + 32 * g + kl;
+      a[g] = uint4{0, 0, 0, 0};
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
-              if (!(bf16_to_f(m[q]) > 0.f)) a[g][q] = 0;
+      for (int nb = 0; nb < NB; ++nb) bw[g][nb] = uint4{0, 0, 0, 0};
+      if (kk < K) {
+        const int t = kk / s.cin, c = kk - t * s.cin;
+        int dz, dy, dx;
+        Taps<DIM>::offsets(t, dz, dy, dx);
+        const int zz = z + dz, yy = row + dy, xx = px + dx;
+        if (zz >= 0 && zz < s.d && yy >= 0 && yy < s.h && xx >= 0 && xx < s.w) {
+          const long pix = (((long)img * s.d + zz) * s.h + yy) * s.w + xx;
+          a[g] = *reinterpret_cast<const uint4 *>(x.p + pix * x.stride + x.off + c);
+          if (mask.p) {
+            const uint4 m = *reinterpret_cast<const uint4 *>(mask.p + pix * mask.stride + mask.off + c);
+            unsigned av[4] = {a[g].x, a[g].y, a[g].z, a[g].w};
+            const unsigned mv[4] = {m.x, m.y, m.z, m.w};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              if (!bf16_pos((bf16_t)(mv[q] & 0xffffu))) av[q] &= 0xffff0000u;
+              if (!bf16_pos((bf16_t)(mv[q] >> 16))) av[q] &= 0x0000ffffu;
+            }
+            a[g] = uint4{av[0], av[1], av[2], av[3]};
           }
         }
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb)
-          bw[g][nb] =
-              *reinterpret_cast<const bf16x4 *>(w + ((long)(nb * 16 + (lane & 15)) * 9 + t) * s.cin + cc * 16 + kq);
+          bw[g][nb] = *reinterpret_cast<const uint4 *>(w + ((long)(nb * 16 + (lane & 15)) * T + t) * s.cin + c);
       }
     }
 #pragma unroll
     for (int g = 0; g < G; ++g) {
-      if (k0 + g < K) {
+      if (k0 + 32 * g < K) {
 #pragma unroll
-        for (int nb = 0; nb < NB; ++nb)
-          acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[g], bw[g][nb], acc[nb], 0, 0, 0);
+        for (int nb = 0; nb < NB; ++nb) acc[nb] = mfma32(a[g], bw[g][nb], acc[nb]);
       }
     }
   }
   // epilogue: rows 4 (lane/16) + j of the wave's 16 pixels, column lane % 16
+  const long rowpix = (((long)img * s.d + z) * s.h + row) * s.w;
 #pragma unroll
   for (int nb = 0; nb < NB; ++nb) {
     const int co = nb * 16 + (lane & 15);
     const float bias = b ? b[co] : 0.f;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      const int p = seg * 64 + wave * 16 + kq + j;
+      const int p = seg * 64 + wave * 16 + 4 * (lane >> 4) + j;
       if (p >= s.w) continue;
       float v = acc[nb][j] + bias;
       if (relu) v = v > 0.f ? v : 0.f;
-      const long pix = ((long)img * s.h + row) * s.w + p;
-      y.p[pix * y.stride + y.off + co] = f_to_bf16(v);
+      y.p[(rowpix + p) * y.stride + y.off + co] = f_to_bf16(v);
     }
   }
 }
 
-// Weight and bias gradient of the 3x3 convolution y = relu(conv(x, w) + b):
+// Weight and bias gradient of y = relu(conv(x, w) + b):
 //   dW[co][t][ci] = sum_p gz[p][co] x[p + off(t)][ci],  db[co] = sum_p gz[p][co]
 // with gz = g * (mask > 0) (mask = the layer's stored output).  A = gz^T
 // (output channels x pixels), B = x shifted by the tap (pixels x input
-// channels).  A workgroup owns one 16 x 16 channel block and one pixel chunk;
-// its four wavefronts take interleaved groups of 16 pixels and are summed
-// through LDS, and the workgroup writes its partial (no atomics) to
-//   part[chunk][co][t][ci], part[chunk][cout*9*cin + co]   (bias, cc == 0 only)
+// channels), K = pixels in steps of 32.  A workgroup owns one 16 x 16
+// channel block, one group of 9 taps (the 3-D kernel has three, one per
+// z offset) and one pixel chunk; each wavefront takes 32 consecutive pixels
+// of every 128.  The tiles reach the matrix cores through LDS: a lane loads
+// 8 channels of one pixel (16 bytes) and stores them transposed, so an
+// operand read is 8 pixels of one channel (one 16-byte LDS read).  The four
+// wavefronts are summed through LDS and the workgroup writes its partial
+// (no atomics) to
+//   part[chunk][co][t][ci], part[chunk][cout*T*cin + co]   (bias, from the
+// workgroups of cc == 0 and tap group 0 only)
 // which k_sum_rows reduces over chunks in a fixed order.
+constexpr int kWgPad = 40;  // LDS row length (BF16) of a 32-pixel tile row; 80 bytes keep 16-byte reads aligned
+
+template <int DIM>
 __global__ void __launch_bounds__(kConvThreads)
-    k_conv3x3_wgrad(View x, View g, View mask, float *__restrict__ part, ConvShape s, int chunk_pix) {
+    k_conv_wgrad(View x, View g, View mask, float *__restrict__ part, ConvShape s, int chunk_pix) {
+  constexpr int T = Taps<DIM>::T;
+  constexpr int NG = T / 9;  // tap groups
+  __shared__ __align__(16) bf16_t tg[4][16][kWgPad];
+  __shared__ __align__(16) bf16_t tx[4][16][kWgPad];
   __shared__ float red[4][36][64];
   __shared__ float bred[4][16];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -145,49 +191,79 @@ __global__ void __launch_bounds__(kConvThreads)
   const int cb = blk % cbs;
   blk /= cbs;
   const int cc = blk % ccs;
-  const int chunk = blk / ccs;
-  const long npix = (long)s.n * s.h * s.w;
+  blk /= ccs;
+  const int grp = blk % NG;
+  const int chunk = blk / NG;
+  const long npix = (long)s.n * s.d * s.h * s.w;
   const long p_begin = (long)chunk * chunk_pix;
   const long p_end = p_begin + chunk_pix < npix ? p_begin + chunk_pix : npix;
   floatx4 acc[9];
 #pragma unroll
   for (int t = 0; t < 9; ++t) acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
   float bsum = 0.f;
-  const int co = cb * 16 + (lane & 15), ci = cc * 16 + (lane & 15);
-  const int kq = 4 * (lane >> 4);
-  for (long p0 = p_begin + wave * 16; p0 < p_end; p0 += 64) {
-    bf16x4 a;
-    int pr[4], pc[4], pi[4];
-    bool pv[4];
+  const int sp = lane >> 1, half = 8 * (lane & 1);  // staging: pixel of the tile, channel half
+  const int kl = 8 * (lane >> 4);                   // operand read: first pixel of this lane
+  // every wavefront runs the same trip count (the barriers below)
+  for (long base = p_begin; base < p_end; base += 128) {
+    const long p = base + wave * 32 + sp;
+    const bool pv = p < p_end;
+    int px = 0, py = 0, pz = 0;
+    if (pv) {
+      px = (int)(p % s.w);
+      const long r = p / s.w;
+      py = (int)(r % s.h);
+      pz = (int)((r / s.h) % s.d);
+    }
+    {  // gz tile [co][pixel]
+      uint4 gv = uint4{0, 0, 0, 0};
+      if (pv) {
+        gv = *reinterpret_cast<const uint4 *>(g.p + p * g.stride + g.off + cb * 16 + half);
+        if (mask.p) {
+          const uint4 m = *reinterpret_cast<const uint4 *>(mask.p + p * mask.stride + mask.off + cb * 16 + half);
+          unsigned av[4] = {gv.x, gv.y, gv.z, gv.w};
+          const unsigned mv[4] = {m.x, m.y, m.z, m.w};
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const long p = p0 + kq + q;
-      pv[q] = p < p_end;
-      const long pp = pv[q] ? p : p_begin;
-      pc[q] = (int)(pp % s.w);
-      const long rest = pp / s.w;
-      pr[q] = (int)(rest % s.h);
-      pi[q] = (int)(rest / s.h);
-      float gv = 0.f;
-      if (pv[q]) {
-        gv = bf16_to_f(g.p[pp * g.stride + g.off + co]);
-        if (mask.p && !(bf16_to_f(mask.p[pp * mask.stride + mask.off + co]) > 0.f)) gv = 0.f;
+          for (int q = 0; q < 4; ++q) {
+            if (!bf16_pos((bf16_t)(mv[q] & 0xffffu))) av[q] &= 0xffff0000u;
+            if (!bf16_pos((bf16_t)(mv[q] >> 16))) av[q] &= 0x0000ffffu;
+          }
+          gv = uint4{av[0], av[1], av[2], av[3]};
+        }
       }
-      a[q] = (short)f_to_bf16(gv);  // exact: gv is a BF16 value or zero
-      bsum += gv;
+      const unsigned gw[4] = {gv.x, gv.y, gv.z, gv.w};
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        tg[wave][half + 2 * q][sp] = (bf16_t)(gw[q] & 0xffffu);
+        tg[wave][half + 2 * q + 1][sp] = (bf16_t)(gw[q] >> 16);
+      }
+    }
+    __syncthreads();
+    const uint4 a = *reinterpret_cast<const uint4 *>(&tg[wave][lane & 15][kl]);
+    {  // bias: this lane's 8 pixels of channel lane % 16
+      const unsigned av[4] = {a.x, a.y, a.z, a.w};
+#pragma unroll
+      for (int q = 0; q < 4; ++q) bsum += bf16_to_f((bf16_t)(av[q] & 0xffffu)) + bf16_to_f((bf16_t)(av[q] >> 16));
     }
 #pragma unroll
     for (int t = 0; t < 9; ++t) {
-      bf16x4 bb;
+      int dz, dy, dx;
+      Taps<DIM>::offsets(grp * 9 + t, dz, dy, dx);
+      uint4 xv = uint4{0, 0, 0, 0};
+      const int zz = pz + dz, yy = py + dy, xx = px + dx;
+      if (pv && zz >= 0 && zz < s.d && yy >= 0 && yy < s.h && xx >= 0 && xx < s.w) {
+        const long q = p + ((long)dz * s.h + dy) * s.w + dx;
+        xv = *reinterpret_cast<const uint4 *>(x.p + q * x.stride + x.off + cc * 16 + half);
+      }
+      const unsigned xw[4] = {xv.x, xv.y, xv.z, xv.w};
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const int yy = pr[q] + t / 3 - 1, xx = pc[q] + t % 3 - 1;
-        bf16_t v = 0;
-        if (pv[q] && yy >= 0 && yy < s.h && xx >= 0 && xx < s.w)
-          v = x.p[(((long)pi[q] * s.h + yy) * s.w + xx) * x.stride + x.off + ci];
-        bb[q] = (short)v;
+        tx[wave][half + 2 * q][sp] = (bf16_t)(xw[q] & 0xffffu);
+        tx[wave][half + 2 * q + 1][sp] = (bf16_t)(xw[q] >> 16);
       }
-      acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bb, acc[t], 0, 0, 0);
+      __syncthreads();
+      const uint4 bb = *reinterpret_cast<const uint4 *>(&tx[wave][lane & 15][kl]);
+      acc[t] = mfma32(a, bb, acc[t]);
+      __syncthreads();  // the next tap overwrites tx
     }
   }
 #pragma unroll
@@ -199,17 +275,17 @@ __global__ void __launch_bounds__(kConvThreads)
   bsum += __shfl_xor(bsum, 32, 64);
   if (lane < 16) bred[wave][lane] = bsum;
   __syncthreads();
-  const long row = (long)s.cout * 9 * s.cin + s.cout;
-  float *dst = part + (long)chunk * row;
+  const long rowlen = (long)s.cout * T * s.cin + s.cout;
+  float *dst = part + (long)chunk * rowlen;
   for (int e = threadIdx.x; e < 36 * 64; e += kConvThreads) {
     const int tj = e >> 6, l = e & 63;
     const int t = tj >> 2, j = tj & 3;
     const float v = red[0][tj][l] + red[1][tj][l] + red[2][tj][l] + red[3][tj][l];
     const int o = cb * 16 + 4 * (l >> 4) + j, c = cc * 16 + (l & 15);
-    dst[((long)o * 9 + t) * s.cin + c] = v;
+    dst[((long)o * T + grp * 9 + t) * s.cin + c] = v;
   }
-  if (cc == 0 && threadIdx.x < 16)
-    dst[(long)s.cout * 9 * s.cin + cb * 16 + threadIdx.x] =
+  if (cc == 0 && grp == 0 && threadIdx.x < 16)
+    dst[(long)s.cout * T * s.cin + cb * 16 + threadIdx.x] =
         bred[0][threadIdx.x] + bred[1][threadIdx.x] + bred[2][threadIdx.x] + bred[3][threadIdx.x];
 }
 

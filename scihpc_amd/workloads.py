@@ -17,6 +17,11 @@ Workloads
   poisson_3d / plane_3d : 256^3 unit cube; Dirichlet RHS from seeded modes
       k <= 8 (N(0,1)/|k|^2); variable variant with a 1000:1 jump across
       z = 0.5 + 0.05 sin(2 pi x) sin(2 pi y), cosine modes, Neumann walls.
+  bubble_3d            : 3-D bubble problem, 128^3 cells in the unit cube,
+      sphere of radius 0.2 at a given centre, rho 1 inside / 1000 outside
+      (smoothed Heaviside of width 1.5h), b = MAC divergence of a seeded
+      smooth wall-tangent velocity (16 modes per component), mean removed;
+      all-Neumann walls.
 """
 from __future__ import annotations
 
@@ -114,3 +119,29 @@ def poisson_3d(n=256, kmax=8, seed=SEED + 3, variable=False, ratio=1000.0):
     else:
         out.update(bc="dirichlet", rhs=f.ravel())
     return out
+
+
+def bubble_3d(n=128, center=(0.5, 0.5, 0.5), radius=0.2, ratio=1000.0, seed=SEED + 4, modes=16, kmax=8):
+    rng = np.random.default_rng(seed)
+    h = 1.0 / n
+    xc = (np.arange(n) + 0.5) * h
+    xf = np.arange(n + 1) * h
+    Z, Y, X = np.meshgrid(xc, xc, xc, indexing="ij")
+    phi = radius - np.sqrt((X - center[0]) ** 2 + (Y - center[1]) ** 2 + (Z - center[2]) ** 2)
+    rho = density(phi, 1.5 * h, ratio)
+    # face velocity of component a: sin(pi k x_a) along its own axis (zero on
+    # the walls it crosses), cos(pi k x_b) along the others
+    def comp(axis):
+        ks, amp = _modes(rng, modes, kmax, 3)
+        f = np.zeros([n + 1 if 2 - ax == axis else n for ax in range(3)])  # (z, y, x) shape
+        for (k1, k2, k3), a in zip(ks, amp):
+            fx = (np.sin if axis == 0 else np.cos)(math.pi * k1 * (xf if axis == 0 else xc))
+            fy = (np.sin if axis == 1 else np.cos)(math.pi * k2 * (xf if axis == 1 else xc))
+            fz = (np.sin if axis == 2 else np.cos)(math.pi * k3 * (xf if axis == 2 else xc))
+            f += a * fz[:, None, None] * fy[None, :, None] * fx[None, None, :]
+        return f
+    u, v, w = comp(0), comp(1), comp(2)
+    b = (u[:, :, 1:] - u[:, :, :-1] + v[:, 1:, :] - v[:, :-1, :] + w[1:, :, :] - w[:-1, :, :]) / h
+    b -= b.mean()
+    return dict(ndim=3, shape=(n, n, n), spacing=(h, h, h), bc="neumann", rho=rho.ravel(), rhs=b.ravel(),
+                phi=phi.ravel())

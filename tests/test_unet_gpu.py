@@ -348,3 +348,208 @@ def test_learned_guess_trained_gains():
         assert st["learned_accepted"] + st["learned_rejected"] == 1
         assert st["cycles"] <= plain["cycles"] + 1
     assert max(errs) < 0.6, errs
+
+
+# ------------------------------------------------------------------ 3-D
+def _grads_close(g, g_ref, g_exact, ndim):
+    """Per-layer gradient bar of the 3-D and smoothed-loss tests.  The GPU
+    differs from the BF16-emulating oracle only by FP32 accumulation order,
+    which flips occasional BF16 roundings of stored activations and
+    gradients; each flip perturbs the backward chain as much as BF16
+    storage itself does.  So a layer's bar is twice what BF16 storage moves
+    that layer's gradient from exact arithmetic (the oracle with
+    emulate=False: 2-25 % on these random problems, most where the
+    smoothing's low-pass gradient meets high-frequency activations), and
+    never below 3e-2 (the same for the head)."""
+    offs, head, _ = U.layout(ndim)
+    T = 3 ** ndim
+    for i, ((cin, cout, _), o) in enumerate(zip(U.SPEC, offs)):
+        sl = slice(o, o + cout * T * cin + cout)
+        noise = rel(g_ref[sl], g_exact[sl])
+        assert rel(g[sl], g_ref[sl]) < max(3e-2, 2.0 * noise), (i, rel(g[sl], g_ref[sl]), noise)
+    noise = rel(g_ref[head:], g_exact[head:])
+    assert rel(g[head:], g_ref[head:]) < max(3e-2, 2.0 * noise), (rel(g[head:], g_ref[head:]), noise)
+
+CONV3_CASES = [(1, 4, 5, 64, 16, 16), (2, 3, 4, 40, 32, 48), (1, 2, 3, 70, 192, 64), (1, 4, 4, 8, 128, 128),
+               (1, 3, 2, 16, 48, 192)]
+
+
+@pytest.mark.parametrize("case", CONV3_CASES, ids=lambda c: "x".join(map(str, c)))
+def test_conv3x3x3_forward(case):
+    n, d, h, w, cin, cout = case
+    rng = np.random.default_rng(sum(case))
+    xb = _bf16_bits(rng.standard_normal((n, d, h, w, cin)))
+    wb = _bf16_bits(rng.standard_normal((cout, 3, 3, 3, cin)) / np.sqrt(27 * cin))
+    b = rng.standard_normal(cout).astype(np.float32)
+    y = np.zeros((n, d, h, w, cout), np.uint16)
+    L = _capi.lib()
+    for relu in (0, 1):
+        print(f"conv3x3x3 {case} relu={relu}", flush=True)
+        _capi.check(L.scihpc_conv3x3x3_forward(xb.ravel(), n, d, h, w, cin, wb.ravel(), b, cout, relu,
+                                               y.reshape(-1)))
+        got = _bits_to_f(y)
+        ref = U.conv3x3(_bits_to_f(xb), _bits_to_f(wb), b.astype(np.float64))
+        if relu:
+            ref = np.maximum(ref, 0.0)
+        ref_b = U.bf16(ref)
+        tol = 2.0 ** -7 * np.abs(ref) + 1e-6 * np.abs(ref).max()
+        assert np.all(np.abs(got - ref_b) <= tol), np.abs(got - ref_b).max()
+
+
+def _faces3(n, D, H, Wd, seed):
+    f = np.random.default_rng(seed).uniform(0.001, 1.0, (n, D, H, Wd, 3)).astype(np.float32)
+    f[:, :, :, -1, 0] = 0.0
+    f[:, :, -1, :, 1] = 0.0
+    f[:, -1, :, :, 2] = 0.0
+    return f
+
+
+@pytest.mark.parametrize("shape", [(2, 8, 8, 16), (1, 16, 8, 8)], ids=lambda s: "x".join(map(str, s)))
+def test_unet3d_forward_backward(shape):
+    """The 3-D network (27-tap convolutions, 2x2x2 pooling): output, loss and
+    every layer's gradient against the oracle, for the mean squared error
+    and the energy-norm loss."""
+    n, D, H, Wd = shape
+    rng = np.random.default_rng(D + H + Wd)
+    p = U.init_params(7, ndim=3).astype(np.float32)
+    pix = D * H * Wd
+    feat = rng.standard_normal((n, pix, 2)).astype(np.float32)
+    tgt = rng.standard_normal((n, pix)).astype(np.float32)
+    faces = _faces3(n, D, H, Wd, 3)
+    print(f"unet3d fwd/bwd {shape}", flush=True)
+    net = UNet(H, Wd, n, depth=D)
+    assert net.num_params == U.layout(3)[2]
+    net.set_params(p)
+    net.out_scale = 1.0
+    inp = feat.reshape(n, D, H, Wd, 2).astype(np.float64)
+    t64 = tgt.reshape(n, D, H, Wd).astype(np.float64)
+    for fc in (None, faces):
+        net.set_samples(feat, tgt, None if fc is None else fc.reshape(n, pix, 3))
+        out = net.forward(np.arange(n))
+        loss = net.backward()
+        g = net.grads()
+        f64 = None if fc is None else fc.astype(np.float64)
+        l_ref, g_ref, out_ref = U.loss_and_grads(p.astype(np.float64), inp, t64, faces=f64)
+        _, g_exact, _ = U.loss_and_grads(p.astype(np.float64), inp, t64, faces=f64, emulate=False)
+        assert rel(out, out_ref) < 1e-2
+        assert abs(loss - l_ref) < 1e-2 * l_ref
+        _grads_close(g, g_ref, g_exact, 3)
+
+
+def test_unet3d_input_flips():
+    D, H, Wd = 8, 8, 16
+    rng = np.random.default_rng(8)
+    p = U.init_params(9, ndim=3).astype(np.float32)
+    feat = rng.standard_normal((1, D * H * Wd, 2)).astype(np.float32)
+    net = UNet(H, Wd, 1, depth=D)
+    net.set_params(p)
+    net.set_samples(feat, np.zeros((1, D * H * Wd), np.float32))
+    inp = feat.reshape(1, D, H, Wd, 2).astype(np.float64)
+    for fl in range(1, 8):
+        axes = [ax for bit, ax in ((1, 3), (2, 2), (4, 1)) if fl & bit]
+        out = net.forward([0], [fl]).reshape(1, D, H, Wd)
+        ref, _ = U.forward(p.astype(np.float64), np.ascontiguousarray(np.flip(inp, axes)))
+        assert rel(out, ref) < 1e-2, fl
+
+
+def test_unet3d_training_reduces_loss():
+    """Adam on the GPU fits a smooth 3-D target: loss falls by >= 10x."""
+    D = H = Wd = 8
+    n = 2
+    z, y, x = np.meshgrid(*(np.linspace(0, 1, 8),) * 3, indexing="ij")
+    tgt = np.stack([np.sin(np.pi * x) * np.cos(np.pi * y) * z, np.cos(np.pi * z) * y]).reshape(n, -1)
+    feat = np.stack([np.stack([np.sin(np.pi * x) * z, y], -1), np.stack([np.cos(np.pi * z), y * x], -1)])
+    net = UNet(H, Wd, n, seed=4, depth=D)
+    net.out_scale = float(np.sqrt(np.mean(tgt ** 2)))
+    net.set_samples(feat.reshape(n, -1, 2).astype(np.float32), tgt.astype(np.float32))
+    first = net.train_step([0, 1], 1e-3)
+    for _ in range(300):
+        last = net.train_step([0, 1], 1e-3, want_loss=False)
+    last = net.train_step([0, 1], 1e-4)
+    print(f"3-D training loss {first:.3e} -> {last:.3e}", flush=True)
+    assert last < 0.1 * first
+
+
+def test_learned3d_features_and_fallback():
+    """3-D multigrid hook: features, targets and faces on a coarse level are
+    block means of the level-0 fields (faces: the coarse operator's, scaled
+    by h^2), and a 3-D network that cannot lower the FP64 residual is
+    rejected with the plain result kept."""
+    n = 32
+    w = W.poisson_3d(n, variable=True)
+    h = 1.0 / n
+    mg = Multigrid(3, n, n, n, h, h, h, "neumann")
+    mg.set_density(w["rho"])
+    mg.set_rhs(w["rhs"])
+    mg.solve(rel_tol=1e-10, precision=PREC_FP64, accel=1)
+    p = mg.sol(0)
+    level, bs = 2, 4
+    f, t, c = mg.learned_features(level, with_target=True, with_faces=True)
+    b = w["rhs"].reshape(n, n, n)
+    sigma = np.sqrt(np.mean(b ** 2))
+
+    def bm(a):
+        m = n // bs
+        return a.reshape(m, bs, m, bs, m, bs).mean(axis=(1, 3, 5)).ravel()
+
+    np.testing.assert_allclose(f[:, 0], bm(b) / sigma, rtol=1e-5, atol=1e-6)
+    np.testing.assert_allclose(f[:, 1], bm(np.log(w["rho"].reshape(n, n, n))) / np.log(1000.0), rtol=1e-5, atol=1e-6)
+    np.testing.assert_allclose(t, bm(p.reshape(n, n, n)) / sigma, rtol=1e-5, atol=1e-6 * np.abs(t).max())
+    m = n // bs
+    cz = c[:, 2].reshape(m, m, m)
+    assert np.all(cz[-1] == 0.0) and np.all(cz[:-1] > 0.0)
+    assert np.all(c[:, 0].reshape(m, m, m)[:, :, -1] == 0.0)
+    # the plane's 1000:1 jump: faces inside the light layer are ~1000x those in the heavy one
+    assert cz[0].mean() / cz[-2].mean() > 100 or cz[-2].mean() / cz[0].mean() > 100
+    plain = mg.solve(rel_tol=1e-8, precision=PREC_MIXED, accel=1)
+    x_plain = mg.sol(0)
+    net = UNet(m, m, 1, depth=m)
+    net.set_params(np.full(net.num_params, np.nan, np.float32))
+    mg.attach_learned(net, level)
+    st = mg.solve(rel_tol=1e-8, precision=PREC_MIXED, accel=1, use_learned=3)
+    assert st["converged"] and st["learned_rejected"] == 1 and st["learned_accepted"] == 0
+    assert st["cycles"] == plain["cycles"]
+    assert rel(mg.sol(0), x_plain) < 1e-12
+    net2 = UNet(m, m, 1)  # a 2-D network does not fit a 3-D solver
+    with pytest.raises(ScihpcError):
+        mg.attach_learned(net2, level)
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_unet_smoothed_energy_loss(ndim):
+    """Energy loss through 2 damped-Jacobi sweeps of the error (and their
+    transposes in the gradient) against the oracle."""
+    shape = (16, 24) if ndim == 2 else (8, 8, 16)
+    n = 2
+    pix = int(np.prod(shape))
+    rng = np.random.default_rng(50 + ndim)
+    p = U.init_params(51, ndim=ndim).astype(np.float32)
+    feat = rng.standard_normal((n, pix, 2)).astype(np.float32)
+    tgt = rng.standard_normal((n, pix)).astype(np.float32)
+    fc = rng.uniform(0.001, 1.0, (n,) + shape + (ndim,)).astype(np.float32)
+    for a in range(ndim):  # walls: the last cell along axis a (x, y, z order) has no +a face
+        idx = [slice(None)] * (ndim + 2)
+        idx[ndim - a] = -1
+        idx[-1] = a
+        fc[tuple(idx)] = 0.0
+    net = UNet(shape[-2], shape[-1], n, depth=shape[0] if ndim == 3 else None)
+    net.set_params(p)
+    net.out_scale = 1.0
+    net.set_samples(feat, tgt, fc.reshape(n, pix, ndim))
+    net.set_loss_smoothing(2, 0.8)
+    net.forward(np.arange(n))
+    loss = net.backward()
+    g = net.grads()
+    inp = feat.reshape((n,) + shape + (2,)).astype(np.float64)
+    l_ref, g_ref, _ = U.loss_and_grads(p.astype(np.float64), inp, tgt.reshape((n,) + shape).astype(np.float64),
+                                       faces=fc.astype(np.float64), smooth=2, omega=0.8)
+    l_plain, _, _ = U.loss_and_grads(p.astype(np.float64), inp, tgt.reshape((n,) + shape).astype(np.float64),
+                                     faces=fc.astype(np.float64))
+    _, g_exact, _ = U.loss_and_grads(p.astype(np.float64), inp, tgt.reshape((n,) + shape).astype(np.float64),
+                                     faces=fc.astype(np.float64), smooth=2, omega=0.8, emulate=False)
+    print(f"smoothed loss {loss:.4e} oracle {l_ref:.4e} (unsmoothed {l_plain:.4e})", flush=True)
+    assert abs(loss - l_ref) < 1e-2 * l_ref
+    assert l_ref < l_plain  # the sweeps remove error energy
+    _grads_close(g, g_ref, g_exact, ndim)
+    with pytest.raises(ScihpcError):
+        net.set_loss_smoothing(-1, 0.8)
