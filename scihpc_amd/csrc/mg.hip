@@ -703,13 +703,31 @@ __device__ __forceinline__ void fcg_dir_sums(const double *__restrict__ part, in
   __shared__ double ta, tb;
   if (threadIdx.x < kWave) {
     double a = 0.0, b = 0.0, sp = 0.0, sr = 0.0, sq = 0.0;
-    for (int k = threadIdx.x; k < nb; k += kWave) {
-      a += part[k];
-      b += part[nb + k];
+    // U partials per lane are loaded before they are added (one memory
+    // round trip per U instead of per partial); the additions keep the
+    // order k, k + kWave, k + 2 kWave, ...
+    constexpr int U = 8;
+    for (int k0 = threadIdx.x; k0 < nb; k0 += U * kWave) {
+      double va[U], vb[U];
+#pragma unroll
+      for (int j = 0; j < U; ++j) {
+        const int k = k0 + j * kWave;
+        va[j] = k < nb ? part[k] : 0.0;
+        vb[j] = k < nb ? part[nb + k] : 0.0;
+      }
+#pragma unroll
+      for (int j = 0; j < U; ++j) {
+        if (k0 + j * kWave < nb) {
+          a += va[j];
+          b += vb[j];
+        }
+      }
       if (corr_n > 0.0) {
-        sp += part[2 * nb + k];
-        sr += part[3 * nb + k];
-        sq += part[4 * nb + k];
+        for (int k = k0; k < nb && k < k0 + U * kWave; k += kWave) {
+          sp += part[2 * nb + k];
+          sr += part[3 * nb + k];
+          sq += part[4 * nb + k];
+        }
       }
     }
     a = wave_sum(a);
@@ -768,10 +786,15 @@ __global__ void __launch_bounds__(kFusedThreads)
   // r and q of the staged region are loaded before alpha's reduction (the
   // two latencies overlap), then combined once alpha is known; the largest
   // tile loads them afterwards (its registers would cost occupancy)
+  // (the small tiles also load x and p of their own cells there, and
+  // update them in the staging pass, so the tile is not read twice)
   constexpr int NS0 = (W * H + kFusedThreads - 1) / kFusedThreads;
   constexpr bool early = NS0 <= 6;
   constexpr int NS = early ? NS0 : 1;
-  double rv[NS], qv[NS];
+  double rv[NS], qv[NS], xv[NS], pv[NS];
+  auto in_tile = [&](int lx, int ly, int gx, int gy) {
+    return lx >= kFH && lx < kFH + TX && ly >= kFH && ly < kFH + TY && gx < F.nx && gy < F.ny;
+  };
 #pragma unroll
   for (int t = 0; t < NS0; ++t) {
     const int q = threadIdx.x + t * kFusedThreads;
@@ -783,6 +806,9 @@ __global__ void __launch_bounds__(kFusedThreads)
       if constexpr (early) {
         rv[t] = in ? u.r_in[g] : 0.0;
         qv[t] = in ? u.q[g] : 0.0;
+        const bool own = in_tile(lx, ly, gx, gy);
+        xv[t] = own ? u.x[g] : 0.0;
+        pv[t] = own ? u.p[g] : 0.0;
       }
       xs[ly][lx] = 0.0f;
       if constexpr (VAR) {
@@ -798,6 +824,7 @@ __global__ void __launch_bounds__(kFusedThreads)
     u.scal[S_PR] = pr;
   }
   const double alpha = pq != 0.0 ? pr / pq : 0.0;  // 0 only after exact convergence
+  double r2 = 0.0;
 #pragma unroll
   for (int t = 0; t < NS0; ++t) {
     const int q = threadIdx.x + t * kFusedThreads;
@@ -805,24 +832,32 @@ __global__ void __launch_bounds__(kFusedThreads)
       const int ly = q / W, lx = q % W;
       const int gx = x0 - kFH + lx, gy = y0 - kFH + ly;
       const bool in = gx >= 0 && gx < F.nx && gy >= 0 && gy < F.ny;
+      const long g = gx + (long)F.nx * gy;
       if constexpr (early) {
-        bs[ly][lx] = in ? float(rv[t] - alpha * qv[t]) : 0.0f;
+        const double rc = rv[t] - alpha * qv[t];
+        bs[ly][lx] = in ? float(rc) : 0.0f;
+        if (in_tile(lx, ly, gx, gy)) {
+          u.x[g] = xv[t] + alpha * pv[t];
+          u.r_out[g] = rc;
+          F.rhs[g] = float(rc);
+          r2 += rc * rc;
+        }
       } else {
-        const long g = gx + (long)F.nx * gy;
         bs[ly][lx] = in ? float(u.r_in[g] - alpha * u.q[g]) : 0.0f;
       }
     }
   }
-  double r2 = 0.0;
-  for (int q = threadIdx.x; q < TX * TY; q += blockDim.x) {
-    const int gx = x0 + q % TX, gy = y0 + q / TX;
-    if (gx < F.nx && gy < F.ny) {
-      const long g = gx + (long)F.nx * gy;
-      u.x[g] += alpha * u.p[g];
-      const double rc = u.r_in[g] - alpha * u.q[g];
-      u.r_out[g] = rc;
-      F.rhs[g] = float(rc);
-      r2 += rc * rc;
+  if constexpr (!early) {
+    for (int q = threadIdx.x; q < TX * TY; q += blockDim.x) {
+      const int gx = x0 + q % TX, gy = y0 + q / TX;
+      if (gx < F.nx && gy < F.ny) {
+        const long g = gx + (long)F.nx * gy;
+        u.x[g] += alpha * u.p[g];
+        const double rc = u.r_in[g] - alpha * u.q[g];
+        u.r_out[g] = rc;
+        F.rhs[g] = float(rc);
+        r2 += rc * rc;
+      }
     }
   }
   r2 = block_sum(r2);  // its barriers also order the staging above
@@ -1996,41 +2031,46 @@ __global__ void __launch_bounds__(kBlock) k_padded_to_mg(const double *__restric
 }
 
 // ------------------------------------------------------ learned guess
-// network features on a level of block size bs (2-D): the block mean of b
-// scaled by 1/sigma_b and the block mean of ln(rho) / ln(1000); optionally
-// the block mean of the level-0 solution scaled by 1/sigma_b (training target)
+// network features on a level of block size bs (W x H x D cells, each the
+// mean of bs^ndim level-0 cells): the block mean of b scaled by 1/sigma_b
+// and the block mean of ln(rho) / ln(1000); optionally the block mean of the
+// level-0 solution scaled by 1/sigma_b (training target)
 __global__ void __launch_bounds__(kBlock) k_learned_features(const double *__restrict__ rhs0,
                                                              const double *__restrict__ rho0,
-                                                             const double *__restrict__ sol0, int nx0, int bs, int W,
-                                                             int H, double inv_sigma, float *__restrict__ feat,
-                                                             float *__restrict__ tgt) {
+                                                             const double *__restrict__ sol0, int nx0, int ny0,
+                                                             int bs, int bsz, int W, int H, int D, double inv_sigma,
+                                                             float *__restrict__ feat, float *__restrict__ tgt) {
   const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= (long)W * H) return;
-  const int x = (int)(p % W), y = (int)(p / W);
+  if (p >= (long)W * H * D) return;
+  const int x = (int)(p % W), y = (int)((p / W) % H), z = (int)(p / ((long)W * H));
   double sb = 0.0, sr = 0.0, sp = 0.0;
-  for (int b = 0; b < bs; ++b)
-    for (int a = 0; a < bs; ++a) {
-      const long c = (long)(y * bs + b) * nx0 + (x * bs + a);
-      sb += rhs0[c];
-      if (rho0) sr += log(rho0[c]);
-      if (sol0) sp += sol0[c];
-    }
-  const double inv = 1.0 / ((double)bs * bs);
+  for (int e = 0; e < bsz; ++e)
+    for (int b = 0; b < bs; ++b)
+      for (int a = 0; a < bs; ++a) {
+        const long c = ((long)(z * bsz + e) * ny0 + (y * bs + b)) * nx0 + (x * bs + a);
+        sb += rhs0[c];
+        if (rho0) sr += log(rho0[c]);
+        if (sol0) sp += sol0[c];
+      }
+  const double inv = 1.0 / ((double)bs * bs * bsz);
   feat[2 * p] = (float)(sb * inv * inv_sigma);
   feat[2 * p + 1] = (float)(sr * inv / 6.907755278982137);  // ln(1000)
   if (tgt) tgt[p] = (float)(sp * inv * inv_sigma);
 }
 
 // faces of a level scaled by its h^2 (face coefficient x h^2 = 1 / mean rho
-// of the face), network layout face[p] = (x+1 face, y+1 face), zero on walls
+// of the face), network layout face[p] = (x+1 face, y+1 face (, z+1 face)),
+// zero on walls
 __global__ void __launch_bounds__(kBlock) k_learned_faces(const double *__restrict__ cx, const double *__restrict__ cy,
-                                                          int W, int H, double hx2, double hy2,
+                                                          const double *__restrict__ cz, int W, int H, int D, int ndim,
+                                                          double hx2, double hy2, double hz2,
                                                           float *__restrict__ face) {
   const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= (long)W * H) return;
-  const int x = (int)(p % W), y = (int)(p / W);
-  face[2 * p] = x + 1 < W ? (float)(cx[p] * hx2) : 0.f;
-  face[2 * p + 1] = y + 1 < H ? (float)(cy[p] * hy2) : 0.f;
+  if (p >= (long)W * H * D) return;
+  const int x = (int)(p % W), y = (int)((p / W) % H), z = (int)(p / ((long)W * H));
+  face[ndim * p] = x + 1 < W ? (float)(cx[p] * hx2) : 0.f;
+  face[ndim * p + 1] = y + 1 < H ? (float)(cy[p] * hy2) : 0.f;
+  if (ndim == 3) face[3 * p + 2] = z + 1 < D ? (float)(cz[p] * hz2) : 0.f;
 }
 
 __global__ void __launch_bounds__(kBlock) k_net_to_level(const float *__restrict__ out, double *__restrict__ sol,
@@ -3279,13 +3319,18 @@ struct scihpc_mg {
     return rr <= o.rel_tol ? SCIHPC_OK : SCIHPC_NOT_CONVERGED;
   }
 
+  // every cell of level l is the block of deg^ndim level-0 cells
+  bool block_coarsening(int l) const {
+    const auto &a = d[l];
+    return a.nx * a.deg == d[0].nx && a.ny * a.deg == d[0].ny && (ndim == 2 || a.nz * a.deg == d[0].nz);
+  }
   // learned features of the current b (and sol when tgt) on the hook's level
   void learned_features(int lvl, double sigma, float *feat, float *tgt) {
     auto &a = d[lvl];
     const int bs = a.deg;
-    k_learned_features<<<ceil_div((long)a.nx * a.ny, kBlock), kBlock, 0, stream>>>(
-        d[0].rhs, variable ? rho0 : nullptr, tgt ? d[0].sol : nullptr, d[0].nx, bs, a.nx, a.ny,
-        sigma > 0.0 ? 1.0 / sigma : 0.0, feat, tgt);
+    k_learned_features<<<ceil_div(a.n, kBlock), kBlock, 0, stream>>>(
+        d[0].rhs, variable ? rho0 : nullptr, tgt ? d[0].sol : nullptr, d[0].nx, d[0].ny, bs, ndim == 3 ? bs : 1, a.nx,
+        a.ny, ndim == 3 ? a.nz : 1, sigma > 0.0 ? 1.0 / sigma : 0.0, feat, tgt);
     SCIHPC_CHECK_LAUNCH();
   }
 
@@ -3332,7 +3377,7 @@ struct scihpc_mg {
   }
   void ensure_learned_io() {
     if (lfeat) return;
-    const long hw = (long)d[learned.level].nx * d[learned.level].ny;
+    const long hw = d[learned.level].n;
     lfeat = dalloc<float>(2 * hw, stream);
     lout = dalloc<float>(hw, stream);
     SCIHPC_HIP(hipStreamSynchronize(stream));
@@ -3357,7 +3402,7 @@ struct scihpc_mg {
       auto &l0 = d[0];
       const int lvl = learned.level;
       auto &a = d[lvl];
-      const long hw = (long)a.nx * a.ny;
+      const long hw = a.n;
       const double sigma = bnorm / std::sqrt((double)l0.n);
       learned_features(lvl, sigma, lfeat, nullptr);
       unet_infer(learned.net, lfeat, lout, stream);
@@ -4032,12 +4077,13 @@ extern "C" int scihpc_mg_attach_learned(scihpc_mg *h, scihpc_unet *net, int leve
       h->learned_streak = h->learned_skip = 0;
       return;
     }
-    SCIHPC_REQUIRE(h->ndim == 2, "the learned guess is 2-D");
+    SCIHPC_REQUIRE(scihpc::unet_ndim(net) == h->ndim, "the network's dimension differs from the solver's");
     SCIHPC_REQUIRE(level >= 0 && level < h->L, "level out of range");
     const auto &a = h->d[level];
-    SCIHPC_REQUIRE(scihpc::unet_width(net) == a.nx && scihpc::unet_height(net) == a.ny,
+    SCIHPC_REQUIRE(scihpc::unet_width(net) == a.nx && scihpc::unet_height(net) == a.ny &&
+                       (h->ndim == 2 || scihpc::unet_depth(net) == a.nz),
                    "network size differs from the level's grid");
-    SCIHPC_REQUIRE(a.nx * a.deg == h->d[0].nx && a.ny * a.deg == h->d[0].ny, "level is not a block coarsening");
+    SCIHPC_REQUIRE(h->block_coarsening(level), "level is not a block coarsening");
     if (h->lfeat && (h->learned.level != level)) {
       SCIHPC_HIP(hipStreamSynchronize(h->stream));
       if (h->lstream) SCIHPC_HIP(hipStreamSynchronize(h->lstream));
@@ -4055,25 +4101,25 @@ extern "C" int scihpc_mg_learned_features(scihpc_mg *h, int level, float *feat, 
   return guarded([&] {
     check_level(h, level);
     SCIHPC_REQUIRE(feat, "feat is null");
-    SCIHPC_REQUIRE(h->ndim == 2, "the learned guess is 2-D");
     SCIHPC_REQUIRE(!faces || h->variable, "faces need the variable operator (set_density)");
     const auto &a = h->d[level];
-    SCIHPC_REQUIRE(a.nx * a.deg == h->d[0].nx && a.ny * a.deg == h->d[0].ny, "level is not a block coarsening");
-    const long hw = (long)a.nx * a.ny;
+    SCIHPC_REQUIRE(h->block_coarsening(level), "level is not a block coarsening");
+    const long hw = a.n;
+    const int nd = h->ndim;
     const double bnorm = std::sqrt(h->vec_norm2(h->d[0].rhs, h->d[0].n));
     const double sigma = bnorm / std::sqrt((double)h->d[0].n);
     float *df = dalloc<float>(2 * hw, h->stream);
     float *dt = target ? dalloc<float>(hw, h->stream) : nullptr;
-    float *dc = faces ? dalloc<float>(2 * hw, h->stream) : nullptr;
+    float *dc = faces ? dalloc<float>((long)nd * hw, h->stream) : nullptr;
     try {
       h->learned_features(level, sigma, df, dt);
       SCIHPC_HIP(hipMemcpyAsync(feat, df, sizeof(float) * 2 * hw, hipMemcpyDeviceToHost, h->stream));
       if (target) SCIHPC_HIP(hipMemcpyAsync(target, dt, sizeof(float) * hw, hipMemcpyDeviceToHost, h->stream));
       if (faces) {
-        k_learned_faces<<<ceil_div(hw, kBlock), kBlock, 0, h->stream>>>(a.cx, a.cy, a.nx, a.ny, a.dx * a.dx,
-                                                                          a.dy * a.dy, dc);
+        k_learned_faces<<<ceil_div(hw, kBlock), kBlock, 0, h->stream>>>(
+            a.cx, a.cy, a.cz, a.nx, a.ny, nd == 3 ? a.nz : 1, nd, a.dx * a.dx, a.dy * a.dy, a.dz * a.dz, dc);
         SCIHPC_CHECK_LAUNCH();
-        SCIHPC_HIP(hipMemcpyAsync(faces, dc, sizeof(float) * 2 * hw, hipMemcpyDeviceToHost, h->stream));
+        SCIHPC_HIP(hipMemcpyAsync(faces, dc, sizeof(float) * nd * hw, hipMemcpyDeviceToHost, h->stream));
       }
       SCIHPC_HIP(hipStreamSynchronize(h->stream));
     } catch (...) {

@@ -382,6 +382,37 @@ def test_lagged_check(precision, accel):
         g.solve(lagged_check=2, **kw)
 
 
+def test_lagged_check_density_changes():
+    """Lagged checks in a time loop whose density changes between solves:
+    every solve meets the tolerance; a harder operator (1000:1 after 10:1)
+    converges at its own count at once (the residual is read every iteration
+    from one before the remembered count); an easier one overshoots at first
+    and is back at its own count within a few solves."""
+    nx, ny = 128, 256
+    hard, easy = W.bubble_2d(nx, ny), W.bubble_2d(nx, ny, ratio=10.0)
+    g = Multigrid(2, nx, ny, 1, 1 / nx, 2 / ny, 1.0, "neumann")
+    kw = dict(rel_tol=1e-9, max_cycles=200, precision=PREC_MIXED, accel=1)
+    own = {}
+    for name, w in (("hard", hard), ("easy", easy)):
+        g.set_density(w["rho"])
+        g.set_rhs(w["rhs"])
+        own[name] = g.solve(**kw)["cycles"]
+    assert own["easy"] < own["hard"] - 2, own
+    seq = ["hard"] * 3 + ["easy"] * 8 + ["hard"] * 2
+    got = []
+    for name in seq:
+        w = hard if name == "hard" else easy
+        g.set_density(w["rho"])
+        g.set_rhs(w["rhs"])
+        st = g.solve(lagged_check=1, **kw)
+        assert st["converged"] and st["rel_residual"] <= 1e-9, (name, st)
+        got.append(st["cycles"])
+    print(f"own counts {own}, lagged sequence {got}", flush=True)
+    assert all(c >= own[n] for c, n in zip(got, seq))
+    assert got[10] == own["easy"]
+    assert got[11] == own["hard"]
+
+
 @pytest.mark.parametrize("lagged", [0, 1], ids=["checked", "lagged"])
 def test_fused_fcg_update_matches_separate_kernels(monkeypatch, lagged):
     """The mixed 2-D FCG update fused into the next level-0 pre-smoothing
