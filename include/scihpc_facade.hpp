@@ -70,8 +70,10 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <initializer_list>
+#include <map>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -488,6 +490,8 @@ class multigrid {
  public:
   multigrid(scalar_data *var, structured_grid *geo)
       : ndim_(var->ndim), nx_(var->nx), ny_(var->ny), nz_(var->nz), dx_(geo->dx), dy_(geo->dy), dz_(geo->dz) {
+    const char *cyc = std::getenv("SCIHPC_VCYCLE");
+    rb_cycle_ = cyc && std::strcmp(cyc, "rb") == 0;
     create(SCIHPC_MG_NEUMANN);
     int L = 0;
     check(scihpc_mg_num_levels(h_, &L));
@@ -526,9 +530,15 @@ class multigrid {
   unsigned int level_num = 0;
   multigrid_base **at = nullptr;
 
-  void v_cycle() {  // multigrid.cpp:43-55
+  // multigrid.cpp:43-55: the reference's steepest-descent V(10,10).  With
+  // SCIHPC_VCYCLE=rb in the environment (read when the multigrid is made)
+  // an unchanged driver's v_cycle() runs this build's FP64 red-black
+  // V(2,2) instead (scihpc_mg_vcycle): the same discrete solution, far
+  // fewer and shorter cycles
+  void v_cycle() {
     at[0]->push();
-    check(scihpc_mg_vcycle_ref(h_));
+    if (rb_cycle_) check(scihpc_mg_vcycle(h_, 2, 2));
+    else check(scihpc_mg_vcycle_ref(h_));
     at[0]->pull_sol();
   }
   void full_cycle() {  // multigrid.cpp:57-79
@@ -553,6 +563,7 @@ class multigrid {
   scihpc_mg *handle() const { return h_; }
 
  private:
+  bool rb_cycle_ = false;
   friend class multigrid_base;
   int ndim_, nx_, ny_, nz_;
   DataType dx_, dy_, dz_;
@@ -1282,6 +1293,17 @@ class projection_method {
 // reference's files; create(id) opens <filename><id>.vtk.  The device does
 // the reorder and byte swap, a background thread writes the file, close()
 // waits for it.
+// node_from_face (wrapper_func.h:42, wrapper_func.cpp:264-293) as the free
+// function: face velocities of ref to node velocities of tgt with tgt's
+// nvel BCs, on a device handle kept per grid for the process (created at
+// the first call with ref's BCs)
+inline void node_from_face(wrapper *ref, wrapper *tgt) {
+  static std::map<const structured_grid *, std::unique_ptr<projection_method>> handles;
+  std::unique_ptr<projection_method> &pm = handles[ref->geo];
+  if (!pm) pm.reset(new projection_method(&ref->vector->x));
+  pm->node_from_face(ref, tgt);
+}
+
 class vtkWriter {
  public:
   vtkWriter(structured_grid *_geo, std::string _filename) : grid(_geo), filename(std::move(_filename)) {

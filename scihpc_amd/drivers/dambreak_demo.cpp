@@ -12,12 +12,15 @@
 // init "indicator": phi = +-1 in/out of the column, then find_sign,
 // stabilized_upon_gradient and redistancing until l2norm < 1e-6 (the
 // reference's initialisation); init "distance": the exact signed distance
-// to the column's free faces.  dt = cfl * h (min_CFL = max_CFL = cfl, as the
-// reference sets 0.01).  Every k steps (k > 0) a snapshot <prefix><id>.vtk
+// to the column's free faces.  dt = min(stable_CFL, cfl) * h: find_dt's own
+// stability bound (wrapper_func.cpp:335-383) caps the step, min_CFL =
+// min(cfl, 0.01) (the reference's driver pins both at 0.01).  A step whose
+// divergence or level-set mass is not finite stops the run with status 3
+// (after printing stopped_at and the snapshot count).  Every k steps (k > 0) a snapshot <prefix><id>.vtk
 // holds phi, pressure, density, the pressure-step right-hand side
 // rhs = div(u*) / dt of the intermediate velocity, and nvel -- the (rhs,
 // rho, p) triples the learned preconditioner trains on.  Prints key/value
-// lines.
+// lines on stdout and a progress line every 100 steps on stderr.
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -91,7 +94,7 @@ int main(int argc, char **argv) {
   param->ls_width = 1.5 * geo.h;
   param->rdt = 0.25 * geo.h;
   param->max_CFL = cfl;
-  param->min_CFL = cfl;
+  param->min_CFL = std::fmin(cfl, 0.01);
   param->Weber_number = -1.0;
   param->ppe_tol = 1e-8;
   param->ppe_initer = 1;
@@ -184,12 +187,19 @@ int main(int argc, char **argv) {
         if (param->iter < 5) flow_solver.projection(&pressure, &phi, &vel);
         else flow_solver.fast_pressure_correction(&pressure, &phi, &vel);
       }
-      flow_solver.node_from_face(&vel, &nvel);
+      node_from_face(&vel, &nvel);
     } else {
       flow_solver.ab_solve_sec(&vel, &nvel, &pressure, &phi);
     }
     max_cycles = std::max(max_cycles, flow_solver.last_cycles);
-    max_div = std::fmax(max_div, divergence(&vel));
+    const double div = divergence(&vel);
+    const double mass = lsf_mass(&phi);
+    if (!std::isfinite(div) || !std::isfinite(mass)) {
+      std::fprintf(stderr, "step %d: divergence %g, level-set mass %g; stopping\n", s, div, mass);
+      std::printf("stopped_at %d\nsnapshots %u\n", s, prefix ? snap_id : 0u);
+      return 3;
+    }
+    max_div = std::fmax(max_div, div);
 
     if (param->iter % 10 == 0) {
       int instep = 0;
@@ -199,6 +209,12 @@ int main(int argc, char **argv) {
       }
     }
     if (snap) snapshot(*vtk, snap_id++);
+    if (s % 100 == 0) {  // progress on stderr (stdout holds the key/value results)
+      std::fprintf(stderr, "step %d t %.4f dt/h %.4f pressure_cycles %d front %.4f seconds %.1f\n", s, param->t,
+                   param->dt / geo.h, flow_solver.last_cycles, front(),
+                   std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+      std::fflush(stderr);
+    }
   }
   const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 

@@ -141,3 +141,27 @@ def test_dambreak_driver_runs(tmp_path):
     assert s2["density"].min() > 1e-3 * 0.99 and s2["density"].max() <= 1.0 + 1e-12
     wet = s2["phi"][0, 0] > 0  # bottom row (z, y, x order)
     assert wet[:8].all() and wet.sum() >= 8
+
+
+@pytest.mark.gpu
+def test_facade_rb_vcycle_switch(tmp_path):
+    """SCIHPC_VCYCLE=rb: an unchanged driver's v_cycle() residual loop runs
+    the red-black V(2,2) instead of the reference's steepest-descent
+    V(10,10), and reaches the same first-order discretisation error with
+    fewer cycles."""
+    assert os.path.exists(DEMO), "build() compiles scihpc_amd/drivers/facade_demo"
+    out = {}
+    for mode in ("ref", "rb"):
+        env = dict(os.environ)
+        env.pop("SCIHPC_VCYCLE", None)
+        if mode == "rb":
+            env["SCIHPC_VCYCLE"] = "rb"
+        print(f"facade_demo 64 v_cycle={mode}", flush=True)
+        res = subprocess.run([DEMO, "64"], capture_output=True, text=True, timeout=120, env=env)
+        assert res.returncode == 0, res.stderr
+        v = dict(line.split() for line in res.stdout.splitlines())
+        out[mode] = (int(v["vcycle_cycles"]), float(v["vcycle_rel_residual"]), float(v["vcycle_error"]))
+    print(out, flush=True)
+    assert out["rb"][1] <= 1e-8 and out["ref"][1] <= 1e-8
+    assert out["rb"][0] < out["ref"][0]
+    assert abs(out["rb"][2] - out["ref"][2]) < 1e-6
