@@ -287,6 +287,94 @@ def bench_3d(args):
     return res
 
 
+def bench_learned3d(reps=3, problems=3):
+    """Config 5's 3-D half: held-out 128^3 bubbles (sphere of radius 0.2 at a
+    seeded position, 1000:1) solved by mixed FCG to 1e-8 with and without
+    the 3-D U-Net guess (network on level 2, 32^3), judged on every solve
+    and forced; the guess's energy-norm error against the FP64 solution."""
+    import numpy as np
+    import torch
+
+    from scihpc_amd import PREC_FP64, PREC_MIXED, Multigrid
+    from scihpc_amd import learned as LRN
+
+    path = os.path.join(LRN.WEIGHTS_DIR, "unet_bubble3d_128_level2.npz")
+    if not os.path.exists(path):
+        return {"attached": False, "note": "no 3-D weights (python -m scihpc_amd.learned --data bubble3d)"}
+    net, meta = LRN.load_net(path)
+    n = 128
+    g = Multigrid(3, n, n, n, 1 / n, 1 / n, 1 / n, "neumann")
+    g.attach_learned(net, meta["level"])
+    rows = []
+    for cfg in LRN.bubble3d_configs(problems, LRN.EVAL_SEED):
+        w = LRN.make_problem(cfg, n)
+        g.set_density(w["rho"])
+        g.set_rhs(w["rhs"])
+        row = {"center": [round(v, 3) for v in cfg["center"]]}
+        for name, learned in (("plain", 0), ("learned_judged", 3), ("learned_forced", 2)):
+            g.solve(rel_tol=1e-8, max_cycles=100, precision=PREC_MIXED, accel=1, use_learned=learned)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                st = g.solve(rel_tol=1e-8, max_cycles=100, precision=PREC_MIXED, accel=1, use_learned=learned)
+            torch.cuda.synchronize()
+            assert st["converged"], st
+            row[name] = {"ms_per_solve": round(1e3 * (time.perf_counter() - t0) / reps, 3), "cycles": st["cycles"]}
+            if learned == 3:
+                row[name]["accepted"] = st["learned_accepted"]
+                row[name]["guess_rel_residual"] = st["learned_residual"]
+        q = LRN.guess_quality(g, [w])[0]
+        row["guess_energy_error"] = round(q["energy"], 4)
+        rows.append(row)
+        log(f"bench: 3-D learned {row}")
+    g.close()
+    net.close()
+    return {"attached": True, "weights": os.path.relpath(path, ROOT), "net_level": meta["level"],
+            "net_grid": [meta["width"], meta["height"], meta["depth"]], "training": meta["meta"],
+            "workload": "bubble_3d 128^3, radius 0.2, rho 1000:1, held-out positions", "rel_tol": 1e-8,
+            "problems": rows}
+
+
+def bench_learned_dambreak(reps=3):
+    """Config 5's 2-D half: the dam-break network (trained on the solver's
+    own 1024x512 dam-break snapshots, steps 10-1040) on two held-out steps
+    (655: an ordinary step; 1310: a hard step before the flow breaks down):
+    mixed FCG to 1e-8 from p = 0, from the network's guess, and with the
+    guess judged inside the solve."""
+    import torch
+
+    from scihpc_amd import PREC_MIXED, Multigrid
+    from scihpc_amd import learned as LRN
+
+    path = os.path.join(LRN.WEIGHTS_DIR, "unet_dambreak_1024x512_level3.npz")
+    if not (os.path.exists(path) and os.path.exists(LRN.DAMBREAK_FIXTURE)):
+        return {"attached": False}
+    net, meta = LRN.load_net(path)
+    g = Multigrid(2, 1024, 512, 1, 4 / 1024, 2 / 512, 1.0, "neumann")
+    g.attach_learned(net, meta["level"])
+    rows = {}
+    for step, w in sorted(LRN.dambreak_fixture_problems().items()):
+        g.set_density(w["rho"])
+        g.set_rhs(w["rhs"])
+        row = {}
+        for name, learned in (("plain", 0), ("from_guess", 2), ("judged", 3)):
+            g.solve(rel_tol=1e-8, max_cycles=200, precision=PREC_MIXED, accel=1, use_learned=learned)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                st = g.solve(rel_tol=1e-8, max_cycles=200, precision=PREC_MIXED, accel=1, use_learned=learned)
+            torch.cuda.synchronize()
+            assert st["converged"], st
+            row[name] = {"ms_per_solve": round(1e3 * (time.perf_counter() - t0) / reps, 3), "cycles": st["cycles"]}
+        rows[str(step)] = row
+        log(f"bench: dam-break learned step {step}: {row}")
+    g.close()
+    net.close()
+    return {"attached": True, "weights": os.path.relpath(path, ROOT), "net_level": meta["level"],
+            "training": meta["meta"], "workload": "dam break 1024x512 A(rho) pressure systems, held-out steps",
+            "rel_tol": 1e-8, "steps": rows}
+
+
 def bench_snapshot(torch, nx=512, ny=1024, reps=5):
     """Legacy-VTK snapshot of the 512x1024 grid as the reference's drivers
     write it (three scalars and one vector per file, vtkWriter.cpp:18-111):
@@ -533,6 +621,8 @@ def main():
     ap.add_argument("--no-momentum", action="store_true", help="skip the momentum predictor timing")
     ap.add_argument("--no-interface", action="store_true", help="skip the interface-maintenance timing")
     ap.add_argument("--no-weno", action="store_true", help="skip the WENO5 TVD-RK3 timing")
+    ap.add_argument("--no-learned3d", action="store_true", help="skip the 128^3 learned-guess row")
+    ap.add_argument("--only-learned3d", action="store_true", help="run only the 128^3 learned-guess row")
     ap.add_argument("--only-rows", action="store_true",
                     help="run only the rows around the solve (momentum predictor, interface maintenance, "
                          "VTK snapshot) and print their JSON (profiling)")
@@ -562,6 +652,9 @@ def main():
         # the known-size device copy that calibrates PMC byte counters
         out3["measured_ceiling"] = bench_hbm_ceiling(torch)
         print(json.dumps(out3), flush=True)
+        return
+    if args.only_learned3d:
+        print(json.dumps(bench_learned3d()), flush=True)
         return
     if args.only_rows:
         rows = {"momentum": bench_momentum(), "interface": bench_interface(), "weno_rk3": bench_weno(),
@@ -849,6 +942,11 @@ def main():
     }
     if not args.no_3d:
         out["plane_3d_256"] = bench_3d(args)
+    if not args.no_learned3d:
+        log("bench: 3-D learned guess on held-out 128^3 bubbles")
+        out["learned_3d_128"] = bench_learned3d()
+        log("bench: dam-break learned guess on held-out steps")
+        out["learned_dambreak"] = bench_learned_dambreak()
     if not args.no_cpu_baseline:
         log("bench: cpu baseline (oracle port, 1 core)")
         out["cpu_baseline"] = cpu_baseline(w, rhs_host, args.cpu_seconds)

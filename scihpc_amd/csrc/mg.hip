@@ -2110,6 +2110,40 @@ __global__ void __launch_bounds__(kBlock) k_guess_dots(DLevel<double> L, const d
   }
 }
 
+// x.b, x.r and g.r of an FCG iterate x, its residual r and a learned guess g
+// (partial[k * nb + block], k = 0, 1, 2)
+__global__ void __launch_bounds__(kBlock) k_combine_dots(const double *__restrict__ x, const double *__restrict__ r,
+                                                         const double *__restrict__ b, const double *__restrict__ g,
+                                                         long n, double *__restrict__ partial, int nb) {
+  const long c = (long)xcd_block() * blockDim.x + threadIdx.x;
+  double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+  if (c < n) {
+    const double xc = x[c], rc = r[c];
+    v0 = xc * b[c];
+    v1 = xc * rc;
+    v2 = g[c] * rc;
+  }
+  v0 = block_sum(v0);
+  v1 = block_sum(v1);
+  v2 = block_sum(v2);
+  if (threadIdx.x == 0) {
+    partial[blockIdx.x] = v0;
+    partial[nb + blockIdx.x] = v1;
+    partial[2 * nb + blockIdx.x] = v2;
+  }
+}
+
+// y = a x + c y
+__global__ void __launch_bounds__(kBlock) k_axpby(const double *__restrict__ x, double *__restrict__ y, long n,
+                                                  double a, double c) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = a * x[i] + c * y[i];
+}
+
+// least gain of the energy-optimal combination of an FCG iterate and the
+// learned guess, as a fraction of |J(x_k)| (learned_combine)
+constexpr double kCombineGain = 0.05;
+
 // x += y
 __global__ void __launch_bounds__(kBlock) k_add(double *__restrict__ x, const double *__restrict__ y, long n) {
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -3130,8 +3164,9 @@ struct scihpc_mg {
       }
       if (guess_ready) {
         learned_overlap = false;
-        if (learned_decide(st, bnorm, std::min(rr, 1.0), false)) {
-          // restart from the guess: x = p0, r = b - A p0 (mixed: b is l0.rhs)
+        vec_norm2_cached_ = bnorm * bnorm;
+        if (learned_combine(st, x, r)) {
+          // restart from the combination: x = a x + c g, r = b - A x (mixed: b is l0.rhs)
           spent += it;
           it = 0;
           cur = 0;
@@ -3348,6 +3383,7 @@ struct scihpc_mg {
   hipEvent_t lev = nullptr;
   double *lpartial = nullptr, *lpartial2 = nullptr, *lscal = nullptr, *lhost = nullptr;
   bool learned_pending = false;
+  double vec_norm2_cached_ = 0.0;  // ||b||^2 of the solve whose guess is pending
   // back-off of a guess that keeps losing (use_learned == 1): after the
   // k-th rejection in a row the next 2^k - 1 solves (at most 63) skip the
   // network; an accepted guess resets it
@@ -3438,6 +3474,78 @@ struct scihpc_mg {
     swap_all();
     SCIHPC_HIP(hipEventRecord(lev, lstream));
     learned_pending = true;
+  }
+
+  // The pending guess g (in d[0].sol) inside a mixed flexible-CG solve,
+  // after k iterations from x = 0: the point of span{x_k, g} with the least
+  // A(rho) energy error -- the error norm CG minimises -- solves
+  // V^T A V c = V^T b for V = [x_k, g] (2 x 2; x_k.A x_k = x_k.b - x_k.r_k,
+  // g.A x_k = g.b - g.r_k, g.b and g.A g from the guess pipeline).  It
+  // replaces x_k (in d[0].sol, for the caller's restart) when its energy
+  // J(v) = b.v - v.A v / 2 (A is negative definite: this is the SPD energy
+  // of -A) is below x_k's by more than kCombineGain of |J(x_k)|.  J(x_k)
+  // - J(x*) is half the squared energy error and |J(x_k)| is about half the
+  // solution's squared energy norm, so the combination must remove at
+  // least that fraction of the solution's energy from the error: enough to
+  // be worth the restart it needs.  Late in a solve the iterate's error is
+  // far below that and the guess is left; early, a guess better than the
+  // iterate is taken.  A guess that cannot do that (NaN, zero, or no better
+  // than the iterate) leaves x_k.
+  // Returns true when replaced.
+  bool learned_combine(scihpc_solve_stats &st, const double *x, const double *r) {
+    double gb = 0.0, gag = 0.0, rel = -1.0;
+    {
+      SCIHPC_REQUIRE(learned_pending, "no learned guess pending");
+      learned_pending = false;
+      SCIHPC_HIP(hipEventSynchronize(lev));
+      const unsigned nb = ceil_div(d[0].n, kBlock);
+      double sg[4] = {0, 0, 0, 0};
+      for (int k = 0; k < 4; ++k)
+        for (unsigned q = 0; q < nb; ++q) sg[k] += lhost[k * nb + q];
+      gb = sg[0];
+      gag = sg[1];
+      const double b2 = vec_norm2_cached_;
+      const double alpha = gag != 0.0 ? gb / gag : 0.0;
+      const double r2 = b2 - 2.0 * alpha * sg[2] + alpha * alpha * sg[3];
+      rel = std::isfinite(r2) && b2 > 0.0 ? std::sqrt(std::fabs(r2) / b2) : -1.0;
+    }
+    st.learned_residual = rel;
+    auto &l0 = d[0];
+    const long n = l0.n;
+    const unsigned nb = ceil_div(n, kBlock);
+    SCIHPC_HIP(hipStreamWaitEvent(stream, lev, 0));
+    k_combine_dots<<<nb, kBlock, 0, stream>>>(x, r, l0.rhs, l0.sol, n, partial, (int)nb);
+    SCIHPC_CHECK_LAUNCH();
+    SCIHPC_HIP(hipMemcpyAsync(host_part, partial, sizeof(double) * 3 * nb, hipMemcpyDeviceToHost, stream));
+    SCIHPC_HIP(hipStreamSynchronize(stream));
+    double xb = 0.0, xr = 0.0, gr = 0.0;
+    for (unsigned q = 0; q < nb; ++q) {
+      xb += host_part[q];
+      xr += host_part[nb + q];
+      gr += host_part[2 * nb + q];
+    }
+    const double xax = xb - xr, xag = gb - gr;
+    const double det = xax * gag - xag * xag;
+    double a = 1.0, c = 0.0;
+    bool ok = std::isfinite(det) && std::isfinite(gb) && gag != 0.0 && std::fabs(det) > 1e-14 * std::fabs(xax * gag);
+    if (ok) {
+      a = (xb * gag - xag * gb) / det;
+      c = (xax * gb - xag * xb) / det;
+      ok = std::isfinite(a) && std::isfinite(c);
+    }
+    auto energy = [&](double ca, double cc) {
+      return ca * xb + cc * gb - 0.5 * (ca * ca * xax + 2.0 * ca * cc * xag + cc * cc * gag);
+    };
+    const double jk = energy(1.0, 0.0);
+    const bool keep = ok && energy(a, c) < jk - kCombineGain * std::fabs(jk);
+    if (!keep) {
+      st.learned_rejected++;
+      return false;
+    }
+    st.learned_accepted++;
+    k_axpby<<<nb, kBlock, 0, stream>>>(x, l0.sol, n, a, c);
+    SCIHPC_CHECK_LAUNCH();
+    return true;
   }
 
   // judge the pending guess: kept if its relative FP64 residual is below

@@ -359,18 +359,33 @@ __global__ void __launch_bounds__(kEB) k_make_input(const float *__restrict__ fe
 inline unsigned nblk(long n) { return ceil_div(n, kEB); }
 
 // y = act(conv(x, w) + b) over s (2-D or 3-D); NB = s.cout / 16 output blocks
+// Small images (fewer pixel tiles than kSplitBelow) split their output
+// channels over workgroups, one or two 16-channel blocks each, so the
+// coarse levels of the network (few pixels, up to 192 channels) still
+// fill the GPU; large ones keep every output block in one workgroup, which
+// reads each input tile once.
+constexpr long kSplitBelow = 1024;
 template <int DIM>
 void launch_conv_dim(const ConvShape &s, View x, View mask, const bf16_t *w, const float *b, View y, int relu,
                      hipStream_t st) {
-  const unsigned grid = (unsigned)((long)s.n * s.d * s.h * ((s.w + 63) / 64));
-  switch (s.cout / 16) {
-    case 1: k_conv<1, DIM><<<grid, kConvThreads, 0, st>>>(x, mask, w, b, y, s, relu); break;
-    case 2: k_conv<2, DIM><<<grid, kConvThreads, 0, st>>>(x, mask, w, b, y, s, relu); break;
-    case 3: k_conv<3, DIM><<<grid, kConvThreads, 0, st>>>(x, mask, w, b, y, s, relu); break;
-    case 4: k_conv<4, DIM><<<grid, kConvThreads, 0, st>>>(x, mask, w, b, y, s, relu); break;
-    case 6: k_conv<6, DIM><<<grid, kConvThreads, 0, st>>>(x, mask, w, b, y, s, relu); break;
-    case 8: k_conv<8, DIM><<<grid, kConvThreads, 0, st>>>(x, mask, w, b, y, s, relu); break;
-    case 12: k_conv<12, DIM><<<grid, kConvThreads, 0, st>>>(x, mask, w, b, y, s, relu); break;
+  const unsigned tiles = (unsigned)((long)s.n * s.d * s.h * ((s.w + 63) / 64));
+  const int blocks = s.cout / 16;
+  if ((long)tiles < kSplitBelow && blocks > 1) {
+    const int per = blocks % 2 == 0 && (long)tiles * blocks >= 4 * kSplitBelow ? 2 : 1;
+    const dim3 grid(tiles, blocks / per);
+    if (per == 2) k_conv<2, DIM><<<grid, kConvThreads, 0, st>>>(x, mask, w, b, y, s, relu);
+    else k_conv<1, DIM><<<grid, kConvThreads, 0, st>>>(x, mask, w, b, y, s, relu);
+    SCIHPC_CHECK_LAUNCH();
+    return;
+  }
+  switch (blocks) {
+    case 1: k_conv<1, DIM><<<tiles, kConvThreads, 0, st>>>(x, mask, w, b, y, s, relu); break;
+    case 2: k_conv<2, DIM><<<tiles, kConvThreads, 0, st>>>(x, mask, w, b, y, s, relu); break;
+    case 3: k_conv<3, DIM><<<tiles, kConvThreads, 0, st>>>(x, mask, w, b, y, s, relu); break;
+    case 4: k_conv<4, DIM><<<tiles, kConvThreads, 0, st>>>(x, mask, w, b, y, s, relu); break;
+    case 6: k_conv<6, DIM><<<tiles, kConvThreads, 0, st>>>(x, mask, w, b, y, s, relu); break;
+    case 8: k_conv<8, DIM><<<tiles, kConvThreads, 0, st>>>(x, mask, w, b, y, s, relu); break;
+    case 12: k_conv<12, DIM><<<tiles, kConvThreads, 0, st>>>(x, mask, w, b, y, s, relu); break;
     default: throw Error(SCIHPC_BAD_ARG, "unsupported convolution width");
   }
   SCIHPC_CHECK_LAUNCH();

@@ -78,7 +78,11 @@ constexpr int kConvThreads = 256;  // 4 wavefronts x 16 pixels = 64 pixels per w
 
 // y = act(conv(x, w) + b); w layout [cout][T][cin] (BF16), b FP32 (may be
 // null).  mask: when given, x is read as x * (mask > 0) (ReLU derivative of
-// the layer whose output gradient x is, for the backward-data use).
+// the layer whose output gradient x is, for the backward-data use).  A
+// workgroup computes NB output blocks of 16 channels, from block
+// NB * blockIdx.y: a grid with gridDim.y > 1 splits the output channels of a
+// small image over more workgroups (the coarse levels have few pixels and
+// many channels).
 template <int NB, int DIM>
 __global__ void __launch_bounds__(kConvThreads)
     k_conv(View x, View mask, const bf16_t *__restrict__ w, const float *__restrict__ b, View y, ConvShape s,
@@ -94,6 +98,7 @@ __global__ void __launch_bounds__(kConvThreads)
   const int z = blk % s.d, img = blk / s.d;
   const int px = seg * 64 + wave * 16 + (lane & 15);  // pixel of this lane's A row
   const int kl = 8 * (lane >> 4);                    // k offset of this lane inside a step
+  const int nb0 = NB * blockIdx.y;                   // first output block of this workgroup
   floatx4 acc[NB];
 #pragma unroll
   for (int nb = 0; nb < NB; ++nb) acc[nb] = floatx4{0.f, 0.f, 0.f, 0.f};
@@ -131,7 +136,8 @@ __global__ void __launch_bounds__(kConvThreads)
         }
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb)
-          bw[g][nb] = *reinterpret_cast<const uint4 *>(w + ((long)(nb * 16 + (lane & 15)) * T + t) * s.cin + c);
+          bw[g][nb] =
+              *reinterpret_cast<const uint4 *>(w + ((long)((nb0 + nb) * 16 + (lane & 15)) * T + t) * s.cin + c);
       }
     }
 #pragma unroll
@@ -146,7 +152,7 @@ __global__ void __launch_bounds__(kConvThreads)
   const long rowpix = (((long)img * s.d + z) * s.h + row) * s.w;
 #pragma unroll
   for (int nb = 0; nb < NB; ++nb) {
-    const int co = nb * 16 + (lane & 15);
+    const int co = (nb0 + nb) * 16 + (lane & 15);
     const float bias = b ? b[co] : 0.f;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
