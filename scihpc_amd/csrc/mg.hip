@@ -698,41 +698,47 @@ __device__ __forceinline__ void fcg_sums_of(double a, double b, double sp, doubl
   pq = a;
   pr = b;
 }
+// The partials reach wave 0 through LDS in chunks of kDirChunk, loaded by
+// the whole workgroup at once (one memory round trip per chunk instead of
+// one per partial and lane); lane l then adds partials l, l + kWave,
+// l + 2 kWave, ... in that order, whatever the workgroup size.
+constexpr int kDirChunk = 512;
 __device__ __forceinline__ void fcg_dir_sums(const double *__restrict__ part, int nb, double corr_n, double &pq,
                                              double &pr) {
   __shared__ double ta, tb;
-  if (threadIdx.x < kWave) {
-    double a = 0.0, b = 0.0, sp = 0.0, sr = 0.0, sq = 0.0;
-    // U partials per lane are loaded before they are added (one memory
-    // round trip per U instead of per partial); the additions keep the
-    // order k, k + kWave, k + 2 kWave, ...
-    constexpr int U = 8;
-    for (int k0 = threadIdx.x; k0 < nb; k0 += U * kWave) {
-      double va[U], vb[U];
-#pragma unroll
-      for (int j = 0; j < U; ++j) {
-        const int k = k0 + j * kWave;
-        va[j] = k < nb ? part[k] : 0.0;
-        vb[j] = k < nb ? part[nb + k] : 0.0;
+  __shared__ double buf[5][kDirChunk];
+  const bool corr = corr_n > 0.0;
+  double a = 0.0, b = 0.0, sp = 0.0, sr = 0.0, sq = 0.0;
+  for (int c0 = 0; c0 < nb; c0 += kDirChunk) {
+    for (int t = threadIdx.x; t < kDirChunk; t += blockDim.x) {
+      const int k = c0 + t;
+      const bool in = k < nb;
+      buf[0][t] = in ? part[k] : 0.0;
+      buf[1][t] = in ? part[nb + k] : 0.0;
+      if (corr) {
+        buf[2][t] = in ? part[2 * nb + k] : 0.0;
+        buf[3][t] = in ? part[3 * nb + k] : 0.0;
+        buf[4][t] = in ? part[4 * nb + k] : 0.0;
       }
-#pragma unroll
-      for (int j = 0; j < U; ++j) {
-        if (k0 + j * kWave < nb) {
-          a += va[j];
-          b += vb[j];
-        }
-      }
-      if (corr_n > 0.0) {
-        for (int k = k0; k < nb && k < k0 + U * kWave; k += kWave) {
-          sp += part[2 * nb + k];
-          sr += part[3 * nb + k];
-          sq += part[4 * nb + k];
+    }
+    __syncthreads();
+    if (threadIdx.x < kWave) {
+      for (int t = threadIdx.x; t < kDirChunk && c0 + t < nb; t += kWave) {
+        a += buf[0][t];
+        b += buf[1][t];
+        if (corr) {
+          sp += buf[2][t];
+          sr += buf[3][t];
+          sq += buf[4][t];
         }
       }
     }
+    __syncthreads();
+  }
+  if (threadIdx.x < kWave) {
     a = wave_sum(a);
     b = wave_sum(b);
-    if (corr_n > 0.0) {
+    if (corr) {
       sp = wave_sum(sp);
       sr = wave_sum(sr);
       sq = wave_sum(sq);
