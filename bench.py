@@ -524,35 +524,43 @@ def cpu_momentum_baseline():
 def bench_interface(reps=5):
     """Interface maintenance at the dam-break shape (1024 x 512): one UCCD
     TVD-RK3 step, the 5 mpls Euler steps of a time step and one
-    redistancing RK3 step, device-resident phi."""
-    import numpy as np
-
+    redistancing RK3 step, device-resident phi; with the sequential line
+    solver (the default, bit-identical to the reference) and with the
+    partitioned one."""
     from scihpc_amd import LevelSet
 
     wl = momentum_workload()
     nx, ny = wl["shape"]
     h = wl["h"]
-    ls = LevelSet(2, nx, ny, 1, h, h, 1.0, ["neumann"] * 6)
-    ls.set_phi(wl["phi"])
-    ls.set_velocity(wl["u"], wl["v"])
-    ls.set_scheme("uccd")
-    dt, lw, ratio = 0.01 * h, 1.5 * h, 1.226e-3
-    ms_rk3, _ = ls.time_step(dt, iters=reps)
-    m0 = ls.mass(lw, ratio)
-    ls.mpls(dt, lw, ratio, m0, count=1)
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        ls.mpls(dt, lw, ratio, m0, count=5)
-    ms_mpls = (time.perf_counter() - t0) / reps * 1e3
-    ls.find_sign(lw)
-    ls.find_delta(lw)
-    ls.redistance(dt, 0.25 * h, 1)
-    t0 = time.perf_counter()
-    ls.redistance(dt, 0.25 * h, reps)
-    ms_red = (time.perf_counter() - t0) / reps * 1e3
-    ls.close()
-    return {"workload": f"dam break {nx}x{ny} level set", "uccd_rk3_step_ms": round(ms_rk3, 4),
-            "mpls_5_steps_ms": round(ms_mpls, 4), "redistance_rk3_step_ms": round(ms_red, 4)}
+    out = {"workload": f"dam break {nx}x{ny} level set"}
+    for solver in ("sequential", "partitioned"):
+        ls = LevelSet(2, nx, ny, 1, h, h, 1.0, ["neumann"] * 6)
+        ls.set_line_solver(solver)
+        ls.set_phi(wl["phi"])
+        ls.set_velocity(wl["u"], wl["v"])
+        ls.set_scheme("uccd")
+        dt, lw, ratio = 0.01 * h, 1.5 * h, 1.226e-3
+        ms_rk3, _ = ls.time_step(dt, iters=reps)
+        m0 = ls.mass(lw, ratio)
+        ls.mpls(dt, lw, ratio, m0, count=1)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ls.mpls(dt, lw, ratio, m0, count=5)
+        ms_mpls = (time.perf_counter() - t0) / reps * 1e3
+        ls.find_sign(lw)
+        ls.find_delta(lw)
+        ls.redistance(dt, 0.25 * h, 1)
+        t0 = time.perf_counter()
+        ls.redistance(dt, 0.25 * h, reps)
+        ms_red = (time.perf_counter() - t0) / reps * 1e3
+        ls.close()
+        row = {"uccd_rk3_step_ms": round(ms_rk3, 4), "mpls_5_steps_ms": round(ms_mpls, 4),
+               "redistance_rk3_step_ms": round(ms_red, 4)}
+        if solver == "sequential":
+            out.update(row)
+        else:
+            out["partitioned_line_solver"] = row
+    return out
 
 
 def cpu_interface_baseline():

@@ -449,6 +449,18 @@ enum { SCIHPC_LS_WENO5 = 0, SCIHPC_LS_UCCD = 1 };
  * uccd_solver.cpp:14-215: compact upwind/downwind line systems chosen by the
  * sign of the node velocity), updated on every padded cell as the reference. */
 SCIHPC_API int scihpc_ls_set_scheme(scihpc_ls *h, int scheme);
+/* How the CCD/UCCD line systems are solved (the twin back substitution of
+ * matrix_solver.cpp:58-83 on the factorised coefficients of twin_dec,
+ * :21-56).  partitioned = 0 (the default): one thread per line in the
+ * reference's order, bit-identical to it.  partitioned = 1: every line cut
+ * into segments that sweep in parallel and exchange their incoming states
+ * through 2x2 segment propagators, then sweep again from the true states
+ * with the reference's arithmetic; the results differ from the sequential
+ * sweep only by the rounding of the carried states, within the sequential
+ * sweep's own rounding error (lines longer than 2176 points always sweep
+ * sequentially).  No counterpart in the reference (its solve is always
+ * sequential); for callers that trade the reference's bits for speed. */
+SCIHPC_API int scihpc_ls_set_line_solver(scihpc_ls *h, int partitioned);
 /* uccd_solver::find_derivatives(f, vel) (upwind = 1, the handle's
  * velocity) or ccd_solver::find_derivatives(f) (ccd_solver.cpp:14-142) of
  * a padded host array: first and second derivatives along every axis. */

@@ -2084,6 +2084,42 @@ static void orc_twin_bks(const double *c, int n, double *s, double *ss, long st)
 #undef SS
 }
 
+/* orc_twin_bks carried out in long double (x86-64: 64-bit significand) on
+ * the same double right-hand sides and factorised coefficients, rounded to
+ * double at the end: the sweep's result with its own rounding error made
+ * about 2^11 times smaller, the yardstick for line solvers that round
+ * differently from the reference's sequential order. */
+static void orc_twin_bks_ld(const double *c, int n, double *s, double *ss) {
+  const double *a[3] = {c, c + n, c + 2 * n}, *b[3] = {c + 3 * n, c + 4 * n, c + 5 * n};
+  const double *aa[3] = {c + 6 * n, c + 7 * n, c + 8 * n}, *bb[3] = {c + 9 * n, c + 10 * n, c + 11 * n};
+  long double *S = malloc(sizeof(long double) * 2 * n), *SS = S + n;
+  for (int i = 0; i < n; i++) {
+    S[i] = s[i];
+    SS[i] = ss[i];
+  }
+  for (int i = 1; i < n; i++) {
+    S[i] = S[i] - ((long double)a[0][i] * S[i - 1] + (long double)b[0][i] * SS[i - 1]);
+    SS[i] = SS[i] - ((long double)aa[0][i] * S[i - 1] + (long double)bb[0][i] * SS[i - 1]);
+  }
+  long double den = (long double)a[1][n - 1] * bb[1][n - 1] - (long double)aa[1][n - 1] * b[1][n - 1];
+  long double sols = -((long double)b[1][n - 1] * SS[n - 1] - (long double)bb[1][n - 1] * S[n - 1]) / den;
+  long double solss = ((long double)a[1][n - 1] * SS[n - 1] - (long double)aa[1][n - 1] * S[n - 1]) / den;
+  S[n - 1] = sols;
+  SS[n - 1] = solss;
+  for (int i = n - 2; i > -1; --i) {
+    const long double si = S[i] - ((long double)a[2][i] * S[i + 1] + (long double)b[2][i] * SS[i + 1]);
+    const long double ssi = SS[i] - ((long double)aa[2][i] * S[i + 1] + (long double)bb[2][i] * SS[i + 1]);
+    den = (long double)a[1][i] * bb[1][i] - (long double)aa[1][i] * b[1][i];
+    S[i] = -((long double)b[1][i] * ssi - (long double)bb[1][i] * si) / den;
+    SS[i] = ((long double)a[1][i] * ssi - (long double)aa[1][i] * si) / den;
+  }
+  for (int i = 0; i < n; i++) {
+    s[i] = (double)S[i];
+    ss[i] = (double)SS[i];
+  }
+  free(S);
+}
+
 /* the boundary closure shared by CCD and UCCD (ccd_base.cpp:63-101) */
 static void orc_compact_ends(double *c, int n, double h) {
   double *a[3] = {c, c + n, c + 2 * n}, *b[3] = {c + 3 * n, c + 4 * n, c + 5 * n};
@@ -2153,7 +2189,7 @@ static void orc_compact_rhs(const double *f, long st, int n, double h, int kind,
 /* every padded line along axis a: CCD (vel = NULL) -> d1 = s, d2 = ss;
  * UCCD -> d1 = upwind or downwind by vel > 0, d2 = mean of the two ss */
 static void orc_compact_axis(const orc_pf *p, int a, double h, const double *f, const double *vel,
-                             double *d1, double *d2) {
+                             double *d1, double *d2, int extended) {
   const int n = p->N[a];
   const long st = p->s[a];
   double *cu = malloc(sizeof(double) * 12 * n), *cd = malloc(sizeof(double) * 12 * n);
@@ -2171,8 +2207,13 @@ static void orc_compact_axis(const orc_pf *p, int a, double h, const double *f, 
         /* the downwind ends equal the upwind ones (uccd_solver.cpp:47-50) */
         sd[0] = su[0]; ssd[0] = ssu[0]; sd[n - 1] = su[n - 1]; ssd[n - 1] = ssu[n - 1];
       }
-      orc_twin_bks(cu, n, su, ssu, 1);
-      if (vel) orc_twin_bks(cd, n, sd, ssd, 1);
+      if (extended) {
+        orc_twin_bks_ld(cu, n, su, ssu);
+        if (vel) orc_twin_bks_ld(cd, n, sd, ssd);
+      } else {
+        orc_twin_bks(cu, n, su, ssu, 1);
+        if (vel) orc_twin_bks(cd, n, sd, ssd, 1);
+      }
       for (int i = 0; i < n; i++) {
         const long c = base + (long)i * st;
         if (vel) {
@@ -2198,15 +2239,28 @@ void orc_compact_derivatives(int ndim, int nx, int ny, int nz, double dx, double
   const double hh[3] = {dx, dy, dz};
   const double *vel[3] = {u, v, w};
   double *d1[3] = {fx, fy, fz}, *d2[3] = {fxx, fyy, fzz};
-  for (int a = 0; a < ndim; a++) orc_compact_axis(&p, a, hh[a], flux, upwind ? vel[a] : NULL, d1[a], d2[a]);
+  for (int a = 0; a < ndim; a++) orc_compact_axis(&p, a, hh[a], flux, upwind ? vel[a] : NULL, d1[a], d2[a], 0);
+}
+
+/* orc_compact_derivatives with the twin sweeps in long double
+ * (orc_twin_bks_ld): the derivatives to within about 2^-11 of the double
+ * sweep's rounding error */
+void orc_compact_derivatives_ld(int ndim, int nx, int ny, int nz, double dx, double dy, double dz,
+                                const double *flux, const double *u, const double *v, const double *w, int upwind,
+                                double *fx, double *fy, double *fz, double *fxx, double *fyy, double *fzz) {
+  const orc_pf p = pf_make(ndim, nx, ny, nz);
+  const double hh[3] = {dx, dy, dz};
+  const double *vel[3] = {u, v, w};
+  double *d1[3] = {fx, fy, fz}, *d2[3] = {fxx, fyy, fzz};
+  for (int a = 0; a < ndim; a++) orc_compact_axis(&p, a, hh[a], flux, upwind ? vel[a] : NULL, d1[a], d2[a], 1);
 }
 
 /* runge_kutta::tvd_rk3(phi, nvel, identity_flux, convection) with the UCCD
  * convection term (source.cpp:8-40, uccd_solver.cpp:14-215) and the
  * scalar BCs after each stage (runge_kutta.cpp:13-49)                  */
-void orc_tvd_rk3_uccd(int ndim, int nx, int ny, int nz, double dx, double dy, double dz, double dt,
-                      const int *types, const double *values, double *phi, const double *u, const double *v,
-                      const double *w, int nsteps) {
+static void orc_tvd_rk3_uccd_impl(int ndim, int nx, int ny, int nz, double dx, double dy, double dz, double dt,
+                                  const int *types, const double *values, double *phi, const double *u,
+                                  const double *v, const double *w, int nsteps, int extended) {
   const orc_pf p = pf_make(ndim, nx, ny, nz);
   const long T = (long)p.N[0] * p.N[1] * p.N[2];
   double *m = calloc((size_t)9 * T, sizeof(double));
@@ -2214,7 +2268,10 @@ void orc_tvd_rk3_uccd(int ndim, int nx, int ny, int nz, double dx, double dy, do
   double *fxx = m + 6 * T, *fyy = m + 7 * T, *fzz = m + 8 * T;
   for (int step = 0; step < nsteps; step++)
     for (int stage = 0; stage < 3; stage++) {
-      orc_compact_derivatives(ndim, nx, ny, nz, dx, dy, dz, phi, u, v, w, 1, fx, fy, fz, fxx, fyy, fzz);
+      if (extended)
+        orc_compact_derivatives_ld(ndim, nx, ny, nz, dx, dy, dz, phi, u, v, w, 1, fx, fy, fz, fxx, fyy, fzz);
+      else
+        orc_compact_derivatives(ndim, nx, ny, nz, dx, dy, dz, phi, u, v, w, 1, fx, fy, fz, fxx, fyy, fzz);
       double *r = s[stage];
       for (long c = 0; c < T; c++) r[c] = -u[c] * fx[c];
       if (ndim > 1)
@@ -2229,6 +2286,19 @@ void orc_tvd_rk3_uccd(int ndim, int nx, int ny, int nz, double dx, double dy, do
       orc_bc_apply(ndim, nx, ny, nz, dx, dy, dz, types, values, -1, phi);
     }
   free(m);
+}
+
+void orc_tvd_rk3_uccd(int ndim, int nx, int ny, int nz, double dx, double dy, double dz, double dt,
+                      const int *types, const double *values, double *phi, const double *u, const double *v,
+                      const double *w, int nsteps) {
+  orc_tvd_rk3_uccd_impl(ndim, nx, ny, nz, dx, dy, dz, dt, types, values, phi, u, v, w, nsteps, 0);
+}
+
+/* orc_tvd_rk3_uccd on orc_compact_derivatives_ld's derivatives */
+void orc_tvd_rk3_uccd_ld(int ndim, int nx, int ny, int nz, double dx, double dy, double dz, double dt,
+                         const int *types, const double *values, double *phi, const double *u, const double *v,
+                         const double *w, int nsteps) {
+  orc_tvd_rk3_uccd_impl(ndim, nx, ny, nz, dx, dy, dz, dt, types, values, phi, u, v, w, nsteps, 1);
 }
 
 /* lsf_mass (wrapper_func.cpp:450-511): density-weighted Heaviside mass,

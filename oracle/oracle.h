@@ -257,12 +257,26 @@ void orc_compact_derivatives(int ndim, int nx, int ny, int nz, double dx, double
                              const double *flux, const double *u, const double *v, const double *w,
                              int upwind, double *fx, double *fy, double *fz, double *fxx, double *fyy,
                              double *fzz);
+/* the same with the twin sweeps in long double (64-bit significand), rounded
+ * to double at the end: the yardstick of a line solver's rounding error
+ * [math: pinned by agreeing with the double sweep to within that sweep's
+ * rounding error, and with the analytic derivatives of smooth fields] */
+void orc_compact_derivatives_ld(int ndim, int nx, int ny, int nz, double dx, double dy, double dz,
+                                const double *flux, const double *u, const double *v, const double *w,
+                                int upwind, double *fx, double *fy, double *fz, double *fxx, double *fyy,
+                                double *fzz);
 void orc_ccd_coefficients(int n, double h, double *c);
 void orc_uccd_coefficients(int n, double h, double *up, double *dn);
 /* tvd_rk3 with the UCCD convection term (source.cpp:8-40)     [golden] */
 void orc_tvd_rk3_uccd(int ndim, int nx, int ny, int nz, double dx, double dy, double dz, double dt,
                       const int *types, const double *values, double *phi, const double *u,
                       const double *v, const double *w, int nsteps);
+/* the same on orc_compact_derivatives_ld's derivatives (the yardstick of a
+ * line solver's rounding error after advection)
+ * [math: pinned through orc_compact_derivatives_ld] */
+void orc_tvd_rk3_uccd_ld(int ndim, int nx, int ny, int nz, double dx, double dy, double dz, double dt,
+                         const int *types, const double *values, double *phi, const double *u,
+                         const double *v, const double *w, int nsteps);
 /* lsf_mass (wrapper_func.cpp:450-511)                         [golden] */
 double orc_lsf_mass(int ndim, int nx, int ny, int nz, double dv, double ls_width, double ratio,
                     int positive_ref, const double *phi);

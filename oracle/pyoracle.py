@@ -120,7 +120,9 @@ class Oracle:
         _sig(L, "orc_momentum_step", _i, [_i, _i, _i, _i, _d, _d, _d, _dp, _ip, _dp] + [_dp] * 10)
         _sig(L, "orc_node_velocity", None, [_i, _i, _i, _i, _d, _d, _d, _ip] + [_dp] * 6)
         _sig(L, "orc_compact_derivatives", None, [_i, _i, _i, _i, _d, _d, _d, _dp, _vp, _vp, _vp, _i] + [_dp] * 6)
+        _sig(L, "orc_compact_derivatives_ld", None, [_i, _i, _i, _i, _d, _d, _d, _dp, _vp, _vp, _vp, _i] + [_dp] * 6)
         _sig(L, "orc_tvd_rk3_uccd", None, [_i, _i, _i, _i, _d, _d, _d, _d, _ip, _dp, _dp, _dp, _dp, _dp, _i])
+        _sig(L, "orc_tvd_rk3_uccd_ld", None, [_i, _i, _i, _i, _d, _d, _d, _d, _ip, _dp, _dp, _dp, _dp, _dp, _i])
         _sig(L, "orc_lsf_mass", _d, [_i, _i, _i, _i, _d, _d, _d, _i, _dp])
         _sig(L, "orc_mpls_euler", None, [_i, _i, _i, _i, _d, _d, _d, _d, _d, _d, _i, _d, _ip, _dp, _dp, _i, _vp])
         _sig(L, "orc_find_sign", None, [_l, _dp, _d, _dp])

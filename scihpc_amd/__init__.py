@@ -384,6 +384,15 @@ class LevelSet:
         code = {"weno5": 0, "uccd": 1}[scheme] if isinstance(scheme, str) else int(scheme)
         check(self._lib.scihpc_ls_set_scheme(self.h, code))
 
+    def set_line_solver(self, solver):
+        """CCD/UCCD line systems: "sequential" (the default, one sweep per
+        line in the reference's order, bit-identical to matrix_solver.cpp's
+        twin_bks) or "partitioned" (segments swept in parallel, carried
+        states exchanged through 2x2 propagators; within the sequential
+        sweep's own rounding error)."""
+        code = {"sequential": 0, "partitioned": 1}[solver]
+        check(self._lib.scihpc_ls_set_line_solver(self.h, code))
+
     def compact_derivatives(self, f, upwind=True):
         """UCCD (by the handle's velocity) or CCD derivatives of a padded
         field: (fx, fy, fz, fxx, fyy, fzz)."""

@@ -147,6 +147,7 @@ SIGNATURES = {
     "scihpc_ls_time_step": [_vp, _d, _i, _pd, _pd],
     "scihpc_ls_stabilize_gradient": [_vp, _dp, _pd],
     "scihpc_ls_set_scheme": [_vp, _i],
+    "scihpc_ls_set_line_solver": [_vp, _i],
     "scihpc_ls_compact_derivatives": [_vp, _dp, _i] + [_dp] * 6,
     "scihpc_ls_mass": [_vp, _d, _d, _i, _pd],
     "scihpc_ls_mpls": [_vp, _d, _d, _d, _i, _d, _i],

@@ -12,6 +12,7 @@
 // buffer; one small kernel per axis then fills that buffer's ghosts in the
 // reference's order (x, y, z; right side before left side).  Corner ghosts
 // are never written, as in the reference.
+#include <algorithm>
 #include <cmath>
 #include <utility>
 #include <vector>
@@ -943,6 +944,187 @@ __global__ void __launch_bounds__(64) k_twin_line(Grid g, int nsys, LineCoefs co
   }
 }
 
+// The twin sweeps partitioned along the line, for the latency of the
+// recurrence: k_twin_line's chain of 2 n dependent steps per line (with two
+// FP64 divisions per backward step) on a few thousand threads becomes about
+// 4 L steps on ceil(n / L) threads per line, L = 9 (17 for lines of more
+// than 1152 points) positions a segment.  Both sweeps are
+// affine recurrences in the state (s, ss):
+//   forward   y_i = r_i - M_i y_{i-1},               M_i = [a0 b0; aa0 bb0]
+//   backward  u_i = D_i^-1 (y_i - N_i u_{i+1}),      D_i = [a1 b1; aa1 bb1],
+//                                                    N_i = [a2 b2; aa2 bb2]
+// so a segment's end state is its end state from a zero incoming state
+// plus a fixed 2x2 propagator times the true incoming state (Phi: the
+// product of -M_i over the segment; Psi: of -D_i^-1 N_i, from the top; both
+// depend only on the coefficients and are formed on the host).  A
+// workgroup holds LINES lines x SEGS segments: (1) every segment runs its
+// recurrence from a zero incoming state (segment 0 from y_0, the top
+// segment from the closure u_{n-1}: both exact), (2) one thread per line
+// chains the segment ends through the propagators (staged in LDS), giving every
+// segment its true incoming state, (3) every segment runs the recurrence
+// again from that state, with k_twin_line's arithmetic, and keeps the
+// result -- so a value differs from the sequential sweep's only through the
+// carried incoming state, whose rounding the contracting recurrences damp
+// (|eig M_i| <= 0.25 and |eig D_i^-1 N_i| <= 0.65 inside the line).  The
+// segment's right-hand sides stay in registers between the passes: HBM
+// sees one read and one write per value.  Thread t: line t % LINES of the
+// group, segment t / LINES; block b: axis, then system, then line group.
+struct LineCarry {
+  const double *phi[3][2], *psi[3][2];  // [axis][system]: SEGS 2x2 propagators, row-major
+};
+
+template <int LINES, int L, bool CL>  // L: positions a segment (in registers); CL: coefficients in LDS
+__global__ void __launch_bounds__(1024) k_twin_part(Grid g, int nsys, LineCoefs co, LineScratch sc, LineCarry cr) {
+#pragma clang fp contract(off)
+  constexpr int SEGS = 1024 / LINES;
+  __shared__ double e0[SEGS][LINES], e1[SEGS][LINES];  // segment end states, then incoming states
+  __shared__ double prop[2][4 * SEGS];                  // Phi and Psi of this axis and system
+  int b = blockIdx.x, a = 0, sys = 0;
+  long lines = 0;
+  for (; a < g.ndim; ++a) {
+    lines = (long)g.N[(a + 1) % 3] * g.N[(a + 2) % 3];
+    const int nbl = (int)((lines + LINES - 1) / LINES);
+    if (b < nbl * nsys) {
+      sys = b / nbl;
+      b -= sys * nbl;
+      break;
+    }
+    b -= nbl * nsys;
+  }
+  if (a >= g.ndim) return;  // uniform over the block
+  const int n = g.N[a];
+  const int Lp = L, ns = (n + L - 1) / L, top = ns - 1;  // ns <= SEGS (host)
+  const int ln = (int)threadIdx.x % LINES, seg = (int)threadIdx.x / LINES;
+  long base = 0;
+  const bool line_ok = line_base(g, a, (long)b * LINES + ln, base);
+  const int lo = seg * Lp;
+  const int len = line_ok && lo < n ? min(Lp, n - lo) : 0;
+  for (int q = threadIdx.x; q < 4 * ns; q += blockDim.x) {
+    prop[0][q] = cr.phi[a][sys][q];
+    prop[1][q] = cr.psi[a][sys][q];
+  }
+  const double *cg = sys == 0 ? co.cu[a] : co.cd[a];
+  // the coefficients in LDS (dynamic, 96 n bytes) when the launch gave it
+  extern __shared__ double cl[];
+  if (CL)
+    for (int q = threadIdx.x; q < 12 * n; q += blockDim.x) cl[q] = cg[q];
+  const double *c = CL ? cl : cg;
+  const double *a0 = c, *a1 = c + n, *a2 = c + 2 * n, *b0 = c + 3 * n, *b1 = c + 4 * n, *b2 = c + 5 * n;
+  const double *aa0 = c + 6 * n, *aa1 = c + 7 * n, *aa2 = c + 8 * n, *bb0 = c + 9 * n, *bb1 = c + 10 * n,
+               *bb2 = c + 11 * n;
+  double *S = (sys == 0 ? sc.su[a] : sc.sd[a]) + base, *SS = (sys == 0 ? sc.ssu[a] : sc.ssd[a]) + base;
+  const long st = g.s[a];
+  double v[L], vv[L];
+#pragma unroll
+  for (int q = 0; q < L; ++q) {
+    if (q < len) {
+      v[q] = S[(long)(lo + q) * st];
+      vv[q] = SS[(long)(lo + q) * st];
+    }
+  }
+  // forward sweep over the segment from (p, pp); keep = store the states
+  auto forward = [&](double p, double pp, bool keep) {
+#pragma unroll
+    for (int q = 0; q < L; ++q) {
+      const int i = lo + q;
+      if (q < len && i > 0) {
+        const double si = v[q] - (a0[i] * p + b0[i] * pp);
+        const double ssi = vv[q] - (aa0[i] * p + bb0[i] * pp);
+        if (keep) {
+          v[q] = si;
+          vv[q] = ssi;
+        }
+        p = si;
+        pp = ssi;
+      }
+    }
+    e0[seg][ln] = p;
+    e1[seg][ln] = pp;
+  };
+  if (CL) __syncthreads();  // the coefficients are staged
+  forward(seg == 0 && len > 0 ? v[0] : 0.0, seg == 0 && len > 0 ? vv[0] : 0.0, false);
+  __syncthreads();
+  if (seg == 0 && line_ok) {
+    double y = e0[0][ln], yy = e1[0][ln];
+#pragma unroll 4
+    for (int k = 1; k < ns; ++k) {
+      const double *F = &prop[0][4 * k];
+      const double h0 = e0[k][ln], h1 = e1[k][ln];
+      e0[k][ln] = y;
+      e1[k][ln] = yy;
+      const double y0 = h0 + (F[0] * y + F[1] * yy), y1 = h1 + (F[2] * y + F[3] * yy);
+      y = y0;
+      yy = y1;
+    }
+  }
+  __syncthreads();
+  forward(seg == 0 ? (len > 0 ? v[0] : 0.0) : e0[seg][ln], seg == 0 ? (len > 0 ? vv[0] : 0.0) : e1[seg][ln], true);
+  // the closure at i = n - 1 (in the top segment's registers)
+  if (seg == top && len > 0) {
+    const int i = n - 1;
+#pragma unroll
+    for (int q = 0; q < L; ++q) {  // static register indices
+      if (q == len - 1) {
+        const double den = a1[i] * bb1[i] - aa1[i] * b1[i];
+        const double sn = -(b1[i] * vv[q] - bb1[i] * v[q]) / den;
+        const double ssn = (a1[i] * vv[q] - aa1[i] * v[q]) / den;
+        v[q] = sn;
+        vv[q] = ssn;
+      }
+    }
+  }
+  __syncthreads();  // the forward pass's stores to e0/e1 are read no more
+  // backward sweep over the segment from the incoming (p, pp) at lo + len
+  auto backward = [&](double p, double pp, bool keep) {
+#pragma unroll
+    for (int q = L - 1; q >= 0; --q) {
+      const int i = lo + q;
+      if (q < len && i < n - 1) {
+        const double si = v[q] - (a2[i] * p + b2[i] * pp);
+        const double ssi = vv[q] - (aa2[i] * p + bb2[i] * pp);
+        const double den = a1[i] * bb1[i] - aa1[i] * b1[i];
+        const double sols = -(b1[i] * ssi - bb1[i] * si) / den;
+        const double solss = (a1[i] * ssi - aa1[i] * si) / den;
+        if (keep) {
+          v[q] = sols;
+          vv[q] = solss;
+        }
+        p = sols;
+        pp = solss;
+      } else if (q < len) {  // i = n - 1: the closure value, already in place
+        p = v[q];
+        pp = vv[q];
+      }
+    }
+    e0[seg][ln] = p;
+    e1[seg][ln] = pp;
+  };
+  backward(0.0, 0.0, false);
+  __syncthreads();
+  if (seg == 0 && line_ok) {
+    double u = e0[top][ln], uu = e1[top][ln];
+#pragma unroll 4
+    for (int k = top - 1; k >= 0; --k) {
+      const double *G = &prop[1][4 * k];
+      const double h0 = e0[k][ln], h1 = e1[k][ln];
+      e0[k][ln] = u;
+      e1[k][ln] = uu;
+      const double u0 = h0 + (G[0] * u + G[1] * uu), u1 = h1 + (G[2] * u + G[3] * uu);
+      u = u0;
+      uu = u1;
+    }
+  }
+  __syncthreads();
+  backward(seg == top ? 0.0 : e0[seg][ln], seg == top ? 0.0 : e1[seg][ln], true);
+#pragma unroll
+  for (int q = 0; q < L; ++q) {
+    if (q < len) {
+      S[(long)(lo + q) * st] = v[q];
+      SS[(long)(lo + q) * st] = vv[q];
+    }
+  }
+}
+
 template <bool UP>
 __global__ void __launch_bounds__(kLsBlock)
     k_compact_select(long total, int ndim, const double *__restrict__ u, const double *__restrict__ v,
@@ -1357,6 +1539,47 @@ static std::vector<double> compact_coefficients(int kind, int n, double h) {
   return c;
 }
 
+// Segment propagators of the partitioned twin sweeps (k_twin_part) for the
+// factorised coefficients cf (12 n) of one line system cut into segments of
+// Lp positions (the last one shorter): phi[4 k..] = product over segment
+// k of -M_i (first position innermost; segment 0 starts at 1),
+// psi[4 k..] = product over segment k of -D_i^-1 N_i from its top position
+// down (the top segment ends at n - 2), both 2x2 row-major.
+static void twin_propagators(const std::vector<double> &cf, int n, int Lp, double *phi, double *psi) {
+  const double *a0 = &cf[0], *a1 = &cf[n], *a2 = &cf[2 * n], *b0 = &cf[3 * n], *b1 = &cf[4 * n], *b2 = &cf[5 * n];
+  const double *aa0 = &cf[6 * n], *aa1 = &cf[7 * n], *aa2 = &cf[8 * n], *bb0 = &cf[9 * n], *bb1 = &cf[10 * n],
+               *bb2 = &cf[11 * n];
+  const int ns = (n + Lp - 1) / Lp;
+  auto mul = [](const double (&x)[4], const double (&y)[4], double (&z)[4]) {
+    z[0] = x[0] * y[0] + x[1] * y[2];
+    z[1] = x[0] * y[1] + x[1] * y[3];
+    z[2] = x[2] * y[0] + x[3] * y[2];
+    z[3] = x[2] * y[1] + x[3] * y[3];
+  };
+  for (int k = 0; k < ns; ++k) {
+    const int lo = k * Lp, hi = std::min(lo + Lp, n);
+    double F[4] = {1, 0, 0, 1}, T[4];
+    for (int i = std::max(lo, 1); i < hi; ++i) {
+      const double m[4] = {-a0[i], -b0[i], -aa0[i], -bb0[i]};
+      mul(m, F, T);
+      std::copy(T, T + 4, F);
+    }
+    std::copy(F, F + 4, phi + 4 * k);
+    double G[4] = {1, 0, 0, 1};
+    for (int i = std::min(hi, n - 1) - 1; i >= lo; --i) {
+      const double den = a1[i] * bb1[i] - aa1[i] * b1[i];
+      const double di[4] = {bb1[i] / den, -b1[i] / den, -aa1[i] / den, a1[i] / den};
+      const double nm[4] = {a2[i], b2[i], aa2[i], bb2[i]};
+      double P[4];
+      mul(di, nm, P);
+      for (double &x : P) x = -x;
+      mul(P, G, T);
+      std::copy(T, T + 4, G);
+    }
+    std::copy(G, G + 4, psi + 4 * k);
+  }
+}
+
 }  // namespace scihpc
 
 using namespace scihpc;
@@ -1400,6 +1623,15 @@ struct scihpc_ls {
   double *fxx = nullptr, *fyy = nullptr, *fzz = nullptr;
   double *sgn = nullptr, *dlt = nullptr, *grd = nullptr, *flux = nullptr;
   double *lpart = nullptr, *lhost = nullptr;
+  // partitioned twin sweeps (chosen by scihpc_ls_set_line_solver): lines
+  // per workgroup (16 or 8; 0 = lines too long to partition in registers,
+  // which always sweep sequentially), the segment propagators
+  // ([axis][CCD, UCCD up, UCCD down][Phi, Psi][segment] 2x2)
+  int twin_lines = 0, twin_len = 0;
+  double *carry = nullptr;
+  bool twin_partitioned = false;
+  int lds_max = 65536;      // LDS bytes a workgroup may hold on this device
+  size_t twin_lds_set = 0;  // dynamic LDS the partitioned kernels have been allowed
 
   void ensure_iface() {
     if (sgn) return;
@@ -1414,6 +1646,25 @@ struct scihpc_ls {
       SCIHPC_HIP(hipMemcpyAsync(coef_up[a], cu.data(), sizeof(double) * 12 * n, hipMemcpyHostToDevice, stream));
       SCIHPC_HIP(hipMemcpyAsync(coef_dn[a], cd.data(), sizeof(double) * 12 * n, hipMemcpyHostToDevice, stream));
       SCIHPC_HIP(hipStreamSynchronize(stream));  // the host vectors go out of scope
+    }
+    int nmax = 0;
+    for (int a = 0; a < g.ndim; ++a) nmax = std::max(nmax, g.N[a]);
+    // 16 lines x 64 segments or 8 x 128, segments of 9 (or 17) positions
+    twin_lines = nmax <= 64 * 9 ? 16 : (nmax <= 128 * 17 ? 8 : 0);
+    twin_len = nmax <= 128 * 9 ? 9 : 17;
+    if (twin_lines) {
+      const int segs = 1024 / twin_lines;
+      SCIHPC_HIP(hipMalloc(reinterpret_cast<void **>(&carry), sizeof(double) * 3 * 3 * 2 * 4 * segs));
+      std::vector<double> all((size_t)3 * 3 * 2 * 4 * segs, 0.0);
+      for (int a = 0; a < g.ndim; ++a)
+        for (int kind = 0; kind < 3; ++kind) {
+          const std::vector<double> cf = compact_coefficients(kind, g.N[a], g.h[a]);
+          double *phi = &all[(size_t)((a * 3 + kind) * 2 + 0) * 4 * segs];
+          double *psi = &all[(size_t)((a * 3 + kind) * 2 + 1) * 4 * segs];
+          twin_propagators(cf, g.N[a], twin_len, phi, psi);
+        }
+      SCIHPC_HIP(hipMemcpyAsync(carry, all.data(), sizeof(double) * all.size(), hipMemcpyHostToDevice, stream));
+      SCIHPC_HIP(hipStreamSynchronize(stream));
     }
     for (int a = 0; a < g.ndim; ++a)
       for (auto &q : ls_scr[a]) q = alloc();
@@ -1448,7 +1699,41 @@ struct scihpc_ls {
     if (upwind) k_compact_rhs<true><<<nbc, kLsBlock, 0, stream>>>(g, f, sc);
     else k_compact_rhs<false><<<nbc, kLsBlock, 0, stream>>>(g, f, sc);
     SCIHPC_CHECK_LAUNCH();
-    k_twin_line<<<ceil_div(work, 64), 64, 0, stream>>>(g, nsys, co, sc);
+    if (twin_lines && twin_partitioned) {
+      const int segs = 1024 / twin_lines;
+      LineCarry cr{};
+      long blocks = 0;
+      for (int a = 0; a < g.ndim; ++a) {
+        for (int sy = 0; sy < 2; ++sy) {
+          const int kind = upwind ? 1 + sy : 0;
+          cr.phi[a][sy] = carry + (size_t)((a * 3 + kind) * 2 + 0) * 4 * segs;
+          cr.psi[a][sy] = carry + (size_t)((a * 3 + kind) * 2 + 1) * 4 * segs;
+        }
+        blocks += (long)ceil_div((long)g.N[(a + 1) % 3] * g.N[(a + 2) % 3], twin_lines) * nsys;
+      }
+      int nmax = 0;
+      for (int a = 0; a < g.ndim; ++a) nmax = std::max(nmax, g.N[a]);
+      const size_t lds = sizeof(double) * 12 * (size_t)nmax;  // + at most 24 KiB static
+      if (twin_len == 9 && lds + 24576 <= (size_t)lds_max) {
+        if (lds > twin_lds_set) {
+          SCIHPC_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_twin_part<16, 9, true>),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+          SCIHPC_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_twin_part<8, 9, true>),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+          twin_lds_set = lds;
+        }
+        if (twin_lines == 16) k_twin_part<16, 9, true><<<(unsigned)blocks, 1024, lds, stream>>>(g, nsys, co, sc, cr);
+        else k_twin_part<8, 9, true><<<(unsigned)blocks, 1024, lds, stream>>>(g, nsys, co, sc, cr);
+      } else if (twin_lines == 16) {
+        k_twin_part<16, 9, false><<<(unsigned)blocks, 1024, 0, stream>>>(g, nsys, co, sc, cr);
+      } else if (twin_len == 9) {
+        k_twin_part<8, 9, false><<<(unsigned)blocks, 1024, 0, stream>>>(g, nsys, co, sc, cr);
+      } else {
+        k_twin_part<8, 17, false><<<(unsigned)blocks, 1024, 0, stream>>>(g, nsys, co, sc, cr);
+      }
+    } else {
+      k_twin_line<<<ceil_div(work, 64), 64, 0, stream>>>(g, nsys, co, sc);
+    }
     SCIHPC_CHECK_LAUNCH();
     double *e2[3] = {nullptr, nullptr, nullptr};
     double **dd = d2 ? d2 : e2;
@@ -1591,6 +1876,12 @@ extern "C" int scihpc_ls_create(int ndim, int nx, int ny, int nz, double dx, dou
         SCIHPC_REQUIRE((h->bc.type[2 * a] == SCIHPC_BC_PERIODIC) == (h->bc.type[2 * a + 1] == SCIHPC_BC_PERIODIC),
                        "periodic must be set on both sides of an axis");
       SCIHPC_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+      {
+        int dev = 0, lds = 0;
+        SCIHPC_HIP(hipGetDevice(&dev));
+        if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && lds > 0)
+          h->lds_max = lds;
+      }
       SCIHPC_HIP(hipEventCreate(&h->ev0));
       SCIHPC_HIP(hipEventCreate(&h->ev1));
       h->phi[0] = h->alloc();
@@ -1622,6 +1913,7 @@ extern "C" int scihpc_ls_destroy(scihpc_ls *h) {
   for (int a = 0; a < 3; ++a)
     for (double *p : {h->coef_ccd[a], h->coef_up[a], h->coef_dn[a]})
       if (p) (void)hipFree(p);
+  if (h->carry) (void)hipFree(h->carry);
   for (auto &axis : h->ls_scr)
     for (double *p : axis)
       if (p) (void)hipFree(p);
@@ -1711,6 +2003,14 @@ extern "C" int scihpc_ls_set_scheme(scihpc_ls *h, int scheme) {
     ls_check(h);
     SCIHPC_REQUIRE(scheme == SCIHPC_LS_WENO5 || scheme == SCIHPC_LS_UCCD, "unknown scheme");
     h->scheme = scheme;
+  });
+}
+
+extern "C" int scihpc_ls_set_line_solver(scihpc_ls *h, int partitioned) {
+  return guarded([&] {
+    ls_check(h);
+    SCIHPC_REQUIRE(partitioned == 0 || partitioned == 1, "partitioned must be 0 or 1");
+    h->twin_partitioned = partitioned != 0;
   });
 }
 

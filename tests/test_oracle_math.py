@@ -319,3 +319,88 @@ def test_bc_closed_forms(oracle):
     G = g.reshape(shp)
     assert np.all(G[7, 3:7] == 0.0) and np.all(G[2, 3:7] == 0.0)  # wall faces
     assert np.array_equal(G[8, 3:7], -A[6, 3:7])
+
+
+def _compact_rhs_line(f, h, kind):
+    """the right-hand sides of oracle.c's orc_compact_rhs, same expressions
+    in IEEE doubles (so the same values)"""
+    n = len(f)
+    s, ss = [0.0] * n, [0.0] * n
+    s[0] = (-3.5 * f[0] + 4.0 * f[1] + -0.5 * f[2]) / h
+    ss[0] = (9.0 * f[0] + -12.0 * f[1] + 3.0 * f[2]) / h / h
+    s[n - 1] = -(-3.5 * f[n - 1] + 4.0 * f[n - 2] + -0.5 * f[n - 3]) / h
+    ss[n - 1] = (9.0 * f[n - 1] + -12.0 * f[n - 2] + 3.0 * f[n - 3]) / h / h
+    c3, c2, c1 = -0.06096119008109, 1.99692238016218, -1.93596119008109
+    for i in range(1, n - 1):
+        if kind == 0:
+            s[i] = 15.0 / 16.0 * (f[i + 1] - f[i - 1]) / h
+        elif kind == 1:
+            s[i] = (c1 * f[i - 1] + c2 * f[i] + c3 * f[i + 1]) / h
+        else:
+            s[i] = -(c3 * f[i - 1] + c2 * f[i] + c1 * f[i + 1]) / h
+        ss[i] = (3.0 * f[i - 1] - 6.0 * f[i] + 3.0 * f[i + 1]) / h / h
+    return s, ss
+
+
+def _twin_exact(c, s, ss):
+    """the twin back substitution in exact rational arithmetic"""
+    from fractions import Fraction as Q
+    n = len(s)
+    a0, a1, a2, b0, b1, b2, aa0, aa1, aa2, bb0, bb1, bb2 = [[Q(x) for x in c[k * n:(k + 1) * n]] for k in range(12)]
+    S, SS = [Q(x) for x in s], [Q(x) for x in ss]
+    for i in range(1, n):
+        S[i], SS[i] = S[i] - (a0[i] * S[i - 1] + b0[i] * SS[i - 1]), SS[i] - (aa0[i] * S[i - 1] + bb0[i] * SS[i - 1])
+    for i in range(n - 1, -1, -1):
+        si, ssi = S[i], SS[i]
+        if i < n - 1:
+            si = si - (a2[i] * S[i + 1] + b2[i] * SS[i + 1])
+            ssi = ssi - (aa2[i] * S[i + 1] + bb2[i] * SS[i + 1])
+        den = a1[i] * bb1[i] - aa1[i] * b1[i]
+        S[i], SS[i] = -(b1[i] * ssi - bb1[i] * si) / den, (a1[i] * ssi - aa1[i] * si) / den
+    return np.array([float(x) for x in S]), np.array([float(x) for x in SS])
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2], ids=["ccd", "uccd-up", "uccd-down"])
+def test_compact_extended_sweep_against_exact(oracle, kind):
+    """orc_compact_derivatives_ld (the twin sweep in long double) against the
+    same factorised system solved in exact rational arithmetic on a rough
+    1-D line: within a few units of the last place of the exact values,
+    and never farther from them than the double sweep."""
+    import ctypes as C
+    L = oracle.lib
+    dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+    L.orc_ccd_coefficients.argtypes = [C.c_int, C.c_double, dp]
+    L.orc_uccd_coefficients.argtypes = [C.c_int, C.c_double, dp, dp]
+    nx = 40
+    n, h = nx + 6, 1.0 / nx
+    rng = np.random.default_rng(5)
+    f = np.sin(9.0 * np.arange(n) * h) + 0.2 * rng.standard_normal(n)
+    c = np.zeros(12 * n)
+    if kind == 0:
+        L.orc_ccd_coefficients(n, h, c)
+    else:
+        other = np.zeros(12 * n)
+        L.orc_uccd_coefficients(n, h, *((c, other) if kind == 1 else (other, c)))
+    s, ss = _compact_rhs_line(list(f), h, kind)
+    if kind == 2:  # the downwind ends are the upwind ones
+        su, ssu = _compact_rhs_line(list(f), h, 1)
+        s[0], ss[0], s[-1], ss[-1] = su[0], ssu[0], su[-1], ssu[-1]
+    es, ess = _twin_exact(c, s, ss)
+    vel = np.full(n, 1.0 if kind == 1 else -1.0)
+    outs = {}
+    for name, fn in (("double", L.orc_compact_derivatives), ("long", L.orc_compact_derivatives_ld)):
+        d1, d2 = np.zeros(n), np.zeros(n)
+        z = np.zeros(n)
+        fn(1, nx, 1, 1, h, 1.0, 1.0, f, vel.ctypes.data if kind else None, None, None, int(kind > 0), d1, z, z,
+           d2, z, z)
+        outs[name] = d1
+        if kind == 0:
+            outs[name + "2"] = d2
+    for key, want in (("", es), ("2", ess)):
+        if key and kind:
+            continue  # UCCD's second derivative mixes both systems
+        err_d = np.max(np.abs(outs["double" + key] - want))
+        err_l = np.max(np.abs(outs["long" + key] - want))
+        scale = np.max(np.abs(want))
+        assert err_l <= 4 * np.finfo(float).eps * scale, (key, err_l / scale)
+        assert err_l <= err_d, (key, err_l, err_d)
