@@ -139,9 +139,12 @@ struct problem_parameters {
 };
 
 // global.cpp:7-28: a zeroed Nx x Ny x Nz array behind DataType***, here one
-// contiguous block (k fastest) so that it crosses the C ABI as is
+// contiguous page-locked block (k fastest, scihpc_host_alloc) so that it
+// crosses the C ABI as is, at the link's rate
 inline DataType ***init_array(int Nx, int Ny, int Nz) {
-  auto *block = new DataType[(size_t)Nx * Ny * Nz]();
+  void *mem = nullptr;
+  check(scihpc_host_alloc(sizeof(DataType) * (size_t)Nx * Ny * Nz, &mem));
+  auto *block = static_cast<DataType *>(mem);
   auto **rows = new DataType *[(size_t)Nx * Ny];
   auto ***arr = new DataType **[Nx];
   for (int i = 0; i < Nx; ++i) {
@@ -153,7 +156,7 @@ inline DataType ***init_array(int Nx, int Ny, int Nz) {
 
 inline void delete3d(DataType ***arr, int, int) {
   if (!arr) return;
-  delete[] arr[0][0];
+  (void)scihpc_host_free(arr[0][0]);
   delete[] arr[0];
   delete[] arr;
 }
@@ -350,6 +353,11 @@ class ls_handle {
     if (!same) {
       reset();
       check(scihpc_ls_create(ndim, n[0], n[1], n[2], h[0], h[1], h[2], types, values, &h_));
+      // SCIHPC_LINE_SOLVER=partitioned: the CCD/UCCD line systems in
+      // parallel segments (scihpc_ls_set_line_solver) instead of the
+      // reference's sequential sweep
+      const char *ls = std::getenv("SCIHPC_LINE_SOLVER");
+      if (ls && std::strcmp(ls, "partitioned") == 0) check(scihpc_ls_set_line_solver(h_, 1));
       ndim_ = ndim;
       std::copy(n, n + 3, n_);
       std::copy(h, h + 3, hs_);
@@ -561,6 +569,9 @@ class multigrid {
     return st;
   }
   scihpc_mg *handle() const { return h_; }
+  // SCIHPC_VCYCLE=rb was set when this multigrid was made: v_cycle() and
+  // the FPC loop of projection_method run red-black V(2,2) cycles
+  bool rb_cycles() const { return rb_cycle_; }
 
  private:
   bool rb_cycle_ = false;
@@ -1219,7 +1230,8 @@ class projection_method {
     detail::need(lsf->dummy, "dummy_data");
     detail::need(pressure->solvers, "shared_solvers");
     problem_parameters *p = pressure->params;
-    scihpc_fpc_params fp{p->dt, p->dt_old, p->dt_old2, p->density_ratio, p->ppe_tol, 100};
+    scihpc_fpc_params fp{p->dt,      p->dt_old, p->dt_old2, p->density_ratio,
+                         p->ppe_tol, 100,       pressure->solvers->mg->rb_cycles() ? 1 : 0};
     int cycles = 0;
     check(scihpc_proj_fpc_step(handle(vel), pressure->solvers->mg->handle(), &fp, base(lsf->dummy->density),
                                base(pressure->scalar->old), base(pressure->scalar->old2), base(vel->vector->x.data),

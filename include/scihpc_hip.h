@@ -24,6 +24,7 @@
 #define SCIHPC_API
 #endif
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -58,6 +59,14 @@ enum {
 
 /* last error message of the calling thread ("" when none) */
 SCIHPC_API const char *scihpc_last_error(void);
+
+/* Zeroed host memory for fields that cross this ABI on every call:
+ * page-locked (hipHostMalloc), so that their copies run at the link's rate
+ * without staging, or ordinary memory when no page-locked memory is left.
+ * Free with scihpc_host_free.  No counterpart in the reference (its
+ * arrays are new[]'d, global.cpp:7-28); for the facade's field types. */
+SCIHPC_API int scihpc_host_alloc(size_t bytes, void **out);
+SCIHPC_API int scihpc_host_free(void *p);
 
 /* ------------------------------------------------------------------ */
 /* Pressure-Poisson multigrid                                          */
@@ -245,6 +254,9 @@ typedef struct {
   double rho12;               /* density_ratio */
   double ppe_tol;             /* absolute residual of the multigrid loop */
   int max_cycles;             /* the reference caps the loop at 100 */
+  int cycle;                  /* 0: the reference's full cycles; 1: this
+                                 build's red-black V(2,2) (scihpc_mg_vcycle),
+                                 same loop and stopping rule */
 } scihpc_fpc_params;
 
 /* One FPC step: pressure_guess_solution (projection_method.cpp:803-815),
@@ -252,8 +264,10 @@ typedef struct {
  * (:659-753): p_hat from p_old, p_old2; MG rhs = rho12/dt div u + sum C
  * p_hat; full cycles of the reference-faithful multigrid (a Neumann
  * constant-coefficient handle whose level-0 sol is the warm start, as
- * SciHPC's persistent at[0]->sol) until ||r|| <= ppe_tol; p, velocity
- * correction, BCs.  Writes u, v, w, p, p_hat and the cycle count. */
+ * SciHPC's persistent at[0]->sol) until ||r|| <= ppe_tol (with cycle = 1
+ * red-black V(2,2) cycles instead, no counterpart in the reference); p,
+ * velocity correction, BCs.  Writes u, v, w, p, p_hat and the cycle
+ * count. */
 SCIHPC_API int scihpc_proj_fpc_step(scihpc_proj *h, scihpc_mg *mg, const scihpc_fpc_params *prm,
                                     const double *rho, const double *p_old,
                                     const double *p_old2, double *u, double *v, double *w,

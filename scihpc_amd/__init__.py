@@ -462,12 +462,14 @@ class Projection:
             pass
 
     def fpc_step(self, mg, rho, p_old, p_old2, u, v, w, dt, dt_old, dt_old2, rho12, ppe_tol=1e-4,
-                 max_cycles=100):
-        """Returns (u, v, w, p, p_hat, cycles) as padded arrays."""
+                 max_cycles=100, cycle="reference"):
+        """Returns (u, v, w, p, p_hat, cycles) as padded arrays.  cycle:
+        "reference" (the reference's full cycles) or "rb" (red-black
+        V(2,2) cycles, same loop and stopping rule)."""
         u, v, w = (_f64(a).copy() for a in (u, v, w))
         p = np.zeros_like(u)
         ph = np.zeros_like(u)
-        prm = FpcParams(dt, dt_old, dt_old2, rho12, ppe_tol, max_cycles)
+        prm = FpcParams(dt, dt_old, dt_old2, rho12, ppe_tol, max_cycles, {"reference": 0, "rb": 1}[cycle])
         cyc = C.c_int()
         check(self._lib.scihpc_proj_fpc_step(self.h, mg.h, C.byref(prm), _f64(rho), _f64(p_old), _f64(p_old2),
                                              u, v, w, p, ph, C.byref(cyc)))

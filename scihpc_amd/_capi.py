@@ -41,7 +41,7 @@ class SolveStats(C.Structure):
 
 class FpcParams(C.Structure):
     _fields_ = [("dt", C.c_double), ("dt_old", C.c_double), ("dt_old2", C.c_double),
-                ("rho12", C.c_double), ("ppe_tol", C.c_double), ("max_cycles", C.c_int)]
+                ("rho12", C.c_double), ("ppe_tol", C.c_double), ("max_cycles", C.c_int), ("cycle", C.c_int)]
 
 
 class FlowParams(C.Structure):

@@ -2453,17 +2453,22 @@ extern "C" int scihpc_proj_fpc_step(scihpc_proj *h, scihpc_mg *mg, const scihpc_
     SCIHPC_CHECK_LAUNCH();
     SCIHPC_HIP(hipStreamSynchronize(h->stream));
     // the loop of fast_pressure_correction (projection_method.cpp:694-706):
-    // full cycles from the handle's current solution until the absolute
-    // residual is <= ppe_tol
+    // full cycles (or, cycle = 1, red-black V(2,2) cycles) from the
+    // handle's current solution until the absolute residual is <= ppe_tol
     SCIHPC_REQUIRE(scihpc_mg_set_rhs_device(mg, h->mg_rhs) == SCIHPC_OK, scihpc_last_error());
     double err = 0.0;
     SCIHPC_REQUIRE(scihpc_mg_residual(mg, 0, &err) == SCIHPC_OK, scihpc_last_error());
     int it = 0;
     while (err > prm->ppe_tol && it < prm->max_cycles) {
       ++it;
-      SCIHPC_REQUIRE(scihpc_mg_fullcycle_ref(mg) == SCIHPC_OK, scihpc_last_error());
+      if (prm->cycle == 1) SCIHPC_REQUIRE(scihpc_mg_vcycle(mg, 2, 2) == SCIHPC_OK, scihpc_last_error());
+      else SCIHPC_REQUIRE(scihpc_mg_fullcycle_ref(mg) == SCIHPC_OK, scihpc_last_error());
       SCIHPC_REQUIRE(scihpc_mg_residual(mg, 0, &err) == SCIHPC_OK, scihpc_last_error());
     }
+    // the reference's cycles leave the level-0 solution mean-free
+    // (compatibility_condition after every relax); the red-black cycles do
+    // not, so the constant is removed here
+    if (prm->cycle == 1 && it > 0) SCIHPC_REQUIRE(scihpc_mg_compat(mg, 0, 1) == SCIHPC_OK, scihpc_last_error());
     h->finish_pressure_and_velocity(mg, h->phat, prm->dt, prm->rho12);
     h->d2h(u, h->u);
     h->d2h(v, h->v);

@@ -3638,6 +3638,46 @@ void set_last_error(const std::string &m) { g_last_error = m; }
 
 extern "C" const char *scihpc_last_error(void) { return scihpc::g_last_error.c_str(); }
 
+namespace {
+std::mutex g_host_mu;
+std::vector<void *> g_host_plain;  // blocks that fell back to calloc
+}  // namespace
+
+extern "C" int scihpc_host_alloc(size_t bytes, void **out) {
+  return scihpc::guarded([&] {
+    SCIHPC_REQUIRE(out, "out is null");
+    *out = nullptr;
+    const size_t n = bytes ? bytes : 1;
+    void *p = nullptr;
+    if (hipHostMalloc(&p, n, hipHostMallocDefault) == hipSuccess && p) {
+      std::memset(p, 0, n);
+    } else {
+      (void)hipGetLastError();  // the refusal is not this call's error
+      p = std::calloc(n, 1);
+      SCIHPC_REQUIRE(p, "host allocation failed");
+      std::lock_guard<std::mutex> lk(g_host_mu);
+      g_host_plain.push_back(p);
+    }
+    *out = p;
+  });
+}
+
+extern "C" int scihpc_host_free(void *p) {
+  return scihpc::guarded([&] {
+    if (!p) return;
+    {
+      std::lock_guard<std::mutex> lk(g_host_mu);
+      auto it = std::find(g_host_plain.begin(), g_host_plain.end(), p);
+      if (it != g_host_plain.end()) {
+        g_host_plain.erase(it);
+        std::free(p);
+        return;
+      }
+    }
+    SCIHPC_HIP(hipHostFree(p));
+  });
+}
+
 static int g_levels(int ndim, int nx, int ny, int nz, std::vector<int> &lx, std::vector<int> &ly,
                     std::vector<int> &lz, std::vector<int> &ld) {
   // multigrid.cpp:8-41: divide by 2, else by 3, while every quotient > 3

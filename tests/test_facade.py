@@ -165,3 +165,35 @@ def test_facade_rb_vcycle_switch(tmp_path):
     assert out["rb"][1] <= 1e-8 and out["ref"][1] <= 1e-8
     assert out["rb"][0] < out["ref"][0]
     assert abs(out["rb"][2] - out["ref"][2]) < 1e-6
+
+
+@pytest.mark.gpu
+def test_dambreak_driver_fast_switches(tmp_path):
+    """The same unchanged dam-break driver with SCIHPC_VCYCLE=rb (red-black
+    V(2,2) cycles in the FPC loop, same absolute stopping rule) and
+    SCIHPC_LINE_SOLVER=partitioned (the CCD/UCCD line systems in parallel
+    segments): the pressure step still leaves the velocity divergence-free
+    and after 640 steps the front, mass and peak speed agree with the
+    reference-faithful run."""
+    exe = os.path.join(ROOT, "scihpc_amd", "drivers", "dambreak_demo")
+    assert os.path.exists(exe), "build() compiles scihpc_amd/drivers/dambreak_demo"
+    out = {}
+    for mode in ("reference", "fast"):
+        env = dict(os.environ)
+        env.pop("SCIHPC_VCYCLE", None)
+        env.pop("SCIHPC_LINE_SOLVER", None)
+        if mode == "fast":
+            env["SCIHPC_VCYCLE"] = "rb"
+            env["SCIHPC_LINE_SOLVER"] = "partitioned"
+        print(f"dambreak_demo 32x32 640 steps {mode}", flush=True)
+        res = subprocess.run([exe, "32", "32", "5", "5", "1", "1", "640", "0.01", "indicator"],
+                             capture_output=True, text=True, timeout=240, env=env)
+        assert res.returncode == 0, res.stderr
+        out[mode] = {k: float(v) for k, v in (line.split() for line in res.stdout.splitlines())}
+    ref, fast = out["reference"], out["fast"]
+    print({k: (ref[k], fast[k]) for k in ("front", "mass_ratio", "max_speed", "max_pressure_cycles", "seconds")},
+          flush=True)
+    assert fast["max_divergence"] < 1e-6 and fast["final_divergence"] < 1e-8
+    assert abs(fast["front"] - ref["front"]) < 1e-3 * ref["front"]
+    assert abs(fast["mass_ratio"] - ref["mass_ratio"]) < 1e-4
+    assert abs(fast["max_speed"] - ref["max_speed"]) < 1e-3 * ref["max_speed"]
