@@ -272,6 +272,23 @@ __global__ void __launch_bounds__(kEB) k_sum_rows(const float *__restrict__ part
   dst[c] = s;
 }
 
+// the same sums for few columns and many rows: block c sums column c, each
+// thread a fixed stride of rows, then a fixed-order tree (deterministic)
+__global__ void __launch_bounds__(kEB) k_sum_rows_tree(const float *__restrict__ part, int rows, long cols,
+                                                       float *__restrict__ dst) {
+  __shared__ float red[kEB];
+  const long c = blockIdx.x;
+  float s = 0.f;
+  for (int r = threadIdx.x; r < rows; r += kEB) s += part[(long)r * cols + c];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = kEB / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) dst[c] = red[0];
+}
+
 __global__ void __launch_bounds__(kEB) k_adam(float *__restrict__ P, const float *__restrict__ G, float *__restrict__ M,
                                               float *__restrict__ V, bf16_t *__restrict__ Pb, long n, float lr,
                                               float b1, float b2, float eps, float c1, float c2) {
@@ -365,27 +382,55 @@ inline unsigned nblk(long n) { return ceil_div(n, kEB); }
 // fill the GPU; large ones keep every output block in one workgroup, which
 // reads each input tile once.
 constexpr long kSplitBelow = 1024;
+// the input tile goes through LDS (k_conv_lds) when it takes at most 96 KiB
+// (2-D up to 224 input channels, 3-D up to 64) and at most four
+// workgroups share it (an output-channel split over more would stage the
+// same tile that many times: slower at 256x512's coarsest level); other
+// shapes read it through the caches (k_conv).  Measured at 256x512, batch
+// 4: the 32- to 192-channel layers of the 64x128 level 48-102 -> 25-71 us,
+// the full-resolution layers unchanged.
+constexpr size_t kConvLdsMax = 98304;
+constexpr int kConvLdsSplitMax = 4;
+template <int NB, int DIM>
+void conv_kernel(dim3 grid, bool lds, size_t bytes, const ConvShape &s, View x, View mask, const bf16_t *w,
+                 const float *b, View y, int relu, hipStream_t st) {
+  if (lds) {
+    static size_t allowed = 65536;  // dynamic LDS this instance may launch with
+    if (bytes > allowed) {
+      SCIHPC_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_conv_lds<NB, DIM>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)kConvLdsMax));
+      allowed = kConvLdsMax;
+    }
+    k_conv_lds<NB, DIM><<<grid, kConvThreads, bytes, st>>>(x, mask, w, b, y, s, relu);
+  } else {
+    k_conv<NB, DIM><<<grid, kConvThreads, 0, st>>>(x, mask, w, b, y, s, relu);
+  }
+}
 template <int DIM>
 void launch_conv_dim(const ConvShape &s, View x, View mask, const bf16_t *w, const float *b, View y, int relu,
                      hipStream_t st) {
   const unsigned tiles = (unsigned)((long)s.n * s.d * s.h * ((s.w + 63) / 64));
   const int blocks = s.cout / 16;
+  const size_t bytes = conv_lds_bytes(DIM, s.cin);
+  bool lds = bytes <= kConvLdsMax;
   if ((long)tiles < kSplitBelow && blocks > 1) {
     const int per = blocks % 2 == 0 && (long)tiles * blocks >= 4 * kSplitBelow ? 2 : 1;
     const dim3 grid(tiles, blocks / per);
-    if (per == 2) k_conv<2, DIM><<<grid, kConvThreads, 0, st>>>(x, mask, w, b, y, s, relu);
-    else k_conv<1, DIM><<<grid, kConvThreads, 0, st>>>(x, mask, w, b, y, s, relu);
+    lds = lds && (int)grid.y <= kConvLdsSplitMax;
+    if (per == 2) conv_kernel<2, DIM>(grid, lds, bytes, s, x, mask, w, b, y, relu, st);
+    else conv_kernel<1, DIM>(grid, lds, bytes, s, x, mask, w, b, y, relu, st);
     SCIHPC_CHECK_LAUNCH();
     return;
   }
+  const dim3 grid(tiles);
   switch (blocks) {
-    case 1: k_conv<1, DIM><<<tiles, kConvThreads, 0, st>>>(x, mask, w, b, y, s, relu); break;
-    case 2: k_conv<2, DIM><<<tiles, kConvThreads, 0, st>>>(x, mask, w, b, y, s, relu); break;
-    case 3: k_conv<3, DIM><<<tiles, kConvThreads, 0, st>>>(x, mask, w, b, y, s, relu); break;
-    case 4: k_conv<4, DIM><<<tiles, kConvThreads, 0, st>>>(x, mask, w, b, y, s, relu); break;
-    case 6: k_conv<6, DIM><<<tiles, kConvThreads, 0, st>>>(x, mask, w, b, y, s, relu); break;
-    case 8: k_conv<8, DIM><<<tiles, kConvThreads, 0, st>>>(x, mask, w, b, y, s, relu); break;
-    case 12: k_conv<12, DIM><<<tiles, kConvThreads, 0, st>>>(x, mask, w, b, y, s, relu); break;
+    case 1: conv_kernel<1, DIM>(grid, lds, bytes, s, x, mask, w, b, y, relu, st); break;
+    case 2: conv_kernel<2, DIM>(grid, lds, bytes, s, x, mask, w, b, y, relu, st); break;
+    case 3: conv_kernel<3, DIM>(grid, lds, bytes, s, x, mask, w, b, y, relu, st); break;
+    case 4: conv_kernel<4, DIM>(grid, lds, bytes, s, x, mask, w, b, y, relu, st); break;
+    case 6: conv_kernel<6, DIM>(grid, lds, bytes, s, x, mask, w, b, y, relu, st); break;
+    case 8: conv_kernel<8, DIM>(grid, lds, bytes, s, x, mask, w, b, y, relu, st); break;
+    case 12: conv_kernel<12, DIM>(grid, lds, bytes, s, x, mask, w, b, y, relu, st); break;
     default: throw Error(SCIHPC_BAD_ARG, "unsupported convolution width");
   }
   SCIHPC_CHECK_LAUNCH();
@@ -662,7 +707,7 @@ struct scihpc_unet {
     else
       k_mse<<<nb0, kEB, 0, stream>>>(out, tgt, dout, np0, lpart);
     k_head_bwd<<<nb0, kEB, 0, stream>>>(dout, v(t0d, 16), P + head_off, v(g_t0d, 16), part, np0);
-    k_sum_rows<<<1, kEB, 0, stream>>>(part, (int)nb0, kHeadC + 1, G + head_off);
+    k_sum_rows_tree<<<kHeadC + 1, kEB, 0, stream>>>(part, (int)nb0, kHeadC + 1, G + head_off);
     SCIHPC_CHECK_LAUNCH();
     const View none{nullptr, 0, 0};
     conv_back(13, n, v(t0c, 16), v(t0d, 16), v(g_t0d, 16), v(g_t0c, 16));

@@ -165,6 +165,114 @@ __global__ void __launch_bounds__(kConvThreads)
   }
 }
 
+// k_conv with the input tile staged in LDS: the workgroup's 64 output
+// pixels need input rows (z, y) + {-1, 0, 1} (3 rows in 2-D, 9 in 3-D) of
+// 66 pixels (one halo pixel each side) x cin channels; they are loaded
+// once, coalesced (a row segment is contiguous in N(D)HWC), masked and
+// zero-padded exactly as k_conv reads them, so every A fragment then comes
+// from LDS instead of nine global reads of each input value.  LDS layout
+// [row][pixel][channel], a pixel padded by 8 channels (16 bytes) to spread
+// the banks; dynamic size kConvRows * 66 * (cin + 8) * 2 bytes.  The
+// k-steps, MFMAs and epilogue are k_conv's, so the results are identical.
+template <int DIM>
+__host__ __device__ constexpr int conv_rows() {
+  return DIM == 3 ? 9 : 3;
+}
+inline size_t conv_lds_bytes(int ndim, int cin) {
+  return (size_t)(ndim == 3 ? 9 : 3) * 66 * (cin + 8) * sizeof(bf16_t);
+}
+
+template <int NB, int DIM>
+__global__ void __launch_bounds__(kConvThreads)
+    k_conv_lds(View x, View mask, const bf16_t *__restrict__ w, const float *__restrict__ b, View y, ConvShape s,
+               int relu) {
+  constexpr int T = Taps<DIM>::T, R = conv_rows<DIM>();
+  extern __shared__ __align__(16) bf16_t tile[];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int segs = (s.w + 63) / 64;
+  int blk = blockIdx.x;
+  const int seg = blk % segs;
+  blk /= segs;
+  const int row = blk % s.h;
+  blk /= s.h;
+  const int z = blk % s.d, img = blk / s.d;
+  const int CP = s.cin + 8;   // LDS pixel pitch (channels)
+  const int c8s = s.cin / 8;  // 16-byte chunks of a pixel
+  for (int q = threadIdx.x; q < R * 66 * c8s; q += kConvThreads) {
+    const int r = q / (66 * c8s), rem = q - r * 66 * c8s;
+    const int p = rem / c8s, c = (rem - p * c8s) * 8;
+    const int zz = z + (DIM == 3 ? r / 3 - 1 : 0), yy = row + (DIM == 3 ? r % 3 : r) - 1, xx = seg * 64 + p - 1;
+    uint4 a = uint4{0, 0, 0, 0};
+    if (zz >= 0 && zz < s.d && yy >= 0 && yy < s.h && xx >= 0 && xx < s.w) {
+      const long pix = (((long)img * s.d + zz) * s.h + yy) * s.w + xx;
+      a = *reinterpret_cast<const uint4 *>(x.p + pix * x.stride + x.off + c);
+      if (mask.p) {
+        const uint4 m = *reinterpret_cast<const uint4 *>(mask.p + pix * mask.stride + mask.off + c);
+        unsigned av[4] = {a.x, a.y, a.z, a.w};
+        const unsigned mv[4] = {m.x, m.y, m.z, m.w};
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          if (!bf16_pos((bf16_t)(mv[u] & 0xffffu))) av[u] &= 0xffff0000u;
+          if (!bf16_pos((bf16_t)(mv[u] >> 16))) av[u] &= 0x0000ffffu;
+        }
+        a = uint4{av[0], av[1], av[2], av[3]};
+      }
+    }
+    *reinterpret_cast<uint4 *>(tile + (r * 66 + p) * CP + c) = a;
+  }
+  __syncthreads();
+  const int pl = wave * 16 + (lane & 15);  // tile pixel of this lane's A row, before the tap's dx
+  const int kl = 8 * (lane >> 4);
+  const int nb0 = NB * blockIdx.y;
+  floatx4 acc[NB];
+#pragma unroll
+  for (int nb = 0; nb < NB; ++nb) acc[nb] = floatx4{0.f, 0.f, 0.f, 0.f};
+  const int K = T * s.cin;
+  constexpr int G = 2;
+  for (int k0 = 0; k0 < K; k0 += 32 * G) {
+    uint4 a[G], bw[G][NB];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const int kk = k0 + 32 * g + kl;
+      a[g] = uint4{0, 0, 0, 0};
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb) bw[g][nb] = uint4{0, 0, 0, 0};
+      if (kk < K) {
+        const int t = kk / s.cin, c = kk - t * s.cin;
+        int dz, dy, dx;
+        Taps<DIM>::offsets(t, dz, dy, dx);
+        const int r = DIM == 3 ? (dz + 1) * 3 + dy + 1 : dy + 1;
+        a[g] = *reinterpret_cast<const uint4 *>(tile + (r * 66 + pl + dx + 1) * CP + c);
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb)
+          bw[g][nb] =
+              *reinterpret_cast<const uint4 *>(w + ((long)((nb0 + nb) * 16 + (lane & 15)) * T + t) * s.cin + c);
+      }
+    }
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      if (k0 + 32 * g < K) {
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) acc[nb] = mfma32(a[g], bw[g][nb], acc[nb]);
+      }
+    }
+  }
+  const long rowpix = (((long)img * s.d + z) * s.h + row) * s.w;
+#pragma unroll
+  for (int nb = 0; nb < NB; ++nb) {
+    const int co = (nb0 + nb) * 16 + (lane & 15);
+    const float bias = b ? b[co] : 0.f;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int p = seg * 64 + wave * 16 + 4 * (lane >> 4) + j;
+      if (p >= s.w) continue;
+      float v = acc[nb][j] + bias;
+      if (relu) v = v > 0.f ? v : 0.f;
+      y.p[(rowpix + p) * y.stride + y.off + co] = f_to_bf16(v);
+    }
+  }
+}
+
 // Weight and bias gradient of y = relu(conv(x, w) + b):
 //   dW[co][t][ci] = sum_p gz[p][co] x[p + off(t)][ci],  db[co] = sum_p gz[p][co]
 // with gz = g * (mask > 0) (mask = the layer's stored output).  A = gz^T
