@@ -262,17 +262,28 @@ __global__ void __launch_bounds__(kEB) k_head_bwd(const float *__restrict__ dout
   }
 }
 
-// dst[c] = sum_r part[r][c] in row order (deterministic)
-__global__ void __launch_bounds__(kEB) k_sum_rows(const float *__restrict__ part, int rows, long cols,
-                                                  float *__restrict__ dst) {
-  const long c = (long)blockIdx.x * kEB + threadIdx.x;
-  if (c >= cols) return;
+// dst[c] = sum_r part[r][c] in a fixed order (deterministic), coalesced: a
+// block owns kSumCols adjacent columns, thread (x, y) sums rows y,
+// y + kSumRowGroups, ... of column x, the kSumRowGroups partials are then
+// added in row-group order
+constexpr int kSumCols = 32, kSumRowGroups = 32;
+__global__ void __launch_bounds__(kSumCols *kSumRowGroups) k_sum_rows_2d(const float *__restrict__ part, int rows,
+                                                                         long cols, float *__restrict__ dst) {
+  __shared__ float red[kSumRowGroups][kSumCols];
+  const long c = (long)blockIdx.x * kSumCols + threadIdx.x;
   float s = 0.f;
-  for (int r = 0; r < rows; ++r) s += part[(long)r * cols + c];
-  dst[c] = s;
+  if (c < cols)
+    for (int r = threadIdx.y; r < rows; r += kSumRowGroups) s += part[(long)r * cols + c];
+  red[threadIdx.y][threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.y == 0 && c < cols) {
+    float t = red[0][threadIdx.x];
+    for (int y = 1; y < kSumRowGroups; ++y) t += red[y][threadIdx.x];
+    dst[c] = t;
+  }
 }
 
-// the same sums for few columns and many rows: block c sums column c, each
+// the same sums for a few columns: block c sums column c, each
 // thread a fixed stride of rows, then a fixed-order tree (deterministic)
 __global__ void __launch_bounds__(kEB) k_sum_rows_tree(const float *__restrict__ part, int rows, long cols,
                                                        float *__restrict__ dst) {
@@ -617,7 +628,9 @@ struct scihpc_unet {
   void wgrad_chunks(int i, int n, int &chunks, int &cpix) const {
     const long npix = pix(L[i].lev, n);
     const int pairs = (L[i].cout / 16) * (L[i].cin / 16) * (T / 9);
-    long c = std::max(1, std::min(64, (1024 + pairs - 1) / pairs));
+    // about 2048 workgroups (a 16x16-channel level-0 layer had 64, a
+    // quarter of the CUs), pixel chunks of at least 128
+    long c = std::max(1, (2048 + pairs - 1) / pairs);
     c = std::min(c, std::max(1L, npix / 128));
     long cp = (npix + c - 1) / c;
     cp = (cp + 127) / 128 * 128;
@@ -631,7 +644,8 @@ struct scihpc_unet {
     const unsigned grid = (unsigned)((L[i].cout / 16) * (L[i].cin / 16) * (T / 9) * chunks);
     if (ndim == 3) k_conv_wgrad<3><<<grid, kConvThreads, 0, stream>>>(x, gy, y, part, s, cpix);
     else k_conv_wgrad<2><<<grid, kConvThreads, 0, stream>>>(x, gy, y, part, s, cpix);
-    k_sum_rows<<<nblk(L[i].size()), kEB, 0, stream>>>(part, chunks, L[i].size(), G + L[i].off);
+    k_sum_rows_2d<<<ceil_div(L[i].size(), kSumCols), dim3(kSumCols, kSumRowGroups), 0, stream>>>(
+        part, chunks, L[i].size(), G + L[i].off);
     SCIHPC_CHECK_LAUNCH();
   }
   void conv_back(int i, int n, View x, View y, View gy, View gx) {

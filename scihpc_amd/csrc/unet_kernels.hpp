@@ -287,7 +287,7 @@ __global__ void __launch_bounds__(kConvThreads)
 // (no atomics) to
 //   part[chunk][co][t][ci], part[chunk][cout*T*cin + co]   (bias, from the
 // workgroups of cc == 0 and tap group 0 only)
-// which k_sum_rows reduces over chunks in a fixed order.
+// which k_sum_rows_2d reduces over chunks in a fixed order.
 constexpr int kWgPad = 40;  // LDS row length (BF16) of a 32-pixel tile row; 80 bytes keep 16-byte reads aligned
 
 template <int DIM>
