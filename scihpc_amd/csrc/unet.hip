@@ -52,18 +52,49 @@ __device__ __forceinline__ long child(long f0, int k, int h, int w) {  // k-th c
   return f0 + (k & 1) + ((k >> 1) & 1) * (2L * w) + (DIM == 3 ? ((k >> 2) & 1) * (4L * h * w) : 0L);
 }
 
-// dst (coarse d x h x w) = mean of the children in src; thread per (pixel, channel)
+// The pooling and upsampling kernels move 8 channels (16 bytes) a thread:
+// every channel count and view offset is a multiple of 16, so a thread's
+// channels are one aligned uint4.  Per channel the arithmetic is the same
+// as one element at a time.
+constexpr int kVec = 8;
+__device__ __forceinline__ void unpack8(uint4 v, float (&f)[kVec]) {
+  const unsigned u[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    f[2 * q] = bf16_to_f((bf16_t)(u[q] & 0xffffu));
+    f[2 * q + 1] = bf16_to_f((bf16_t)(u[q] >> 16));
+  }
+}
+__device__ __forceinline__ uint4 pack8(const float (&f)[kVec]) {
+  unsigned u[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) u[q] = (unsigned)f_to_bf16(f[2 * q]) | ((unsigned)f_to_bf16(f[2 * q + 1]) << 16);
+  return uint4{u[0], u[1], u[2], u[3]};
+}
+__device__ __forceinline__ uint4 &at8(View v, long p, int ch) {
+  return *reinterpret_cast<uint4 *>(v.p + p * v.stride + v.off + ch);
+}
+
+// dst (coarse d x h x w) = mean of the children in src; thread per (pixel, 8 channels)
 template <int DIM>
 __global__ void __launch_bounds__(kEB) k_pool2(View src, View dst, int n, int d, int h, int w, int c) {
   const long e = (long)blockIdx.x * kEB + threadIdx.x;
-  if (e >= (long)n * d * h * w * c) return;
-  const int ch = (int)(e % c);
-  const long p = e / c;
+  const int c8 = c / kVec;
+  if (e >= (long)n * d * h * w * c8) return;
+  const int ch = (int)(e % c8) * kVec;
+  const long p = e / c8;
   const long f0 = child0<DIM>(p, d, h, w);
-  float s = 0.f;
+  float s[kVec] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-  for (int k = 0; k < (1 << DIM); ++k) s += bf16_to_f(src.p[child<DIM>(f0, k, h, w) * src.stride + src.off + ch]);
-  dst.p[p * dst.stride + dst.off + ch] = f_to_bf16(s * (1.f / (1 << DIM)));
+  for (int k = 0; k < (1 << DIM); ++k) {
+    float v[kVec];
+    unpack8(at8(src, child<DIM>(f0, k, h, w), ch), v);
+#pragma unroll
+    for (int q = 0; q < kVec; ++q) s[q] += v[q];
+  }
+#pragma unroll
+  for (int q = 0; q < kVec; ++q) s[q] = s[q] * (1.f / (1 << DIM));
+  at8(dst, p, ch) = pack8(s);
 }
 
 // gf (fine) += gc[parent] / 2^DIM: gradient of k_pool2 added to the
@@ -72,13 +103,17 @@ template <int DIM>
 __global__ void __launch_bounds__(kEB) k_pool2_bwd_add(View gf, View gc, int n, int d, int h, int w, int c) {
   const long e = (long)blockIdx.x * kEB + threadIdx.x;
   const long nf = (long)n * (DIM == 3 ? 2 * d : 1) * (2 * h) * (2 * w);
-  if (e >= nf * c) return;
-  const int ch = (int)(e % c);
-  const long p = e / c;
+  const int c8 = c / kVec;
+  if (e >= nf * c8) return;
+  const int ch = (int)(e % c8) * kVec;
+  const long p = e / c8;
   const long pc = parent<DIM>(p, d, h, w);
-  const float v = bf16_to_f(gf.p[p * gf.stride + gf.off + ch]) +
-                  (1.f / (1 << DIM)) * bf16_to_f(gc.p[pc * gc.stride + gc.off + ch]);
-  gf.p[p * gf.stride + gf.off + ch] = f_to_bf16(v);
+  float a[kVec], b[kVec];
+  unpack8(at8(gf, p, ch), a);
+  unpack8(at8(gc, pc, ch), b);
+#pragma unroll
+  for (int q = 0; q < kVec; ++q) a[q] = a[q] + (1.f / (1 << DIM)) * b[q];
+  at8(gf, p, ch) = pack8(a);
 }
 
 // dst (fine) = src[parent] (nearest upsampling)
@@ -86,24 +121,31 @@ template <int DIM>
 __global__ void __launch_bounds__(kEB) k_up2(View src, View dst, int n, int d, int h, int w, int c) {
   const long e = (long)blockIdx.x * kEB + threadIdx.x;
   const long nf = (long)n * (DIM == 3 ? 2 * d : 1) * (2 * h) * (2 * w);
-  if (e >= nf * c) return;
-  const int ch = (int)(e % c);
-  const long p = e / c;
-  dst.p[p * dst.stride + dst.off + ch] = src.p[parent<DIM>(p, d, h, w) * src.stride + src.off + ch];
+  const int c8 = c / kVec;
+  if (e >= nf * c8) return;
+  const int ch = (int)(e % c8) * kVec;
+  const long p = e / c8;
+  at8(dst, p, ch) = at8(src, parent<DIM>(p, d, h, w), ch);
 }
 
 // gc (coarse) = sum over the children of gf: gradient of k_up2
 template <int DIM>
 __global__ void __launch_bounds__(kEB) k_up2_bwd(View gf, View gc, int n, int d, int h, int w, int c) {
   const long e = (long)blockIdx.x * kEB + threadIdx.x;
-  if (e >= (long)n * d * h * w * c) return;
-  const int ch = (int)(e % c);
-  const long p = e / c;
+  const int c8 = c / kVec;
+  if (e >= (long)n * d * h * w * c8) return;
+  const int ch = (int)(e % c8) * kVec;
+  const long p = e / c8;
   const long f0 = child0<DIM>(p, d, h, w);
-  float s = 0.f;
+  float s[kVec] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-  for (int k = 0; k < (1 << DIM); ++k) s += bf16_to_f(gf.p[child<DIM>(f0, k, h, w) * gf.stride + gf.off + ch]);
-  gc.p[p * gc.stride + gc.off + ch] = f_to_bf16(s);
+  for (int k = 0; k < (1 << DIM); ++k) {
+    float v[kVec];
+    unpack8(at8(gf, child<DIM>(f0, k, h, w), ch), v);
+#pragma unroll
+    for (int q = 0; q < kVec; ++q) s[q] += v[q];
+  }
+  at8(gc, p, ch) = pack8(s);
 }
 
 // out[p] = b + sum_c x[p][c] w[c] (FP32 head weights)
@@ -406,12 +448,9 @@ template <int NB, int DIM>
 void conv_kernel(dim3 grid, bool lds, size_t bytes, const ConvShape &s, View x, View mask, const bf16_t *w,
                  const float *b, View y, int relu, hipStream_t st) {
   if (lds) {
-    static size_t allowed = 65536;  // dynamic LDS this instance may launch with
-    if (bytes > allowed) {
+    if (bytes > 65536)  // past the default dynamic LDS limit (a host-side attribute, cheap to set again)
       SCIHPC_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_conv_lds<NB, DIM>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)kConvLdsMax));
-      allowed = kConvLdsMax;
-    }
     k_conv_lds<NB, DIM><<<grid, kConvThreads, bytes, st>>>(x, mask, w, b, y, s, relu);
   } else {
     k_conv<NB, DIM><<<grid, kConvThreads, 0, st>>>(x, mask, w, b, y, s, relu);
@@ -654,28 +693,28 @@ struct scihpc_unet {
   }
   void pool(View src, View dst, int lev, int n, int c, hipStream_t st) {  // lev = coarse level
     const int d = dz(lev), h = H >> lev, w = W >> lev;
-    const unsigned g = nblk((long)n * d * h * w * c);
+    const unsigned g = nblk((long)n * d * h * w * (c / kVec));
     if (ndim == 3) k_pool2<3><<<g, kEB, 0, st>>>(src, dst, n, d, h, w, c);
     else k_pool2<2><<<g, kEB, 0, st>>>(src, dst, n, d, h, w, c);
     SCIHPC_CHECK_LAUNCH();
   }
   void up(View src, View dst, int lev, int n, int c, hipStream_t st) {  // lev = coarse level
     const int d = dz(lev), h = H >> lev, w = W >> lev;
-    const unsigned g = nblk(pix(lev - 1, n) * c);
+    const unsigned g = nblk(pix(lev - 1, n) * (c / kVec));
     if (ndim == 3) k_up2<3><<<g, kEB, 0, st>>>(src, dst, n, d, h, w, c);
     else k_up2<2><<<g, kEB, 0, st>>>(src, dst, n, d, h, w, c);
     SCIHPC_CHECK_LAUNCH();
   }
   void up_bwd(View gf, View gc, int lev, int n, int c) {  // lev = coarse level
     const int d = dz(lev), h = H >> lev, w = W >> lev;
-    const unsigned g = nblk(pix(lev, n) * c);
+    const unsigned g = nblk(pix(lev, n) * (c / kVec));
     if (ndim == 3) k_up2_bwd<3><<<g, kEB, 0, stream>>>(gf, gc, n, d, h, w, c);
     else k_up2_bwd<2><<<g, kEB, 0, stream>>>(gf, gc, n, d, h, w, c);
     SCIHPC_CHECK_LAUNCH();
   }
   void pool_bwd_add(View gf, View gc, int lev, int n, int c) {  // lev = coarse level
     const int d = dz(lev), h = H >> lev, w = W >> lev;
-    const unsigned g = nblk(pix(lev - 1, n) * c);
+    const unsigned g = nblk(pix(lev - 1, n) * (c / kVec));
     if (ndim == 3) k_pool2_bwd_add<3><<<g, kEB, 0, stream>>>(gf, gc, n, d, h, w, c);
     else k_pool2_bwd_add<2><<<g, kEB, 0, stream>>>(gf, gc, n, d, h, w, c);
     SCIHPC_CHECK_LAUNCH();
