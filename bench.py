@@ -375,6 +375,31 @@ def bench_learned_dambreak(reps=3):
             "rel_tol": 1e-8, "steps": rows}
 
 
+def bench_unet(iters=10):
+    """The learned preconditioner's network (4 levels, 16/32/64/128
+    channels, seed 3) at 256x512, batch 4, on random samples: ms and
+    TFLOP/s of a forward pass and of a training step (forward, backward,
+    Adam) on the matrix cores."""
+    import numpy as np
+
+    from scihpc_amd import UNet
+
+    nt = UNet(512, 256, 4, seed=3)
+    rng = np.random.default_rng(5)
+    nt.out_scale = 1.0
+    nt.set_samples(rng.standard_normal((4, 512 * 256, 2)).astype(np.float32),
+                   rng.standard_normal((4, 512 * 256)).astype(np.float32))
+    ms_tr, fl_tr = nt.time(4, train=True, iters=iters)
+    ms_fw, fl_fw = nt.time(4, train=False, iters=iters)
+    nt.close()
+    return {"grid": [256, 512], "batch": 4,
+            "train_step_ms": round(ms_tr, 3), "train_tflops": round(fl_tr / (ms_tr * 1e-3) / 1e12, 2),
+            "forward_ms": round(ms_fw, 3), "forward_tflops": round(fl_fw / (ms_fw * 1e-3) / 1e12, 2),
+            "mfma_peak_tflops": MFMA_BF16_PEAK,
+            "mfma_frac_train": round(fl_tr / (ms_tr * 1e-3) / 1e12 / MFMA_BF16_PEAK, 4),
+            "mfma_frac_forward": round(fl_fw / (ms_fw * 1e-3) / 1e12 / MFMA_BF16_PEAK, 4)}
+
+
 def bench_snapshot(torch, nx=512, ny=1024, reps=5):
     """Legacy-VTK snapshot of the 512x1024 grid as the reference's drivers
     write it (three scalars and one vector per file, vtkWriter.cpp:18-111):
@@ -631,6 +656,7 @@ def main():
     ap.add_argument("--no-weno", action="store_true", help="skip the WENO5 TVD-RK3 timing")
     ap.add_argument("--no-learned3d", action="store_true", help="skip the 128^3 learned-guess row")
     ap.add_argument("--only-learned3d", action="store_true", help="run only the 128^3 learned-guess row")
+    ap.add_argument("--only-unet", action="store_true", help="run only the U-Net timing row (profiling)")
     ap.add_argument("--only-rows", action="store_true",
                     help="run only the rows around the solve (momentum predictor, interface maintenance, "
                          "VTK snapshot) and print their JSON (profiling)")
@@ -663,6 +689,9 @@ def main():
         return
     if args.only_learned3d:
         print(json.dumps(bench_learned3d()), flush=True)
+        return
+    if args.only_unet:
+        print(json.dumps(bench_unet()), flush=True)
         return
     if args.only_rows:
         rows = {"momentum": bench_momentum(), "interface": bench_interface(), "weno_rk3": bench_weno(),
@@ -847,19 +876,7 @@ def main():
     unet = None
     if not args.no_unet:
         log("bench: U-Net training step (256x512, batch 4)")
-        nt = UNet(512, 256, 4, seed=3)
-        rng = np.random.default_rng(5)
-        nt.out_scale = 1.0
-        nt.set_samples(rng.standard_normal((4, 512 * 256, 2)).astype(np.float32),
-                       rng.standard_normal((4, 512 * 256)).astype(np.float32))
-        ms_tr, fl_tr = nt.time(4, train=True, iters=10)
-        ms_fw, fl_fw = nt.time(4, train=False, iters=10)
-        unet = {"grid": [256, 512], "batch": 4,
-                "train_step_ms": round(ms_tr, 3), "train_tflops": round(fl_tr / (ms_tr * 1e-3) / 1e12, 2),
-                "forward_ms": round(ms_fw, 3), "forward_tflops": round(fl_fw / (ms_fw * 1e-3) / 1e12, 2),
-                "mfma_peak_tflops": MFMA_BF16_PEAK,
-                "mfma_frac_train": round(fl_tr / (ms_tr * 1e-3) / 1e12 / MFMA_BF16_PEAK, 4)}
-        nt.close()
+        unet = bench_unet()
 
     momentum = None
     if not args.no_momentum:
