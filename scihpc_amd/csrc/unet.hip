@@ -472,6 +472,24 @@ void launch_conv_dim(const ConvShape &s, View x, View mask, const bf16_t *w, con
     SCIHPC_CHECK_LAUNCH();
     return;
   }
+  // large, narrow layers: weights in registers across several tiles
+  if (DIM == 2 && (long)tiles >= 2048) {
+    const int ks = (Taps<DIM>::T * s.cin + 31) / 32;
+    const unsigned g = (tiles + kConvTilesPerWG - 1) / kConvTilesPerWG;
+#define SCIHPC_RW(NB_, KS_)                                                                              \
+  if (blocks == NB_ && ks == KS_) {                                                                      \
+    k_conv_rw<NB_, DIM, KS_><<<g, kConvThreads, 0, st>>>(x, mask, w, b, y, s, relu, (int)tiles);          \
+    SCIHPC_CHECK_LAUNCH();                                                                               \
+    return;                                                                                              \
+  }
+    SCIHPC_RW(1, 5)
+    SCIHPC_RW(1, 9)
+    SCIHPC_RW(1, 14)
+    SCIHPC_RW(2, 5)
+    SCIHPC_RW(2, 9)
+    SCIHPC_RW(3, 5)
+#undef SCIHPC_RW
+  }
   const dim3 grid(tiles);
   switch (blocks) {
     case 1: conv_kernel<1, DIM>(grid, lds, bytes, s, x, mask, w, b, y, relu, st); break;

@@ -273,6 +273,95 @@ __global__ void __launch_bounds__(kConvThreads)
   }
 }
 
+// k_conv for the large, narrow layers (full-resolution images, few
+// channels): a workgroup loads the layer's weight fragments for all KS
+// k-steps into registers once, then computes kConvTilesPerWG pixel tiles,
+// each with all of its KS input fragments loaded before the first MFMA, so
+// a tile waits on one round of loads instead of one per two k-steps.  The
+// k-steps run in k_conv's order with the same operands: identical results.
+constexpr int kConvTilesPerWG = 4;
+
+template <int NB, int DIM, int KS>
+__global__ void __launch_bounds__(kConvThreads)
+    k_conv_rw(View x, View mask, const bf16_t *__restrict__ w, const float *__restrict__ b, View y, ConvShape s,
+              int relu, int ntiles) {
+  constexpr int T = Taps<DIM>::T;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int kl = 8 * (lane >> 4);
+  const int K = T * s.cin;
+  uint4 bw[KS][NB];
+#pragma unroll
+  for (int ks = 0; ks < KS; ++ks) {
+    const int kk = 32 * ks + kl;
+    const int t = kk / s.cin, c = kk - t * s.cin;
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb)
+      bw[ks][nb] = kk < K ? *reinterpret_cast<const uint4 *>(w + ((long)(nb * 16 + (lane & 15)) * T + t) * s.cin + c)
+                          : uint4{0, 0, 0, 0};
+  }
+  float bias[NB];
+#pragma unroll
+  for (int nb = 0; nb < NB; ++nb) bias[nb] = b ? b[nb * 16 + (lane & 15)] : 0.f;
+  const int segs = (s.w + 63) / 64;
+  const int t0 = blockIdx.x * kConvTilesPerWG, t1 = min(t0 + kConvTilesPerWG, ntiles);
+  for (int tile = t0; tile < t1; ++tile) {
+    int blk = tile;
+    const int seg = blk % segs;
+    blk /= segs;
+    const int row = blk % s.h;
+    blk /= s.h;
+    const int z = blk % s.d, img = blk / s.d;
+    const int px = seg * 64 + wave * 16 + (lane & 15);
+    uint4 a[KS];
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+      const int kk = 32 * ks + kl;
+      a[ks] = uint4{0, 0, 0, 0};
+      if (kk < K) {
+        const int t = kk / s.cin, c = kk - t * s.cin;
+        int dz, dy, dx;
+        Taps<DIM>::offsets(t, dz, dy, dx);
+        const int zz = z + dz, yy = row + dy, xx = px + dx;
+        if (zz >= 0 && zz < s.d && yy >= 0 && yy < s.h && xx >= 0 && xx < s.w) {
+          const long pix = (((long)img * s.d + zz) * s.h + yy) * s.w + xx;
+          a[ks] = *reinterpret_cast<const uint4 *>(x.p + pix * x.stride + x.off + c);
+          if (mask.p) {
+            const uint4 m = *reinterpret_cast<const uint4 *>(mask.p + pix * mask.stride + mask.off + c);
+            unsigned av[4] = {a[ks].x, a[ks].y, a[ks].z, a[ks].w};
+            const unsigned mv[4] = {m.x, m.y, m.z, m.w};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              if (!bf16_pos((bf16_t)(mv[q] & 0xffffu))) av[q] &= 0xffff0000u;
+              if (!bf16_pos((bf16_t)(mv[q] >> 16))) av[q] &= 0x0000ffffu;
+            }
+            a[ks] = uint4{av[0], av[1], av[2], av[3]};
+          }
+        }
+      }
+    }
+    floatx4 acc[NB];
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) acc[nb] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb) acc[nb] = mfma32(a[ks], bw[ks][nb], acc[nb]);
+    const long rowpix = (((long)img * s.d + z) * s.h + row) * s.w;
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) {
+      const int co = nb * 16 + (lane & 15);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int p = seg * 64 + wave * 16 + 4 * (lane >> 4) + j;
+        if (p >= s.w) continue;
+        float v = acc[nb][j] + bias[nb];
+        if (relu) v = v > 0.f ? v : 0.f;
+        y.p[(rowpix + p) * y.stride + y.off + co] = f_to_bf16(v);
+      }
+    }
+  }
+}
+
 // Weight and bias gradient of y = relu(conv(x, w) + b):
 //   dW[co][t][ci] = sum_p gz[p][co] x[p + off(t)][ci],  db[co] = sum_p gz[p][co]
 // with gz = g * (mask > 0) (mask = the layer's stored output).  A = gz^T
