@@ -444,17 +444,32 @@ constexpr long kSplitBelow = 1024;
 // the full-resolution layers unchanged.
 constexpr size_t kConvLdsMax = 98304;
 constexpr int kConvLdsSplitMax = 4;
+template <int NB, int DIM, int G>
+void conv_kernel_g(dim3 grid, bool lds, size_t bytes, const ConvShape &s, View x, View mask, const bf16_t *w,
+                   const float *b, View y, int relu, hipStream_t st) {
+  if (lds) {
+    if (bytes > 65536)  // past the default dynamic LDS limit (a host-side attribute, cheap to set again)
+      SCIHPC_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_conv_lds<NB, DIM, G>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)kConvLdsMax));
+    k_conv_lds<NB, DIM, G><<<grid, kConvThreads, bytes, st>>>(x, mask, w, b, y, s, relu);
+  } else {
+    k_conv<NB, DIM, G><<<grid, kConvThreads, 0, st>>>(x, mask, w, b, y, s, relu);
+  }
+}
+// deep reductions (32 k-steps or more) over an LDS tile with one or two
+// output blocks keep eight k-steps of weight operands in flight instead of
+// two (the 192-channel 64x128 layer 101 -> 91 us; through the caches, at
+// 32x64, no gain)
 template <int NB, int DIM>
 void conv_kernel(dim3 grid, bool lds, size_t bytes, const ConvShape &s, View x, View mask, const bf16_t *w,
                  const float *b, View y, int relu, hipStream_t st) {
-  if (lds) {
-    if (bytes > 65536)  // past the default dynamic LDS limit (a host-side attribute, cheap to set again)
-      SCIHPC_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_conv_lds<NB, DIM>),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)kConvLdsMax));
-    k_conv_lds<NB, DIM><<<grid, kConvThreads, bytes, st>>>(x, mask, w, b, y, s, relu);
-  } else {
-    k_conv<NB, DIM><<<grid, kConvThreads, 0, st>>>(x, mask, w, b, y, s, relu);
+  if constexpr (NB <= 2) {
+    if (lds && (Taps<DIM>::T * s.cin + 31) / 32 >= 32) {
+      conv_kernel_g<NB, DIM, 8>(grid, lds, bytes, s, x, mask, w, b, y, relu, st);
+      return;
+    }
   }
+  conv_kernel_g<NB, DIM, 2>(grid, lds, bytes, s, x, mask, w, b, y, relu, st);
 }
 template <int DIM>
 void launch_conv_dim(const ConvShape &s, View x, View mask, const bf16_t *w, const float *b, View y, int relu,

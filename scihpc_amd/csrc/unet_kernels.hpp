@@ -83,7 +83,7 @@ constexpr int kConvThreads = 256;  // 4 wavefronts x 16 pixels = 64 pixels per w
 // NB * blockIdx.y: a grid with gridDim.y > 1 splits the output channels of a
 // small image over more workgroups (the coarse levels have few pixels and
 // many channels).
-template <int NB, int DIM>
+template <int NB, int DIM, int G = 2>  // G: k-steps whose operands are loaded together
 __global__ void __launch_bounds__(kConvThreads)
     k_conv(View x, View mask, const bf16_t *__restrict__ w, const float *__restrict__ b, View y, ConvShape s,
            int relu) {
@@ -105,7 +105,6 @@ __global__ void __launch_bounds__(kConvThreads)
   const int K = T * s.cin;
   // the operands of G consecutive k steps are loaded before their MFMAs, so
   // G loads are in flight at once instead of one dependent load per step
-  constexpr int G = 2;
   for (int k0 = 0; k0 < K; k0 += 32 * G) {
     uint4 a[G], bw[G][NB];
 #pragma unroll
@@ -182,7 +181,7 @@ inline size_t conv_lds_bytes(int ndim, int cin) {
   return (size_t)(ndim == 3 ? 9 : 3) * 66 * (cin + 8) * sizeof(bf16_t);
 }
 
-template <int NB, int DIM>
+template <int NB, int DIM, int G = 2>  // G: k-steps whose operands are loaded together
 __global__ void __launch_bounds__(kConvThreads)
     k_conv_lds(View x, View mask, const bf16_t *__restrict__ w, const float *__restrict__ b, View y, ConvShape s,
                int relu) {
@@ -228,7 +227,6 @@ __global__ void __launch_bounds__(kConvThreads)
 #pragma unroll
   for (int nb = 0; nb < NB; ++nb) acc[nb] = floatx4{0.f, 0.f, 0.f, 0.f};
   const int K = T * s.cin;
-  constexpr int G = 2;
   for (int k0 = 0; k0 < K; k0 += 32 * G) {
     uint4 a[G], bw[G][NB];
 #pragma unroll
