@@ -2434,6 +2434,7 @@ extern "C" int scihpc_proj_fpc_step(scihpc_proj *h, scihpc_mg *mg, const scihpc_
   return guarded([&] {
     SCIHPC_REQUIRE(h && prm && rho && p_old && p_old2 && u && v && w && p && phat, "null argument");
     SCIHPC_REQUIRE(prm->dt > 0 && prm->dt_old2 != 0 && prm->rho12 > 0 && prm->max_cycles >= 0, "bad parameters");
+    SCIHPC_REQUIRE(prm->cycle == 0 || prm->cycle == 1, "cycle must be 0 (reference full cycles) or 1 (red-black)");
     h->check_mg(mg);
     const Grid &g = h->g;
     h->h2d(h->rho, rho);

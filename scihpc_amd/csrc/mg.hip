@@ -389,6 +389,8 @@ __global__ void __launch_bounds__(kBlock) k_coarse_solve(DLevel<T> G, const doub
 // constant stride keeps blockDim.x (a load from the dispatch packet) out of
 // their loops
 constexpr int kBlockThreads = 1024;
+// k_block_vcycle's dense coarsest solve gives each thread at most two rows
+static_assert(kMaxCoarse <= 2 * kBlockThreads, "coarsest rows exceed two per thread");
 
 // LDS-resident levels hold < 2^31 cells: 32-bit indices throughout
 template <typename T, int DIM, bool VAR>

@@ -110,6 +110,24 @@ def test_fpc_step_rb_cycles(oracle, ndim, n):
         assert rel(got, ref) < 1e-6
 
 
+def test_fpc_step_rejects_unknown_cycle():
+    """scihpc_fpc_params.cycle other than 0 or 1 is a bad argument."""
+    import ctypes as C
+
+    from scihpc_amd import _capi
+
+    n = (16, 16, 1)
+    mg = Multigrid(2, *n, 1 / 16, 1 / 16, 1.0, "neumann")
+    pj = Projection(2, *n, 1 / 16, 1 / 16, 1.0, [3, 3, 4, 4, 3, 3])
+    T = int(np.prod(padded_shape(2, *n)))
+    a = [np.ones(T) for _ in range(8)]
+    prm = _capi.FpcParams(0.01, 0.01, 0.01, 0.001, 1e-9, 10, 2)
+    cyc = C.c_int()
+    rc = _capi.lib().scihpc_proj_fpc_step(pj.h, mg.h, C.byref(prm), *a, C.byref(cyc))
+    assert rc == _capi.BAD_ARG
+    assert b"cycle" in _capi.lib().scihpc_last_error()
+
+
 def _interior_div(ndim, n, u, v, w, h):
     shp = padded_shape(ndim, *n)
     U, V, Wz = (a.reshape(shp) for a in (u, v, w))
