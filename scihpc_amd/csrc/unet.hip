@@ -436,14 +436,15 @@ inline unsigned nblk(long n) { return ceil_div(n, kEB); }
 // reads each input tile once.
 constexpr long kSplitBelow = 1024;
 // the input tile goes through LDS (k_conv_lds) when it takes at most 96 KiB
-// (2-D up to 224 input channels, 3-D up to 64) and at most four
-// workgroups share it (an output-channel split over more would stage the
-// same tile that many times: slower at 256x512's coarsest level); other
+// (2-D up to 224 input channels, 3-D up to 64) and at most six workgroups
+// share it (an output-channel split over more would stage the same tile
+// that many times: slower at 256x512's coarsest level, split 8); other
 // shapes read it through the caches (k_conv).  Measured at 256x512, batch
-// 4: the 32- to 192-channel layers of the 64x128 level 48-102 -> 25-71 us,
-// the full-resolution layers unchanged.
+// 4: the 64x128 layers 48 -> 25 us (32-64 input channels, split 4) and
+// 101 -> 70 us (64 -> 192 channels, split 6), the full-resolution layers
+// unchanged.
 constexpr size_t kConvLdsMax = 98304;
-constexpr int kConvLdsSplitMax = 4;
+constexpr int kConvLdsSplitMax = 6;
 template <int NB, int DIM, int G>
 void conv_kernel_g(dim3 grid, bool lds, size_t bytes, const ConvShape &s, View x, View mask, const bf16_t *w,
                    const float *b, View y, int relu, hipStream_t st) {
