@@ -429,12 +429,15 @@ __global__ void __launch_bounds__(kEB) k_make_input(const float *__restrict__ fe
 inline unsigned nblk(long n) { return ceil_div(n, kEB); }
 
 // y = act(conv(x, w) + b) over s (2-D or 3-D); NB = s.cout / 16 output blocks
-// Small images (fewer pixel tiles than kSplitBelow) split their output
-// channels over workgroups, one or two 16-channel blocks each, so the
-// coarse levels of the network (few pixels, up to 192 channels) still
-// fill the GPU; large ones keep every output block in one workgroup, which
-// reads each input tile once.
-constexpr long kSplitBelow = 1024;
+// Tiny images (fewer than 64 pixel tiles, a quarter of the CUs) split
+// their output channels over workgroups, one or two 16-channel blocks
+// each; larger ones keep every output block in one workgroup, which stages
+// or reads each input tile once and reuses every A fragment for all its
+// MFMAs.  With the LDS tile the split costs more than the parallelism it
+// buys: at 256x512, batch 4, splitting every image under 1024 tiles gave a
+// 0.485 ms forward pass, splitting under 64 gives 0.443 ms (the 32x64
+// layers 46 -> 37 us, the 64x128 ones 25-90 -> 30-64 us).
+constexpr long kSplitBelow = 64;
 // the input tile goes through LDS (k_conv_lds) when it takes at most 96 KiB
 // (2-D up to 224 input channels, 3-D up to 64) and at most six workgroups
 // share it (an output-channel split over more would stage the same tile
