@@ -375,7 +375,17 @@ __global__ void __launch_bounds__(kConvThreads)
 //   part[chunk][co][t][ci], part[chunk][cout*T*cin + co]   (bias, from the
 // workgroups of cc == 0 and tap group 0 only)
 // which k_sum_rows_2d reduces over chunks in a fixed order.
-constexpr int kWgPad = 40;  // LDS row length (BF16) of a 32-pixel tile row; 80 bytes keep 16-byte reads aligned
+constexpr int kWgPad = 40;
+
+// LDS exchange among the lanes of one wavefront: a wavefront's LDS
+// operations complete in order, so ordering the compiler's accesses (the
+// wavefront-scope fences) and keeping the lanes together is enough; no
+// workgroup barrier
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}  // LDS row length (BF16) of a 32-pixel tile row; 80 bytes keep 16-byte reads aligned
 
 template <int DIM>
 __global__ void __launch_bounds__(kConvThreads)
@@ -404,7 +414,8 @@ __global__ void __launch_bounds__(kConvThreads)
   float bsum = 0.f;
   const int sp = lane >> 1, half = 8 * (lane & 1);  // staging: pixel of the tile, channel half
   const int kl = 8 * (lane >> 4);                   // operand read: first pixel of this lane
-  // every wavefront runs the same trip count (the barriers below)
+  // the tiles tg[wave] and tx[wave] are each wavefront's own: the loop
+  // synchronises within a wavefront only
   for (long base = p_begin; base < p_end; base += 128) {
     const long p = base + wave * 32 + sp;
     const bool pv = p < p_end;
@@ -414,6 +425,20 @@ __global__ void __launch_bounds__(kConvThreads)
       const long r = p / s.w;
       py = (int)(r % s.h);
       pz = (int)((r / s.h) % s.d);
+    }
+    // the shifted inputs of all 9 taps are loaded first, so their latencies
+    // overlap instead of one per tap
+    uint4 xv[9];
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+      int dz, dy, dx;
+      Taps<DIM>::offsets(grp * 9 + t, dz, dy, dx);
+      xv[t] = uint4{0, 0, 0, 0};
+      const int zz = pz + dz, yy = py + dy, xx = px + dx;
+      if (pv && zz >= 0 && zz < s.d && yy >= 0 && yy < s.h && xx >= 0 && xx < s.w) {
+        const long q = p + ((long)dz * s.h + dy) * s.w + dx;
+        xv[t] = *reinterpret_cast<const uint4 *>(x.p + q * x.stride + x.off + cc * 16 + half);
+      }
     }
     {  // gz tile [co][pixel]
       uint4 gv = uint4{0, 0, 0, 0};
@@ -438,7 +463,7 @@ __global__ void __launch_bounds__(kConvThreads)
         tg[wave][half + 2 * q + 1][sp] = (bf16_t)(gw[q] >> 16);
       }
     }
-    __syncthreads();
+    wave_lds_sync();  // tg is this wavefront's own
     const uint4 a = *reinterpret_cast<const uint4 *>(&tg[wave][lane & 15][kl]);
     {  // bias: this lane's 8 pixels of channel lane % 16
       const unsigned av[4] = {a.x, a.y, a.z, a.w};
@@ -447,24 +472,16 @@ __global__ void __launch_bounds__(kConvThreads)
     }
 #pragma unroll
     for (int t = 0; t < 9; ++t) {
-      int dz, dy, dx;
-      Taps<DIM>::offsets(grp * 9 + t, dz, dy, dx);
-      uint4 xv = uint4{0, 0, 0, 0};
-      const int zz = pz + dz, yy = py + dy, xx = px + dx;
-      if (pv && zz >= 0 && zz < s.d && yy >= 0 && yy < s.h && xx >= 0 && xx < s.w) {
-        const long q = p + ((long)dz * s.h + dy) * s.w + dx;
-        xv = *reinterpret_cast<const uint4 *>(x.p + q * x.stride + x.off + cc * 16 + half);
-      }
-      const unsigned xw[4] = {xv.x, xv.y, xv.z, xv.w};
+      const unsigned xw[4] = {xv[t].x, xv[t].y, xv[t].z, xv[t].w};
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         tx[wave][half + 2 * q][sp] = (bf16_t)(xw[q] & 0xffffu);
         tx[wave][half + 2 * q + 1][sp] = (bf16_t)(xw[q] >> 16);
       }
-      __syncthreads();
+      wave_lds_sync();
       const uint4 bb = *reinterpret_cast<const uint4 *>(&tx[wave][lane & 15][kl]);
       acc[t] = mfma32(a, bb, acc[t]);
-      __syncthreads();  // the next tap overwrites tx
+      wave_lds_sync();  // the next tap overwrites tx
     }
   }
 #pragma unroll
