@@ -452,6 +452,14 @@ template <int NB, int DIM, int G>
 void conv_kernel_g(dim3 grid, bool lds, size_t bytes, const ConvShape &s, View x, View mask, const bf16_t *w,
                    const float *b, View y, int relu, hipStream_t st) {
   if (lds) {
+    if constexpr (NB % 2 == 0 && G == 2) {  // several output blocks: the wavefronts split them
+      constexpr int WS = NB % 4 == 0 ? 4 : 2;
+      if (bytes > 65536)
+        SCIHPC_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_conv_ldsn<NB, DIM, WS>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)kConvLdsMax));
+      k_conv_ldsn<NB, DIM, WS><<<grid, kConvThreads, bytes, st>>>(x, mask, w, b, y, s, relu);
+      return;
+    }
     if (bytes > 65536)  // past the default dynamic LDS limit (a host-side attribute, cheap to set again)
       SCIHPC_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_conv_lds<NB, DIM, G>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)kConvLdsMax));

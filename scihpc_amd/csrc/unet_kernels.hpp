@@ -181,21 +181,13 @@ inline size_t conv_lds_bytes(int ndim, int cin) {
   return (size_t)(ndim == 3 ? 9 : 3) * 66 * (cin + 8) * sizeof(bf16_t);
 }
 
-template <int NB, int DIM, int G = 2>  // G: k-steps whose operands are loaded together
-__global__ void __launch_bounds__(kConvThreads)
-    k_conv_lds(View x, View mask, const bf16_t *__restrict__ w, const float *__restrict__ b, View y, ConvShape s,
-               int relu) {
-  constexpr int T = Taps<DIM>::T, R = conv_rows<DIM>();
-  extern __shared__ __align__(16) bf16_t tile[];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int segs = (s.w + 63) / 64;
-  int blk = blockIdx.x;
-  const int seg = blk % segs;
-  blk /= segs;
-  const int row = blk % s.h;
-  blk /= s.h;
-  const int z = blk % s.d, img = blk / s.d;
-  const int CP = s.cin + 8;   // LDS pixel pitch (channels)
+// stage input rows (z, y) + {-1, 0, 1} x 66 pixels x cin of the workgroup's
+// tile into LDS, masked and zero-padded as k_conv reads them
+template <int DIM>
+__device__ __forceinline__ void conv_stage_tile(View x, View mask, const ConvShape &s, int img, int z, int row, int seg,
+                                                bf16_t *tile) {
+  constexpr int R = conv_rows<DIM>();
+  const int CP = s.cin + 8;  // LDS pixel pitch (channels)
   const int c8s = s.cin / 8;  // 16-byte chunks of a pixel
   for (int q = threadIdx.x; q < R * 66 * c8s; q += kConvThreads) {
     const int r = q / (66 * c8s), rem = q - r * 66 * c8s;
@@ -219,6 +211,24 @@ __global__ void __launch_bounds__(kConvThreads)
     }
     *reinterpret_cast<uint4 *>(tile + (r * 66 + p) * CP + c) = a;
   }
+}
+
+template <int NB, int DIM, int G = 2>  // G: k-steps whose operands are loaded together
+__global__ void __launch_bounds__(kConvThreads)
+    k_conv_lds(View x, View mask, const bf16_t *__restrict__ w, const float *__restrict__ b, View y, ConvShape s,
+               int relu) {
+  constexpr int T = Taps<DIM>::T;
+  extern __shared__ __align__(16) bf16_t tile[];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int segs = (s.w + 63) / 64;
+  int blk = blockIdx.x;
+  const int seg = blk % segs;
+  blk /= segs;
+  const int row = blk % s.h;
+  blk /= s.h;
+  const int z = blk % s.d, img = blk / s.d;
+  conv_stage_tile<DIM>(x, mask, s, img, z, row, seg, tile);
+  const int CP = s.cin + 8;
   __syncthreads();
   const int pl = wave * 16 + (lane & 15);  // tile pixel of this lane's A row, before the tap's dx
   const int kl = 8 * (lane >> 4);
@@ -268,6 +278,93 @@ __global__ void __launch_bounds__(kConvThreads)
       if (relu) v = v > 0.f ? v : 0.f;
       y.p[(rowpix + p) * y.stride + y.off + co] = f_to_bf16(v);
     }
+  }
+}
+
+// k_conv_lds for layers with several output blocks: the four wavefronts
+// split the NB output blocks WS ways (WS = 2 or 4, NB a multiple of WS) and
+// the tile's four 16-pixel groups 4 / WS ways.  Wavefront w computes the WS
+// pixel groups from (w / WS) * WS on, for the NB / WS output blocks from
+// (w % WS) * NB / WS on, so each weight fragment is loaded by 4 / WS
+// wavefronts instead of all four.  Each
+// (pixel group, output block) sees the same operands in the same k order as
+// in k_conv: identical results.
+template <int NB, int DIM, int WS>
+__global__ void __launch_bounds__(kConvThreads)
+    k_conv_ldsn(View x, View mask, const bf16_t *__restrict__ w, const float *__restrict__ b, View y, ConvShape s,
+                int relu) {
+  static_assert((WS == 2 || WS == 4) && NB % WS == 0, "the wavefronts split the output blocks evenly");
+  constexpr int T = Taps<DIM>::T, NW = NB / WS, MP = WS;
+  extern __shared__ __align__(16) bf16_t tile[];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int segs = (s.w + 63) / 64;
+  int blk = blockIdx.x;
+  const int seg = blk % segs;
+  blk /= segs;
+  const int row = blk % s.h;
+  blk /= s.h;
+  const int z = blk % s.d, img = blk / s.d;
+  conv_stage_tile<DIM>(x, mask, s, img, z, row, seg, tile);
+  const int CP = s.cin + 8;
+  __syncthreads();
+  const int kl = 8 * (lane >> 4);
+  const int nbw = NB * blockIdx.y + NW * (wave % WS);  // first output block of this wavefront
+  const int pg0 = MP * (wave / WS);                      // first 16-pixel group of this wavefront
+  floatx4 acc[MP][NW];
+#pragma unroll
+  for (int m = 0; m < MP; ++m)
+#pragma unroll
+    for (int nb = 0; nb < NW; ++nb) acc[m][nb] = floatx4{0.f, 0.f, 0.f, 0.f};
+  const int K = T * s.cin;
+  constexpr int G = 2;
+  for (int k0 = 0; k0 < K; k0 += 32 * G) {
+    uint4 a[G][MP], bw[G][NW];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const int kk = k0 + 32 * g + kl;
+#pragma unroll
+      for (int m = 0; m < MP; ++m) a[g][m] = uint4{0, 0, 0, 0};
+#pragma unroll
+      for (int nb = 0; nb < NW; ++nb) bw[g][nb] = uint4{0, 0, 0, 0};
+      if (kk < K) {
+        const int t = kk / s.cin, c = kk - t * s.cin;
+        int dz, dy, dx;
+        Taps<DIM>::offsets(t, dz, dy, dx);
+        const int r = DIM == 3 ? (dz + 1) * 3 + dy + 1 : dy + 1;
+#pragma unroll
+        for (int m = 0; m < MP; ++m)
+          a[g][m] = *reinterpret_cast<const uint4 *>(tile + (r * 66 + (pg0 + m) * 16 + (lane & 15) + dx + 1) * CP + c);
+#pragma unroll
+        for (int nb = 0; nb < NW; ++nb)
+          bw[g][nb] =
+              *reinterpret_cast<const uint4 *>(w + ((long)((nbw + nb) * 16 + (lane & 15)) * T + t) * s.cin + c);
+      }
+    }
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      if (k0 + 32 * g < K) {
+#pragma unroll
+        for (int m = 0; m < MP; ++m)
+#pragma unroll
+          for (int nb = 0; nb < NW; ++nb) acc[m][nb] = mfma32(a[g][m], bw[g][nb], acc[m][nb]);
+      }
+    }
+  }
+  const long rowpix = (((long)img * s.d + z) * s.h + row) * s.w;
+#pragma unroll
+  for (int nb = 0; nb < NW; ++nb) {
+    const int co = (nbw + nb) * 16 + (lane & 15);
+    const float bias = b ? b[co] : 0.f;
+#pragma unroll
+    for (int m = 0; m < MP; ++m)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int p = seg * 64 + (pg0 + m) * 16 + 4 * (lane >> 4) + j;
+        if (p >= s.w) continue;
+        float v = acc[m][nb][j] + bias;
+        if (relu) v = v > 0.f ? v : 0.f;
+        y.p[(rowpix + p) * y.stride + y.off + co] = f_to_bf16(v);
+      }
   }
 }
 
