@@ -38,6 +38,12 @@ def _bits_to_f(u):
 
 CONV_CASES = [(1, 8, 64, 16, 16), (2, 5, 40, 48, 16), (1, 7, 100, 32, 64), (1, 4, 16, 192, 64),
               (1, 3, 130, 128, 128), (2, 6, 24, 16, 48), (1, 1, 9, 64, 96), (1, 4, 12, 16, 192)]
+# 64 pixel tiles or more (no output-channel split) with all output blocks in
+# one workgroup over an LDS input tile, the wavefronts splitting them:
+# 2, 4, 6, 8 and 12 blocks, ragged and full 64-pixel segments, and a tile
+# past the default 64 KiB of dynamic LDS (192 input channels)
+CONV_CASES += [(1, 16, 200, 64, 32), (1, 32, 130, 32, 64), (2, 16, 100, 64, 96), (1, 64, 64, 64, 128),
+               (1, 64, 70, 48, 192), (1, 32, 66, 192, 192)]
 
 
 @pytest.mark.parametrize("case", CONV_CASES, ids=lambda c: "x".join(map(str, c)))
@@ -372,6 +378,10 @@ def _grads_close(g, g_ref, g_exact, ndim):
 
 CONV3_CASES = [(1, 4, 5, 64, 16, 16), (2, 3, 4, 40, 32, 48), (1, 2, 3, 70, 192, 64), (1, 4, 4, 8, 128, 128),
                (1, 3, 2, 16, 48, 192)]
+# the 3-D LDS-tile path with the wavefronts splitting 2, 4, 6, 8 and 12
+# output blocks (48 input channels: a tile past 64 KiB of dynamic LDS)
+CONV3_CASES += [(1, 4, 16, 70, 32, 32), (1, 4, 8, 100, 48, 64), (2, 4, 8, 40, 32, 96), (1, 8, 8, 64, 64, 128),
+                (1, 4, 16, 20, 16, 192)]
 
 
 @pytest.mark.parametrize("case", CONV3_CASES, ids=lambda c: "x".join(map(str, c)))
