@@ -447,6 +447,13 @@ constexpr long kSplitBelow = 64;
 // 101 -> 70 us (64 -> 192 channels, split 6), the full-resolution layers
 // unchanged.
 constexpr size_t kConvLdsMax = 98304;
+// weight-gradient workgroups aimed at per layer (the pixel chunk shrinks
+// until there are about this many).  With the next pixels' loads in flight
+// during the current ones' work, longer chunks pay: at 256x512, batch 4,
+// 2048 / 1024 / 512 / 384 / 256 workgroups gave a training step of 1.57 /
+// 1.52 / 1.47 / 1.51 / 1.63 ms (fewer partial rows to reduce, fewer
+// epilogues, but below 512 too few workgroups to fill the CUs).
+constexpr int kWgradWGs = 512;
 constexpr int kConvLdsSplitMax = 6;
 template <int NB, int DIM, int G>
 void conv_kernel_g(dim3 grid, bool lds, size_t bytes, const ConvShape &s, View x, View mask, const bf16_t *w,
@@ -712,9 +719,8 @@ struct scihpc_unet {
   void wgrad_chunks(int i, int n, int &chunks, int &cpix) const {
     const long npix = pix(L[i].lev, n);
     const int pairs = (L[i].cout / 16) * (L[i].cin / 16) * (T / 9);
-    // about 2048 workgroups (a 16x16-channel level-0 layer had 64, a
-    // quarter of the CUs), pixel chunks of at least 128
-    long c = std::max(1, (2048 + pairs - 1) / pairs);
+    // about kWgradWGs workgroups, pixel chunks of at least 128
+    long c = std::max(1, (kWgradWGs + pairs - 1) / pairs);
     c = std::min(c, std::max(1L, npix / 128));
     long cp = (npix + c - 1) / c;
     cp = (cp + 127) / 128 * 128;

@@ -512,8 +512,11 @@ __global__ void __launch_bounds__(kConvThreads)
   const int sp = lane >> 1, half = 8 * (lane & 1);  // staging: pixel of the tile, channel half
   const int kl = 8 * (lane >> 4);                   // operand read: first pixel of this lane
   // the tiles tg[wave] and tx[wave] are each wavefront's own: the loop
-  // synchronises within a wavefront only
-  for (long base = p_begin; base < p_end; base += 128) {
+  // synchronises within a wavefront only.  The loads of the next 128 pixels
+  // (every tap's shifted input, the output gradient and its mask) are issued
+  // before the current pixels are transposed and multiplied, so their
+  // latency overlaps that work.
+  auto load = [&](long base, uint4(&xv)[9], uint4 &gv, uint4 &mv) {
     const long p = base + wave * 32 + sp;
     const bool pv = p < p_end;
     int px = 0, py = 0, pz = 0;
@@ -523,9 +526,6 @@ __global__ void __launch_bounds__(kConvThreads)
       py = (int)(r % s.h);
       pz = (int)((r / s.h) % s.d);
     }
-    // the shifted inputs of all 9 taps are loaded first, so their latencies
-    // overlap instead of one per tap
-    uint4 xv[9];
 #pragma unroll
     for (int t = 0; t < 9; ++t) {
       int dz, dy, dx;
@@ -537,23 +537,34 @@ __global__ void __launch_bounds__(kConvThreads)
         xv[t] = *reinterpret_cast<const uint4 *>(x.p + q * x.stride + x.off + cc * 16 + half);
       }
     }
-    {  // gz tile [co][pixel]
-      uint4 gv = uint4{0, 0, 0, 0};
-      if (pv) {
-        gv = *reinterpret_cast<const uint4 *>(g.p + p * g.stride + g.off + cb * 16 + half);
-        if (mask.p) {
-          const uint4 m = *reinterpret_cast<const uint4 *>(mask.p + p * mask.stride + mask.off + cb * 16 + half);
-          unsigned av[4] = {gv.x, gv.y, gv.z, gv.w};
-          const unsigned mv[4] = {m.x, m.y, m.z, m.w};
+    gv = uint4{0, 0, 0, 0};
+    mv = uint4{0x3f803f80u, 0x3f803f80u, 0x3f803f80u, 0x3f803f80u};  // BF16 1.0: nothing masked
+    if (pv) {
+      gv = *reinterpret_cast<const uint4 *>(g.p + p * g.stride + g.off + cb * 16 + half);
+      if (mask.p) mv = *reinterpret_cast<const uint4 *>(mask.p + p * mask.stride + mask.off + cb * 16 + half);
+    }
+  };
+  uint4 xv[9], gv, mv;
+  load(p_begin, xv, gv, mv);
+  for (long base = p_begin; base < p_end; base += 128) {
+    uint4 xn[9], gn, mn;
+    if (base + 128 < p_end) {
+      load(base + 128, xn, gn, mn);
+    } else {
 #pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            if (!bf16_pos((bf16_t)(mv[q] & 0xffffu))) av[q] &= 0xffff0000u;
-            if (!bf16_pos((bf16_t)(mv[q] >> 16))) av[q] &= 0x0000ffffu;
-          }
-          gv = uint4{av[0], av[1], av[2], av[3]};
+      for (int t = 0; t < 9; ++t) xn[t] = uint4{0, 0, 0, 0};
+      gn = mn = uint4{0, 0, 0, 0};
+    }
+    {  // gz tile [co][pixel], the ReLU mask applied
+      unsigned gw[4] = {gv.x, gv.y, gv.z, gv.w};
+      if (mask.p) {
+        const unsigned mw[4] = {mv.x, mv.y, mv.z, mv.w};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          if (!bf16_pos((bf16_t)(mw[q] & 0xffffu))) gw[q] &= 0xffff0000u;
+          if (!bf16_pos((bf16_t)(mw[q] >> 16))) gw[q] &= 0x0000ffffu;
         }
       }
-      const unsigned gw[4] = {gv.x, gv.y, gv.z, gv.w};
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         tg[wave][half + 2 * q][sp] = (bf16_t)(gw[q] & 0xffffu);
@@ -580,6 +591,10 @@ __global__ void __launch_bounds__(kConvThreads)
       acc[t] = mfma32(a, bb, acc[t]);
       wave_lds_sync();  // the next tap overwrites tx
     }
+#pragma unroll
+    for (int t = 0; t < 9; ++t) xv[t] = xn[t];
+    gv = gn;
+    mv = mn;
   }
 #pragma unroll
   for (int t = 0; t < 9; ++t)
