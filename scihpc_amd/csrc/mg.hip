@@ -1686,7 +1686,7 @@ __global__ void __launch_bounds__(kZmX *kZmY)
         if (j > 0) off += cD * P(c - sy);
         if (j < L.ny - 1) off += cU * P(c + sy);
         diag = -(cR + cL + cU + cD);
-        const double cF = L.cz[c], cB = k > 0 ? czb : 0.0;
+        const double cF = __builtin_nontemporal_load(L.cz + c), cB = k > 0 ? czb : 0.0;
         if (k > 0) off += cB * pb;
         if (k < L.nz - 1) off += cF * pf;
         diag -= (cF + cB);
@@ -1702,7 +1702,7 @@ __global__ void __launch_bounds__(kZmX *kZmY)
         diag = L.dirichlet ? L.dconst : -cc;
       }
       const double qc = off + diag * pc;
-      const double rc = r[c];
+      const double rc = __builtin_nontemporal_load(r + c);
       p[c] = pc;
       q[c] = qc;
       vpq += pc * qc;
@@ -1739,7 +1739,8 @@ __global__ void __launch_bounds__(kZmX *kZmY)
 // zero coefficient), the partial sums add a column's cells first.  The
 // right-hand side and the z faces, each read once, are non-temporal loads
 // (256^3 FP64 pass 146.6 -> 144.0 us; the x and y faces, which the row and
-// column neighbours read again, gain nothing from it).
+// column neighbours read again, gain nothing from it), as are the arrays
+// that k_rr3d_zm and k_fcg_direction_zm read once (rhs; r and the z faces).
 template <bool VAR>
 __global__ void __launch_bounds__(kZmX *kZmY)
     k_residual_zm(DLevel<double> L, int store, double *__restrict__ partial, const double *__restrict__ flag,
@@ -1848,7 +1849,7 @@ __global__ void __launch_bounds__(kZmX *kZmY)
         z3_stencil<T, 2>(F, i, j, kf, i > 0 ? x[c - 1] : T(0), i < F.nx - 1 ? x[c + 1] : T(0),
                          j > 0 ? x[c - sy] : T(0), j < F.ny - 1 ? x[c + sy] : T(0), xb, xf, cL, cR, cD, cU, cB,
                          cF, off, diag);
-        s += F.rhs[c] - (off + diag * xc);
+        s += __builtin_nontemporal_load(F.rhs + c) - (off + diag * xc);
         xb = xc;
         xc = xf;
         rb = rcv;
