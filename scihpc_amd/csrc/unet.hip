@@ -509,9 +509,9 @@ void launch_conv_dim(const ConvShape &s, View x, View mask, const bf16_t *w, con
   // large, narrow layers: weights in registers across several tiles
   if (DIM == 2 && (long)tiles >= 2048) {
     const int ks = (Taps<DIM>::T * s.cin + 31) / 32;
-    const unsigned g = (tiles + kConvTilesPerWG - 1) / kConvTilesPerWG;
 #define SCIHPC_RW(NB_, KS_)                                                                              \
   if (blocks == NB_ && ks == KS_) {                                                                      \
+    const unsigned g = (tiles + rw_tiles<KS_>() - 1) / rw_tiles<KS_>();                                  \
     k_conv_rw<NB_, DIM, KS_><<<g, kConvThreads, 0, st>>>(x, mask, w, b, y, s, relu, (int)tiles);          \
     SCIHPC_CHECK_LAUNCH();                                                                               \
     return;                                                                                              \

@@ -375,6 +375,13 @@ __global__ void __launch_bounds__(kConvThreads)
 // a tile waits on one round of loads instead of one per two k-steps.  The
 // k-steps run in k_conv's order with the same operands: identical results.
 constexpr int kConvTilesPerWG = 4;
+// the deepest reductions (KS >= 14 k-steps: the 48 -> 16 decoder layer) take
+// 8 tiles a workgroup (55.0 -> 50.1 us at 256x512, batch 4); 8 for every
+// layer cost the 128x256 layers, whose 2048 tiles then fill 256 workgroups
+template <int KS>
+constexpr int rw_tiles() {
+  return KS >= 14 ? 2 * kConvTilesPerWG : kConvTilesPerWG;
+}
 
 template <int NB, int DIM, int KS>
 __global__ void __launch_bounds__(kConvThreads)
@@ -398,7 +405,7 @@ __global__ void __launch_bounds__(kConvThreads)
 #pragma unroll
   for (int nb = 0; nb < NB; ++nb) bias[nb] = b ? b[nb * 16 + (lane & 15)] : 0.f;
   const int segs = (s.w + 63) / 64;
-  const int t0 = blockIdx.x * kConvTilesPerWG, t1 = min(t0 + kConvTilesPerWG, ntiles);
+  const int t0 = blockIdx.x * rw_tiles<KS>(), t1 = min(t0 + rw_tiles<KS>(), ntiles);
   for (int tile = t0; tile < t1; ++tile) {
     int blk = tile;
     const int seg = blk % segs;
