@@ -447,6 +447,7 @@ constexpr long kSplitBelow = 64;
 // 101 -> 70 us (64 -> 192 channels, split 6), the full-resolution layers
 // unchanged.
 constexpr size_t kConvLdsMax = 98304;
+constexpr int kConvLdsSplitMax = 6;
 // weight-gradient workgroups aimed at per layer (the pixel chunk shrinks
 // until there are about this many).  With the next pixels' loads in flight
 // during the current ones' work, longer chunks pay: at 256x512, batch 4,
@@ -454,7 +455,6 @@ constexpr size_t kConvLdsMax = 98304;
 // 1.52 / 1.47 / 1.51 / 1.63 ms (fewer partial rows to reduce, fewer
 // epilogues, but below 512 too few workgroups to fill the CUs).
 constexpr int kWgradWGs = 512;
-constexpr int kConvLdsSplitMax = 6;
 template <int NB, int DIM, int G>
 void conv_kernel_g(dim3 grid, bool lds, size_t bytes, const ConvShape &s, View x, View mask, const bf16_t *w,
                    const float *b, View y, int relu, hipStream_t st) {

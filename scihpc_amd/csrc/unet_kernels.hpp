@@ -286,9 +286,8 @@ __global__ void __launch_bounds__(kConvThreads)
 // the tile's four 16-pixel groups 4 / WS ways.  Wavefront w computes the WS
 // pixel groups from (w / WS) * WS on, for the NB / WS output blocks from
 // (w % WS) * NB / WS on, so each weight fragment is loaded by 4 / WS
-// wavefronts instead of all four.  Each
-// (pixel group, output block) sees the same operands in the same k order as
-// in k_conv: identical results.
+// wavefronts instead of all four.  Each (pixel group, output block) sees
+// the same operands in the same k order as in k_conv: identical results.
 template <int NB, int DIM, int WS>
 __global__ void __launch_bounds__(kConvThreads)
     k_conv_ldsn(View x, View mask, const bf16_t *__restrict__ w, const float *__restrict__ b, View y, ConvShape s,
