@@ -288,7 +288,7 @@ __global__ void __launch_bounds__(kConvThreads)
 // (w % WS) * NB / WS on, so each weight fragment is loaded by 4 / WS
 // wavefronts instead of all four.  Each (pixel group, output block) sees
 // the same operands in the same k order as in k_conv: identical results.
-template <int NB, int DIM, int WS>
+template <int NB, int DIM, int WS, int G = 2>
 __global__ void __launch_bounds__(kConvThreads)
     k_conv_ldsn(View x, View mask, const bf16_t *__restrict__ w, const float *__restrict__ b, View y, ConvShape s,
                 int relu) {
@@ -315,7 +315,6 @@ __global__ void __launch_bounds__(kConvThreads)
 #pragma unroll
     for (int nb = 0; nb < NW; ++nb) acc[m][nb] = floatx4{0.f, 0.f, 0.f, 0.f};
   const int K = T * s.cin;
-  constexpr int G = 2;
   for (int k0 = 0; k0 < K; k0 += 32 * G) {
     uint4 a[G][MP], bw[G][NW];
 #pragma unroll
