@@ -461,10 +461,12 @@ void conv_kernel_g(dim3 grid, bool lds, size_t bytes, const ConvShape &s, View x
   if (lds) {
     if constexpr (NB % 2 == 0 && G == 2) {  // several output blocks: the wavefronts split them
       constexpr int WS = NB % 4 == 0 ? 4 : 2;
-      // four k-steps of operands in flight where the registers allow it
-      // (a wavefront loads only its NB / WS weight fragments a k-step): the
-      // 2- and 4-block layers 48.2 -> 45.8 and 19.1 -> 18.4 us at 256x512
-      constexpr int GN = NB <= 4 ? 4 : 2;
+      // four k-steps of operands in flight where they pay (a wavefront
+      // loads only its NB / WS weight fragments a k-step): at 256x512 the
+      // 2-, 4- and 8-block layers 48.2 -> 45.8, 19.1 -> 18.4 and
+      // 20.5 -> 19.0 us; the 6-block layer (three fragments a wavefront)
+      // 35.3 -> 40.0 us, so it keeps two
+      constexpr int GN = NB == 2 || NB == 4 || NB == 8 ? 4 : 2;
       if (bytes > 65536)
         SCIHPC_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_conv_ldsn<NB, DIM, WS, GN>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)kConvLdsMax));
